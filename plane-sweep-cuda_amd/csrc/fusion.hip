@@ -1,0 +1,514 @@
+// fusion.hip -- TV-L1 histogram volumetric fusion for gfx950.
+//
+// Grid storage is SoA: u[V], v[V], p[3][V] (px, py, pz planes) and hist[V][8]
+// uint16 counts, x fastest, then y, then z.  The reference keeps one 28-byte
+// AoS fusionvoxel<8> per voxel (src/inc/structs.h:131-189) and reads all of
+// it in every kernel; SoA lets each step read only what it uses with
+// coalesced 4- and 16-byte accesses.
+//
+// Histogram votes reproduce FusionUpdateHistogram_kernel (src/fusion.cu:5-43)
+// bit for bit: voxel centre with the double division of worldCoords
+// (src/inc/fusion.h:360-364), float projection, bilinear depth lookup
+// (src/inc/dev_functions.h:33-38) and the bin rule of updateHist
+// (src/inc/fusion.h:883-907), all without FMA contraction.
+//
+// The U step's proxHist (src/inc/fusion.h:851-858) inserts u and p_1..p_8
+// into the sorted bin centres with sortedHist::insert (src/inc/structs.h:
+// 212-284), which drops values below the running minimum; since the minimum
+// is always the bin centre -1, the result equals max(-1, median of the 17
+// values).  Here the median is taken in closed form: sort the 9 data values
+// with a sorting network, then select the 9th smallest of the union with the
+// 8 (sorted, constant) bin centres.
+#include "psw_common.h"
+
+namespace psw {
+
+constexpr int FB = PSW_FUSION_BINS;
+
+struct FusionView {
+    int w, h, d;
+    float a[3], b[3];
+    float* u;
+    float* v;
+    float* p;
+    uint16_t* hist;
+    size_t V;
+};
+
+static inline FusionView view_of(const psw_fusion_grid* g) {
+    FusionView f;
+    f.w = g->width; f.h = g->height; f.d = g->depth;
+    for (int i = 0; i < 3; ++i) { f.a[i] = g->volume[i]; f.b[i] = g->volume[3 + i]; }
+    f.u = g->u; f.v = g->v; f.p = g->p; f.hist = g->hist;
+    f.V = (size_t)g->width * g->height * g->depth;
+    return f;
+}
+
+// Voxel centre in world coordinates (worldCoords, fusion.h:360-364).
+__device__ inline void voxel_centre(const FusionView& f, int x, int y, int z, float& cx, float& cy, float& cz) {
+    const float fx = (float)((x + .5) / (double)f.w);
+    const float fy = (float)((y + .5) / (double)f.h);
+    const float fz = (float)((z + .5) / (double)f.d);
+    const float sx = f.b[0] - f.a[0], sy = f.b[1] - f.a[1], sz = f.b[2] - f.a[2];
+    cx = f.a[0] + sx * fx;
+    cy = f.a[1] + sy * fy;
+    cz = f.a[2] + sz * fz;
+}
+
+struct Cam {
+    M3f K, R;
+    V3f T;
+};
+
+// Bin of one depthmap vote for a voxel centre c, or -1 if it projects
+// outside the image (fusion.cu:17-41, fusion.h:883-907).
+__device__ inline int vote_bin(const Cam& cam, const float* __restrict__ depth, int W, int H, float thr, float cx,
+                               float cy, float cz) {
+    float c0, c1, c2;
+    mv3(cam.R, cx, cy, cz, c0, c1, c2);
+    c0 = c0 + cam.T.x; c1 = c1 + cam.T.y; c2 = c2 + cam.T.z;
+    float k0, k1, k2;
+    mv3(cam.K, c0, c1, c2, k0, k1, k2);
+    const float px = k0 / k2, py = k1 / k2;
+    if ((px < 0.f) || (px > (float)(W - 1)) || (py < 0.f) || (py > (float)(H - 1))) return -1;
+    const float flx = floorf(px), fly = floorf(py);
+    const int x0 = (int)fmaxf(flx, 0.f), y0 = (int)fmaxf(fly, 0.f);
+    const int x1 = (int)fminf((float)(x0 + 1), (float)(W - 1)), y1 = (int)fminf((float)(y0 + 1), (float)(H - 1));
+    const float frx = px - flx, fry = py - fly;
+    const float a0 = depth[x0 + (size_t)y0 * W], a1 = depth[x1 + (size_t)y0 * W];
+    const float b0 = depth[x0 + (size_t)y1 * W], b1 = depth[x1 + (size_t)y1 * W];
+    const float l0 = a0 + fry * (b0 - a0), l1 = a1 + fry * (b1 - a1);
+    const float dep = l0 + frx * (l1 - l0);
+    const float sd = dep - k2;
+    if (sd >= thr) return FB - 1;
+    if (sd <= -thr) return 0;
+    const float q = fminf(roundf((sd + thr) / (2.f * thr) * (float)(FB - 3) + 1.f), (float)(FB - 2));
+    // q is in [1, 6] for thr > 0 (and 6 for NaN, as fminf returns the number);
+    // the lower clamp only keeps the index inside the histogram.
+    return (int)fmaxf(q, 0.f);
+}
+
+__global__ void fusion_hist_kernel(FusionView f, Cam cam, const float* __restrict__ depth, int W, int H, float thr) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= f.w || y >= f.h || z >= f.d) return;
+    float cx, cy, cz;
+    voxel_centre(f, x, y, z, cx, cy, cz);
+    const int bin = vote_bin(cam, depth, W, H, thr, cx, cy, cz);
+    if (bin < 0) return;
+    const size_t i = ((size_t)z * f.h + y) * f.w + x;
+    f.hist[i * FB + bin] = (uint16_t)(f.hist[i * FB + bin] + 1);
+}
+
+constexpr int FH_BATCH = 24;  // depthmaps per launch (kernel-argument space)
+
+struct CamBatch {
+    Cam cam[FH_BATCH];
+    int m;
+};
+
+// Votes of up to FH_BATCH depthmaps per voxel, counted in registers and added
+// to the stored histogram once.  Counts are integers, so the order of the
+// votes does not matter and the result equals m single updates (mod 2^16).
+__global__ void fusion_hist_batch_kernel(FusionView f, CamBatch cb, const float* __restrict__ depths, int W, int H,
+                                         float thr) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= f.w || y >= f.h || z >= f.d) return;
+    float cx, cy, cz;
+    voxel_centre(f, x, y, z, cx, cy, cz);
+    uint32_t cnt[FB];
+#pragma unroll
+    for (int k = 0; k < FB; ++k) cnt[k] = 0;
+    const size_t plane = (size_t)W * H;
+    for (int j = 0; j < cb.m; ++j) {
+        const int bin = vote_bin(cb.cam[j], depths + (size_t)j * plane, W, H, thr, cx, cy, cz);
+#pragma unroll
+        for (int k = 0; k < FB; ++k) cnt[k] += (bin == k) ? 1u : 0u;
+    }
+    const size_t i = ((size_t)z * f.h + y) * f.w + x;
+    uint4* hp = reinterpret_cast<uint4*>(f.hist + i * FB);
+    uint4 h = *hp;
+    uint32_t w4[4] = {h.x, h.y, h.z, h.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t lo = (w4[k] + cnt[2 * k]) & 0xffffu;
+        const uint32_t hi = ((w4[k] >> 16) + cnt[2 * k + 1]) & 0xffffu;
+        w4[k] = lo | (hi << 16);
+    }
+    *hp = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+}
+
+// ---- primal-dual ----------------------------------------------------------
+
+__device__ inline void cswap(float& a, float& b) {
+    const float lo = fminf(a, b), hi = fmaxf(a, b);
+    a = lo;
+    b = hi;
+}
+
+// max(-1, median of {centres} U {u, p_1..p_8}), p_i = u + (tau*lambda)*W_i,
+// W_i = -sum_{j<i} h_j + sum_{j>=i} h_j (fusion.h:813-838).
+__device__ inline float prox_hist(float u, const uint16_t* __restrict__ hp, float tl) {
+    const uint4 hv = *reinterpret_cast<const uint4*>(hp);
+    int h[FB];
+    h[0] = hv.x & 0xffff; h[1] = hv.x >> 16; h[2] = hv.y & 0xffff; h[3] = hv.y >> 16;
+    h[4] = hv.z & 0xffff; h[5] = hv.z >> 16; h[6] = hv.w & 0xffff; h[7] = hv.w >> 16;
+    int total = 0;
+#pragma unroll
+    for (int k = 0; k < FB; ++k) total += h[k];
+    float a[9];
+    a[0] = u;
+    int below = 0;
+#pragma unroll
+    for (int i = 1; i <= FB; ++i) {
+        below += h[i - 1];
+        const int Wi = total - 2 * below;
+        a[i] = u + tl * (float)Wi;
+    }
+    // Sorting network for 9 values (25 compare-exchanges).
+    cswap(a[0], a[1]); cswap(a[3], a[4]); cswap(a[6], a[7]);
+    cswap(a[1], a[2]); cswap(a[4], a[5]); cswap(a[7], a[8]);
+    cswap(a[0], a[1]); cswap(a[3], a[4]); cswap(a[6], a[7]);
+    cswap(a[0], a[3]); cswap(a[3], a[6]); cswap(a[0], a[3]);
+    cswap(a[1], a[4]); cswap(a[4], a[7]); cswap(a[1], a[4]);
+    cswap(a[2], a[5]); cswap(a[5], a[8]); cswap(a[2], a[5]);
+    cswap(a[1], a[3]); cswap(a[5], a[7]); cswap(a[2], a[6]);
+    cswap(a[4], a[6]); cswap(a[2], a[4]); cswap(a[2], a[3]);
+    cswap(a[5], a[6]);
+    // 9th smallest of a (sorted, 9) and the centres c (sorted, 8):
+    // min over splits i = 1..9 of max(a[i-1], c[8-i]).  Centres of 8 bins
+    // (binParams, fusion.h:1041-1054): -1, -1, -0.6, -0.2, 0.2, 0.6, 1, 1.
+    const float c1 = 2.f * 1.f / 5.f - 1.f, c2 = 2.f * 2.f / 5.f - 1.f, c3 = 2.f * 3.f / 5.f - 1.f,
+                c4 = 2.f * 4.f / 5.f - 1.f;
+    float m = a[8];
+    m = fminf(m, fmaxf(a[7], -1.f));
+    m = fminf(m, fmaxf(a[6], -1.f));
+    m = fminf(m, fmaxf(a[5], c1));
+    m = fminf(m, fmaxf(a[4], c2));
+    m = fminf(m, fmaxf(a[3], c3));
+    m = fminf(m, fmaxf(a[2], c4));
+    m = fminf(m, fmaxf(a[1], 1.f));
+    m = fminf(m, fmaxf(a[0], 1.f));
+    return fmaxf(m, -1.f);
+}
+
+struct PdArgs {
+    FusionView f;
+    const float* pin;   // dual read
+    float* pout;        // dual written
+    const float* uin;
+    float* uout;
+    const float* vin;
+    float* vout;
+    float tau, lambda, tl, sigma;
+};
+
+// FusionUpdateU_kernel (fusion.cu:45-57) with divPBwd (fusion.h:793-802).
+__global__ void fusion_u_kernel(FusionView f, float tau, float tl) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= f.w || y >= f.h || z >= f.d) return;
+    const size_t V = f.V, i = ((size_t)z * f.h + y) * f.w + x;
+    const float* px = f.p;
+    const float* py = f.p + V;
+    const float* pz = f.p + 2 * V;
+    float div = px[i] + py[i] + pz[i];
+    if (x > 0) div -= px[i - 1];
+    if (y > 0) div -= py[i - f.w];
+    if (z > 0) div -= pz[i - (size_t)f.w * f.h];
+    const float un = f.u[i];
+    const float uu = un - tau * (-div);
+    const float unew = prox_hist(uu, f.hist + i * FB, tl);
+    f.u[i] = unew;
+    f.v[i] = 2.f * unew - un;
+}
+
+// FusionUpdateP_kernel (fusion.cu:59-70): p <- project(p + sigma grad+ v).
+__global__ void fusion_p_kernel(FusionView f, float sigma) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= f.w || y >= f.h || z >= f.d) return;
+    const size_t V = f.V, i = ((size_t)z * f.h + y) * f.w + x;
+    const float vv = f.v[i];
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+    if (x < f.w - 1) gx = f.v[i + 1] - vv;
+    if (y < f.h - 1) gy = f.v[i + f.w] - vv;
+    if (z < f.d - 1) gz = f.v[i + (size_t)f.w * f.h] - vv;
+    float ax = f.p[i] + sigma * gx, ay = f.p[V + i] + sigma * gy, az = f.p[2 * V + i] + sigma * gz;
+    const float dn = fmaxf(1.f, sqrtf(ax * ax + ay * ay + az * az));
+    f.p[i] = ax / dn;
+    f.p[V + i] = ay / dn;
+    f.p[2 * V + i] = az / dn;
+}
+
+// Fused iteration: P step of iteration k (from v_k) followed by the U step of
+// iteration k+1 (from the new p).  The U step at voxel i needs the new p at
+// i-x, i-y, i-z, which this thread recomputes from p_k and v_k (same
+// arithmetic, same result as the owner thread).  p and v are ping-ponged
+// because neighbouring voxels read their old values; u is read and written
+// only at the own voxel.
+__device__ inline void p_new_at(const PdArgs& a, size_t V, int x, int y, int z, size_t i, float& ox, float& oy,
+                                float& oz) {
+    const FusionView& f = a.f;
+    const float vv = a.vin[i];
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+    if (x < f.w - 1) gx = a.vin[i + 1] - vv;
+    if (y < f.h - 1) gy = a.vin[i + f.w] - vv;
+    if (z < f.d - 1) gz = a.vin[i + (size_t)f.w * f.h] - vv;
+    const float ax = a.pin[i] + a.sigma * gx, ay = a.pin[V + i] + a.sigma * gy, az = a.pin[2 * V + i] + a.sigma * gz;
+    const float dn = fmaxf(1.f, sqrtf(ax * ax + ay * ay + az * az));
+    ox = ax / dn;
+    oy = ay / dn;
+    oz = az / dn;
+}
+
+__global__ void fusion_pd_fused_kernel(PdArgs a) {
+    const FusionView& f = a.f;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= f.w || y >= f.h || z >= f.d) return;
+    const size_t V = f.V, i = ((size_t)z * f.h + y) * f.w + x;
+    const size_t sxy = (size_t)f.w * f.h;
+    float cx, cy, cz;
+    p_new_at(a, V, x, y, z, i, cx, cy, cz);
+    a.pout[i] = cx;
+    a.pout[V + i] = cy;
+    a.pout[2 * V + i] = cz;
+    float div = cx + cy + cz;
+    float tx, ty, tz;
+    if (x > 0) { p_new_at(a, V, x - 1, y, z, i - 1, tx, ty, tz); div -= tx; }
+    if (y > 0) { p_new_at(a, V, x, y - 1, z, i - f.w, tx, ty, tz); div -= ty; }
+    if (z > 0) { p_new_at(a, V, x, y, z - 1, i - sxy, tx, ty, tz); div -= tz; }
+    const float un = a.uin[i];
+    const float uu = un - a.tau * (-div);
+    const float unew = prox_hist(uu, f.hist + i * FB, a.tl);
+    a.uout[i] = unew;
+    a.vout[i] = 2.f * unew - un;
+}
+
+__global__ void surface_count_kernel(const float* __restrict__ u, size_t n, unsigned long long* out) {
+    __shared__ unsigned int part[256 / 64];
+    unsigned int c = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float v = u[i];
+        c += (v < 0.f && v > -1.f) ? 1u : 0u;
+    }
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int k = 0; k < 4; ++k) t += part[k];
+        atomicAdd(out, t);
+    }
+}
+
+static dim3 vox_grid(const FusionView& f, dim3 blk) {
+    return dim3((unsigned)((f.w + blk.x - 1) / blk.x), (unsigned)((f.h + blk.y - 1) / blk.y), (unsigned)f.d);
+}
+
+static int check_grid(const psw_fusion_grid* g) {
+    if (!g || !g->u || !g->v || !g->p || !g->hist) { set_error("fusion grid not allocated"); return PSW_ERR_ARG; }
+    if (g->width < 1 || g->height < 1 || g->depth < 1) { set_error("empty fusion grid"); return PSW_ERR_ARG; }
+    if (g->depth > 65535) { set_error("grid depth above 65535"); return PSW_ERR_ARG; }
+    return PSW_OK;
+}
+
+static Cam cam_of(const float* K, const float* R, const float* T) {
+    Cam c;
+    c.K = m3_from(K);
+    c.R = m3_from(R);
+    c.T = v3_from(T);
+    return c;
+}
+
+}  // namespace psw
+
+using namespace psw;
+
+extern "C" {
+
+int psw_fusion_alloc(psw_fusion_grid* g, int width, int height, int depth, const float* volume) {
+    PSW_CHECK_ARG(g && volume, "NULL pointer");
+    PSW_CHECK_ARG(width >= 1 && height >= 1 && depth >= 1 && depth <= 65535, "bad grid size");
+    std::memset(g, 0, sizeof(*g));
+    g->width = width; g->height = height; g->depth = depth;
+    for (int i = 0; i < 6; ++i) g->volume[i] = volume[i];
+    const size_t V = (size_t)width * height * depth;
+    hipError_t e = hipMalloc((void**)&g->u, V * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&g->v, V * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&g->p, 3 * V * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&g->hist, V * FB * sizeof(uint16_t));
+    if (e != hipSuccess) {
+        int s = hip_status(e, "psw_fusion_alloc", PSW_ERR_ALLOC);
+        psw_fusion_free(g);
+        return s;
+    }
+    int s = psw_fusion_clear(g, nullptr);
+    if (s == PSW_OK) PSW_TRY_HIP(hipDeviceSynchronize(), PSW_ERR_DEVICE);
+    return s;
+}
+
+int psw_fusion_free(psw_fusion_grid* g) {
+    if (!g) return PSW_OK;
+    if (g->u) (void)hipFree(g->u);
+    if (g->v) (void)hipFree(g->v);
+    if (g->p) (void)hipFree(g->p);
+    if (g->hist) (void)hipFree(g->hist);
+    g->u = g->v = g->p = nullptr;
+    g->hist = nullptr;
+    return PSW_OK;
+}
+
+int psw_fusion_clear(const psw_fusion_grid* g, psw_stream_t s) {
+    int st = check_grid(g);
+    if (st != PSW_OK) return st;
+    const size_t V = (size_t)g->width * g->height * g->depth;
+    if ((st = psw_fill_f32(g->u, 0.f, V, s)) != PSW_OK) return st;
+    if ((st = psw_fill_f32(g->v, 0.f, V, s)) != PSW_OK) return st;
+    if ((st = psw_fill_f32(g->p, 0.f, 3 * V, s)) != PSW_OK) return st;
+    return psw_fill_u32(reinterpret_cast<uint32_t*>(g->hist), 0u, V * FB / 2, s);
+}
+
+int psw_fusion_update_histogram(const psw_fusion_grid* g, const float* d_depthmap, int width, int height,
+                                const float* K, const float* R, const float* T, float threshold, psw_stream_t s) {
+    int st = check_grid(g);
+    if (st != PSW_OK) return st;
+    PSW_CHECK_ARG(!(threshold < 0.f), "threshold must not be negative");
+    PSW_CHECK_ARG(d_depthmap && K && R && T, "NULL pointer");
+    PSW_CHECK_ARG(width >= 1 && height >= 1, "bad depthmap size");
+    const FusionView f = view_of(g);
+    const dim3 blk(64, 4, 1);
+    fusion_hist_kernel<<<vox_grid(f, blk), blk, 0, S(s)>>>(f, cam_of(K, R, T), d_depthmap, width, height, threshold);
+    return launch_status("fusion_hist_kernel");
+}
+
+int psw_fusion_update_histogram_batch(const psw_fusion_grid* g, const float* d_depthmaps, int m, int width, int height,
+                                      const float* K, const float* R, const float* T, float threshold,
+                                      psw_stream_t s) {
+    int st = check_grid(g);
+    if (st != PSW_OK) return st;
+    PSW_CHECK_ARG(!(threshold < 0.f), "threshold must not be negative");
+    PSW_CHECK_ARG(d_depthmaps && K && R && T, "NULL pointer");
+    PSW_CHECK_ARG(m >= 0 && width >= 1 && height >= 1, "bad depthmap batch");
+    const FusionView f = view_of(g);
+    const dim3 blk(64, 4, 1);
+    for (int j0 = 0; j0 < m; j0 += FH_BATCH) {
+        CamBatch cb;
+        cb.m = std::min(FH_BATCH, m - j0);
+        for (int j = 0; j < cb.m; ++j) cb.cam[j] = cam_of(K + 9 * (j0 + j), R + 9 * (j0 + j), T + 3 * (j0 + j));
+        fusion_hist_batch_kernel<<<vox_grid(f, blk), blk, 0, S(s)>>>(f, cb, d_depthmaps + (size_t)j0 * width * height,
+                                                                     width, height, threshold);
+        st = launch_status("fusion_hist_batch_kernel");
+        if (st != PSW_OK) return st;
+    }
+    return PSW_OK;
+}
+
+int psw_fusion_update_u(const psw_fusion_grid* g, double tau, double lambda, psw_stream_t s) {
+    int st = check_grid(g);
+    if (st != PSW_OK) return st;
+    const FusionView f = view_of(g);
+    const float tf = (float)tau, lf = (float)lambda;
+    const dim3 blk(64, 4, 1);
+    fusion_u_kernel<<<vox_grid(f, blk), blk, 0, S(s)>>>(f, tf, tf * lf);
+    return launch_status("fusion_u_kernel");
+}
+
+int psw_fusion_update_p(const psw_fusion_grid* g, double sigma, psw_stream_t s) {
+    int st = check_grid(g);
+    if (st != PSW_OK) return st;
+    const FusionView f = view_of(g);
+    const dim3 blk(64, 4, 1);
+    fusion_p_kernel<<<vox_grid(f, blk), blk, 0, S(s)>>>(f, (float)sigma);
+    return launch_status("fusion_p_kernel");
+}
+
+int psw_fusion_update_iteration(const psw_fusion_grid* g, const float* d_depthmap, int width, int height,
+                                const float* K, const float* R, const float* T, float threshold, double tau,
+                                double lambda, double sigma, psw_stream_t s) {
+    int st = psw_fusion_update_histogram(g, d_depthmap, width, height, K, R, T, threshold, s);
+    if (st != PSW_OK) return st;
+    st = psw_fusion_update_u(g, tau, lambda, s);
+    if (st != PSW_OK) return st;
+    return psw_fusion_update_p(g, sigma, s);
+}
+
+int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double lambda, double sigma, psw_stream_t s) {
+    int st = check_grid(g);
+    if (st != PSW_OK) return st;
+    PSW_CHECK_ARG(niters >= 0, "niters must be >= 0");
+    if (niters == 0) return PSW_OK;
+    const FusionView f = view_of(g);
+    const size_t V = f.V;
+    // Iteration 1's U step on its own, then niters-1 fused (P_k, U_{k+1})
+    // launches, then the last P step: the same sequence U,P,U,P,...,U,P.
+    st = psw_fusion_update_u(g, tau, lambda, s);
+    if (st != PSW_OK) return st;
+    if (niters > 1) {
+        // Second dual/primal buffers for the ping-pong.
+        float *p2 = nullptr, *v2 = nullptr;
+        PSW_TRY_HIP(hipMalloc((void**)&p2, 3 * V * sizeof(float)), PSW_ERR_ALLOC);
+        hipError_t e = hipMalloc((void**)&v2, V * sizeof(float));
+        if (e != hipSuccess) {
+            (void)hipFree(p2);
+            return hip_status(e, "hipMalloc", PSW_ERR_ALLOC);
+        }
+        PdArgs a;
+        a.f = f;
+        a.tau = (float)tau; a.lambda = (float)lambda; a.tl = a.tau * a.lambda; a.sigma = (float)sigma;
+        float* pb[2] = {g->p, p2};
+        float* vb[2] = {g->v, v2};
+        const dim3 blk(64, 4, 1), grd = vox_grid(f, blk);
+        int cur = 0;
+        for (int k = 1; k < niters && st == PSW_OK; ++k) {
+            a.pin = pb[cur]; a.pout = pb[cur ^ 1];
+            a.vin = vb[cur]; a.vout = vb[cur ^ 1];
+            a.uin = g->u; a.uout = g->u;
+            fusion_pd_fused_kernel<<<grd, blk, 0, S(s)>>>(a);
+            st = launch_status("fusion_pd_fused_kernel");
+            cur ^= 1;
+        }
+        if (st == PSW_OK && cur == 1) {
+            // results live in the second buffers: move them home
+            PSW_TRY_HIP(hipMemcpyAsync(g->p, p2, 3 * V * sizeof(float), hipMemcpyDeviceToDevice, S(s)), PSW_ERR_COPY);
+            PSW_TRY_HIP(hipMemcpyAsync(g->v, v2, V * sizeof(float), hipMemcpyDeviceToDevice, S(s)), PSW_ERR_COPY);
+        }
+        (void)hipStreamSynchronize(S(s));
+        (void)hipFree(p2);
+        (void)hipFree(v2);
+        if (st != PSW_OK) return st;
+    }
+    return psw_fusion_update_p(g, sigma, s);
+}
+
+int psw_fusion_surface_count(const psw_fusion_grid* g, int64_t* count, psw_stream_t s) {
+    int st = check_grid(g);
+    if (st != PSW_OK) return st;
+    PSW_CHECK_ARG(count, "count is NULL");
+    const size_t V = (size_t)g->width * g->height * g->depth;
+    unsigned long long* d = nullptr;
+    PSW_TRY_HIP(hipMalloc((void**)&d, sizeof(unsigned long long)), PSW_ERR_ALLOC);
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(unsigned long long), S(s));
+    if (e == hipSuccess) {
+        size_t blocks = (V + 255) / 256;
+        if (blocks > 4096) blocks = 4096;
+        surface_count_kernel<<<(unsigned)blocks, 256, 0, S(s)>>>(g->u, V, d);
+        e = hipGetLastError();
+    }
+    unsigned long long h = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, S(s));
+    if (e == hipSuccess) e = hipStreamSynchronize(S(s));
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_status(e, "psw_fusion_surface_count");
+    *count = (int64_t)h;
+    return PSW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,508 @@
+// sweep_fused.hip -- fused plane sweep for gfx950.
+//
+// One workgroup owns a 32x16 tile of reference pixels and sweeps every
+// (plane, source) pair over it without leaving the CU:
+//   warp   : homography + bilinear sample of the source on the tile plus a
+//            window-radius halo, into LDS (reflect-101 halo positions, as the
+//            reference's box filters, kernels.cu:203-204 / :228-229);
+//   column : per column, register-blocked sums of I, I^2 and Ref*I over the
+//            window (4 output rows per thread), into LDS;
+//   row    : per pixel pair, sums of the column means -> mean, E[I^2],
+//            E[Ref*I] -> std, NCC (kernels.cu:46-64, :103-120), reduction.
+// The reference instead writes eleven full-frame images to HBM per
+// (plane, source) (src/planesweep.cpp:206-249).
+//
+// Two reductions share the per-(pixel, plane, source) NCC:
+//   reference : per source, best NCC over planes (strict >, zero start) and
+//               its depth; depth averaged over sources whose best NCC >
+//               nccthresh; zfar where none (src/planesweep.cpp:125-131,
+//               kernels.cu:66-101, :145-157, :331-340);
+//   aggregate : cost(p) = -(sum_s NCC_s) / N; argmin (first), minimum and
+//               second-smallest cost.
+// With FAST=false every sum, product and quotient is taken in the reference's
+// order (column mean = sum/win, row mean of column means = sum/win), so the
+// NCC is bit-identical to the reference's float arithmetic.  FAST=true
+// multiplies by 1/win and lets the compiler fuse multiply-adds.
+#include "psw_common.h"
+
+namespace psw {
+
+constexpr int SW_TX = 32;       // tile width  (pixels)
+constexpr int SW_TY = 16;       // tile height (pixels)
+constexpr int SW_THREADS = 256; // 4 waves
+constexpr int SW_CR = 4;        // rows per thread in the column pass
+constexpr int SW_PX = 2;        // pixels per thread in the row pass
+
+struct SweepArgs {
+    const float* ref;
+    const float* src;     // [N][H][W]
+    const float* Hs;      // [N][P][9]
+    const float* depths;  // [P] (reference mode)
+    int W, H, N, P;
+    int off;              // pixel-coordinate offset of the homography (1 = reference)
+    int tiles_x, ntiles;
+    float stdthresh, nccthresh, zfar;
+    float* depth_out;     // reference mode
+    int32_t* argmin;      // aggregate mode
+    float* cmin;
+    float* csec;
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8).  Give
+// each XCD a contiguous band of tiles so that the source rows its tiles warp
+// from stay in that XCD's L2.
+__device__ inline int xcd_tile(int b, int ntiles) {
+    const int per = (ntiles + 7) / 8;
+    const int xcd = b & 7, k = b >> 3;
+    const int full = ntiles - (per - 1) * 8;  // XCDs that get 'per' tiles
+    int t;
+    if (xcd < full) t = xcd * per + k;
+    else t = full * per + (xcd - full) * (per - 1) + k;
+    return t;
+}
+
+template <bool FAST>
+__device__ inline float scale_mean(float s, float win, float rwin) {
+    if constexpr (FAST) return s * rwin;
+    else return s / win;
+}
+
+template <int R, int MODE, bool FAST>
+__global__ __launch_bounds__(SW_THREADS) void sweep_fused_kernel(SweepArgs a) {
+    constexpr int TW = SW_TX + 2 * R;
+    constexpr int TH = SW_TY + 2 * R;
+    constexpr int WIN = 2 * R + 1;
+    constexpr int NCG = SW_TY / SW_CR;          // column-pass row groups
+    constexpr int NCOL = NCG * TW;              // column-pass work items
+    static_assert(NCOL <= SW_THREADS, "window too large for the tile");
+    __shared__ float sR[TH * TW];
+    __shared__ float sW[TH * TW];
+    __shared__ float sC[3][SW_TY * TW];
+
+    const int b = blockIdx.x;
+    if (b >= a.ntiles) return;
+    const int t = xcd_tile(b, a.ntiles);
+    const int x0 = (t % a.tiles_x) * SW_TX;
+    const int y0 = (t / a.tiles_x) * SW_TY;
+    const int tid = threadIdx.x;
+    const int W = a.W, H = a.H;
+    const float win = (float)WIN, rwin = 1.f / (float)WIN;
+
+    // Actual (reflected) image coordinate of each halo-tile position this
+    // thread warps; positions past W-1+R / H-1+R only feed discarded pixels.
+    constexpr int NWARP = (TH * TW + SW_THREADS - 1) / SW_THREADS;
+    int wgx[NWARP], wgy[NWARP];
+#pragma unroll
+    for (int k = 0; k < NWARP; ++k) {
+        const int i = tid + k * SW_THREADS;
+        const int ty = i / TW, tx = i - (i / TW) * TW;
+        wgx[k] = reflect101(min(x0 - R + tx, W - 1 + R), W);
+        wgy[k] = reflect101(min(y0 - R + ty, H - 1 + R), H);
+    }
+#pragma unroll
+    for (int k = 0; k < NWARP; ++k) {
+        const int i = tid + k * SW_THREADS;
+        if (i < TH * TW) sR[i] = a.ref[(size_t)wgy[k] * W + wgx[k]];
+    }
+    __syncthreads();
+
+    // Column-pass item of this thread.
+    const bool col_active = tid < NCOL;
+    const int ccx = tid % TW, cg = tid / TW;
+    // Row-pass pixels of this thread: (ox, oy), (ox+1, oy).
+    const int rox = SW_PX * (tid % (SW_TX / SW_PX));
+    const int roy = tid / (SW_TX / SW_PX);
+
+    // ---- reference window statistics (src/planesweep.cpp:108-119) ----
+    if (col_active) {
+        float rv[SW_CR + 2 * R];
+#pragma unroll
+        for (int i = 0; i < SW_CR + 2 * R; ++i) rv[i] = sR[(cg * SW_CR + i) * TW + ccx];
+#pragma unroll
+        for (int j = 0; j < SW_CR; ++j) {
+            float s1 = 0.f, s2 = 0.f;
+#pragma unroll
+            for (int i = 0; i < WIN; ++i) {
+                s1 += rv[j + i];
+                s2 += rv[j + i] * rv[j + i];
+            }
+            sC[0][(cg * SW_CR + j) * TW + ccx] = scale_mean<FAST>(s1, win, rwin);
+            sC[1][(cg * SW_CR + j) * TW + ccx] = scale_mean<FAST>(s2, win, rwin);
+        }
+    }
+    __syncthreads();
+    float rmean[SW_PX], rstd[SW_PX];
+    {
+        float c0[SW_PX + 2 * R], c1[SW_PX + 2 * R];
+#pragma unroll
+        for (int i = 0; i < SW_PX + 2 * R; ++i) {
+            c0[i] = sC[0][roy * TW + rox + i];
+            c1[i] = sC[1][roy * TW + rox + i];
+        }
+#pragma unroll
+        for (int k = 0; k < SW_PX; ++k) {
+            float m = 0.f, q = 0.f;
+#pragma unroll
+            for (int i = 0; i < WIN; ++i) {
+                m += c0[k + i];
+                q += c1[k + i];
+            }
+            m = scale_mean<FAST>(m, win, rwin);
+            q = scale_mean<FAST>(q, win, rwin);
+            const float var = q - m * m;
+            rmean[k] = m;
+            rstd[k] = var > 0.f ? sqrtf(var) : 0.f;
+        }
+    }
+    __syncthreads();
+
+    // One (plane, source) step: warp into sW, window sums, NCC of my pixels.
+    auto plane_ncc = [&](const float* __restrict__ src, const float* __restrict__ Hm, float ncc[SW_PX]) {
+        const float h0 = Hm[0], h1 = Hm[1], h2 = Hm[2], h3 = Hm[3], h4 = Hm[4], h5 = Hm[5], h6 = Hm[6],
+                    h7 = Hm[7], h8 = Hm[8];
+        const float offf = (float)a.off;
+#pragma unroll
+        for (int k = 0; k < NWARP; ++k) {
+            const int i = tid + k * SW_THREADS;
+            if (i < TH * TW) {
+                const float px = (float)(wgx[k] + a.off), py = (float)(wgy[k] + a.off);
+                float xs, ys;
+                if constexpr (FAST) {
+#pragma clang fp contract(fast)
+                    const float vx = h0 * px + h1 * py + h2;
+                    const float vy = h3 * px + h4 * py + h5;
+                    const float vz = h6 * px + h7 * py + h8;
+                    const float rz = __builtin_amdgcn_rcpf(vz);
+                    xs = vx * rz - offf;
+                    ys = vy * rz - offf;
+                } else {
+                    const float vx = dot3(h0, h1, h2, px, py, 1.f);
+                    const float vy = dot3(h3, h4, h5, px, py, 1.f);
+                    const float vz = dot3(h6, h7, h8, px, py, 1.f);
+                    xs = vx / vz - offf;
+                    ys = vy / vz - offf;
+                }
+                sW[i] = bilinear_sample(src, W, H, xs, ys);
+            }
+        }
+        __syncthreads();
+        if (col_active) {
+            float wv[SW_CR + 2 * R], rv[SW_CR + 2 * R];
+#pragma unroll
+            for (int i = 0; i < SW_CR + 2 * R; ++i) {
+                wv[i] = sW[(cg * SW_CR + i) * TW + ccx];
+                rv[i] = sR[(cg * SW_CR + i) * TW + ccx];
+            }
+#pragma unroll
+            for (int j = 0; j < SW_CR; ++j) {
+                float s1 = 0.f, s2 = 0.f, s3 = 0.f;
+#pragma unroll
+                for (int i = 0; i < WIN; ++i) {
+                    const float w = wv[j + i];
+                    s1 += w;
+                    s2 += w * w;
+                    s3 += rv[j + i] * w;
+                }
+                sC[0][(cg * SW_CR + j) * TW + ccx] = scale_mean<FAST>(s1, win, rwin);
+                sC[1][(cg * SW_CR + j) * TW + ccx] = scale_mean<FAST>(s2, win, rwin);
+                sC[2][(cg * SW_CR + j) * TW + ccx] = scale_mean<FAST>(s3, win, rwin);
+            }
+        }
+        __syncthreads();
+        float c0[SW_PX + 2 * R], c1[SW_PX + 2 * R], c2[SW_PX + 2 * R];
+#pragma unroll
+        for (int i = 0; i < SW_PX + 2 * R; ++i) {
+            c0[i] = sC[0][roy * TW + rox + i];
+            c1[i] = sC[1][roy * TW + rox + i];
+            c2[i] = sC[2][roy * TW + rox + i];
+        }
+#pragma unroll
+        for (int k = 0; k < SW_PX; ++k) {
+            float m = 0.f, q = 0.f, pm = 0.f;
+#pragma unroll
+            for (int i = 0; i < WIN; ++i) {
+                m += c0[k + i];
+                q += c1[k + i];
+                pm += c2[k + i];
+            }
+            m = scale_mean<FAST>(m, win, rwin);
+            q = scale_mean<FAST>(q, win, rwin);
+            pm = scale_mean<FAST>(pm, win, rwin);
+            const float var = q - m * m;
+            const float sd = var > 0.f ? sqrtf(var) : 0.f;
+            if ((rstd[k] < a.stdthresh) || (sd < a.stdthresh)) ncc[k] = 0.f;
+            else ncc[k] = (pm - rmean[k] * m) / (rstd[k] * sd);
+        }
+        // No barrier here: the next warp writes sW (its readers passed the
+        // barrier above); the next column pass writes sC only after the next
+        // barrier, which every thread reaches after finishing this row pass.
+    };
+
+    const size_t npix = (size_t)W * H;
+    if constexpr (MODE == 0) {
+        float gsum[SW_PX], gcnt[SW_PX];
+#pragma unroll
+        for (int k = 0; k < SW_PX; ++k) { gsum[k] = 0.f; gcnt[k] = 0.f; }
+        for (int s = 0; s < a.N; ++s) {
+            const float* src = a.src + (size_t)s * npix;
+            float best[SW_PX], bd[SW_PX];
+#pragma unroll
+            for (int k = 0; k < SW_PX; ++k) { best[k] = 0.f; bd[k] = 0.f; }
+            for (int p = 0; p < a.P; ++p) {
+                float ncc[SW_PX];
+                plane_ncc(src, a.Hs + ((size_t)s * a.P + p) * 9, ncc);
+                const float d = a.depths[p];
+#pragma unroll
+                for (int k = 0; k < SW_PX; ++k)
+                    if (ncc[k] > best[k]) { best[k] = ncc[k]; bd[k] = d; }
+            }
+#pragma unroll
+            for (int k = 0; k < SW_PX; ++k)
+                if (best[k] > a.nccthresh) { gsum[k] += bd[k]; gcnt[k] += 1.f; }
+        }
+#pragma unroll
+        for (int k = 0; k < SW_PX; ++k) {
+            const int x = x0 + rox + k, y = y0 + roy;
+            if (x < W && y < H) {
+                float v = (gcnt[k] != 0.f) ? gsum[k] / gcnt[k] : __builtin_nanf("");
+                if (v != v) v = a.zfar;
+                a.depth_out[(size_t)y * W + x] = v;
+            }
+        }
+    } else {
+        float cmin[SW_PX], csec[SW_PX];
+        int arg[SW_PX];
+        const float inf = __builtin_inff();
+#pragma unroll
+        for (int k = 0; k < SW_PX; ++k) { cmin[k] = inf; csec[k] = inf; arg[k] = -1; }
+        const float fN = (float)a.N;
+        for (int p = 0; p < a.P; ++p) {
+            float acc[SW_PX];
+#pragma unroll
+            for (int k = 0; k < SW_PX; ++k) acc[k] = 0.f;
+            for (int s = 0; s < a.N; ++s) {
+                float ncc[SW_PX];
+                plane_ncc(a.src + (size_t)s * npix, a.Hs + ((size_t)s * a.P + p) * 9, ncc);
+#pragma unroll
+                for (int k = 0; k < SW_PX; ++k) acc[k] += ncc[k];
+            }
+#pragma unroll
+            for (int k = 0; k < SW_PX; ++k) {
+                const float c = -acc[k] / fN;
+                if (c < cmin[k]) { csec[k] = cmin[k]; cmin[k] = c; arg[k] = p; }
+                else if (c < csec[k]) csec[k] = c;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < SW_PX; ++k) {
+            const int x = x0 + rox + k, y = y0 + roy;
+            if (x < W && y < H) {
+                const size_t i = (size_t)y * W + x;
+                a.argmin[i] = arg[k];
+                a.cmin[i] = cmin[k];
+                a.csec[i] = csec[k];
+            }
+        }
+    }
+}
+
+// ---- FP64 homographies -----------------------------------------------------
+
+struct HomArgs {
+    double K[9], Kinv[9];
+    double Rrel[PSW_MAX_SOURCES][9];
+    double trel[PSW_MAX_SOURCES][3];
+    int N, P;
+    const float* depths;
+    float* H;
+};
+
+__device__ __host__ inline double d_dot3(double a0, double a1, double a2, double b0, double b1, double b2) {
+    return a0 * b0 + a1 * b1 + a2 * b2;
+}
+__device__ __host__ inline void d_mm3(const double* A, const double* B, double* C) {
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            C[3 * i + j] = d_dot3(A[3 * i], A[3 * i + 1], A[3 * i + 2], B[j], B[3 + j], B[6 + j]);
+}
+
+// H = K (Rrel + trel e3^T / d) K^-1, then H / H(2,2) (src/planesweep.cpp:196-203),
+// in double; one thread per (source, plane).
+__global__ void sweep_homography_kernel(HomArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.N * a.P) return;
+    const int s = i / a.P, p = i - (i / a.P) * a.P;
+    const double d = (double)a.depths[p];
+    double M[9];
+    for (int j = 0; j < 9; ++j) M[j] = a.Rrel[s][j];
+    // trel e3^T has trel in its third column; the rest is 0 / d = 0.
+    M[2] = M[2] + a.trel[s][0] / d;
+    M[5] = M[5] + a.trel[s][1] / d;
+    M[8] = M[8] + a.trel[s][2] / d;
+    M[0] = M[0] + 0.0 / d; M[1] = M[1] + 0.0 / d;
+    M[3] = M[3] + 0.0 / d; M[4] = M[4] + 0.0 / d;
+    M[6] = M[6] + 0.0 / d; M[7] = M[7] + 0.0 / d;
+    double KM[9], Hm[9];
+    d_mm3(a.K, M, KM);
+    d_mm3(KM, a.Kinv, Hm);
+    const double h22 = Hm[8];
+    float* out = a.H + (size_t)i * 9;
+    for (int j = 0; j < 9; ++j) out[j] = (float)(Hm[j] / h22);
+}
+
+// Matrix3D::inv in double (src/inc/structs.h:565-593 order).
+static void d_inv3(const double* m, double* o) {
+    const double det = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+                       m[2] * (m[3] * m[7] - m[4] * m[6]);
+    const double r[9] = {m[4] * m[8] - m[5] * m[7], m[2] * m[7] - m[1] * m[8], m[1] * m[5] - m[2] * m[4],
+                         m[5] * m[6] - m[3] * m[8], m[0] * m[8] - m[2] * m[6], m[2] * m[3] - m[0] * m[5],
+                         m[3] * m[7] - m[4] * m[6], m[1] * m[6] - m[0] * m[7], m[0] * m[4] - m[1] * m[3]};
+    for (int j = 0; j < 9; ++j) o[j] = r[j] / det;
+}
+static void d_trans3(const double* m, double* o) {
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) o[3 * j + i] = m[3 * i + j];
+}
+static void d_mv3(const double* M, const double* v, double* o) {
+    for (int i = 0; i < 3; ++i) o[i] = d_dot3(M[3 * i], M[3 * i + 1], M[3 * i + 2], v[0], v[1], v[2]);
+}
+
+// PlaneSweep::RelativeMatrices (src/planesweep.cpp:538-549) in double.
+static void d_relative(const float* Rref, const float* tref, const float* Rsrc, const float* tsrc, bool alt,
+                       double* Rrel, double* trel) {
+    double Rr[9], Rs[9], tr[3], ts[3];
+    for (int j = 0; j < 9; ++j) { Rr[j] = Rref[j]; Rs[j] = Rsrc[j]; }
+    for (int j = 0; j < 3; ++j) { tr[j] = tref[j]; ts[j] = tsrc[j]; }
+    if (!alt) {
+        double Ri[9], Rt[3];
+        d_inv3(Rr, Ri);
+        d_mm3(Rs, Ri, Rrel);
+        d_mv3(Rrel, tr, Rt);
+        for (int j = 0; j < 3; ++j) trel[j] = ts[j] - Rt[j];
+    } else {
+        double RsT[9], dt[3];
+        d_trans3(Rs, RsT);
+        d_mm3(RsT, Rr, Rrel);
+        for (int j = 0; j < 3; ++j) dt[j] = tr[j] - ts[j];
+        d_mv3(RsT, dt, trel);
+    }
+}
+
+template <int R>
+static int launch_sweep_r(const SweepArgs& a, int mode, bool fast, hipStream_t st) {
+    const dim3 grid((unsigned)a.ntiles), block(SW_THREADS);
+    if (mode == 0) {
+        if (fast) sweep_fused_kernel<R, 0, true><<<grid, block, 0, st>>>(a);
+        else sweep_fused_kernel<R, 0, false><<<grid, block, 0, st>>>(a);
+    } else {
+        if (fast) sweep_fused_kernel<R, 1, true><<<grid, block, 0, st>>>(a);
+        else sweep_fused_kernel<R, 1, false><<<grid, block, 0, st>>>(a);
+    }
+    return launch_status("sweep_fused_kernel");
+}
+
+static int launch_sweep(SweepArgs a, int winsize, int mode, bool fast, hipStream_t st) {
+    a.tiles_x = (a.W + SW_TX - 1) / SW_TX;
+    const int tiles_y = (a.H + SW_TY - 1) / SW_TY;
+    a.ntiles = a.tiles_x * tiles_y;
+    switch (winsize / 2) {
+        case 0: return launch_sweep_r<0>(a, mode, fast, st);
+        case 1: return launch_sweep_r<1>(a, mode, fast, st);
+        case 2: return launch_sweep_r<2>(a, mode, fast, st);
+        case 3: return launch_sweep_r<3>(a, mode, fast, st);
+        case 4: return launch_sweep_r<4>(a, mode, fast, st);
+        case 5: return launch_sweep_r<5>(a, mode, fast, st);
+        case 6: return launch_sweep_r<6>(a, mode, fast, st);
+        case 7: return launch_sweep_r<7>(a, mode, fast, st);
+        default:
+            set_error("window size %d not supported (odd, 1..15)", winsize);
+            return PSW_ERR_UNSUPPORTED;
+    }
+}
+
+static int check_sweep_args(const float* d_ref, const float* d_src, int width, int height, int nsrc,
+                            const float* d_H, int nplanes, int winsize) {
+    if (!d_ref || !d_src || !d_H) { set_error("NULL pointer"); return PSW_ERR_ARG; }
+    if (nsrc < 1 || nplanes < 1) { set_error("need at least one source and one plane"); return PSW_ERR_ARG; }
+    if (winsize < 1 || winsize % 2 == 0) {  // PlaneSweep::setWindowSize, planesweep.h:234
+        set_error("Window size must be an odd number (got %d)", winsize);
+        return PSW_ERR_ARG;
+    }
+    if (width < 2 || height < 2 || width <= winsize / 2 || height <= winsize / 2) {
+        set_error("image %d x %d too small for window %d", width, height, winsize);
+        return PSW_ERR_ARG;
+    }
+    return PSW_OK;
+}
+
+__global__ void argmin_depth_kernel(float* out, const int32_t* arg, const float* depths, int n, float fallback) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const int a = arg[i];
+        out[i] = a >= 0 ? depths[a] : fallback;
+    }
+}
+
+}  // namespace psw
+
+using namespace psw;
+
+extern "C" {
+
+int psw_sweep_homographies(float* d_H, const float* d_depths, int nplanes, int nsrc, const float* K,
+                           const float* Rref, const float* tref, const float* Rsrc, const float* tsrc,
+                           int alternative, psw_stream_t s) {
+    PSW_CHECK_ARG(d_H && d_depths && K && Rref && tref && Rsrc && tsrc, "NULL pointer");
+    PSW_CHECK_ARG(nsrc >= 1 && nsrc <= PSW_MAX_SOURCES, "nsrc must be 1..PSW_MAX_SOURCES");
+    PSW_CHECK_ARG(nplanes >= 1, "nplanes must be positive");
+    HomArgs a;
+    for (int j = 0; j < 9; ++j) a.K[j] = K[j];
+    d_inv3(a.K, a.Kinv);
+    for (int i = 0; i < nsrc; ++i) d_relative(Rref, tref, Rsrc + 9 * i, tsrc + 3 * i, alternative != 0, a.Rrel[i], a.trel[i]);
+    a.N = nsrc;
+    a.P = nplanes;
+    a.depths = d_depths;
+    a.H = d_H;
+    const int n = nsrc * nplanes;
+    sweep_homography_kernel<<<(n + 63) / 64, 64, 0, S(s)>>>(a);
+    return launch_status("sweep_homography_kernel");
+}
+
+int psw_sweep_fused_reference(const float* d_ref, const float* d_src, int width, int height, int nsrc,
+                              const float* d_H, const float* d_depths, int nplanes, int winsize, float stdthresh,
+                              float nccthresh, float zfar, float* d_depth_out, psw_stream_t s) {
+    int st = check_sweep_args(d_ref, d_src, width, height, nsrc, d_H, nplanes, winsize);
+    if (st != PSW_OK) return st;
+    PSW_CHECK_ARG(d_depths && d_depth_out, "NULL pointer");
+    SweepArgs a = {};
+    a.ref = d_ref; a.src = d_src; a.Hs = d_H; a.depths = d_depths;
+    a.W = width; a.H = height; a.N = nsrc; a.P = nplanes; a.off = 1;
+    a.stdthresh = stdthresh; a.nccthresh = nccthresh; a.zfar = zfar;
+    a.depth_out = d_depth_out;
+    return launch_sweep(a, winsize, 0, false, S(s));
+}
+
+int psw_sweep_fused_aggregate(const float* d_ref, const float* d_src, int width, int height, int nsrc,
+                              const float* d_H, int nplanes, int winsize, float stdthresh, int coord_offset, int fast,
+                              int32_t* d_argmin, float* d_cost_min, float* d_cost_second, psw_stream_t s) {
+    int st = check_sweep_args(d_ref, d_src, width, height, nsrc, d_H, nplanes, winsize);
+    if (st != PSW_OK) return st;
+    PSW_CHECK_ARG(d_argmin && d_cost_min && d_cost_second, "NULL pointer");
+    PSW_CHECK_ARG(coord_offset == 0 || coord_offset == 1, "coord_offset must be 0 or 1");
+    SweepArgs a = {};
+    a.ref = d_ref; a.src = d_src; a.Hs = d_H;
+    a.W = width; a.H = height; a.N = nsrc; a.P = nplanes; a.off = coord_offset;
+    a.stdthresh = stdthresh;
+    a.argmin = d_argmin; a.cmin = d_cost_min; a.csec = d_cost_second;
+    return launch_sweep(a, winsize, 1, fast != 0, S(s));
+}
+
+int psw_sweep_argmin_depth(float* d_depth, const int32_t* d_argmin, const float* d_depths, int npixels,
+                           float fallback, psw_stream_t s) {
+    PSW_CHECK_ARG(d_depth && d_argmin && d_depths, "NULL pointer");
+    if (npixels <= 0) return PSW_OK;
+    argmin_depth_kernel<<<(npixels + 255) / 256, 256, 0, S(s)>>>(d_depth, d_argmin, d_depths, npixels, fallback);
+    return launch_status("argmin_depth_kernel");
+}
+
+}  // extern "C"

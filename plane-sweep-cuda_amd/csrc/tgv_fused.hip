@@ -1,0 +1,641 @@
+// tgv_fused.hip -- TGV2 multiview primal-dual depth refinement for gfx950.
+//
+// The reference runs, per iteration, P, Q, prodsum:=0, N x R and U as 4+N
+// launches over full-frame images (src/planesweep.cpp:485-509).  Here one
+// kernel does the whole iteration for a 64x8 tile:
+//   phase 1: ubar, u1bar on the tile plus a one-pixel halo into LDS, formed
+//            on the fly as 2*u_k - u_{k-1} (the value the reference's U step
+//            stores as ubar, TGV2_kernels.cu:334-336);
+//   phase 2: new p (TGV2_kernels.cu:340-365) and q (:54-85) on the tile plus
+//            its left/top halo, into LDS and, for the tile, to HBM;
+//   phase 3: r of every source and prodsum (:87-105) in registers, then the
+//            tensor-weighted u / u1 step (:300-338) from the LDS duals.
+// Buffers read at neighbours by other tiles are never written in place:
+// the primal (u, u1x, u1y) cycles through three buffers (k-1, k, k+1) and
+// the duals through two.  q_w is not stored: the reference updates it with
+// exactly the q_z formula from the same zero start, so it always equals q_z.
+//
+// Arithmetic is FP32 throughout (the reference's double temporaries in the
+// P and Q steps become float); FP64 is used only in the energy reductions
+// that decide stopping.
+#include "psw_common.h"
+
+namespace psw {
+
+constexpr int TG_TX = 64;
+constexpr int TG_TY = 8;
+constexpr int TG_THREADS = 256;
+
+struct TgvLayout {
+    size_t n;
+    size_t U, P, Q, r, It, Iu, T, u0, ref, src, total;
+    void init(int W, int H, int N) {
+        n = (size_t)W * H;
+        size_t o = 0;
+        U = o;  o += 9 * n;      // [3 buffers][u, u1x, u1y]
+        P = o;  o += 4 * n;      // [2 buffers][px, py]
+        Q = o;  o += 6 * n;      // [2 buffers][qx, qy, qz]
+        r = o;  o += (size_t)N * n;
+        It = o; o += (size_t)N * n;
+        Iu = o; o += (size_t)N * n;
+        T = o;  o += 3 * n;      // T11, T12 (= T21), T22
+        u0 = o; o += n;
+        ref = o; o += n;         // normalised reference
+        src = o; o += (size_t)N * n;
+        total = o;
+    }
+};
+
+struct TgvCtrl {
+    int done;
+    int iters_at_done;
+    int pad[2];
+    double energy;
+    double energy_at_done;
+};
+
+}  // namespace psw
+
+struct psw_tgv {
+    int W, H, N;
+    psw::TgvLayout L;
+    float* base;
+    double* partials;     // per-block energy partial sums
+    int npartials;
+    psw::TgvCtrl* ctrl;   // device
+    psw::TgvCtrl* hctrl;  // pinned host mirror
+    int k;                // iterations since the warp started
+    bool have_images;
+};
+
+namespace psw {
+
+struct TgvPoses {
+    M3f K, invK;
+    M3f Rrel[PSW_MAX_SOURCES];
+    M3f RK[PSW_MAX_SOURCES];  // Rrel * invK
+    V3f trel[PSW_MAX_SOURCES];
+    float fx, fy;
+};
+
+struct TgvPtrs {
+    const float* ucur;   // [3][n] u_k
+    const float* uprev;  // [3][n] u_{k-1}
+    float* unext;        // [3][n] u_{k+1}
+    const float* pcur;   // [2][n]
+    float* pnext;
+    const float* qcur;   // [3][n]
+    float* qnext;
+    float* r;            // [N][n]
+    const float* It;
+    const float* Iu;
+    const float* T;      // [3][n]
+    const float* u0;
+    const TgvCtrl* ctrl;
+    int W, H, N;
+    float lambda, alpha0, alpha1, tau, sigma;
+};
+
+// ---- set-up kernels --------------------------------------------------------
+
+__global__ void tgv_scale_kernel(float* __restrict__ out, const float* __restrict__ in, size_t n) {
+    const float sc = 1.f / 255.f;  // element_scale(Ref, 1/255.f), planesweep.cpp:419
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = in[i] * sc;
+}
+
+// Anisotropic_diffusion_tensor_kernel (TGV2_kernels.cu:257-298); T21 = T12.
+__global__ void tgv_tensor_kernel(float* __restrict__ T, const float* __restrict__ img, int W, int H, float beta,
+                                  float gamma) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const size_t n = (size_t)W * H, i = (size_t)y * W + x;
+    const int xn = min(x + 1, W - 1), yn = min(y + 1, H - 1);
+    float gx = img[(size_t)y * W + xn] - img[i];
+    float gy = img[(size_t)yn * W + x] - img[i];
+    const float d = sqrtf(gx * gx + gy * gy);
+    float t11 = 1.f, t12 = 0.f, t22 = 1.f;
+    if (d > 0.f) {
+        gx = gx / d;
+        gy = gy / d;
+        const float k = expf(-beta * powf(d, gamma));
+        t11 = k * gx * gx + gy * gy;
+        t12 = (k - 1.f) * gx * gy;
+        t22 = k * gy * gy + gx * gx;
+    }
+    T[i] = t11;
+    T[n + i] = t12;
+    T[2 * n + i] = t22;
+}
+
+// Warped source intensity at pixel (x, y) for depth u (transform_coordinates
+// TGV2_kernels.cu:156-177 + bilinear kernels.cu:4-29), 1-based convention.
+__device__ inline float tgv_warp(const TgvPoses& pz, int j, const float* __restrict__ src, int W, int H, int x, int y,
+                                 float u, float& X, float& Y, float& Z) {
+    float a0, a1, a2;
+    mv3(pz.invK, (float)(x + 1), (float)(y + 1), 1.f, a0, a1, a2);
+    a0 = u * a0; a1 = u * a1; a2 = u * a2;
+    float b0, b1, b2;
+    mv3(pz.Rrel[j], a0, a1, a2, b0, b1, b2);
+    X = b0 + pz.trel[j].x;
+    Y = b1 + pz.trel[j].y;
+    Z = b2 + pz.trel[j].z;
+    float c0, c1, c2;
+    mv3(pz.K, X, Y, Z, c0, c1, c2);
+    return bilinear_sample(src, W, H, c0 / c2 - 1.f, c1 / c2 - 1.f);
+}
+
+// One warp's linearisation for every source (planesweep.cpp:460-483):
+// It_j = I_j(warp(u0)) - Ref, Iu_j = dI_j/du (TGV2_kernels.cu:194-255), r_j = 0.
+__global__ void tgv_linearise_kernel(TgvPoses pz, const float* __restrict__ src, const float* __restrict__ ref,
+                                     const float* __restrict__ u0, float* __restrict__ It, float* __restrict__ Iu,
+                                     float* __restrict__ r, int W, int H, int N) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const size_t n = (size_t)W * H, i = (size_t)y * W + x;
+    const int xn = min(x + 1, W - 1), yn = min(y + 1, H - 1);
+    const float uc = u0[i], ux = u0[(size_t)y * W + xn], uy = u0[(size_t)yn * W + x];
+    for (int j = 0; j < N; ++j) {
+        const float* S = src + (size_t)j * n;
+        float X, Y, Z, Xd, Yd, Zd;
+        const float Ic = tgv_warp(pz, j, S, W, H, x, y, uc, X, Y, Z);
+        const float Ix = tgv_warp(pz, j, S, W, H, xn, y, ux, Xd, Yd, Zd);
+        const float Iy = tgv_warp(pz, j, S, W, H, x, yn, uy, Xd, Yd, Zd);
+        float dX, dY, dZ;
+        mv3(pz.RK[j], (float)(x + 1), (float)(y + 1), 1.f, dX, dY, dZ);
+        const float dfx = pz.fx * (dX * Z - X * dZ) / (Z * Z);
+        const float dfy = pz.fy * (dY * Z - Y * dZ) / (Z * Z);
+        Iu[(size_t)j * n + i] = (Ix - Ic) * dfx + (Iy - Ic) * dfy;
+        It[(size_t)j * n + i] = Ic - ref[i];
+        r[(size_t)j * n + i] = 0.f;
+    }
+}
+
+// Start of a warp: u_k = u_{k-1} = u (so ubar = u), u1 = 0 in both, duals 0.
+__global__ void tgv_warp_reset_kernel(float* __restrict__ U, float* __restrict__ P, float* __restrict__ Q,
+                                      float* __restrict__ u0, size_t n, int cur, int prev) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float u = U[(size_t)cur * 3 * n + i];
+        u0[i] = u;
+        U[(size_t)prev * 3 * n + i] = u;
+        U[(size_t)cur * 3 * n + n + i] = 0.f;
+        U[(size_t)cur * 3 * n + 2 * n + i] = 0.f;
+        U[(size_t)prev * 3 * n + n + i] = 0.f;
+        U[(size_t)prev * 3 * n + 2 * n + i] = 0.f;
+        P[i] = 0.f; P[n + i] = 0.f;
+        Q[i] = 0.f; Q[n + i] = 0.f; Q[2 * n + i] = 0.f;
+    }
+}
+
+// ---- the fused primal-dual iteration --------------------------------------
+
+__global__ __launch_bounds__(TG_THREADS) void tgv_iter_kernel(TgvPtrs a) {
+    constexpr int HX = TG_TX + 2, HY = TG_TY + 2;   // ubar region: x0-1..x0+64, y0-1..y0+8
+    constexpr int DX = TG_TX + 1, DY = TG_TY + 1;   // dual region: x0-1..x0+63, y0-1..y0+7
+    __shared__ float sUb[3][HY][HX];
+    __shared__ float sP[2][DY][DX];
+    __shared__ float sQ[3][DY][DX];
+
+    if (a.ctrl->done) return;
+    const int W = a.W, H = a.H;
+    const size_t n = (size_t)W * H;
+    const int x0 = blockIdx.x * TG_TX, y0 = blockIdx.y * TG_TY;
+    const int tid = threadIdx.x;
+
+    // phase 1: ubar / u1bar at clamped coordinates (clamping reproduces the
+    // reference's min(x+1, W-1) neighbours, TGV2_kernels.cu:352-353).
+    for (int i = tid; i < HX * HY; i += TG_THREADS) {
+        const int hy = i / HX, hx = i - (i / HX) * HX;
+        const int gx = min(max(x0 - 1 + hx, 0), W - 1);
+        const int gy = min(max(y0 - 1 + hy, 0), H - 1);
+        const size_t g = (size_t)gy * W + gx;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) sUb[f][hy][hx] = 2.f * a.ucur[f * n + g] - a.uprev[f * n + g];
+    }
+    __syncthreads();
+
+    // phase 2: new p, q on the tile plus left/top halo.
+    const float as1 = a.alpha1 * a.sigma, as0 = a.alpha0 * a.sigma;
+    for (int i = tid; i < DX * DY; i += TG_THREADS) {
+        const int dy = i / DX, dx = i - (i / DX) * DX;
+        const int vx = x0 - 1 + dx, vy = y0 - 1 + dy;
+        const int gx = min(max(vx, 0), W - 1), gy = min(max(vy, 0), H - 1);
+        const size_t g = (size_t)gy * W + gx;
+        const int hx = dx, hy = dy;  // ubar index of this position (+1 -> right / down neighbour)
+        const float ub = sUb[0][hy][hx], u1xb = sUb[1][hy][hx], u1yb = sUb[2][hy][hx];
+        // TGV2_updateP_tensor_weighed
+        const float t11 = a.T[g], t12 = a.T[n + g], t22 = a.T[2 * n + g];
+        const float gxv = sUb[0][hy][hx + 1] - ub - u1xb;
+        const float gyv = sUb[0][hy + 1][hx] - ub - u1yb;
+        float px = a.pcur[g] + as1 * (t11 * gxv + t12 * gyv);
+        float py = a.pcur[n + g] + as1 * (t12 * gxv + t22 * gyv);
+        const float dp = fmaxf(1.f, sqrtf(px * px + py * py));
+        px = px / dp;
+        py = py / dp;
+        // TGV2_updateQ (q_w == q_z)
+        const float dx_u1x = sUb[1][hy][hx + 1] - u1xb;
+        const float dy_u1x = sUb[1][hy + 1][hx] - u1xb;
+        const float dx_u1y = sUb[2][hy][hx + 1] - u1yb;
+        const float dy_u1y = sUb[2][hy + 1][hx] - u1yb;
+        float qx = a.qcur[g] + as0 * dx_u1x;
+        float qy = a.qcur[n + g] + as0 * dy_u1y;
+        float qz = a.qcur[2 * n + g] + as0 * (dy_u1x + dx_u1y) / 2.0f;
+        const float dq = fmaxf(1.f, sqrtf(qx * qx + qy * qy + qz * qz + qz * qz));
+        qx = qx / dq;
+        qy = qy / dq;
+        qz = qz / dq;
+        sP[0][dy][dx] = px; sP[1][dy][dx] = py;
+        sQ[0][dy][dx] = qx; sQ[1][dy][dx] = qy; sQ[2][dy][dx] = qz;
+        if (dx > 0 && dy > 0 && vx < W && vy < H) {
+            a.pnext[g] = px; a.pnext[n + g] = py;
+            a.qnext[g] = qx; a.qnext[n + g] = qy; a.qnext[2 * n + g] = qz;
+        }
+    }
+    __syncthreads();
+
+    // phase 3: r, prodsum and the primal step for the tile pixels.
+    const float sl = a.sigma * a.lambda;
+    for (int i = tid; i < TG_TX * TG_TY; i += TG_THREADS) {
+        const int ty = i / TG_TX, tx = i - (i / TG_TX) * TG_TX;
+        const int x = x0 + tx, y = y0 + ty;
+        if (x >= W || y >= H) continue;
+        const size_t g = (size_t)y * W + x;
+        const float u = a.ucur[g], u1x = a.ucur[n + g], u1y = a.ucur[2 * n + g];
+        const float du = u - a.u0[g];
+        float ps = 0.f;
+        for (int j = 0; j < a.N; ++j) {
+            const size_t gj = (size_t)j * n + g;
+            const float iu = a.Iu[gj];
+            float rr = a.r[gj] + sl * (a.It[gj] + du * iu);
+            rr = rr / fmaxf(1.f, fabsf(rr));
+            a.r[gj] = rr;
+            ps += rr * iu;
+        }
+        const int cx = tx + 1, cy = ty + 1;              // dual index of this pixel
+        const int lx = x > 0 ? cx - 1 : cx;              // max(x-1, 0)
+        const int ty_ = y > 0 ? cy - 1 : cy;             // max(y-1, 0)
+        const float t11 = a.T[g], t12 = a.T[n + g], t22 = a.T[2 * n + g];
+        const float c_px = sP[0][cy][cx], c_py = sP[1][cy][cx];
+        const float xp_px = sP[0][cy][lx], xp_py = sP[1][cy][lx];
+        const float yp_px = sP[0][ty_][cx], yp_py = sP[1][ty_][cx];
+        const float un = u - a.tau * (-a.alpha1 * (t11 * (c_px - xp_px) + t12 * (c_py - xp_py) +
+                                                   t12 * (c_px - yp_px) + t22 * (c_py - yp_py)) +
+                                      a.lambda * ps);
+        const float u1xn = u1x - a.tau * (-a.alpha1 * (t11 * c_px + t12 * c_py) -
+                                          a.alpha0 * (sQ[0][cy][cx] - sQ[0][cy][lx] + sQ[2][cy][cx] - sQ[2][ty_][cx]));
+        const float u1yn = u1y - a.tau * (-a.alpha1 * (t12 * c_px + t22 * c_py) -
+                                          a.alpha0 * (sQ[2][cy][cx] - sQ[2][cy][lx] + sQ[1][cy][cx] - sQ[1][ty_][cx]));
+        a.unext[g] = un;
+        a.unext[n + g] = u1xn;
+        a.unext[2 * n + g] = u1yn;
+    }
+}
+
+// ---- FP64 energy -------------------------------------------------------------
+
+__device__ inline double block_reduce_sum(double v) {
+    __shared__ double wsum[TG_THREADS / 64];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wsum[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < TG_THREADS / 64; ++k) t += wsum[k];
+    return t;
+}
+
+// Per-block partial sums of alpha1 |T(grad u - u1)| + alpha0 |eps(u1)|_F +
+// lambda sum_j |It_j + (u - u0) Iu_j| with the iteration's clamped forward
+// differences.
+__global__ __launch_bounds__(TG_THREADS) void tgv_energy_kernel(const float* __restrict__ U, const float* __restrict__ T,
+                                                             const float* __restrict__ u0, const float* __restrict__ It,
+                                                             const float* __restrict__ Iu, int W, int H, int N,
+                                                             double lambda, double alpha0, double alpha1,
+                                                             double* __restrict__ partials) {
+    const size_t n = (size_t)W * H;
+    double e = 0.0;
+    for (size_t i = (size_t)blockIdx.x * TG_THREADS + threadIdx.x; i < n; i += (size_t)gridDim.x * TG_THREADS) {
+        const int y = (int)(i / W), x = (int)(i - (size_t)y * W);
+        const size_t ix = (size_t)y * W + min(x + 1, W - 1), iy = (size_t)min(y + 1, H - 1) * W + x;
+        const double u = U[i], ux = U[ix], uy = U[iy];
+        const double a1 = U[n + i], a1x = U[n + ix], a1y = U[n + iy];
+        const double b1 = U[2 * n + i], b1x = U[2 * n + ix], b1y = U[2 * n + iy];
+        const double gx = (ux - u) - a1, gy = (uy - u) - b1;
+        const double t11 = T[i], t12 = T[n + i], t22 = T[2 * n + i];
+        const double p = t11 * gx + t12 * gy, q = t12 * gx + t22 * gy;
+        const double ex = a1x - a1, ey = b1y - b1, ez = 0.5 * ((a1y - a1) + (b1x - b1));
+        double ed = 0.0;
+        const double du = u - (double)u0[i];
+        for (int j = 0; j < N; ++j) ed += fabs((double)It[(size_t)j * n + i] + du * (double)Iu[(size_t)j * n + i]);
+        e += alpha1 * sqrt(p * p + q * q) + alpha0 * sqrt(ex * ex + ey * ey + 2.0 * ez * ez) + lambda * ed;
+    }
+    const double t = block_reduce_sum(e);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// Sum of the partials in a fixed order, then the stop test.
+__global__ __launch_bounds__(TG_THREADS) void tgv_energy_finalize_kernel(const double* __restrict__ partials, int np,
+                                                                      TgvCtrl* ctrl, int k, double tol, double e_star) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < np; i += TG_THREADS) v += partials[i];
+    const double E = block_reduce_sum(v);
+    if (threadIdx.x == 0) {
+        ctrl->energy = E;
+        if (!ctrl->done && e_star > 0.0 && fabs(E - e_star) <= tol * e_star) {
+            ctrl->done = 1;
+            ctrl->iters_at_done = k;
+            ctrl->energy_at_done = E;
+        }
+    }
+}
+
+constexpr int TG_ENERGY_BLOCKS = 1024;
+
+static inline TgvPtrs tgv_ptrs(psw_tgv* t, const psw_tgv_params* prm) {
+    const TgvLayout& L = t->L;
+    float* B = t->base;
+    const size_t n = L.n;
+    const int cur = t->k % 3, prev = (t->k + 2) % 3, next = (t->k + 1) % 3;
+    const int pc = t->k % 2, pn = (t->k + 1) % 2;
+    TgvPtrs p;
+    p.ucur = B + L.U + (size_t)cur * 3 * n;
+    p.uprev = B + L.U + (size_t)prev * 3 * n;
+    p.unext = B + L.U + (size_t)next * 3 * n;
+    p.pcur = B + L.P + (size_t)pc * 2 * n;
+    p.pnext = B + L.P + (size_t)pn * 2 * n;
+    p.qcur = B + L.Q + (size_t)pc * 3 * n;
+    p.qnext = B + L.Q + (size_t)pn * 3 * n;
+    p.r = B + L.r;
+    p.It = B + L.It;
+    p.Iu = B + L.Iu;
+    p.T = B + L.T;
+    p.u0 = B + L.u0;
+    p.ctrl = t->ctrl;
+    p.W = t->W; p.H = t->H; p.N = t->N;
+    p.lambda = (float)prm->lambda; p.alpha0 = (float)prm->alpha0; p.alpha1 = (float)prm->alpha1;
+    p.tau = (float)prm->tau; p.sigma = (float)prm->sigma;
+    return p;
+}
+
+static int tgv_launch_iters(psw_tgv* t, int niters, const psw_tgv_params* prm, hipStream_t st) {
+    const dim3 grid((unsigned)((t->W + TG_TX - 1) / TG_TX), (unsigned)((t->H + TG_TY - 1) / TG_TY));
+    for (int i = 0; i < niters; ++i) {
+        tgv_iter_kernel<<<grid, TG_THREADS, 0, st>>>(tgv_ptrs(t, prm));
+        int s = launch_status("tgv_iter_kernel");
+        if (s != PSW_OK) return s;
+        t->k++;
+    }
+    return PSW_OK;
+}
+
+static int tgv_launch_energy(psw_tgv* t, const psw_tgv_params* prm, double tol, double e_star, hipStream_t st) {
+    const TgvLayout& L = t->L;
+    const float* U = t->base + L.U + (size_t)(t->k % 3) * 3 * L.n;
+    int blocks = (int)((L.n + TG_THREADS - 1) / TG_THREADS);
+    if (blocks > t->npartials) blocks = t->npartials;
+    tgv_energy_kernel<<<blocks, TG_THREADS, 0, st>>>(U, t->base + L.T, t->base + L.u0, t->base + L.It, t->base + L.Iu,
+                                                     t->W, t->H, t->N, prm->lambda, prm->alpha0, prm->alpha1,
+                                                     t->partials);
+    int s = launch_status("tgv_energy_kernel");
+    if (s != PSW_OK) return s;
+    tgv_energy_finalize_kernel<<<1, TG_THREADS, 0, st>>>(t->partials, blocks, t->ctrl, t->k, tol, e_star);
+    return launch_status("tgv_energy_finalize_kernel");
+}
+
+__global__ void tgv_ctrl_reset_kernel(TgvCtrl* c) {
+    c->done = 0;
+    c->iters_at_done = 0;
+    c->energy = 0.0;
+    c->energy_at_done = 0.0;
+}
+
+static int tgv_reset_ctrl(psw_tgv* t, hipStream_t st) {
+    tgv_ctrl_reset_kernel<<<1, 1, 0, st>>>(t->ctrl);
+    return launch_status("tgv_ctrl_reset_kernel");
+}
+
+}  // namespace psw
+
+using namespace psw;
+
+extern "C" {
+
+size_t psw_tgv_state_bytes(int width, int height, int nsrc) {
+    TgvLayout L;
+    L.init(width, height, nsrc);
+    return L.total * sizeof(float) + TG_ENERGY_BLOCKS * sizeof(double) + sizeof(TgvCtrl);
+}
+
+int psw_tgv_create(psw_tgv** out, int width, int height, int nsrc) {
+    PSW_CHECK_ARG(out, "out is NULL");
+    *out = nullptr;
+    PSW_CHECK_ARG(width >= 2 && height >= 2, "image must be at least 2 x 2");
+    PSW_CHECK_ARG(nsrc >= 1 && nsrc <= PSW_MAX_SOURCES, "nsrc must be 1..PSW_MAX_SOURCES");
+    psw_tgv* t = new psw_tgv();
+    t->W = width; t->H = height; t->N = nsrc;
+    t->L.init(width, height, nsrc);
+    t->npartials = TG_ENERGY_BLOCKS;
+    t->k = 0;
+    t->have_images = false;
+    hipError_t e = hipMalloc((void**)&t->base, t->L.total * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&t->partials, TG_ENERGY_BLOCKS * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&t->ctrl, sizeof(TgvCtrl));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&t->hctrl, sizeof(TgvCtrl), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMemset(t->base, 0, t->L.total * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(t->ctrl, 0, sizeof(TgvCtrl));
+    if (e != hipSuccess) {
+        int s = hip_status(e, "psw_tgv_create", PSW_ERR_ALLOC);
+        psw_tgv_destroy(t);
+        return s;
+    }
+    *out = t;
+    return PSW_OK;
+}
+
+int psw_tgv_destroy(psw_tgv* t) {
+    if (!t) return PSW_OK;
+    if (t->base) (void)hipFree(t->base);
+    if (t->partials) (void)hipFree(t->partials);
+    if (t->ctrl) (void)hipFree(t->ctrl);
+    if (t->hctrl) (void)hipHostFree(t->hctrl);
+    delete t;
+    return PSW_OK;
+}
+
+int psw_tgv_set_images(psw_tgv* t, const float* d_ref, const float* d_src, double beta, double gamma,
+                       psw_stream_t s) {
+    PSW_CHECK_ARG(t && d_ref && d_src, "NULL pointer");
+    const TgvLayout& L = t->L;
+    tgv_scale_kernel<<<512, 256, 0, S(s)>>>(t->base + L.ref, d_ref, L.n);
+    int st = launch_status("tgv_scale_kernel");
+    if (st != PSW_OK) return st;
+    tgv_scale_kernel<<<2048, 256, 0, S(s)>>>(t->base + L.src, d_src, L.n * t->N);
+    st = launch_status("tgv_scale_kernel");
+    if (st != PSW_OK) return st;
+    const dim3 blk(64, 4), grd((unsigned)((t->W + 63) / 64), (unsigned)((t->H + 3) / 4));
+    tgv_tensor_kernel<<<grd, blk, 0, S(s)>>>(t->base + L.T, t->base + L.ref, t->W, t->H, (float)beta, (float)gamma);
+    st = launch_status("tgv_tensor_kernel");
+    if (st != PSW_OK) return st;
+    t->have_images = true;
+    return PSW_OK;
+}
+
+int psw_tgv_set_u(psw_tgv* t, const float* d_u, float value, psw_stream_t s) {
+    PSW_CHECK_ARG(t, "NULL solver");
+    float* u = t->base + t->L.U + (size_t)(t->k % 3) * 3 * t->L.n;
+    if (d_u) {
+        PSW_TRY_HIP(hipMemcpyAsync(u, d_u, t->L.n * sizeof(float), hipMemcpyDeviceToDevice, S(s)), PSW_ERR_COPY);
+        return PSW_OK;
+    }
+    return psw_fill_f32(u, value, t->L.n, s);
+}
+
+int psw_tgv_linearise(psw_tgv* t, const float* K, const float* Rrel, const float* trel, psw_stream_t s) {
+    PSW_CHECK_ARG(t && K && Rrel && trel, "NULL pointer");
+    PSW_CHECK_ARG(t->have_images, "psw_tgv_set_images must come first");
+    const TgvLayout& L = t->L;
+    const size_t n = L.n;
+    const int cur = t->k % 3, prev = (t->k + 2) % 3;
+    tgv_warp_reset_kernel<<<1024, 256, 0, S(s)>>>(t->base + L.U, t->base + L.P + (size_t)(t->k % 2) * 2 * n,
+                                                  t->base + L.Q + (size_t)(t->k % 2) * 3 * n, t->base + L.u0, n, cur,
+                                                  prev);
+    int st = launch_status("tgv_warp_reset_kernel");
+    if (st != PSW_OK) return st;
+    TgvPoses pz;
+    pz.K = m3_from(K);
+    // Matrix3D::inv (structs.h:584-593) in float, as PlaneSweep::setK stores it.
+    {
+        const float* m = K;
+        const float det = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+                          m[2] * (m[3] * m[7] - m[4] * m[6]);
+        const float r[9] = {m[4] * m[8] - m[5] * m[7], m[2] * m[7] - m[1] * m[8], m[1] * m[5] - m[2] * m[4],
+                            m[5] * m[6] - m[3] * m[8], m[0] * m[8] - m[2] * m[6], m[2] * m[3] - m[0] * m[5],
+                            m[3] * m[7] - m[4] * m[6], m[1] * m[6] - m[0] * m[7], m[0] * m[4] - m[1] * m[3]};
+        for (int j = 0; j < 9; ++j) pz.invK.m[j] = r[j] / det;
+    }
+    for (int j = 0; j < t->N; ++j) {
+        pz.Rrel[j] = m3_from(Rrel + 9 * j);
+        pz.trel[j] = v3_from(trel + 3 * j);
+        pz.RK[j] = mm3(pz.Rrel[j], pz.invK);
+    }
+    pz.fx = (float)(double)K[0];
+    pz.fy = (float)(double)K[4];
+    const dim3 blk(64, 4), grd((unsigned)((t->W + 63) / 64), (unsigned)((t->H + 3) / 4));
+    tgv_linearise_kernel<<<grd, blk, 0, S(s)>>>(pz, t->base + L.src, t->base + L.ref, t->base + L.u0, t->base + L.It,
+                                                t->base + L.Iu, t->base + L.r, t->W, t->H, t->N);
+    st = launch_status("tgv_linearise_kernel");
+    if (st != PSW_OK) return st;
+    return tgv_reset_ctrl(t, S(s));
+}
+
+int psw_tgv_iterate(psw_tgv* t, int niters, const psw_tgv_params* prm, psw_stream_t s) {
+    PSW_CHECK_ARG(t && prm, "NULL pointer");
+    PSW_CHECK_ARG(niters >= 0, "niters must be >= 0");
+    return tgv_launch_iters(t, niters, prm, S(s));
+}
+
+int psw_tgv_energy(psw_tgv* t, const psw_tgv_params* prm, double* energy, psw_stream_t s) {
+    PSW_CHECK_ARG(t && prm && energy, "NULL pointer");
+    int st = tgv_launch_energy(t, prm, 0.0, 0.0, S(s));
+    if (st != PSW_OK) return st;
+    PSW_TRY_HIP(hipMemcpyAsync(t->hctrl, t->ctrl, sizeof(TgvCtrl), hipMemcpyDeviceToHost, S(s)), PSW_ERR_COPY);
+    PSW_TRY_HIP(hipStreamSynchronize(S(s)), PSW_ERR_DEVICE);
+    *energy = t->hctrl->energy;
+    return PSW_OK;
+}
+
+int psw_tgv_solve(psw_tgv* t, int max_iters, int check_every, double tol, double e_star, const psw_tgv_params* prm,
+                  int* iters_done, double* energy_out, psw_stream_t s) {
+    PSW_CHECK_ARG(t && prm, "NULL pointer");
+    PSW_CHECK_ARG(max_iters >= 0 && check_every >= 1, "bad iteration counts");
+    hipStream_t st = S(s);
+    int rc = tgv_reset_ctrl(t, st);
+    if (rc != PSW_OK) return rc;
+    const int k_start = t->k;
+    // Keep at most two checks in flight: the host waits for the check two
+    // chunks back, so the device never idles and at most two chunks run as
+    // no-ops (they test ctrl->done and return) after the stop.
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    PSW_TRY_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming), PSW_ERR_DEVICE);
+    PSW_TRY_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming), PSW_ERR_DEVICE);
+    int done_iters = -1, chunk_idx = 0;
+    int launched = 0;
+    while (launched < max_iters) {
+        const int c = std::min(check_every, max_iters - launched);
+        rc = tgv_launch_iters(t, c, prm, st);
+        if (rc != PSW_OK) break;
+        launched += c;
+        rc = tgv_launch_energy(t, prm, tol, e_star, st);
+        if (rc != PSW_OK) break;
+        hipEvent_t e = ev[chunk_idx & 1];
+        if (chunk_idx >= 2) {
+            // the event slot holds the check two chunks back: wait and read it
+            if (hipEventSynchronize(e) != hipSuccess) { rc = PSW_ERR_DEVICE; break; }
+            if (t->hctrl->done) { done_iters = t->hctrl->iters_at_done; break; }
+        }
+        (void)hipMemcpyAsync(t->hctrl, t->ctrl, sizeof(TgvCtrl), hipMemcpyDeviceToHost, st);
+        (void)hipEventRecord(e, st);
+        chunk_idx++;
+    }
+    (void)hipStreamSynchronize(st);
+    (void)hipMemcpy(t->hctrl, t->ctrl, sizeof(TgvCtrl), hipMemcpyDeviceToHost);
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (rc != PSW_OK) return rc;
+    if (t->hctrl->done) done_iters = t->hctrl->iters_at_done;
+    if (done_iters >= 0) t->k = done_iters;  // later kernels were no-ops
+    if (iters_done) *iters_done = t->k - k_start;
+    if (energy_out) *energy_out = t->hctrl->done ? t->hctrl->energy_at_done : t->hctrl->energy;
+    // leave ctrl cleared so that plain psw_tgv_iterate calls run
+    return tgv_reset_ctrl(t, st) == PSW_OK && hipStreamSynchronize(st) == hipSuccess ? PSW_OK : PSW_ERR_DEVICE;
+}
+
+int psw_tgv_get_u(psw_tgv* t, float* d_out, psw_stream_t s) {
+    PSW_CHECK_ARG(t && d_out, "NULL pointer");
+    const float* u = t->base + t->L.U + (size_t)(t->k % 3) * 3 * t->L.n;
+    PSW_TRY_HIP(hipMemcpyAsync(d_out, u, t->L.n * sizeof(float), hipMemcpyDeviceToDevice, S(s)), PSW_ERR_COPY);
+    return PSW_OK;
+}
+
+int psw_tgv_get_state(psw_tgv* t, float* d_out, psw_stream_t s) {
+    PSW_CHECK_ARG(t && d_out, "NULL pointer");
+    const TgvLayout& L = t->L;
+    const size_t n = L.n, N = (size_t)t->N;
+    const float* B = t->base;
+    auto cp = [&](float* dst, const float* src, size_t cnt) {
+        return hipMemcpyAsync(dst, src, cnt * sizeof(float), hipMemcpyDeviceToDevice, S(s));
+    };
+    const float* U = B + L.U + (size_t)(t->k % 3) * 3 * n;
+    const float* P = B + L.P + (size_t)(t->k % 2) * 2 * n;
+    const float* Q = B + L.Q + (size_t)(t->k % 2) * 3 * n;
+    float* o = d_out;
+    PSW_TRY_HIP(cp(o, U, 3 * n), PSW_ERR_COPY); o += 3 * n;
+    PSW_TRY_HIP(cp(o, P, 2 * n), PSW_ERR_COPY); o += 2 * n;
+    PSW_TRY_HIP(cp(o, Q, 3 * n), PSW_ERR_COPY); o += 3 * n;
+    PSW_TRY_HIP(cp(o, B + L.It, N * n), PSW_ERR_COPY); o += N * n;
+    PSW_TRY_HIP(cp(o, B + L.Iu, N * n), PSW_ERR_COPY); o += N * n;
+    PSW_TRY_HIP(cp(o, B + L.r, N * n), PSW_ERR_COPY); o += N * n;
+    PSW_TRY_HIP(cp(o, B + L.T, 3 * n), PSW_ERR_COPY); o += 3 * n;
+    PSW_TRY_HIP(cp(o, B + L.u0, n), PSW_ERR_COPY);
+    return PSW_OK;
+}
+
+int psw_tgv_run(psw_tgv* t, const float* K, const float* Rrel, const float* trel, int niters, int warps,
+                const psw_tgv_params* prm, const float* d_u_init, float* d_u_out, psw_stream_t s) {
+    PSW_CHECK_ARG(t && K && Rrel && trel && prm && d_u_out, "NULL pointer");
+    PSW_CHECK_ARG(niters >= 0 && warps >= 0, "negative iteration count");
+    int st = psw_tgv_set_u(t, d_u_init, 1.f, s);   // cold start u = 1 (planesweep.cpp:414)
+    if (st != PSW_OK) return st;
+    for (int l = 0; l < warps; ++l) {
+        st = psw_tgv_linearise(t, K, Rrel, trel, s);
+        if (st != PSW_OK) return st;
+        st = tgv_launch_iters(t, niters, prm, S(s));
+        if (st != PSW_OK) return st;
+    }
+    return psw_tgv_get_u(t, d_u_out, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,350 @@
+"""GPU parity: the HIP kernels (called through the C ABI) against the CPU oracle.
+
+Bars (stated per test):
+  - reference-order kernels: bit-identical to the float oracle;
+  - argmin plane: >= 99.9 % agreement with the FP64 oracle, disagreements
+    only where the FP64 costs of the two planes are within a near-tie margin;
+  - histogram counts: bit-identical to the float oracle;
+  - refined depth / fused u: within 1e-4 relative of the FP64 oracle
+    (relative to max(|ref|, 1) for u, whose range is about [-1, 1]).
+Sizes are small so that the oracle finishes in seconds.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import one_based, rel_err
+
+pytestmark = pytest.mark.gpu
+
+STD_T = 1e-4      # DEFAULT_STD_THRESHOLD (defines.h:29)
+NCC_T = 0.5       # DEFAULT_NCC_THRESHOLD (defines.h:30)
+
+
+def _print_case(name, **kw):
+    print(f"[case] {name} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+# ---- stage 1: sweep ------------------------------------------------------------------
+
+@pytest.mark.parametrize("winsize", [5, 9])
+def test_sweep_reference_launchers_bit_exact(gpu, views_tiny, winsize):
+    """RunAlgorithm over the per-kernel launchers == float oracle, bit for bit."""
+    import refdriver
+    v = views_tiny
+    K1 = one_based(v["K"])
+    planes = oracle.reference_planes(0.5, 6.0, 12)
+    _print_case("sweep_reference_launchers", win=winsize, P=planes.size)
+    got = refdriver.run_algorithm(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], planes, winsize,
+                                  STD_T, NCC_T, 6.0)
+    exp = oracle.sweep_ref(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], planes, winsize,
+                           STD_T, NCC_T, 6.0)
+    np.testing.assert_array_equal(got, exp)
+
+
+@pytest.mark.parametrize("winsize,nplanes", [(5, 24), (9, 16), (3, 8)])
+def test_sweep_fused_reference_bit_exact(gpu, views_small, winsize, nplanes):
+    from psw_amd import stages
+    v = views_small
+    K1 = one_based(v["K"])
+    planes = stages.reference_planes(0.5, 6.0, nplanes)
+    np.testing.assert_array_equal(planes, oracle.reference_planes(0.5, 6.0, nplanes))
+    _print_case("sweep_fused_reference", win=winsize, P=planes.size)
+    got = stages.sweep_reference(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], planes, winsize,
+                                 STD_T, NCC_T, 6.0)
+    exp = oracle.sweep_ref(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], planes, winsize,
+                           STD_T, NCC_T, 6.0)
+    np.testing.assert_array_equal(got, exp)
+
+
+def test_sweep_homographies_fp64_match_oracle(gpu, views_small):
+    """GPU FP64 homography construction == oracle double arithmetic rounded to float."""
+    from psw_amd import stages
+    v = views_small
+    planes = stages.inverse_depth_planes(0.3, 8.0, 32)
+    got = stages.homographies_device(planes, v["K"], v["Rref"], v["tref"], v["Rsrc"], v["tsrc"])
+    for i in range(v["Rsrc"].shape[0]):
+        for p in range(planes.size):
+            exp = oracle.homography_f64(v["K"], v["Rref"], v["tref"], v["Rsrc"][i], v["tsrc"][i], planes[p])
+            np.testing.assert_array_equal(got[i, p].reshape(3, 3), exp.astype(np.float32))
+
+
+@pytest.mark.parametrize("winsize,offset", [(7, 0), (9, 0), (5, 1)])
+def test_sweep_fused_aggregate_bit_exact(gpu, views_small, winsize, offset):
+    from psw_amd import stages
+    v = views_small
+    K = v["K"] if offset == 0 else one_based(v["K"])
+    planes = stages.inverse_depth_planes(0.5, 6.0, 24)
+    _print_case("sweep_fused_aggregate", win=winsize, off=offset)
+    arg, cmin, csec = stages.sweep_aggregate(v["ref"], v["src"], K, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], planes,
+                                             winsize, STD_T, coord_offset=offset, fast=False)
+    ea, ec, es = oracle.sweep_agg(v["ref"], v["src"], K, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], planes, winsize,
+                                  STD_T, coord_offset=offset)
+    np.testing.assert_array_equal(arg, ea)
+    np.testing.assert_array_equal(cmin, ec)
+    np.testing.assert_array_equal(csec, es)
+
+
+def _near_tie_check(arg, a64, c64_planes_fn, margin):
+    dis = np.argwhere(arg != a64)
+    for (y, x) in dis:
+        c = c64_planes_fn(y, x)
+        assert abs(c[arg[y, x]] - c[a64[y, x]]) <= margin, (y, x, c[arg[y, x]], c[a64[y, x]])
+
+
+@pytest.mark.parametrize("fast", [False, True])
+def test_sweep_aggregate_vs_fp64(gpu, views_small, fast):
+    """Argmin agreement >= 99.9 % with the FP64 oracle; disagreements are near-ties."""
+    from psw_amd import stages
+    v = views_small
+    planes = stages.inverse_depth_planes(0.5, 6.0, 32)
+    _print_case("sweep_aggregate_vs_fp64", fast=fast)
+    arg, cmin, _ = stages.sweep_aggregate(v["ref"], v["src"], v["K"], v["Rref"], v["tref"], v["Rsrc"], v["tsrc"],
+                                          planes, 7, STD_T, fast=fast)
+    a64, c64, _ = oracle.sweep_agg(v["ref"], v["src"], v["K"], v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], planes, 7,
+                                   STD_T, prec="f64")
+    agree = float(np.mean(arg == a64))
+    print(f"argmin agreement vs FP64: {agree:.6f} ({int(np.sum(arg != a64))} of {arg.size})")
+    assert agree >= 0.999
+    # disagreeing pixels: the GPU's minimum cost equals the FP64 minimum to a near-tie margin
+    mask = arg != a64
+    if mask.any():
+        assert np.max(np.abs(cmin[mask] - c64[mask])) <= (1e-3 if fast else 1e-5)
+
+
+# ---- stage 2: TGV2 ----------------------------------------------------------------------
+
+TGV_KW = dict(lambda_=1.0, alpha0=2.0, alpha1=1.5, tau=0.02, sigma=1.0, beta=0.0, gamma=1.0)
+
+
+def test_tgv_reference_launchers_vs_oracle(gpu, views_tiny):
+    """PlaneSweep::TGV over the per-kernel launchers vs the float oracle."""
+    import refdriver
+    v = views_tiny
+    K1 = one_based(v["K"])
+    got = refdriver.tgv(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], 20, 2, **TGV_KW)
+    exp = oracle.tgv(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], 20, 2, **TGV_KW)
+    err = rel_err(got, exp)
+    print(f"launchers vs f32 oracle: max rel {err.max():.3e}, identical {np.mean(got == exp):.4f}")
+    assert err.max() <= 1e-6
+
+
+@pytest.mark.parametrize("niters,warps", [(30, 3), (200, 1)])
+def test_tgv_fused_vs_fp64_oracle(gpu, views_small, niters, warps):
+    """Fused TGV2 (FP32) within 1e-4 relative of the FP64 oracle."""
+    from psw_amd import stages
+    v = views_small
+    K1 = one_based(v["K"])
+    prm = stages.tgv_params(**TGV_KW)
+    _print_case("tgv_fused", niters=niters, warps=warps)
+    u = stages.tgv_run(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], niters, warps, prm)
+    u64 = oracle.tgv(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], niters, warps, prec="f64",
+                     **TGV_KW)
+    u32 = oracle.tgv(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], niters, warps, **TGV_KW)
+    e64 = np.abs(u - u64) / np.abs(u64)
+    e32 = np.abs(u - u32) / np.abs(u32)
+    print(f"vs FP64 oracle: max rel {e64.max():.3e} mean {e64.mean():.3e}; vs FP32 oracle: max rel {e32.max():.3e}")
+    assert e64.max() <= 1e-4
+
+
+def test_tgv_warm_start_and_state(gpu, views_small):
+    """u_init replaces the cold start; state invariants |p|,|q|,|r| <= 1."""
+    from psw_amd import stages
+    v = views_small
+    K1 = one_based(v["K"])
+    prm = stages.tgv_params(**TGV_KW)
+    u_init = np.clip(v["depth"], 0.5, 6.0).astype(np.float32)
+    u, st = stages.tgv_run(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], 40, 2, prm,
+                           u_init=u_init, state=True)
+    u64 = oracle.tgv(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], 40, 2, prec="f64",
+                     u_init=u_init, **TGV_KW)
+    assert (np.abs(u - u64) / np.abs(u64)).max() <= 1e-4
+    eps = 1e-6
+    assert np.max(np.hypot(st["px"], st["py"])) <= 1 + eps
+    assert np.max(np.sqrt(st["qx"] ** 2 + st["qy"] ** 2 + 2 * st["qz"] ** 2)) <= 1 + eps
+    assert np.max(np.abs(st["r"])) <= 1 + eps
+
+
+def test_tgv_energy_matches_oracle(gpu, views_small):
+    from psw_amd import stages
+    from psw_amd.device import DeviceArray
+    v = views_small
+    K1 = one_based(v["K"])
+    prm = stages.tgv_params(**TGV_KW)
+    H, W = v["ref"].shape
+    N = v["src"].shape[0]
+    Rrel, trel = stages.relative_all(v["Rref"], v["tref"], v["Rsrc"], v["tsrc"])
+    t = stages.TgvSolver(W, H, N)
+    d_ref, d_src = DeviceArray.from_host(v["ref"]), DeviceArray.from_host(v["src"])
+    t.set_images(d_ref, d_src)
+    t.set_u(value=1.0)
+    t.linearise(K1, Rrel, trel)
+    t.iterate(25, prm)
+    e_gpu = t.energy(prm)
+    s = t.get_state()
+    T4 = np.stack([s["T11"], s["T12"], s["T12"], s["T22"]])
+    e_orc = oracle.tgv_energy(s["u"], s["u1x"], s["u1y"], s["u0"], T4, s["It"], s["Iu"], 1.0, 2.0, 1.5)
+    print(f"energy gpu {e_gpu:.12e} oracle {e_orc:.12e}")
+    assert abs(e_gpu - e_orc) <= 1e-12 * abs(e_orc)
+
+
+def test_tgv_linearisation_vs_oracle(gpu, views_small):
+    from psw_amd import stages
+    from psw_amd.device import DeviceArray
+    v = views_small
+    K1 = one_based(v["K"])
+    H, W = v["ref"].shape
+    N = v["src"].shape[0]
+    u0 = np.clip(v["depth"] * 1.03, 0.5, 6.0).astype(np.float32)
+    Rrel, trel = stages.relative_all(v["Rref"], v["tref"], v["Rsrc"], v["tsrc"])
+    t = stages.TgvSolver(W, H, N)
+    t.set_images(DeviceArray.from_host(v["ref"]), DeviceArray.from_host(v["src"]))
+    t.set_u(DeviceArray.from_host(u0))
+    t.linearise(K1, Rrel, trel)
+    s = t.get_state()
+    It, Iu, T4 = oracle.tgv_linearise(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], u0)
+    np.testing.assert_array_equal(s["T11"], T4[0])
+    np.testing.assert_array_equal(s["T12"], T4[1])
+    np.testing.assert_array_equal(s["T22"], T4[3])
+    np.testing.assert_array_equal(s["It"], It)
+    # Iu: the oracle keeps the reference's double products, the kernel is FP32
+    assert np.max(np.abs(s["Iu"] - Iu)) <= 1e-5 * max(1.0, np.abs(Iu).max())
+
+
+def test_tgv_solve_stops_at_gap(gpu, views_small):
+    """psw_tgv_solve stops once |E - E*| <= tol E* (E* from a long run)."""
+    from psw_amd import stages
+    from psw_amd.device import DeviceArray
+    v = views_small
+    K1 = one_based(v["K"])
+    prm = stages.tgv_params(**TGV_KW)
+    H, W = v["ref"].shape
+    N = v["src"].shape[0]
+    Rrel, trel = stages.relative_all(v["Rref"], v["tref"], v["Rsrc"], v["tsrc"])
+    d_ref, d_src = DeviceArray.from_host(v["ref"]), DeviceArray.from_host(v["src"])
+
+    def fresh():
+        t = stages.TgvSolver(W, H, N)
+        t.set_images(d_ref, d_src)
+        t.set_u(value=1.0)
+        t.linearise(K1, Rrel, trel)
+        return t
+
+    long = fresh()
+    long.iterate(3000, prm)
+    e_star = long.energy(prm)
+    t = fresh()
+    iters, e = t.solve(3000, 10, 1e-3, e_star, prm)
+    print(f"E* {e_star:.6e}; stopped after {iters} iterations at E {e:.6e}")
+    assert iters < 3000 and iters % 10 == 0
+    assert abs(e - e_star) <= 1e-3 * e_star
+    assert abs(t.energy(prm) - e) <= 1e-12 * e   # the state is the one at the stop
+
+
+# ---- stage 3: fusion ---------------------------------------------------------------------
+
+def _fusion_inputs(n=3, W=96, H=72, grid=(24, 20, 28)):
+    from psw_amd import scene
+    o = scene.orbit_depthmaps(seed=11, n=n, width=W, height=H)
+    vol = np.array([-3.0, -1.6, -4.0, 3.0, 1.6, 6.0], np.float32)
+    return o, vol, grid
+
+
+def test_fusion_histogram_bit_exact(gpu):
+    from psw_amd import stages
+    from psw_amd.device import DeviceArray
+    o, vol, (gw, gh, gd) = _fusion_inputs()
+    thr = 0.25
+    f = stages.Fusion(gw, gh, gd, vol)
+    hist = np.zeros((gd, gh, gw, 8), np.uint16)
+    H, W = o["depths"].shape[1:]
+    for k in range(o["depths"].shape[0]):
+        d = DeviceArray.from_host(o["depths"][k])
+        f.update_histogram(d, W, H, o["K"], o["R"][k], o["t"][k], thr)
+        oracle.fusion_hist(hist, vol, o["depths"][k], o["K"], o["R"][k], o["t"][k], thr)
+    got = f.download()["hist"]
+    print(f"votes: {int(hist.sum())}")
+    assert hist.sum() > 0
+    np.testing.assert_array_equal(got, hist)
+    # batched launch == sequential updates
+    g2 = stages.Fusion(gw, gh, gd, vol)
+    dall = DeviceArray.from_host(o["depths"])
+    g2.update_histogram_batch(dall, o["depths"].shape[0], W, H, np.repeat(o["K"][None], o["R"].shape[0], 0), o["R"],
+                              o["t"], thr)
+    np.testing.assert_array_equal(g2.download()["hist"], hist)
+
+
+def test_fusion_pd_vs_oracle(gpu):
+    """U/P steps, interleaved with histograms (FusionUpdateIteration) and the
+    fused solve, against the FP64 oracle fed the same histograms."""
+    from psw_amd import stages
+    from psw_amd.device import DeviceArray
+    o, vol, (gw, gh, gd) = _fusion_inputs(n=4)
+    thr, tau, lam, sig = 0.25, 0.1, 0.5, 1.0
+    H, W = o["depths"].shape[1:]
+    f = stages.Fusion(gw, gh, gd, vol)
+    hist = np.zeros((gd, gh, gw, 8), np.uint16)
+    u64, v64, p64 = np.zeros((gd, gh, gw)), np.zeros((gd, gh, gw)), np.zeros((3, gd, gh, gw))
+    u32, v32, p32 = [np.zeros_like(a, np.float32) for a in (u64, v64, p64)]
+    for k in range(o["depths"].shape[0]):
+        d = DeviceArray.from_host(o["depths"][k])
+        f.update_iteration(d, W, H, o["K"], o["R"][k], o["t"][k], thr, tau, lam, sig)
+        oracle.fusion_hist(hist, vol, o["depths"][k], o["K"], o["R"][k], o["t"][k], thr)
+        for (u, v, p, prec) in ((u64, v64, p64, "f64"), (u32, v32, p32, "f32")):
+            oracle.fusion_u(u, v, p, hist, tau, lam, prec=prec)
+            oracle.fusion_p(p, v, sig, prec=prec)
+    g = f.download()
+    np.testing.assert_array_equal(g["hist"], hist)
+    e = rel_err(g["u"], u64)
+    print(f"interleaved: u vs FP64 max {e.max():.3e}; vs FP32 {rel_err(g['u'], u32).max():.3e}")
+    assert e.max() <= 1e-4
+    # then the fused solve
+    n = 60
+    f.solve(n, tau, lam, sig)
+    for _ in range(n):
+        oracle.fusion_u(u64, v64, p64, hist, tau, lam, prec="f64")
+        oracle.fusion_p(p64, v64, sig, prec="f64")
+    g = f.download()
+    e = rel_err(g["u"], u64)
+    ep = np.abs(g["p"] - p64).max()
+    print(f"solve {n}: u vs FP64 max {e.max():.3e} mean {e.mean():.3e}; p max abs {ep:.3e}")
+    assert e.max() <= 1e-4
+    assert ep <= 1e-4
+    assert f.surface_count() == oracle.surface_count(g["u"])
+
+
+# ---- TVL1 ---------------------------------------------------------------------------------
+
+def test_tvl1_vs_oracle(gpu, views_small):
+    from psw_amd import stages
+    v = views_small
+    depth = np.clip(v["depth"] + 0.05 * np.sin(np.arange(v["depth"].size)).reshape(v["depth"].shape), 0.5, 6.0)
+    depth = depth.astype(np.float32)
+    got = stages.tvl1_denoise(depth, v["ref"], 0.5, 6.0, niters=50)
+    e32 = oracle.tvl1(depth, v["ref"], 0.5, 6.0, niters=50)
+    e64 = oracle.tvl1(depth, v["ref"], 0.5, 6.0, niters=50, prec="f64")
+    print(f"TVL1 vs f32 oracle max abs {np.abs(got - e32).max():.3e}; vs f64 max rel {rel_err(got, e64).max():.3e}")
+    assert rel_err(got, e64).max() <= 1e-4
+    assert np.abs(got - e32).max() <= 1e-5
+
+
+# ---- error paths (forced from the host) -----------------------------------------------------
+
+def test_error_paths(gpu, views_tiny):
+    from psw_amd import stages
+    from psw_amd._lib import PswError
+    v = views_tiny
+    K1 = one_based(v["K"])
+    planes = stages.reference_planes(0.5, 6.0, 4)
+    with pytest.raises(PswError) as e:   # even window (PlaneSweep::setWindowSize, planesweep.h:234)
+        stages.sweep_reference(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], planes, 4, STD_T,
+                               NCC_T, 6.0)
+    assert e.value.status == 1
+    with pytest.raises(PswError):        # too many sources
+        stages.TgvSolver(8, 8, 17)
+    from psw_amd._lib import Dim3, call
+    from psw_amd.device import DeviceArray
+    d = DeviceArray((4, 4))
+    with pytest.raises(PswError) as e:   # block above 1024 threads: launch error
+        call("psw_set_value", d.ptr, 1.0, 4, 4, Dim3(1, 1, 1), Dim3(64, 32, 1), None)
+    assert e.value.status == 2
