@@ -43,27 +43,106 @@ def log(*a):
 
 
 def cpu_baseline(v, planes, K0, K1, iters_per_map):
-    """FP32 oracle (the CPU restatement, OpenMP) on a bounded sample of the
-    same workload, scaled to one depthmap."""
+    """FP32 oracle (the CPU restatement, OpenMP) on one complete depthmap of
+    the same workload: the full aggregated sweep and as many TGV2 iterations
+    as the GPU needed to reach the gap."""
     import oracle
-    from psw_amd import scene  # noqa: F401
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    nsp = 2
     t0 = time.perf_counter()
-    oracle.sweep_agg(v["ref"], v["src"], K0, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], planes[:nsp], WIN, STD_T)
-    t_sweep = (time.perf_counter() - t0) / nsp * NPLANES
+    oracle.sweep_agg(v["ref"], v["src"], K0, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], planes, WIN, STD_T)
+    t_sweep = time.perf_counter() - t0
     t0 = time.perf_counter()
-    oracle.tgv(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], 0, 1, **TGV)
-    t_setup = time.perf_counter() - t0
-    nit = 10
-    t0 = time.perf_counter()
-    oracle.tgv(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], nit, 1, **TGV)
-    t_it = (time.perf_counter() - t0 - t_setup) / nit
-    t_map = t_sweep + t_setup + t_it * iters_per_map
+    oracle.tgv(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], iters_per_map, 1, **TGV)
+    t_tgv = time.perf_counter() - t0
+    t_map = t_sweep + t_tgv
     return {"value": 1.0 / t_map, "unit": "depthmaps/s", "cores": cores, "kind": "port",
-            "sample": f"FP32 oracle: aggregated sweep over {nsp} of {NPLANES} planes x {NSRC} sources and "
-                      f"{nit} TGV2 iterations at 640x480, scaled to {NPLANES} planes + {iters_per_map} iterations",
-            "sweep_s_per_map": t_sweep, "tgv_s_per_iter": t_it, "tgv_setup_s": t_setup}
+            "sample": f"one full depthmap on the FP32 oracle: aggregated sweep over all {NPLANES} planes x {NSRC} "
+                      f"sources and {iters_per_map} TGV2 iterations at 640x480 (single run)",
+            "sweep_s": t_sweep, "tgv_s": t_tgv}
+
+
+# ---- fusion workload: 256^3 grid, 50 depthmaps, 300 primal-dual iterations ----
+
+FUSION_GRID = 256
+FUSION_MAPS = 50
+FUSION_ITERS = 300
+FUSION_VOLUME = np.array([-3.0, -1.6, -4.0, 3.0, 1.6, 6.0], np.float32)
+FUSION_BYTES_PER_VOXEL_ITER = 56   # fused PD: read p 12, v 4, u 4, hist 16; write p 12, u 4, v 4
+
+
+def fusion_workload(seed):
+    from psw_amd import scene
+    o = scene.orbit_depthmaps(seed=seed, n=FUSION_MAPS, width=W, height=H, noise_sigma=0.01)
+    edge = float(np.mean((FUSION_VOLUME[3:] - FUSION_VOLUME[:3]) / FUSION_GRID))
+    return o, 5.0 * edge   # truncation band of +-5 voxel edges
+
+
+def run_fusion(o, thr, stream, ev):
+    from psw_amd import stages
+    from psw_amd.device import DeviceArray
+    g = FUSION_GRID
+    f = stages.Fusion(g, g, g, FUSION_VOLUME)
+    d_maps = DeviceArray.from_host(o["depths"])
+    Ks = np.repeat(o["K"][None], FUSION_MAPS, 0)
+    res = {}
+    for rep in range(2):          # first repetition warms up
+        f.clear(stream.handle)
+        ev[0].record(stream)
+        f.update_histogram_batch(d_maps, FUSION_MAPS, W, H, Ks, o["R"], o["t"], thr, stream.handle)
+        ev[1].record(stream)
+        f.solve(FUSION_ITERS, 0.1, 0.5, 1.0, stream.handle)
+        ev[2].record(stream)
+        ev[2].synchronize()
+        res = {"hist_ms": ev[0].elapsed_ms(ev[1]), "pd_ms": ev[1].elapsed_ms(ev[2])}
+    vox = g ** 3
+    res["surface_voxels"] = f.surface_count(stream.handle)
+    res["voxel_iters_per_s"] = vox * FUSION_ITERS / (res["pd_ms"] * 1e-3)
+    res["pd_gbs_algorithmic"] = FUSION_BYTES_PER_VOXEL_ITER * vox * FUSION_ITERS / (res["pd_ms"] * 1e-3) / 1e9
+    res["hist_voxel_maps_per_s"] = vox * FUSION_MAPS / (res["hist_ms"] * 1e-3)
+    res["config"] = (f"{g}^3 voxels over the room box, {FUSION_MAPS} depthmaps 640x480 (ray-cast depth + 1 cm "
+                     f"noise, orbiting cameras), 8 uint16 bins, band +-{thr:.3f} m (5 voxels), "
+                     f"{FUSION_ITERS} PD iterations, tau 0.1 lambda 0.5 sigma 1")
+    return res
+
+
+def probe(names, seed):
+    """Small fixed workloads for rocprofv3 runs (kernel stats and counters)."""
+    import psw_amd
+    from psw_amd import scene, stages
+    from psw_amd.device import DeviceArray, Event, Stream
+    assert psw_amd.device_count() > 0
+    st = Stream()
+    s = st.handle
+    for name in names:
+        log(f"probe {name}")
+        if name == "copy":
+            n = 64 * 1024 * 1024          # 256 MiB of floats
+            a, b = DeviceArray((n,)), DeviceArray((n,))
+            a.fill(1.0, s)
+            for _ in range(5):
+                psw_amd._lib.call("psw_copy_f32", b.ptr, a.ptr, n, s)
+        elif name in ("tgv", "sweep", "sweep_exact"):
+            v = scene.make_views(seed=seed, width=W, height=H, nsrc=NSRC)
+            planes = stages.inverse_depth_planes(ZNEAR, ZFAR, NPLANES)
+            sw = stages.SweepAggregate(v["ref"], v["src"], v["K"], v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], planes,
+                                       WIN, STD_T)
+            if name == "tgv":
+                t = stages.TgvSolver(W, H, NSRC)
+                t.set_images(sw.d_ref, sw.d_src, 0.0, 1.0, s)
+                Rrel, trel = stages.relative_all(v["Rref"], v["tref"], v["Rsrc"], v["tsrc"])
+                t.set_u(value=1.0, stream=s)
+                t.linearise(scene.one_based(v["K"]), Rrel, trel, s)
+                t.iterate(100, stages.tgv_params(**TGV), s)
+            else:
+                sw.build_homographies(s)
+                for _ in range(3):
+                    sw.run(fast=(name == "sweep"), stream=s)
+        elif name == "fusion":
+            o, thr = fusion_workload(seed)
+            ev = [Event() for _ in range(3)]
+            log(json.dumps(run_fusion(o, thr, st, ev)))
+        st.synchronize()
+    log("probe done")
 
 
 def main():
@@ -77,11 +156,18 @@ def main():
     ap.add_argument("--estar-iters", type=int, default=6000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-fusion", action="store_true", help="skip the 256^3 fusion measurement")
+    ap.add_argument("--probe", default="", help="comma list of tgv,sweep,sweep_exact,copy,fusion: run only these "
+                                                "small workloads (for rocprofv3) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU design is described in DESIGN.md, section 'Sharding')", file=sys.stderr)
         sys.exit(2)
+
+    if args.probe:
+        probe(args.probe.split(","), args.seed)
+        return
 
     import psw_amd
     from psw_amd import scene, stages
@@ -199,6 +285,13 @@ def main():
                    "e_star": e_star, "sweep_kernel_ms": sweep_kernel_ms, "sweep_tflops_declared": sweep_tflops,
                    "sweep_frac_fp32_peak": sweep_tflops / FP32_PEAK_TFLOPS},
     }
+    if not args.no_fusion:
+        t0 = time.perf_counter()
+        o, thr = fusion_workload(args.seed)
+        log(f"fusion depthmaps rendered in {time.perf_counter() - t0:.1f} s")
+        rec["fusion"] = run_fusion(o, thr, st, ev)
+        log(f"fusion: histogram {rec['fusion']['hist_ms']:.2f} ms, {FUSION_ITERS} PD iterations "
+            f"{rec['fusion']['pd_ms']:.2f} ms ({rec['fusion']['pd_gbs_algorithmic']:.0f} GB/s algorithmic)")
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
         rec["cpu_baseline"] = cpu_baseline(v, planes, K0, K1, int(round(np.mean(iters_used))))

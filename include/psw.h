@@ -71,6 +71,8 @@ int psw_memcpy_dtoh(void* dst, const void* d_src, size_t bytes, psw_stream_t s);
 int psw_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes, psw_stream_t s);
 int psw_fill_f32(float* d_ptr, float value, size_t count, psw_stream_t s);   /* kernel fill, graph-safe */
 int psw_fill_u32(uint32_t* d_ptr, uint32_t value, size_t count, psw_stream_t s);
+/* Kernel copy of count floats (count % 4 == 0, 16-byte aligned), graph-safe. */
+int psw_copy_f32(float* d_dst, const float* d_src, size_t count, psw_stream_t s);
 int psw_stream_create(psw_stream_t* s);
 int psw_stream_destroy(psw_stream_t s);
 int psw_stream_synchronize(psw_stream_t s);

@@ -64,6 +64,11 @@ __global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t v, size_t n) 
         p[i] = v;
 }
 
+__global__ void copy_f32_kernel(float4* __restrict__ dst, const float4* __restrict__ src, size_t n4) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 static unsigned fill_blocks(size_t n) {
     size_t b = (n + 255) / 256;
     if (b > 8192) b = 8192;
@@ -186,6 +191,16 @@ int psw_fill_u32(uint32_t* d_ptr, uint32_t value, size_t count, psw_stream_t s) 
     PSW_CHECK_ARG(d_ptr, "d_ptr is NULL");
     fill_u32_kernel<<<fill_blocks(count), 256, 0, S(s)>>>(d_ptr, value, count);
     return launch_status("fill_u32_kernel");
+}
+
+int psw_copy_f32(float* d_dst, const float* d_src, size_t count, psw_stream_t s) {
+    if (count == 0) return PSW_OK;
+    PSW_CHECK_ARG(d_dst && d_src, "NULL pointer");
+    PSW_CHECK_ARG(count % 4 == 0 && ((uintptr_t)d_dst % 16) == 0 && ((uintptr_t)d_src % 16) == 0,
+                  "count must be a multiple of 4 and pointers 16-byte aligned");
+    copy_f32_kernel<<<fill_blocks(count / 4), 256, 0, S(s)>>>(reinterpret_cast<float4*>(d_dst),
+                                                             reinterpret_cast<const float4*>(d_src), count / 4);
+    return launch_status("copy_f32_kernel");
 }
 
 int psw_stream_create(psw_stream_t* s) {

@@ -62,6 +62,7 @@ _SIGS = {
     "psw_memcpy_dtod": (c_int, [D, D, c_size_t, c_void_p]),
     "psw_fill_f32": (c_int, [D, c_float, c_size_t, c_void_p]),
     "psw_fill_u32": (c_int, [D, ctypes.c_uint32, c_size_t, c_void_p]),
+    "psw_copy_f32": (c_int, [D, D, c_size_t, c_void_p]),
     "psw_stream_create": (c_int, [P(c_void_p)]),
     "psw_stream_destroy": (c_int, [c_void_p]),
     "psw_stream_synchronize": (c_int, [c_void_p]),
