@@ -85,6 +85,19 @@ __host__ __device__ inline M3f mm3(const M3f& A, const M3f& B) {
     return C;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8).  Map
+// block b to work item t so that each XCD gets a contiguous range of items:
+// with items in raster order, neighbouring tiles then share an XCD and the
+// halo cache lines one tile reads were usually just fetched by its neighbour
+// into the same L2.  A bijection of [0, nitems).
+__device__ inline int xcd_tile(int b, int nitems) {
+    const int per = (nitems + 7) / 8;
+    const int xcd = b & 7, k = b >> 3;
+    const int full = nitems - (per - 1) * 8;  // XCDs that get 'per' items
+    if (xcd < full) return xcd * per + k;
+    return full * per + (xcd - full) * (per - 1) + k;
+}
+
 // Reflect-101 border of the reference's box filters (kernels.cu:203-204).
 __device__ inline int reflect101(int k, int n) {
     if (k < 0) k = -k;

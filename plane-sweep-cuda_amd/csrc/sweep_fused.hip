@@ -41,25 +41,16 @@ struct SweepArgs {
     int W, H, N, P;
     int off;              // pixel-coordinate offset of the homography (1 = reference)
     int tiles_x, ntiles;
+    int G;                // plane groups (aggregate mode): workgroups per tile
     float stdthresh, nccthresh, zfar;
     float* depth_out;     // reference mode
-    int32_t* argmin;      // aggregate mode
+    int32_t* argmin;      // aggregate mode (G == 1) ...
     float* cmin;
     float* csec;
+    int32_t* ws_arg;      // ... or per-group partials [G][H*W] (G > 1)
+    float* ws_cmin;
+    float* ws_csec;
 };
-
-// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8).  Give
-// each XCD a contiguous band of tiles so that the source rows its tiles warp
-// from stay in that XCD's L2.
-__device__ inline int xcd_tile(int b, int ntiles) {
-    const int per = (ntiles + 7) / 8;
-    const int xcd = b & 7, k = b >> 3;
-    const int full = ntiles - (per - 1) * 8;  // XCDs that get 'per' tiles
-    int t;
-    if (xcd < full) t = xcd * per + k;
-    else t = full * per + (xcd - full) * (per - 1) + k;
-    return t;
-}
 
 template <bool FAST>
 __device__ inline float scale_mean(float s, float win, float rwin) {
@@ -74,14 +65,17 @@ __global__ __launch_bounds__(SW_THREADS) void sweep_fused_kernel(SweepArgs a) {
     constexpr int WIN = 2 * R + 1;
     constexpr int NCG = SW_TY / SW_CR;          // column-pass row groups
     constexpr int NCOL = NCG * TW;              // column-pass work items
+    constexpr int CV = SW_CR + 2 * R;           // column values per column item
+    constexpr int RV = SW_PX + 2 * R;           // row values per row item
     static_assert(NCOL <= SW_THREADS, "window too large for the tile");
     __shared__ float sR[TH * TW];
     __shared__ float sW[TH * TW];
     __shared__ float sC[3][SW_TY * TW];
 
-    const int b = blockIdx.x;
-    if (b >= a.ntiles) return;
-    const int t = xcd_tile(b, a.ntiles);
+    if ((int)blockIdx.x >= a.ntiles * a.G) return;
+    const int item = xcd_tile(blockIdx.x, a.ntiles * a.G);
+    const int t = item / a.G, g = item - (item / a.G) * a.G;
+    const int p0 = (g * a.P) / a.G, p1 = ((g + 1) * a.P) / a.G;
     const int x0 = (t % a.tiles_x) * SW_TX;
     const int y0 = (t / a.tiles_x) * SW_TY;
     const int tid = threadIdx.x;
@@ -106,18 +100,19 @@ __global__ __launch_bounds__(SW_THREADS) void sweep_fused_kernel(SweepArgs a) {
     }
     __syncthreads();
 
-    // Column-pass item of this thread.
+    // Column-pass item of this thread and its reference values (constant over
+    // the whole sweep, so they live in registers).
     const bool col_active = tid < NCOL;
-    const int ccx = tid % TW, cg = tid / TW;
+    const int ccx = tid % TW, cg = min(tid / TW, NCG - 1);
+    float rv[CV];
+#pragma unroll
+    for (int i = 0; i < CV; ++i) rv[i] = sR[(cg * SW_CR + i) * TW + ccx];
     // Row-pass pixels of this thread: (ox, oy), (ox+1, oy).
     const int rox = SW_PX * (tid % (SW_TX / SW_PX));
     const int roy = tid / (SW_TX / SW_PX);
 
     // ---- reference window statistics (src/planesweep.cpp:108-119) ----
     if (col_active) {
-        float rv[SW_CR + 2 * R];
-#pragma unroll
-        for (int i = 0; i < SW_CR + 2 * R; ++i) rv[i] = sR[(cg * SW_CR + i) * TW + ccx];
 #pragma unroll
         for (int j = 0; j < SW_CR; ++j) {
             float s1 = 0.f, s2 = 0.f;
@@ -133,9 +128,9 @@ __global__ __launch_bounds__(SW_THREADS) void sweep_fused_kernel(SweepArgs a) {
     __syncthreads();
     float rmean[SW_PX], rstd[SW_PX];
     {
-        float c0[SW_PX + 2 * R], c1[SW_PX + 2 * R];
+        float c0[RV], c1[RV];
 #pragma unroll
-        for (int i = 0; i < SW_PX + 2 * R; ++i) {
+        for (int i = 0; i < RV; ++i) {
             c0[i] = sC[0][roy * TW + rox + i];
             c1[i] = sC[1][roy * TW + rox + i];
         }
@@ -187,51 +182,97 @@ __global__ __launch_bounds__(SW_THREADS) void sweep_fused_kernel(SweepArgs a) {
         }
         __syncthreads();
         if (col_active) {
-            float wv[SW_CR + 2 * R], rv[SW_CR + 2 * R];
+            float wv[CV];
 #pragma unroll
-            for (int i = 0; i < SW_CR + 2 * R; ++i) {
-                wv[i] = sW[(cg * SW_CR + i) * TW + ccx];
-                rv[i] = sR[(cg * SW_CR + i) * TW + ccx];
-            }
-#pragma unroll
-            for (int j = 0; j < SW_CR; ++j) {
+            for (int i = 0; i < CV; ++i) wv[i] = sW[(cg * SW_CR + i) * TW + ccx];
+            if constexpr (FAST) {
+#pragma clang fp contract(fast)
+                // running window sums down the column
                 float s1 = 0.f, s2 = 0.f, s3 = 0.f;
 #pragma unroll
                 for (int i = 0; i < WIN; ++i) {
-                    const float w = wv[j + i];
-                    s1 += w;
-                    s2 += w * w;
-                    s3 += rv[j + i] * w;
+                    s1 += wv[i];
+                    s2 += wv[i] * wv[i];
+                    s3 += rv[i] * wv[i];
                 }
-                sC[0][(cg * SW_CR + j) * TW + ccx] = scale_mean<FAST>(s1, win, rwin);
-                sC[1][(cg * SW_CR + j) * TW + ccx] = scale_mean<FAST>(s2, win, rwin);
-                sC[2][(cg * SW_CR + j) * TW + ccx] = scale_mean<FAST>(s3, win, rwin);
+#pragma unroll
+                for (int j = 0; j < SW_CR; ++j) {
+                    if (j > 0) {
+                        const float wa = wv[j + 2 * R], wd = wv[j - 1];
+                        s1 += wa - wd;
+                        s2 += wa * wa - wd * wd;
+                        s3 += rv[j + 2 * R] * wa - rv[j - 1] * wd;
+                    }
+                    sC[0][(cg * SW_CR + j) * TW + ccx] = s1 * rwin;
+                    sC[1][(cg * SW_CR + j) * TW + ccx] = s2 * rwin;
+                    sC[2][(cg * SW_CR + j) * TW + ccx] = s3 * rwin;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < SW_CR; ++j) {
+                    float s1 = 0.f, s2 = 0.f, s3 = 0.f;
+#pragma unroll
+                    for (int i = 0; i < WIN; ++i) {
+                        const float w = wv[j + i];
+                        s1 += w;
+                        s2 += w * w;
+                        s3 += rv[j + i] * w;
+                    }
+                    sC[0][(cg * SW_CR + j) * TW + ccx] = s1 / win;
+                    sC[1][(cg * SW_CR + j) * TW + ccx] = s2 / win;
+                    sC[2][(cg * SW_CR + j) * TW + ccx] = s3 / win;
+                }
             }
         }
         __syncthreads();
-        float c0[SW_PX + 2 * R], c1[SW_PX + 2 * R], c2[SW_PX + 2 * R];
+        float c0[RV], c1[RV], c2[RV];
 #pragma unroll
-        for (int i = 0; i < SW_PX + 2 * R; ++i) {
+        for (int i = 0; i < RV; ++i) {
             c0[i] = sC[0][roy * TW + rox + i];
             c1[i] = sC[1][roy * TW + rox + i];
             c2[i] = sC[2][roy * TW + rox + i];
         }
-#pragma unroll
-        for (int k = 0; k < SW_PX; ++k) {
+        if constexpr (FAST) {
+#pragma clang fp contract(fast)
             float m = 0.f, q = 0.f, pm = 0.f;
 #pragma unroll
-            for (int i = 0; i < WIN; ++i) {
-                m += c0[k + i];
-                q += c1[k + i];
-                pm += c2[k + i];
+            for (int k = 0; k < SW_PX; ++k) {
+                if (k == 0) {
+#pragma unroll
+                    for (int i = 0; i < WIN; ++i) {
+                        m += c0[i];
+                        q += c1[i];
+                        pm += c2[i];
+                    }
+                } else {
+                    m += c0[k + 2 * R] - c0[k - 1];
+                    q += c1[k + 2 * R] - c1[k - 1];
+                    pm += c2[k + 2 * R] - c2[k - 1];
+                }
+                const float mm = m * rwin, qq = q * rwin, pp = pm * rwin;
+                const float var = qq - mm * mm;
+                const float sd = var > 0.f ? __builtin_amdgcn_sqrtf(var) : 0.f;
+                if ((rstd[k] < a.stdthresh) || (sd < a.stdthresh)) ncc[k] = 0.f;
+                else ncc[k] = (pp - rmean[k] * mm) * __builtin_amdgcn_rcpf(rstd[k] * sd);
             }
-            m = scale_mean<FAST>(m, win, rwin);
-            q = scale_mean<FAST>(q, win, rwin);
-            pm = scale_mean<FAST>(pm, win, rwin);
-            const float var = q - m * m;
-            const float sd = var > 0.f ? sqrtf(var) : 0.f;
-            if ((rstd[k] < a.stdthresh) || (sd < a.stdthresh)) ncc[k] = 0.f;
-            else ncc[k] = (pm - rmean[k] * m) / (rstd[k] * sd);
+        } else {
+#pragma unroll
+            for (int k = 0; k < SW_PX; ++k) {
+                float m = 0.f, q = 0.f, pm = 0.f;
+#pragma unroll
+                for (int i = 0; i < WIN; ++i) {
+                    m += c0[k + i];
+                    q += c1[k + i];
+                    pm += c2[k + i];
+                }
+                m = m / win;
+                q = q / win;
+                pm = pm / win;
+                const float var = q - m * m;
+                const float sd = var > 0.f ? sqrtf(var) : 0.f;
+                if ((rstd[k] < a.stdthresh) || (sd < a.stdthresh)) ncc[k] = 0.f;
+                else ncc[k] = (pm - rmean[k] * m) / (rstd[k] * sd);
+            }
         }
         // No barrier here: the next warp writes sW (its readers passed the
         // barrier above); the next column pass writes sC only after the next
@@ -276,7 +317,7 @@ __global__ __launch_bounds__(SW_THREADS) void sweep_fused_kernel(SweepArgs a) {
 #pragma unroll
         for (int k = 0; k < SW_PX; ++k) { cmin[k] = inf; csec[k] = inf; arg[k] = -1; }
         const float fN = (float)a.N;
-        for (int p = 0; p < a.P; ++p) {
+        for (int p = p0; p < p1; ++p) {
             float acc[SW_PX];
 #pragma unroll
             for (int k = 0; k < SW_PX; ++k) acc[k] = 0.f;
@@ -293,17 +334,53 @@ __global__ __launch_bounds__(SW_THREADS) void sweep_fused_kernel(SweepArgs a) {
                 else if (c < csec[k]) csec[k] = c;
             }
         }
+        int32_t* oa = a.argmin;
+        float* oc = a.cmin;
+        float* os = a.csec;
+        if (a.G > 1) {
+            oa = a.ws_arg + (size_t)g * npix;
+            oc = a.ws_cmin + (size_t)g * npix;
+            os = a.ws_csec + (size_t)g * npix;
+        }
 #pragma unroll
         for (int k = 0; k < SW_PX; ++k) {
             const int x = x0 + rox + k, y = y0 + roy;
             if (x < W && y < H) {
                 const size_t i = (size_t)y * W + x;
-                a.argmin[i] = arg[k];
-                a.cmin[i] = cmin[k];
-                a.csec[i] = csec[k];
+                oa[i] = arg[k];
+                oc[i] = cmin[k];
+                os[i] = csec[k];
             }
         }
     }
+}
+
+// Merge of the plane-group partials in plane order.  For groups A (lower
+// planes) then B: if B.min < A.min the argmin moves to B and the second best
+// is min(A.min, B.second), else it stays and the second best is
+// min(A.second, B.min) -- exactly what the sequential strict-< scan over all
+// planes produces.
+__global__ void sweep_merge_kernel(const float* __restrict__ wc, const int32_t* __restrict__ wa,
+                                   const float* __restrict__ ws, int G, size_t npix, int32_t* __restrict__ arg,
+                                   float* __restrict__ cmin, float* __restrict__ csec) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    float c = wc[i], s = ws[i];
+    int a = wa[i];
+    for (int g = 1; g < G; ++g) {
+        const float c2 = wc[(size_t)g * npix + i], s2 = ws[(size_t)g * npix + i];
+        const int a2 = wa[(size_t)g * npix + i];
+        if (c2 < c) {
+            s = (s2 < c) ? s2 : c;
+            c = c2;
+            a = a2;
+        } else if (c2 < s) {
+            s = c2;
+        }
+    }
+    arg[i] = a;
+    cmin[i] = c;
+    csec[i] = s;
 }
 
 // ---- FP64 homographies -----------------------------------------------------
@@ -390,7 +467,7 @@ static void d_relative(const float* Rref, const float* tref, const float* Rsrc, 
 
 template <int R>
 static int launch_sweep_r(const SweepArgs& a, int mode, bool fast, hipStream_t st) {
-    const dim3 grid((unsigned)a.ntiles), block(SW_THREADS);
+    const dim3 grid((unsigned)(a.ntiles * a.G)), block(SW_THREADS);
     if (mode == 0) {
         if (fast) sweep_fused_kernel<R, 0, true><<<grid, block, 0, st>>>(a);
         else sweep_fused_kernel<R, 0, false><<<grid, block, 0, st>>>(a);
@@ -402,9 +479,6 @@ static int launch_sweep_r(const SweepArgs& a, int mode, bool fast, hipStream_t s
 }
 
 static int launch_sweep(SweepArgs a, int winsize, int mode, bool fast, hipStream_t st) {
-    a.tiles_x = (a.W + SW_TX - 1) / SW_TX;
-    const int tiles_y = (a.H + SW_TY - 1) / SW_TY;
-    a.ntiles = a.tiles_x * tiles_y;
     switch (winsize / 2) {
         case 0: return launch_sweep_r<0>(a, mode, fast, st);
         case 1: return launch_sweep_r<1>(a, mode, fast, st);
@@ -418,6 +492,20 @@ static int launch_sweep(SweepArgs a, int winsize, int mode, bool fast, hipStream
             set_error("window size %d not supported (odd, 1..15)", winsize);
             return PSW_ERR_UNSUPPORTED;
     }
+}
+
+// Tiles of the image, and plane groups per tile so that a launch has enough
+// workgroups to fill 256 CUs (about 3000, i.e. two rounds at six resident
+// workgroups per CU).  PSW_SWEEP_GROUPS overrides the choice.
+static void sweep_layout(SweepArgs& a, bool allow_groups) {
+    a.tiles_x = (a.W + SW_TX - 1) / SW_TX;
+    const int tiles_y = (a.H + SW_TY - 1) / SW_TY;
+    a.ntiles = a.tiles_x * tiles_y;
+    a.G = 1;
+    if (!allow_groups) return;
+    int G = (3072 + a.ntiles / 2) / a.ntiles;
+    if (const char* e = getenv("PSW_SWEEP_GROUPS")) G = atoi(e);
+    a.G = std::max(1, std::min(G, std::min(a.P, 64)));
 }
 
 static int check_sweep_args(const float* d_ref, const float* d_src, int width, int height, int nsrc,
@@ -479,6 +567,7 @@ int psw_sweep_fused_reference(const float* d_ref, const float* d_src, int width,
     a.W = width; a.H = height; a.N = nsrc; a.P = nplanes; a.off = 1;
     a.stdthresh = stdthresh; a.nccthresh = nccthresh; a.zfar = zfar;
     a.depth_out = d_depth_out;
+    sweep_layout(a, false);
     return launch_sweep(a, winsize, 0, false, S(s));
 }
 
@@ -494,7 +583,24 @@ int psw_sweep_fused_aggregate(const float* d_ref, const float* d_src, int width,
     a.W = width; a.H = height; a.N = nsrc; a.P = nplanes; a.off = coord_offset;
     a.stdthresh = stdthresh;
     a.argmin = d_argmin; a.cmin = d_cost_min; a.csec = d_cost_second;
-    return launch_sweep(a, winsize, 1, fast != 0, S(s));
+    sweep_layout(a, true);
+    const size_t npix = (size_t)width * height;
+    void* ws = nullptr;
+    if (a.G > 1) {
+        // stream-ordered scratch for the per-group partials (pooled by the runtime)
+        PSW_TRY_HIP(hipMallocAsync(&ws, (size_t)a.G * npix * 12, S(s)), PSW_ERR_ALLOC);
+        a.ws_cmin = static_cast<float*>(ws);
+        a.ws_csec = a.ws_cmin + (size_t)a.G * npix;
+        a.ws_arg = reinterpret_cast<int32_t*>(a.ws_csec + (size_t)a.G * npix);
+    }
+    st = launch_sweep(a, winsize, 1, fast != 0, S(s));
+    if (st == PSW_OK && a.G > 1) {
+        sweep_merge_kernel<<<(unsigned)((npix + 255) / 256), 256, 0, S(s)>>>(a.ws_cmin, a.ws_arg, a.ws_csec, a.G, npix,
+                                                                           d_argmin, d_cost_min, d_cost_second);
+        st = launch_status("sweep_merge_kernel");
+    }
+    if (ws) (void)hipFreeAsync(ws, S(s));
+    return st;
 }
 
 int psw_sweep_argmin_depth(float* d_depth, const int32_t* d_argmin, const float* d_depths, int npixels,

@@ -201,7 +201,9 @@ __global__ __launch_bounds__(TG_THREADS) void tgv_iter_kernel(TgvPtrs a) {
     if (a.ctrl->done) return;
     const int W = a.W, H = a.H;
     const size_t n = (size_t)W * H;
-    const int x0 = blockIdx.x * TG_TX, y0 = blockIdx.y * TG_TY;
+    const int tiles_x = (W + TG_TX - 1) / TG_TX, ntiles = tiles_x * ((H + TG_TY - 1) / TG_TY);
+    const int tile = xcd_tile(blockIdx.x, ntiles);
+    const int x0 = (tile % tiles_x) * TG_TX, y0 = (tile / tiles_x) * TG_TY;
     const int tid = threadIdx.x;
 
     // phase 1: ubar / u1bar at clamped coordinates (clamping reproduces the
@@ -381,7 +383,7 @@ static inline TgvPtrs tgv_ptrs(psw_tgv* t, const psw_tgv_params* prm) {
 }
 
 static int tgv_launch_iters(psw_tgv* t, int niters, const psw_tgv_params* prm, hipStream_t st) {
-    const dim3 grid((unsigned)((t->W + TG_TX - 1) / TG_TX), (unsigned)((t->H + TG_TY - 1) / TG_TY));
+    const dim3 grid((unsigned)(((t->W + TG_TX - 1) / TG_TX) * ((t->H + TG_TY - 1) / TG_TY)));
     for (int i = 0; i < niters; ++i) {
         tgv_iter_kernel<<<grid, TG_THREADS, 0, st>>>(tgv_ptrs(t, prm));
         int s = launch_status("tgv_iter_kernel");
