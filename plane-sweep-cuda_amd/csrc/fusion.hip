@@ -292,6 +292,47 @@ __global__ void fusion_pd_fused_kernel(PdArgs a) {
     a.vout[i] = 2.f * unew - un;
 }
 
+// The same fused iteration with each thread marching FZ_CHUNK consecutive z
+// planes of one (x, y) column: the new p of plane z-1 stays in a register, so
+// only the first plane of a chunk recomputes its z-1 neighbour.  Every value
+// goes through the same arithmetic in the same order as in
+// fusion_pd_fused_kernel, so both kernels give identical results.
+constexpr int FZ_CHUNK = 16;
+
+__global__ __launch_bounds__(256) void fusion_pd_zmarch_kernel(PdArgs a) {
+    const FusionView& f = a.f;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z0 = blockIdx.z * FZ_CHUNK;
+    if (x >= f.w || y >= f.h || z0 >= f.d) return;
+    const int z1 = min(z0 + FZ_CHUNK, f.d);
+    const size_t V = f.V, sxy = (size_t)f.w * f.h;
+    size_t i = ((size_t)z0 * f.h + y) * f.w + x;
+    float pz_below = 0.f;
+    if (z0 > 0) {
+        float tx, ty;
+        p_new_at(a, V, x, y, z0 - 1, i - sxy, tx, ty, pz_below);
+    }
+    for (int z = z0; z < z1; ++z, i += sxy) {
+        float cx, cy, cz;
+        p_new_at(a, V, x, y, z, i, cx, cy, cz);
+        a.pout[i] = cx;
+        a.pout[V + i] = cy;
+        a.pout[2 * V + i] = cz;
+        float div = cx + cy + cz;
+        float tx, ty, tz;
+        if (x > 0) { p_new_at(a, V, x - 1, y, z, i - 1, tx, ty, tz); div -= tx; }
+        if (y > 0) { p_new_at(a, V, x, y - 1, z, i - f.w, tx, ty, tz); div -= ty; }
+        if (z > 0) div -= pz_below;
+        pz_below = cz;
+        const float un = a.uin[i];
+        const float uu = un - a.tau * (-div);
+        const float unew = prox_hist(uu, f.hist + i * FB, a.tl);
+        a.uout[i] = unew;
+        a.vout[i] = 2.f * unew - un;
+    }
+}
+
 __global__ void surface_count_kernel(const float* __restrict__ u, size_t n, unsigned long long* out) {
     __shared__ unsigned int part[256 / 64];
     unsigned int c = 0;
@@ -465,14 +506,25 @@ int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double la
         a.tau = (float)tau; a.lambda = (float)lambda; a.tl = a.tau * a.lambda; a.sigma = (float)sigma;
         float* pb[2] = {g->p, p2};
         float* vb[2] = {g->v, v2};
-        const dim3 blk(64, 4, 1), grd = vox_grid(f, blk);
+        // PSW_FUSION_PD=0 selects the one-voxel-per-thread kernel (same results).
+        const char* env = getenv("PSW_FUSION_PD");
+        const bool zmarch = !(env && atoi(env) == 0);
+        const dim3 blk(64, 4, 1);
+        const dim3 grd = zmarch ? dim3((unsigned)((f.w + 63) / 64), (unsigned)((f.h + 3) / 4),
+                                       (unsigned)((f.d + FZ_CHUNK - 1) / FZ_CHUNK))
+                                : vox_grid(f, blk);
         int cur = 0;
         for (int k = 1; k < niters && st == PSW_OK; ++k) {
             a.pin = pb[cur]; a.pout = pb[cur ^ 1];
             a.vin = vb[cur]; a.vout = vb[cur ^ 1];
             a.uin = g->u; a.uout = g->u;
-            fusion_pd_fused_kernel<<<grd, blk, 0, S(s)>>>(a);
-            st = launch_status("fusion_pd_fused_kernel");
+            if (zmarch) {
+                fusion_pd_zmarch_kernel<<<grd, blk, 0, S(s)>>>(a);
+                st = launch_status("fusion_pd_zmarch_kernel");
+            } else {
+                fusion_pd_fused_kernel<<<grd, blk, 0, S(s)>>>(a);
+                st = launch_status("fusion_pd_fused_kernel");
+            }
             cur ^= 1;
         }
         if (st == PSW_OK && cur == 1) {

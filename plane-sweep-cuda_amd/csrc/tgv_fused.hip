@@ -23,8 +23,7 @@
 namespace psw {
 
 constexpr int TG_TX = 64;
-constexpr int TG_TY = 8;
-constexpr int TG_THREADS = 256;
+constexpr int TG_THREADS = 256;   // energy kernels
 
 struct TgvLayout {
     size_t n;
@@ -191,9 +190,13 @@ __global__ void tgv_warp_reset_kernel(float* __restrict__ U, float* __restrict__
 
 // ---- the fused primal-dual iteration --------------------------------------
 
-__global__ __launch_bounds__(TG_THREADS) void tgv_iter_kernel(TgvPtrs a) {
-    constexpr int HX = TG_TX + 2, HY = TG_TY + 2;   // ubar region: x0-1..x0+64, y0-1..y0+8
-    constexpr int DX = TG_TX + 1, DY = TG_TY + 1;   // dual region: x0-1..x0+63, y0-1..y0+7
+// Tile TG_TX x TY pixels, NT threads.  Variants (selected by tgv_variant()):
+// 64x8 with 256 or 512 threads, 64x4 with 256 threads.
+template <int TY, int NT>
+__global__ __launch_bounds__(NT) void tgv_iter_kernel(TgvPtrs a) {
+    constexpr int TG_TY = TY, TG_THREADS = NT;
+    constexpr int HX = TG_TX + 2, HY = TG_TY + 2;   // ubar region: x0-1..x0+TX, y0-1..y0+TY
+    constexpr int DX = TG_TX + 1, DY = TG_TY + 1;   // dual region: x0-1..x0+TX-1, y0-1..y0+TY-1
     __shared__ float sUb[3][HY][HX];
     __shared__ float sP[2][DY][DX];
     __shared__ float sQ[3][DY][DX];
@@ -382,10 +385,26 @@ static inline TgvPtrs tgv_ptrs(psw_tgv* t, const psw_tgv_params* prm) {
     return p;
 }
 
+// Launch shape of the iteration kernel: 0 = 64x8 tile / 256 threads,
+// 1 = 64x8 / 512, 2 = 64x4 / 256 (PSW_TGV_VARIANT overrides the default).
+static int tgv_variant() {
+    static int v = [] {
+        const char* e = getenv("PSW_TGV_VARIANT");
+        int x = e ? atoi(e) : 1;
+        return (x < 0 || x > 2) ? 1 : x;
+    }();
+    return v;
+}
+
 static int tgv_launch_iters(psw_tgv* t, int niters, const psw_tgv_params* prm, hipStream_t st) {
-    const dim3 grid((unsigned)(((t->W + TG_TX - 1) / TG_TX) * ((t->H + TG_TY - 1) / TG_TY)));
+    const int var = tgv_variant();
+    const int ty = var == 2 ? 4 : 8;
+    const dim3 grid((unsigned)(((t->W + TG_TX - 1) / TG_TX) * ((t->H + ty - 1) / ty)));
     for (int i = 0; i < niters; ++i) {
-        tgv_iter_kernel<<<grid, TG_THREADS, 0, st>>>(tgv_ptrs(t, prm));
+        const TgvPtrs p = tgv_ptrs(t, prm);
+        if (var == 0) tgv_iter_kernel<8, 256><<<grid, 256, 0, st>>>(p);
+        else if (var == 1) tgv_iter_kernel<8, 512><<<grid, 512, 0, st>>>(p);
+        else tgv_iter_kernel<4, 256><<<grid, 256, 0, st>>>(p);
         int s = launch_status("tgv_iter_kernel");
         if (s != PSW_OK) return s;
         t->k++;

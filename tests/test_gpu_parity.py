@@ -319,6 +319,25 @@ def test_fusion_pd_vs_oracle(gpu):
     assert f.surface_count() == oracle.surface_count(g["u"])
 
 
+def test_fusion_solve_kernels_identical(gpu, monkeypatch):
+    """The z-marching PD kernel and the one-voxel-per-thread kernel give
+    bit-identical grids (depth 28 leaves a partial z chunk)."""
+    from psw_amd import stages
+    from psw_amd.device import DeviceArray
+    o, vol, (gw, gh, gd) = _fusion_inputs(n=3)
+    H, W = o["depths"].shape[1:]
+    out = []
+    for mode in ("1", "0"):
+        monkeypatch.setenv("PSW_FUSION_PD", mode)
+        f = stages.Fusion(gw, gh, gd, vol)
+        for k in range(o["depths"].shape[0]):
+            f.update_histogram(DeviceArray.from_host(o["depths"][k]), W, H, o["K"], o["R"][k], o["t"][k], 0.25)
+        f.solve(41, 0.1, 0.5, 1.0)
+        out.append(f.download())
+    for key in ("u", "v", "p", "hist"):
+        np.testing.assert_array_equal(out[0][key], out[1][key])
+
+
 # ---- TVL1 ---------------------------------------------------------------------------------
 
 def test_tvl1_vs_oracle(gpu, views_small):
