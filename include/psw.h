@@ -292,6 +292,15 @@ int psw_tgv_set_u(psw_tgv* t, const float* d_u, float value, psw_stream_t s);
  * (src/planesweep.cpp:445-483).  K (1-based convention), Rrel[nsrc*9],
  * trel[nsrc*3] host arrays. */
 int psw_tgv_linearise(psw_tgv* t, const float* K, const float* Rrel, const float* trel, psw_stream_t s);
+/* Warm start within the current linearisation (no counterpart in the
+ * reference, which always restarts a warp from u0 with u1 = 0): u := d_u
+ * (device, W*H) or u_value when d_u is NULL, u1 := d_u1 (device, 2*W*H:
+ * u1x then u1y) or 0 when NULL; duals p, q, r := 0.  u0, It and Iu stay. */
+int psw_tgv_set_primal(psw_tgv* t, const float* d_u, float u_value, const float* d_u1, psw_stream_t s);
+/* Load the duals (no counterpart in the reference; for warm starts and
+ * state restore): p (2*W*H: px, py), q (3*W*H: qx, qy, qz), r (nsrc*W*H),
+ * device arrays; a NULL array sets that dual to 0. */
+int psw_tgv_set_duals(psw_tgv* t, const float* d_p, const float* d_q, const float* d_r, psw_stream_t s);
 /* Run niters fused primal-dual iterations. */
 int psw_tgv_iterate(psw_tgv* t, int niters, const psw_tgv_params* prm, psw_stream_t s);
 /* FP64 primal energy of the current iterate (synchronous). */

@@ -188,6 +188,27 @@ __global__ void tgv_warp_reset_kernel(float* __restrict__ U, float* __restrict__
     }
 }
 
+// Warm start inside a linearisation: both time levels of u and u1 take the
+// given fields (u1 = 0 when none is given), duals p, q, r restart from 0; u0,
+// It and Iu (the linearisation) are left alone.
+__global__ void tgv_set_primal_kernel(float* __restrict__ U, float* __restrict__ P, float* __restrict__ Q,
+                                      float* __restrict__ r, const float* __restrict__ du, float uval,
+                                      const float* __restrict__ du1, size_t n, int N, int cur, int prev) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float u = du ? du[i] : uval;
+        const float ax = du1 ? du1[i] : 0.f, ay = du1 ? du1[n + i] : 0.f;
+        U[(size_t)cur * 3 * n + i] = u;
+        U[(size_t)prev * 3 * n + i] = u;
+        U[(size_t)cur * 3 * n + n + i] = ax;
+        U[(size_t)cur * 3 * n + 2 * n + i] = ay;
+        U[(size_t)prev * 3 * n + n + i] = ax;
+        U[(size_t)prev * 3 * n + 2 * n + i] = ay;
+        P[i] = 0.f; P[n + i] = 0.f;
+        Q[i] = 0.f; Q[n + i] = 0.f; Q[2 * n + i] = 0.f;
+        for (int j = 0; j < N; ++j) r[(size_t)j * n + i] = 0.f;
+    }
+}
+
 // ---- the fused primal-dual iteration --------------------------------------
 
 // Tile TG_TX x TY pixels, NT threads.  Variants (selected by tgv_variant()):
@@ -388,12 +409,9 @@ static inline TgvPtrs tgv_ptrs(psw_tgv* t, const psw_tgv_params* prm) {
 // Launch shape of the iteration kernel: 0 = 64x8 tile / 256 threads,
 // 1 = 64x8 / 512, 2 = 64x4 / 256 (PSW_TGV_VARIANT overrides the default).
 static int tgv_variant() {
-    static int v = [] {
-        const char* e = getenv("PSW_TGV_VARIANT");
-        int x = e ? atoi(e) : 1;
-        return (x < 0 || x > 2) ? 1 : x;
-    }();
-    return v;
+    const char* e = getenv("PSW_TGV_VARIANT");
+    const int x = e ? atoi(e) : 1;
+    return (x < 0 || x > 2) ? 1 : x;
 }
 
 static int tgv_launch_iters(psw_tgv* t, int niters, const psw_tgv_params* prm, hipStream_t st) {
@@ -548,6 +566,40 @@ int psw_tgv_linearise(psw_tgv* t, const float* K, const float* Rrel, const float
     tgv_linearise_kernel<<<grd, blk, 0, S(s)>>>(pz, t->base + L.src, t->base + L.ref, t->base + L.u0, t->base + L.It,
                                                 t->base + L.Iu, t->base + L.r, t->W, t->H, t->N);
     st = launch_status("tgv_linearise_kernel");
+    if (st != PSW_OK) return st;
+    return tgv_reset_ctrl(t, S(s));
+}
+
+int psw_tgv_set_primal(psw_tgv* t, const float* d_u, float u_value, const float* d_u1, psw_stream_t s) {
+    PSW_CHECK_ARG(t, "NULL solver");
+    const TgvLayout& L = t->L;
+    const size_t n = L.n;
+    const int cur = t->k % 3, prev = (t->k + 2) % 3;
+    tgv_set_primal_kernel<<<1024, 256, 0, S(s)>>>(t->base + L.U, t->base + L.P + (size_t)(t->k % 2) * 2 * n,
+                                                  t->base + L.Q + (size_t)(t->k % 2) * 3 * n, t->base + L.r, d_u,
+                                                  u_value, d_u1, n, t->N, cur, prev);
+    int st = launch_status("tgv_set_primal_kernel");
+    if (st != PSW_OK) return st;
+    return tgv_reset_ctrl(t, S(s));
+}
+
+int psw_tgv_set_duals(psw_tgv* t, const float* d_p, const float* d_q, const float* d_r, psw_stream_t s) {
+    PSW_CHECK_ARG(t, "NULL solver");
+    const TgvLayout& L = t->L;
+    const size_t n = L.n;
+    float* P = t->base + L.P + (size_t)(t->k % 2) * 2 * n;
+    float* Q = t->base + L.Q + (size_t)(t->k % 2) * 3 * n;
+    float* R = t->base + L.r;
+    int st = PSW_OK;
+    if (d_p) PSW_TRY_HIP(hipMemcpyAsync(P, d_p, 2 * n * sizeof(float), hipMemcpyDeviceToDevice, S(s)), PSW_ERR_COPY);
+    else st = psw_fill_f32(P, 0.f, 2 * n, s);
+    if (st != PSW_OK) return st;
+    if (d_q) PSW_TRY_HIP(hipMemcpyAsync(Q, d_q, 3 * n * sizeof(float), hipMemcpyDeviceToDevice, S(s)), PSW_ERR_COPY);
+    else st = psw_fill_f32(Q, 0.f, 3 * n, s);
+    if (st != PSW_OK) return st;
+    if (d_r) PSW_TRY_HIP(hipMemcpyAsync(R, d_r, (size_t)t->N * n * sizeof(float), hipMemcpyDeviceToDevice, S(s)),
+                         PSW_ERR_COPY);
+    else st = psw_fill_f32(R, 0.f, (size_t)t->N * n, s);
     if (st != PSW_OK) return st;
     return tgv_reset_ctrl(t, S(s));
 }

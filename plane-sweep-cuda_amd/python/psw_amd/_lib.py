@@ -134,6 +134,8 @@ _SIGS = {
     "psw_tgv_state_bytes": (c_size_t, [c_int, c_int, c_int]),
     "psw_tgv_set_images": (c_int, [c_void_p, D, D, c_double, c_double, c_void_p]),
     "psw_tgv_set_u": (c_int, [c_void_p, D, c_float, c_void_p]),
+    "psw_tgv_set_primal": (c_int, [c_void_p, D, c_float, D, c_void_p]),
+    "psw_tgv_set_duals": (c_int, [c_void_p, D, D, D, c_void_p]),
     "psw_tgv_linearise": (c_int, [c_void_p, H, H, H, c_void_p]),
     "psw_tgv_iterate": (c_int, [c_void_p, c_int, P(TgvParams), c_void_p]),
     "psw_tgv_energy": (c_int, [c_void_p, P(TgvParams), P(c_double), c_void_p]),

@@ -164,6 +164,15 @@ class TgvSolver:
         K, Rrel, trel = _f32(K).ravel(), _f32(Rrel).ravel(), _f32(trel).ravel()
         call("psw_tgv_linearise", self.h, _hp(K), _hp(Rrel), _hp(trel), stream)
 
+    def set_primal(self, d_u=None, value=1.0, d_u1=None, stream=None):
+        """Warm start inside the current linearisation; duals restart from 0."""
+        call("psw_tgv_set_primal", self.h, d_u.ptr if d_u is not None else None, float(value),
+             d_u1.ptr if d_u1 is not None else None, stream)
+
+    def set_duals(self, d_p=None, d_q=None, d_r=None, stream=None):
+        """Load p (2,H,W), q (3,H,W), r (N,H,W) device arrays; None means 0."""
+        call("psw_tgv_set_duals", self.h, *[d.ptr if d is not None else None for d in (d_p, d_q, d_r)], stream)
+
     def iterate(self, niters, prm, stream=None):
         call("psw_tgv_iterate", self.h, int(niters), ctypes.byref(prm), stream)
 
