@@ -366,6 +366,100 @@ int psw_tvl1_denoise(const float* d_depth, const float* d_ref, int width, int he
                      int niters, double lambda, double tau, double sigma, double theta, double beta, double gamma,
                      float* d_out, float* d_work, psw_stream_t s);
 
+/* ---------------------------------------------------------------------- */
+/* Seeded synthetic scenes (no reference counterpart: the reference only     */
+/* ships ten ICL-NUIM frames).  A room is an axis-aligned interior box with  */
+/* axis-aligned boxes inside; every surface carries a 4-octave solid value-  */
+/* noise texture in [0, 1].                                                  */
+/* ---------------------------------------------------------------------- */
+#define PSW_SCENE_MAX_BOXES 32
+typedef struct {
+    double lo[3], hi[3];                    /* interior box */
+    int nboxes;
+    double box_lo[PSW_SCENE_MAX_BOXES][3];
+    double box_hi[PSW_SCENE_MAX_BOXES][3];
+    uint32_t octave_salt[4];                /* texture hash salt per octave */
+} psw_room;
+
+/* Ray-cast one pinhole view in double precision: d_depth = z in the camera
+ * frame, d_img = scale * (texture + N(0, noise_sigma^2)) with a counter-based
+ * Gaussian keyed by noise_seed (noise_sigma = 0: no noise).  invK: inverse of
+ * the 0-based K (row-major double[9]); R, t: world->camera (X_c = R X_w + t).
+ * Either output may be NULL. */
+int psw_scene_render(const psw_room* room, const double* invK, const double* R, const double* t, int width,
+                     int height, double noise_sigma, uint64_t noise_seed, float scale, float* d_img, float* d_depth,
+                     psw_stream_t s);
+
+/* ---------------------------------------------------------------------- */
+/* Learned TGV2 warm start (no reference counterpart).  A 4-level U-Net      */
+/* (3x3 convolutions, 32/48/64/96 channels, 480,264 parameters) in BF16 on   */
+/* the matrix cores with FP32 accumulation; forward, backward and Adam are   */
+/* hand-written kernels.  Input: the aggregated sweep of a view (reference   */
+/* image, argmin plane, min and second-best cost).  Output: u, u1, p, q to   */
+/* start psw_tgv_set_primal / psw_tgv_set_duals from.                        */
+/* Tensors live in "slots": slot b of a batch is image b.  width and height  */
+/* must be multiples of 8; batch * width * height <= max_pixels.            */
+/* ---------------------------------------------------------------------- */
+typedef struct psw_unet psw_unet; /* opaque */
+
+size_t psw_unet_param_count(void);
+/* Allocate for up to max_pixels = batch * width * height; parameters are
+ * initialised with psw_unet_init(seed = 1). */
+int psw_unet_create(psw_unet** out, int64_t max_pixels);
+int psw_unet_destroy(psw_unet* n);
+/* He-normal weights from a seeded host stream; the head starts near zero so
+ * the untrained warm start is the sweep depth.  Synchronous. */
+int psw_unet_init(psw_unet* n, uint64_t seed);
+/* Flat FP32 parameters: per layer W[3][3][c_in][c_out] then bias[c_out],
+ * layers in network order, head W[32][8] and bias[8] last.  Setting resets
+ * Adam's moments.  Synchronous. */
+int psw_unet_set_params(psw_unet* n, const float* params, size_t count);
+int psw_unet_get_params(psw_unet* n, float* params, size_t count);
+int psw_unet_get_grads(psw_unet* n, float* grads, size_t count);
+/* Slot input from a sweep: d_ref (0..255), d_argmin / d_cmin / d_csec of
+ * psw_sweep_fused_aggregate, d_depths[nplanes] the plane depths; pixels with
+ * no valid plane get depth `fallback`.  Also keeps the sweep depth of the slot. */
+int psw_unet_features(psw_unet* n, int slot, int width, int height, const float* d_ref, const int32_t* d_argmin,
+                      const float* d_cmin, const float* d_csec, const float* d_depths, int nplanes, float fallback,
+                      psw_stream_t s);
+/* Raw slot input (height, width, 8) FP32 and sweep depth (tests). */
+int psw_unet_set_input(psw_unet* n, int slot, int width, int height, const float* d_x, const float* d_depth,
+                       psw_stream_t s);
+/* Training target of a slot from a converged TGV2 state: 8 planes of
+ * width*height (u, u1x, u1y, px, py, qx, qy, qz, as psw_tgv_get_state's first
+ * eight), scaled against the slot's sweep depth. */
+int psw_unet_targets(psw_unet* n, int slot, int width, int height, const float* d_state, psw_stream_t s);
+int psw_unet_set_target(psw_unet* n, int slot, int width, int height, const float* d_target, psw_stream_t s);
+/* Forward pass over slots 0..batch-1. */
+int psw_unet_forward(psw_unet* n, int batch, int width, int height, psw_stream_t s);
+/* Output of a slot, (height, width, 8) FP32. */
+int psw_unet_output(psw_unet* n, int slot, int width, int height, float* d_out, psw_stream_t s);
+/* Warm-start fields of a slot (any may be NULL): d_u (W*H), d_u1 (2*W*H),
+ * d_p (2*W*H), d_q (3*W*H), in psw_tgv_set_primal / set_duals layout. */
+int psw_unet_warm_start(psw_unet* n, int slot, int width, int height, float* d_u, float* d_u1, float* d_p,
+                        float* d_q, psw_stream_t s);
+/* Forward, smooth-L1 loss (channel_weights[8]) against the slots' targets,
+ * backward into the gradient buffer.  loss (nullable, synchronous when set):
+ * mean over pixels of the weighted per-channel losses. */
+int psw_unet_loss_grad(psw_unet* n, int batch, int width, int height, const float* channel_weights, double* loss,
+                       psw_stream_t s);
+/* One Adam update (bias-corrected, step >= 1) from the gradient buffer. */
+int psw_unet_adam_step(psw_unet* n, double lr, double beta1, double beta2, double eps, int step, psw_stream_t s);
+/* Forward FLOPs (2 per multiply-add) of one pass. */
+int psw_unet_flops(int width, int height, int batch, double* fwd_flops);
+/* Single layers in NHWC bf16 (uint16 bit patterns), for tests and tools;
+ * these allocate temporaries and return after the work is done.
+ * Forward: y = conv3x3([xa | nearest-up(xb)], w[9][ca+cb][cout]) + bias, as
+ * bf16 ReLU (relu_bf16 = 1) or FP32 (0).  Data gradient: dx[.., cin] FP32 of
+ * dz[.., cout].  Weight gradient: dwb[(9*(ca+cb)+1)*cout] FP32, bias last. */
+int psw_conv3x3_bf16(const uint16_t* d_xa, int ca, const uint16_t* d_xb, int cb, const float* d_w,
+                     const float* d_bias, int cout, int batch, int width, int height, int relu_bf16, void* d_y,
+                     psw_stream_t s);
+int psw_conv3x3_bf16_dgrad(const uint16_t* d_dz, int cout, const float* d_w, int cin, int batch, int width,
+                           int height, float* d_dx, psw_stream_t s);
+int psw_conv3x3_bf16_wgrad(const uint16_t* d_xa, int ca, const uint16_t* d_xb, int cb, const uint16_t* d_dz,
+                           int cout, int batch, int width, int height, float* d_dwb, psw_stream_t s);
+
 #ifdef __cplusplus
 }
 #endif

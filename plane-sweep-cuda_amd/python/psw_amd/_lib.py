@@ -41,6 +41,15 @@ class FusionGrid(ctypes.Structure):
                 ("u", c_void_p), ("v", c_void_p), ("p", c_void_p), ("hist", c_void_p)]
 
 
+PSW_SCENE_MAX_BOXES = 32
+
+
+class PswRoom(ctypes.Structure):
+    _fields_ = [("lo", c_double * 3), ("hi", c_double * 3), ("nboxes", c_int),
+                ("box_lo", (c_double * 3) * PSW_SCENE_MAX_BOXES), ("box_hi", (c_double * 3) * PSW_SCENE_MAX_BOXES),
+                ("octave_salt", ctypes.c_uint32 * 4)]
+
+
 PSW_OK = 0
 PSW_MAX_SOURCES = 16
 
@@ -158,6 +167,29 @@ _SIGS = {
     "psw_fusion_solve": (c_int, [P(FusionGrid), c_int, c_double, c_double, c_double, c_void_p]),
     "psw_fusion_surface_count": (c_int, [P(FusionGrid), P(c_int64), c_void_p]),
     # TVL1
+    "psw_scene_render": (c_int, [P(PswRoom), H, H, H, c_int, c_int, c_double, ctypes.c_uint64, c_float, D, D,
+                                 c_void_p]),
+    # learned warm start
+    "psw_unet_param_count": (c_size_t, []),
+    "psw_unet_create": (c_int, [P(c_void_p), ctypes.c_int64]),
+    "psw_unet_destroy": (c_int, [c_void_p]),
+    "psw_unet_init": (c_int, [c_void_p, ctypes.c_uint64]),
+    "psw_unet_set_params": (c_int, [c_void_p, H, c_size_t]),
+    "psw_unet_get_params": (c_int, [c_void_p, H, c_size_t]),
+    "psw_unet_get_grads": (c_int, [c_void_p, H, c_size_t]),
+    "psw_unet_features": (c_int, [c_void_p, c_int, c_int, c_int, D, D, D, D, D, c_int, c_float, c_void_p]),
+    "psw_unet_set_input": (c_int, [c_void_p, c_int, c_int, c_int, D, D, c_void_p]),
+    "psw_unet_targets": (c_int, [c_void_p, c_int, c_int, c_int, D, c_void_p]),
+    "psw_unet_set_target": (c_int, [c_void_p, c_int, c_int, c_int, D, c_void_p]),
+    "psw_unet_forward": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p]),
+    "psw_unet_output": (c_int, [c_void_p, c_int, c_int, c_int, D, c_void_p]),
+    "psw_unet_warm_start": (c_int, [c_void_p, c_int, c_int, c_int, D, D, D, D, c_void_p]),
+    "psw_unet_loss_grad": (c_int, [c_void_p, c_int, c_int, c_int, H, P(c_double), c_void_p]),
+    "psw_unet_adam_step": (c_int, [c_void_p, c_double, c_double, c_double, c_double, c_int, c_void_p]),
+    "psw_unet_flops": (c_int, [c_int, c_int, c_int, P(c_double)]),
+    "psw_conv3x3_bf16": (c_int, [D, c_int, D, c_int, D, D, c_int, c_int, c_int, c_int, c_int, D, c_void_p]),
+    "psw_conv3x3_bf16_dgrad": (c_int, [D, c_int, D, c_int, c_int, c_int, c_int, D, c_void_p]),
+    "psw_conv3x3_bf16_wgrad": (c_int, [D, c_int, D, c_int, D, c_int, c_int, c_int, c_int, D, c_void_p]),
     "psw_tvl1_denoise": (c_int, [D, D, c_int, c_int, c_float, c_float, c_int, c_double, c_double, c_double, c_double,
                                  c_double, c_double, D, D, c_void_p]),
 }

@@ -53,6 +53,10 @@ class DeviceArray:
             assert self.nbytes % 4 == 0
             call("psw_fill_u32", self.ptr, int(value), self.nbytes // 4, stream)
 
+    def view(self, i):
+        """Non-owning view of item i along the first axis."""
+        return DeviceView(self, i)
+
     def free(self):
         if getattr(self, "_p", None):
             lib.psw_free(ctypes.c_void_p(self._p))
@@ -63,6 +67,21 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+class DeviceView(DeviceArray):
+    """Item i along the first axis of a DeviceArray (keeps the parent alive)."""
+
+    def __init__(self, parent, i):
+        assert 0 <= i < parent.shape[0]
+        self.parent = parent
+        self.shape = parent.shape[1:]
+        self.dtype = parent.dtype
+        self.nbytes = parent.nbytes // parent.shape[0]
+        self._p = parent._p + int(i) * self.nbytes
+
+    def free(self):
+        self._p = None
 
 
 class Stream:

@@ -130,10 +130,13 @@ class Room:
         R = np.asarray(R, np.float64)
         t = np.asarray(t, np.float64).reshape(3)
         u, v = np.meshgrid(np.arange(width, dtype=np.float64), np.arange(height, dtype=np.float64))
-        pix = np.stack([u.ravel(), v.ravel(), np.ones(u.size)], axis=1)
-        rays_cam = pix @ np.linalg.inv(K).T            # z component = 1
-        dirs = rays_cam @ R                            # R^T applied to each row
-        C = -R.T @ t
+        u, v = u.ravel(), v.ravel()
+        # Element-wise products in a fixed order (no BLAS), so that the GPU
+        # renderer (psw_scene_render) reproduces every ray bit for bit.
+        iK = np.linalg.inv(K)
+        rc = [iK[j, 0] * u + iK[j, 1] * v + iK[j, 2] for j in range(3)]    # z component = 1
+        dirs = np.stack([rc[0] * R[0, j] + rc[1] * R[1, j] + rc[2] * R[2, j] for j in range(3)], axis=1)
+        C = np.array([-(R[0, j] * t[0] + R[1, j] * t[1] + R[2, j] * t[2]) for j in range(3)])
         s = self.raycast(C, dirs)
         pts = C + s[:, None] * dirs
         img = texture(pts, self.seed).reshape(height, width)
@@ -211,3 +214,65 @@ def orbit_depthmaps(seed, n=50, width=640, height=480, radius=1.2, centre=(0.0, 
         ts.append(t)
         depths.append(dep)
     return {"K": K, "R": np.array(Rs), "t": np.array(ts), "depths": np.array(depths, np.float32), "room": room}
+
+
+# ---- GPU renderer (psw_scene_render) -----------------------------------------------
+
+def room_struct(room):
+    """The psw_room C struct of a Room (layout, and the per-octave texture
+    hash salts that _hash derives from the seed)."""
+    from ._lib import PSW_SCENE_MAX_BOXES, PswRoom
+    assert len(room.boxes) <= PSW_SCENE_MAX_BOXES
+    r = PswRoom()
+    for j in range(3):
+        r.lo[j], r.hi[j] = float(room.lo[j]), float(room.hi[j])
+    r.nboxes = len(room.boxes)
+    for b, (lo, hi) in enumerate(room.boxes):
+        for j in range(3):
+            r.box_lo[b][j], r.box_hi[b][j] = float(lo[j]), float(hi[j])
+    for k in range(4):
+        r.octave_salt[k] = ((room.seed + 101 * k) * 0x9E3779B1) & 0xFFFFFFFF
+    return r
+
+
+def render_gpu(room, K, R, t, width, height, noise_sigma=0.01, noise_seed=0, scale=1.0, d_img=None, d_depth=None,
+               stream=None):
+    """Room.render on the GPU into device arrays (allocated when not given).
+    Depth and the noise-free image equal Room.render's; the pixel noise is a
+    counter-based Gaussian keyed by noise_seed (not numpy's PCG64 stream)."""
+    import ctypes
+    from ._lib import call
+    from .device import DeviceArray
+    iK = np.ascontiguousarray(np.linalg.inv(np.asarray(K, np.float64)))
+    Rd = np.ascontiguousarray(np.asarray(R, np.float64))
+    td = np.ascontiguousarray(np.asarray(t, np.float64).reshape(3))
+    if d_img is None:
+        d_img = DeviceArray((height, width))
+    if d_depth is None:
+        d_depth = DeviceArray((height, width))
+    rs = room_struct(room)
+    call("psw_scene_render", ctypes.byref(rs), iK.ctypes.data, Rd.ctypes.data, td.ctypes.data, int(width),
+         int(height), float(noise_sigma), int(noise_seed) & 0xFFFFFFFFFFFFFFFF, float(scale),
+         d_img.ptr if d_img is not False else None, d_depth.ptr if d_depth is not False else None, stream)
+    return d_img, d_depth
+
+
+def make_views_gpu(seed, width=640, height=480, nsrc=4, noise_sigma=0.01, n_boxes=10, scale=255.0, focal=None,
+                   K=None, stream=None):
+    """make_views rendered on the GPU: same room, K and poses; images stay in
+    device memory (d_ref (H, W), d_src (N, H, W), d_depth (H, W)).  K may be
+    given (0-based), e.g. to render a window of a larger camera."""
+    from .device import DeviceArray
+    room = Room(seed, n_boxes=n_boxes)
+    if K is None:
+        K = intrinsics(width, height, FOCAL * width / 640.0 if focal is None else focal)
+    K = np.asarray(K, np.float32)
+    Rref, tref = np.eye(3, dtype=np.float32), np.zeros(3, np.float32)
+    Rs, ts = random_poses(nsrc, seed)
+    d_ref, d_depth = render_gpu(room, K, Rref, tref, width, height, noise_sigma, seed * 31 + 7, scale, stream=stream)
+    d_src = DeviceArray((nsrc, height, width))
+    for i in range(nsrc):
+        render_gpu(room, K, Rs[i], ts[i], width, height, noise_sigma, seed * 31 + 8 + i, scale,
+                   d_img=d_src.view(i), d_depth=False, stream=stream)
+    return {"d_ref": d_ref, "d_src": d_src, "d_depth": d_depth, "K": K, "Rref": Rref, "tref": tref, "Rsrc": Rs,
+            "tsrc": ts, "room": room}

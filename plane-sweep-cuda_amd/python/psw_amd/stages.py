@@ -83,13 +83,18 @@ class SweepAggregate:
 
     def __init__(self, ref, src, K, Rref, tref, Rsrc, tsrc, planes, winsize, stdthresh, coord_offset=0,
                  alternative=0):
-        ref, src = _f32(ref), _f32(src)
-        self.H_img, self.W = ref.shape
-        self.N = src.shape[0]
+        if isinstance(ref, DeviceArray):      # images already in device memory
+            self.d_ref, self.d_src = ref, src
+            self.H_img, self.W = ref.shape
+            self.N = src.shape[0]
+        else:
+            ref, src = _f32(ref), _f32(src)
+            self.H_img, self.W = ref.shape
+            self.N = src.shape[0]
+            self.d_ref, self.d_src = DeviceArray.from_host(ref), DeviceArray.from_host(src)
         self.planes = _f32(planes)
         self.P = self.planes.size
         self.winsize, self.stdthresh, self.coord_offset = int(winsize), float(stdthresh), int(coord_offset)
-        self.d_ref, self.d_src = DeviceArray.from_host(ref), DeviceArray.from_host(src)
         self.d_dep = DeviceArray.from_host(self.planes)
         self.d_H = DeviceArray((self.N, self.P, 9))
         self.poses = [_f32(x).ravel() for x in (K, Rref, tref, Rsrc, tsrc)]
@@ -191,6 +196,12 @@ class TgvSolver:
         d = DeviceArray((self.H, self.W))
         call("psw_tgv_get_u", self.h, d.ptr, stream)
         return d.to_host()
+
+    def state_device(self, stream=None):
+        """Full state planes (as get_state) in a new device array."""
+        d = DeviceArray((8 + 3 * self.N + 4, self.H, self.W))
+        call("psw_tgv_get_state", self.h, d.ptr, stream)
+        return d
 
     def get_state(self, stream=None):
         n = self.W * self.H
