@@ -1,16 +1,23 @@
 #!/usr/bin/env python3
-"""Throughput of the plain multiview-stereo pipeline on one MI355X.
+"""Throughput of the multiview-stereo pipeline on one MI355X, learned
+warm-start TGV2 against plain TGV2 at equal energy gap.
 
-Workload (one step = one depthmap): a seeded synthetic room, 640x480 float32
-reference view and 8 source views (0..255 intensities), pinhole K fx=fy=480,
-cx=319.5, cy=239.5; poses within 5 cm / 2 degrees of the reference.
+Workload (one step = one refined 640x480 depthmap): a seeded synthetic room
+rendered on the GPU, 640x480 float32 reference view and 8 source views
+(0..255 intensities), pinhole K fx=fy=480, cx=319.5, cy=239.5; poses within
+5 cm / 2 degrees of the reference.
   sweep : 256 fronto-parallel planes uniform in inverse depth over 0.3-8 m,
           9x9 NCC window, aggregated cost -(1/N) sum_s NCC, fused kernel with
           argmin / min / second-best (no cost volume in HBM);
-  TGV2  : cold start u = 1, one linearisation, lambda=1, alpha0=2, alpha1=1.5,
-          tau=0.02, sigma=1, iterated until the primal energy is within 1e-4
-          (relative) of E*, the converged energy of the same problem
-          (computed before the timed region), capped at 1000 iterations.
+  TGV2  : linearised at the sweep's argmin depth, lambda=1, alpha0=2,
+          alpha1=1.5, tau=0.02, sigma=1, iterated until the primal energy is
+          within 1e-4 (relative) of E*, the energy of the same problem after
+          40000 iterations from the plain start (computed before the timed
+          region), checked every 10 iterations, capped at 50000;
+  starts: learned = the BF16 U-Net's (u, u1, p, q) from the sweep results
+          (the headline value, network time included); plain = the
+          reference's warp start u = u0 (the sweep depth), u1 = p = q = r = 0;
+          cold = the reference's constant u = 1.
 Inputs are resident in HBM when the timed region starts.  Prints one JSON line.
 """
 import argparse
@@ -31,7 +38,9 @@ ZNEAR, ZFAR = 0.3, 8.0
 STD_T = 1e-4
 TGV = dict(lambda_=1.0, alpha0=2.0, alpha1=1.5, tau=0.02, sigma=1.0, beta=0.0, gamma=1.0)
 GAP_TOL = 1e-4
-MAX_ITERS = 1000
+MAX_ITERS = 50000
+ESTAR_ITERS = 40000
+BF16_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 SWEEP_FLOP_PER_EVAL = 50          # declared FLOP per (pixel, plane, source) NCC evaluation
@@ -42,23 +51,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(v, planes, K0, K1, iters_per_map):
-    """FP32 oracle (the CPU restatement, OpenMP) on one complete depthmap of
-    the same workload: the full aggregated sweep and as many TGV2 iterations
-    as the GPU needed to reach the gap."""
+def cpu_baseline(v, planes, K0, K1, ds, plain_iters, sample_iters=300):
+    """FP32 oracle (the CPU restatement, OpenMP) on the same inputs: the full
+    aggregated sweep of one depthmap, then `sample_iters` TGV2 iterations
+    linearised at the GPU's sweep depth; the depthmap rate is the sweep time
+    plus the sampled per-iteration time times the plain pipeline's iteration
+    count (the CPU path has no learned start)."""
     import oracle
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t0 = time.perf_counter()
     oracle.sweep_agg(v["ref"], v["src"], K0, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], planes, WIN, STD_T)
     t_sweep = time.perf_counter() - t0
     t0 = time.perf_counter()
-    oracle.tgv(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], iters_per_map, 1, **TGV)
-    t_tgv = time.perf_counter() - t0
-    t_map = t_sweep + t_tgv
+    oracle.tgv(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], sample_iters, 1, u_init=ds, **TGV)
+    t_it = (time.perf_counter() - t0) / sample_iters
+    t_map = t_sweep + t_it * plain_iters
     return {"value": 1.0 / t_map, "unit": "depthmaps/s", "cores": cores, "kind": "port",
-            "sample": f"one full depthmap on the FP32 oracle: aggregated sweep over all {NPLANES} planes x {NSRC} "
-                      f"sources and {iters_per_map} TGV2 iterations at 640x480 (single run)",
-            "sweep_s": t_sweep, "tgv_s": t_tgv}
+            "sample": f"FP32 oracle: one full aggregated sweep ({NPLANES} planes x {NSRC} sources, 640x480) and "
+                      f"{sample_iters} TGV2 iterations, scaled to the plain pipeline's {plain_iters} iterations",
+            "sweep_s": t_sweep, "tgv_iter_ms": t_it * 1e3}
 
 
 # ---- fusion workload: 256^3 grid, 50 depthmaps, 300 primal-dual iterations ----
@@ -72,7 +83,7 @@ FUSION_BYTES_PER_VOXEL_ITER = 56   # fused PD: read p 12, v 4, u 4, hist 16; wri
 
 def fusion_workload(seed):
     from psw_amd import scene
-    o = scene.orbit_depthmaps(seed=seed, n=FUSION_MAPS, width=W, height=H, noise_sigma=0.01)
+    o = scene.orbit_depthmaps_gpu(seed=seed, n=FUSION_MAPS, width=W, height=H, noise_sigma=0.01)
     edge = float(np.mean((FUSION_VOLUME[3:] - FUSION_VOLUME[:3]) / FUSION_GRID))
     return o, 5.0 * edge   # truncation band of +-5 voxel edges
 
@@ -105,14 +116,72 @@ def run_fusion(o, thr, stream, ev):
     return res
 
 
+class Pipeline:
+    """Device state of the benchmarked pipeline for one seeded view."""
+
+    def __init__(self, seed, weights):
+        import psw_amd
+        from psw_amd import scene, stages, warmstart
+        from psw_amd.device import Event, Stream
+        assert psw_amd.device_count() > 0, "no GPU"
+        self.st = Stream()
+        s = self.st.handle
+        t0 = time.perf_counter()
+        self.v = scene.make_views_gpu(seed, W, H, NSRC, stream=s)
+        self.st.synchronize()
+        log(f"scene rendered in {time.perf_counter() - t0:.2f} s")
+        self.planes = stages.inverse_depth_planes(ZNEAR, ZFAR, NPLANES)
+        v = self.v
+        self.K0 = v["K"]
+        self.K1 = scene.one_based(self.K0)
+        self.sweep = stages.SweepAggregate(v["d_ref"], v["d_src"], self.K0, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"],
+                                           self.planes, WIN, STD_T, coord_offset=0)
+        self.tgv = stages.TgvSolver(W, H, NSRC)
+        self.tgv.set_images(self.sweep.d_ref, self.sweep.d_src, TGV["beta"], TGV["gamma"], s)
+        self.Rrel, self.trel = stages.relative_all(v["Rref"], v["tref"], v["Rsrc"], v["tsrc"])
+        self.prm = stages.tgv_params(**TGV)
+        self.ws = warmstart.WarmStarter(W, H, weights) if weights else None
+        self.ev = [Event() for _ in range(4)]
+
+    def sweep_and_linearise(self, fast=True):
+        s = self.st.handle
+        self.sweep.build_homographies(s)
+        self.sweep.run(fast=fast, stream=s)
+        d_s = self.sweep.depth(ZFAR, s)
+        return d_s
+
+    def run(self, mode, e_star, max_iters, check_every, fast=True):
+        """One depthmap; mode: learned | plain | cold.  Returns stage times
+        (ms), iterations and the energy at the stop."""
+        s, ev, st = self.st.handle, self.ev, self.st
+        ev[0].record(st)
+        d_s = self.sweep_and_linearise(fast)
+        ev[1].record(st)
+        if mode == "learned":
+            self.ws.predict(self.sweep, ZFAR, s)
+        ev[2].record(st)
+        self.tgv.set_u(d_s, stream=s)
+        self.tgv.linearise(self.K1, self.Rrel, self.trel, s)
+        if mode == "learned":
+            self.ws.start(self.tgv, s)
+        elif mode == "cold":
+            self.tgv.set_primal(None, 1.0, None, s)
+        iters, e = self.tgv.solve(max_iters, check_every, GAP_TOL, e_star, self.prm, s)
+        ev[3].record(st)
+        ev[3].synchronize()
+        return {"sweep_ms": ev[0].elapsed_ms(ev[1]), "net_ms": ev[1].elapsed_ms(ev[2]),
+                "tgv_ms": ev[2].elapsed_ms(ev[3]), "iters": iters, "energy": e}
+
+
 def probe(names, seed):
     """Small fixed workloads for rocprofv3 runs (kernel stats and counters)."""
     import psw_amd
-    from psw_amd import scene, stages
+    from psw_amd import warmstart
     from psw_amd.device import DeviceArray, Event, Stream
     assert psw_amd.device_count() > 0
     st = Stream()
     s = st.handle
+    pl = None
     for name in names:
         log(f"probe {name}")
         if name == "copy":
@@ -121,124 +190,124 @@ def probe(names, seed):
             a.fill(1.0, s)
             for _ in range(5):
                 psw_amd._lib.call("psw_copy_f32", b.ptr, a.ptr, n, s)
-        elif name in ("tgv", "sweep", "sweep_exact"):
-            v = scene.make_views(seed=seed, width=W, height=H, nsrc=NSRC)
-            planes = stages.inverse_depth_planes(ZNEAR, ZFAR, NPLANES)
-            sw = stages.SweepAggregate(v["ref"], v["src"], v["K"], v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], planes,
-                                       WIN, STD_T)
+        elif name in ("tgv", "sweep", "sweep_exact", "unet"):
+            if pl is None:
+                pl = Pipeline(seed, warmstart.DEFAULT_WEIGHTS if name == "unet" else None)
             if name == "tgv":
-                t = stages.TgvSolver(W, H, NSRC)
-                t.set_images(sw.d_ref, sw.d_src, 0.0, 1.0, s)
-                Rrel, trel = stages.relative_all(v["Rref"], v["tref"], v["Rsrc"], v["tsrc"])
-                t.set_u(value=1.0, stream=s)
-                t.linearise(scene.one_based(v["K"]), Rrel, trel, s)
-                t.iterate(100, stages.tgv_params(**TGV), s)
+                d_s = pl.sweep_and_linearise()
+                pl.tgv.set_u(d_s, stream=pl.st.handle)
+                pl.tgv.linearise(pl.K1, pl.Rrel, pl.trel, pl.st.handle)
+                pl.tgv.iterate(100, pl.prm, pl.st.handle)
+            elif name == "unet":
+                if pl.ws is None:
+                    pl.ws = warmstart.WarmStarter(W, H, warmstart.DEFAULT_WEIGHTS)
+                pl.sweep_and_linearise()
+                for _ in range(5):
+                    pl.ws.predict(pl.sweep, ZFAR, pl.st.handle)
             else:
-                sw.build_homographies(s)
                 for _ in range(3):
-                    sw.run(fast=(name == "sweep"), stream=s)
+                    pl.sweep_and_linearise(fast=(name == "sweep"))
         elif name == "fusion":
             o, thr = fusion_workload(seed)
             ev = [Event() for _ in range(3)]
             log(json.dumps(run_fusion(o, thr, st, ev)))
         st.synchronize()
+        if pl is not None:
+            pl.st.synchronize()
     log("probe done")
 
 
 def main():
+    from psw_amd import warmstart
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=5, help="timed learned warm-start depthmaps")
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--plain-steps", type=int, default=2, help="timed plain-start depthmaps")
     ap.add_argument("--exact-sweep", action="store_true", help="reference-order (bit-exact) sweep arithmetic")
     ap.add_argument("--max-iters", type=int, default=MAX_ITERS)
     ap.add_argument("--check-every", type=int, default=10)
-    ap.add_argument("--estar-iters", type=int, default=6000)
+    ap.add_argument("--estar-iters", type=int, default=ESTAR_ITERS)
+    ap.add_argument("--weights", default=warmstart.DEFAULT_WEIGHTS)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cold", action="store_true", help="skip the constant-start (u = 1) run")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-fusion", action="store_true", help="skip the 256^3 fusion measurement")
-    ap.add_argument("--probe", default="", help="comma list of tgv,sweep,sweep_exact,copy,fusion: run only these "
-                                                "small workloads (for rocprofv3) and exit")
+    ap.add_argument("--probe", default="", help="comma list of tgv,sweep,sweep_exact,unet,copy,fusion: run only "
+                                                "these small workloads (for rocprofv3) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU design is described in DESIGN.md, section 'Sharding')", file=sys.stderr)
         sys.exit(2)
-
     if args.probe:
         probe(args.probe.split(","), args.seed)
         return
+    if not os.path.exists(args.weights):
+        sys.exit(f"bench.py: trained warm-start weights not found at {args.weights} "
+                 "(python -m psw_amd.warmstart train --out <file>)")
 
     import psw_amd
-    from psw_amd import scene, stages
-    from psw_amd.device import Event, Stream
-
-    assert psw_amd.device_count() > 0, "no GPU"
     log("device:", psw_amd.device_name())
-    t0 = time.perf_counter()
-    v = scene.make_views(seed=args.seed, width=W, height=H, nsrc=NSRC)
-    log(f"scene generated in {time.perf_counter() - t0:.1f} s")
-    planes = stages.inverse_depth_planes(ZNEAR, ZFAR, NPLANES)
-    K0 = v["K"]
-    K1 = scene.one_based(K0)
-    st = Stream()
-    s = st.handle
-    sweep = stages.SweepAggregate(v["ref"], v["src"], K0, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], planes, WIN,
-                                  STD_T, coord_offset=0)
-    tgv = stages.TgvSolver(W, H, NSRC)
-    tgv.set_images(sweep.d_ref, sweep.d_src, TGV["beta"], TGV["gamma"], s)
-    Rrel, trel = stages.relative_all(v["Rref"], v["tref"], v["Rsrc"], v["tsrc"])
-    prm = stages.tgv_params(**TGV)
+    pl = Pipeline(args.seed, args.weights)
+    s, st, ev = pl.st.handle, pl.st, pl.ev
     fast = not args.exact_sweep
 
-    # E*: converged energy of the cold-start problem (not timed)
-    tgv.set_u(value=1.0, stream=s)
-    tgv.linearise(K1, Rrel, trel, s)
-    tgv.iterate(args.estar_iters - 1000, prm, s)
-    e_prev = tgv.energy(prm, s)
-    tgv.iterate(1000, prm, s)
-    e_star = tgv.energy(prm, s)
-    log(f"E* = {e_star:.9e} after {args.estar_iters} iterations (change over the last 1000: "
-        f"{abs(e_prev - e_star) / e_star:.2e})")
-
-    ev = [Event() for _ in range(3)]
-    sweep_ms, tgv_ms, iters_used, gaps = [], [], [], []
-
-    def step(timed):
-        ev[0].record(st)
-        sweep.build_homographies(s)
-        sweep.run(fast=fast, stream=s)
-        ev[1].record(st)
-        tgv.set_u(value=1.0, stream=s)
-        tgv.linearise(K1, Rrel, trel, s)
-        iters, e = tgv.solve(args.max_iters, args.check_every, GAP_TOL, e_star, prm, s)
-        ev[2].record(st)
-        ev[2].synchronize()
-        if timed:
-            sweep_ms.append(ev[0].elapsed_ms(ev[1]))
-            tgv_ms.append(ev[1].elapsed_ms(ev[2]))
-            iters_used.append(iters)
-            gaps.append(abs(e - e_star) / e_star)
-
-    for _ in range(args.warmup):
-        step(False)
+    # E*: the same problem (linearised at the sweep depth) iterated from the
+    # plain start; not timed
+    d_s = pl.sweep_and_linearise(fast)
+    pl.tgv.set_u(d_s, stream=s)
+    pl.tgv.linearise(pl.K1, pl.Rrel, pl.trel, s)
+    pl.tgv.iterate(args.estar_iters - 5000, pl.prm, s)
+    e_prev = pl.tgv.energy(pl.prm, s)
+    pl.tgv.iterate(5000, pl.prm, s)
+    e_star = pl.tgv.energy(pl.prm, s)
+    u_star = pl.tgv.get_u(s)
     st.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step(True)
-    st.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = elapsed * 1e3 / args.steps
-    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step; sweep {np.mean(sweep_ms):.3f} ms, "
-        f"TGV2 {np.mean(tgv_ms):.3f} ms ({np.mean(iters_used):.0f} iterations)")
+    ds_host = d_s.to_host(s)
+    log(f"E* = {e_star:.9e} after {args.estar_iters} iterations (change over the last 5000: "
+        f"{(e_prev - e_star) / e_star:.2e})")
+
+    def timed(mode, k, warm):
+        for _ in range(warm):
+            pl.run(mode, e_star, args.max_iters, args.check_every, fast)
+        st.synchronize()
+        recs = []
+        t0 = time.perf_counter()
+        for _ in range(k):
+            recs.append(pl.run(mode, e_star, args.max_iters, args.check_every, fast))
+        st.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / k
+        u = pl.tgv.get_u(s)
+        st.synchronize()
+        agg = {key: float(np.mean([r[key] for r in recs])) for key in ("sweep_ms", "net_ms", "tgv_ms", "iters")}
+        agg.update(ms_per_depthmap=ms, depthmaps_per_s=1e3 / ms,
+                   rel_gap_at_stop=float(max(abs(r["energy"] - e_star) / e_star for r in recs)),
+                   reached_gap=bool(all(abs(r["energy"] - e_star) <= GAP_TOL * e_star for r in recs)),
+                   rms_rel_diff_vs_estar_state=float(np.sqrt(np.mean(((u - u_star) / u_star) ** 2))),
+                   median_rel_err_vs_true_depth=float(np.median(np.abs(u - pl.v["d_depth"].to_host())
+                                                                / pl.v["d_depth"].to_host())))
+        return agg
+
+    learned = timed("learned", args.steps, args.warmup)
+    log(f"learned: {learned['ms_per_depthmap']:.3f} ms/depthmap ({learned['iters']:.0f} TGV2 iterations, "
+        f"network {learned['net_ms']:.3f} ms)")
+    plain = timed("plain", args.plain_steps, 0)
+    log(f"plain: {plain['ms_per_depthmap']:.3f} ms/depthmap ({plain['iters']:.0f} TGV2 iterations)")
+    cold = None
+    if not args.no_cold:
+        cold = timed("cold", 1, 0)
+        log(f"cold: {cold['ms_per_depthmap']:.3f} ms/depthmap ({cold['iters']:.0f} iterations, gap "
+            f"{cold['rel_gap_at_stop']:.2e})")
+    ms_per_step = learned["ms_per_depthmap"]
 
     # Per-launch duration of the fused TGV2 iteration kernel (same stream and
     # buffers): events around back-to-back launches.
-    tgv.set_u(value=1.0, stream=s)
-    tgv.linearise(K1, Rrel, trel, s)
+    pl.tgv.set_u(d_s, stream=s)
+    pl.tgv.linearise(pl.K1, pl.Rrel, pl.trel, s)
     nl = 200
     ev[0].record(st)
-    tgv.iterate(nl, prm, s)
+    pl.tgv.iterate(nl, pl.prm, s)
     ev[1].record(st)
     ev[1].synchronize()
     tgv_launch_ms = ev[0].elapsed_ms(ev[1]) / nl
@@ -246,11 +315,19 @@ def main():
     achieved = tgv_bytes / (tgv_launch_ms * 1e-3) / 1e9
     # the sweep kernel alone
     ev[0].record(st)
-    sweep.run(fast=fast, stream=s)
+    pl.sweep.run(fast=fast, stream=s)
     ev[1].record(st)
     ev[1].synchronize()
     sweep_kernel_ms = ev[0].elapsed_ms(ev[1])
     sweep_tflops = SWEEP_FLOP_PER_EVAL * W * H * NPLANES * NSRC / (sweep_kernel_ms * 1e-3) / 1e12
+    # the network alone (features, forward, warm-start fields)
+    ev[0].record(st)
+    for _ in range(10):
+        pl.ws.predict(pl.sweep, ZFAR, s)
+    ev[1].record(st)
+    ev[1].synchronize()
+    net_ms = ev[0].elapsed_ms(ev[1]) / 10
+    net_tflops = warmstart.forward_flops(W, H) / (net_ms * 1e-3) / 1e12
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "tgv_iter_kernel_traffic.json")
@@ -258,10 +335,9 @@ def main():
         with open(tfile) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
 
-    value = 1e3 / ms_per_step
     rec = {
         "metric": "640x480 depthmaps/s at equal TGV2 gap: learned warm-start vs plain TGV2",
-        "value": value,
+        "value": 1e3 / ms_per_step,
         "unit": "depthmaps/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -271,30 +347,39 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded room scene, generated in-process)",
-        "config": {"workload": "plain pipeline (cold-start TGV2, no learned warm start): 640x480 reference + 8 "
-                               "sources, 256 inverse-depth planes 0.3-8 m, 9x9 NCC, fused sweep+argmin, TGV2 to "
-                               "relative energy gap 1e-4 (cap 1000 iterations)",
-                   "sweep_arith": "fast" if fast else "reference-order",
-                   "tgv_gap_tol": GAP_TOL, "check_every": args.check_every, "seed": args.seed},
+        "data": "synthetic (seeded room scene rendered on the GPU)",
+        "config": {"workload": "learned warm-start pipeline: 640x480 reference + 8 sources, 256 inverse-depth planes "
+                               "0.3-8 m, 9x9 NCC, fused sweep+argmin, BF16 U-Net warm start, TGV2 linearised at the "
+                               "sweep depth to relative energy gap 1e-4 of E* (40000-iteration plain run)",
+                   "sweep_arith": "fast" if fast else "reference-order", "tgv_gap_tol": GAP_TOL,
+                   "check_every": args.check_every, "max_iters": args.max_iters, "estar_iters": args.estar_iters,
+                   "seed": args.seed, "network_dtype": "bf16 (fp32 accumulation)"},
+        "speedup_vs_plain": plain["ms_per_depthmap"] / ms_per_step,
+        "learned": learned,
+        "plain": plain,
+        "cold": cold,
+        "e_star": e_star,
         "roofline": {"kernel": "tgv_iter_kernel", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": tgv_bytes, "launch_ms": tgv_launch_ms},
-        "stages": {"sweep_ms": float(np.mean(sweep_ms)), "tgv_ms": float(np.mean(tgv_ms)),
-                   "tgv_iterations": float(np.mean(iters_used)), "tgv_rel_gap_at_stop": float(np.max(gaps)),
-                   "e_star": e_star, "sweep_kernel_ms": sweep_kernel_ms, "sweep_tflops_declared": sweep_tflops,
-                   "sweep_frac_fp32_peak": sweep_tflops / FP32_PEAK_TFLOPS},
+        "stages": {"sweep_kernel_ms": sweep_kernel_ms, "sweep_tflops_declared": sweep_tflops,
+                   "sweep_frac_fp32_peak": sweep_tflops / FP32_PEAK_TFLOPS, "unet_ms": net_ms,
+                   "unet_tflops": net_tflops, "unet_frac_bf16_peak": net_tflops / BF16_PEAK_TFLOPS,
+                   "unet_fwd_gflop": warmstart.forward_flops(W, H) / 1e9},
     }
     if not args.no_fusion:
         t0 = time.perf_counter()
         o, thr = fusion_workload(args.seed)
         log(f"fusion depthmaps rendered in {time.perf_counter() - t0:.1f} s")
-        rec["fusion"] = run_fusion(o, thr, st, ev)
+        from psw_amd.device import Event, Stream
+        rec["fusion"] = run_fusion(o, thr, Stream(), [Event() for _ in range(3)])
         log(f"fusion: histogram {rec['fusion']['hist_ms']:.2f} ms, {FUSION_ITERS} PD iterations "
             f"{rec['fusion']['pd_ms']:.2f} ms ({rec['fusion']['pd_gbs_algorithmic']:.0f} GB/s algorithmic)")
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
-        rec["cpu_baseline"] = cpu_baseline(v, planes, K0, K1, int(round(np.mean(iters_used))))
+        vh = {"ref": pl.v["d_ref"].to_host(), "src": pl.v["d_src"].to_host(), "Rref": pl.v["Rref"],
+              "tref": pl.v["tref"], "Rsrc": pl.v["Rsrc"], "tsrc": pl.v["tsrc"]}
+        rec["cpu_baseline"] = cpu_baseline(vh, pl.planes, pl.K0, pl.K1, ds_host, int(round(plain["iters"])))
         log(f"cpu baseline sample took {time.perf_counter() - t0:.1f} s")
     else:
         rec["cpu_baseline"] = None

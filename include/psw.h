@@ -353,6 +353,10 @@ int psw_fusion_update_iteration(const psw_fusion_grid* g, const float* d_depthma
 /* niters primal-dual iterations (U then P each), fused kernels. */
 int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double lambda, double sigma,
                      psw_stream_t s);
+/* FP64 TV-L1 histogram energy of the current u (no reference counterpart):
+ * sum |grad u| + lambda sum_k h_k |u - c_k| with the P step's forward
+ * differences and the bin centres; synchronous. */
+int psw_fusion_energy(const psw_fusion_grid* g, double lambda, double* energy, psw_stream_t s);
 /* Voxels with -1 < u < 0 (src/pclviewer.cpp:1062); synchronous. */
 int psw_fusion_surface_count(const psw_fusion_grid* g, int64_t* count, psw_stream_t s);
 

@@ -53,6 +53,7 @@ _SIGS = {
     "orc_fusion_u_f64": (None, [_vp, _vp, _vp, _vp, _i, _i, _i, _d, _d]),
     "orc_fusion_p_f32": (None, [_vp, _vp, _i, _i, _i, _d]),
     "orc_fusion_p_f64": (None, [_vp, _vp, _i, _i, _i, _d]),
+    "orc_fusion_energy_f64": (_d, [_vp, _vp, _i, _i, _i, _d]),
     "orc_prox_hist_insertion_f32": (_f, [_f, _vp, _f, _f]),
     "orc_surface_count_f32": (_i64, [_vp, _i64]),
     "orc_surface_count_f64": (_i64, [_vp, _i64]),
@@ -252,6 +253,14 @@ def fusion_p(p, v, sigma, prec="f32"):
     gd, gh, gw = v.shape
     fn = _lib.orc_fusion_p_f32 if prec == "f32" else _lib.orc_fusion_p_f64
     fn(_p(p), _p(v), gw, gh, gd, sigma)
+
+
+def fusion_energy(u, hist, lambda_):
+    """FP64 TV-L1 histogram energy of u (gd, gh, gw) with hist (gd, gh, gw, 8)."""
+    u = np.ascontiguousarray(u, np.float64)
+    hist = np.ascontiguousarray(hist, np.uint16)
+    gd, gh, gw = u.shape
+    return _lib.orc_fusion_energy_f64(_p(u), _p(hist), gw, gh, gd, lambda_)
 
 
 def prox_hist_insertion(u, W, tau, lambda_):

@@ -931,6 +931,30 @@ void orc_fusion_p_f64(double* p, const double* v, int gw, int gh, int gd, double
     fusion_p<double>(p, v, gw, gh, gd, sigma);
 }
 
+// TV-L1 histogram energy of u (no counterpart in the reference, which has no
+// energy): sum_i |grad u_i|_2 + lambda sum_i sum_k h_k(i) |u_i - c_k|, with the
+// forward differences of FusionUpdateP_kernel (src/fusion.cu:59-70, zero at
+// the far faces) and the bin centres of binParams (src/inc/fusion.h:1041-1054).
+// Sequential double sum over voxels in x, y, z order.
+double orc_fusion_energy_f64(const double* u, const uint16_t* hist, int gw, int gh, int gd, double lambda) {
+    float cf[NB];
+    bin_centers(NB, cf);
+    const size_t sxy = (size_t)gw * gh;
+    double e = 0.0;
+    for (int z = 0; z < gd; ++z)
+        for (int y = 0; y < gh; ++y)
+            for (int x = 0; x < gw; ++x) {
+                const size_t i = (size_t)z * sxy + (size_t)y * gw + x;
+                const double gx = x < gw - 1 ? u[i + 1] - u[i] : 0.0;
+                const double gy = y < gh - 1 ? u[i + gw] - u[i] : 0.0;
+                const double gz = z < gd - 1 ? u[i + sxy] - u[i] : 0.0;
+                double d = 0.0;
+                for (int k = 0; k < NB; ++k) d += (double)hist[i * NB + k] * std::fabs(u[i] - (double)cf[k]);
+                e += std::sqrt(gx * gx + gy * gy + gz * gz) + lambda * d;
+            }
+    return e;
+}
+
 float orc_prox_hist_insertion_f32(float u, const int* Wv, float tau, float lambda) {
     float c[NB];
     bin_centers(NB, c);

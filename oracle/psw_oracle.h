@@ -164,6 +164,10 @@ void orc_fusion_u_f64(double* u, double* v, const double* p, const uint16_t* his
 void orc_fusion_p_f32(float* p, const float* v, int gw, int gh, int gd, double sigma);
 void orc_fusion_p_f64(double* p, const double* v, int gw, int gh, int gd, double sigma);
 
+/* TV-L1 histogram energy sum |grad u| + lambda sum_k h_k |u - c_k| in FP64
+ * (forward differences as FusionUpdateP_kernel; no reference counterpart). */
+double orc_fusion_energy_f64(const double* u, const uint16_t* hist, int gw, int gh, int gd, double lambda);
+
 /* proxHist through the reference's sortedHist insertion
  * (src/inc/fusion.h:851-858, src/inc/structs.h:212-284). W[8] = W_1..W_8. */
 float orc_prox_hist_insertion_f32(float u, const int* Wi, float tau, float lambda);

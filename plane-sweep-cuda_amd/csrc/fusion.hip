@@ -333,6 +333,103 @@ __global__ __launch_bounds__(256) void fusion_pd_zmarch_kernel(PdArgs a) {
     }
 }
 
+// z-marching with the x-1 and y-1 neighbours' new duals taken from the
+// threads that own them (through LDS) instead of recomputed: a 64x8 block
+// recomputes only its left column and top row halo.  Same arithmetic and
+// order as fusion_pd_fused_kernel, so the results are identical.
+constexpr int FZL_TX = 64, FZL_TY = 8;
+
+__global__ __launch_bounds__(FZL_TX * FZL_TY) void fusion_pd_zlds_kernel(PdArgs a) {
+    __shared__ float sPx[2][FZL_TY][FZL_TX];
+    __shared__ float sPy[2][FZL_TY][FZL_TX];
+    const FusionView& f = a.f;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int x = blockIdx.x * FZL_TX + tx;
+    const int y = blockIdx.y * FZL_TY + ty;
+    const int z0 = blockIdx.z * FZ_CHUNK;
+    const int z1 = min(z0 + FZ_CHUNK, f.d);   // uniform over the block
+    const bool live = x < f.w && y < f.h;
+    const size_t V = f.V, sxy = (size_t)f.w * f.h;
+    size_t i = live ? ((size_t)z0 * f.h + y) * f.w + x : 0;
+    float pz_below = 0.f;
+    if (live && z0 > 0) {
+        float tx_, ty_;
+        p_new_at(a, V, x, y, z0 - 1, i - sxy, tx_, ty_, pz_below);
+    }
+    for (int z = z0, buf = 0; z < z1; ++z, i += sxy, buf ^= 1) {
+        float cx = 0.f, cy = 0.f, cz = 0.f;
+        if (live) p_new_at(a, V, x, y, z, i, cx, cy, cz);
+        sPx[buf][ty][tx] = cx;
+        sPy[buf][ty][tx] = cy;
+        __syncthreads();   // one barrier per plane: the other buffer is written next plane
+        if (live) {
+            a.pout[i] = cx;
+            a.pout[V + i] = cy;
+            a.pout[2 * V + i] = cz;
+            float div = cx + cy + cz;
+            if (x > 0) {
+                float nx = 0.f;
+                if (tx > 0) {
+                    nx = sPx[buf][ty][tx - 1];
+                } else {
+                    float t1, t2;
+                    p_new_at(a, V, x - 1, y, z, i - 1, nx, t1, t2);
+                }
+                div -= nx;
+            }
+            if (y > 0) {
+                float ny = 0.f;
+                if (ty > 0) {
+                    ny = sPy[buf][ty - 1][tx];
+                } else {
+                    float t1, t2;
+                    p_new_at(a, V, x, y - 1, z, i - f.w, t1, ny, t2);
+                }
+                div -= ny;
+            }
+            if (z > 0) div -= pz_below;
+            pz_below = cz;
+            const float un = a.uin[i];
+            const float uu = un - a.tau * (-div);
+            const float unew = prox_hist(uu, f.hist + i * FB, a.tl);
+            a.uout[i] = unew;
+            a.vout[i] = 2.f * unew - un;
+        }
+    }
+}
+
+// FP64 TV-L1 histogram energy: per-block partials of
+// sum_i |grad u_i| + lambda sum_k h_k(i) |u_i - c_k| over a grid-stride loop
+// (fixed assignment of voxels to blocks, so the result is reproducible).
+constexpr int FE_BLOCKS = 1024;
+
+__global__ __launch_bounds__(256) void fusion_energy_kernel(FusionView f, double lambda, double* __restrict__ part) {
+    __shared__ double red[4];
+    const size_t sxy = (size_t)f.w * f.h;
+    const double c[FB] = {-1.0, -1.0, (double)(2.f * 1.f / 5.f - 1.f), (double)(2.f * 2.f / 5.f - 1.f),
+                          (double)(2.f * 3.f / 5.f - 1.f), (double)(2.f * 4.f / 5.f - 1.f), 1.0, 1.0};
+    double e = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < f.V; i += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % f.w);
+        const size_t t = i / f.w;
+        const int y = (int)(t % f.h), z = (int)(t / f.h);
+        const double u = f.u[i];
+        const double gx = x < f.w - 1 ? (double)f.u[i + 1] - u : 0.0;
+        const double gy = y < f.h - 1 ? (double)f.u[i + f.w] - u : 0.0;
+        const double gz = z < f.d - 1 ? (double)f.u[i + sxy] - u : 0.0;
+        const uint4 hv = *reinterpret_cast<const uint4*>(f.hist + i * FB);
+        const uint32_t hw[4] = {hv.x, hv.y, hv.z, hv.w};
+        double d = 0.0;
+#pragma unroll
+        for (int k = 0; k < FB; ++k) d += (double)((hw[k >> 1] >> (16 * (k & 1))) & 0xffffu) * fabs(u - c[k]);
+        e += sqrt(gx * gx + gy * gy + gz * gz) + lambda * d;
+    }
+    for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 __global__ void surface_count_kernel(const float* __restrict__ u, size_t n, unsigned long long* out) {
     __shared__ unsigned int part[256 / 64];
     unsigned int c = 0;
@@ -506,11 +603,14 @@ int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double la
         a.tau = (float)tau; a.lambda = (float)lambda; a.tl = a.tau * a.lambda; a.sigma = (float)sigma;
         float* pb[2] = {g->p, p2};
         float* vb[2] = {g->v, v2};
-        // PSW_FUSION_PD=0 selects the one-voxel-per-thread kernel (same results).
+        // PSW_FUSION_PD selects the kernel (identical results): 0 one voxel per
+        // thread, 1 z-marching, 2 z-marching with LDS-shared neighbour duals.
         const char* env = getenv("PSW_FUSION_PD");
-        const bool zmarch = !(env && atoi(env) == 0);
-        const dim3 blk(64, 4, 1);
-        const dim3 grd = zmarch ? dim3((unsigned)((f.w + 63) / 64), (unsigned)((f.h + 3) / 4),
+        int mode = env ? atoi(env) : 1;
+        if (mode < 0 || mode > 2) mode = 1;
+        const bool zmarch = mode >= 1;
+        const dim3 blk = mode == 2 ? dim3(FZL_TX, FZL_TY, 1) : dim3(64, 4, 1);
+        const dim3 grd = zmarch ? dim3((unsigned)((f.w + blk.x - 1) / blk.x), (unsigned)((f.h + blk.y - 1) / blk.y),
                                        (unsigned)((f.d + FZ_CHUNK - 1) / FZ_CHUNK))
                                 : vox_grid(f, blk);
         int cur = 0;
@@ -518,7 +618,10 @@ int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double la
             a.pin = pb[cur]; a.pout = pb[cur ^ 1];
             a.vin = vb[cur]; a.vout = vb[cur ^ 1];
             a.uin = g->u; a.uout = g->u;
-            if (zmarch) {
+            if (mode == 2) {
+                fusion_pd_zlds_kernel<<<grd, blk, 0, S(s)>>>(a);
+                st = launch_status("fusion_pd_zlds_kernel");
+            } else if (zmarch) {
                 fusion_pd_zmarch_kernel<<<grd, blk, 0, S(s)>>>(a);
                 st = launch_status("fusion_pd_zmarch_kernel");
             } else {
@@ -538,6 +641,25 @@ int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double la
         if (st != PSW_OK) return st;
     }
     return psw_fusion_update_p(g, sigma, s);
+}
+
+int psw_fusion_energy(const psw_fusion_grid* g, double lambda, double* energy, psw_stream_t s) {
+    int st = check_grid(g);
+    if (st != PSW_OK) return st;
+    PSW_CHECK_ARG(energy, "energy is NULL");
+    double* d = nullptr;
+    PSW_TRY_HIP(hipMalloc((void**)&d, FE_BLOCKS * sizeof(double)), PSW_ERR_ALLOC);
+    fusion_energy_kernel<<<FE_BLOCKS, 256, 0, S(s)>>>(view_of(g), lambda, d);
+    hipError_t e = hipGetLastError();
+    double h[FE_BLOCKS];
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, S(s));
+    if (e == hipSuccess) e = hipStreamSynchronize(S(s));
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_status(e, "psw_fusion_energy");
+    double t = 0.0;
+    for (int i = 0; i < FE_BLOCKS; ++i) t += h[i];
+    *energy = t;
+    return PSW_OK;
 }
 
 int psw_fusion_surface_count(const psw_fusion_grid* g, int64_t* count, psw_stream_t s) {

@@ -284,6 +284,12 @@ class Fusion:
     def solve(self, niters, tau, lambda_, sigma, stream=None):
         call("psw_fusion_solve", ctypes.byref(self.g), int(niters), float(tau), float(lambda_), float(sigma), stream)
 
+    def energy(self, lambda_, stream=None):
+        """FP64 TV-L1 histogram energy of the current u (synchronous)."""
+        e = ctypes.c_double()
+        call("psw_fusion_energy", ctypes.byref(self.g), float(lambda_), ctypes.byref(e), stream)
+        return e.value
+
     def surface_count(self, stream=None):
         c = ctypes.c_int64()
         call("psw_fusion_surface_count", ctypes.byref(self.g), ctypes.byref(c), stream)

@@ -162,7 +162,7 @@ class Dataset:
     """Sweep results and converged TGV2 states of many scene windows, kept in
     device memory (48 bytes per pixel)."""
 
-    def __init__(self, crop_w, crop_h, tgv_iters=8000, stream=None, log=None):
+    def __init__(self, crop_w, crop_h, tgv_iters=40000, stream=None, log=None):
         self.cw, self.ch = int(crop_w), int(crop_h)
         self.tgv_iters = int(tgv_iters)
         self.items = []
@@ -203,7 +203,7 @@ class Dataset:
         net.targets(slot, self.cw, self.ch, it["state"], s)
 
 
-def train(n_scenes=600, steps=3000, batch=8, crop=(256, 256), lr=2e-3, tgv_iters=8000, seed0=100000, log=print,
+def train(n_scenes=600, steps=3000, batch=8, crop=(256, 256), lr=2e-3, tgv_iters=40000, seed0=100000, log=print,
           out=None, init_seed=1):
     st = Stream()
     s = st.handle
@@ -260,7 +260,7 @@ def main(argv=None):
     t.add_argument("--batch", type=int, default=8)
     t.add_argument("--crop", type=int, default=256)
     t.add_argument("--lr", type=float, default=2e-3)
-    t.add_argument("--tgv-iters", type=int, default=8000)
+    t.add_argument("--tgv-iters", type=int, default=40000)
     t.add_argument("--out", default=DEFAULT_WEIGHTS)
     a = ap.parse_args(argv)
     if a.cmd == "train":

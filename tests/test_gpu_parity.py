@@ -402,25 +402,32 @@ def test_fusion_pd_vs_oracle(gpu):
     assert e.max() <= 1e-4
     assert ep <= 1e-4
     assert f.surface_count() == oracle.surface_count(g["u"])
+    # FP64 energy of the solved u: GPU block partials vs the oracle's sequential sum
+    e_gpu = f.energy(lam)
+    e_orc = oracle.fusion_energy(g["u"].astype(np.float64), g["hist"], lam)
+    print(f"fusion energy GPU {e_gpu:.12e} oracle {e_orc:.12e}")
+    assert abs(e_gpu - e_orc) <= 1e-12 * abs(e_orc)
 
 
 def test_fusion_solve_kernels_identical(gpu, monkeypatch):
-    """The z-marching PD kernel and the one-voxel-per-thread kernel give
-    bit-identical grids (depth 28 leaves a partial z chunk)."""
+    """The z-marching PD kernels (registers only, and with LDS-shared
+    neighbour duals) and the one-voxel-per-thread kernel give bit-identical
+    grids (depth 28 leaves a partial z chunk; 20 rows a partial 8-row block)."""
     from psw_amd import stages
     from psw_amd.device import DeviceArray
     o, vol, (gw, gh, gd) = _fusion_inputs(n=3)
     H, W = o["depths"].shape[1:]
     out = []
-    for mode in ("1", "0"):
+    for mode in ("1", "0", "2"):
         monkeypatch.setenv("PSW_FUSION_PD", mode)
         f = stages.Fusion(gw, gh, gd, vol)
         for k in range(o["depths"].shape[0]):
             f.update_histogram(DeviceArray.from_host(o["depths"][k]), W, H, o["K"], o["R"][k], o["t"][k], 0.25)
         f.solve(41, 0.1, 0.5, 1.0)
         out.append(f.download())
-    for key in ("u", "v", "p", "hist"):
-        np.testing.assert_array_equal(out[0][key], out[1][key])
+    for o in out[1:]:
+        for key in ("u", "v", "p", "hist"):
+            np.testing.assert_array_equal(out[0][key], o[key])
 
 
 # ---- TVL1 ---------------------------------------------------------------------------------

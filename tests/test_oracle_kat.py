@@ -382,6 +382,29 @@ def test_fusion_pd_invariants():
     assert np.all(ue == 0)
 
 
+def test_fusion_energy_closed_forms():
+    """TV-L1 histogram energy: a unit spike in the interior has gradient
+    norm sqrt(3) at itself and 1 at each of its three backward neighbours;
+    a lone vote in bin k at u = 0 costs lambda |c_k|; constant u = c_k with
+    all votes in bin k costs nothing."""
+    shape = (4, 5, 6)
+    u = np.zeros(shape)
+    h = np.zeros(shape + (8,), np.uint16)
+    assert oracle.fusion_energy(u, h, 0.5) == 0.0
+    u[2, 2, 3] = 1.0
+    np.testing.assert_allclose(oracle.fusion_energy(u, h, 0.5), 3.0 + np.sqrt(3.0), rtol=1e-15)
+    # binParams (src/inc/fusion.h:1041-1054) in float: c_i = 2 (i - 1) / (8 - 3) - 1, ends at -1 and 1
+    f32 = np.float32
+    centres = np.array([-1.0] + [f32(2) * f32(i - 1) / f32(5) - f32(1) for i in range(1, 7)] + [1.0], np.float64)
+    for k in range(8):
+        h = np.zeros(shape + (8,), np.uint16)
+        h[1, 3, 4, k] = 3
+        np.testing.assert_allclose(oracle.fusion_energy(np.zeros(shape), h, 0.7), 0.7 * 3 * abs(centres[k]),
+                                   rtol=1e-15)
+        h[..., k] = 2
+        assert oracle.fusion_energy(np.full(shape, centres[k]), h, 0.7) == 0.0
+
+
 # ---- TVL1 -------------------------------------------------------------------------------------
 
 def test_tvl1_constant_depth_is_fixed():

@@ -73,3 +73,11 @@ def test_gpu_renderer_noise(gpu):
     assert abs(n.mean()) < 2e-3 and abs(n.std() - 0.05) < 2e-3
     np.testing.assert_array_equal(a.to_host(), b.to_host())
     assert not np.array_equal(a.to_host(), c.to_host())
+
+
+@pytest.mark.gpu
+def test_gpu_orbit_depthmaps_identical(gpu):
+    a = scene.orbit_depthmaps(seed=5, n=3, width=48, height=36, noise_sigma=0.01)
+    b = scene.orbit_depthmaps_gpu(seed=5, n=3, width=48, height=36, noise_sigma=0.01)
+    for k in ("K", "R", "t", "depths"):
+        np.testing.assert_array_equal(a[k], b[k])

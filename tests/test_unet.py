@@ -151,10 +151,30 @@ def test_network_forward_matches_numpy(gpu):
     assert err <= 1e-2
 
 
+def test_network_gradient_matches_numpy_backprop(gpu):
+    """Every parameter block's GPU gradient against float64 backpropagation
+    with the same bf16 rounding points (weights, activations, dZ): 1 % in
+    norm (FP32 versus float64 accumulation; dZ may round differently)."""
+    from psw_amd import warmstart
+    from unet_ref import param_blocks, unet_grad
+    B, W, H = 2, 16, 32
+    net, x, t = _tiny_net(4, B, W, H)
+    theta = net.get_params()
+    net.loss_grad(B, W, H)
+    g = net.get_grads().astype(np.float64)
+    ref = unet_grad(theta, x, t, warmstart.CHANNEL_WEIGHTS)
+    for blk, (lo, hi) in enumerate(param_blocks()):
+        e = np.linalg.norm(g[lo:hi] - ref[lo:hi]) / max(np.linalg.norm(ref[lo:hi]), 1e-12)
+        print(f"block {blk}: |g| {np.linalg.norm(ref[lo:hi]):.4e}, rel diff {e:.2e}")
+        assert e <= 1e-2, (blk, e)
+
+
 def test_network_gradient_finite_differences(gpu):
     """The GPU gradient (bf16 network) against central differences of the
-    float64 model without rounding, along random directions in each block;
-    5 % (bf16 rounding of weights and activations moves the gradient)."""
+    float64 model without rounding, along random directions in blocks of the
+    full-resolution levels (head, d0, e0) and of level 2 (d2a): 5 %.  (At
+    the 2x2 bottleneck of this 16x16 input, bf16 versus float64 ReLU masks
+    differ enough that only the rounding-matched backprop test applies.)"""
     from psw_amd import warmstart
     from unet_ref import huber_loss, param_blocks, unet_forward
     B, W, H = 1, 16, 16
@@ -167,7 +187,7 @@ def test_network_gradient_finite_differences(gpu):
     wc = warmstart.CHANNEL_WEIGHTS
     rng = np.random.default_rng(5)
     f = lambda th: huber_loss(unet_forward(th, x, round_bf16=False), t, wc)   # noqa: E731
-    for blk in [len(blocks) - 2, len(blocks) - 1, 26, 27, 16, 14, 7, 1, 0]:
+    for blk in [len(blocks) - 2, len(blocks) - 1, 26, 27, 16, 1, 0]:
         lo, hi = blocks[blk]
         v = np.zeros_like(theta)
         v[lo:hi] = rng.standard_normal(hi - lo)

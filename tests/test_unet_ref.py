@@ -64,3 +64,32 @@ def test_network_model_shapes_and_zero_head():
     out = unet_forward(th, rng.standard_normal((1, 16, 8, 8)), round_bf16=False)
     assert out.shape == (1, 16, 8, 8)
     np.testing.assert_array_equal(out, np.broadcast_to(th[b0:b1], out.shape))
+
+
+def test_network_backprop_matches_finite_differences():
+    """unet_grad on the float64 model without rounding against central
+    differences along a random direction in every parameter block (pins the
+    routing of concatenation, pooling and upsampling gradients)."""
+    from unet_ref import huber_loss, param_blocks, unet_forward, unet_grad
+    rng = np.random.default_rng(2)
+    blocks = param_blocks()
+    n = blocks[-1][1]
+    th = np.zeros(n)
+    for k in range(0, len(blocks) - 2, 2):
+        lo, hi = blocks[k]
+        fan = (hi - lo) / (blocks[k + 1][1] - blocks[k + 1][0]) / 1.0
+        th[lo:hi] = rng.standard_normal(hi - lo) * np.sqrt(2.0 / fan)
+        th[blocks[k + 1][0]:blocks[k + 1][1]] = rng.standard_normal(blocks[k + 1][1] - blocks[k + 1][0]) * 0.05
+    lo, hi = blocks[-2]
+    th[lo:hi] = rng.standard_normal(hi - lo) * 0.3
+    x = rng.uniform(-1, 1, (1, 16, 16, 8))
+    t = rng.uniform(-0.8, 0.8, (1, 16, 16, 8))
+    wc = np.array([1.0, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5])
+    g = unet_grad(th, x, t, wc, round_bf16=False)
+    f = lambda p: huber_loss(unet_forward(p, x, round_bf16=False), t, wc)   # noqa: E731
+    for k, (lo, hi) in enumerate(blocks):
+        v = np.zeros(n)
+        v[lo:hi] = rng.standard_normal(hi - lo)
+        eps = 1e-6
+        fd = (f(th + eps * v) - f(th - eps * v)) / (2 * eps)
+        np.testing.assert_allclose(np.dot(g, v), fd, rtol=1e-4, atol=1e-9, err_msg=f"block {k}")

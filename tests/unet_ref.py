@@ -103,3 +103,59 @@ def huber_loss(out, tgt, wc):
     d = out - tgt
     ad = np.abs(d)
     return float(np.sum(np.where(ad <= 1, 0.5 * d * d, ad - 0.5) * np.asarray(wc, np.float64)) / (d.size // 8))
+
+
+def unet_grad(theta, x, tgt, wc, round_bf16=True):
+    """Gradient of huber_loss(unet_forward(theta, x), tgt) with the GPU's
+    rounding points: bf16 weights, activations and ReLU-masked output
+    gradients dZ (round_bf16=False: none); float64 arithmetic elsewhere.
+    Returns the flat gradient."""
+    rb = (lambda a: bf16_val(bf16_bits(a)).astype(np.float64)) if round_bf16 else (lambda a: np.asarray(a, np.float64))
+    blocks = param_blocks()
+    th = np.asarray(theta, np.float64)
+    act = [rb(np.asarray(x, np.float64))]
+    pools = []
+    inputs = []
+    src_a = [0, 1, -1, 3, -2, 5, -3, 7, 6, 9, 4, 11, 2, 13]
+    src_b = [None] * 8 + [8, None, 10, None, 12, None]
+    for L, (ca, cb, co, _) in enumerate(LAYERS):
+        a = act[src_a[L]] if src_a[L] >= 0 else pools[-src_a[L] - 1]
+        inp = a if src_b[L] is None else np.concatenate([a, up2(act[src_b[L]])], axis=3)
+        inputs.append(inp)
+        (w0, w1), (b0, b1) = blocks[2 * L], blocks[2 * L + 1]
+        act.append(rb(np.maximum(conv_ref(inp, rb(th[w0:w1]).reshape(9, ca + cb, co), th[b0:b1]), 0.0)))
+        if L in (1, 3, 5):
+            y = act[-1]
+            pools.append(rb(0.25 * ((y[:, 0::2, 0::2] + y[:, 0::2, 1::2]) + (y[:, 1::2, 0::2] + y[:, 1::2, 1::2]))))
+    (h0, h1), (c0, c1) = blocks[-2], blocks[-1]
+    Wh = th[h0:h1].reshape(32, 8)
+    out = act[-1] @ Wh + th[c0:c1]
+    n = out.size // 8
+    gout = np.clip(out - tgt, -1, 1) * np.asarray(wc, np.float64) / n
+    grad = np.zeros_like(th)
+    grad[h0:h1] = np.einsum("bhwc,bhwo->co", act[-1], gout).ravel()
+    grad[c0:c1] = gout.sum(axis=(0, 1, 2))
+    gact = [None] * 15
+    gpool = [None] * 3
+    gact[14] = gout @ Wh.T
+    for L in range(13, -1, -1):
+        ca, cb, co, _ = LAYERS[L]
+        dz = rb(np.where(act[L + 1] > 0, gact[L + 1], 0.0))
+        dw, db = wgrad_ref(inputs[L], dz)
+        (w0, w1), (b0, b1) = blocks[2 * L], blocks[2 * L + 1]
+        grad[w0:w1] = dw.ravel()
+        grad[b0:b1] = db
+        if L == 0:
+            break
+        dx = dgrad_ref(dz, rb(th[w0:w1]).reshape(9, ca + cb, co))
+        if cb:
+            gact[src_a[L]] = dx[..., :ca]
+            g = dx[..., ca:]
+            gact[src_b[L]] = (g[:, 0::2, 0::2] + g[:, 0::2, 1::2]) + (g[:, 1::2, 0::2] + g[:, 1::2, 1::2])
+        elif src_a[L] >= 0:
+            gact[src_a[L]] = dx
+        else:
+            pl = -src_a[L] - 1
+            gpool[pl] = dx
+            gact[2 * pl + 2] = gact[2 * pl + 2] + 0.25 * up2(dx)
+    return grad
