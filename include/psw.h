@@ -301,6 +301,16 @@ int psw_tgv_set_primal(psw_tgv* t, const float* d_u, float u_value, const float*
  * state restore): p (2*W*H: px, py), q (3*W*H: qx, qy, qz), r (nsrc*W*H),
  * device arrays; a NULL array sets that dual to 0. */
 int psw_tgv_set_duals(psw_tgv* t, const float* d_p, const float* d_q, const float* d_r, psw_stream_t s);
+/* Pixels counted by psw_tgv_energy / psw_tgv_solve (no reference
+ * counterpart).  A pixel is left out when q = tau sigma lambda^2 sum_j Iu_j^2
+ * exceeds qmax there or at a 4-neighbour: with the reference's fixed step
+ * sizes the primal-dual iteration need not converge at such pixels (it can
+ * oscillate without end), and their energy would hide the convergence of
+ * the rest.  qmax <= 0 counts every pixel (the default, restored by each
+ * psw_tgv_linearise).  excluded (nullable, synchronous when set) receives
+ * the number of pixels left out. */
+int psw_tgv_set_energy_mask(psw_tgv* t, double qmax, const psw_tgv_params* prm, int64_t* excluded,
+                            psw_stream_t s);
 /* Run niters fused primal-dual iterations. */
 int psw_tgv_iterate(psw_tgv* t, int niters, const psw_tgv_params* prm, psw_stream_t s);
 /* FP64 primal energy of the current iterate (synchronous). */

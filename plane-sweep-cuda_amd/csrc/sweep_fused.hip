@@ -58,8 +58,10 @@ __device__ inline float scale_mean(float s, float win, float rwin) {
     else return s / win;
 }
 
+// One work item: plane group g of tile t (item = t * G + g).  Called by all
+// threads of a workgroup with the same item (it contains barriers).
 template <int R, int MODE, bool FAST>
-__global__ __launch_bounds__(SW_THREADS) void sweep_fused_kernel(SweepArgs a) {
+__device__ __forceinline__ void sweep_item(const SweepArgs& a, const int item) {
     constexpr int TW = SW_TX + 2 * R;
     constexpr int TH = SW_TY + 2 * R;
     constexpr int WIN = 2 * R + 1;
@@ -72,8 +74,6 @@ __global__ __launch_bounds__(SW_THREADS) void sweep_fused_kernel(SweepArgs a) {
     __shared__ float sW[TH * TW];
     __shared__ float sC[3][SW_TY * TW];
 
-    if ((int)blockIdx.x >= a.ntiles * a.G) return;
-    const int item = xcd_tile(blockIdx.x, a.ntiles * a.G);
     const int t = item / a.G, g = item - (item / a.G) * a.G;
     const int p0 = (g * a.P) / a.G, p1 = ((g + 1) * a.P) / a.G;
     const int x0 = (t % a.tiles_x) * SW_TX;
@@ -352,6 +352,38 @@ __global__ __launch_bounds__(SW_THREADS) void sweep_fused_kernel(SweepArgs a) {
                 os[i] = csec[k];
             }
         }
+    }
+}
+
+template <int R, int MODE, bool FAST>
+__global__ __launch_bounds__(SW_THREADS) void sweep_fused_kernel(SweepArgs a) {
+    if ((int)blockIdx.x >= a.ntiles * a.G) return;
+    sweep_item<R, MODE, FAST>(a, xcd_tile(blockIdx.x, a.ntiles * a.G));
+}
+
+// Many frames (uneven sizes, plane and source counts) in one persistent
+// launch: workgroups take items from a device counter until all frames'
+// items are done.  Item g belongs to the frame f with off[f] <= g < off[f+1].
+// Every item is computed exactly as in sweep_fused_kernel.
+template <int R, int MODE, bool FAST>
+__global__ __launch_bounds__(SW_THREADS) void sweep_batch_kernel(const SweepArgs* __restrict__ frames,
+                                                                 const int* __restrict__ off, int nframes, int total,
+                                                                 unsigned* __restrict__ counter) {
+    __shared__ int s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int gi = s_item;
+        __syncthreads();  // all threads hold gi before thread 0 takes the next one
+        if (gi >= total) break;
+        int lo = 0, hi = nframes - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (off[mid] <= gi) lo = mid;
+            else hi = mid - 1;
+        }
+        const SweepArgs a = frames[lo];
+        sweep_item<R, MODE, FAST>(a, gi - off[lo]);
     }
 }
 

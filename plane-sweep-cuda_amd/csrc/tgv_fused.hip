@@ -27,7 +27,7 @@ constexpr int TG_THREADS = 256;   // energy kernels
 
 struct TgvLayout {
     size_t n;
-    size_t U, P, Q, r, It, Iu, T, u0, ref, src, total;
+    size_t U, P, Q, r, It, Iu, T, u0, ref, src, mask, total;
     void init(int W, int H, int N) {
         n = (size_t)W * H;
         size_t o = 0;
@@ -41,6 +41,7 @@ struct TgvLayout {
         u0 = o; o += n;
         ref = o; o += n;         // normalised reference
         src = o; o += (size_t)N * n;
+        mask = o; o += n;        // energy weight per pixel (1 = counted)
         total = o;
     }
 };
@@ -209,12 +210,44 @@ __global__ void tgv_set_primal_kernel(float* __restrict__ U, float* __restrict__
     }
 }
 
+// Energy weights: 0 where the data block of the primal-dual step is far from
+// stable -- q = tau sigma lambda^2 sum_j Iu_j^2 above qmax at the pixel or a
+// 4-neighbour (the reference's fixed steps let such pixels oscillate without
+// end) -- and 1 elsewhere.  qmax <= 0 counts every pixel.
+__global__ void tgv_mask_kernel(const float* __restrict__ Iu, int W, int H, int N, double scale, double qmax,
+                                float* __restrict__ mask, unsigned long long* __restrict__ excluded) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    bool keep = true;
+    if (x < W && y < H && qmax > 0.0) {
+        const size_t n = (size_t)W * H;
+        const int xs[5] = {x, x - 1, x + 1, x, x};
+        const int ys[5] = {y, y, y, y - 1, y + 1};
+        for (int k = 0; k < 5; ++k) {
+            if (xs[k] < 0 || xs[k] >= W || ys[k] < 0 || ys[k] >= H) continue;
+            const size_t i = (size_t)ys[k] * W + xs[k];
+            double q = 0.0;
+            for (int j = 0; j < N; ++j) {
+                const double v = Iu[(size_t)j * n + i];
+                q += v * v;
+            }
+            if (q * scale > qmax) keep = false;
+        }
+    }
+    if (x < W && y < H) {
+        mask[(size_t)y * W + x] = keep ? 1.f : 0.f;
+        if (!keep && excluded) atomicAdd(excluded, 1ull);
+    }
+}
+
 // ---- the fused primal-dual iteration --------------------------------------
 
 // Tile TG_TX x TY pixels, NT threads.  Variants (selected by tgv_variant()):
 // 64x8 with 256 or 512 threads, 64x4 with 256 threads.
+// One tile of one iteration; called by every thread of the workgroup (it
+// contains barriers).
 template <int TY, int NT>
-__global__ __launch_bounds__(NT) void tgv_iter_kernel(TgvPtrs a) {
+__device__ __forceinline__ void tgv_iter_tile(const TgvPtrs& a, const int tile) {
     constexpr int TG_TY = TY, TG_THREADS = NT;
     constexpr int HX = TG_TX + 2, HY = TG_TY + 2;   // ubar region: x0-1..x0+TX, y0-1..y0+TY
     constexpr int DX = TG_TX + 1, DY = TG_TY + 1;   // dual region: x0-1..x0+TX-1, y0-1..y0+TY-1
@@ -222,11 +255,9 @@ __global__ __launch_bounds__(NT) void tgv_iter_kernel(TgvPtrs a) {
     __shared__ float sP[2][DY][DX];
     __shared__ float sQ[3][DY][DX];
 
-    if (a.ctrl->done) return;
     const int W = a.W, H = a.H;
     const size_t n = (size_t)W * H;
-    const int tiles_x = (W + TG_TX - 1) / TG_TX, ntiles = tiles_x * ((H + TG_TY - 1) / TG_TY);
-    const int tile = xcd_tile(blockIdx.x, ntiles);
+    const int tiles_x = (W + TG_TX - 1) / TG_TX;
     const int x0 = (tile % tiles_x) * TG_TX, y0 = (tile / tiles_x) * TG_TY;
     const int tid = threadIdx.x;
 
@@ -319,6 +350,13 @@ __global__ __launch_bounds__(NT) void tgv_iter_kernel(TgvPtrs a) {
     }
 }
 
+template <int TY, int NT>
+__global__ __launch_bounds__(NT) void tgv_iter_kernel(TgvPtrs a) {
+    if (a.ctrl->done) return;
+    const int ntiles = ((a.W + TG_TX - 1) / TG_TX) * ((a.H + TY - 1) / TY);
+    tgv_iter_tile<TY, NT>(a, xcd_tile(blockIdx.x, ntiles));
+}
+
 // ---- FP64 energy -------------------------------------------------------------
 
 __device__ inline double block_reduce_sum(double v) {
@@ -336,14 +374,15 @@ __device__ inline double block_reduce_sum(double v) {
 // Per-block partial sums of alpha1 |T(grad u - u1)| + alpha0 |eps(u1)|_F +
 // lambda sum_j |It_j + (u - u0) Iu_j| with the iteration's clamped forward
 // differences.
-__global__ __launch_bounds__(TG_THREADS) void tgv_energy_kernel(const float* __restrict__ U, const float* __restrict__ T,
-                                                             const float* __restrict__ u0, const float* __restrict__ It,
-                                                             const float* __restrict__ Iu, int W, int H, int N,
-                                                             double lambda, double alpha0, double alpha1,
-                                                             double* __restrict__ partials) {
+// Block bid of nblk: grid-stride over the frame's pixels; returns the block
+// sum on thread 0.
+__device__ inline double tgv_energy_block(const float* __restrict__ U, const float* __restrict__ T,
+                                          const float* __restrict__ u0, const float* __restrict__ It,
+                                          const float* __restrict__ Iu, const float* __restrict__ mask, int W, int H,
+                                          int N, double lambda, double alpha0, double alpha1, int bid, int nblk) {
     const size_t n = (size_t)W * H;
     double e = 0.0;
-    for (size_t i = (size_t)blockIdx.x * TG_THREADS + threadIdx.x; i < n; i += (size_t)gridDim.x * TG_THREADS) {
+    for (size_t i = (size_t)bid * TG_THREADS + threadIdx.x; i < n; i += (size_t)nblk * TG_THREADS) {
         const int y = (int)(i / W), x = (int)(i - (size_t)y * W);
         const size_t ix = (size_t)y * W + min(x + 1, W - 1), iy = (size_t)min(y + 1, H - 1) * W + x;
         const double u = U[i], ux = U[ix], uy = U[iy];
@@ -356,9 +395,19 @@ __global__ __launch_bounds__(TG_THREADS) void tgv_energy_kernel(const float* __r
         double ed = 0.0;
         const double du = u - (double)u0[i];
         for (int j = 0; j < N; ++j) ed += fabs((double)It[(size_t)j * n + i] + du * (double)Iu[(size_t)j * n + i]);
-        e += alpha1 * sqrt(p * p + q * q) + alpha0 * sqrt(ex * ex + ey * ey + 2.0 * ez * ez) + lambda * ed;
+        const double ei = alpha1 * sqrt(p * p + q * q) + alpha0 * sqrt(ex * ex + ey * ey + 2.0 * ez * ez) + lambda * ed;
+        if (mask[i] != 0.f) e += ei;
     }
-    const double t = block_reduce_sum(e);
+    return block_reduce_sum(e);
+}
+
+__global__ __launch_bounds__(TG_THREADS) void tgv_energy_kernel(const float* __restrict__ U, const float* __restrict__ T,
+                                                             const float* __restrict__ u0, const float* __restrict__ It,
+                                                             const float* __restrict__ Iu,
+                                                             const float* __restrict__ mask, int W, int H, int N,
+                                                             double lambda, double alpha0, double alpha1,
+                                                             double* __restrict__ partials) {
+    const double t = tgv_energy_block(U, T, u0, It, Iu, mask, W, H, N, lambda, alpha0, alpha1, blockIdx.x, gridDim.x);
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
@@ -436,8 +485,8 @@ static int tgv_launch_energy(psw_tgv* t, const psw_tgv_params* prm, double tol, 
     int blocks = (int)((L.n + TG_THREADS - 1) / TG_THREADS);
     if (blocks > t->npartials) blocks = t->npartials;
     tgv_energy_kernel<<<blocks, TG_THREADS, 0, st>>>(U, t->base + L.T, t->base + L.u0, t->base + L.It, t->base + L.Iu,
-                                                     t->W, t->H, t->N, prm->lambda, prm->alpha0, prm->alpha1,
-                                                     t->partials);
+                                                     t->base + L.mask, t->W, t->H, t->N, prm->lambda, prm->alpha0,
+                                                     prm->alpha1, t->partials);
     int s = launch_status("tgv_energy_kernel");
     if (s != PSW_OK) return s;
     tgv_energy_finalize_kernel<<<1, TG_THREADS, 0, st>>>(t->partials, blocks, t->ctrl, t->k, tol, e_star);
@@ -489,6 +538,12 @@ int psw_tgv_create(psw_tgv** out, int width, int height, int nsrc) {
         int s = hip_status(e, "psw_tgv_create", PSW_ERR_ALLOC);
         psw_tgv_destroy(t);
         return s;
+    }
+    int fs = psw_fill_f32(t->base + t->L.mask, 1.f, t->L.n, nullptr);
+    if (fs == PSW_OK) fs = hipDeviceSynchronize() == hipSuccess ? PSW_OK : PSW_ERR_DEVICE;
+    if (fs != PSW_OK) {
+        psw_tgv_destroy(t);
+        return fs;
     }
     *out = t;
     return PSW_OK;
@@ -567,6 +622,7 @@ int psw_tgv_linearise(psw_tgv* t, const float* K, const float* Rrel, const float
                                                 t->base + L.Iu, t->base + L.r, t->W, t->H, t->N);
     st = launch_status("tgv_linearise_kernel");
     if (st != PSW_OK) return st;
+    if ((st = psw_fill_f32(t->base + L.mask, 1.f, n, s)) != PSW_OK) return st;
     return tgv_reset_ctrl(t, S(s));
 }
 
@@ -602,6 +658,32 @@ int psw_tgv_set_duals(psw_tgv* t, const float* d_p, const float* d_q, const floa
     else st = psw_fill_f32(R, 0.f, (size_t)t->N * n, s);
     if (st != PSW_OK) return st;
     return tgv_reset_ctrl(t, S(s));
+}
+
+int psw_tgv_set_energy_mask(psw_tgv* t, double qmax, const psw_tgv_params* prm, int64_t* excluded,
+                            psw_stream_t s) {
+    PSW_CHECK_ARG(t && prm, "NULL pointer");
+    const TgvLayout& L = t->L;
+    unsigned long long* d = nullptr;
+    if (excluded) {
+        PSW_TRY_HIP(hipMalloc((void**)&d, sizeof(unsigned long long)), PSW_ERR_ALLOC);
+        PSW_TRY_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), S(s)), PSW_ERR_DEVICE);
+    }
+    const dim3 blk(64, 4), grd((unsigned)((t->W + 63) / 64), (unsigned)((t->H + 3) / 4));
+    const double scale = prm->tau * prm->sigma * prm->lambda * prm->lambda;
+    tgv_mask_kernel<<<grd, blk, 0, S(s)>>>(t->base + L.Iu, t->W, t->H, t->N, scale, qmax, t->base + L.mask, d);
+    int st = launch_status("tgv_mask_kernel");
+    if (excluded) {
+        unsigned long long h = 0;
+        if (st == PSW_OK) {
+            hipError_t e = hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, S(s));
+            if (e == hipSuccess) e = hipStreamSynchronize(S(s));
+            if (e != hipSuccess) st = hip_status(e, "psw_tgv_set_energy_mask");
+        }
+        (void)hipFree(d);
+        *excluded = (int64_t)h;
+    }
+    return st;
 }
 
 int psw_tgv_iterate(psw_tgv* t, int niters, const psw_tgv_params* prm, psw_stream_t s) {

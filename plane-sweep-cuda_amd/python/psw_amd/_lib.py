@@ -145,6 +145,7 @@ _SIGS = {
     "psw_tgv_set_u": (c_int, [c_void_p, D, c_float, c_void_p]),
     "psw_tgv_set_primal": (c_int, [c_void_p, D, c_float, D, c_void_p]),
     "psw_tgv_set_duals": (c_int, [c_void_p, D, D, D, c_void_p]),
+    "psw_tgv_set_energy_mask": (c_int, [c_void_p, c_double, P(TgvParams), P(ctypes.c_int64), c_void_p]),
     "psw_tgv_linearise": (c_int, [c_void_p, H, H, H, c_void_p]),
     "psw_tgv_iterate": (c_int, [c_void_p, c_int, P(TgvParams), c_void_p]),
     "psw_tgv_energy": (c_int, [c_void_p, P(TgvParams), P(c_double), c_void_p]),

@@ -178,6 +178,15 @@ class TgvSolver:
         """Load p (2,H,W), q (3,H,W), r (N,H,W) device arrays; None means 0."""
         call("psw_tgv_set_duals", self.h, *[d.ptr if d is not None else None for d in (d_p, d_q, d_r)], stream)
 
+    def set_energy_mask(self, qmax, prm, stream=None, count=True):
+        """Leave out of the energy the pixels whose data step is unstable
+        (q = tau sigma lambda^2 sum Iu^2 > qmax at them or a 4-neighbour);
+        returns how many were left out (synchronous) when count is set."""
+        c = ctypes.c_int64()
+        call("psw_tgv_set_energy_mask", self.h, float(qmax), ctypes.byref(prm), ctypes.byref(c) if count else None,
+             stream)
+        return c.value if count else None
+
     def iterate(self, niters, prm, stream=None):
         call("psw_tgv_iterate", self.h, int(niters), ctypes.byref(prm), stream)
 
