@@ -26,7 +26,7 @@ from ._lib import call, lib
 from .device import DeviceArray, Stream
 
 # loss weights of the 8 output channels: depth correction, u1 (2), p (2), q (3)
-CHANNEL_WEIGHTS = np.array([1.0, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5], np.float32)
+CHANNEL_WEIGHTS = np.array([4.0, 1.0, 1.0, 2.0, 2.0, 1.0, 1.0, 1.0], np.float32)
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "..", "weights",
                                "unet_warmstart.npz")
 
@@ -203,7 +203,7 @@ class Dataset:
         net.targets(slot, self.cw, self.ch, it["state"], s)
 
 
-def train(n_scenes=600, steps=3000, batch=8, crop=(256, 256), lr=2e-3, tgv_iters=40000, seed0=100000, log=print,
+def train(n_scenes=1000, steps=20000, batch=8, crop=(256, 256), lr=1e-3, tgv_iters=20000, seed0=100000, log=print,
           out=None, init_seed=1):
     st = Stream()
     s = st.handle
@@ -215,7 +215,7 @@ def train(n_scenes=600, steps=3000, batch=8, crop=(256, 256), lr=2e-3, tgv_iters
         x0 = int(rng.integers(0, (FULL_W - cw) // 8 + 1)) * 8
         y0 = int(rng.integers(0, (FULL_H - ch) // 8 + 1)) * 8
         ds.add_scene(seed0 + i, x0, y0)
-        if (i + 1) % 50 == 0:
+        if (i + 1) % 100 == 0:
             st.synchronize()
             log(f"[data] {i + 1}/{n_scenes} scenes in {time.time() - t0:.1f} s")
     st.synchronize()
@@ -229,7 +229,7 @@ def train(n_scenes=600, steps=3000, batch=8, crop=(256, 256), lr=2e-3, tgv_iters
         idx = rng.integers(0, n_tr, batch)
         for b, i in enumerate(idx):
             ds.load_slot(net, b, int(i))
-        want = step % 100 == 0 or step == 1
+        want = step % 1000 == 0 or step == 1
         loss = net.loss_grad(batch, cw, ch, want_loss=want, stream=s)
         cur_lr = lr * 0.5 * (1 + np.cos(np.pi * (step - 1) / steps))
         net.adam_step(cur_lr, step, stream=s)
@@ -255,12 +255,12 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
     t = sub.add_parser("train")
-    t.add_argument("--scenes", type=int, default=600)
-    t.add_argument("--steps", type=int, default=3000)
+    t.add_argument("--scenes", type=int, default=1000)
+    t.add_argument("--steps", type=int, default=20000)
     t.add_argument("--batch", type=int, default=8)
     t.add_argument("--crop", type=int, default=256)
-    t.add_argument("--lr", type=float, default=2e-3)
-    t.add_argument("--tgv-iters", type=int, default=40000)
+    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--tgv-iters", type=int, default=20000)
     t.add_argument("--out", default=DEFAULT_WEIGHTS)
     a = ap.parse_args(argv)
     if a.cmd == "train":
