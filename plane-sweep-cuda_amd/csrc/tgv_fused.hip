@@ -31,10 +31,10 @@ struct TgvLayout {
     void init(int W, int H, int N) {
         n = (size_t)W * H;
         size_t o = 0;
-        U = o;  o += 9 * n;      // [3 buffers][u, u1x, u1y]
-        P = o;  o += 4 * n;      // [2 buffers][px, py]
-        Q = o;  o += 6 * n;      // [2 buffers][qx, qy, qz]
-        r = o;  o += (size_t)N * n;
+        U = o;  o += 12 * n;     // [4 slots][u, u1x, u1y]: u_k in slot k % 4
+        P = o;  o += 6 * n;      // [3 slots][px, py]: p_k in slot k % 3
+        Q = o;  o += 9 * n;      // [3 slots][qx, qy, qz]
+        r = o;  o += (size_t)3 * N * n;  // [3 slots][N][n]
         It = o; o += (size_t)N * n;
         Iu = o; o += (size_t)N * n;
         T = o;  o += 3 * n;      // T11, T12 (= T21), T22
@@ -78,15 +78,24 @@ struct TgvPoses {
     float fx, fy;
 };
 
+// Slots by iteration count k (so that launches doing one or two iterations
+// mix freely and a no-op launch after the stop leaves the mapping intact):
+// u_k in U slot k % 4, p_k / q_k / r_k in slot k % 3.  A launch reads the
+// slots of k (and k-1 for u) and writes those of k+1 (and k+2).
 struct TgvPtrs {
     const float* ucur;   // [3][n] u_k
     const float* uprev;  // [3][n] u_{k-1}
     float* unext;        // [3][n] u_{k+1}
+    float* unext2;       // [3][n] u_{k+2}
     const float* pcur;   // [2][n]
     float* pnext;
+    float* pnext2;
     const float* qcur;   // [3][n]
     float* qnext;
-    float* r;            // [N][n]
+    float* qnext2;
+    const float* rcur;   // [N][n]
+    float* rnext;
+    float* rnext2;
     const float* It;
     const float* Iu;
     const float* T;      // [3][n]
@@ -242,6 +251,52 @@ __global__ void tgv_mask_kernel(const float* __restrict__ Iu, int W, int H, int 
 
 // ---- the fused primal-dual iteration --------------------------------------
 
+// Point updates shared by the one- and two-iteration kernels (same
+// expression order, so both give identical results).
+// TGV2_updateP_tensor_weighed + TGV2_updateQ (q_w == q_z) at one position:
+// ub*/u1xb*/u1yb* are ubar at the position (c), its right (r) and lower (d)
+// neighbour (the position itself at the image's last column / row).
+__device__ __forceinline__ void pq_update(float& px, float& py, float& qx, float& qy, float& qz, float t11, float t12,
+                                          float t22, float ubc, float ubr, float ubd, float u1xc, float u1xr,
+                                          float u1xd, float u1yc, float u1yr, float u1yd, float as1, float as0) {
+    const float gxv = ubr - ubc - u1xc;
+    const float gyv = ubd - ubc - u1yc;
+    px = px + as1 * (t11 * gxv + t12 * gyv);
+    py = py + as1 * (t12 * gxv + t22 * gyv);
+    const float dp = fmaxf(1.f, sqrtf(px * px + py * py));
+    px = px / dp;
+    py = py / dp;
+    const float dx_u1x = u1xr - u1xc;
+    const float dy_u1x = u1xd - u1xc;
+    const float dx_u1y = u1yr - u1yc;
+    const float dy_u1y = u1yd - u1yc;
+    qx = qx + as0 * dx_u1x;
+    qy = qy + as0 * dy_u1y;
+    qz = qz + as0 * (dy_u1x + dx_u1y) / 2.0f;
+    const float dq = fmaxf(1.f, sqrtf(qx * qx + qy * qy + qz * qz + qz * qz));
+    qx = qx / dq;
+    qy = qy / dq;
+    qz = qz / dq;
+}
+
+// TGV2_updateR at one pixel and source (uses u_k, not ubar).
+__device__ __forceinline__ float r_update(float r, float it, float du, float iu, float sl) {
+    const float rr = r + sl * (it + du * iu);
+    return rr / fmaxf(1.f, fabsf(rr));
+}
+
+// TGV2_updateU_tensor_weighed at one pixel: c = own duals, l = left
+// neighbour's, u = upper neighbour's (own at the first column / row).
+__device__ __forceinline__ void u_update(float u, float u1x, float u1y, float t11, float t12, float t22, float c_px,
+                                         float c_py, float l_px, float l_py, float u_px, float u_py, float q0c,
+                                         float q0l, float q1c, float q1u, float q2c, float q2l, float q2u, float ps,
+                                         const TgvPtrs& a, float& un, float& u1xn, float& u1yn) {
+    un = u - a.tau * (-a.alpha1 * (t11 * (c_px - l_px) + t12 * (c_py - l_py) + t12 * (c_px - u_px) + t22 * (c_py - u_py)) +
+                      a.lambda * ps);
+    u1xn = u1x - a.tau * (-a.alpha1 * (t11 * c_px + t12 * c_py) - a.alpha0 * (q0c - q0l + q2c - q2u));
+    u1yn = u1y - a.tau * (-a.alpha1 * (t12 * c_px + t22 * c_py) - a.alpha0 * (q2c - q2l + q1c - q1u));
+}
+
 // Tile TG_TX x TY pixels, NT threads.  Variants (selected by tgv_variant()):
 // 64x8 with 256 or 512 threads, 64x4 with 256 threads.
 // One tile of one iteration; called by every thread of the workgroup (it
@@ -281,28 +336,12 @@ __device__ __forceinline__ void tgv_iter_tile(const TgvPtrs& a, const int tile) 
         const int gx = min(max(vx, 0), W - 1), gy = min(max(vy, 0), H - 1);
         const size_t g = (size_t)gy * W + gx;
         const int hx = dx, hy = dy;  // ubar index of this position (+1 -> right / down neighbour)
-        const float ub = sUb[0][hy][hx], u1xb = sUb[1][hy][hx], u1yb = sUb[2][hy][hx];
-        // TGV2_updateP_tensor_weighed
         const float t11 = a.T[g], t12 = a.T[n + g], t22 = a.T[2 * n + g];
-        const float gxv = sUb[0][hy][hx + 1] - ub - u1xb;
-        const float gyv = sUb[0][hy + 1][hx] - ub - u1yb;
-        float px = a.pcur[g] + as1 * (t11 * gxv + t12 * gyv);
-        float py = a.pcur[n + g] + as1 * (t12 * gxv + t22 * gyv);
-        const float dp = fmaxf(1.f, sqrtf(px * px + py * py));
-        px = px / dp;
-        py = py / dp;
-        // TGV2_updateQ (q_w == q_z)
-        const float dx_u1x = sUb[1][hy][hx + 1] - u1xb;
-        const float dy_u1x = sUb[1][hy + 1][hx] - u1xb;
-        const float dx_u1y = sUb[2][hy][hx + 1] - u1yb;
-        const float dy_u1y = sUb[2][hy + 1][hx] - u1yb;
-        float qx = a.qcur[g] + as0 * dx_u1x;
-        float qy = a.qcur[n + g] + as0 * dy_u1y;
-        float qz = a.qcur[2 * n + g] + as0 * (dy_u1x + dx_u1y) / 2.0f;
-        const float dq = fmaxf(1.f, sqrtf(qx * qx + qy * qy + qz * qz + qz * qz));
-        qx = qx / dq;
-        qy = qy / dq;
-        qz = qz / dq;
+        float px = a.pcur[g], py = a.pcur[n + g];
+        float qx = a.qcur[g], qy = a.qcur[n + g], qz = a.qcur[2 * n + g];
+        pq_update(px, py, qx, qy, qz, t11, t12, t22, sUb[0][hy][hx], sUb[0][hy][hx + 1], sUb[0][hy + 1][hx],
+                  sUb[1][hy][hx], sUb[1][hy][hx + 1], sUb[1][hy + 1][hx], sUb[2][hy][hx], sUb[2][hy][hx + 1],
+                  sUb[2][hy + 1][hx], as1, as0);
         sP[0][dy][dx] = px; sP[1][dy][dx] = py;
         sQ[0][dy][dx] = qx; sQ[1][dy][dx] = qy; sQ[2][dy][dx] = qz;
         if (dx > 0 && dy > 0 && vx < W && vy < H) {
@@ -325,9 +364,8 @@ __device__ __forceinline__ void tgv_iter_tile(const TgvPtrs& a, const int tile) 
         for (int j = 0; j < a.N; ++j) {
             const size_t gj = (size_t)j * n + g;
             const float iu = a.Iu[gj];
-            float rr = a.r[gj] + sl * (a.It[gj] + du * iu);
-            rr = rr / fmaxf(1.f, fabsf(rr));
-            a.r[gj] = rr;
+            const float rr = r_update(a.rcur[gj], a.It[gj], du, iu, sl);
+            a.rnext[gj] = rr;
             ps += rr * iu;
         }
         const int cx = tx + 1, cy = ty + 1;              // dual index of this pixel
@@ -337,17 +375,182 @@ __device__ __forceinline__ void tgv_iter_tile(const TgvPtrs& a, const int tile) 
         const float c_px = sP[0][cy][cx], c_py = sP[1][cy][cx];
         const float xp_px = sP[0][cy][lx], xp_py = sP[1][cy][lx];
         const float yp_px = sP[0][ty_][cx], yp_py = sP[1][ty_][cx];
-        const float un = u - a.tau * (-a.alpha1 * (t11 * (c_px - xp_px) + t12 * (c_py - xp_py) +
-                                                   t12 * (c_px - yp_px) + t22 * (c_py - yp_py)) +
-                                      a.lambda * ps);
-        const float u1xn = u1x - a.tau * (-a.alpha1 * (t11 * c_px + t12 * c_py) -
-                                          a.alpha0 * (sQ[0][cy][cx] - sQ[0][cy][lx] + sQ[2][cy][cx] - sQ[2][ty_][cx]));
-        const float u1yn = u1y - a.tau * (-a.alpha1 * (t12 * c_px + t22 * c_py) -
-                                          a.alpha0 * (sQ[2][cy][cx] - sQ[2][cy][lx] + sQ[1][cy][cx] - sQ[1][ty_][cx]));
+        float un, u1xn, u1yn;
+        u_update(u, u1x, u1y, t11, t12, t22, c_px, c_py, xp_px, xp_py, yp_px, yp_py, sQ[0][cy][cx], sQ[0][cy][lx],
+                 sQ[1][cy][cx], sQ[1][ty_][cx], sQ[2][cy][cx], sQ[2][cy][lx], sQ[2][ty_][cx], ps, a, un, u1xn, u1yn);
         a.unext[g] = un;
         a.unext[n + g] = u1xn;
         a.unext[2 * n + g] = u1yn;
     }
+}
+
+// Two iterations (k -> k+1 -> k+2) of a 64x8 tile in one launch.  Each
+// iteration widens the dependence by one pixel, so iteration k+1 runs on the
+// tile plus a 1-pixel ring (computing p, q on a 2-pixel ring and reading
+// ubar_k on a 2-pixel ring plus the right / lower neighbours), entirely in
+// LDS; only the tile's u_{k+1}, u_{k+2}, p_{k+2}, q_{k+2}, r_{k+2} reach HBM.
+// Regions (global offsets from the tile origin x0, y0; positions outside the
+// image read clamped coordinates and are never used by pixels inside it,
+// because left/up neighbours at the first column/row and right/down
+// neighbours at the last are the position itself, as in the reference):
+//   A  ubar_k          x0-2 .. x0+65, y0-2 .. y0+9    (68 x 12)
+//   B  p, q_{k+1}      x0-2 .. x0+64, y0-2 .. y0+8    (67 x 11)
+//   C  u_{k+1}, u_k    x0-1 .. x0+64, y0-1 .. y0+8    (66 x 10)
+//   D  p, q_{k+2}      x0-1 .. x0+63, y0-1 .. y0+7    (65 x  9)
+//   tile u, r_{k+2}    x0   .. x0+63, y0   .. y0+7    (64 x  8)
+constexpr int T2_TY = 8, T2_THREADS = 512, T2_MAXN = 16;
+
+__device__ __forceinline__ int clampi(int v, int hi) { return min(max(v, 0), hi); }
+
+__device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile) {
+    constexpr int TX = TG_TX, TY = T2_TY, NT = T2_THREADS;
+    constexpr int AX = TX + 4, AY = TY + 4;
+    constexpr int BX = TX + 3, BY = TY + 3;
+    constexpr int CX = TX + 2, CY = TY + 2;
+    constexpr int DX = TX + 1, DY = TY + 1;
+    __shared__ float sA[3][AY][AX];   // ubar_k on A; then ubar_{k+1} on C (A index = C index + 1)
+    __shared__ float sUk[3][CY][CX];  // u_k on C
+    __shared__ float sU1[3][CY][CX];  // u_{k+1} on C
+    __shared__ float sP[2][BY][BX];   // p_{k+1} on B; then p_{k+2} on D in place (B index = D index + 1)
+    __shared__ float sQ[3][BY][BX];
+    __shared__ float sR[T2_MAXN][TY][TX];  // r_{k+1} of the tile pixels
+
+    const int W = a.W, H = a.H;
+    const size_t n = (size_t)W * H;
+    const int tiles_x = (W + TX - 1) / TX;
+    const int x0 = (tile % tiles_x) * TX, y0 = (tile / tiles_x) * TY;
+    const int tid = threadIdx.x;
+    const float as1 = a.alpha1 * a.sigma, as0 = a.alpha0 * a.sigma;
+    const float sl = a.sigma * a.lambda;
+
+    // A: ubar_k; C: u_k
+    for (int i = tid; i < AX * AY; i += NT) {
+        const int iy = i / AX, ix = i - iy * AX;
+        const size_t g = (size_t)clampi(y0 - 2 + iy, H - 1) * W + clampi(x0 - 2 + ix, W - 1);
+#pragma unroll
+        for (int f = 0; f < 3; ++f) sA[f][iy][ix] = 2.f * a.ucur[f * n + g] - a.uprev[f * n + g];
+    }
+    for (int i = tid; i < CX * CY; i += NT) {
+        const int iy = i / CX, ix = i - iy * CX;
+        const size_t g = (size_t)clampi(y0 - 1 + iy, H - 1) * W + clampi(x0 - 1 + ix, W - 1);
+#pragma unroll
+        for (int f = 0; f < 3; ++f) sUk[f][iy][ix] = a.ucur[f * n + g];
+    }
+    __syncthreads();
+
+    // B: p, q of iteration k+1
+    for (int i = tid; i < BX * BY; i += NT) {
+        const int iy = i / BX, ix = i - iy * BX;
+        const int gx = x0 - 2 + ix, gy = y0 - 2 + iy;
+        const size_t g = (size_t)clampi(gy, H - 1) * W + clampi(gx, W - 1);
+        const int rx = gx < W - 1 ? ix + 1 : ix, dy = gy < H - 1 ? iy + 1 : iy;
+        float px = a.pcur[g], py = a.pcur[n + g];
+        float qx = a.qcur[g], qy = a.qcur[n + g], qz = a.qcur[2 * n + g];
+        pq_update(px, py, qx, qy, qz, a.T[g], a.T[n + g], a.T[2 * n + g], sA[0][iy][ix], sA[0][iy][rx], sA[0][dy][ix],
+                  sA[1][iy][ix], sA[1][iy][rx], sA[1][dy][ix], sA[2][iy][ix], sA[2][iy][rx], sA[2][dy][ix], as1, as0);
+        sP[0][iy][ix] = px; sP[1][iy][ix] = py;
+        sQ[0][iy][ix] = qx; sQ[1][iy][ix] = qy; sQ[2][iy][ix] = qz;
+    }
+    __syncthreads();
+
+    // C: r and u of iteration k+1 (tile pixels first, one per thread, so
+    // that each thread keeps its pixel's r_{k+1} in sR; then the ring)
+    for (int i = tid; i < CX * CY; i += NT) {
+        int ix, iy;
+        if (i < TX * TY) {
+            iy = i / TX + 1;
+            ix = i - (i / TX) * TX + 1;
+        } else {
+            const int k = i - TX * TY;            // ring: top row, bottom row, left column, right column
+            if (k < CX) { iy = 0; ix = k; }
+            else if (k < 2 * CX) { iy = CY - 1; ix = k - CX; }
+            else if (k < 2 * CX + TY) { iy = k - 2 * CX + 1; ix = 0; }
+            else { iy = k - 2 * CX - TY + 1; ix = CX - 1; }
+        }
+        const int gx = x0 - 1 + ix, gy = y0 - 1 + iy;
+        const size_t g = (size_t)clampi(gy, H - 1) * W + clampi(gx, W - 1);
+        const float u = sUk[0][iy][ix], u1x = sUk[1][iy][ix], u1y = sUk[2][iy][ix];
+        const float du = u - a.u0[g];
+        const bool own = i < TX * TY;
+        float ps = 0.f;
+        for (int j = 0; j < a.N; ++j) {
+            const size_t gj = (size_t)j * n + g;
+            const float iu = a.Iu[gj];
+            const float rr = r_update(a.rcur[gj], a.It[gj], du, iu, sl);
+            if (own) sR[j][iy - 1][ix - 1] = rr;
+            ps += rr * iu;
+        }
+        const int bx = ix + 1, by = iy + 1;               // B index of this position
+        const int lx = gx > 0 ? bx - 1 : bx, uy = gy > 0 ? by - 1 : by;
+        float un, u1xn, u1yn;
+        u_update(u, u1x, u1y, a.T[g], a.T[n + g], a.T[2 * n + g], sP[0][by][bx], sP[1][by][bx], sP[0][by][lx],
+                 sP[1][by][lx], sP[0][uy][bx], sP[1][uy][bx], sQ[0][by][bx], sQ[0][by][lx], sQ[1][by][bx],
+                 sQ[1][uy][bx], sQ[2][by][bx], sQ[2][by][lx], sQ[2][uy][bx], ps, a, un, u1xn, u1yn);
+        sU1[0][iy][ix] = un; sU1[1][iy][ix] = u1xn; sU1[2][iy][ix] = u1yn;
+    }
+    __syncthreads();
+    // ubar_{k+1} on C, into sA (ubar_k is no longer read)
+    for (int i = tid; i < CX * CY; i += NT) {
+        const int iy = i / CX, ix = i - iy * CX;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) sA[f][iy + 1][ix + 1] = 2.f * sU1[f][iy][ix] - sUk[f][iy][ix];
+    }
+    __syncthreads();
+
+    // D: p, q of iteration k+2 (in place over p, q_{k+1} at the same positions)
+    for (int i = tid; i < DX * DY; i += NT) {
+        const int iy = i / DX, ix = i - iy * DX;
+        const int gx = x0 - 1 + ix, gy = y0 - 1 + iy;
+        const size_t g = (size_t)clampi(gy, H - 1) * W + clampi(gx, W - 1);
+        const int ax = ix + 1, ay = iy + 1;               // A index (ubar_{k+1})
+        const int rx = gx < W - 1 ? ax + 1 : ax, dy = gy < H - 1 ? ay + 1 : ay;
+        const int bx = ix + 1, by = iy + 1;               // B index
+        float px = sP[0][by][bx], py = sP[1][by][bx];
+        float qx = sQ[0][by][bx], qy = sQ[1][by][bx], qz = sQ[2][by][bx];
+        pq_update(px, py, qx, qy, qz, a.T[g], a.T[n + g], a.T[2 * n + g], sA[0][ay][ax], sA[0][ay][rx], sA[0][dy][ax],
+                  sA[1][ay][ax], sA[1][ay][rx], sA[1][dy][ax], sA[2][ay][ax], sA[2][ay][rx], sA[2][dy][ax], as1, as0);
+        sP[0][by][bx] = px; sP[1][by][bx] = py;
+        sQ[0][by][bx] = qx; sQ[1][by][bx] = qy; sQ[2][by][bx] = qz;
+        if (ix > 0 && iy > 0 && gx < W && gy < H) {
+            a.pnext2[g] = px; a.pnext2[n + g] = py;
+            a.qnext2[g] = qx; a.qnext2[n + g] = qy; a.qnext2[2 * n + g] = qz;
+        }
+    }
+    __syncthreads();
+
+    // tile: r and u of iteration k+2
+    {
+        const int ty = tid / TX, tx = tid - (tid / TX) * TX;
+        const int x = x0 + tx, y = y0 + ty;
+        if (x < W && y < H) {
+            const size_t g = (size_t)y * W + x;
+            const int cx = tx + 1, cy = ty + 1;           // C index
+            const float u = sU1[0][cy][cx], u1x = sU1[1][cy][cx], u1y = sU1[2][cy][cx];
+            const float du = u - a.u0[g];
+            float ps = 0.f;
+            for (int j = 0; j < a.N; ++j) {
+                const size_t gj = (size_t)j * n + g;
+                const float iu = a.Iu[gj];
+                const float rr = r_update(sR[j][ty][tx], a.It[gj], du, iu, sl);
+                a.rnext2[gj] = rr;
+                ps += rr * iu;
+            }
+            const int bx = tx + 2, by = ty + 2;           // B index of this pixel (D index + 1)
+            const int lx = x > 0 ? bx - 1 : bx, uy = y > 0 ? by - 1 : by;
+            float un, u1xn, u1yn;
+            u_update(u, u1x, u1y, a.T[g], a.T[n + g], a.T[2 * n + g], sP[0][by][bx], sP[1][by][bx], sP[0][by][lx],
+                     sP[1][by][lx], sP[0][uy][bx], sP[1][uy][bx], sQ[0][by][bx], sQ[0][by][lx], sQ[1][by][bx],
+                     sQ[1][uy][bx], sQ[2][by][bx], sQ[2][by][lx], sQ[2][uy][bx], ps, a, un, u1xn, u1yn);
+            a.unext2[g] = un; a.unext2[n + g] = u1xn; a.unext2[2 * n + g] = u1yn;
+            a.unext[g] = u; a.unext[n + g] = u1x; a.unext[2 * n + g] = u1y;
+        }
+    }
+}
+
+__global__ __launch_bounds__(T2_THREADS) void tgv_iter2_kernel(TgvPtrs a) {
+    if (a.ctrl->done) return;
+    const int ntiles = ((a.W + TG_TX - 1) / TG_TX) * ((a.H + T2_TY - 1) / T2_TY);
+    tgv_iter2_tile(a, xcd_tile(blockIdx.x, ntiles));
 }
 
 template <int TY, int NT>
@@ -433,17 +636,22 @@ static inline TgvPtrs tgv_ptrs(psw_tgv* t, const psw_tgv_params* prm) {
     const TgvLayout& L = t->L;
     float* B = t->base;
     const size_t n = L.n;
-    const int cur = t->k % 3, prev = (t->k + 2) % 3, next = (t->k + 1) % 3;
-    const int pc = t->k % 2, pn = (t->k + 1) % 2;
+    const int k = t->k;
+    const size_t rn = (size_t)t->N * n;
     TgvPtrs p;
-    p.ucur = B + L.U + (size_t)cur * 3 * n;
-    p.uprev = B + L.U + (size_t)prev * 3 * n;
-    p.unext = B + L.U + (size_t)next * 3 * n;
-    p.pcur = B + L.P + (size_t)pc * 2 * n;
-    p.pnext = B + L.P + (size_t)pn * 2 * n;
-    p.qcur = B + L.Q + (size_t)pc * 3 * n;
-    p.qnext = B + L.Q + (size_t)pn * 3 * n;
-    p.r = B + L.r;
+    p.ucur = B + L.U + (size_t)(k % 4) * 3 * n;
+    p.uprev = B + L.U + (size_t)((k + 3) % 4) * 3 * n;
+    p.unext = B + L.U + (size_t)((k + 1) % 4) * 3 * n;
+    p.unext2 = B + L.U + (size_t)((k + 2) % 4) * 3 * n;
+    p.pcur = B + L.P + (size_t)(k % 3) * 2 * n;
+    p.pnext = B + L.P + (size_t)((k + 1) % 3) * 2 * n;
+    p.pnext2 = B + L.P + (size_t)((k + 2) % 3) * 2 * n;
+    p.qcur = B + L.Q + (size_t)(k % 3) * 3 * n;
+    p.qnext = B + L.Q + (size_t)((k + 1) % 3) * 3 * n;
+    p.qnext2 = B + L.Q + (size_t)((k + 2) % 3) * 3 * n;
+    p.rcur = B + L.r + (size_t)(k % 3) * rn;
+    p.rnext = B + L.r + (size_t)((k + 1) % 3) * rn;
+    p.rnext2 = B + L.r + (size_t)((k + 2) % 3) * rn;
     p.It = B + L.It;
     p.Iu = B + L.Iu;
     p.T = B + L.T;
@@ -463,10 +671,30 @@ static int tgv_variant() {
     return (x < 0 || x > 2) ? 1 : x;
 }
 
+// Two iterations per launch (tgv_iter2_kernel) when PSW_TGV_DOUBLE=1 and the
+// frame has at most T2_MAXN sources; identical results either way.  Off by
+// default: on MI355X at 640x480 x 8 sources it moves 1.6x fewer bytes per
+// iteration but runs 16.5 us per iteration against 13.2 us for the
+// one-iteration kernel (73 KB of LDS per workgroup leaves two workgroups per
+// CU, too few to hide the latency of its dependent phases).
+static bool tgv_double() {
+    const char* e = getenv("PSW_TGV_DOUBLE");
+    return e && atoi(e) == 1;
+}
+
 static int tgv_launch_iters(psw_tgv* t, int niters, const psw_tgv_params* prm, hipStream_t st) {
     const int var = tgv_variant();
     const int ty = var == 2 ? 4 : 8;
     const dim3 grid((unsigned)(((t->W + TG_TX - 1) / TG_TX) * ((t->H + ty - 1) / ty)));
+    if (tgv_double() && t->N <= T2_MAXN) {
+        const dim3 grid2((unsigned)(((t->W + TG_TX - 1) / TG_TX) * ((t->H + T2_TY - 1) / T2_TY)));
+        for (; niters >= 2; niters -= 2) {
+            tgv_iter2_kernel<<<grid2, T2_THREADS, 0, st>>>(tgv_ptrs(t, prm));
+            int s = launch_status("tgv_iter2_kernel");
+            if (s != PSW_OK) return s;
+            t->k += 2;
+        }
+    }
     for (int i = 0; i < niters; ++i) {
         const TgvPtrs p = tgv_ptrs(t, prm);
         if (var == 0) tgv_iter_kernel<8, 256><<<grid, 256, 0, st>>>(p);
@@ -481,7 +709,7 @@ static int tgv_launch_iters(psw_tgv* t, int niters, const psw_tgv_params* prm, h
 
 static int tgv_launch_energy(psw_tgv* t, const psw_tgv_params* prm, double tol, double e_star, hipStream_t st) {
     const TgvLayout& L = t->L;
-    const float* U = t->base + L.U + (size_t)(t->k % 3) * 3 * L.n;
+    const float* U = t->base + L.U + (size_t)(t->k % 4) * 3 * L.n;
     int blocks = (int)((L.n + TG_THREADS - 1) / TG_THREADS);
     if (blocks > t->npartials) blocks = t->npartials;
     tgv_energy_kernel<<<blocks, TG_THREADS, 0, st>>>(U, t->base + L.T, t->base + L.u0, t->base + L.It, t->base + L.Iu,
@@ -579,7 +807,7 @@ int psw_tgv_set_images(psw_tgv* t, const float* d_ref, const float* d_src, doubl
 
 int psw_tgv_set_u(psw_tgv* t, const float* d_u, float value, psw_stream_t s) {
     PSW_CHECK_ARG(t, "NULL solver");
-    float* u = t->base + t->L.U + (size_t)(t->k % 3) * 3 * t->L.n;
+    float* u = t->base + t->L.U + (size_t)(t->k % 4) * 3 * t->L.n;
     if (d_u) {
         PSW_TRY_HIP(hipMemcpyAsync(u, d_u, t->L.n * sizeof(float), hipMemcpyDeviceToDevice, S(s)), PSW_ERR_COPY);
         return PSW_OK;
@@ -592,9 +820,9 @@ int psw_tgv_linearise(psw_tgv* t, const float* K, const float* Rrel, const float
     PSW_CHECK_ARG(t->have_images, "psw_tgv_set_images must come first");
     const TgvLayout& L = t->L;
     const size_t n = L.n;
-    const int cur = t->k % 3, prev = (t->k + 2) % 3;
-    tgv_warp_reset_kernel<<<1024, 256, 0, S(s)>>>(t->base + L.U, t->base + L.P + (size_t)(t->k % 2) * 2 * n,
-                                                  t->base + L.Q + (size_t)(t->k % 2) * 3 * n, t->base + L.u0, n, cur,
+    const int cur = t->k % 4, prev = (t->k + 3) % 4;
+    tgv_warp_reset_kernel<<<1024, 256, 0, S(s)>>>(t->base + L.U, t->base + L.P + (size_t)(t->k % 3) * 2 * n,
+                                                  t->base + L.Q + (size_t)(t->k % 3) * 3 * n, t->base + L.u0, n, cur,
                                                   prev);
     int st = launch_status("tgv_warp_reset_kernel");
     if (st != PSW_OK) return st;
@@ -619,7 +847,8 @@ int psw_tgv_linearise(psw_tgv* t, const float* K, const float* Rrel, const float
     pz.fy = (float)(double)K[4];
     const dim3 blk(64, 4), grd((unsigned)((t->W + 63) / 64), (unsigned)((t->H + 3) / 4));
     tgv_linearise_kernel<<<grd, blk, 0, S(s)>>>(pz, t->base + L.src, t->base + L.ref, t->base + L.u0, t->base + L.It,
-                                                t->base + L.Iu, t->base + L.r, t->W, t->H, t->N);
+                                                t->base + L.Iu, t->base + L.r + (size_t)(t->k % 3) * t->N * n, t->W,
+                                                t->H, t->N);
     st = launch_status("tgv_linearise_kernel");
     if (st != PSW_OK) return st;
     if ((st = psw_fill_f32(t->base + L.mask, 1.f, n, s)) != PSW_OK) return st;
@@ -630,9 +859,10 @@ int psw_tgv_set_primal(psw_tgv* t, const float* d_u, float u_value, const float*
     PSW_CHECK_ARG(t, "NULL solver");
     const TgvLayout& L = t->L;
     const size_t n = L.n;
-    const int cur = t->k % 3, prev = (t->k + 2) % 3;
-    tgv_set_primal_kernel<<<1024, 256, 0, S(s)>>>(t->base + L.U, t->base + L.P + (size_t)(t->k % 2) * 2 * n,
-                                                  t->base + L.Q + (size_t)(t->k % 2) * 3 * n, t->base + L.r, d_u,
+    const int cur = t->k % 4, prev = (t->k + 3) % 4;
+    tgv_set_primal_kernel<<<1024, 256, 0, S(s)>>>(t->base + L.U, t->base + L.P + (size_t)(t->k % 3) * 2 * n,
+                                                  t->base + L.Q + (size_t)(t->k % 3) * 3 * n,
+                                                  t->base + L.r + (size_t)(t->k % 3) * t->N * n, d_u,
                                                   u_value, d_u1, n, t->N, cur, prev);
     int st = launch_status("tgv_set_primal_kernel");
     if (st != PSW_OK) return st;
@@ -643,9 +873,9 @@ int psw_tgv_set_duals(psw_tgv* t, const float* d_p, const float* d_q, const floa
     PSW_CHECK_ARG(t, "NULL solver");
     const TgvLayout& L = t->L;
     const size_t n = L.n;
-    float* P = t->base + L.P + (size_t)(t->k % 2) * 2 * n;
-    float* Q = t->base + L.Q + (size_t)(t->k % 2) * 3 * n;
-    float* R = t->base + L.r;
+    float* P = t->base + L.P + (size_t)(t->k % 3) * 2 * n;
+    float* Q = t->base + L.Q + (size_t)(t->k % 3) * 3 * n;
+    float* R = t->base + L.r + (size_t)(t->k % 3) * t->N * n;
     int st = PSW_OK;
     if (d_p) PSW_TRY_HIP(hipMemcpyAsync(P, d_p, 2 * n * sizeof(float), hipMemcpyDeviceToDevice, S(s)), PSW_ERR_COPY);
     else st = psw_fill_f32(P, 0.f, 2 * n, s);
@@ -750,7 +980,7 @@ int psw_tgv_solve(psw_tgv* t, int max_iters, int check_every, double tol, double
 
 int psw_tgv_get_u(psw_tgv* t, float* d_out, psw_stream_t s) {
     PSW_CHECK_ARG(t && d_out, "NULL pointer");
-    const float* u = t->base + t->L.U + (size_t)(t->k % 3) * 3 * t->L.n;
+    const float* u = t->base + t->L.U + (size_t)(t->k % 4) * 3 * t->L.n;
     PSW_TRY_HIP(hipMemcpyAsync(d_out, u, t->L.n * sizeof(float), hipMemcpyDeviceToDevice, S(s)), PSW_ERR_COPY);
     return PSW_OK;
 }
@@ -763,16 +993,16 @@ int psw_tgv_get_state(psw_tgv* t, float* d_out, psw_stream_t s) {
     auto cp = [&](float* dst, const float* src, size_t cnt) {
         return hipMemcpyAsync(dst, src, cnt * sizeof(float), hipMemcpyDeviceToDevice, S(s));
     };
-    const float* U = B + L.U + (size_t)(t->k % 3) * 3 * n;
-    const float* P = B + L.P + (size_t)(t->k % 2) * 2 * n;
-    const float* Q = B + L.Q + (size_t)(t->k % 2) * 3 * n;
+    const float* U = B + L.U + (size_t)(t->k % 4) * 3 * n;
+    const float* P = B + L.P + (size_t)(t->k % 3) * 2 * n;
+    const float* Q = B + L.Q + (size_t)(t->k % 3) * 3 * n;
     float* o = d_out;
     PSW_TRY_HIP(cp(o, U, 3 * n), PSW_ERR_COPY); o += 3 * n;
     PSW_TRY_HIP(cp(o, P, 2 * n), PSW_ERR_COPY); o += 2 * n;
     PSW_TRY_HIP(cp(o, Q, 3 * n), PSW_ERR_COPY); o += 3 * n;
     PSW_TRY_HIP(cp(o, B + L.It, N * n), PSW_ERR_COPY); o += N * n;
     PSW_TRY_HIP(cp(o, B + L.Iu, N * n), PSW_ERR_COPY); o += N * n;
-    PSW_TRY_HIP(cp(o, B + L.r, N * n), PSW_ERR_COPY); o += N * n;
+    PSW_TRY_HIP(cp(o, B + L.r + (size_t)(t->k % 3) * N * n, N * n), PSW_ERR_COPY); o += N * n;
     PSW_TRY_HIP(cp(o, B + L.T, 3 * n), PSW_ERR_COPY); o += 3 * n;
     PSW_TRY_HIP(cp(o, B + L.u0, n), PSW_ERR_COPY);
     return PSW_OK;

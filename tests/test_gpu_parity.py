@@ -171,18 +171,37 @@ def test_tgv_fused_vs_fp64_oracle(gpu, views_small, niters, warps):
 
 
 def test_tgv_launch_shapes_identical(gpu, views_small, monkeypatch):
-    """The three tile/thread shapes of the iteration kernel give identical
-    iterates (each pixel's arithmetic does not depend on the tiling)."""
+    """The three tile/thread shapes of the one-iteration kernel and the
+    two-iterations-per-launch kernel give identical iterates (each pixel's
+    arithmetic does not depend on the tiling); the full state (u, u1, p, q,
+    r) is compared after an odd number of iterations, so the two-iteration
+    path also ends with a one-iteration launch."""
     from psw_amd import stages
+    from psw_amd.device import DeviceArray
     v = views_small
     K1 = one_based(v["K"])
+    Hh, W = v["ref"].shape
+    N = v["src"].shape[0]
     prm = stages.tgv_params(**TGV_KW)
-    out = []
-    for var in ("0", "1", "2"):
+    Rrel, trel = stages.relative_all(v["Rref"], v["tref"], v["Rsrc"], v["tsrc"])
+    d_ref, d_src = DeviceArray.from_host(v["ref"]), DeviceArray.from_host(v["src"])
+    d_init = DeviceArray.from_host(np.clip(v["depth"], 0.5, 6.0).astype(np.float32))
+    states = []
+    for double, var in (("0", "0"), ("0", "1"), ("0", "2"), ("1", "1"), ("1", "0")):
+        monkeypatch.setenv("PSW_TGV_DOUBLE", double)
         monkeypatch.setenv("PSW_TGV_VARIANT", var)
-        out.append(stages.tgv_run(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], 25, 2, prm))
-    np.testing.assert_array_equal(out[0], out[1])
-    np.testing.assert_array_equal(out[0], out[2])
+        t = stages.TgvSolver(W, Hh, N)
+        t.set_images(d_ref, d_src, prm.beta, prm.gamma)
+        t.set_u(d_init)
+        t.linearise(K1, Rrel, trel)
+        t.iterate(37, prm)
+        states.append(t.get_state())
+        # and the reference driver (warps x iterations) through the same kernels
+        states[-1]["run"] = stages.tgv_run(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], 25, 2,
+                                           prm)
+    for st in states[1:]:
+        for k in ("u", "u1x", "u1y", "px", "py", "qx", "qy", "qz", "r", "run"):
+            np.testing.assert_array_equal(states[0][k], st[k], err_msg=k)
 
 
 def test_tgv_set_primal_and_duals(gpu, views_small):
