@@ -267,6 +267,25 @@ int psw_sweep_fused_aggregate(const float* d_ref, const float* d_src, int width,
 int psw_sweep_argmin_depth(float* d_depth, const int32_t* d_argmin, const float* d_depths, int npixels,
                            float fallback, psw_stream_t s);
 
+/* Many frames of uneven size, source and plane counts in one persistent
+ * launch (no reference counterpart): workgroups take (tile, plane group)
+ * items of all frames from a device counter.  Each frame's results are
+ * bit-identical to psw_sweep_fused_aggregate on that frame.  groups: plane
+ * groups per tile (0 = chosen from the batch size); workgroups: resident
+ * workgroups (0 = 2048).  Returns after the work is done. */
+typedef struct {
+    const float* d_ref;        /* [height][width], 0..255 */
+    const float* d_src;        /* [nsrc][height][width] */
+    int width, height, nsrc;
+    const float* d_H;          /* [nsrc][nplanes][9], psw_sweep_homographies */
+    int nplanes;
+    int32_t* d_argmin;         /* outputs, [height][width] */
+    float* d_cost_min;
+    float* d_cost_second;
+} psw_sweep_frame;
+int psw_sweep_fused_aggregate_batch(const psw_sweep_frame* frames, int nframes, int winsize, float stdthresh,
+                                    int coord_offset, int fast, int groups, int workgroups, psw_stream_t s);
+
 /* ---------------------------------------------------------------------- */
 /* Stage 2: TGV2 solver.  State lives in one device allocation; one fused    */
 /* kernel per primal-dual iteration (P, Q, R, U of src/planesweep.cpp:485-509)*/
@@ -320,6 +339,21 @@ int psw_tgv_energy(psw_tgv* t, const psw_tgv_params* prm, double* energy, psw_st
  * the test.  iters_done / energy_out (nullable) report the stop. */
 int psw_tgv_solve(psw_tgv* t, int max_iters, int check_every, double tol, double e_star,
                   const psw_tgv_params* prm, int* iters_done, double* energy_out, psw_stream_t s);
+/* Many solvers (frames of any size and source count) per launch (no
+ * reference counterpart): every iteration is one launch over the tiles of all
+ * frames, each frame with its own parameters prm[f], energy target e_star[f]
+ * and stop; a stopped frame's tiles become no-ops.  With use_graph (and a
+ * non-NULL stream) a chunk of check_every iterations plus the energy check is
+ * captured once in a hipGraph and replayed.  variant: iteration kernel shape
+ * (0, 1, 2 as PSW_TGV_VARIANT; -1 = that variable or the default).  Per frame,
+ * results are bit-identical to psw_tgv_iterate / psw_tgv_solve with the same
+ * check_every.  iters_done[f], energy_out[f] (nullable) report the stops;
+ * both calls return after the work is done. */
+int psw_tgv_iterate_batch(psw_tgv* const* solvers, int nsolvers, int niters, const psw_tgv_params* prm, int variant,
+                          psw_stream_t s);
+int psw_tgv_solve_batch(psw_tgv* const* solvers, int nsolvers, const psw_tgv_params* prm, const double* e_star,
+                        int max_iters, int check_every, double tol, int variant, int use_graph, int* iters_done,
+                        double* energy_out, psw_stream_t s);
 /* Copy u (depth) / full state out.  Planes of the state: u, u1x, u1y, px, py,
  * qx, qy, qz, It[N], Iu[N], r[N], T11, T12, T22, u0. */
 int psw_tgv_get_u(psw_tgv* t, float* d_out, psw_stream_t s);

@@ -25,6 +25,8 @@
 // multiplies by 1/win and lets the compiler fuse multiply-adds.
 #include "psw_common.h"
 
+#include <vector>
+
 namespace psw {
 
 constexpr int SW_TX = 32;       // tile width  (pixels)
@@ -633,6 +635,100 @@ int psw_sweep_fused_aggregate(const float* d_ref, const float* d_src, int width,
     }
     if (ws) (void)hipFreeAsync(ws, S(s));
     return st;
+}
+
+int psw_sweep_fused_aggregate_batch(const psw_sweep_frame* frames, int nframes, int winsize, float stdthresh,
+                                    int coord_offset, int fast, int groups, int workgroups, psw_stream_t s) {
+    PSW_CHECK_ARG(frames && nframes >= 1, "no frames");
+    PSW_CHECK_ARG(coord_offset == 0 || coord_offset == 1, "coord_offset must be 0 or 1");
+    PSW_CHECK_ARG(groups >= 0 && workgroups >= 0, "groups and workgroups must not be negative");
+    std::vector<SweepArgs> args((size_t)nframes);
+    std::vector<int> off((size_t)nframes + 1, 0);
+    long total_tiles = 0;
+    for (int f = 0; f < nframes; ++f) {
+        const psw_sweep_frame& fr = frames[f];
+        int st = check_sweep_args(fr.d_ref, fr.d_src, fr.width, fr.height, fr.nsrc, fr.d_H, fr.nplanes, winsize);
+        if (st != PSW_OK) {
+            const std::string msg = get_error();
+            set_error("frame %d: %s", f, msg.c_str());
+            return st;
+        }
+        PSW_CHECK_ARG(fr.d_argmin && fr.d_cost_min && fr.d_cost_second, "NULL output pointer");
+        SweepArgs& a = args[(size_t)f];
+        a = SweepArgs{};
+        a.ref = fr.d_ref; a.src = fr.d_src; a.Hs = fr.d_H;
+        a.W = fr.width; a.H = fr.height; a.N = fr.nsrc; a.P = fr.nplanes; a.off = coord_offset;
+        a.stdthresh = stdthresh;
+        a.argmin = fr.d_argmin; a.cmin = fr.d_cost_min; a.csec = fr.d_cost_second;
+        sweep_layout(a, false);
+        total_tiles += a.ntiles;
+    }
+    // Plane groups: enough items across the whole batch to fill the GPU
+    // (about 3000 workgroups' worth) unless given.
+    int G = groups > 0 ? groups : (int)std::max(1L, (3072 + total_tiles / 2) / std::max(1L, total_tiles));
+    size_t ws_bytes = 0;
+    for (int f = 0; f < nframes; ++f) {
+        SweepArgs& a = args[(size_t)f];
+        a.G = std::max(1, std::min(G, std::min(a.P, 64)));
+        off[(size_t)f + 1] = off[(size_t)f] + a.ntiles * a.G;
+        if (a.G > 1) ws_bytes += (size_t)a.G * a.W * a.H * 12;
+    }
+    const int total = off[(size_t)nframes];
+    const size_t args_bytes = sizeof(SweepArgs) * (size_t)nframes, off_bytes = sizeof(int) * ((size_t)nframes + 1);
+    const size_t head = (args_bytes + off_bytes + 16 + 255) / 256 * 256;
+    char* dev = nullptr;
+    PSW_TRY_HIP(hipMalloc((void**)&dev, head + ws_bytes), PSW_ERR_ALLOC);
+    // per-frame group partials
+    size_t o = head;
+    for (int f = 0; f < nframes; ++f) {
+        SweepArgs& a = args[(size_t)f];
+        if (a.G > 1) {
+            const size_t npix = (size_t)a.W * a.H;
+            a.ws_cmin = reinterpret_cast<float*>(dev + o);
+            a.ws_csec = a.ws_cmin + (size_t)a.G * npix;
+            a.ws_arg = reinterpret_cast<int32_t*>(a.ws_csec + (size_t)a.G * npix);
+            o += (size_t)a.G * npix * 12;
+        }
+    }
+    SweepArgs* d_args = reinterpret_cast<SweepArgs*>(dev);
+    int* d_off = reinterpret_cast<int*>(dev + args_bytes);
+    unsigned* d_counter = reinterpret_cast<unsigned*>(dev + args_bytes + off_bytes);
+    hipStream_t st = S(s);
+    int rc = PSW_OK;
+    hipError_t e = hipMemcpyAsync(d_args, args.data(), args_bytes, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_off, off.data(), off_bytes, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) rc = hip_status(e, "psw_sweep_fused_aggregate_batch", PSW_ERR_COPY);
+    if (rc == PSW_OK) rc = psw_fill_u32(d_counter, 0u, 1, s);
+    const int wg = workgroups > 0 ? std::min(workgroups, total) : std::min(total, 2048);
+    if (rc == PSW_OK) {
+        const dim3 grid((unsigned)wg), block(SW_THREADS);
+        switch (winsize / 2) {
+#define PSW_SB(r)                                                                                              \
+    case r:                                                                                                    \
+        if (fast) sweep_batch_kernel<r, 1, true><<<grid, block, 0, st>>>(d_args, d_off, nframes, total, d_counter); \
+        else sweep_batch_kernel<r, 1, false><<<grid, block, 0, st>>>(d_args, d_off, nframes, total, d_counter);  \
+        break;
+            PSW_SB(0) PSW_SB(1) PSW_SB(2) PSW_SB(3) PSW_SB(4) PSW_SB(5) PSW_SB(6) PSW_SB(7)
+#undef PSW_SB
+            default:
+                set_error("window size %d not supported (odd, 1..15)", winsize);
+                rc = PSW_ERR_UNSUPPORTED;
+        }
+        if (rc == PSW_OK) rc = launch_status("sweep_batch_kernel");
+    }
+    for (int f = 0; f < nframes && rc == PSW_OK; ++f) {
+        const SweepArgs& a = args[(size_t)f];
+        if (a.G <= 1) continue;
+        const size_t npix = (size_t)a.W * a.H;
+        sweep_merge_kernel<<<(unsigned)((npix + 255) / 256), 256, 0, st>>>(a.ws_cmin, a.ws_arg, a.ws_csec, a.G, npix,
+                                                                         a.argmin, a.cmin, a.csec);
+        rc = launch_status("sweep_merge_kernel");
+    }
+    // the host arrays and the workspace live until the work is done
+    const hipError_t se = hipStreamSynchronize(st);
+    (void)hipFree(dev);
+    if (rc == PSW_OK && se != hipSuccess) rc = hip_status(se, "psw_sweep_fused_aggregate_batch");
+    return rc;
 }
 
 int psw_sweep_argmin_depth(float* d_depth, const int32_t* d_argmin, const float* d_depths, int npixels,

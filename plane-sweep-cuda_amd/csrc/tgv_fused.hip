@@ -20,6 +20,8 @@
 // that decide stopping.
 #include "psw_common.h"
 
+#include <vector>
+
 namespace psw {
 
 constexpr int TG_TX = 64;
@@ -614,30 +616,38 @@ __global__ __launch_bounds__(TG_THREADS) void tgv_energy_kernel(const float* __r
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-// Sum of the partials in a fixed order, then the stop test.
-__global__ __launch_bounds__(TG_THREADS) void tgv_energy_finalize_kernel(const double* __restrict__ partials, int np,
-                                                                      TgvCtrl* ctrl, int k, double tol, double e_star) {
+// Sum of the partials in a fixed order, then the stop test; returns (on
+// thread 0) whether this check stopped the solve.
+__device__ inline bool tgv_finalize_block(const double* __restrict__ partials, int np, TgvCtrl* ctrl, int k, double tol,
+                                          double e_star) {
     double v = 0.0;
     for (int i = threadIdx.x; i < np; i += TG_THREADS) v += partials[i];
     const double E = block_reduce_sum(v);
+    bool stopped = false;
     if (threadIdx.x == 0) {
         ctrl->energy = E;
         if (!ctrl->done && e_star > 0.0 && fabs(E - e_star) <= tol * e_star) {
             ctrl->done = 1;
             ctrl->iters_at_done = k;
             ctrl->energy_at_done = E;
+            stopped = true;
         }
     }
+    return stopped;
+}
+
+__global__ __launch_bounds__(TG_THREADS) void tgv_energy_finalize_kernel(const double* __restrict__ partials, int np,
+                                                                      TgvCtrl* ctrl, int k, double tol, double e_star) {
+    (void)tgv_finalize_block(partials, np, ctrl, k, tol, e_star);
 }
 
 constexpr int TG_ENERGY_BLOCKS = 1024;
 
-static inline TgvPtrs tgv_ptrs(psw_tgv* t, const psw_tgv_params* prm) {
-    const TgvLayout& L = t->L;
-    float* B = t->base;
+__host__ __device__ inline TgvPtrs tgv_ptrs_at(float* B, const TgvLayout& L, int W, int H, int N, int k,
+                                                const TgvCtrl* ctrl, float lambda, float alpha0, float alpha1,
+                                                float tau, float sigma) {
     const size_t n = L.n;
-    const int k = t->k;
-    const size_t rn = (size_t)t->N * n;
+    const size_t rn = (size_t)N * n;
     TgvPtrs p;
     p.ucur = B + L.U + (size_t)(k % 4) * 3 * n;
     p.uprev = B + L.U + (size_t)((k + 3) % 4) * 3 * n;
@@ -656,11 +666,15 @@ static inline TgvPtrs tgv_ptrs(psw_tgv* t, const psw_tgv_params* prm) {
     p.Iu = B + L.Iu;
     p.T = B + L.T;
     p.u0 = B + L.u0;
-    p.ctrl = t->ctrl;
-    p.W = t->W; p.H = t->H; p.N = t->N;
-    p.lambda = (float)prm->lambda; p.alpha0 = (float)prm->alpha0; p.alpha1 = (float)prm->alpha1;
-    p.tau = (float)prm->tau; p.sigma = (float)prm->sigma;
+    p.ctrl = ctrl;
+    p.W = W; p.H = H; p.N = N;
+    p.lambda = lambda; p.alpha0 = alpha0; p.alpha1 = alpha1; p.tau = tau; p.sigma = sigma;
     return p;
+}
+
+static inline TgvPtrs tgv_ptrs(psw_tgv* t, const psw_tgv_params* prm) {
+    return tgv_ptrs_at(t->base, t->L, t->W, t->H, t->N, t->k, t->ctrl, (float)prm->lambda, (float)prm->alpha0,
+                       (float)prm->alpha1, (float)prm->tau, (float)prm->sigma);
 }
 
 // Launch shape of the iteration kernel: 0 = 64x8 tile / 256 threads,
@@ -726,6 +740,71 @@ __global__ void tgv_ctrl_reset_kernel(TgvCtrl* c) {
     c->iters_at_done = 0;
     c->energy = 0.0;
     c->energy_at_done = 0.0;
+}
+
+// ---- many frames per launch ---------------------------------------------------
+
+// One frame of a batch as the device sees it; the iteration count of every
+// frame is its own start k0 plus a batch counter kdev that a one-thread
+// kernel advances after each iteration launch (so a captured chunk of
+// launches can be replayed).
+struct TgvFrameDev {
+    float* base;
+    TgvLayout L;
+    int W, H, N, k0;
+    TgvCtrl* ctrl;
+    double* partials;
+    int eblocks;
+    float lambda, alpha0, alpha1, tau, sigma;
+    double dlambda, dalpha0, dalpha1, e_star;
+};
+
+__device__ inline int batch_frame(const int* __restrict__ off, int nf, int b) {
+    int lo = 0, hi = nf - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int TY, int NT>
+__global__ __launch_bounds__(NT) void tgv_iter_batch_kernel(const TgvFrameDev* __restrict__ fr,
+                                                            const int* __restrict__ toff, int nf,
+                                                            const int* __restrict__ kdev) {
+    const int f = batch_frame(toff, nf, blockIdx.x);
+    const TgvFrameDev& d = fr[f];
+    if (d.ctrl->done) return;
+    const TgvPtrs a = tgv_ptrs_at(d.base, d.L, d.W, d.H, d.N, d.k0 + *kdev, d.ctrl, d.lambda, d.alpha0, d.alpha1, d.tau,
+                                  d.sigma);
+    tgv_iter_tile<TY, NT>(a, (int)blockIdx.x - toff[f]);
+}
+
+__global__ void tgv_batch_advance_kernel(int* kdev, int by) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *kdev += by;
+}
+
+__global__ __launch_bounds__(TG_THREADS) void tgv_energy_batch_kernel(const TgvFrameDev* __restrict__ fr,
+                                                                   const int* __restrict__ eoff, int nf,
+                                                                   const int* __restrict__ kdev) {
+    const int f = batch_frame(eoff, nf, blockIdx.x);
+    const TgvFrameDev& d = fr[f];
+    const int bid = (int)blockIdx.x - eoff[f];
+    const size_t n = d.L.n;
+    const float* U = d.base + d.L.U + (size_t)((d.k0 + *kdev) % 4) * 3 * n;
+    const double t = tgv_energy_block(U, d.base + d.L.T, d.base + d.L.u0, d.base + d.L.It, d.base + d.L.Iu,
+                                      d.base + d.L.mask, d.W, d.H, d.N, d.dlambda, d.dalpha0, d.dalpha1, bid,
+                                      d.eblocks);
+    if (threadIdx.x == 0) d.partials[bid] = t;
+}
+
+__global__ __launch_bounds__(TG_THREADS) void tgv_finalize_batch_kernel(const TgvFrameDev* __restrict__ fr,
+                                                                     const int* __restrict__ kdev, double tol,
+                                                                     int* __restrict__ ndone) {
+    const TgvFrameDev& d = fr[blockIdx.x];
+    const bool stopped = tgv_finalize_block(d.partials, d.eblocks, d.ctrl, d.k0 + *kdev, tol, d.e_star);
+    if (threadIdx.x == 0 && stopped) atomicAdd(ndone, 1);
 }
 
 static int tgv_reset_ctrl(psw_tgv* t, hipStream_t st) {
@@ -1021,6 +1100,172 @@ int psw_tgv_run(psw_tgv* t, const float* K, const float* Rrel, const float* trel
         if (st != PSW_OK) return st;
     }
     return psw_tgv_get_u(t, d_u_out, s);
+}
+
+
+// Device tables of a batch: frames, tile and energy-block offsets, the
+// iteration counter and the stopped-frame counter.
+struct TgvBatch {
+    char* dev = nullptr;
+    TgvFrameDev* frames = nullptr;
+    int* toff = nullptr;
+    int* eoff = nullptr;
+    int* kdev = nullptr;
+    int* ndone = nullptr;
+    int tiles = 0, eblocks = 0, ty = 8, nt = 512;
+    ~TgvBatch() {
+        if (dev) (void)hipFree(dev);
+    }
+};
+
+static int tgv_batch_setup(psw_tgv* const* ts, int nf, const psw_tgv_params* prm, const double* e_star, int variant,
+                           TgvBatch& b, hipStream_t st) {
+    PSW_CHECK_ARG(ts && prm && nf >= 1, "NULL pointer or no frames");
+    const int var = variant >= 0 && variant <= 2 ? variant : tgv_variant();
+    b.ty = var == 2 ? 4 : 8;
+    b.nt = var == 1 ? 512 : 256;
+    std::vector<TgvFrameDev> fr((size_t)nf);
+    std::vector<int> toff((size_t)nf + 1, 0), eoff((size_t)nf + 1, 0);
+    for (int f = 0; f < nf; ++f) {
+        psw_tgv* t = ts[f];
+        PSW_CHECK_ARG(t, "NULL solver in the batch");
+        TgvFrameDev& d = fr[(size_t)f];
+        d.base = t->base; d.L = t->L; d.W = t->W; d.H = t->H; d.N = t->N; d.k0 = t->k;
+        d.ctrl = t->ctrl; d.partials = t->partials;
+        int blocks = (int)((t->L.n + TG_THREADS - 1) / TG_THREADS);
+        d.eblocks = blocks > t->npartials ? t->npartials : blocks;   // as tgv_launch_energy
+        const psw_tgv_params& q = prm[f];
+        d.lambda = (float)q.lambda; d.alpha0 = (float)q.alpha0; d.alpha1 = (float)q.alpha1;
+        d.tau = (float)q.tau; d.sigma = (float)q.sigma;
+        d.dlambda = q.lambda; d.dalpha0 = q.alpha0; d.dalpha1 = q.alpha1;
+        d.e_star = e_star ? e_star[f] : 0.0;
+        toff[(size_t)f + 1] = toff[(size_t)f] + ((t->W + TG_TX - 1) / TG_TX) * ((t->H + b.ty - 1) / b.ty);
+        eoff[(size_t)f + 1] = eoff[(size_t)f] + d.eblocks;
+    }
+    b.tiles = toff[(size_t)nf];
+    b.eblocks = eoff[(size_t)nf];
+    const size_t fb = sizeof(TgvFrameDev) * (size_t)nf, ob = sizeof(int) * ((size_t)nf + 1);
+    PSW_TRY_HIP(hipMalloc((void**)&b.dev, fb + 2 * ob + 2 * sizeof(int)), PSW_ERR_ALLOC);
+    b.frames = reinterpret_cast<TgvFrameDev*>(b.dev);
+    b.toff = reinterpret_cast<int*>(b.dev + fb);
+    b.eoff = reinterpret_cast<int*>(b.dev + fb + ob);
+    b.kdev = reinterpret_cast<int*>(b.dev + fb + 2 * ob);
+    b.ndone = b.kdev + 1;
+    PSW_TRY_HIP(hipMemcpyAsync(b.frames, fr.data(), fb, hipMemcpyHostToDevice, st), PSW_ERR_COPY);
+    PSW_TRY_HIP(hipMemcpyAsync(b.toff, toff.data(), ob, hipMemcpyHostToDevice, st), PSW_ERR_COPY);
+    PSW_TRY_HIP(hipMemcpyAsync(b.eoff, eoff.data(), ob, hipMemcpyHostToDevice, st), PSW_ERR_COPY);
+    int rc = psw_fill_u32(reinterpret_cast<uint32_t*>(b.kdev), 0u, 2, st);
+    for (int f = 0; f < nf && rc == PSW_OK; ++f) rc = tgv_reset_ctrl(ts[f], st);
+    // the host tables must outlive the copies
+    if (rc == PSW_OK) PSW_TRY_HIP(hipStreamSynchronize(st), PSW_ERR_DEVICE);
+    return rc;
+}
+
+// c iterations of every frame (each followed by the counter advance).
+static int tgv_batch_iters(const TgvBatch& b, int nf, int c, hipStream_t st) {
+    for (int i = 0; i < c; ++i) {
+        if (b.ty == 4) tgv_iter_batch_kernel<4, 256><<<b.tiles, 256, 0, st>>>(b.frames, b.toff, nf, b.kdev);
+        else if (b.nt == 512) tgv_iter_batch_kernel<8, 512><<<b.tiles, 512, 0, st>>>(b.frames, b.toff, nf, b.kdev);
+        else tgv_iter_batch_kernel<8, 256><<<b.tiles, 256, 0, st>>>(b.frames, b.toff, nf, b.kdev);
+        tgv_batch_advance_kernel<<<1, 64, 0, st>>>(b.kdev, 1);
+    }
+    return launch_status("tgv_iter_batch_kernel");
+}
+
+static int tgv_batch_check(const TgvBatch& b, int nf, double tol, hipStream_t st) {
+    tgv_energy_batch_kernel<<<b.eblocks, TG_THREADS, 0, st>>>(b.frames, b.eoff, nf, b.kdev);
+    tgv_finalize_batch_kernel<<<nf, TG_THREADS, 0, st>>>(b.frames, b.kdev, tol, b.ndone);
+    return launch_status("tgv_energy_batch_kernel");
+}
+
+int psw_tgv_iterate_batch(psw_tgv* const* solvers, int nsolvers, int niters, const psw_tgv_params* prm, int variant,
+                          psw_stream_t s) {
+    PSW_CHECK_ARG(niters >= 0, "niters must be >= 0");
+    hipStream_t st = S(s);
+    TgvBatch b;
+    int rc = tgv_batch_setup(solvers, nsolvers, prm, nullptr, variant, b, st);
+    if (rc == PSW_OK) rc = tgv_batch_iters(b, nsolvers, niters, st);
+    if (rc == PSW_OK) PSW_TRY_HIP(hipStreamSynchronize(st), PSW_ERR_DEVICE);
+    if (rc != PSW_OK) return rc;
+    for (int f = 0; f < nsolvers; ++f) solvers[f]->k += niters;
+    return PSW_OK;
+}
+
+int psw_tgv_solve_batch(psw_tgv* const* solvers, int nsolvers, const psw_tgv_params* prm, const double* e_star,
+                        int max_iters, int check_every, double tol, int variant, int use_graph, int* iters_done,
+                        double* energy_out, psw_stream_t s) {
+    PSW_CHECK_ARG(e_star, "NULL e_star");
+    PSW_CHECK_ARG(max_iters >= 0 && check_every >= 1, "bad iteration counts");
+    hipStream_t st = S(s);
+    TgvBatch b;
+    int rc = tgv_batch_setup(solvers, nsolvers, prm, e_star, variant, b, st);
+    if (rc != PSW_OK) return rc;
+    int* h_ndone = nullptr;
+    PSW_TRY_HIP(hipHostMalloc((void**)&h_ndone, sizeof(int), hipHostMallocDefault), PSW_ERR_ALLOC);
+    *h_ndone = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipError_t e = hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
+    if (e != hipSuccess) rc = hip_status(e, "hipEventCreate");
+    const int full = max_iters / check_every;
+    if (rc == PSW_OK && use_graph && full > 0 && st != nullptr) {   // the null stream cannot be captured
+        // one chunk: check_every iterations, the check, the stopped count to the host
+        e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            rc = tgv_batch_iters(b, nsolvers, check_every, st);
+            if (rc == PSW_OK) rc = tgv_batch_check(b, nsolvers, tol, st);
+            if (rc == PSW_OK) e = hipMemcpyAsync(h_ndone, b.ndone, sizeof(int), hipMemcpyDeviceToHost, st);
+            hipError_t e2 = hipStreamEndCapture(st, &graph);
+            if (e == hipSuccess) e = e2;
+            if (e == hipSuccess && rc == PSW_OK) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        }
+        if (e != hipSuccess && rc == PSW_OK) rc = hip_status(e, "hipGraph capture");
+    }
+    int launched = 0, chunk = 0;
+    while (rc == PSW_OK && launched < max_iters) {
+        const int c = std::min(check_every, max_iters - launched);
+        if (exec && c == check_every) {
+            e = hipGraphLaunch(exec, st);
+            if (e != hipSuccess) { rc = hip_status(e, "hipGraphLaunch"); break; }
+        } else {
+            rc = tgv_batch_iters(b, nsolvers, c, st);
+            if (rc == PSW_OK) rc = tgv_batch_check(b, nsolvers, tol, st);
+            if (rc != PSW_OK) break;
+            e = hipMemcpyAsync(h_ndone, b.ndone, sizeof(int), hipMemcpyDeviceToHost, st);
+            if (e != hipSuccess) { rc = hip_status(e, "hipMemcpyAsync"); break; }
+        }
+        launched += c;
+        e = hipEventRecord(ev[chunk & 1], st);
+        if (e != hipSuccess) { rc = hip_status(e, "hipEventRecord"); break; }
+        // one chunk in flight: wait for the previous chunk's count (a later
+        // chunk may already have raised it; the count only grows)
+        if (chunk >= 1) {
+            if (hipEventSynchronize(ev[(chunk - 1) & 1]) != hipSuccess) { rc = PSW_ERR_DEVICE; break; }
+            if (*h_ndone >= nsolvers) break;
+        }
+        chunk++;
+    }
+    (void)hipStreamSynchronize(st);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (int k = 0; k < 2; ++k)
+        if (ev[k]) (void)hipEventDestroy(ev[k]);
+    (void)hipHostFree(h_ndone);
+    if (rc != PSW_OK) return rc;
+    for (int f = 0; f < nsolvers; ++f) {
+        psw_tgv* t = solvers[f];
+        const int k0 = t->k;
+        PSW_TRY_HIP(hipMemcpy(t->hctrl, t->ctrl, sizeof(TgvCtrl), hipMemcpyDeviceToHost), PSW_ERR_COPY);
+        t->k = t->hctrl->done ? t->hctrl->iters_at_done : k0 + launched;
+        if (iters_done) iters_done[f] = t->k - k0;
+        if (energy_out) energy_out[f] = t->hctrl->done ? t->hctrl->energy_at_done : t->hctrl->energy;
+        rc = tgv_reset_ctrl(t, st);
+        if (rc != PSW_OK) return rc;
+    }
+    PSW_TRY_HIP(hipStreamSynchronize(st), PSW_ERR_DEVICE);
+    return PSW_OK;
 }
 
 }  // extern "C"

@@ -44,6 +44,12 @@ class FusionGrid(ctypes.Structure):
 PSW_SCENE_MAX_BOXES = 32
 
 
+class SweepFrame(ctypes.Structure):
+    _fields_ = [("d_ref", c_void_p), ("d_src", c_void_p), ("width", c_int), ("height", c_int), ("nsrc", c_int),
+                ("d_H", c_void_p), ("nplanes", c_int), ("d_argmin", c_void_p), ("d_cost_min", c_void_p),
+                ("d_cost_second", c_void_p)]
+
+
 class PswRoom(ctypes.Structure):
     _fields_ = [("lo", c_double * 3), ("hi", c_double * 3), ("nboxes", c_int),
                 ("box_lo", (c_double * 3) * PSW_SCENE_MAX_BOXES), ("box_hi", (c_double * 3) * PSW_SCENE_MAX_BOXES),
@@ -146,6 +152,11 @@ _SIGS = {
     "psw_tgv_set_primal": (c_int, [c_void_p, D, c_float, D, c_void_p]),
     "psw_tgv_set_duals": (c_int, [c_void_p, D, D, D, c_void_p]),
     "psw_tgv_set_energy_mask": (c_int, [c_void_p, c_double, P(TgvParams), P(ctypes.c_int64), c_void_p]),
+    "psw_tgv_iterate_batch": (c_int, [P(c_void_p), c_int, c_int, P(TgvParams), c_int, c_void_p]),
+    "psw_tgv_solve_batch": (c_int, [P(c_void_p), c_int, P(TgvParams), P(c_double), c_int, c_int, c_double, c_int, c_int,
+                                    P(c_int), P(c_double), c_void_p]),
+    "psw_sweep_fused_aggregate_batch": (c_int, [P(SweepFrame), c_int, c_int, c_float, c_int, c_int, c_int, c_int,
+                                                c_void_p]),
     "psw_tgv_linearise": (c_int, [c_void_p, H, H, H, c_void_p]),
     "psw_tgv_iterate": (c_int, [c_void_p, c_int, P(TgvParams), c_void_p]),
     "psw_tgv_energy": (c_int, [c_void_p, P(TgvParams), P(c_double), c_void_p]),
