@@ -222,6 +222,92 @@ def probe(names, seed):
     log("probe done")
 
 
+def run_batch(args):
+    """The uneven batch: 32 seeded frames, resolution from
+    {320x240, 640x480, 1242x375}, 4-8 sources, 64-256 planes per frame; sweep
+    and TGV2 (plain start, linearised at the sweep depth, to the energy gap)
+    for the whole batch in batched launches, against the same frames one at
+    a time through the single-frame entry points."""
+    import psw_amd
+    from psw_amd import batch, stages
+    from psw_amd.device import Stream
+    from psw_amd.device import synchronize as device_synchronize
+    log("device:", psw_amd.device_name())
+    st = Stream()
+    s = st.handle
+    specs = batch.uneven_workload(args.seed, args.batch_frames)
+    t0 = time.perf_counter()
+    frames = [batch.Frame(sp, ZNEAR, ZFAR, WIN, STD_T) for sp in specs]
+    device_synchronize()
+    npx = sum(sp["W"] * sp["H"] for sp in specs)
+    log(f"{len(frames)} frames ({npx / 1e6:.2f} Mpx) rendered in {time.perf_counter() - t0:.2f} s")
+    prm = stages.tgv_params(**TGV)
+    best, timings = batch.autotune(frames, prm, stream=st, log=log)
+    log(f"autotuned: {best}")
+    solvers = [f.tgv for f in frames]
+    sweeps = [f.sweep for f in frames]
+
+    def sweep_all():
+        for sw in sweeps:
+            sw.build_homographies(s)
+        batch.sweep_batch(sweeps, True, best["sweep_groups"], best["sweep_workgroups"], s)
+
+    def linearise_all():
+        for f in frames:
+            f.linearise(prm, QMAX, s)
+
+    # E* per frame (not timed): batched iterations from the plain start
+    sweep_all()
+    linearise_all()
+    batch.tgv_iterate_batch(solvers, args.batch_estar_iters, prm, best["tgv_variant"], s)
+    e_star = [t.energy(prm, s) for t in solvers]
+    st.synchronize()
+
+    def batched():
+        sweep_all()
+        linearise_all()
+        return batch.tgv_solve_batch(solvers, prm, e_star, args.max_iters, args.check_every, GAP_TOL,
+                                     best["tgv_variant"], True, s)
+
+    batched()                                   # warm-up
+    st.synchronize()
+    t0 = time.perf_counter()
+    iters, energies = batched()
+    st.synchronize()
+    t_batch = time.perf_counter() - t0
+    gaps = [abs(e - es) / es for e, es in zip(energies, e_star)]
+    # the same frames one at a time through the single-frame entry points
+    t0 = time.perf_counter()
+    seq_ms = []
+    for f, es in zip(frames, e_star):
+        t1 = time.perf_counter()
+        f.sweep.build_homographies(s)
+        f.sweep.run(fast=True, stream=s)
+        f.linearise(prm, QMAX, s)
+        f.tgv.solve(args.max_iters, args.check_every, GAP_TOL, es, prm, s)
+        st.synchronize()
+        seq_ms.append((time.perf_counter() - t1) * 1e3)
+    t_seq = time.perf_counter() - t0
+    rec = {
+        "metric": "uneven-batch depthmaps/s (32 frames, sweep + TGV2 to the energy gap)",
+        "value": len(frames) / t_batch, "unit": "depthmaps/s", "n_gpus": 1, "higher_is_better": True,
+        "batch_ms": t_batch * 1e3, "sequential_ms": t_seq * 1e3, "speedup_vs_sequential": t_seq / t_batch,
+        "dtype": "f32", "data": "synthetic (seeded room scenes rendered on the GPU)",
+        "config": {"workload": "32 frames, resolution in {320x240, 640x480, 1242x375}, 4-8 sources, 64-256 "
+                               "inverse-depth planes 0.3-8 m, 9x9 NCC, TGV2 plain start to energy gap 1e-4 of E* "
+                               f"({args.batch_estar_iters}-iteration run), energy mask qmax {QMAX}",
+                   "frames": specs, "megapixels": npx / 1e6, "seed": args.seed},
+        "autotune": {"chosen": best,
+                     "sweep_ms": {f"groups{g}_wg{w}": v for (g, w), v in timings["sweep"].items()},
+                     "tgv_us_per_iteration": {f"variant{v}": ms * 1e3 for v, ms in timings["tgv"].items()}},
+        "iterations": [int(i) for i in iters], "max_rel_gap_at_stop": float(max(gaps)),
+        "sequential_per_frame_ms": seq_ms,
+    }
+    log(f"batch: {t_batch * 1e3:.1f} ms for {len(frames)} frames ({len(frames) / t_batch:.2f} depthmaps/s); "
+        f"one at a time: {t_seq * 1e3:.1f} ms")
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     from psw_amd import warmstart
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
@@ -238,6 +324,9 @@ def main():
     ap.add_argument("--no-cold", action="store_true", help="skip the constant-start (u = 1) run")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-fusion", action="store_true", help="skip the 256^3 fusion measurement")
+    ap.add_argument("--batch", action="store_true", help="run the uneven 32-frame batch instead (its own JSON line)")
+    ap.add_argument("--batch-frames", type=int, default=32)
+    ap.add_argument("--batch-estar-iters", type=int, default=20000)
     ap.add_argument("--probe", default="", help="comma list of tgv,sweep,sweep_exact,unet,copy,fusion: run only "
                                                 "these small workloads (for rocprofv3) and exit")
     args = ap.parse_args()
@@ -247,6 +336,9 @@ def main():
         sys.exit(2)
     if args.probe:
         probe(args.probe.split(","), args.seed)
+        return
+    if args.batch:
+        run_batch(args)
         return
     if not os.path.exists(args.weights):
         sys.exit(f"bench.py: trained warm-start weights not found at {args.weights} "

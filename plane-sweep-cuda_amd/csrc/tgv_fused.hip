@@ -1113,9 +1113,38 @@ struct TgvBatch {
     int* kdev = nullptr;
     int* ndone = nullptr;
     int tiles = 0, eblocks = 0, ty = 8, nt = 512;
+    int nact = 0;                       // frames in the device tables
+    std::vector<TgvFrameDev> all;       // every frame of the batch
+    std::vector<int> active;            // indices (into all) still iterating
     ~TgvBatch() {
         if (dev) (void)hipFree(dev);
     }
+    // Device tables for the active frames only (stopped frames no longer
+    // cost a dispatch or an energy pass).  Synchronous.
+    int upload(hipStream_t st) {
+        nact = (int)active.size();
+        std::vector<TgvFrameDev> fr;
+        std::vector<int> toff(1, 0), eoff(1, 0);
+        for (int f : active) {
+            const TgvFrameDev& d = all[(size_t)f];
+            fr.push_back(d);
+            toff.push_back(toff.back() + ((d.W + TG_TX - 1) / TG_TX) * ((d.H + ty - 1) / ty));
+            eoff.push_back(eoff.back() + d.eblocks);
+        }
+        tiles = toff.back();
+        eblocks = eoff.back();
+        if (nact == 0) return PSW_OK;
+        PSW_TRY_HIP(hipMemcpyAsync(frames, fr.data(), sizeof(TgvFrameDev) * fr.size(), hipMemcpyHostToDevice, st),
+                    PSW_ERR_COPY);
+        PSW_TRY_HIP(hipMemcpyAsync(toff_d(), toff.data(), sizeof(int) * toff.size(), hipMemcpyHostToDevice, st),
+                    PSW_ERR_COPY);
+        PSW_TRY_HIP(hipMemcpyAsync(eoff_d(), eoff.data(), sizeof(int) * eoff.size(), hipMemcpyHostToDevice, st),
+                    PSW_ERR_COPY);
+        PSW_TRY_HIP(hipStreamSynchronize(st), PSW_ERR_DEVICE);
+        return PSW_OK;
+    }
+    int* toff_d() const { return toff; }
+    int* eoff_d() const { return eoff; }
 };
 
 static int tgv_batch_setup(psw_tgv* const* ts, int nf, const psw_tgv_params* prm, const double* e_star, int variant,
@@ -1124,12 +1153,12 @@ static int tgv_batch_setup(psw_tgv* const* ts, int nf, const psw_tgv_params* prm
     const int var = variant >= 0 && variant <= 2 ? variant : tgv_variant();
     b.ty = var == 2 ? 4 : 8;
     b.nt = var == 1 ? 512 : 256;
-    std::vector<TgvFrameDev> fr((size_t)nf);
-    std::vector<int> toff((size_t)nf + 1, 0), eoff((size_t)nf + 1, 0);
+    b.all.assign((size_t)nf, TgvFrameDev{});
+    b.active.clear();
     for (int f = 0; f < nf; ++f) {
         psw_tgv* t = ts[f];
         PSW_CHECK_ARG(t, "NULL solver in the batch");
-        TgvFrameDev& d = fr[(size_t)f];
+        TgvFrameDev& d = b.all[(size_t)f];
         d.base = t->base; d.L = t->L; d.W = t->W; d.H = t->H; d.N = t->N; d.k0 = t->k;
         d.ctrl = t->ctrl; d.partials = t->partials;
         int blocks = (int)((t->L.n + TG_THREADS - 1) / TG_THREADS);
@@ -1139,11 +1168,8 @@ static int tgv_batch_setup(psw_tgv* const* ts, int nf, const psw_tgv_params* prm
         d.tau = (float)q.tau; d.sigma = (float)q.sigma;
         d.dlambda = q.lambda; d.dalpha0 = q.alpha0; d.dalpha1 = q.alpha1;
         d.e_star = e_star ? e_star[f] : 0.0;
-        toff[(size_t)f + 1] = toff[(size_t)f] + ((t->W + TG_TX - 1) / TG_TX) * ((t->H + b.ty - 1) / b.ty);
-        eoff[(size_t)f + 1] = eoff[(size_t)f] + d.eblocks;
+        b.active.push_back(f);
     }
-    b.tiles = toff[(size_t)nf];
-    b.eblocks = eoff[(size_t)nf];
     const size_t fb = sizeof(TgvFrameDev) * (size_t)nf, ob = sizeof(int) * ((size_t)nf + 1);
     PSW_TRY_HIP(hipMalloc((void**)&b.dev, fb + 2 * ob + 2 * sizeof(int)), PSW_ERR_ALLOC);
     b.frames = reinterpret_cast<TgvFrameDev*>(b.dev);
@@ -1151,18 +1177,15 @@ static int tgv_batch_setup(psw_tgv* const* ts, int nf, const psw_tgv_params* prm
     b.eoff = reinterpret_cast<int*>(b.dev + fb + ob);
     b.kdev = reinterpret_cast<int*>(b.dev + fb + 2 * ob);
     b.ndone = b.kdev + 1;
-    PSW_TRY_HIP(hipMemcpyAsync(b.frames, fr.data(), fb, hipMemcpyHostToDevice, st), PSW_ERR_COPY);
-    PSW_TRY_HIP(hipMemcpyAsync(b.toff, toff.data(), ob, hipMemcpyHostToDevice, st), PSW_ERR_COPY);
-    PSW_TRY_HIP(hipMemcpyAsync(b.eoff, eoff.data(), ob, hipMemcpyHostToDevice, st), PSW_ERR_COPY);
     int rc = psw_fill_u32(reinterpret_cast<uint32_t*>(b.kdev), 0u, 2, st);
     for (int f = 0; f < nf && rc == PSW_OK; ++f) rc = tgv_reset_ctrl(ts[f], st);
-    // the host tables must outlive the copies
-    if (rc == PSW_OK) PSW_TRY_HIP(hipStreamSynchronize(st), PSW_ERR_DEVICE);
+    if (rc == PSW_OK) rc = b.upload(st);
     return rc;
 }
 
 // c iterations of every frame (each followed by the counter advance).
-static int tgv_batch_iters(const TgvBatch& b, int nf, int c, hipStream_t st) {
+static int tgv_batch_iters(const TgvBatch& b, int c, hipStream_t st) {
+    const int nf = b.nact;
     for (int i = 0; i < c; ++i) {
         if (b.ty == 4) tgv_iter_batch_kernel<4, 256><<<b.tiles, 256, 0, st>>>(b.frames, b.toff, nf, b.kdev);
         else if (b.nt == 512) tgv_iter_batch_kernel<8, 512><<<b.tiles, 512, 0, st>>>(b.frames, b.toff, nf, b.kdev);
@@ -1172,7 +1195,8 @@ static int tgv_batch_iters(const TgvBatch& b, int nf, int c, hipStream_t st) {
     return launch_status("tgv_iter_batch_kernel");
 }
 
-static int tgv_batch_check(const TgvBatch& b, int nf, double tol, hipStream_t st) {
+static int tgv_batch_check(const TgvBatch& b, double tol, hipStream_t st) {
+    const int nf = b.nact;
     tgv_energy_batch_kernel<<<b.eblocks, TG_THREADS, 0, st>>>(b.frames, b.eoff, nf, b.kdev);
     tgv_finalize_batch_kernel<<<nf, TG_THREADS, 0, st>>>(b.frames, b.kdev, tol, b.ndone);
     return launch_status("tgv_energy_batch_kernel");
@@ -1184,7 +1208,7 @@ int psw_tgv_iterate_batch(psw_tgv* const* solvers, int nsolvers, int niters, con
     hipStream_t st = S(s);
     TgvBatch b;
     int rc = tgv_batch_setup(solvers, nsolvers, prm, nullptr, variant, b, st);
-    if (rc == PSW_OK) rc = tgv_batch_iters(b, nsolvers, niters, st);
+    if (rc == PSW_OK) rc = tgv_batch_iters(b, niters, st);
     if (rc == PSW_OK) PSW_TRY_HIP(hipStreamSynchronize(st), PSW_ERR_DEVICE);
     if (rc != PSW_OK) return rc;
     for (int f = 0; f < nsolvers; ++f) solvers[f]->k += niters;
@@ -1210,28 +1234,32 @@ int psw_tgv_solve_batch(psw_tgv* const* solvers, int nsolvers, const psw_tgv_par
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
     if (e != hipSuccess) rc = hip_status(e, "hipEventCreate");
     const int full = max_iters / check_every;
-    if (rc == PSW_OK && use_graph && full > 0 && st != nullptr) {   // the null stream cannot be captured
-        // one chunk: check_every iterations, the check, the stopped count to the host
-        e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-        if (e == hipSuccess) {
-            rc = tgv_batch_iters(b, nsolvers, check_every, st);
-            if (rc == PSW_OK) rc = tgv_batch_check(b, nsolvers, tol, st);
-            if (rc == PSW_OK) e = hipMemcpyAsync(h_ndone, b.ndone, sizeof(int), hipMemcpyDeviceToHost, st);
-            hipError_t e2 = hipStreamEndCapture(st, &graph);
-            if (e == hipSuccess) e = e2;
-            if (e == hipSuccess && rc == PSW_OK) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        }
-        if (e != hipSuccess && rc == PSW_OK) rc = hip_status(e, "hipGraph capture");
-    }
-    int launched = 0, chunk = 0;
+    const bool graphs = use_graph && full > 0 && st != nullptr;   // the null stream cannot be captured
+    // one chunk: check_every iterations, the check, the stopped count to the host
+    auto capture = [&]() -> int {
+        if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
+        if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+        hipError_t ce = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (ce != hipSuccess) return hip_status(ce, "hipStreamBeginCapture");
+        int crc = tgv_batch_iters(b, check_every, st);
+        if (crc == PSW_OK) crc = tgv_batch_check(b, tol, st);
+        if (crc == PSW_OK) ce = hipMemcpyAsync(h_ndone, b.ndone, sizeof(int), hipMemcpyDeviceToHost, st);
+        const hipError_t ee = hipStreamEndCapture(st, &graph);
+        if (ce == hipSuccess) ce = ee;
+        if (ce == hipSuccess && crc == PSW_OK) ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (crc != PSW_OK) return crc;
+        return ce == hipSuccess ? PSW_OK : hip_status(ce, "hipGraph capture");
+    };
+    if (rc == PSW_OK && graphs) rc = capture();
+    int launched = 0, chunk = 0, seen_done = 0;
     while (rc == PSW_OK && launched < max_iters) {
         const int c = std::min(check_every, max_iters - launched);
         if (exec && c == check_every) {
             e = hipGraphLaunch(exec, st);
             if (e != hipSuccess) { rc = hip_status(e, "hipGraphLaunch"); break; }
         } else {
-            rc = tgv_batch_iters(b, nsolvers, c, st);
-            if (rc == PSW_OK) rc = tgv_batch_check(b, nsolvers, tol, st);
+            rc = tgv_batch_iters(b, c, st);
+            if (rc == PSW_OK) rc = tgv_batch_check(b, tol, st);
             if (rc != PSW_OK) break;
             e = hipMemcpyAsync(h_ndone, b.ndone, sizeof(int), hipMemcpyDeviceToHost, st);
             if (e != hipSuccess) { rc = hip_status(e, "hipMemcpyAsync"); break; }
@@ -1244,6 +1272,26 @@ int psw_tgv_solve_batch(psw_tgv* const* solvers, int nsolvers, const psw_tgv_par
         if (chunk >= 1) {
             if (hipEventSynchronize(ev[(chunk - 1) & 1]) != hipSuccess) { rc = PSW_ERR_DEVICE; break; }
             if (*h_ndone >= nsolvers) break;
+            if (*h_ndone > seen_done) {
+                // frames stopped: let the queued chunk finish, drop them from
+                // the tables (their launches would only return) and recapture
+                if (hipStreamSynchronize(st) != hipSuccess) { rc = PSW_ERR_DEVICE; break; }
+                seen_done = *h_ndone;
+                if (seen_done >= nsolvers) break;
+                std::vector<int> still;
+                for (int f : b.active) {
+                    TgvCtrl c2;
+                    if (hipMemcpy(&c2, solvers[f]->ctrl, sizeof(TgvCtrl), hipMemcpyDeviceToHost) != hipSuccess) {
+                        rc = PSW_ERR_COPY;
+                        break;
+                    }
+                    if (!c2.done) still.push_back(f);
+                }
+                if (rc != PSW_OK || still.empty()) break;
+                b.active.swap(still);
+                rc = b.upload(st);
+                if (rc == PSW_OK && graphs) rc = capture();
+            }
         }
         chunk++;
     }
