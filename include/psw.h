@@ -401,6 +401,13 @@ int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double la
  * sum |grad u| + lambda sum_k h_k |u - c_k| with the P step's forward
  * differences and the bin centres; synchronous. */
 int psw_fusion_energy(const psw_fusion_grid* g, double lambda, double* energy, psw_stream_t s);
+/* Fused-surface points (src/pclviewer.cpp:1058-1074): world coordinates of
+ * the centres of the voxels with -1 < u < 0 (worldCoords, fusion.h:360-364),
+ * in the reference's z, y, x loop order, into d_xyz [capacity][3] (device).
+ * *count receives the total (which may exceed capacity; only capacity points
+ * are written).  Synchronous. */
+int psw_fusion_surface_points(const psw_fusion_grid* g, float* d_xyz, int64_t capacity, int64_t* count,
+                              psw_stream_t s);
 /* Voxels with -1 < u < 0 (src/pclviewer.cpp:1062); synchronous. */
 int psw_fusion_surface_count(const psw_fusion_grid* g, int64_t* count, psw_stream_t s);
 

@@ -21,6 +21,8 @@
 // 8 (sorted, constant) bin centres.
 #include "psw_common.h"
 
+#include <vector>
+
 namespace psw {
 
 constexpr int FB = PSW_FUSION_BINS;
@@ -430,6 +432,60 @@ __global__ __launch_bounds__(256) void fusion_energy_kernel(FusionView f, double
     if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
+// Fused-surface points (src/pclviewer.cpp:1058-1074): voxels with
+// -1 < u < 0, as world coordinates of the voxel centres, in the reference's
+// z, y, x loop order.  Two passes over fixed voxel ranges: counts per
+// workgroup, then (after an exclusive scan of the counts) an ordered write
+// with a wave ballot prefix inside each workgroup.
+constexpr int SP_BLOCK = 256, SP_PER_THREAD = 16;
+
+__device__ inline bool on_surface(float u) { return u < 0.f && u > -1.f; }
+
+__global__ __launch_bounds__(SP_BLOCK) void surface_points_count_kernel(const float* __restrict__ u, size_t n,
+                                                                        unsigned* __restrict__ counts) {
+    __shared__ unsigned wsum[SP_BLOCK / 64];
+    const size_t base = (size_t)blockIdx.x * SP_BLOCK * SP_PER_THREAD;
+    unsigned c = 0;
+    for (int k = 0; k < SP_PER_THREAD; ++k) {
+        const size_t i = base + (size_t)k * SP_BLOCK + threadIdx.x;
+        if (i < n && on_surface(u[i])) ++c;
+    }
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+__global__ __launch_bounds__(SP_BLOCK) void surface_points_write_kernel(FusionView f, const unsigned* __restrict__ start,
+                                                                        float* __restrict__ xyz, size_t cap) {
+    __shared__ unsigned wcount[SP_BLOCK / 64];
+    const size_t base = (size_t)blockIdx.x * SP_BLOCK * SP_PER_THREAD;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned running = start[blockIdx.x];
+    for (int k = 0; k < SP_PER_THREAD; ++k) {
+        const size_t i = base + (size_t)k * SP_BLOCK + threadIdx.x;
+        const bool hit = i < f.V && on_surface(f.u[i]);
+        const unsigned long long mask = __ballot(hit);
+        if (lane == 0) wcount[wave] = (unsigned)__popcll(mask);
+        __syncthreads();
+        unsigned before = running;
+        for (int w = 0; w < wave; ++w) before += wcount[w];
+        before += (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+        if (hit && before < cap) {
+            const int x = (int)(i % f.w);
+            const size_t t = i / f.w;
+            const int y = (int)(t % f.h), z = (int)(t / f.h);
+            float cx, cy, cz;
+            voxel_centre(f, x, y, z, cx, cy, cz);
+            xyz[3 * (size_t)before] = cx;
+            xyz[3 * (size_t)before + 1] = cy;
+            xyz[3 * (size_t)before + 2] = cz;
+        }
+        running += wcount[0] + wcount[1] + wcount[2] + wcount[3];
+        __syncthreads();
+    }
+}
+
 __global__ void surface_count_kernel(const float* __restrict__ u, size_t n, unsigned long long* out) {
     __shared__ unsigned int part[256 / 64];
     unsigned int c = 0;
@@ -659,6 +715,50 @@ int psw_fusion_energy(const psw_fusion_grid* g, double lambda, double* energy, p
     double t = 0.0;
     for (int i = 0; i < FE_BLOCKS; ++i) t += h[i];
     *energy = t;
+    return PSW_OK;
+}
+
+int psw_fusion_surface_points(const psw_fusion_grid* g, float* d_xyz, int64_t capacity, int64_t* count,
+                              psw_stream_t s) {
+    int st = check_grid(g);
+    if (st != PSW_OK) return st;
+    PSW_CHECK_ARG(count, "count is NULL");
+    PSW_CHECK_ARG(capacity >= 0 && (capacity == 0 || d_xyz), "bad output buffer");
+    const FusionView f = view_of(g);
+    const size_t per = (size_t)SP_BLOCK * SP_PER_THREAD;
+    const size_t nb = (f.V + per - 1) / per;
+    PSW_CHECK_ARG(nb < (1u << 31), "grid too large");
+    unsigned* d = nullptr;
+    PSW_TRY_HIP(hipMalloc((void**)&d, 2 * nb * sizeof(unsigned)), PSW_ERR_ALLOC);
+    std::vector<unsigned> h(nb);
+    hipStream_t hs = S(s);
+    surface_points_count_kernel<<<(unsigned)nb, SP_BLOCK, 0, hs>>>(f.u, f.V, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, nb * sizeof(unsigned), hipMemcpyDeviceToHost, hs);
+    if (e == hipSuccess) e = hipStreamSynchronize(hs);
+    uint64_t total = 0;
+    if (e == hipSuccess) {
+        for (size_t b = 0; b < nb; ++b) {   // exclusive scan (counts fit: at most 4096 per block)
+            const unsigned c = h[b];
+            h[b] = (unsigned)std::min<uint64_t>(total, 0xffffffffu);
+            total += c;
+        }
+        if (total >= 0xffffffffull) {
+            (void)hipFree(d);
+            PSW_CHECK_ARG(false, "too many surface voxels");
+        }
+        if (capacity > 0 && total > 0) {
+            e = hipMemcpyAsync(d + nb, h.data(), nb * sizeof(unsigned), hipMemcpyHostToDevice, hs);
+            if (e == hipSuccess) {
+                surface_points_write_kernel<<<(unsigned)nb, SP_BLOCK, 0, hs>>>(f, d + nb, d_xyz, (size_t)capacity);
+                e = hipGetLastError();
+            }
+            if (e == hipSuccess) e = hipStreamSynchronize(hs);
+        }
+    }
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_status(e, "psw_fusion_surface_points");
+    *count = (int64_t)total;
     return PSW_OK;
 }
 

@@ -178,6 +178,7 @@ _SIGS = {
                                             c_double, c_void_p]),
     "psw_fusion_solve": (c_int, [P(FusionGrid), c_int, c_double, c_double, c_double, c_void_p]),
     "psw_fusion_energy": (c_int, [P(FusionGrid), c_double, P(c_double), c_void_p]),
+    "psw_fusion_surface_points": (c_int, [P(FusionGrid), D, ctypes.c_int64, P(ctypes.c_int64), c_void_p]),
     "psw_fusion_surface_count": (c_int, [P(FusionGrid), P(c_int64), c_void_p]),
     # TVL1
     "psw_scene_render": (c_int, [P(PswRoom), H, H, H, c_int, c_int, c_double, ctypes.c_uint64, c_float, D, D,

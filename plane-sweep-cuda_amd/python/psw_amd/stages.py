@@ -267,6 +267,7 @@ class Fusion:
         call("psw_fusion_alloc", ctypes.byref(self.g), int(w), int(h), int(d), _hp(vol))
         self.shape = (int(d), int(h), int(w))
         self.V = int(w) * int(h) * int(d)
+        self.volume = vol.copy()
 
     def clear(self, stream=None):
         call("psw_fusion_clear", ctypes.byref(self.g), stream)
@@ -298,6 +299,18 @@ class Fusion:
         e = ctypes.c_double()
         call("psw_fusion_energy", ctypes.byref(self.g), float(lambda_), ctypes.byref(e), stream)
         return e.value
+
+    def surface_points(self, stream=None):
+        """(n, 3) float32 world coordinates of the voxels with -1 < u < 0,
+        in z, y, x order."""
+        c = ctypes.c_int64()
+        call("psw_fusion_surface_points", ctypes.byref(self.g), None, 0, ctypes.byref(c), stream)
+        n = c.value
+        if n == 0:
+            return np.zeros((0, 3), np.float32)
+        d = DeviceArray((n, 3))
+        call("psw_fusion_surface_points", ctypes.byref(self.g), d.ptr, n, ctypes.byref(c), stream)
+        return d.to_host()
 
     def surface_count(self, stream=None):
         c = ctypes.c_int64()
