@@ -120,6 +120,43 @@ def run_fusion(o, thr, stream, ev):
     return res
 
 
+SPARSE_ITERS = 500
+SPARSE_BYTES_PER_PIXEL_ITER = 80   # read u 12, p 8, q 12, T 12, sparse 4; write u 12, p 8, q 12
+
+
+def run_sparse(pl, stream, ev, seed):
+    """Sparse-depth TGV2 densification at 640x480: the sweep depthmap as the
+    dense start, the scene's true depth at 5 % of the pixels as the sparse
+    input.  Per-iteration time from a 500- and a 0-iteration call."""
+    from psw_amd import stages
+    from psw_amd.device import DeviceArray
+    s = stream.handle
+    d_dense = pl.sweep_and_linearise()
+    pl.st.synchronize()
+    rng = np.random.default_rng(seed + 77)
+    gt = pl.v["d_depth"].to_host()
+    d_sparse = DeviceArray.from_host(np.where(rng.random(gt.shape) < 0.05, gt, 0).astype(np.float32))
+    ws = DeviceArray((stages.tgv_sparse_workspace_bytes(W, H) // 4,))
+    kw = dict(alpha0=2.0, alpha1=1.5, tau=0.02, sigma=1.0, theta=1.0, beta=0.0, gamma=1.0)
+
+    def timed(k):
+        ev[0].record(stream)
+        stages.tgv_sparse(pl.v["d_ref"], d_dense, d_sparse, ZFAR, k, workspace=ws, stream=s, **kw)
+        ev[1].record(stream)
+        ev[1].synchronize()
+        return ev[0].elapsed_ms(ev[1])
+
+    timed(20)
+    t_full, t_zero = timed(SPARSE_ITERS), timed(0)
+    it_ms = (t_full - t_zero) / SPARSE_ITERS
+    gbs = SPARSE_BYTES_PER_PIXEL_ITER * W * H / (it_ms * 1e-3) / 1e9
+    return {"kernel": "tgv_sparse_iter_kernel", "iter_us": it_ms * 1e3, "call_ms": t_full,
+            "setup_output_ms": t_zero, "gbs_algorithmic": gbs, "frac_hbm_peak": gbs / HBM_PEAK_GBS,
+            "bytes_per_pixel_iter": SPARSE_BYTES_PER_PIXEL_ITER,
+            "config": f"{W}x{H}, sweep depth as the dense start, true depth at 5% of pixels, {SPARSE_ITERS} "
+                      "iterations, alpha0 2 alpha1 1.5 tau 0.02 sigma 1"}
+
+
 class Pipeline:
     """Device state of the benchmarked pipeline for one seeded view."""
 
@@ -212,6 +249,10 @@ def probe(names, seed):
             else:
                 for _ in range(3):
                     pl.sweep_and_linearise(fast=(name == "sweep"))
+        elif name == "sparse":
+            if pl is None:
+                pl = Pipeline(seed, None)
+            log(json.dumps(run_sparse(pl, st, [Event() for _ in range(2)], seed)))
         elif name == "fusion":
             o, thr = fusion_workload(seed)
             ev = [Event() for _ in range(3)]
@@ -327,7 +368,7 @@ def main():
     ap.add_argument("--batch", action="store_true", help="run the uneven 32-frame batch instead (its own JSON line)")
     ap.add_argument("--batch-frames", type=int, default=32)
     ap.add_argument("--batch-estar-iters", type=int, default=20000)
-    ap.add_argument("--probe", default="", help="comma list of tgv,sweep,sweep_exact,unet,copy,fusion: run only "
+    ap.add_argument("--probe", default="", help="comma list of tgv,sweep,sweep_exact,unet,copy,fusion,sparse: run only "
                                                 "these small workloads (for rocprofv3) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -466,6 +507,10 @@ def main():
                    "unet_tflops": net_tflops, "unet_frac_bf16_peak": net_tflops / BF16_PEAK_TFLOPS,
                    "unet_fwd_gflop": warmstart.forward_flops(W, H) / 1e9},
     }
+    from psw_amd.device import Event, Stream
+    rec["sparse_tgv"] = run_sparse(pl, Stream(), [Event() for _ in range(2)], args.seed)
+    log(f"sparse TGV2: {rec['sparse_tgv']['iter_us']:.2f} us per iteration "
+        f"({rec['sparse_tgv']['gbs_algorithmic']:.0f} GB/s algorithmic)")
     if not args.no_fusion:
         t0 = time.perf_counter()
         o, thr = fusion_workload(args.seed)

@@ -363,6 +363,22 @@ int psw_tgv_get_state(psw_tgv* t, float* d_out, psw_stream_t s);
 int psw_tgv_run(psw_tgv* t, const float* K, const float* Rrel, const float* trel, int niters, int warps,
                 const psw_tgv_params* prm, const float* d_u_init, float* d_u_out, psw_stream_t s);
 
+/* Sparse-depth TGV2 densification, fused (one launch per iteration).
+ * Replaces PlaneSweep::TGVdenoiseFromSparse (src/planesweep.cpp:596-674,
+ * kernels TGV2_kernels.cu:340-365, :54-85, :399-445; work in progress in the
+ * reference and not called by its GUI).  d_ref: reference image 0..255;
+ * d_dense: the initial depthmap (the reference's `depthmap`); d_sparse:
+ * depth where known (> 0), else <= 0; output depth in d_out.  Semantics of
+ * the reference driver: u1 and u1bar are the same arrays in its U step, and
+ * P reads u, so theta has no effect.  d_workspace: NULL (allocated and freed
+ * here; the call then synchronises the stream) or at least
+ * psw_tgv_sparse_workspace_bytes(width, height) bytes of device memory. */
+size_t psw_tgv_sparse_workspace_bytes(int width, int height);
+int psw_tgv_sparse_denoise(const float* d_ref, const float* d_dense, const float* d_sparse, int width, int height,
+                           float zfar, int niters, double alpha0, double alpha1, double tau, double sigma,
+                           double theta, double beta, double gamma, void* d_workspace, size_t workspace_bytes,
+                           float* d_out, psw_stream_t s);
+
 /* ---------------------------------------------------------------------- */
 /* Stage 3: histogram volumetric fusion on an SoA voxel grid.               */
 /* ---------------------------------------------------------------------- */

@@ -59,6 +59,10 @@ _SIGS = {
     "orc_surface_count_f64": (_i64, [_vp, _i64]),
     "orc_tvl1_f32": (_i, [_vp, _vp, _i, _i, _f, _f, _i, _d, _d, _d, _d, _d, _d, _vp]),
     "orc_tvl1_f64": (_i, [_vp, _vp, _i, _i, _f, _f, _i, _d, _d, _d, _d, _d, _d, _vp]),
+    "orc_tgv_sparse_f32": (_i, [_vp, _vp, _vp, _i, _i, _f, _i, _d, _d, _d, _d, _d, _d, _d, _vp, _vp]),
+    "orc_tgv_sparse_f64": (_i, [_vp, _vp, _vp, _i, _i, _f, _i, _d, _d, _d, _d, _d, _d, _d, _vp, _vp]),
+    "orc_compute3d_f32": (None, [_vp, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp]),
+    "orc_compute3d_f64": (None, [_vp, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp]),
 }
 for _n, (_r, _a) in _SIGS.items():
     _fn = getattr(_lib, _n)
@@ -284,4 +288,30 @@ def tvl1(depth, ref, znear, zfar, niters=100, lambda_=0.3, tau=0.02, sigma=6.0, 
     fn = _lib.orc_tvl1_f32 if prec == "f32" else _lib.orc_tvl1_f64
     rc = fn(_p(depth), _p(ref), W, Hh, znear, zfar, niters, lambda_, tau, sigma, theta, beta, gamma, _p(out))
     assert rc == 0
+    return out
+
+
+def tgv_sparse(ref, dense, sparse, zfar, niters, alpha0=2.0, alpha1=1.5, tau=0.02, sigma=1.0, theta=1.0, beta=0.0,
+               gamma=1.0, prec="f32", u1=False):
+    """PlaneSweep::TGVdenoiseFromSparse (src/planesweep.cpp:596-674)."""
+    ref, dense, sparse = _c(ref), _c(dense), _c(sparse)
+    Hh, W = ref.shape
+    T = _ft(prec)
+    out = np.zeros((Hh, W), T)
+    v = np.zeros((2, Hh, W), T) if u1 else None
+    fn = _lib.orc_tgv_sparse_f32 if prec == "f32" else _lib.orc_tgv_sparse_f64
+    rc = fn(_p(ref), _p(dense), _p(sparse), W, Hh, zfar, niters, alpha0, alpha1, tau, sigma, theta, beta, gamma,
+            _p(out), _p(v))
+    assert rc == 0, rc
+    return (out, v) if u1 else out
+
+
+def compute3d(depth, K, Rref, tref, alternative=0, prec="f32"):
+    """PlaneSweep::get3Dcoordinates (src/planesweep.cpp:551-586): X, Y, Z."""
+    depth = _c(depth)
+    Hh, W = depth.shape
+    T = _ft(prec)
+    out = [np.zeros((Hh, W), T) for _ in range(3)]
+    fn = _lib.orc_compute3d_f32 if prec == "f32" else _lib.orc_compute3d_f64
+    fn(_p(depth), W, Hh, *[_p(_c(x).ravel()) for x in (K, Rref, tref)], alternative, *[_p(o) for o in out])
     return out

@@ -403,46 +403,59 @@ void linearise(TgvState<T>& S, int j, const M3<T>& K, const M3<T>& invK, const M
         }
 }
 
+/* TGV2_updateP_tensor_weighed_kernel, TGV2_kernels.cu:340-365: p from the
+ * forward differences of ub minus (u1xb, u1yb), projected to the unit ball. */
+template <class T>
+void step_p_tensor(T* Px, T* Py, const T* T1, const T* T2, const T* T3, const T* T4, const T* ub, const T* u1xb,
+                   const T* u1yb, T alpha1, T sigma, int W, int H) {
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+        for (int xx = 0; xx < W; ++xx) {
+            const size_t i = (size_t)y * W + xx;
+            const int xn = std::min(xx + 1, W - 1), yn = std::min(y + 1, H - 1);
+            double x = (double)(ub[(size_t)y * W + xn] - ub[i] - u1xb[i]);
+            double yv = (double)(ub[(size_t)yn * W + xx] - ub[i] - u1yb[i]);
+            const T as = alpha1 * sigma;
+            double dx = (double)Px[i] + (double)as * ((double)T1[i] * x + (double)T2[i] * yv);
+            double dy = (double)Py[i] + (double)as * ((double)T3[i] * x + (double)T4[i] * yv);
+            double d = (double)tmax<T>((T)1, (T)std::sqrt(dx * dx + dy * dy));
+            Px[i] = (T)(dx / d);
+            Py[i] = (T)(dy / d);
+        }
+}
+
+/* TGV2_updateQ_kernel, TGV2_kernels.cu:54-85 */
+template <class T>
+void step_q(T* Qx, T* Qy, T* Qz, T* Qw, const T* u1xb, const T* u1yb, T alpha0, T sigma, int W, int H) {
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+        for (int xx = 0; xx < W; ++xx) {
+            const size_t i = (size_t)y * W + xx;
+            const int xn = std::min(xx + 1, W - 1), yn = std::min(y + 1, H - 1);
+            T dx_u1x = u1xb[(size_t)y * W + xn] - u1xb[i];
+            T dy_u1x = u1xb[(size_t)yn * W + xx] - u1xb[i];
+            T dx_u1y = u1yb[(size_t)y * W + xn] - u1yb[i];
+            T dy_u1y = u1yb[(size_t)yn * W + xx] - u1yb[i];
+            const T as = alpha0 * sigma;
+            double dx = (double)(Qx[i] + as * dx_u1x);
+            double dy = (double)(Qy[i] + as * dy_u1y);
+            double dz = (double)(Qz[i] + as * (dy_u1x + dx_u1y) / (T)2);
+            double dw = (double)(Qw[i] + as * (dy_u1x + dx_u1y) / (T)2);
+            double d = (double)tmax<T>((T)1, (T)std::sqrt(dx * dx + dy * dy + dz * dz + dw * dw));
+            Qx[i] = (T)(dx / d);
+            Qy[i] = (T)(dy / d);
+            Qz[i] = (T)(dz / d);
+            Qw[i] = (T)(dw / d);
+        }
+}
+
 /* One primal-dual iteration, src/planesweep.cpp:485-509. */
 template <class T>
 void tgv_iteration(TgvState<T>& S, T lambda, T alpha0, T alpha1, T tau, T sigma) {
     const int W = S.W, H = S.H;
-    /* TGV2_updateP_tensor_weighed_kernel, TGV2_kernels.cu:340-365 */
-#pragma omp parallel for schedule(static)
-    for (int y = 0; y < H; ++y)
-        for (int xx = 0; xx < W; ++xx) {
-            const size_t i = (size_t)y * W + xx;
-            const int xn = std::min(xx + 1, W - 1), yn = std::min(y + 1, H - 1);
-            double x = (double)(S.ubar[(size_t)y * W + xn] - S.ubar[i] - S.u1xbar[i]);
-            double yv = (double)(S.ubar[(size_t)yn * W + xx] - S.ubar[i] - S.u1ybar[i]);
-            const T as = alpha1 * sigma;
-            double dx = (double)S.Px[i] + (double)as * ((double)S.T1[i] * x + (double)S.T2[i] * yv);
-            double dy = (double)S.Py[i] + (double)as * ((double)S.T3[i] * x + (double)S.T4[i] * yv);
-            double d = (double)tmax<T>((T)1, (T)std::sqrt(dx * dx + dy * dy));
-            S.Px[i] = (T)(dx / d);
-            S.Py[i] = (T)(dy / d);
-        }
-    /* TGV2_updateQ_kernel, TGV2_kernels.cu:54-85 */
-#pragma omp parallel for schedule(static)
-    for (int y = 0; y < H; ++y)
-        for (int xx = 0; xx < W; ++xx) {
-            const size_t i = (size_t)y * W + xx;
-            const int xn = std::min(xx + 1, W - 1), yn = std::min(y + 1, H - 1);
-            T dx_u1x = S.u1xbar[(size_t)y * W + xn] - S.u1xbar[i];
-            T dy_u1x = S.u1xbar[(size_t)yn * W + xx] - S.u1xbar[i];
-            T dx_u1y = S.u1ybar[(size_t)y * W + xn] - S.u1ybar[i];
-            T dy_u1y = S.u1ybar[(size_t)yn * W + xx] - S.u1ybar[i];
-            const T as = alpha0 * sigma;
-            double dx = (double)(S.Qx[i] + as * dx_u1x);
-            double dy = (double)(S.Qy[i] + as * dy_u1y);
-            double dz = (double)(S.Qz[i] + as * (dy_u1x + dx_u1y) / (T)2);
-            double dw = (double)(S.Qw[i] + as * (dy_u1x + dx_u1y) / (T)2);
-            double d = (double)tmax<T>((T)1, (T)std::sqrt(dx * dx + dy * dy + dz * dz + dw * dw));
-            S.Qx[i] = (T)(dx / d);
-            S.Qy[i] = (T)(dy / d);
-            S.Qz[i] = (T)(dz / d);
-            S.Qw[i] = (T)(dw / d);
-        }
+    step_p_tensor(S.Px.data(), S.Py.data(), S.T1.data(), S.T2.data(), S.T3.data(), S.T4.data(), S.ubar.data(),
+                  S.u1xbar.data(), S.u1ybar.data(), alpha1, sigma, W, H);
+    step_q(S.Qx.data(), S.Qy.data(), S.Qz.data(), S.Qw.data(), S.u1xbar.data(), S.u1ybar.data(), alpha0, sigma, W, H);
     /* prodsum := 0, then TGV2_updateR_kernel per source, TGV2_kernels.cu:87-105 */
     const T sl = sigma * lambda;
 #pragma omp parallel for schedule(static)
@@ -542,6 +555,106 @@ int tgv(const float* ref, const float* src, int W, int H, int N, const float* Kf
             }
     }
     return 0;
+}
+
+/* TGV2_updateU_sparseDepthTensor_kernel, TGV2_kernels.cu:399-432: the
+ * primal step with the sparse-depth data term (prox of (w/2)(u - Ds)^2) and
+ * the over-relaxation of (u, u1) into the bar arrays.  Per pixel in program
+ * order, so that aliased u1 / u1bar pointers behave as in the reference's
+ * driver (the bar write then reads the fresh u1). */
+template <class T>
+void step_u_sparse_tensor(T* u, T* u1x, T* u1y, T* ubar, T* u1xbar, T* u1ybar, const T* T1, const T* T2,
+                          const T* T3, const T* T4, const T* Px, const T* Py, const T* Qx, const T* Qy, const T* Qz,
+                          const T* w, const T* Ds, T alpha0, T alpha1, T tau, T theta, int W, int H) {
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t i = (size_t)y * W + x;
+            const int xp = std::max(x - 1, 0), yp = std::max(y - 1, 0);
+            const size_t ixp = (size_t)y * W + xp, iyp = (size_t)yp * W + x;
+            const T c_px = Px[i], c_py = Py[i], xp_px = Px[ixp], yp_px = Px[iyp], xp_py = Py[ixp], yp_py = Py[iyp];
+            u[i] = (u[i] + tau * (alpha1 * (T1[i] * (c_px - xp_px) + T2[i] * (c_py - xp_py) + T3[i] * (c_px - yp_px) +
+                                            T4[i] * (c_py - yp_py)) +
+                                  w[i] * Ds[i])) /
+                   ((T)1 + tau * w[i]);
+            u1x[i] = u1x[i] - tau * (-alpha1 * (T1[i] * c_px + T2[i] * c_py) - alpha0 * (Qx[i] - Qx[ixp] + Qz[i] - Qz[iyp]));
+            u1y[i] = u1y[i] - tau * (-alpha1 * (T3[i] * c_px + T4[i] * c_py) - alpha0 * (Qz[i] - Qz[ixp] + Qy[i] - Qy[iyp]));
+            ubar[i] = u[i] + theta * (u[i] - ubar[i]);
+            u1xbar[i] = u1x[i] + theta * (u1x[i] - u1xbar[i]);
+            u1ybar[i] = u1y[i] + theta * (u1y[i] - u1ybar[i]);
+        }
+}
+
+/* PlaneSweep::TGVdenoiseFromSparse, src/planesweep.cpp:596-674.
+ * weights = (sparse > 0) (calculateWeights_sparseDepth, TGV2_kernels.cu:
+ * 434-445); Ds = sparse / zfar and u = ubar = dense / zfar (element_scale by
+ * 1.f/zfar); the reference image / 255 gives the tensor; then per iteration
+ * P (on u and u1), Q (on u1) and the sparse U step, whose u1 and u1bar
+ * arguments are the same arrays (vxbar, vybar); output u * zfar.  p, q and
+ * u1 start at zero.  ubar is written but never read, so theta does not
+ * change the result. */
+template <class T>
+int tgv_sparse(const float* ref, const float* dense, const float* sparse, int W, int H, float zfar, int niters,
+               double alpha0, double alpha1, double tau, double sigma, double theta, double beta, double gamma, T* out,
+               T* u1_out) {
+    if (W < 2 || H < 2 || niters < 0) return 1;
+    const size_t n = (size_t)W * H;
+    std::vector<T> rf(n), u(n), ubar(n), vx(n, 0), vy(n, 0), Px(n, 0), Py(n, 0), Qx(n, 0), Qy(n, 0), Qz(n, 0),
+        Qw(n, 0), w(n), Ds(n), T1(n), T2(n), T3(n), T4(n);
+    const T sz = (T)1 / (T)zfar, s255 = (T)1 / (T)255;
+    for (size_t i = 0; i < n; ++i) {
+        Ds[i] = (T)sparse[i];
+        w[i] = Ds[i] > (T)0 ? (T)1 : (T)0;
+        Ds[i] = Ds[i] * sz;
+        ubar[i] = (T)dense[i] * sz;
+        u[i] = ubar[i];
+        rf[i] = (T)ref[i] * s255;
+    }
+    tensor(rf.data(), W, H, (T)beta, (T)gamma, T1.data(), T2.data(), T3.data(), T4.data());
+    const T a0 = (T)alpha0, a1 = (T)alpha1, tf = (T)tau, sf = (T)sigma, th = (T)theta;
+    for (int it = 0; it < niters; ++it) {
+        step_p_tensor(Px.data(), Py.data(), T1.data(), T2.data(), T3.data(), T4.data(), u.data(), vx.data(), vy.data(),
+                      a1, sf, W, H);
+        step_q(Qx.data(), Qy.data(), Qz.data(), Qw.data(), vx.data(), vy.data(), a0, sf, W, H);
+        step_u_sparse_tensor(u.data(), vx.data(), vy.data(), ubar.data(), vx.data(), vy.data(), T1.data(), T2.data(),
+                             T3.data(), T4.data(), Px.data(), Py.data(), Qx.data(), Qy.data(), Qz.data(), w.data(),
+                             Ds.data(), a0, a1, tf, th, W, H);
+    }
+    const T zf = (T)zfar;
+    for (size_t i = 0; i < n; ++i) out[i] = u[i] * zf;
+    if (u1_out) {
+        std::copy(vx.begin(), vx.end(), u1_out);
+        std::copy(vy.begin(), vy.end(), u1_out + n);
+    }
+    return 0;
+}
+
+/* compute3D_kernel, src/kernels.cu:405-429, as PlaneSweep::get3Dcoordinates
+ * (src/planesweep.cpp:551-586) drives it: (Rr, t) = RelativeMatrices(Rref,
+ * tref, I, 0), X = Rr * ((z * invK) * (x+1, y+1, 1)) + t for z < 5.5, else
+ * z := -9 with x, y left as they were (zero here). */
+template <class T>
+void compute3d(const float* depth, int W, int H, const float* Kf, const float* Rref, const float* tref, int alt, T* X,
+               T* Y, T* Z) {
+    M3<T> Rr, I;
+    V3<T> t;
+    for (int r = 0; r < 3; ++r) I.r[r] = mk<T>(r == 0, r == 1, r == 2);
+    relative(load_m<T>(Rref), load_v<T>(tref), I, mk<T>(0, 0, 0), alt != 0, Rr, t);
+    const M3<T> invK = inv(load_m<T>(Kf));
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t i = (size_t)y * W + x;
+            const T z = (T)depth[i];
+            X[i] = 0; Y[i] = 0;
+            if ((double)z < 5.5) {
+                M3<T> zK;
+                for (int r = 0; r < 3; ++r) zK.r[r] = z * invK.r[r];
+                const V3<T> a = mv(Rr, mv(zK, mk<T>((T)(x + 1), (T)(y + 1), (T)1))) + t;
+                X[i] = a.x; Y[i] = a.y; Z[i] = a.z;
+            } else {
+                Z[i] = (T)-9;
+            }
+        }
 }
 
 template <class S>
@@ -982,6 +1095,28 @@ int orc_tvl1_f32(const float* depth, const float* ref, int W, int H, float znear
 int orc_tvl1_f64(const float* depth, const float* ref, int W, int H, float znear, float zfar, int niters,
                  double lambda, double tau, double sigma, double theta, double beta, double gamma, double* out) {
     return tvl1<double>(depth, ref, W, H, znear, zfar, niters, lambda, tau, sigma, theta, beta, gamma, out);
+}
+
+int orc_tgv_sparse_f32(const float* ref, const float* dense, const float* sparse, int W, int H, float zfar,
+                       int niters, double alpha0, double alpha1, double tau, double sigma, double theta, double beta,
+                       double gamma, float* out, float* u1_out) {
+    return tgv_sparse<float>(ref, dense, sparse, W, H, zfar, niters, alpha0, alpha1, tau, sigma, theta, beta, gamma,
+                             out, u1_out);
+}
+int orc_tgv_sparse_f64(const float* ref, const float* dense, const float* sparse, int W, int H, float zfar,
+                       int niters, double alpha0, double alpha1, double tau, double sigma, double theta, double beta,
+                       double gamma, double* out, double* u1_out) {
+    return tgv_sparse<double>(ref, dense, sparse, W, H, zfar, niters, alpha0, alpha1, tau, sigma, theta, beta, gamma,
+                              out, u1_out);
+}
+
+void orc_compute3d_f32(const float* depth, int W, int H, const float* K, const float* Rref, const float* tref,
+                       int alt, float* X, float* Y, float* Z) {
+    compute3d<float>(depth, W, H, K, Rref, tref, alt, X, Y, Z);
+}
+void orc_compute3d_f64(const float* depth, int W, int H, const float* K, const float* Rref, const float* tref,
+                       int alt, double* X, double* Y, double* Z) {
+    compute3d<double>(depth, W, H, K, Rref, tref, alt, X, Y, Z);
 }
 
 }  // extern "C"

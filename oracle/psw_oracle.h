@@ -187,6 +187,25 @@ int orc_tvl1_f64(const float* depth, const float* ref, int W, int H, float znear
                  int niters, double lambda, double tau, double sigma, double theta,
                  double beta, double gamma, double* out);
 
+/* ---- sparse-depth TGV2 and 3-D coordinates ----------------------------- */
+
+/* PlaneSweep::TGVdenoiseFromSparse (src/planesweep.cpp:596-674): reference
+ * image 0..255, dense depth (the initial u), sparse depth (> 0 where
+ * known); output depth; u1_out (2 x W*H, may be NULL) gets u1x, u1y. */
+int orc_tgv_sparse_f32(const float* ref, const float* dense, const float* sparse, int W, int H, float zfar,
+                       int niters, double alpha0, double alpha1, double tau, double sigma, double theta,
+                       double beta, double gamma, float* out, float* u1_out);
+int orc_tgv_sparse_f64(const float* ref, const float* dense, const float* sparse, int W, int H, float zfar,
+                       int niters, double alpha0, double alpha1, double tau, double sigma, double theta,
+                       double beta, double gamma, double* out, double* u1_out);
+
+/* PlaneSweep::get3Dcoordinates + compute3D_kernel (src/planesweep.cpp:
+ * 551-586, src/kernels.cu:405-429): world coordinates of a depthmap. */
+void orc_compute3d_f32(const float* depth, int W, int H, const float* K, const float* Rref, const float* tref,
+                       int alt, float* X, float* Y, float* Z);
+void orc_compute3d_f64(const float* depth, int W, int H, const float* K, const float* Rref, const float* tref,
+                       int alt, double* X, double* Y, double* Z);
+
 #ifdef __cplusplus
 }
 #endif

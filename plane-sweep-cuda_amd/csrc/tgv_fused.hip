@@ -19,6 +19,7 @@
 // P and Q steps become float); FP64 is used only in the energy reductions
 // that decide stopping.
 #include "psw_common.h"
+#include "tgv_point.h"
 
 #include <vector>
 
@@ -123,18 +124,8 @@ __global__ void tgv_tensor_kernel(float* __restrict__ T, const float* __restrict
     if (x >= W || y >= H) return;
     const size_t n = (size_t)W * H, i = (size_t)y * W + x;
     const int xn = min(x + 1, W - 1), yn = min(y + 1, H - 1);
-    float gx = img[(size_t)y * W + xn] - img[i];
-    float gy = img[(size_t)yn * W + x] - img[i];
-    const float d = sqrtf(gx * gx + gy * gy);
-    float t11 = 1.f, t12 = 0.f, t22 = 1.f;
-    if (d > 0.f) {
-        gx = gx / d;
-        gy = gy / d;
-        const float k = expf(-beta * powf(d, gamma));
-        t11 = k * gx * gx + gy * gy;
-        t12 = (k - 1.f) * gx * gy;
-        t22 = k * gy * gy + gx * gx;
-    }
+    float t11, t12, t22;
+    tensor_point(img[(size_t)y * W + xn] - img[i], img[(size_t)yn * W + x] - img[i], beta, gamma, t11, t12, t22);
     T[i] = t11;
     T[n + i] = t12;
     T[2 * n + i] = t22;
@@ -254,32 +245,8 @@ __global__ void tgv_mask_kernel(const float* __restrict__ Iu, int W, int H, int 
 // ---- the fused primal-dual iteration --------------------------------------
 
 // Point updates shared by the one- and two-iteration kernels (same
-// expression order, so both give identical results).
-// TGV2_updateP_tensor_weighed + TGV2_updateQ (q_w == q_z) at one position:
-// ub*/u1xb*/u1yb* are ubar at the position (c), its right (r) and lower (d)
-// neighbour (the position itself at the image's last column / row).
-__device__ __forceinline__ void pq_update(float& px, float& py, float& qx, float& qy, float& qz, float t11, float t12,
-                                          float t22, float ubc, float ubr, float ubd, float u1xc, float u1xr,
-                                          float u1xd, float u1yc, float u1yr, float u1yd, float as1, float as0) {
-    const float gxv = ubr - ubc - u1xc;
-    const float gyv = ubd - ubc - u1yc;
-    px = px + as1 * (t11 * gxv + t12 * gyv);
-    py = py + as1 * (t12 * gxv + t22 * gyv);
-    const float dp = fmaxf(1.f, sqrtf(px * px + py * py));
-    px = px / dp;
-    py = py / dp;
-    const float dx_u1x = u1xr - u1xc;
-    const float dy_u1x = u1xd - u1xc;
-    const float dx_u1y = u1yr - u1yc;
-    const float dy_u1y = u1yd - u1yc;
-    qx = qx + as0 * dx_u1x;
-    qy = qy + as0 * dy_u1y;
-    qz = qz + as0 * (dy_u1x + dx_u1y) / 2.0f;
-    const float dq = fmaxf(1.f, sqrtf(qx * qx + qy * qy + qz * qz + qz * qz));
-    qx = qx / dq;
-    qy = qy / dq;
-    qz = qz / dq;
-}
+// expression order, so both give identical results); the dual update
+// pq_update is in tgv_point.h.
 
 // TGV2_updateR at one pixel and source (uses u_k, not ubar).
 __device__ __forceinline__ float r_update(float r, float it, float du, float iu, float sl) {

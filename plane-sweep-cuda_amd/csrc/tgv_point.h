@@ -1,0 +1,51 @@
+// tgv_point.h -- TGV2 point updates (diffusion tensor, duals) shared by the dense
+// (tgv_fused.hip) and sparse-depth (tgv_sparse.hip) fused kernels.
+#pragma once
+
+namespace psw {
+
+// TGV2_updateP_tensor_weighed + TGV2_updateQ (q_w == q_z) at one position:
+// ub*/u1xb*/u1yb* are ubar at the position (c), its right (r) and lower (d)
+// neighbour (the position itself at the image's last column / row).
+__device__ __forceinline__ void pq_update(float& px, float& py, float& qx, float& qy, float& qz, float t11, float t12,
+                                          float t22, float ubc, float ubr, float ubd, float u1xc, float u1xr,
+                                          float u1xd, float u1yc, float u1yr, float u1yd, float as1, float as0) {
+    const float gxv = ubr - ubc - u1xc;
+    const float gyv = ubd - ubc - u1yc;
+    px = px + as1 * (t11 * gxv + t12 * gyv);
+    py = py + as1 * (t12 * gxv + t22 * gyv);
+    const float dp = fmaxf(1.f, sqrtf(px * px + py * py));
+    px = px / dp;
+    py = py / dp;
+    const float dx_u1x = u1xr - u1xc;
+    const float dy_u1x = u1xd - u1xc;
+    const float dx_u1y = u1yr - u1yc;
+    const float dy_u1y = u1yd - u1yc;
+    qx = qx + as0 * dx_u1x;
+    qy = qy + as0 * dy_u1y;
+    qz = qz + as0 * (dy_u1x + dx_u1y) / 2.0f;
+    const float dq = fmaxf(1.f, sqrtf(qx * qx + qy * qy + qz * qz + qz * qz));
+    qx = qx / dq;
+    qy = qy / dq;
+    qz = qz / dq;
+}
+
+// Anisotropic_diffusion_tensor_kernel (TGV2_kernels.cu:257-298) at one
+// pixel from the forward differences (gx, gy) of the scaled image; T21 = T12.
+__device__ __forceinline__ void tensor_point(float gx, float gy, float beta, float gamma, float& t11, float& t12,
+                                             float& t22) {
+    const float d = sqrtf(gx * gx + gy * gy);
+    t11 = 1.f;
+    t12 = 0.f;
+    t22 = 1.f;
+    if (d > 0.f) {
+        gx = gx / d;
+        gy = gy / d;
+        const float k = expf(-beta * powf(d, gamma));
+        t11 = k * gx * gx + gy * gy;
+        t12 = (k - 1.f) * gx * gy;
+        t22 = k * gy * gy + gx * gx;
+    }
+}
+
+}  // namespace psw

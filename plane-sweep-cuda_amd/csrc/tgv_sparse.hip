@@ -1,0 +1,247 @@
+// tgv_sparse.hip -- sparse-depth TGV2 densification for gfx950
+// (PlaneSweep::TGVdenoiseFromSparse, src/planesweep.cpp:596-674).
+//
+// The reference runs three launches per iteration -- P (TGV2_kernels.cu:
+// 340-365), Q (:54-85) and the sparse-depth U step (:399-432) -- over 19
+// full-frame images.  Here one launch per iteration does all three for a
+// 64x8 tile:
+//   phase 1: u, u1x, u1y of iteration k on the tile plus a one-pixel halo
+//            into LDS;
+//   phase 2: p, q of iteration k+1 on the tile plus its left/top halo, into
+//            LDS and, for the tile, to HBM;
+//   phase 3: the primal step of the tile pixels from the LDS duals.
+// Buffers read at neighbours by other tiles are never written in place: u
+// (3 planes), p (2) and q (3) alternate between two slots by iteration
+// parity.
+//
+// Semantics are those of the reference driver.  It passes the same arrays
+// as u1 and u1bar to its U kernel, so in program order u1bar is the fresh u1
+// after every step, and its P step reads u, not ubar; ubar is written but
+// never read, so theta has no effect on the result (the entry point takes it
+// for the interface).  The data weight w = (sparse > 0) and the data term
+// Ds = sparse * (1.f / zfar) come from the caller's sparse depth on the fly
+// (calculateWeights_sparseDepth and element_scale, same roundings).
+//
+// Arithmetic is FP32 throughout (the reference's double temporaries in P and
+// Q become float, as in the dense solver).  HBM per pixel and iteration:
+// reads u 12 B, p 8, q 12, T 12, sparse 4 (48 B); writes u 12, p 8, q 12
+// (32 B): 80 B in all.
+#include "psw_common.h"
+#include "tgv_point.h"
+
+namespace psw {
+
+constexpr int TS_TX = 64, TS_TY = 8, TS_THREADS = 512;
+
+struct SparsePtrs {
+    const float* ucur;    // [3][n]: u, u1x, u1y at iteration k
+    float* unext;         // [3][n] at k+1
+    const float* pcur;    // [2][n]
+    float* pnext;
+    const float* qcur;    // [3][n]: qx, qy, qz (q_w == q_z)
+    float* qnext;
+    const float* T;       // [3][n]: T11, T12 (= T21), T22
+    const float* sparse;  // [n] the caller's sparse depth, unscaled
+    int W, H;
+    float alpha0, alpha1, tau, sigma, sz;
+};
+
+// Per-pixel set-up: the tensor of ref / 255 (Anisotropic_diffusion_tensor,
+// TGV2_kernels.cu:257-298, with the image scaled on load: the same float
+// products the reference's element_scale stores), u = dense / zfar and zero
+// u1, p, q in slot 0.
+__global__ void tgv_sparse_setup_kernel(float* __restrict__ T, float* __restrict__ U, float* __restrict__ P,
+                                        float* __restrict__ Q, const float* __restrict__ ref,
+                                        const float* __restrict__ dense, int W, int H, float beta, float gamma,
+                                        float sz) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const size_t n = (size_t)W * H, i = (size_t)y * W + x;
+    const float sc = 1.f / 255.f;
+    const int xn = min(x + 1, W - 1), yn = min(y + 1, H - 1);
+    const float c = ref[i] * sc;
+    float t11, t12, t22;
+    tensor_point(ref[(size_t)y * W + xn] * sc - c, ref[(size_t)yn * W + x] * sc - c, beta, gamma, t11, t12, t22);
+    T[i] = t11;
+    T[n + i] = t12;
+    T[2 * n + i] = t22;
+    U[i] = dense[i] * sz;
+    U[n + i] = 0.f;
+    U[2 * n + i] = 0.f;
+    P[i] = 0.f;
+    P[n + i] = 0.f;
+    Q[i] = 0.f;
+    Q[n + i] = 0.f;
+    Q[2 * n + i] = 0.f;
+}
+
+__global__ __launch_bounds__(TS_THREADS) void tgv_sparse_iter_kernel(SparsePtrs a) {
+    constexpr int HX = TS_TX + 2, HY = TS_TY + 2;   // u region: x0-1..x0+TX, y0-1..y0+TY
+    constexpr int DX = TS_TX + 1, DY = TS_TY + 1;   // dual region: x0-1..x0+TX-1, y0-1..y0+TY-1
+    __shared__ float sU[3][HY][HX];
+    __shared__ float sP[2][DY][DX];
+    __shared__ float sQ[3][DY][DX];
+
+    const int W = a.W, H = a.H;
+    const size_t n = (size_t)W * H;
+    const int tiles_x = (W + TS_TX - 1) / TS_TX;
+    const int ntiles = tiles_x * ((H + TS_TY - 1) / TS_TY);
+    const int tile = xcd_tile(blockIdx.x, ntiles);
+    const int x0 = (tile % tiles_x) * TS_TX, y0 = (tile / tiles_x) * TS_TY;
+    const int tid = threadIdx.x;
+
+    // phase 1: u, u1 at clamped coordinates (the reference's min(x+1, W-1)
+    // neighbours, TGV2_kernels.cu:352-353).
+    for (int i = tid; i < HX * HY; i += TS_THREADS) {
+        const int hy = i / HX, hx = i - hy * HX;
+        const int gx = min(max(x0 - 1 + hx, 0), W - 1);
+        const int gy = min(max(y0 - 1 + hy, 0), H - 1);
+        const size_t g = (size_t)gy * W + gx;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) sU[f][hy][hx] = a.ucur[f * n + g];
+    }
+    __syncthreads();
+
+    // phase 2: p, q on the tile plus its left/top halo.
+    const float as1 = a.alpha1 * a.sigma, as0 = a.alpha0 * a.sigma;
+    for (int i = tid; i < DX * DY; i += TS_THREADS) {
+        const int dy = i / DX, dx = i - dy * DX;
+        const int vx = x0 - 1 + dx, vy = y0 - 1 + dy;
+        const int gx = min(max(vx, 0), W - 1), gy = min(max(vy, 0), H - 1);
+        const size_t g = (size_t)gy * W + gx;
+        float px = a.pcur[g], py = a.pcur[n + g];
+        float qx = a.qcur[g], qy = a.qcur[n + g], qz = a.qcur[2 * n + g];
+        pq_update(px, py, qx, qy, qz, a.T[g], a.T[n + g], a.T[2 * n + g], sU[0][dy][dx], sU[0][dy][dx + 1],
+                  sU[0][dy + 1][dx], sU[1][dy][dx], sU[1][dy][dx + 1], sU[1][dy + 1][dx], sU[2][dy][dx],
+                  sU[2][dy][dx + 1], sU[2][dy + 1][dx], as1, as0);
+        sP[0][dy][dx] = px;
+        sP[1][dy][dx] = py;
+        sQ[0][dy][dx] = qx;
+        sQ[1][dy][dx] = qy;
+        sQ[2][dy][dx] = qz;
+        if (dx > 0 && dy > 0 && vx < W && vy < H) {
+            a.pnext[g] = px;
+            a.pnext[n + g] = py;
+            a.qnext[g] = qx;
+            a.qnext[n + g] = qy;
+            a.qnext[2 * n + g] = qz;
+        }
+    }
+    __syncthreads();
+
+    // phase 3: TGV2_updateU_sparseDepthTensor at the tile pixels.
+    for (int i = tid; i < TS_TX * TS_TY; i += TS_THREADS) {
+        const int ty = i / TS_TX, tx = i - ty * TS_TX;
+        const int x = x0 + tx, y = y0 + ty;
+        if (x >= W || y >= H) continue;
+        const size_t g = (size_t)y * W + x;
+        const int cx = tx + 1, cy = ty + 1;       // dual (and u) index of this pixel
+        const int lx = x > 0 ? cx - 1 : cx;       // max(x-1, 0)
+        const int uy = y > 0 ? cy - 1 : cy;       // max(y-1, 0)
+        const float u = sU[0][cy][cx], u1x = sU[1][cy][cx], u1y = sU[2][cy][cx];
+        const float t11 = a.T[g], t12 = a.T[n + g], t22 = a.T[2 * n + g];
+        const float c_px = sP[0][cy][cx], c_py = sP[1][cy][cx];
+        const float xp_px = sP[0][cy][lx], xp_py = sP[1][cy][lx];
+        const float yp_px = sP[0][uy][cx], yp_py = sP[1][uy][cx];
+        const float sp = a.sparse[g];
+        const float wv = sp > 0.f ? 1.f : 0.f;
+        const float ds = sp * a.sz;
+        const float un = (u + a.tau * (a.alpha1 * (t11 * (c_px - xp_px) + t12 * (c_py - xp_py) +
+                                                   t12 * (c_px - yp_px) + t22 * (c_py - yp_py)) +
+                                       wv * ds)) /
+                         (1.f + a.tau * wv);
+        const float u1xn = u1x - a.tau * (-a.alpha1 * (t11 * c_px + t12 * c_py) -
+                                          a.alpha0 * (sQ[0][cy][cx] - sQ[0][cy][lx] + sQ[2][cy][cx] - sQ[2][uy][cx]));
+        const float u1yn = u1y - a.tau * (-a.alpha1 * (t12 * c_px + t22 * c_py) -
+                                          a.alpha0 * (sQ[2][cy][cx] - sQ[2][cy][lx] + sQ[1][cy][cx] - sQ[1][uy][cx]));
+        a.unext[g] = un;
+        a.unext[n + g] = u1xn;
+        a.unext[2 * n + g] = u1yn;
+    }
+}
+
+__global__ void tgv_sparse_output_kernel(float* __restrict__ out, const float* __restrict__ u, size_t n, float zfar) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = u[i] * zfar;
+}
+
+// Workspace: T 3n, then two slots of (u 3n, p 2n, q 3n).
+static size_t sparse_floats(size_t n) { return 3 * n + 2 * 8 * n; }
+
+}  // namespace psw
+
+using namespace psw;
+
+extern "C" {
+
+size_t psw_tgv_sparse_workspace_bytes(int width, int height) {
+    if (width <= 0 || height <= 0) return 0;
+    return sparse_floats((size_t)width * height) * sizeof(float);
+}
+
+int psw_tgv_sparse_denoise(const float* d_ref, const float* d_dense, const float* d_sparse, int width, int height,
+                           float zfar, int niters, double alpha0, double alpha1, double tau, double sigma,
+                           double theta, double beta, double gamma, void* d_workspace, size_t workspace_bytes,
+                           float* d_out, psw_stream_t s) {
+    (void)theta;
+    PSW_CHECK_ARG(d_ref && d_dense && d_sparse && d_out, "NULL image pointer");
+    PSW_CHECK_ARG(width > 0 && height > 0 && (size_t)width * height < (1ull << 31), "bad image size");
+    PSW_CHECK_ARG(niters >= 0, "niters < 0");
+    PSW_CHECK_ARG(zfar > 0.f && std::isfinite(zfar), "zfar must be positive and finite");
+    const size_t n = (size_t)width * height;
+    const size_t need = sparse_floats(n) * sizeof(float);
+    PSW_CHECK_ARG(!d_workspace || workspace_bytes >= need, "workspace too small");
+    float* ws = static_cast<float*>(d_workspace);
+    bool owned = false;
+    if (!ws) {
+        PSW_TRY_HIP(hipMalloc((void**)&ws, need), PSW_ERR_ALLOC);
+        owned = true;
+    }
+    float* T = ws;
+    float* slot[2] = {ws + 3 * n, ws + 11 * n};   // each: u 3n, p 2n, q 3n
+    hipStream_t hs = S(s);
+    const float sz = 1.f / zfar;
+    const dim3 blk(64, 4), grd((width + 63) / 64, (height + 3) / 4);
+    tgv_sparse_setup_kernel<<<grd, blk, 0, hs>>>(T, slot[0], slot[0] + 3 * n, slot[0] + 5 * n, d_ref, d_dense, width,
+                                                  height, (float)beta, (float)gamma, sz);
+    hipError_t e = hipGetLastError();
+    const int ntiles = ((width + TS_TX - 1) / TS_TX) * ((height + TS_TY - 1) / TS_TY);
+    SparsePtrs a;
+    a.T = T;
+    a.sparse = d_sparse;
+    a.W = width;
+    a.H = height;
+    a.alpha0 = (float)alpha0;
+    a.alpha1 = (float)alpha1;
+    a.tau = (float)tau;
+    a.sigma = (float)sigma;
+    a.sz = sz;
+    for (int k = 0; k < niters && e == hipSuccess; ++k) {
+        float* c = slot[k & 1];
+        float* nx = slot[(k + 1) & 1];
+        a.ucur = c;
+        a.unext = nx;
+        a.pcur = c + 3 * n;
+        a.pnext = nx + 3 * n;
+        a.qcur = c + 5 * n;
+        a.qnext = nx + 5 * n;
+        tgv_sparse_iter_kernel<<<ntiles, TS_THREADS, 0, hs>>>(a);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+        tgv_sparse_output_kernel<<<blocks, 256, 0, hs>>>(d_out, slot[niters & 1], n, zfar);
+        e = hipGetLastError();
+    }
+    if (owned) {
+        // the frees wait for the launches above; report their errors first
+        const hipError_t e2 = hipStreamSynchronize(hs);
+        if (e == hipSuccess) e = e2;
+        (void)hipFree(ws);
+    }
+    if (e != hipSuccess) return hip_status(e, "psw_tgv_sparse_denoise");
+    return PSW_OK;
+}
+
+}  // extern "C"

@@ -339,12 +339,40 @@ bool PlaneSweep::TGVSolve(int max_iters, int check_every, double tol, double e_s
     return true;
 }
 
-bool PlaneSweep::TGVdenoiseFromSparse(int, char**, const CamImage<float>&, const unsigned int, const double,
-                                      const double, const double, const double, const double, const double,
-                                      const double) {
-    lasterror = "TGVdenoiseFromSparse is work in progress in the reference and is not provided";
-    std::cerr << lasterror << std::endl;
-    return false;
+bool PlaneSweep::TGVdenoiseFromSparse(int argc, char** argv, const CamImage<float>& depth, const unsigned int niters,
+                                      const double alpha0, const double alpha1, const double tau, const double sigma,
+                                      const double theta, const double beta, const double gamma) {
+    Timer tm;
+    std::printf("\nStarting TGV denoising...\n\n");
+    if (deviceInit(argc, argv) == NO_CUDA_DEVICE) return false;
+    const int w = (int)HostRef.width(), h = (int)HostRef.height();
+    const size_t n = (size_t)w * h;
+    if (w < 1 || h < 1 || depth.width() != (size_t)w || depth.height() != (size_t)h ||
+        depthmap.width() != (size_t)w || depthmap.height() != (size_t)h)
+        return fail("TGVdenoiseFromSparse", PSW_ERR_ARG);
+    depthmapTGV.reset(w, h);
+    if (d_depthmap) {   // holds the result afterwards, as in the reference
+        psw_free(d_depthmap);
+        d_depthmap = nullptr;
+    }
+    void* p = nullptr;
+    int st = psw_malloc(&p, n * 4);
+    if (st != PSW_OK) return fail("TGVdenoiseFromSparse", st);
+    d_depthmap = static_cast<float*>(p);
+    DevBuf dref(n * 4), ddense(n * 4), dsparse(n * 4);
+    for (DevBuf* b : {&dref, &ddense, &dsparse})
+        if (b->status != PSW_OK) return fail("TGVdenoiseFromSparse", b->status);
+    st = psw_memcpy_htod(dref.p, HostRef.data(), n * 4, nullptr);
+    if (st == PSW_OK) st = psw_memcpy_htod(ddense.p, depthmap.data(), n * 4, nullptr);
+    if (st == PSW_OK) st = psw_memcpy_htod(dsparse.p, depth.data(), n * 4, nullptr);
+    if (st == PSW_OK)
+        st = psw_tgv_sparse_denoise(dref.f(), ddense.f(), dsparse.f(), w, h, zfar, (int)niters, alpha0, alpha1, tau,
+                                    sigma, theta, beta, gamma, nullptr, 0, d_depthmap, nullptr);
+    if (st == PSW_OK) st = psw_memcpy_dtoh(depthmapTGV.data(), d_depthmap, n * 4, nullptr);
+    if (st != PSW_OK) return fail("TGVdenoiseFromSparse", st);
+    ConvertDepthtoUChar(depthmapTGV, depthmap8uTGV);
+    std::cout << "Time taken for the TGV to complete is " << tm.ms() << "ms\n\n";
+    return true;
 }
 
 void PlaneSweep::get3Dcoordinates(CamImage<float>*& x, CamImage<float>*& y, CamImage<float>*& z) {
@@ -364,6 +392,10 @@ void PlaneSweep::get3Dcoordinates(CamImage<float>*& x, CamImage<float>*& y, CamI
     invK.toArray(iK);
     int st = Px.status | Py.status | X.status;
     if (st == PSW_OK) st = psw_memcpy_htod(X.p, depthmapdenoised.data(), n * 4, nullptr);
+    // x, y of pixels at or beyond the depth limit are not written by the
+    // kernel (uninitialised in the reference); they read 0 here.
+    if (st == PSW_OK) st = psw_fill_f32(Px.f(), 0.f, n, nullptr);
+    if (st == PSW_OK) st = psw_fill_f32(Py.f(), 0.f, n, nullptr);
     const psw_dim3 zd = {0, 0, 0};
     if (st == PSW_OK) st = psw_compute3D(Px.f(), Py.f(), X.f(), R9, t3, iK, w, h, zd, zd, nullptr);
     coord_x.reset(w, h);

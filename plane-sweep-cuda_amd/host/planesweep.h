@@ -77,8 +77,10 @@ public:
              const double alpha0 = DEFAULT_TGV_ALPHA0, const double alpha1 = DEFAULT_TGV_ALPHA1,
              const double tau = DEFAULT_TGV_TAU, const double sigma = DEFAULT_TGV_SIGMA,
              const double beta = DEFAULT_TGV_BETA, const double gamma = DEFAULT_TGV_GAMMA);
-    // Work in progress in the reference (planesweep.cpp:596-674, in/out
-    // aliasing); not provided: returns false with a message.
+    // PlaneSweep::TGVdenoiseFromSparse (planesweep.cpp:596-674, work in
+    // progress in the reference): densifies `depth` (> 0 where known) from
+    // the current depthmap with the fused sparse-depth TGV2 kernel; result in
+    // depthmapTGV and in the device buffer d_depthmap.
     bool TGVdenoiseFromSparse(int argc, char** argv, const CamImage<float>& depth, const unsigned int niters,
                               const double alpha0, const double alpha1, const double tau, const double sigma,
                               const double theta, const double beta, const double gamma);

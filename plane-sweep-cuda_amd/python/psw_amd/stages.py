@@ -256,6 +256,30 @@ def tgv_run(ref, src, K, Rref, tref, Rsrc, tsrc, niters, warps, prm, u_init=None
     return u
 
 
+def tgv_sparse(ref, dense, sparse, zfar, niters, alpha0=2.0, alpha1=1.5, tau=0.02, sigma=1.0, theta=1.0, beta=0.0,
+               gamma=1.0, workspace=None, stream=None):
+    """PlaneSweep::TGVdenoiseFromSparse (planesweep.cpp:596-674) on the fused
+    kernel: reference image 0..255, dense initial depth, sparse depth (> 0
+    where known).  Inputs may be host arrays or DeviceArrays; returns a
+    DeviceArray when they are device-resident, else a host array.
+    workspace: a DeviceArray of at least tgv_sparse_workspace_bytes() bytes,
+    or None (allocated per call)."""
+    dev = isinstance(ref, DeviceArray)
+    d_ref, d_dense, d_sparse = [x if isinstance(x, DeviceArray) else DeviceArray.from_host(_f32(x))
+                                for x in (ref, dense, sparse)]
+    Hh, W = d_ref.shape
+    d_out = DeviceArray((Hh, W))
+    call("psw_tgv_sparse_denoise", d_ref.ptr, d_dense.ptr, d_sparse.ptr, W, Hh, float(zfar), int(niters),
+         float(alpha0), float(alpha1), float(tau), float(sigma), float(theta), float(beta), float(gamma),
+         workspace.ptr if workspace is not None else None, workspace.nbytes if workspace is not None else 0,
+         d_out.ptr, stream)
+    return d_out if dev else d_out.to_host()
+
+
+def tgv_sparse_workspace_bytes(W, H):
+    return int(lib.psw_tgv_sparse_workspace_bytes(int(W), int(H)))
+
+
 # ---- stage 3: fusion ---------------------------------------------------------------
 
 class Fusion:

@@ -71,6 +71,20 @@ int main(int argc, char** argv) {
     writef(dir + "/out_tvl1.bin", ps.getDepthmapDenoised()->data(), n * 4);
     if (!ps.TGV(argc, argv, (unsigned)prm[5], (unsigned)prm[6], prm[7])) return 5;
     writef(dir + "/out_tgv.bin", ps.getDepthmapTGV()->data(), n * 4);
+    // world coordinates of the TVL1 depthmap, then sparse-depth TGV2 from
+    // every 7th TVL1 depth over the sweep depthmap
+    {
+        CamImage<float>*cx = nullptr, *cy = nullptr, *cz = nullptr;
+        ps.get3Dcoordinates(cx, cy, cz);
+        if (!cx || !cy || !cz) return 12;
+        writef(dir + "/out_x.bin", cx->data(), n * 4);
+        writef(dir + "/out_y.bin", cy->data(), n * 4);
+        writef(dir + "/out_z.bin", cz->data(), n * 4);
+        CamImage<float> sparse(W, H);
+        for (size_t i = 0; i < n; ++i) sparse.data()[i] = i % 7 == 0 ? ps.getDepthmapDenoised()->data()[i] : 0.f;
+        if (!ps.TGVdenoiseFromSparse(argc, argv, sparse, 20, 2.0, 1.5, 0.02, 1.0, 1.0, 0.0, 1.0)) return 13;
+        writef(dir + "/out_tgv_sparse.bin", ps.getDepthmapTGV()->data(), n * 4);
+    }
     // error convention: an even window is rejected by the setter, a run with
     // too few planes fails with false
     ps.setWindowSize(4);

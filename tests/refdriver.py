@@ -118,3 +118,50 @@ def tgv(ref, src, K, Rref, tref, Rsrc, tsrc, niters, warps, lambda_=0.5, alpha0=
                  u1xbar.ptr, u1ybar.ptr, Px.ptr, Py.ptr, qx.ptr, qy.ptr, qz.ptr, qw.ptr, prodsum.ptr, float(alpha0),
                  float(alpha1), float(tau), float(lambda_), W, H, Z, Z, None)
     return u.to_host()
+
+
+def tgv_sparse(ref, dense, sparse, zfar, niters, alpha0=2.0, alpha1=1.5, tau=0.02, sigma=1.0, theta=1.0, beta=0.0,
+               gamma=1.0):
+    """PlaneSweep::TGVdenoiseFromSparse (src/planesweep.cpp:596-674) over
+    the launchers, with its aliasing: vxbar / vybar are both u1 and u1bar of
+    the U step."""
+    ref = np.ascontiguousarray(ref, np.float32)
+    H, W = ref.shape
+    f = lambda: _zeros(H, W)  # noqa: E731
+    px, py, qx, qy, qz, qw, ubar, vx, vy, weights, T1, T2, T3, T4 = [f() for _ in range(14)]
+    Ds = DeviceArray.from_host(np.ascontiguousarray(sparse, np.float32))
+    Ref = DeviceArray.from_host(ref)
+    ubar.upload(np.ascontiguousarray(dense, np.float32))
+    sz = np.float32(1) / np.float32(zfar)
+    call("psw_calculateWeights_sparseDepth", weights.ptr, Ds.ptr, W, H, Z, Z, None)
+    call("psw_element_scale", Ds.ptr, sz, W, H, Z, Z, None)
+    call("psw_element_scale", ubar.ptr, sz, W, H, Z, Z, None)
+    u = f()
+    call("psw_memcpy_dtod", u.ptr, ubar.ptr, u.nbytes, None)
+    call("psw_element_scale", Ref.ptr, np.float32(1) / np.float32(255), W, H, Z, Z, None)
+    call("psw_Anisotropic_diffusion_tensor", T1.ptr, T2.ptr, T3.ptr, T4.ptr, Ref.ptr, float(beta), float(gamma), W, H,
+         Z, Z, None)
+    for _ in range(niters):
+        call("psw_TGV2_updateP_tensor_weighed", px.ptr, py.ptr, T1.ptr, T2.ptr, T3.ptr, T4.ptr, u.ptr, vx.ptr, vy.ptr,
+             float(alpha1), float(sigma), W, H, Z, Z, None)
+        call("psw_TGV2_updateQ", qx.ptr, qy.ptr, qz.ptr, qw.ptr, vx.ptr, vy.ptr, float(alpha0), float(sigma), W, H, Z,
+             Z, None)
+        call("psw_TGV2_updateU_sparseDepthTensor", u.ptr, vx.ptr, vy.ptr, ubar.ptr, vx.ptr, vy.ptr, T1.ptr, T2.ptr,
+             T3.ptr, T4.ptr, px.ptr, py.ptr, qx.ptr, qy.ptr, qz.ptr, qw.ptr, weights.ptr, Ds.ptr, float(alpha0),
+             float(alpha1), float(tau), float(theta), W, H, Z, Z, None)
+    call("psw_element_scale", u.ptr, np.float32(zfar), W, H, Z, Z, None)
+    return u.to_host()
+
+
+def compute3d(depth, K, Rref, tref, alternative=0):
+    """PlaneSweep::get3Dcoordinates (src/planesweep.cpp:551-586) over
+    psw_compute3D; x, y of pixels beyond the depth limit read 0."""
+    depth = np.ascontiguousarray(depth, np.float32)
+    H, W = depth.shape
+    Rr, t = stages.relative_matrices(Rref, tref, np.eye(3, dtype=np.float32), np.zeros(3, np.float32), alternative)
+    invK = stages.mat3_inverse(np.ascontiguousarray(K, np.float32))
+    px, py = _zeros(H, W), _zeros(H, W)
+    X = DeviceArray.from_host(depth)
+    call("psw_compute3D", px.ptr, py.ptr, X.ptr, _hp(np.ascontiguousarray(Rr, np.float32)),
+         _hp(np.ascontiguousarray(t, np.float32)), _hp(invK), W, H, Z, Z, None)
+    return px.to_host(), py.to_host(), X.to_host()

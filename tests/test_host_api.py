@@ -63,6 +63,12 @@ def test_host_api_end_to_end(gpu, views_tiny, tmp_path):
     tg = oracle.tgv(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], tgv_iters, tgv_warps,
                     lambda_=lam, prec="f64")
     assert rel_err(rd("out_tgv.bin").reshape(H, W), tg).max() <= 1e-4
+    tvd = rd("out_tvl1.bin").reshape(H, W)
+    for name, e in zip(("out_x.bin", "out_y.bin", "out_z.bin"), oracle.compute3d(tvd, K1, v["Rref"], v["tref"])):
+        np.testing.assert_array_equal(rd(name).reshape(H, W), e)
+    sparse = np.where(np.arange(H * W).reshape(H, W) % 7 == 0, tvd, 0).astype(np.float32)
+    ts = oracle.tgv_sparse(v["ref"], exp, sparse, zfar, 20, prec="f64")
+    assert rel_err(rd("out_tgv_sparse.bin").reshape(H, W), ts).max() <= 1e-4
     refmean = oracle.windowed_mean(oracle.windowed_mean(v["ref"], win, False, "col"), win, False, "row")
     np.testing.assert_array_equal(rd("out_refmean.bin").reshape(H, W), refmean)
     # fusion through fusionData<8> / FusionUpdateIteration<8>
