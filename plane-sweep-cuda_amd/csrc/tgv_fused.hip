@@ -21,6 +21,8 @@
 #include "psw_common.h"
 #include "tgv_point.h"
 
+#include <cstring>
+#include <algorithm>
 #include <vector>
 
 namespace psw {
@@ -66,7 +68,9 @@ struct psw_tgv {
     double* partials;     // per-block energy partial sums
     int npartials;
     psw::TgvCtrl* ctrl;   // device
-    psw::TgvCtrl* hctrl;  // pinned host mirror
+    psw::TgvCtrl* hctrl;  // pinned host copy (filled by copies)
+    psw::TgvCtrl* hmirror;  // pinned, mapped host memory written by checking launches
+    psw::TgvCtrl* dmirror;  // its device address
     int k;                // iterations since the warp started
     bool have_images;
 };
@@ -266,18 +270,95 @@ __device__ __forceinline__ void u_update(float u, float u1x, float u1y, float t1
     u1yn = u1y - a.tau * (-a.alpha1 * (t12 * c_px + t22 * c_py) - a.alpha0 * (q2c - q2l + q1c - q1u));
 }
 
+// Energy stop test inside the iteration launches.  The launch that starts
+// from state k also sums the energy of state k (it holds u_k in LDS with the
+// halo the forward differences need) and stores one partial per tile; one
+// extra workgroup of the next launch adds those partials in tile order
+// (after the kernel boundary, so with plain loads) and applies the stop
+// test.  A stop at k thus takes effect one launch later; that launch has
+// written state k+2 into other slots (u cycles through 4, p, q, r through
+// 3), so state k stays intact and the launches after it are no-ops.
+struct TgvCheck {
+    double* partials;            // this launch's partials, one per tile (energy launches)
+    const double* fin_partials;  // the partials to finalise (finalising launches)
+    int fin_count;
+    int fin_k;                   // the state they belong to
+    TgvCtrl* ctrl;               // device
+    TgvCtrl* mirror;             // pinned host memory: the host polls it
+    const float* mask;
+    double lambda, alpha0, alpha1, tol, e_star;
+};
+
+// alpha1 |T(grad u - u1)| + alpha0 |eps(u1)|_F + lambda ed at one pixel, with
+// the iteration's clamped forward differences (x / y = right / lower
+// neighbour); ed = sum_j |It_j + (u - u0) Iu_j|.
+__device__ __forceinline__ double energy_point(double u, double ux, double uy, double a1, double a1x, double a1y,
+                                               double b1, double b1x, double b1y, double t11, double t12, double t22,
+                                               double ed, double lambda, double alpha0, double alpha1) {
+    const double gx = (ux - u) - a1, gy = (uy - u) - b1;
+    const double p = t11 * gx + t12 * gy, q = t12 * gx + t22 * gy;
+    const double ex = a1x - a1, ey = b1y - b1, ez = 0.5 * ((a1y - a1) + (b1x - b1));
+    return alpha1 * sqrt(p * p + q * q) + alpha0 * sqrt(ex * ex + ey * ey + 2.0 * ez * ez) + lambda * ed;
+}
+
+// Sum over the NT threads of a workgroup, in a fixed order; valid on thread 0.
+template <int NT>
+__device__ inline double block_sum(double v) {
+    __shared__ double wsum[NT / 64];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < NT / 64; ++w) t += wsum[w];
+    return t;
+}
+
+// Sum of the partials (the first 256 threads in a fixed pattern, so the
+// result does not depend on NT), then the stop test.
+constexpr int FIN_THREADS = 256;
+
+template <int NT>
+__device__ void check_finalize(const TgvCheck& c) {
+    double v = 0.0;
+    if (threadIdx.x < FIN_THREADS)
+        for (int i = threadIdx.x; i < c.fin_count; i += FIN_THREADS) v += c.fin_partials[i];
+    const double E = block_sum<NT>(v);
+    if (threadIdx.x == 0) {
+        TgvCtrl* d = c.ctrl;
+        volatile TgvCtrl* m = c.mirror;
+        d->energy = E;
+        m->energy = E;
+        if (!d->done && c.e_star > 0.0 && fabs(E - c.e_star) <= c.tol * c.e_star) {
+            d->iters_at_done = c.fin_k;
+            d->energy_at_done = E;
+            d->done = 1;
+            m->iters_at_done = c.fin_k;
+            m->energy_at_done = E;
+            __threadfence_system();
+            m->done = 1;
+        }
+    }
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void tgv_check_finalize_kernel(TgvCheck c) {
+    check_finalize<NT>(c);
+}
+
 // Tile TG_TX x TY pixels, NT threads.  Variants (selected by tgv_variant()):
 // 64x8 with 256 or 512 threads, 64x4 with 256 threads.
 // One tile of one iteration; called by every thread of the workgroup (it
 // contains barriers).
-template <int TY, int NT>
-__device__ __forceinline__ void tgv_iter_tile(const TgvPtrs& a, const int tile) {
+template <int TY, int NT, bool CHECK = false>
+__device__ __forceinline__ void tgv_iter_tile(const TgvPtrs& a, const int tile, const TgvCheck* chk = nullptr) {
     constexpr int TG_TY = TY, TG_THREADS = NT;
     constexpr int HX = TG_TX + 2, HY = TG_TY + 2;   // ubar region: x0-1..x0+TX, y0-1..y0+TY
     constexpr int DX = TG_TX + 1, DY = TG_TY + 1;   // dual region: x0-1..x0+TX-1, y0-1..y0+TY-1
     __shared__ float sUb[3][HY][HX];
     __shared__ float sP[2][DY][DX];
     __shared__ float sQ[3][DY][DX];
+    __shared__ float sUc[CHECK ? 3 : 1][CHECK ? HY : 1][CHECK ? HX : 1];   // u_k (checking launches)
 
     const int W = a.W, H = a.H;
     const size_t n = (size_t)W * H;
@@ -293,7 +374,11 @@ __device__ __forceinline__ void tgv_iter_tile(const TgvPtrs& a, const int tile) 
         const int gy = min(max(y0 - 1 + hy, 0), H - 1);
         const size_t g = (size_t)gy * W + gx;
 #pragma unroll
-        for (int f = 0; f < 3; ++f) sUb[f][hy][hx] = 2.f * a.ucur[f * n + g] - a.uprev[f * n + g];
+        for (int f = 0; f < 3; ++f) {
+            const float uc = a.ucur[f * n + g];
+            sUb[f][hy][hx] = 2.f * uc - a.uprev[f * n + g];
+            if (CHECK) sUc[f][hy][hx] = uc;
+        }
     }
     __syncthreads();
 
@@ -320,22 +405,36 @@ __device__ __forceinline__ void tgv_iter_tile(const TgvPtrs& a, const int tile) 
     }
     __syncthreads();
 
-    // phase 3: r, prodsum and the primal step for the tile pixels.
+    // phase 3: r, prodsum and the primal step for the tile pixels (and, in
+    // a checking launch, the energy of state k).
     const float sl = a.sigma * a.lambda;
+    double eacc = 0.0;
     for (int i = tid; i < TG_TX * TG_TY; i += TG_THREADS) {
         const int ty = i / TG_TX, tx = i - (i / TG_TX) * TG_TX;
         const int x = x0 + tx, y = y0 + ty;
         if (x >= W || y >= H) continue;
         const size_t g = (size_t)y * W + x;
         const float u = a.ucur[g], u1x = a.ucur[n + g], u1y = a.ucur[2 * n + g];
-        const float du = u - a.u0[g];
+        const float u0g = a.u0[g];
+        const float du = u - u0g;
+        const double dud = (double)u - (double)u0g;
         float ps = 0.f;
+        double ed = 0.0;
         for (int j = 0; j < a.N; ++j) {
             const size_t gj = (size_t)j * n + g;
             const float iu = a.Iu[gj];
-            const float rr = r_update(a.rcur[gj], a.It[gj], du, iu, sl);
+            const float it = a.It[gj];
+            const float rr = r_update(a.rcur[gj], it, du, iu, sl);
             a.rnext[gj] = rr;
             ps += rr * iu;
+            if (CHECK) ed += fabs((double)it + dud * (double)iu);
+        }
+        if (CHECK && chk->mask[g] != 0.f) {
+            const int hx = tx + 1, hy = ty + 1;          // u_k index; +1 is the clamped right / lower neighbour
+            eacc += energy_point(sUc[0][hy][hx], sUc[0][hy][hx + 1], sUc[0][hy + 1][hx], sUc[1][hy][hx],
+                                 sUc[1][hy][hx + 1], sUc[1][hy + 1][hx], sUc[2][hy][hx], sUc[2][hy][hx + 1],
+                                 sUc[2][hy + 1][hx], a.T[g], a.T[n + g], a.T[2 * n + g], ed, chk->lambda,
+                                 chk->alpha0, chk->alpha1);
         }
         const int cx = tx + 1, cy = ty + 1;              // dual index of this pixel
         const int lx = x > 0 ? cx - 1 : cx;              // max(x-1, 0)
@@ -350,6 +449,10 @@ __device__ __forceinline__ void tgv_iter_tile(const TgvPtrs& a, const int tile) 
         a.unext[g] = un;
         a.unext[n + g] = u1xn;
         a.unext[2 * n + g] = u1yn;
+    }
+    if (CHECK) {
+        const double e = block_sum<NT>(eacc);
+        if (threadIdx.x == 0) chk->partials[tile] = e;
     }
 }
 
@@ -529,6 +632,20 @@ __global__ __launch_bounds__(NT) void tgv_iter_kernel(TgvPtrs a) {
     tgv_iter_tile<TY, NT>(a, xcd_tile(blockIdx.x, ntiles));
 }
 
+// One iteration k -> k+1 that also sums the energy of state k (ENERGY)
+// and / or finalises the previous energy launch's sum in one extra
+// workgroup (FINAL).
+template <int TY, int NT, bool ENERGY, bool FINAL>
+__global__ __launch_bounds__(NT) void tgv_iter_check_kernel(TgvPtrs a, TgvCheck c) {
+    if (a.ctrl->done) return;
+    const int ntiles = ((a.W + TG_TX - 1) / TG_TX) * ((a.H + TY - 1) / TY);
+    if (FINAL && (int)blockIdx.x == ntiles) {
+        check_finalize<NT>(c);
+        return;
+    }
+    tgv_iter_tile<TY, NT, ENERGY>(a, xcd_tile(blockIdx.x, ntiles), &c);
+}
+
 // ---- FP64 energy -------------------------------------------------------------
 
 __device__ inline double block_reduce_sum(double v) {
@@ -557,18 +674,12 @@ __device__ inline double tgv_energy_block(const float* __restrict__ U, const flo
     for (size_t i = (size_t)bid * TG_THREADS + threadIdx.x; i < n; i += (size_t)nblk * TG_THREADS) {
         const int y = (int)(i / W), x = (int)(i - (size_t)y * W);
         const size_t ix = (size_t)y * W + min(x + 1, W - 1), iy = (size_t)min(y + 1, H - 1) * W + x;
-        const double u = U[i], ux = U[ix], uy = U[iy];
-        const double a1 = U[n + i], a1x = U[n + ix], a1y = U[n + iy];
-        const double b1 = U[2 * n + i], b1x = U[2 * n + ix], b1y = U[2 * n + iy];
-        const double gx = (ux - u) - a1, gy = (uy - u) - b1;
-        const double t11 = T[i], t12 = T[n + i], t22 = T[2 * n + i];
-        const double p = t11 * gx + t12 * gy, q = t12 * gx + t22 * gy;
-        const double ex = a1x - a1, ey = b1y - b1, ez = 0.5 * ((a1y - a1) + (b1x - b1));
+        if (mask[i] == 0.f) continue;
         double ed = 0.0;
-        const double du = u - (double)u0[i];
+        const double du = (double)U[i] - (double)u0[i];
         for (int j = 0; j < N; ++j) ed += fabs((double)It[(size_t)j * n + i] + du * (double)Iu[(size_t)j * n + i]);
-        const double ei = alpha1 * sqrt(p * p + q * q) + alpha0 * sqrt(ex * ex + ey * ey + 2.0 * ez * ez) + lambda * ed;
-        if (mask[i] != 0.f) e += ei;
+        e += energy_point(U[i], U[ix], U[iy], U[n + i], U[n + ix], U[n + iy], U[2 * n + i], U[2 * n + ix],
+                          U[2 * n + iy], T[i], T[n + i], T[2 * n + i], ed, lambda, alpha0, alpha1);
     }
     return block_reduce_sum(e);
 }
@@ -661,6 +772,35 @@ static int tgv_variant() {
 static bool tgv_double() {
     const char* e = getenv("PSW_TGV_DOUBLE");
     return e && atoi(e) == 1;
+}
+
+// Partials of the checking launches: one per tile of the smallest tile shape.
+static int tgv_tile_partials(int W, int H) { return ((W + TG_TX - 1) / TG_TX) * ((H + 3) / 4); }
+
+// One iteration launch of the solve loop: ENERGY sums the energy of the
+// current state into c.partials, FINAL finalises c.fin_partials in one
+// extra workgroup.
+template <int TY, int NT>
+static void tgv_launch_check(const TgvPtrs& p, const TgvCheck& c, bool energy, bool fin, int ntiles, hipStream_t st) {
+    const unsigned g = (unsigned)(ntiles + (fin ? 1 : 0));
+    if (energy && fin) tgv_iter_check_kernel<TY, NT, true, true><<<g, NT, 0, st>>>(p, c);
+    else if (energy) tgv_iter_check_kernel<TY, NT, true, false><<<g, NT, 0, st>>>(p, c);
+    else if (fin) tgv_iter_check_kernel<TY, NT, false, true><<<g, NT, 0, st>>>(p, c);
+    else tgv_iter_kernel<TY, NT><<<g, NT, 0, st>>>(p);
+}
+
+static int tgv_launch_one(psw_tgv* t, const psw_tgv_params* prm, const TgvCheck& c, bool energy, bool fin,
+                          hipStream_t st) {
+    const int var = tgv_variant();
+    const int ty = var == 2 ? 4 : 8;
+    const int ntiles = ((t->W + TG_TX - 1) / TG_TX) * ((t->H + ty - 1) / ty);
+    const TgvPtrs p = tgv_ptrs(t, prm);
+    if (var == 0) tgv_launch_check<8, 256>(p, c, energy, fin, ntiles, st);
+    else if (var == 1) tgv_launch_check<8, 512>(p, c, energy, fin, ntiles, st);
+    else tgv_launch_check<4, 256>(p, c, energy, fin, ntiles, st);
+    const int s = launch_status(energy || fin ? "tgv_iter_check_kernel" : "tgv_iter_kernel");
+    if (s == PSW_OK) t->k++;
+    return s;
 }
 
 static int tgv_launch_iters(psw_tgv* t, int niters, const psw_tgv_params* prm, hipStream_t st) {
@@ -788,7 +928,8 @@ extern "C" {
 size_t psw_tgv_state_bytes(int width, int height, int nsrc) {
     TgvLayout L;
     L.init(width, height, nsrc);
-    return L.total * sizeof(float) + TG_ENERGY_BLOCKS * sizeof(double) + sizeof(TgvCtrl);
+    return L.total * sizeof(float) +
+           2 * std::max(TG_ENERGY_BLOCKS, tgv_tile_partials(width, height)) * sizeof(double) + sizeof(TgvCtrl);
 }
 
 int psw_tgv_create(psw_tgv** out, int width, int height, int nsrc) {
@@ -803,9 +944,16 @@ int psw_tgv_create(psw_tgv** out, int width, int height, int nsrc) {
     t->k = 0;
     t->have_images = false;
     hipError_t e = hipMalloc((void**)&t->base, t->L.total * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc((void**)&t->partials, TG_ENERGY_BLOCKS * sizeof(double));
+    // the stand-alone energy pass uses the first TG_ENERGY_BLOCKS; the solve
+    // loop alternates between two halves of tile partials
+    const int np = 2 * std::max(TG_ENERGY_BLOCKS, tgv_tile_partials(width, height));
+    if (e == hipSuccess) e = hipMalloc((void**)&t->partials, np * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void**)&t->ctrl, sizeof(TgvCtrl));
     if (e == hipSuccess) e = hipHostMalloc((void**)&t->hctrl, sizeof(TgvCtrl), hipHostMallocDefault);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**)&t->hmirror, sizeof(TgvCtrl), hipHostMallocMapped | hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&t->dmirror, t->hmirror, 0);
+    if (e == hipSuccess) std::memset(t->hmirror, 0, sizeof(TgvCtrl));
     if (e == hipSuccess) e = hipMemset(t->base, 0, t->L.total * sizeof(float));
     if (e == hipSuccess) e = hipMemset(t->ctrl, 0, sizeof(TgvCtrl));
     if (e != hipSuccess) {
@@ -829,6 +977,7 @@ int psw_tgv_destroy(psw_tgv* t) {
     if (t->partials) (void)hipFree(t->partials);
     if (t->ctrl) (void)hipFree(t->ctrl);
     if (t->hctrl) (void)hipHostFree(t->hctrl);
+    if (t->hmirror) (void)hipHostFree(t->hmirror);
     delete t;
     return PSW_OK;
 }
@@ -986,31 +1135,79 @@ int psw_tgv_solve(psw_tgv* t, int max_iters, int check_every, double tol, double
     int rc = tgv_reset_ctrl(t, st);
     if (rc != PSW_OK) return rc;
     const int k_start = t->k;
-    // Keep at most two checks in flight: the host waits for the check two
-    // chunks back, so the device never idles and at most two chunks run as
-    // no-ops (they test ctrl->done and return) after the stop.
+    std::memset(t->hmirror, 0, sizeof(TgvCtrl));   // no launch of this solver writes it now
+    volatile const TgvCtrl* mirror = t->hmirror;
+    const int half = std::max(TG_ENERGY_BLOCKS, tgv_tile_partials(t->W, t->H));
+    const int ntiles = ((t->W + TG_TX - 1) / TG_TX) * ((t->H + (tgv_variant() == 2 ? 4 : 8) - 1) /
+                                                        (tgv_variant() == 2 ? 4 : 8));
+    TgvCheck chk;
+    chk.partials = nullptr;
+    chk.fin_partials = nullptr;
+    chk.fin_count = ntiles;
+    chk.fin_k = 0;
+    chk.ctrl = t->ctrl;
+    chk.mirror = t->dmirror;
+    chk.mask = t->base + t->L.mask;
+    chk.lambda = prm->lambda;
+    chk.alpha0 = prm->alpha0;
+    chk.alpha1 = prm->alpha1;
+    chk.tol = tol;
+    chk.e_star = e_star;
+    // Chunks of check_every iterations.  The first launch of every chunk
+    // after the first sums the energy of the state the previous chunk ended
+    // at (into one of two partial buffers, alternately); the next launch
+    // finalises it and writes the outcome to pinned host memory.  At most
+    // two chunks are in flight: the host waits for the chunk two back, so the
+    // device never idles and at most two chunks run as no-ops (they test
+    // ctrl->done and return) after the stop.
     hipEvent_t ev[2] = {nullptr, nullptr};
     PSW_TRY_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming), PSW_ERR_DEVICE);
     PSW_TRY_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming), PSW_ERR_DEVICE);
-    int done_iters = -1, chunk_idx = 0;
+    int done_iters = -1, chunk_idx = 0, nchecks = 0;
     int launched = 0;
-    while (launched < max_iters) {
+    bool stopped = false, pending = false;   // pending: an energy sum not yet finalised
+    double* pend_buf = nullptr;
+    int pend_k = 0;
+    while (launched < max_iters && rc == PSW_OK) {
         const int c = std::min(check_every, max_iters - launched);
-        rc = tgv_launch_iters(t, c, prm, st);
+        for (int i = 0; i < c && rc == PSW_OK; ++i) {
+            const bool energy = i == 0 && launched > 0;
+            const bool fin = pending;
+            if (fin) {
+                chk.fin_partials = pend_buf;
+                chk.fin_k = pend_k;
+            }
+            if (energy) chk.partials = t->partials + (size_t)(nchecks & 1) * half;
+            const int k_before = t->k;
+            rc = tgv_launch_one(t, prm, chk, energy, fin, st);
+            if (energy) {
+                pend_buf = chk.partials;
+                pend_k = k_before;
+                nchecks++;
+            }
+            pending = energy;
+        }
         if (rc != PSW_OK) break;
         launched += c;
-        rc = tgv_launch_energy(t, prm, tol, e_star, st);
-        if (rc != PSW_OK) break;
         hipEvent_t e = ev[chunk_idx & 1];
         if (chunk_idx >= 2) {
-            // the event slot holds the check two chunks back: wait and read it
+            // the event slot holds the chunk two back: wait for it
             if (hipEventSynchronize(e) != hipSuccess) { rc = PSW_ERR_DEVICE; break; }
-            if (t->hctrl->done) { done_iters = t->hctrl->iters_at_done; break; }
+            if (mirror->done) { done_iters = mirror->iters_at_done; stopped = true; break; }
         }
-        (void)hipMemcpyAsync(t->hctrl, t->ctrl, sizeof(TgvCtrl), hipMemcpyDeviceToHost, st);
         (void)hipEventRecord(e, st);
         chunk_idx++;
     }
+    if (rc == PSW_OK && pending && !stopped) {
+        // the last energy sum has no following launch: finalise it alone
+        chk.fin_partials = pend_buf;
+        chk.fin_k = pend_k;
+        if (tgv_variant() == 1) tgv_check_finalize_kernel<512><<<1, 512, 0, st>>>(chk);
+        else tgv_check_finalize_kernel<256><<<1, 256, 0, st>>>(chk);
+        rc = launch_status("tgv_check_finalize_kernel");
+    }
+    // the state the last chunk ended at has not been tested yet
+    if (rc == PSW_OK && !stopped) rc = tgv_launch_energy(t, prm, tol, e_star, st);
     (void)hipStreamSynchronize(st);
     (void)hipMemcpy(t->hctrl, t->ctrl, sizeof(TgvCtrl), hipMemcpyDeviceToHost);
     (void)hipEventDestroy(ev[0]);

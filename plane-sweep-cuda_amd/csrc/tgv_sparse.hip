@@ -4,7 +4,7 @@
 // The reference runs three launches per iteration -- P (TGV2_kernels.cu:
 // 340-365), Q (:54-85) and the sparse-depth U step (:399-432) -- over 19
 // full-frame images.  Here one launch per iteration does all three for a
-// 64x8 tile:
+// 64xTY tile:
 //   phase 1: u, u1x, u1y of iteration k on the tile plus a one-pixel halo
 //            into LDS;
 //   phase 2: p, q of iteration k+1 on the tile plus its left/top halo, into
@@ -29,9 +29,11 @@
 #include "psw_common.h"
 #include "tgv_point.h"
 
+#include <cstdlib>
+
 namespace psw {
 
-constexpr int TS_TX = 64, TS_TY = 8, TS_THREADS = 512;
+constexpr int TS_TX = 64;
 
 struct SparsePtrs {
     const float* ucur;    // [3][n]: u, u1x, u1y at iteration k
@@ -76,6 +78,8 @@ __global__ void tgv_sparse_setup_kernel(float* __restrict__ T, float* __restrict
     Q[2 * n + i] = 0.f;
 }
 
+// Tile 64 x TY pixels, NT threads (variants chosen by sparse_variant()).
+template <int TS_TY, int TS_THREADS>
 __global__ __launch_bounds__(TS_THREADS) void tgv_sparse_iter_kernel(SparsePtrs a) {
     constexpr int HX = TS_TX + 2, HY = TS_TY + 2;   // u region: x0-1..x0+TX, y0-1..y0+TY
     constexpr int DX = TS_TX + 1, DY = TS_TY + 1;   // dual region: x0-1..x0+TX-1, y0-1..y0+TY-1
@@ -166,6 +170,22 @@ __global__ void tgv_sparse_output_kernel(float* __restrict__ out, const float* _
         out[i] = u[i] * zfar;
 }
 
+// Launch shapes: 0 = 64x4 / 256 threads, 1 = 64x8 / 512 (default),
+// 2 = 64x8 / 256, 3 = 64x16 / 512.  Read from PSW_TGV_SPARSE_VARIANT on each
+// call; results do not depend on it.
+static int sparse_variant() {
+    const char* e = std::getenv("PSW_TGV_SPARSE_VARIANT");
+    const int v = e ? std::atoi(e) : 1;
+    return (v >= 0 && v <= 3) ? v : 1;
+}
+
+template <int TY, int NT>
+static hipError_t sparse_launch(const SparsePtrs& a, hipStream_t hs) {
+    const int ntiles = ((a.W + TS_TX - 1) / TS_TX) * ((a.H + TY - 1) / TY);
+    tgv_sparse_iter_kernel<TY, NT><<<ntiles, NT, 0, hs>>>(a);
+    return hipGetLastError();
+}
+
 // Workspace: T 3n, then two slots of (u 3n, p 2n, q 3n).
 static size_t sparse_floats(size_t n) { return 3 * n + 2 * 8 * n; }
 
@@ -206,7 +226,7 @@ int psw_tgv_sparse_denoise(const float* d_ref, const float* d_dense, const float
     tgv_sparse_setup_kernel<<<grd, blk, 0, hs>>>(T, slot[0], slot[0] + 3 * n, slot[0] + 5 * n, d_ref, d_dense, width,
                                                   height, (float)beta, (float)gamma, sz);
     hipError_t e = hipGetLastError();
-    const int ntiles = ((width + TS_TX - 1) / TS_TX) * ((height + TS_TY - 1) / TS_TY);
+    const int variant = sparse_variant();
     SparsePtrs a;
     a.T = T;
     a.sparse = d_sparse;
@@ -226,8 +246,12 @@ int psw_tgv_sparse_denoise(const float* d_ref, const float* d_dense, const float
         a.pnext = nx + 3 * n;
         a.qcur = c + 5 * n;
         a.qnext = nx + 5 * n;
-        tgv_sparse_iter_kernel<<<ntiles, TS_THREADS, 0, hs>>>(a);
-        e = hipGetLastError();
+        switch (variant) {
+            case 0: e = sparse_launch<4, 256>(a, hs); break;
+            case 2: e = sparse_launch<8, 256>(a, hs); break;
+            case 3: e = sparse_launch<16, 512>(a, hs); break;
+            default: e = sparse_launch<8, 512>(a, hs); break;
+        }
     }
     if (e == hipSuccess) {
         const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);

@@ -349,6 +349,52 @@ def test_tgv_solve_stops_at_gap(gpu, views_small):
     assert iters < 3000 and iters % 10 == 0
     assert abs(e - e_star) <= 1e-3 * e_star
     assert abs(t.energy(prm) - e) <= 1e-12 * e   # the state is the one at the stop
+    # the same stop as testing the energy after every chunk of 10 with the
+    # stand-alone energy pass (the solve tests it inside the next launch)
+    ref = fresh()
+    k = 0
+    while True:
+        ref.iterate(10, prm)
+        k += 10
+        if abs(ref.energy(prm) - e_star) <= 1e-3 * e_star:
+            break
+    assert k == iters
+    sa, sb = ref.get_state(), t.get_state()
+    for name in ("u", "u1x", "u1y", "px", "py", "qx", "qy", "qz", "r"):
+        np.testing.assert_array_equal(sa[name], sb[name], err_msg=name)
+
+
+@pytest.mark.parametrize("variant", ["0", "1", "2"])
+def test_tgv_solve_max_iters_and_variants(gpu, views_small, monkeypatch, variant):
+    """Every launch shape stops at the same iteration with the same state;
+    a solve that reaches max_iters reports max_iters and the energy of the
+    final state."""
+    from psw_amd import stages
+    from psw_amd.device import DeviceArray
+    monkeypatch.setenv("PSW_TGV_VARIANT", variant)
+    v = views_small
+    K1 = one_based(v["K"])
+    prm = stages.tgv_params(**TGV_KW)
+    H, W = v["ref"].shape
+    N = v["src"].shape[0]
+    Rrel, trel = stages.relative_all(v["Rref"], v["tref"], v["Rsrc"], v["tsrc"])
+    d_ref, d_src = DeviceArray.from_host(v["ref"]), DeviceArray.from_host(v["src"])
+    t = stages.TgvSolver(W, H, N)
+    t.set_images(d_ref, d_src)
+    t.set_u(value=1.0)
+    t.linearise(K1, Rrel, trel)
+    iters, e = t.solve(95, 10, 1e-9, 1.0, prm)          # E* = 1: never within tolerance
+    assert iters == 95
+    assert abs(t.energy(prm) - e) <= 1e-12 * e
+    t.set_u(value=1.0)
+    t.linearise(K1, Rrel, trel)
+    t.iterate(400, prm)
+    e_star = t.energy(prm)
+    t.set_u(value=1.0)
+    t.linearise(K1, Rrel, trel)
+    it2, e2 = t.solve(400, 7, 2e-2, e_star, prm)
+    assert it2 < 400 and it2 % 7 == 0 and abs(e2 - e_star) <= 2e-2 * e_star
+    assert abs(t.energy(prm) - e2) <= 1e-12 * e2
 
 
 # ---- stage 3: fusion ---------------------------------------------------------------------

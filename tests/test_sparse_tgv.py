@@ -196,3 +196,16 @@ def test_compute3d_bit_exact(gpu, alternative):
     exp = oracle.compute3d(depth, K, R, t, alternative)
     for g, e in zip(got, exp):
         np.testing.assert_array_equal(g, e)
+
+
+@pytest.mark.gpu
+def test_sparse_launch_shapes_identical(gpu, monkeypatch):
+    """The tile shape (PSW_TGV_SPARSE_VARIANT) never changes results."""
+    from psw_amd import stages
+    v, dense, sparse = _scene(W=161, H=83)
+    outs = []
+    for var in ("1", "0", "2", "3"):
+        monkeypatch.setenv("PSW_TGV_SPARSE_VARIANT", var)
+        outs.append(stages.tgv_sparse(v["ref"], dense, sparse, 8.0, 25, **KW))
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0])
