@@ -441,7 +441,11 @@ def main():
     ms_per_step = learned["ms_per_depthmap"]
 
     # Per-launch duration of the fused TGV2 iteration kernel (same stream and
-    # buffers): events around back-to-back launches.
+    # buffers): events around back-to-back launches.  By default a launch
+    # does two iterations (tgv_iter2_kernel); PSW_TGV_DOUBLE=0 selects the
+    # one-iteration kernel.
+    two = os.environ.get("PSW_TGV_DOUBLE", "1") != "0"
+    tgv_kernel, iters_per_launch = ("tgv_iter2_kernel", 2) if two else ("tgv_iter_kernel", 1)
     pl.tgv.set_u(d_s, stream=s)
     pl.tgv.linearise(pl.K1, pl.Rrel, pl.trel, s)
     nl = 200
@@ -449,8 +453,8 @@ def main():
     pl.tgv.iterate(nl, pl.prm, s)
     ev[1].record(st)
     ev[1].synchronize()
-    tgv_launch_ms = ev[0].elapsed_ms(ev[1]) / nl
-    tgv_bytes = 4 * TGV_FLOATS_PER_PX * W * H
+    tgv_launch_ms = ev[0].elapsed_ms(ev[1]) / nl * iters_per_launch
+    tgv_bytes = 4 * TGV_FLOATS_PER_PX * W * H * iters_per_launch
     achieved = tgv_bytes / (tgv_launch_ms * 1e-3) / 1e9
     # the sweep kernel alone
     ev[0].record(st)
@@ -469,7 +473,7 @@ def main():
     net_tflops = warmstart.forward_flops(W, H) / (net_ms * 1e-3) / 1e12
 
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "tgv_iter_kernel_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", tgv_kernel + "_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
@@ -499,9 +503,12 @@ def main():
         "cold": cold,
         "e_star": e_star,
         "energy_mask": {"qmax": QMAX, "excluded_pixels": excluded, "estar_change_last_5000": (e_prev - e_star) / e_star},
-        "roofline": {"kernel": "tgv_iter_kernel", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"kernel": tgv_kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_launch": tgv_bytes, "launch_ms": tgv_launch_ms},
+                     "bytes_per_launch": tgv_bytes, "launch_ms": tgv_launch_ms,
+                     "iterations_per_launch": iters_per_launch,
+                     "bytes_note": f"algorithmic bytes of {iters_per_launch} TGV2 iteration(s): "
+                                   f"{4 * TGV_FLOATS_PER_PX} B per pixel per iteration"},
         "stages": {"sweep_kernel_ms": sweep_kernel_ms, "sweep_tflops_declared": sweep_tflops,
                    "sweep_frac_fp32_peak": sweep_tflops / FP32_PEAK_TFLOPS, "unet_ms": net_ms,
                    "unet_tflops": net_tflops, "unet_frac_bf16_peak": net_tflops / BF16_PEAK_TFLOPS,

@@ -350,7 +350,9 @@ __global__ __launch_bounds__(NT) void tgv_check_finalize_kernel(TgvCheck c) {
 // 64x8 with 256 or 512 threads, 64x4 with 256 threads.
 // One tile of one iteration; called by every thread of the workgroup (it
 // contains barriers).
-template <int TY, int NT, bool CHECK = false>
+// NS: the most sources the instance handles (a.N <= NS); all sources' r, It,
+// Iu of a pixel are loaded before any r is stored, so the loads overlap.
+template <int TY, int NT, int NS, bool CHECK = false>
 __device__ __forceinline__ void tgv_iter_tile(const TgvPtrs& a, const int tile, const TgvCheck* chk = nullptr) {
     constexpr int TG_TY = TY, TG_THREADS = NT;
     constexpr int HX = TG_TX + 2, HY = TG_TY + 2;   // ubar region: x0-1..x0+TX, y0-1..y0+TY
@@ -418,16 +420,26 @@ __device__ __forceinline__ void tgv_iter_tile(const TgvPtrs& a, const int tile, 
         const float u0g = a.u0[g];
         const float du = u - u0g;
         const double dud = (double)u - (double)u0g;
+        float rv[NS], itv[NS], iuv[NS];
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            if (j < a.N) {
+                const size_t gj = (size_t)j * n + g;
+                rv[j] = a.rcur[gj];
+                itv[j] = a.It[gj];
+                iuv[j] = a.Iu[gj];
+            }
+        }
         float ps = 0.f;
         double ed = 0.0;
-        for (int j = 0; j < a.N; ++j) {
-            const size_t gj = (size_t)j * n + g;
-            const float iu = a.Iu[gj];
-            const float it = a.It[gj];
-            const float rr = r_update(a.rcur[gj], it, du, iu, sl);
-            a.rnext[gj] = rr;
-            ps += rr * iu;
-            if (CHECK) ed += fabs((double)it + dud * (double)iu);
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            if (j < a.N) {
+                const float rr = r_update(rv[j], itv[j], du, iuv[j], sl);
+                a.rnext[(size_t)j * n + g] = rr;
+                ps += rr * iuv[j];
+                if (CHECK) ed += fabs((double)itv[j] + dud * (double)iuv[j]);
+            }
         }
         if (CHECK && chk->mask[g] != 0.f) {
             const int hx = tx + 1, hy = ty + 1;          // u_k index; +1 is the clamped right / lower neighbour
@@ -474,8 +486,13 @@ constexpr int T2_TY = 8, T2_THREADS = 512, T2_MAXN = 16;
 
 __device__ __forceinline__ int clampi(int v, int hi) { return min(max(v, 0), hi); }
 
+// One tile pixel per thread (NT == TX * TY): thread t owns pixel (t / TX,
+// t % TX) in both iterations and keeps its r_{k+1}, It, Iu, u0 and T in
+// registers (NS >= a.N sources), so no source plane is read twice.
+template <int NS>
 __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile) {
     constexpr int TX = TG_TX, TY = T2_TY, NT = T2_THREADS;
+    static_assert(NT == TX * TY, "one tile pixel per thread");
     constexpr int AX = TX + 4, AY = TY + 4;
     constexpr int BX = TX + 3, BY = TY + 3;
     constexpr int CX = TX + 2, CY = TY + 2;
@@ -485,7 +502,6 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile)
     __shared__ float sU1[3][CY][CX];  // u_{k+1} on C
     __shared__ float sP[2][BY][BX];   // p_{k+1} on B; then p_{k+2} on D in place (B index = D index + 1)
     __shared__ float sQ[3][BY][BX];
-    __shared__ float sR[T2_MAXN][TY][TX];  // r_{k+1} of the tile pixels
 
     const int W = a.W, H = a.H;
     const size_t n = (size_t)W * H;
@@ -508,6 +524,22 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile)
 #pragma unroll
         for (int f = 0; f < 3; ++f) sUk[f][iy][ix] = a.ucur[f * n + g];
     }
+    // this thread's tile pixel (clamped coordinates outside the image: computed, never stored)
+    const int ty = tid / TX, tx = tid - (tid / TX) * TX;
+    const int x = x0 + tx, y = y0 + ty;
+    const size_t gt = (size_t)clampi(y, H - 1) * W + clampi(x, W - 1);
+    float rv[NS], itv[NS], iuv[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        if (j < a.N) {
+            const size_t gj = (size_t)j * n + gt;
+            rv[j] = a.rcur[gj];
+            itv[j] = a.It[gj];
+            iuv[j] = a.Iu[gj];
+        }
+    }
+    const float u0t = a.u0[gt];
+    const float t11 = a.T[gt], t12 = a.T[n + gt], t22 = a.T[2 * n + gt];
     __syncthreads();
 
     // B: p, q of iteration k+1
@@ -525,34 +557,45 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile)
     }
     __syncthreads();
 
-    // C: r and u of iteration k+1 (tile pixels first, one per thread, so
-    // that each thread keeps its pixel's r_{k+1} in sR; then the ring)
-    for (int i = tid; i < CX * CY; i += NT) {
-        int ix, iy;
-        if (i < TX * TY) {
-            iy = i / TX + 1;
-            ix = i - (i / TX) * TX + 1;
-        } else {
-            const int k = i - TX * TY;            // ring: top row, bottom row, left column, right column
-            if (k < CX) { iy = 0; ix = k; }
-            else if (k < 2 * CX) { iy = CY - 1; ix = k - CX; }
-            else if (k < 2 * CX + TY) { iy = k - 2 * CX + 1; ix = 0; }
-            else { iy = k - 2 * CX - TY + 1; ix = CX - 1; }
+    // C: r and u of iteration k+1; the tile pixel from registers (its r_{k+1}
+    // stays there), then the ring
+    {
+        const int ix = tx + 1, iy = ty + 1;
+        const float u = sUk[0][iy][ix], u1x = sUk[1][iy][ix], u1y = sUk[2][iy][ix];
+        const float du = u - u0t;
+        float ps = 0.f;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            if (j < a.N) {
+                rv[j] = r_update(rv[j], itv[j], du, iuv[j], sl);
+                ps += rv[j] * iuv[j];
+            }
         }
+        const int bx = ix + 1, by = iy + 1;               // B index of this position
+        const int lx = x > 0 ? bx - 1 : bx, uy = y > 0 ? by - 1 : by;
+        float un, u1xn, u1yn;
+        u_update(u, u1x, u1y, t11, t12, t22, sP[0][by][bx], sP[1][by][bx], sP[0][by][lx], sP[1][by][lx],
+                 sP[0][uy][bx], sP[1][uy][bx], sQ[0][by][bx], sQ[0][by][lx], sQ[1][by][bx], sQ[1][uy][bx],
+                 sQ[2][by][bx], sQ[2][by][lx], sQ[2][uy][bx], ps, a, un, u1xn, u1yn);
+        sU1[0][iy][ix] = un; sU1[1][iy][ix] = u1xn; sU1[2][iy][ix] = u1yn;
+    }
+    for (int k = tid; k < CX * CY - TX * TY; k += NT) {  // ring: top row, bottom row, left column, right column
+        int ix, iy;
+        if (k < CX) { iy = 0; ix = k; }
+        else if (k < 2 * CX) { iy = CY - 1; ix = k - CX; }
+        else if (k < 2 * CX + TY) { iy = k - 2 * CX + 1; ix = 0; }
+        else { iy = k - 2 * CX - TY + 1; ix = CX - 1; }
         const int gx = x0 - 1 + ix, gy = y0 - 1 + iy;
         const size_t g = (size_t)clampi(gy, H - 1) * W + clampi(gx, W - 1);
         const float u = sUk[0][iy][ix], u1x = sUk[1][iy][ix], u1y = sUk[2][iy][ix];
         const float du = u - a.u0[g];
-        const bool own = i < TX * TY;
         float ps = 0.f;
         for (int j = 0; j < a.N; ++j) {
             const size_t gj = (size_t)j * n + g;
             const float iu = a.Iu[gj];
-            const float rr = r_update(a.rcur[gj], a.It[gj], du, iu, sl);
-            if (own) sR[j][iy - 1][ix - 1] = rr;
-            ps += rr * iu;
+            ps += r_update(a.rcur[gj], a.It[gj], du, iu, sl) * iu;
         }
-        const int bx = ix + 1, by = iy + 1;               // B index of this position
+        const int bx = ix + 1, by = iy + 1;
         const int lx = gx > 0 ? bx - 1 : bx, uy = gy > 0 ? by - 1 : by;
         float un, u1xn, u1yn;
         u_update(u, u1x, u1y, a.T[g], a.T[n + g], a.T[2 * n + g], sP[0][by][bx], sP[1][by][bx], sP[0][by][lx],
@@ -591,51 +634,49 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile)
     __syncthreads();
 
     // tile: r and u of iteration k+2
-    {
-        const int ty = tid / TX, tx = tid - (tid / TX) * TX;
-        const int x = x0 + tx, y = y0 + ty;
-        if (x < W && y < H) {
-            const size_t g = (size_t)y * W + x;
-            const int cx = tx + 1, cy = ty + 1;           // C index
-            const float u = sU1[0][cy][cx], u1x = sU1[1][cy][cx], u1y = sU1[2][cy][cx];
-            const float du = u - a.u0[g];
-            float ps = 0.f;
-            for (int j = 0; j < a.N; ++j) {
-                const size_t gj = (size_t)j * n + g;
-                const float iu = a.Iu[gj];
-                const float rr = r_update(sR[j][ty][tx], a.It[gj], du, iu, sl);
-                a.rnext2[gj] = rr;
-                ps += rr * iu;
+    if (x < W && y < H) {
+        const size_t g = (size_t)y * W + x;
+        const int cx = tx + 1, cy = ty + 1;               // C index
+        const float u = sU1[0][cy][cx], u1x = sU1[1][cy][cx], u1y = sU1[2][cy][cx];
+        const float du = u - u0t;
+        float ps = 0.f;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            if (j < a.N) {
+                const float rr = r_update(rv[j], itv[j], du, iuv[j], sl);
+                a.rnext2[(size_t)j * n + g] = rr;
+                ps += rr * iuv[j];
             }
-            const int bx = tx + 2, by = ty + 2;           // B index of this pixel (D index + 1)
-            const int lx = x > 0 ? bx - 1 : bx, uy = y > 0 ? by - 1 : by;
-            float un, u1xn, u1yn;
-            u_update(u, u1x, u1y, a.T[g], a.T[n + g], a.T[2 * n + g], sP[0][by][bx], sP[1][by][bx], sP[0][by][lx],
-                     sP[1][by][lx], sP[0][uy][bx], sP[1][uy][bx], sQ[0][by][bx], sQ[0][by][lx], sQ[1][by][bx],
-                     sQ[1][uy][bx], sQ[2][by][bx], sQ[2][by][lx], sQ[2][uy][bx], ps, a, un, u1xn, u1yn);
-            a.unext2[g] = un; a.unext2[n + g] = u1xn; a.unext2[2 * n + g] = u1yn;
-            a.unext[g] = u; a.unext[n + g] = u1x; a.unext[2 * n + g] = u1y;
         }
+        const int bx = tx + 2, by = ty + 2;               // B index of this pixel (D index + 1)
+        const int lx = x > 0 ? bx - 1 : bx, uy = y > 0 ? by - 1 : by;
+        float un, u1xn, u1yn;
+        u_update(u, u1x, u1y, t11, t12, t22, sP[0][by][bx], sP[1][by][bx], sP[0][by][lx], sP[1][by][lx],
+                 sP[0][uy][bx], sP[1][uy][bx], sQ[0][by][bx], sQ[0][by][lx], sQ[1][by][bx], sQ[1][uy][bx],
+                 sQ[2][by][bx], sQ[2][by][lx], sQ[2][uy][bx], ps, a, un, u1xn, u1yn);
+        a.unext2[g] = un; a.unext2[n + g] = u1xn; a.unext2[2 * n + g] = u1yn;
+        a.unext[g] = u; a.unext[n + g] = u1x; a.unext[2 * n + g] = u1y;
     }
 }
 
+template <int NS>
 __global__ __launch_bounds__(T2_THREADS) void tgv_iter2_kernel(TgvPtrs a) {
     if (a.ctrl->done) return;
     const int ntiles = ((a.W + TG_TX - 1) / TG_TX) * ((a.H + T2_TY - 1) / T2_TY);
-    tgv_iter2_tile(a, xcd_tile(blockIdx.x, ntiles));
+    tgv_iter2_tile<NS>(a, xcd_tile(blockIdx.x, ntiles));
 }
 
-template <int TY, int NT>
+template <int TY, int NT, int NS>
 __global__ __launch_bounds__(NT) void tgv_iter_kernel(TgvPtrs a) {
     if (a.ctrl->done) return;
     const int ntiles = ((a.W + TG_TX - 1) / TG_TX) * ((a.H + TY - 1) / TY);
-    tgv_iter_tile<TY, NT>(a, xcd_tile(blockIdx.x, ntiles));
+    tgv_iter_tile<TY, NT, NS>(a, xcd_tile(blockIdx.x, ntiles));
 }
 
 // One iteration k -> k+1 that also sums the energy of state k (ENERGY)
 // and / or finalises the previous energy launch's sum in one extra
 // workgroup (FINAL).
-template <int TY, int NT, bool ENERGY, bool FINAL>
+template <int TY, int NT, int NS, bool ENERGY, bool FINAL>
 __global__ __launch_bounds__(NT) void tgv_iter_check_kernel(TgvPtrs a, TgvCheck c) {
     if (a.ctrl->done) return;
     const int ntiles = ((a.W + TG_TX - 1) / TG_TX) * ((a.H + TY - 1) / TY);
@@ -643,7 +684,7 @@ __global__ __launch_bounds__(NT) void tgv_iter_check_kernel(TgvPtrs a, TgvCheck 
         check_finalize<NT>(c);
         return;
     }
-    tgv_iter_tile<TY, NT, ENERGY>(a, xcd_tile(blockIdx.x, ntiles), &c);
+    tgv_iter_tile<TY, NT, NS, ENERGY>(a, xcd_tile(blockIdx.x, ntiles), &c);
 }
 
 // ---- FP64 energy -------------------------------------------------------------
@@ -763,16 +804,22 @@ static int tgv_variant() {
     return (x < 0 || x > 2) ? 1 : x;
 }
 
-// Two iterations per launch (tgv_iter2_kernel) when PSW_TGV_DOUBLE=1 and the
-// frame has at most T2_MAXN sources; identical results either way.  Off by
-// default: on MI355X at 640x480 x 8 sources it moves 1.6x fewer bytes per
-// iteration but runs 16.5 us per iteration against 13.2 us for the
-// one-iteration kernel (73 KB of LDS per workgroup leaves two workgroups per
-// CU, too few to hide the latency of its dependent phases).
+// Two iterations per launch (tgv_iter2_kernel) unless PSW_TGV_DOUBLE=0;
+// identical results either way.
 static bool tgv_double() {
     const char* e = getenv("PSW_TGV_DOUBLE");
-    return e && atoi(e) == 1;
+    return !(e && atoi(e) == 0);
 }
+
+static int tgv_launch_double(psw_tgv* t, const psw_tgv_params* prm, hipStream_t st) {
+    const dim3 grid2((unsigned)(((t->W + TG_TX - 1) / TG_TX) * ((t->H + T2_TY - 1) / T2_TY)));
+    if (t->N <= 8) tgv_iter2_kernel<8><<<grid2, T2_THREADS, 0, st>>>(tgv_ptrs(t, prm));
+    else tgv_iter2_kernel<T2_MAXN><<<grid2, T2_THREADS, 0, st>>>(tgv_ptrs(t, prm));
+    const int s = launch_status("tgv_iter2_kernel");
+    if (s == PSW_OK) t->k += 2;
+    return s;
+}
+
 
 // Partials of the checking launches: one per tile of the smallest tile shape.
 static int tgv_tile_partials(int W, int H) { return ((W + TG_TX - 1) / TG_TX) * ((H + 3) / 4); }
@@ -780,13 +827,20 @@ static int tgv_tile_partials(int W, int H) { return ((W + TG_TX - 1) / TG_TX) * 
 // One iteration launch of the solve loop: ENERGY sums the energy of the
 // current state into c.partials, FINAL finalises c.fin_partials in one
 // extra workgroup.
+template <int TY, int NT, int NS>
+static void tgv_launch_check_ns(const TgvPtrs& p, const TgvCheck& c, bool energy, bool fin, int ntiles,
+                                hipStream_t st) {
+    const unsigned g = (unsigned)(ntiles + (fin ? 1 : 0));
+    if (energy && fin) tgv_iter_check_kernel<TY, NT, NS, true, true><<<g, NT, 0, st>>>(p, c);
+    else if (energy) tgv_iter_check_kernel<TY, NT, NS, true, false><<<g, NT, 0, st>>>(p, c);
+    else if (fin) tgv_iter_check_kernel<TY, NT, NS, false, true><<<g, NT, 0, st>>>(p, c);
+    else tgv_iter_kernel<TY, NT, NS><<<g, NT, 0, st>>>(p);
+}
+
 template <int TY, int NT>
 static void tgv_launch_check(const TgvPtrs& p, const TgvCheck& c, bool energy, bool fin, int ntiles, hipStream_t st) {
-    const unsigned g = (unsigned)(ntiles + (fin ? 1 : 0));
-    if (energy && fin) tgv_iter_check_kernel<TY, NT, true, true><<<g, NT, 0, st>>>(p, c);
-    else if (energy) tgv_iter_check_kernel<TY, NT, true, false><<<g, NT, 0, st>>>(p, c);
-    else if (fin) tgv_iter_check_kernel<TY, NT, false, true><<<g, NT, 0, st>>>(p, c);
-    else tgv_iter_kernel<TY, NT><<<g, NT, 0, st>>>(p);
+    if (p.N <= 8) tgv_launch_check_ns<TY, NT, 8>(p, c, energy, fin, ntiles, st);
+    else tgv_launch_check_ns<TY, NT, PSW_MAX_SOURCES>(p, c, energy, fin, ntiles, st);
 }
 
 static int tgv_launch_one(psw_tgv* t, const psw_tgv_params* prm, const TgvCheck& c, bool energy, bool fin,
@@ -808,19 +862,17 @@ static int tgv_launch_iters(psw_tgv* t, int niters, const psw_tgv_params* prm, h
     const int ty = var == 2 ? 4 : 8;
     const dim3 grid((unsigned)(((t->W + TG_TX - 1) / TG_TX) * ((t->H + ty - 1) / ty)));
     if (tgv_double() && t->N <= T2_MAXN) {
-        const dim3 grid2((unsigned)(((t->W + TG_TX - 1) / TG_TX) * ((t->H + T2_TY - 1) / T2_TY)));
         for (; niters >= 2; niters -= 2) {
-            tgv_iter2_kernel<<<grid2, T2_THREADS, 0, st>>>(tgv_ptrs(t, prm));
-            int s = launch_status("tgv_iter2_kernel");
+            const int s = tgv_launch_double(t, prm, st);
             if (s != PSW_OK) return s;
-            t->k += 2;
         }
     }
+    TgvCheck none{};
     for (int i = 0; i < niters; ++i) {
         const TgvPtrs p = tgv_ptrs(t, prm);
-        if (var == 0) tgv_iter_kernel<8, 256><<<grid, 256, 0, st>>>(p);
-        else if (var == 1) tgv_iter_kernel<8, 512><<<grid, 512, 0, st>>>(p);
-        else tgv_iter_kernel<4, 256><<<grid, 256, 0, st>>>(p);
+        if (var == 0) tgv_launch_check<8, 256>(p, none, false, false, (int)grid.x, st);
+        else if (var == 1) tgv_launch_check<8, 512>(p, none, false, false, (int)grid.x, st);
+        else tgv_launch_check<4, 256>(p, none, false, false, (int)grid.x, st);
         int s = launch_status("tgv_iter_kernel");
         if (s != PSW_OK) return s;
         t->k++;
@@ -885,7 +937,7 @@ __global__ __launch_bounds__(NT) void tgv_iter_batch_kernel(const TgvFrameDev* _
     if (d.ctrl->done) return;
     const TgvPtrs a = tgv_ptrs_at(d.base, d.L, d.W, d.H, d.N, d.k0 + *kdev, d.ctrl, d.lambda, d.alpha0, d.alpha1, d.tau,
                                   d.sigma);
-    tgv_iter_tile<TY, NT>(a, (int)blockIdx.x - toff[f]);
+    tgv_iter_tile<TY, NT, PSW_MAX_SOURCES>(a, (int)blockIdx.x - toff[f]);
 }
 
 __global__ void tgv_batch_advance_kernel(int* kdev, int by) {
@@ -1163,6 +1215,11 @@ int psw_tgv_solve(psw_tgv* t, int max_iters, int check_every, double tol, double
     hipEvent_t ev[2] = {nullptr, nullptr};
     PSW_TRY_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming), PSW_ERR_DEVICE);
     PSW_TRY_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming), PSW_ERR_DEVICE);
+    // The energy and finalising launches are one-iteration launches: a stop
+    // at state k is then seen by the launch from k+1, which writes state k+2
+    // only (u cycles through 4 slots, p, q, r through 3), so states k and
+    // k-1 stay intact.  The other launches of a chunk do two iterations each.
+    const bool dbl = tgv_double() && t->N <= T2_MAXN;
     int done_iters = -1, chunk_idx = 0, nchecks = 0;
     int launched = 0;
     bool stopped = false, pending = false;   // pending: an energy sum not yet finalised
@@ -1170,9 +1227,14 @@ int psw_tgv_solve(psw_tgv* t, int max_iters, int check_every, double tol, double
     int pend_k = 0;
     while (launched < max_iters && rc == PSW_OK) {
         const int c = std::min(check_every, max_iters - launched);
-        for (int i = 0; i < c && rc == PSW_OK; ++i) {
+        for (int i = 0; i < c && rc == PSW_OK;) {
             const bool energy = i == 0 && launched > 0;
             const bool fin = pending;
+            if (!energy && !fin && dbl && c - i >= 2) {   // two plain iterations in one launch
+                rc = tgv_launch_double(t, prm, st);
+                i += 2;
+                continue;
+            }
             if (fin) {
                 chk.fin_partials = pend_buf;
                 chk.fin_k = pend_k;
@@ -1186,6 +1248,7 @@ int psw_tgv_solve(psw_tgv* t, int max_iters, int check_every, double tol, double
                 nchecks++;
             }
             pending = energy;
+            ++i;
         }
         if (rc != PSW_OK) break;
         launched += c;

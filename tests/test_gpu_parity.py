@@ -364,14 +364,15 @@ def test_tgv_solve_stops_at_gap(gpu, views_small):
         np.testing.assert_array_equal(sa[name], sb[name], err_msg=name)
 
 
-@pytest.mark.parametrize("variant", ["0", "1", "2"])
-def test_tgv_solve_max_iters_and_variants(gpu, views_small, monkeypatch, variant):
-    """Every launch shape stops at the same iteration with the same state;
-    a solve that reaches max_iters reports max_iters and the energy of the
-    final state."""
+@pytest.mark.parametrize("variant,double", [("0", "1"), ("1", "1"), ("2", "1"), ("1", "0")])
+def test_tgv_solve_max_iters_and_variants(gpu, views_small, monkeypatch, variant, double):
+    """Every launch shape, with and without two-iteration launches, stops at
+    the same iteration with the same state; a solve that reaches max_iters
+    reports max_iters and the energy of the final state."""
     from psw_amd import stages
     from psw_amd.device import DeviceArray
     monkeypatch.setenv("PSW_TGV_VARIANT", variant)
+    monkeypatch.setenv("PSW_TGV_DOUBLE", double)
     v = views_small
     K1 = one_based(v["K"])
     prm = stages.tgv_params(**TGV_KW)
@@ -395,6 +396,18 @@ def test_tgv_solve_max_iters_and_variants(gpu, views_small, monkeypatch, variant
     it2, e2 = t.solve(400, 7, 2e-2, e_star, prm)
     assert it2 < 400 and it2 % 7 == 0 and abs(e2 - e_star) <= 2e-2 * e_star
     assert abs(t.energy(prm) - e2) <= 1e-12 * e2
+    s_stop = t.get_state()
+    t.iterate(3, prm)                                    # resumes from the stop (needs u_{k-1} intact)
+    ref = stages.TgvSolver(W, H, N)
+    ref.set_images(d_ref, d_src)
+    ref.set_u(value=1.0)
+    ref.linearise(K1, Rrel, trel)
+    ref.iterate(it2, prm)
+    s_ref = ref.get_state()
+    for name in ("u", "u1x", "u1y", "px", "py", "qx", "qy", "qz", "r"):
+        np.testing.assert_array_equal(s_stop[name], s_ref[name], err_msg=name)
+    ref.iterate(3, prm)
+    np.testing.assert_array_equal(t.get_u(), ref.get_u())
 
 
 # ---- stage 3: fusion ---------------------------------------------------------------------
