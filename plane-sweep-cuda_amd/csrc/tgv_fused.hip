@@ -904,9 +904,11 @@ __global__ void tgv_ctrl_reset_kernel(TgvCtrl* c) {
 // ---- many frames per launch ---------------------------------------------------
 
 // One frame of a batch as the device sees it; the iteration count of every
-// frame is its own start k0 plus a batch counter kdev that a one-thread
-// kernel advances after each iteration launch (so a captured chunk of
-// launches can be replayed).
+// frame is its own start k0 plus a batch counter.  The counter lives in two
+// device copies: an iteration launch reads copy rd and (block 0) writes the
+// advanced count to copy wr != rd, so no launch writes what its own blocks
+// read, and a captured chunk of launches can be replayed; the finalising
+// kernel of a chunk moves the count back to copy 0.
 struct TgvFrameDev {
     float* base;
     TgvLayout L;
@@ -930,20 +932,34 @@ __device__ inline int batch_frame(const int* __restrict__ off, int nf, int b) {
 
 template <int TY, int NT>
 __global__ __launch_bounds__(NT) void tgv_iter_batch_kernel(const TgvFrameDev* __restrict__ fr,
-                                                            const int* __restrict__ toff, int nf,
-                                                            const int* __restrict__ kdev) {
+                                                            const int* __restrict__ toff, int nf, int* kd, int rd,
+                                                            int wr) {
+    const int kb = kd[rd];
+    if (blockIdx.x == 0 && threadIdx.x == 0) kd[wr] = kb + 1;
     const int f = batch_frame(toff, nf, blockIdx.x);
     const TgvFrameDev& d = fr[f];
     if (d.ctrl->done) return;
-    const TgvPtrs a = tgv_ptrs_at(d.base, d.L, d.W, d.H, d.N, d.k0 + *kdev, d.ctrl, d.lambda, d.alpha0, d.alpha1, d.tau,
+    const TgvPtrs a = tgv_ptrs_at(d.base, d.L, d.W, d.H, d.N, d.k0 + kb, d.ctrl, d.lambda, d.alpha0, d.alpha1, d.tau,
                                   d.sigma);
     tgv_iter_tile<TY, NT, PSW_MAX_SOURCES>(a, (int)blockIdx.x - toff[f]);
 }
 
-__global__ void tgv_batch_advance_kernel(int* kdev, int by) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *kdev += by;
+// Two iterations of every active frame (64x8 tiles).
+template <int NS>
+__global__ __launch_bounds__(T2_THREADS) void tgv_iter2_batch_kernel(const TgvFrameDev* __restrict__ fr,
+                                                                    const int* __restrict__ toff, int nf, int* kd,
+                                                                    int rd, int wr) {
+    const int kb = kd[rd];
+    if (blockIdx.x == 0 && threadIdx.x == 0) kd[wr] = kb + 2;
+    const int f = batch_frame(toff, nf, blockIdx.x);
+    const TgvFrameDev& d = fr[f];
+    if (d.ctrl->done) return;
+    const TgvPtrs a = tgv_ptrs_at(d.base, d.L, d.W, d.H, d.N, d.k0 + kb, d.ctrl, d.lambda, d.alpha0, d.alpha1, d.tau,
+                                  d.sigma);
+    tgv_iter2_tile<NS>(a, (int)blockIdx.x - toff[f]);
 }
 
+// kdev: the copy of the batch counter that holds the current count.
 __global__ __launch_bounds__(TG_THREADS) void tgv_energy_batch_kernel(const TgvFrameDev* __restrict__ fr,
                                                                    const int* __restrict__ eoff, int nf,
                                                                    const int* __restrict__ kdev) {
@@ -959,11 +975,13 @@ __global__ __launch_bounds__(TG_THREADS) void tgv_energy_batch_kernel(const TgvF
 }
 
 __global__ __launch_bounds__(TG_THREADS) void tgv_finalize_batch_kernel(const TgvFrameDev* __restrict__ fr,
-                                                                     const int* __restrict__ kdev, double tol,
+                                                                     int* kd, int rd, double tol,
                                                                      int* __restrict__ ndone) {
     const TgvFrameDev& d = fr[blockIdx.x];
-    const bool stopped = tgv_finalize_block(d.partials, d.eblocks, d.ctrl, d.k0 + *kdev, tol, d.e_star);
+    const int kb = kd[rd];
+    const bool stopped = tgv_finalize_block(d.partials, d.eblocks, d.ctrl, d.k0 + kb, tol, d.e_star);
     if (threadIdx.x == 0 && stopped) atomicAdd(ndone, 1);
+    if (rd != 0 && blockIdx.x == 0 && threadIdx.x == 0) kd[0] = kb;   // the next chunk starts from copy 0
 }
 
 static int tgv_reset_ctrl(psw_tgv* t, hipStream_t st) {
@@ -1337,9 +1355,10 @@ struct TgvBatch {
     TgvFrameDev* frames = nullptr;
     int* toff = nullptr;
     int* eoff = nullptr;
-    int* kdev = nullptr;
+    int* kdev = nullptr;                // two copies of the batch counter
     int* ndone = nullptr;
     int tiles = 0, eblocks = 0, ty = 8, nt = 512;
+    bool dbl = false;                   // two iterations per launch
     int nact = 0;                       // frames in the device tables
     std::vector<TgvFrameDev> all;       // every frame of the batch
     std::vector<int> active;            // indices (into all) still iterating
@@ -1378,8 +1397,10 @@ static int tgv_batch_setup(psw_tgv* const* ts, int nf, const psw_tgv_params* prm
                            TgvBatch& b, hipStream_t st) {
     PSW_CHECK_ARG(ts && prm && nf >= 1, "NULL pointer or no frames");
     const int var = variant >= 0 && variant <= 2 ? variant : tgv_variant();
-    b.ty = var == 2 ? 4 : 8;
-    b.nt = var == 1 ? 512 : 256;
+    b.dbl = tgv_double();
+    for (int f = 0; f < nf; ++f) b.dbl = b.dbl && ts[f] && ts[f]->N <= T2_MAXN;
+    b.ty = var == 2 && !b.dbl ? 4 : 8;    // the two-iteration kernel tiles 64x8
+    b.nt = var == 1 || b.dbl ? 512 : 256;
     b.all.assign((size_t)nf, TgvFrameDev{});
     b.active.clear();
     for (int f = 0; f < nf; ++f) {
@@ -1398,34 +1419,52 @@ static int tgv_batch_setup(psw_tgv* const* ts, int nf, const psw_tgv_params* prm
         b.active.push_back(f);
     }
     const size_t fb = sizeof(TgvFrameDev) * (size_t)nf, ob = sizeof(int) * ((size_t)nf + 1);
-    PSW_TRY_HIP(hipMalloc((void**)&b.dev, fb + 2 * ob + 2 * sizeof(int)), PSW_ERR_ALLOC);
+    PSW_TRY_HIP(hipMalloc((void**)&b.dev, fb + 2 * ob + 3 * sizeof(int)), PSW_ERR_ALLOC);
     b.frames = reinterpret_cast<TgvFrameDev*>(b.dev);
     b.toff = reinterpret_cast<int*>(b.dev + fb);
     b.eoff = reinterpret_cast<int*>(b.dev + fb + ob);
     b.kdev = reinterpret_cast<int*>(b.dev + fb + 2 * ob);
-    b.ndone = b.kdev + 1;
-    int rc = psw_fill_u32(reinterpret_cast<uint32_t*>(b.kdev), 0u, 2, st);
+    b.ndone = b.kdev + 2;
+    int rc = psw_fill_u32(reinterpret_cast<uint32_t*>(b.kdev), 0u, 3, st);
     for (int f = 0; f < nf && rc == PSW_OK; ++f) rc = tgv_reset_ctrl(ts[f], st);
     if (rc == PSW_OK) rc = b.upload(st);
     return rc;
 }
 
-// c iterations of every frame (each followed by the counter advance).
-static int tgv_batch_iters(const TgvBatch& b, int c, hipStream_t st) {
+// c iterations of every frame, starting from counter copy 0; *cur gets the
+// copy that holds the count afterwards.
+static int tgv_batch_iters(const TgvBatch& b, int c, hipStream_t st, int* cur) {
     const int nf = b.nact;
-    for (int i = 0; i < c; ++i) {
-        if (b.ty == 4) tgv_iter_batch_kernel<4, 256><<<b.tiles, 256, 0, st>>>(b.frames, b.toff, nf, b.kdev);
-        else if (b.nt == 512) tgv_iter_batch_kernel<8, 512><<<b.tiles, 512, 0, st>>>(b.frames, b.toff, nf, b.kdev);
-        else tgv_iter_batch_kernel<8, 256><<<b.tiles, 256, 0, st>>>(b.frames, b.toff, nf, b.kdev);
-        tgv_batch_advance_kernel<<<1, 64, 0, st>>>(b.kdev, 1);
+    int rd = 0;
+    bool all8 = true;
+    for (int f : b.active) all8 = all8 && b.all[(size_t)f].N <= 8;
+    for (int i = 0; i < c;) {
+        const int wr = 1 - rd;
+        if (b.dbl && c - i >= 2) {
+            if (all8) tgv_iter2_batch_kernel<8><<<b.tiles, T2_THREADS, 0, st>>>(b.frames, b.toff, nf, b.kdev, rd, wr);
+            else tgv_iter2_batch_kernel<T2_MAXN><<<b.tiles, T2_THREADS, 0, st>>>(b.frames, b.toff, nf, b.kdev, rd, wr);
+            i += 2;
+        } else {
+            if (b.ty == 4)
+                tgv_iter_batch_kernel<4, 256><<<b.tiles, 256, 0, st>>>(b.frames, b.toff, nf, b.kdev, rd, wr);
+            else if (b.nt == 512)
+                tgv_iter_batch_kernel<8, 512><<<b.tiles, 512, 0, st>>>(b.frames, b.toff, nf, b.kdev, rd, wr);
+            else
+                tgv_iter_batch_kernel<8, 256><<<b.tiles, 256, 0, st>>>(b.frames, b.toff, nf, b.kdev, rd, wr);
+            i += 1;
+        }
+        rd = wr;
     }
+    *cur = rd;
     return launch_status("tgv_iter_batch_kernel");
 }
 
-static int tgv_batch_check(const TgvBatch& b, double tol, hipStream_t st) {
+// The energy test of every active frame at the count in copy cur; the
+// finaliser moves the count to copy 0.
+static int tgv_batch_check(const TgvBatch& b, double tol, int cur, hipStream_t st) {
     const int nf = b.nact;
-    tgv_energy_batch_kernel<<<b.eblocks, TG_THREADS, 0, st>>>(b.frames, b.eoff, nf, b.kdev);
-    tgv_finalize_batch_kernel<<<nf, TG_THREADS, 0, st>>>(b.frames, b.kdev, tol, b.ndone);
+    tgv_energy_batch_kernel<<<b.eblocks, TG_THREADS, 0, st>>>(b.frames, b.eoff, nf, b.kdev + cur);
+    tgv_finalize_batch_kernel<<<nf, TG_THREADS, 0, st>>>(b.frames, b.kdev, cur, tol, b.ndone);
     return launch_status("tgv_energy_batch_kernel");
 }
 
@@ -1435,7 +1474,8 @@ int psw_tgv_iterate_batch(psw_tgv* const* solvers, int nsolvers, int niters, con
     hipStream_t st = S(s);
     TgvBatch b;
     int rc = tgv_batch_setup(solvers, nsolvers, prm, nullptr, variant, b, st);
-    if (rc == PSW_OK) rc = tgv_batch_iters(b, niters, st);
+    int cur = 0;
+    if (rc == PSW_OK) rc = tgv_batch_iters(b, niters, st, &cur);
     if (rc == PSW_OK) PSW_TRY_HIP(hipStreamSynchronize(st), PSW_ERR_DEVICE);
     if (rc != PSW_OK) return rc;
     for (int f = 0; f < nsolvers; ++f) solvers[f]->k += niters;
@@ -1468,8 +1508,9 @@ int psw_tgv_solve_batch(psw_tgv* const* solvers, int nsolvers, const psw_tgv_par
         if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
         hipError_t ce = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
         if (ce != hipSuccess) return hip_status(ce, "hipStreamBeginCapture");
-        int crc = tgv_batch_iters(b, check_every, st);
-        if (crc == PSW_OK) crc = tgv_batch_check(b, tol, st);
+        int cur = 0;
+        int crc = tgv_batch_iters(b, check_every, st, &cur);
+        if (crc == PSW_OK) crc = tgv_batch_check(b, tol, cur, st);
         if (crc == PSW_OK) ce = hipMemcpyAsync(h_ndone, b.ndone, sizeof(int), hipMemcpyDeviceToHost, st);
         const hipError_t ee = hipStreamEndCapture(st, &graph);
         if (ce == hipSuccess) ce = ee;
@@ -1485,8 +1526,9 @@ int psw_tgv_solve_batch(psw_tgv* const* solvers, int nsolvers, const psw_tgv_par
             e = hipGraphLaunch(exec, st);
             if (e != hipSuccess) { rc = hip_status(e, "hipGraphLaunch"); break; }
         } else {
-            rc = tgv_batch_iters(b, c, st);
-            if (rc == PSW_OK) rc = tgv_batch_check(b, tol, st);
+            int cur = 0;
+            rc = tgv_batch_iters(b, c, st, &cur);
+            if (rc == PSW_OK) rc = tgv_batch_check(b, tol, cur, st);
             if (rc != PSW_OK) break;
             e = hipMemcpyAsync(h_ndone, b.ndone, sizeof(int), hipMemcpyDeviceToHost, st);
             if (e != hipSuccess) { rc = hip_status(e, "hipMemcpyAsync"); break; }
