@@ -29,6 +29,11 @@ namespace psw {
 
 constexpr int TG_TX = 64;
 constexpr int TG_THREADS = 256;   // energy kernels
+// u cycles through 6 slots, p, q, r through 3.  Two two-iteration launches
+// from state k (the energy launch and the finalising launch of a check)
+// write u of k+1..k+4 and p, q, r of k+2 and k+4, so a stop at k leaves u_k,
+// u_{k-1}, p_k, q_k, r_k in place.
+constexpr int TG_USLOTS = 6;
 
 struct TgvLayout {
     size_t n;
@@ -36,7 +41,7 @@ struct TgvLayout {
     void init(int W, int H, int N) {
         n = (size_t)W * H;
         size_t o = 0;
-        U = o;  o += 12 * n;     // [4 slots][u, u1x, u1y]: u_k in slot k % 4
+        U = o;  o += 3 * TG_USLOTS * n;  // [TG_USLOTS slots][u, u1x, u1y]: u_k in slot k % TG_USLOTS
         P = o;  o += 6 * n;      // [3 slots][px, py]: p_k in slot k % 3
         Q = o;  o += 9 * n;      // [3 slots][qx, qy, qz]
         r = o;  o += (size_t)3 * N * n;  // [3 slots][N][n]
@@ -87,7 +92,7 @@ struct TgvPoses {
 
 // Slots by iteration count k (so that launches doing one or two iterations
 // mix freely and a no-op launch after the stop leaves the mapping intact):
-// u_k in U slot k % 4, p_k / q_k / r_k in slot k % 3.  A launch reads the
+// u_k in U slot k % TG_USLOTS, p_k / q_k / r_k in slot k % 3.  A launch reads the
 // slots of k (and k-1 for u) and writes those of k+1 (and k+2).
 struct TgvPtrs {
     const float* ucur;   // [3][n] u_k
@@ -489,8 +494,8 @@ __device__ __forceinline__ int clampi(int v, int hi) { return min(max(v, 0), hi)
 // One tile pixel per thread (NT == TX * TY): thread t owns pixel (t / TX,
 // t % TX) in both iterations and keeps its r_{k+1}, It, Iu, u0 and T in
 // registers (NS >= a.N sources), so no source plane is read twice.
-template <int NS>
-__device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile) {
+template <int NS, bool ENERGY = false>
+__device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile, const TgvCheck* chk = nullptr) {
     constexpr int TX = TG_TX, TY = T2_TY, NT = T2_THREADS;
     static_assert(NT == TX * TY, "one tile pixel per thread");
     constexpr int AX = TX + 4, AY = TY + 4;
@@ -558,19 +563,28 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile)
     __syncthreads();
 
     // C: r and u of iteration k+1; the tile pixel from registers (its r_{k+1}
-    // stays there), then the ring
+    // stays there), then the ring.  An energy launch also sums the energy of
+    // state k at the tile pixel (u_k with its right / lower neighbours in sUk).
+    double eacc = 0.0;
     {
         const int ix = tx + 1, iy = ty + 1;
         const float u = sUk[0][iy][ix], u1x = sUk[1][iy][ix], u1y = sUk[2][iy][ix];
         const float du = u - u0t;
+        const double dud = (double)u - (double)u0t;
         float ps = 0.f;
+        double ed = 0.0;
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
             if (j < a.N) {
+                if (ENERGY) ed += fabs((double)itv[j] + dud * (double)iuv[j]);
                 rv[j] = r_update(rv[j], itv[j], du, iuv[j], sl);
                 ps += rv[j] * iuv[j];
             }
         }
+        if (ENERGY && x < W && y < H && chk->mask[gt] != 0.f)
+            eacc = energy_point(u, sUk[0][iy][ix + 1], sUk[0][iy + 1][ix], u1x, sUk[1][iy][ix + 1], sUk[1][iy + 1][ix],
+                                u1y, sUk[2][iy][ix + 1], sUk[2][iy + 1][ix], t11, t12, t22, ed, chk->lambda,
+                                chk->alpha0, chk->alpha1);
         const int bx = ix + 1, by = iy + 1;               // B index of this position
         const int lx = x > 0 ? bx - 1 : bx, uy = y > 0 ? by - 1 : by;
         float un, u1xn, u1yn;
@@ -657,6 +671,10 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile)
         a.unext2[g] = un; a.unext2[n + g] = u1xn; a.unext2[2 * n + g] = u1yn;
         a.unext[g] = u; a.unext[n + g] = u1x; a.unext[2 * n + g] = u1y;
     }
+    if (ENERGY) {
+        const double e = block_sum<NT>(eacc);
+        if (tid == 0) chk->partials[tile] = e;
+    }
 }
 
 template <int NS>
@@ -664,6 +682,20 @@ __global__ __launch_bounds__(T2_THREADS) void tgv_iter2_kernel(TgvPtrs a) {
     if (a.ctrl->done) return;
     const int ntiles = ((a.W + TG_TX - 1) / TG_TX) * ((a.H + T2_TY - 1) / T2_TY);
     tgv_iter2_tile<NS>(a, xcd_tile(blockIdx.x, ntiles));
+}
+
+// Two iterations from state k that also sum the energy of state k (ENERGY)
+// and / or finalise the previous energy launch's sum in one extra
+// workgroup (FINAL).
+template <int NS, bool ENERGY, bool FINAL>
+__global__ __launch_bounds__(T2_THREADS) void tgv_iter2_check_kernel(TgvPtrs a, TgvCheck c) {
+    if (a.ctrl->done) return;
+    const int ntiles = ((a.W + TG_TX - 1) / TG_TX) * ((a.H + T2_TY - 1) / T2_TY);
+    if (FINAL && (int)blockIdx.x == ntiles) {
+        check_finalize<T2_THREADS>(c);
+        return;
+    }
+    tgv_iter2_tile<NS, ENERGY>(a, xcd_tile(blockIdx.x, ntiles), &c);
 }
 
 template <int TY, int NT, int NS>
@@ -768,10 +800,10 @@ __host__ __device__ inline TgvPtrs tgv_ptrs_at(float* B, const TgvLayout& L, int
     const size_t n = L.n;
     const size_t rn = (size_t)N * n;
     TgvPtrs p;
-    p.ucur = B + L.U + (size_t)(k % 4) * 3 * n;
-    p.uprev = B + L.U + (size_t)((k + 3) % 4) * 3 * n;
-    p.unext = B + L.U + (size_t)((k + 1) % 4) * 3 * n;
-    p.unext2 = B + L.U + (size_t)((k + 2) % 4) * 3 * n;
+    p.ucur = B + L.U + (size_t)(k % TG_USLOTS) * 3 * n;
+    p.uprev = B + L.U + (size_t)((k + TG_USLOTS - 1) % TG_USLOTS) * 3 * n;
+    p.unext = B + L.U + (size_t)((k + 1) % TG_USLOTS) * 3 * n;
+    p.unext2 = B + L.U + (size_t)((k + 2) % TG_USLOTS) * 3 * n;
     p.pcur = B + L.P + (size_t)(k % 3) * 2 * n;
     p.pnext = B + L.P + (size_t)((k + 1) % 3) * 2 * n;
     p.pnext2 = B + L.P + (size_t)((k + 2) % 3) * 2 * n;
@@ -811,11 +843,26 @@ static bool tgv_double() {
     return !(e && atoi(e) == 0);
 }
 
-static int tgv_launch_double(psw_tgv* t, const psw_tgv_params* prm, hipStream_t st) {
-    const dim3 grid2((unsigned)(((t->W + TG_TX - 1) / TG_TX) * ((t->H + T2_TY - 1) / T2_TY)));
-    if (t->N <= 8) tgv_iter2_kernel<8><<<grid2, T2_THREADS, 0, st>>>(tgv_ptrs(t, prm));
-    else tgv_iter2_kernel<T2_MAXN><<<grid2, T2_THREADS, 0, st>>>(tgv_ptrs(t, prm));
-    const int s = launch_status("tgv_iter2_kernel");
+static int tgv_tiles2(const psw_tgv* t) { return ((t->W + TG_TX - 1) / TG_TX) * ((t->H + T2_TY - 1) / T2_TY); }
+
+template <int NS>
+static void tgv_launch_double_ns(const TgvPtrs& p, const TgvCheck* c, bool energy, bool fin, int ntiles,
+                                 hipStream_t st) {
+    const unsigned g = (unsigned)(ntiles + (fin ? 1 : 0));
+    if (energy && fin) tgv_iter2_check_kernel<NS, true, true><<<g, T2_THREADS, 0, st>>>(p, *c);
+    else if (energy) tgv_iter2_check_kernel<NS, true, false><<<g, T2_THREADS, 0, st>>>(p, *c);
+    else if (fin) tgv_iter2_check_kernel<NS, false, true><<<g, T2_THREADS, 0, st>>>(p, *c);
+    else tgv_iter2_kernel<NS><<<g, T2_THREADS, 0, st>>>(p);
+}
+
+// Two iterations in one launch; with a check (c non-NULL) the launch may
+// also sum the energy of the current state and / or finalise a previous sum.
+static int tgv_launch_double(psw_tgv* t, const psw_tgv_params* prm, hipStream_t st, const TgvCheck* c = nullptr,
+                             bool energy = false, bool fin = false) {
+    const TgvPtrs p = tgv_ptrs(t, prm);
+    if (t->N <= 8) tgv_launch_double_ns<8>(p, c, energy, fin, tgv_tiles2(t), st);
+    else tgv_launch_double_ns<T2_MAXN>(p, c, energy, fin, tgv_tiles2(t), st);
+    const int s = launch_status(energy || fin ? "tgv_iter2_check_kernel" : "tgv_iter2_kernel");
     if (s == PSW_OK) t->k += 2;
     return s;
 }
@@ -882,7 +929,7 @@ static int tgv_launch_iters(psw_tgv* t, int niters, const psw_tgv_params* prm, h
 
 static int tgv_launch_energy(psw_tgv* t, const psw_tgv_params* prm, double tol, double e_star, hipStream_t st) {
     const TgvLayout& L = t->L;
-    const float* U = t->base + L.U + (size_t)(t->k % 4) * 3 * L.n;
+    const float* U = t->base + L.U + (size_t)(t->k % TG_USLOTS) * 3 * L.n;
     int blocks = (int)((L.n + TG_THREADS - 1) / TG_THREADS);
     if (blocks > t->npartials) blocks = t->npartials;
     tgv_energy_kernel<<<blocks, TG_THREADS, 0, st>>>(U, t->base + L.T, t->base + L.u0, t->base + L.It, t->base + L.Iu,
@@ -967,7 +1014,7 @@ __global__ __launch_bounds__(TG_THREADS) void tgv_energy_batch_kernel(const TgvF
     const TgvFrameDev& d = fr[f];
     const int bid = (int)blockIdx.x - eoff[f];
     const size_t n = d.L.n;
-    const float* U = d.base + d.L.U + (size_t)((d.k0 + *kdev) % 4) * 3 * n;
+    const float* U = d.base + d.L.U + (size_t)((d.k0 + *kdev) % TG_USLOTS) * 3 * n;
     const double t = tgv_energy_block(U, d.base + d.L.T, d.base + d.L.u0, d.base + d.L.It, d.base + d.L.Iu,
                                       d.base + d.L.mask, d.W, d.H, d.N, d.dlambda, d.dalpha0, d.dalpha1, bid,
                                       d.eblocks);
@@ -1072,7 +1119,7 @@ int psw_tgv_set_images(psw_tgv* t, const float* d_ref, const float* d_src, doubl
 
 int psw_tgv_set_u(psw_tgv* t, const float* d_u, float value, psw_stream_t s) {
     PSW_CHECK_ARG(t, "NULL solver");
-    float* u = t->base + t->L.U + (size_t)(t->k % 4) * 3 * t->L.n;
+    float* u = t->base + t->L.U + (size_t)(t->k % TG_USLOTS) * 3 * t->L.n;
     if (d_u) {
         PSW_TRY_HIP(hipMemcpyAsync(u, d_u, t->L.n * sizeof(float), hipMemcpyDeviceToDevice, S(s)), PSW_ERR_COPY);
         return PSW_OK;
@@ -1085,7 +1132,7 @@ int psw_tgv_linearise(psw_tgv* t, const float* K, const float* Rrel, const float
     PSW_CHECK_ARG(t->have_images, "psw_tgv_set_images must come first");
     const TgvLayout& L = t->L;
     const size_t n = L.n;
-    const int cur = t->k % 4, prev = (t->k + 3) % 4;
+    const int cur = t->k % TG_USLOTS, prev = (t->k + TG_USLOTS - 1) % TG_USLOTS;
     tgv_warp_reset_kernel<<<1024, 256, 0, S(s)>>>(t->base + L.U, t->base + L.P + (size_t)(t->k % 3) * 2 * n,
                                                   t->base + L.Q + (size_t)(t->k % 3) * 3 * n, t->base + L.u0, n, cur,
                                                   prev);
@@ -1124,7 +1171,7 @@ int psw_tgv_set_primal(psw_tgv* t, const float* d_u, float u_value, const float*
     PSW_CHECK_ARG(t, "NULL solver");
     const TgvLayout& L = t->L;
     const size_t n = L.n;
-    const int cur = t->k % 4, prev = (t->k + 3) % 4;
+    const int cur = t->k % TG_USLOTS, prev = (t->k + TG_USLOTS - 1) % TG_USLOTS;
     tgv_set_primal_kernel<<<1024, 256, 0, S(s)>>>(t->base + L.U, t->base + L.P + (size_t)(t->k % 3) * 2 * n,
                                                   t->base + L.Q + (size_t)(t->k % 3) * 3 * n,
                                                   t->base + L.r + (size_t)(t->k % 3) * t->N * n, d_u,
@@ -1233,40 +1280,44 @@ int psw_tgv_solve(psw_tgv* t, int max_iters, int check_every, double tol, double
     hipEvent_t ev[2] = {nullptr, nullptr};
     PSW_TRY_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming), PSW_ERR_DEVICE);
     PSW_TRY_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming), PSW_ERR_DEVICE);
-    // The energy and finalising launches are one-iteration launches: a stop
-    // at state k is then seen by the launch from k+1, which writes state k+2
-    // only (u cycles through 4 slots, p, q, r through 3), so states k and
-    // k-1 stay intact.  The other launches of a chunk do two iterations each.
+    // A check is an energy launch from state k and the finalising launch
+    // after it; both do two iterations (or, near the end of a chunk, both
+    // one), so a stop at k leaves u_k, u_{k-1}, p_k, q_k, r_k in place (slot
+    // counts at TG_USLOTS).  The other launches do two iterations each.
     const bool dbl = tgv_double() && t->N <= T2_MAXN;
+    const int ntiles2 = tgv_tiles2(t);
     int done_iters = -1, chunk_idx = 0, nchecks = 0;
     int launched = 0;
     bool stopped = false, pending = false;   // pending: an energy sum not yet finalised
+    bool pend_two = false;
     double* pend_buf = nullptr;
-    int pend_k = 0;
+    int pend_k = 0, pend_count = 0;
     while (launched < max_iters && rc == PSW_OK) {
         const int c = std::min(check_every, max_iters - launched);
         for (int i = 0; i < c && rc == PSW_OK;) {
             const bool energy = i == 0 && launched > 0;
             const bool fin = pending;
-            if (!energy && !fin && dbl && c - i >= 2) {   // two plain iterations in one launch
-                rc = tgv_launch_double(t, prm, st);
-                i += 2;
-                continue;
-            }
+            bool two;
+            if (fin) two = pend_two;                        // the same kind as its energy launch
+            else if (energy) two = dbl && c - i >= 4;       // room for a two-iteration finalising launch
+            else two = dbl && c - i >= 2;
             if (fin) {
                 chk.fin_partials = pend_buf;
                 chk.fin_k = pend_k;
+                chk.fin_count = pend_count;
             }
             if (energy) chk.partials = t->partials + (size_t)(nchecks & 1) * half;
             const int k_before = t->k;
-            rc = tgv_launch_one(t, prm, chk, energy, fin, st);
+            rc = two ? tgv_launch_double(t, prm, st, &chk, energy, fin) : tgv_launch_one(t, prm, chk, energy, fin, st);
             if (energy) {
                 pend_buf = chk.partials;
                 pend_k = k_before;
+                pend_count = two ? ntiles2 : ntiles;
+                pend_two = two;
                 nchecks++;
             }
             pending = energy;
-            ++i;
+            i += two ? 2 : 1;
         }
         if (rc != PSW_OK) break;
         launched += c;
@@ -1283,6 +1334,7 @@ int psw_tgv_solve(psw_tgv* t, int max_iters, int check_every, double tol, double
         // the last energy sum has no following launch: finalise it alone
         chk.fin_partials = pend_buf;
         chk.fin_k = pend_k;
+        chk.fin_count = pend_count;
         if (tgv_variant() == 1) tgv_check_finalize_kernel<512><<<1, 512, 0, st>>>(chk);
         else tgv_check_finalize_kernel<256><<<1, 256, 0, st>>>(chk);
         rc = launch_status("tgv_check_finalize_kernel");
@@ -1304,7 +1356,7 @@ int psw_tgv_solve(psw_tgv* t, int max_iters, int check_every, double tol, double
 
 int psw_tgv_get_u(psw_tgv* t, float* d_out, psw_stream_t s) {
     PSW_CHECK_ARG(t && d_out, "NULL pointer");
-    const float* u = t->base + t->L.U + (size_t)(t->k % 4) * 3 * t->L.n;
+    const float* u = t->base + t->L.U + (size_t)(t->k % TG_USLOTS) * 3 * t->L.n;
     PSW_TRY_HIP(hipMemcpyAsync(d_out, u, t->L.n * sizeof(float), hipMemcpyDeviceToDevice, S(s)), PSW_ERR_COPY);
     return PSW_OK;
 }
@@ -1317,7 +1369,7 @@ int psw_tgv_get_state(psw_tgv* t, float* d_out, psw_stream_t s) {
     auto cp = [&](float* dst, const float* src, size_t cnt) {
         return hipMemcpyAsync(dst, src, cnt * sizeof(float), hipMemcpyDeviceToDevice, S(s));
     };
-    const float* U = B + L.U + (size_t)(t->k % 4) * 3 * n;
+    const float* U = B + L.U + (size_t)(t->k % TG_USLOTS) * 3 * n;
     const float* P = B + L.P + (size_t)(t->k % 3) * 2 * n;
     const float* Q = B + L.Q + (size_t)(t->k % 3) * 3 * n;
     float* o = d_out;
