@@ -494,9 +494,9 @@ __device__ __forceinline__ int clampi(int v, int hi) { return min(max(v, 0), hi)
 // One tile pixel per thread (NT == TX * TY): thread t owns pixel (t / TX,
 // t % TX) in both iterations and keeps its r_{k+1}, It, Iu, u0 and T in
 // registers (NS >= a.N sources), so no source plane is read twice.
-template <int NS, bool ENERGY = false>
+template <int NS, bool ENERGY = false, int TY = T2_TY>
 __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile, const TgvCheck* chk = nullptr) {
-    constexpr int TX = TG_TX, TY = T2_TY, NT = T2_THREADS;
+    constexpr int TX = TG_TX, NT = TG_TX * TY;
     static_assert(NT == TX * TY, "one tile pixel per thread");
     constexpr int AX = TX + 4, AY = TY + 4;
     constexpr int BX = TX + 3, BY = TY + 3;
@@ -507,6 +507,7 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile,
     __shared__ float sU1[3][CY][CX];  // u_{k+1} on C
     __shared__ float sP[2][BY][BX];   // p_{k+1} on B; then p_{k+2} on D in place (B index = D index + 1)
     __shared__ float sQ[3][BY][BX];
+    __shared__ float sT[3][BY][BX];   // T11, T12, T22 on B (read again on D and the C ring)
 
     const int W = a.W, H = a.H;
     const size_t n = (size_t)W * H;
@@ -516,6 +517,17 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile,
     const float as1 = a.alpha1 * a.sigma, as0 = a.alpha0 * a.sigma;
     const float sl = a.sigma * a.lambda;
 
+    // T, p_k, q_k on B (p, q are then updated in place)
+    for (int i = tid; i < BX * BY; i += NT) {
+        const int iy = i / BX, ix = i - iy * BX;
+        const size_t g = (size_t)clampi(y0 - 2 + iy, H - 1) * W + clampi(x0 - 2 + ix, W - 1);
+#pragma unroll
+        for (int f = 0; f < 3; ++f) sT[f][iy][ix] = a.T[f * n + g];
+        sP[0][iy][ix] = a.pcur[g];
+        sP[1][iy][ix] = a.pcur[n + g];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) sQ[f][iy][ix] = a.qcur[f * n + g];
+    }
     // A: ubar_k; C: u_k
     for (int i = tid; i < AX * AY; i += NT) {
         const int iy = i / AX, ix = i - iy * AX;
@@ -551,12 +563,12 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile,
     for (int i = tid; i < BX * BY; i += NT) {
         const int iy = i / BX, ix = i - iy * BX;
         const int gx = x0 - 2 + ix, gy = y0 - 2 + iy;
-        const size_t g = (size_t)clampi(gy, H - 1) * W + clampi(gx, W - 1);
         const int rx = gx < W - 1 ? ix + 1 : ix, dy = gy < H - 1 ? iy + 1 : iy;
-        float px = a.pcur[g], py = a.pcur[n + g];
-        float qx = a.qcur[g], qy = a.qcur[n + g], qz = a.qcur[2 * n + g];
-        pq_update(px, py, qx, qy, qz, a.T[g], a.T[n + g], a.T[2 * n + g], sA[0][iy][ix], sA[0][iy][rx], sA[0][dy][ix],
-                  sA[1][iy][ix], sA[1][iy][rx], sA[1][dy][ix], sA[2][iy][ix], sA[2][iy][rx], sA[2][dy][ix], as1, as0);
+        float px = sP[0][iy][ix], py = sP[1][iy][ix];
+        float qx = sQ[0][iy][ix], qy = sQ[1][iy][ix], qz = sQ[2][iy][ix];
+        pq_update(px, py, qx, qy, qz, sT[0][iy][ix], sT[1][iy][ix], sT[2][iy][ix], sA[0][iy][ix], sA[0][iy][rx],
+                  sA[0][dy][ix], sA[1][iy][ix], sA[1][iy][rx], sA[1][dy][ix], sA[2][iy][ix], sA[2][iy][rx],
+                  sA[2][dy][ix], as1, as0);
         sP[0][iy][ix] = px; sP[1][iy][ix] = py;
         sQ[0][iy][ix] = qx; sQ[1][iy][ix] = qy; sQ[2][iy][ix] = qz;
     }
@@ -612,9 +624,9 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile,
         const int bx = ix + 1, by = iy + 1;
         const int lx = gx > 0 ? bx - 1 : bx, uy = gy > 0 ? by - 1 : by;
         float un, u1xn, u1yn;
-        u_update(u, u1x, u1y, a.T[g], a.T[n + g], a.T[2 * n + g], sP[0][by][bx], sP[1][by][bx], sP[0][by][lx],
-                 sP[1][by][lx], sP[0][uy][bx], sP[1][uy][bx], sQ[0][by][bx], sQ[0][by][lx], sQ[1][by][bx],
-                 sQ[1][uy][bx], sQ[2][by][bx], sQ[2][by][lx], sQ[2][uy][bx], ps, a, un, u1xn, u1yn);
+        u_update(u, u1x, u1y, sT[0][by][bx], sT[1][by][bx], sT[2][by][bx], sP[0][by][bx], sP[1][by][bx],
+                 sP[0][by][lx], sP[1][by][lx], sP[0][uy][bx], sP[1][uy][bx], sQ[0][by][bx], sQ[0][by][lx],
+                 sQ[1][by][bx], sQ[1][uy][bx], sQ[2][by][bx], sQ[2][by][lx], sQ[2][uy][bx], ps, a, un, u1xn, u1yn);
         sU1[0][iy][ix] = un; sU1[1][iy][ix] = u1xn; sU1[2][iy][ix] = u1yn;
     }
     __syncthreads();
@@ -636,8 +648,9 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile,
         const int bx = ix + 1, by = iy + 1;               // B index
         float px = sP[0][by][bx], py = sP[1][by][bx];
         float qx = sQ[0][by][bx], qy = sQ[1][by][bx], qz = sQ[2][by][bx];
-        pq_update(px, py, qx, qy, qz, a.T[g], a.T[n + g], a.T[2 * n + g], sA[0][ay][ax], sA[0][ay][rx], sA[0][dy][ax],
-                  sA[1][ay][ax], sA[1][ay][rx], sA[1][dy][ax], sA[2][ay][ax], sA[2][ay][rx], sA[2][dy][ax], as1, as0);
+        pq_update(px, py, qx, qy, qz, sT[0][by][bx], sT[1][by][bx], sT[2][by][bx], sA[0][ay][ax], sA[0][ay][rx],
+                  sA[0][dy][ax], sA[1][ay][ax], sA[1][ay][rx], sA[1][dy][ax], sA[2][ay][ax], sA[2][ay][rx],
+                  sA[2][dy][ax], as1, as0);
         sP[0][by][bx] = px; sP[1][by][bx] = py;
         sQ[0][by][bx] = qx; sQ[1][by][bx] = qy; sQ[2][by][bx] = qz;
         if (ix > 0 && iy > 0 && gx < W && gy < H) {
@@ -677,25 +690,25 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile,
     }
 }
 
-template <int NS>
-__global__ __launch_bounds__(T2_THREADS) void tgv_iter2_kernel(TgvPtrs a) {
+template <int NS, int TY>
+__global__ __launch_bounds__(TG_TX * TY) void tgv_iter2_kernel(TgvPtrs a) {
     if (a.ctrl->done) return;
-    const int ntiles = ((a.W + TG_TX - 1) / TG_TX) * ((a.H + T2_TY - 1) / T2_TY);
-    tgv_iter2_tile<NS>(a, xcd_tile(blockIdx.x, ntiles));
+    const int ntiles = ((a.W + TG_TX - 1) / TG_TX) * ((a.H + TY - 1) / TY);
+    tgv_iter2_tile<NS, false, TY>(a, xcd_tile(blockIdx.x, ntiles));
 }
 
 // Two iterations from state k that also sum the energy of state k (ENERGY)
 // and / or finalise the previous energy launch's sum in one extra
 // workgroup (FINAL).
-template <int NS, bool ENERGY, bool FINAL>
-__global__ __launch_bounds__(T2_THREADS) void tgv_iter2_check_kernel(TgvPtrs a, TgvCheck c) {
+template <int NS, int TY, bool ENERGY, bool FINAL>
+__global__ __launch_bounds__(TG_TX * TY) void tgv_iter2_check_kernel(TgvPtrs a, TgvCheck c) {
     if (a.ctrl->done) return;
-    const int ntiles = ((a.W + TG_TX - 1) / TG_TX) * ((a.H + T2_TY - 1) / T2_TY);
+    const int ntiles = ((a.W + TG_TX - 1) / TG_TX) * ((a.H + TY - 1) / TY);
     if (FINAL && (int)blockIdx.x == ntiles) {
-        check_finalize<T2_THREADS>(c);
+        check_finalize<TG_TX * TY>(c);
         return;
     }
-    tgv_iter2_tile<NS, ENERGY>(a, xcd_tile(blockIdx.x, ntiles), &c);
+    tgv_iter2_tile<NS, ENERGY, TY>(a, xcd_tile(blockIdx.x, ntiles), &c);
 }
 
 template <int TY, int NT, int NS>
@@ -843,16 +856,27 @@ static bool tgv_double() {
     return !(e && atoi(e) == 0);
 }
 
-static int tgv_tiles2(const psw_tgv* t) { return ((t->W + TG_TX - 1) / TG_TX) * ((t->H + T2_TY - 1) / T2_TY); }
+// Tile height of the two-iteration kernel: 8 (512 threads) or 4 (256
+// threads), PSW_TGV2_TY; identical results.
+static int tgv2_ty() {
+    const char* e = getenv("PSW_TGV2_TY");
+    return e && atoi(e) == 4 ? 4 : 8;
+}
 
-template <int NS>
+static int tgv_tiles2(const psw_tgv* t) {
+    const int ty = tgv2_ty();
+    return ((t->W + TG_TX - 1) / TG_TX) * ((t->H + ty - 1) / ty);
+}
+
+template <int NS, int TY>
 static void tgv_launch_double_ns(const TgvPtrs& p, const TgvCheck* c, bool energy, bool fin, int ntiles,
                                  hipStream_t st) {
     const unsigned g = (unsigned)(ntiles + (fin ? 1 : 0));
-    if (energy && fin) tgv_iter2_check_kernel<NS, true, true><<<g, T2_THREADS, 0, st>>>(p, *c);
-    else if (energy) tgv_iter2_check_kernel<NS, true, false><<<g, T2_THREADS, 0, st>>>(p, *c);
-    else if (fin) tgv_iter2_check_kernel<NS, false, true><<<g, T2_THREADS, 0, st>>>(p, *c);
-    else tgv_iter2_kernel<NS><<<g, T2_THREADS, 0, st>>>(p);
+    constexpr int NT = TG_TX * TY;
+    if (energy && fin) tgv_iter2_check_kernel<NS, TY, true, true><<<g, NT, 0, st>>>(p, *c);
+    else if (energy) tgv_iter2_check_kernel<NS, TY, true, false><<<g, NT, 0, st>>>(p, *c);
+    else if (fin) tgv_iter2_check_kernel<NS, TY, false, true><<<g, NT, 0, st>>>(p, *c);
+    else tgv_iter2_kernel<NS, TY><<<g, NT, 0, st>>>(p);
 }
 
 // Two iterations in one launch; with a check (c non-NULL) the launch may
@@ -860,8 +884,14 @@ static void tgv_launch_double_ns(const TgvPtrs& p, const TgvCheck* c, bool energ
 static int tgv_launch_double(psw_tgv* t, const psw_tgv_params* prm, hipStream_t st, const TgvCheck* c = nullptr,
                              bool energy = false, bool fin = false) {
     const TgvPtrs p = tgv_ptrs(t, prm);
-    if (t->N <= 8) tgv_launch_double_ns<8>(p, c, energy, fin, tgv_tiles2(t), st);
-    else tgv_launch_double_ns<T2_MAXN>(p, c, energy, fin, tgv_tiles2(t), st);
+    const int nt2 = tgv_tiles2(t);
+    if (tgv2_ty() == 4) {
+        if (t->N <= 8) tgv_launch_double_ns<8, 4>(p, c, energy, fin, nt2, st);
+        else tgv_launch_double_ns<T2_MAXN, 4>(p, c, energy, fin, nt2, st);
+    } else {
+        if (t->N <= 8) tgv_launch_double_ns<8, 8>(p, c, energy, fin, nt2, st);
+        else tgv_launch_double_ns<T2_MAXN, 8>(p, c, energy, fin, nt2, st);
+    }
     const int s = launch_status(energy || fin ? "tgv_iter2_check_kernel" : "tgv_iter2_kernel");
     if (s == PSW_OK) t->k += 2;
     return s;
@@ -1003,7 +1033,7 @@ __global__ __launch_bounds__(T2_THREADS) void tgv_iter2_batch_kernel(const TgvFr
     if (d.ctrl->done) return;
     const TgvPtrs a = tgv_ptrs_at(d.base, d.L, d.W, d.H, d.N, d.k0 + kb, d.ctrl, d.lambda, d.alpha0, d.alpha1, d.tau,
                                   d.sigma);
-    tgv_iter2_tile<NS>(a, (int)blockIdx.x - toff[f]);
+    tgv_iter2_tile<NS, false, T2_TY>(a, (int)blockIdx.x - toff[f]);
 }
 
 // kdev: the copy of the batch counter that holds the current count.

@@ -187,9 +187,11 @@ def test_tgv_launch_shapes_identical(gpu, views_small, monkeypatch):
     d_ref, d_src = DeviceArray.from_host(v["ref"]), DeviceArray.from_host(v["src"])
     d_init = DeviceArray.from_host(np.clip(v["depth"], 0.5, 6.0).astype(np.float32))
     states = []
-    for double, var in (("0", "0"), ("0", "1"), ("0", "2"), ("1", "1"), ("1", "0")):
+    for double, var, ty2 in (("0", "0", "8"), ("0", "1", "8"), ("0", "2", "8"), ("1", "1", "8"), ("1", "0", "8"),
+                             ("1", "1", "4")):
         monkeypatch.setenv("PSW_TGV_DOUBLE", double)
         monkeypatch.setenv("PSW_TGV_VARIANT", var)
+        monkeypatch.setenv("PSW_TGV2_TY", ty2)
         t = stages.TgvSolver(W, Hh, N)
         t.set_images(d_ref, d_src, prm.beta, prm.gamma)
         t.set_u(d_init)
@@ -364,8 +366,9 @@ def test_tgv_solve_stops_at_gap(gpu, views_small):
         np.testing.assert_array_equal(sa[name], sb[name], err_msg=name)
 
 
-@pytest.mark.parametrize("variant,double", [("0", "1"), ("1", "1"), ("2", "1"), ("1", "0")])
-def test_tgv_solve_max_iters_and_variants(gpu, views_small, monkeypatch, variant, double):
+@pytest.mark.parametrize("variant,double,ty2", [("0", "1", "8"), ("1", "1", "8"), ("2", "1", "8"), ("1", "0", "8"),
+                                               ("1", "1", "4")])
+def test_tgv_solve_max_iters_and_variants(gpu, views_small, monkeypatch, variant, double, ty2):
     """Every launch shape, with and without two-iteration launches, stops at
     the same iteration with the same state; a solve that reaches max_iters
     reports max_iters and the energy of the final state."""
@@ -373,6 +376,7 @@ def test_tgv_solve_max_iters_and_variants(gpu, views_small, monkeypatch, variant
     from psw_amd.device import DeviceArray
     monkeypatch.setenv("PSW_TGV_VARIANT", variant)
     monkeypatch.setenv("PSW_TGV_DOUBLE", double)
+    monkeypatch.setenv("PSW_TGV2_TY", ty2)
     v = views_small
     K1 = one_based(v["K"])
     prm = stages.tgv_params(**TGV_KW)
