@@ -16,9 +16,10 @@
 // into the sorted bin centres with sortedHist::insert (src/inc/structs.h:
 // 212-284), which drops values below the running minimum; since the minimum
 // is always the bin centre -1, the result equals max(-1, median of the 17
-// values).  Here the median is taken in closed form: sort the 9 data values
-// with a sorting network, then select the 9th smallest of the union with the
-// 8 (sorted, constant) bin centres.
+// values).  Here the median is taken in closed form: the 8 values p_i are
+// already ordered, so inserting u sorts the 9 data values, then the 9th
+// smallest of the union with the 8 (sorted, constant) bin centres is
+// selected.
 #include "psw_common.h"
 
 #include <vector>
@@ -145,12 +146,6 @@ __global__ void fusion_hist_batch_kernel(FusionView f, CamBatch cb, const float*
 
 // ---- primal-dual ----------------------------------------------------------
 
-__device__ inline void cswap(float& a, float& b) {
-    const float lo = fminf(a, b), hi = fmaxf(a, b);
-    a = lo;
-    b = hi;
-}
-
 // max(-1, median of {centres} U {u, p_1..p_8}), p_i = u + (tau*lambda)*W_i,
 // W_i = -sum_{j<i} h_j + sum_{j>=i} h_j (fusion.h:813-838).
 __device__ inline float prox_hist(float u, const uint16_t* __restrict__ hp, float tl) {
@@ -170,16 +165,22 @@ __device__ inline float prox_hist(float u, const uint16_t* __restrict__ hp, floa
         const int Wi = total - 2 * below;
         a[i] = u + tl * (float)Wi;
     }
-    // Sorting network for 9 values (25 compare-exchanges).
-    cswap(a[0], a[1]); cswap(a[3], a[4]); cswap(a[6], a[7]);
-    cswap(a[1], a[2]); cswap(a[4], a[5]); cswap(a[7], a[8]);
-    cswap(a[0], a[1]); cswap(a[3], a[4]); cswap(a[6], a[7]);
-    cswap(a[0], a[3]); cswap(a[3], a[6]); cswap(a[0], a[3]);
-    cswap(a[1], a[4]); cswap(a[4], a[7]); cswap(a[1], a[4]);
-    cswap(a[2], a[5]); cswap(a[5], a[8]); cswap(a[2], a[5]);
-    cswap(a[1], a[3]); cswap(a[5], a[7]); cswap(a[2], a[6]);
-    cswap(a[4], a[6]); cswap(a[2], a[4]); cswap(a[2], a[3]);
-    cswap(a[5], a[6]);
+    // Sort the 9 values.  W_i falls with i, and u + tl*W is monotone in W in
+    // float, so p_8 <= ... <= p_1 already: q = (p_8, ..., p_1) is sorted and
+    // only u has to be inserted, s[j] = max(q[j-1], min(q[j], u)).
+    {
+        const float q0 = a[8], q1 = a[7], q2 = a[6], q3 = a[5], q4 = a[4], q5 = a[3], q6 = a[2], q7 = a[1];
+        const float uu = a[0];
+        a[0] = fminf(q0, uu);
+        a[1] = fmaxf(q0, fminf(q1, uu));
+        a[2] = fmaxf(q1, fminf(q2, uu));
+        a[3] = fmaxf(q2, fminf(q3, uu));
+        a[4] = fmaxf(q3, fminf(q4, uu));
+        a[5] = fmaxf(q4, fminf(q5, uu));
+        a[6] = fmaxf(q5, fminf(q6, uu));
+        a[7] = fmaxf(q6, fminf(q7, uu));
+        a[8] = fmaxf(q7, uu);
+    }
     // 9th smallest of a (sorted, 9) and the centres c (sorted, 8):
     // min over splits i = 1..9 of max(a[i-1], c[8-i]).  Centres of 8 bins
     // (binParams, fusion.h:1041-1054): -1, -1, -0.6, -0.2, 0.2, 0.6, 1, 1.
@@ -242,10 +243,16 @@ __global__ void fusion_p_kernel(FusionView f, float sigma) {
     if (y < f.h - 1) gy = f.v[i + f.w] - vv;
     if (z < f.d - 1) gz = f.v[i + (size_t)f.w * f.h] - vv;
     float ax = f.p[i] + sigma * gx, ay = f.p[V + i] + sigma * gy, az = f.p[2 * V + i] + sigma * gz;
-    const float dn = fmaxf(1.f, sqrtf(ax * ax + ay * ay + az * az));
-    f.p[i] = ax / dn;
-    f.p[V + i] = ay / dn;
-    f.p[2 * V + i] = az / dn;
+    const float s2 = ax * ax + ay * ay + az * az;
+    if (s2 > 1.f) {   // x / fmaxf(1, sqrt(s)) is x itself for s <= 1
+        const float dn = sqrtf(s2);
+        ax = ax / dn;
+        ay = ay / dn;
+        az = az / dn;
+    }
+    f.p[i] = ax;
+    f.p[V + i] = ay;
+    f.p[2 * V + i] = az;
 }
 
 // Fused iteration: P step of iteration k (from v_k) followed by the U step of
@@ -263,10 +270,17 @@ __device__ inline void p_new_at(const PdArgs& a, size_t V, int x, int y, int z, 
     if (y < f.h - 1) gy = a.vin[i + f.w] - vv;
     if (z < f.d - 1) gz = a.vin[i + (size_t)f.w * f.h] - vv;
     const float ax = a.pin[i] + a.sigma * gx, ay = a.pin[V + i] + a.sigma * gy, az = a.pin[2 * V + i] + a.sigma * gz;
-    const float dn = fmaxf(1.f, sqrtf(ax * ax + ay * ay + az * az));
-    ox = ax / dn;
-    oy = ay / dn;
-    oz = az / dn;
+    // x / fmaxf(1, sqrt(s)) is x itself for s <= 1: no root or division there
+    const float s2 = ax * ax + ay * ay + az * az;
+    ox = ax;
+    oy = ay;
+    oz = az;
+    if (s2 > 1.f) {
+        const float dn = sqrtf(s2);
+        ox = ax / dn;
+        oy = ay / dn;
+        oz = az / dn;
+    }
 }
 
 __global__ void fusion_pd_fused_kernel(PdArgs a) {
@@ -301,11 +315,21 @@ __global__ void fusion_pd_fused_kernel(PdArgs a) {
 // fusion_pd_fused_kernel, so both kernels give identical results.
 constexpr int FZ_CHUNK = 16;
 
-__global__ __launch_bounds__(256) void fusion_pd_zmarch_kernel(PdArgs a) {
+// Blocks of 64x4 columns, one-dimensional grid: block b takes tile
+// xcd_tile(b) in x, y, z-chunk order, so that each XCD works through a
+// contiguous range of tiles and the y-halo rows a block reads were mostly
+// just fetched into the same L2 by its neighbour.
+constexpr int FZ_BX = 64, FZ_BY = 4;
+
+__global__ __launch_bounds__(FZ_BX * FZ_BY) void fusion_pd_zmarch_kernel(PdArgs a) {
     const FusionView& f = a.f;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int z0 = blockIdx.z * FZ_CHUNK;
+    const int tx_n = (f.w + FZ_BX - 1) / FZ_BX, ty_n = (f.h + FZ_BY - 1) / FZ_BY;
+    const int ntiles = tx_n * ty_n * ((f.d + FZ_CHUNK - 1) / FZ_CHUNK);
+    const int t = xcd_tile(blockIdx.x, ntiles);
+    const int bx = t % tx_n, by = (t / tx_n) % ty_n, bz = t / (tx_n * ty_n);
+    const int x = bx * FZ_BX + (int)(threadIdx.x % FZ_BX);
+    const int y = by * FZ_BY + (int)(threadIdx.x / FZ_BX);
+    const int z0 = bz * FZ_CHUNK;
     if (x >= f.w || y >= f.h || z0 >= f.d) return;
     const int z1 = min(z0 + FZ_CHUNK, f.d);
     const size_t V = f.V, sxy = (size_t)f.w * f.h;
@@ -669,6 +693,8 @@ int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double la
         const dim3 grd = zmarch ? dim3((unsigned)((f.w + blk.x - 1) / blk.x), (unsigned)((f.h + blk.y - 1) / blk.y),
                                        (unsigned)((f.d + FZ_CHUNK - 1) / FZ_CHUNK))
                                 : vox_grid(f, blk);
+        const unsigned zm_blocks = (unsigned)(((f.w + FZ_BX - 1) / FZ_BX) * ((f.h + FZ_BY - 1) / FZ_BY) *
+                                              ((f.d + FZ_CHUNK - 1) / FZ_CHUNK));
         int cur = 0;
         for (int k = 1; k < niters && st == PSW_OK; ++k) {
             a.pin = pb[cur]; a.pout = pb[cur ^ 1];
@@ -678,7 +704,7 @@ int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double la
                 fusion_pd_zlds_kernel<<<grd, blk, 0, S(s)>>>(a);
                 st = launch_status("fusion_pd_zlds_kernel");
             } else if (zmarch) {
-                fusion_pd_zmarch_kernel<<<grd, blk, 0, S(s)>>>(a);
+                fusion_pd_zmarch_kernel<<<zm_blocks, FZ_BX * FZ_BY, 0, S(s)>>>(a);
                 st = launch_status("fusion_pd_zmarch_kernel");
             } else {
                 fusion_pd_fused_kernel<<<grd, blk, 0, S(s)>>>(a);

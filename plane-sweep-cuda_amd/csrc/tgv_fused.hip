@@ -260,7 +260,8 @@ __global__ void tgv_mask_kernel(const float* __restrict__ Iu, int W, int H, int 
 // TGV2_updateR at one pixel and source (uses u_k, not ubar).
 __device__ __forceinline__ float r_update(float r, float it, float du, float iu, float sl) {
     const float rr = r + sl * (it + du * iu);
-    return rr / fmaxf(1.f, fabsf(rr));
+    // rr / fmaxf(1, |rr|): rr itself for |rr| <= 1, exactly +-1 above
+    return fabsf(rr) > 1.f ? copysignf(1.f, rr) : rr;
 }
 
 // TGV2_updateU_tensor_weighed at one pixel: c = own duals, l = left

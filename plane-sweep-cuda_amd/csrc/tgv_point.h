@@ -14,9 +14,14 @@ __device__ __forceinline__ void pq_update(float& px, float& py, float& qx, float
     const float gyv = ubd - ubc - u1yc;
     px = px + as1 * (t11 * gxv + t12 * gyv);
     py = py + as1 * (t12 * gxv + t22 * gyv);
-    const float dp = fmaxf(1.f, sqrtf(px * px + py * py));
-    px = px / dp;
-    py = py / dp;
+    // x / fmaxf(1, sqrt(s)) is x itself for s <= 1 (and for NaN s), so the
+    // root and the divisions are only needed outside the unit ball.
+    const float sp = px * px + py * py;
+    if (sp > 1.f) {
+        const float dp = sqrtf(sp);
+        px = px / dp;
+        py = py / dp;
+    }
     const float dx_u1x = u1xr - u1xc;
     const float dy_u1x = u1xd - u1xc;
     const float dx_u1y = u1yr - u1yc;
@@ -24,10 +29,13 @@ __device__ __forceinline__ void pq_update(float& px, float& py, float& qx, float
     qx = qx + as0 * dx_u1x;
     qy = qy + as0 * dy_u1y;
     qz = qz + as0 * (dy_u1x + dx_u1y) / 2.0f;
-    const float dq = fmaxf(1.f, sqrtf(qx * qx + qy * qy + qz * qz + qz * qz));
-    qx = qx / dq;
-    qy = qy / dq;
-    qz = qz / dq;
+    const float sq = qx * qx + qy * qy + qz * qz + qz * qz;
+    if (sq > 1.f) {
+        const float dq = sqrtf(sq);
+        qx = qx / dq;
+        qy = qy / dq;
+        qz = qz / dq;
+    }
 }
 
 // Anisotropic_diffusion_tensor_kernel (TGV2_kernels.cu:257-298) at one
