@@ -503,9 +503,9 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile,
     constexpr int BX = TX + 3, BY = TY + 3;
     constexpr int CX = TX + 2, CY = TY + 2;
     constexpr int DX = TX + 1, DY = TY + 1;
-    __shared__ float sA[3][AY][AX];   // ubar_k on A; then ubar_{k+1} on C (A index = C index + 1)
+    __shared__ float sA[3][AY][AX];   // ubar_k on A; then ubar_{k+1} on C (A index = C index + 1), formed
+                                      // by the thread that computes u_{k+1} at the position
     __shared__ float sUk[3][CY][CX];  // u_k on C
-    __shared__ float sU1[3][CY][CX];  // u_{k+1} on C
     __shared__ float sP[2][BY][BX];   // p_{k+1} on B; then p_{k+2} on D in place (B index = D index + 1)
     __shared__ float sQ[3][BY][BX];
     __shared__ float sT[3][BY][BX];   // T11, T12, T22 on B (read again on D and the C ring)
@@ -575,10 +575,13 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile,
     }
     __syncthreads();
 
-    // C: r and u of iteration k+1; the tile pixel from registers (its r_{k+1}
-    // stays there), then the ring.  An energy launch also sums the energy of
-    // state k at the tile pixel (u_k with its right / lower neighbours in sUk).
+    // C: r and u of iteration k+1 and ubar_{k+1} = 2 u_{k+1} - u_k into sA
+    // (ubar_k is no longer read); the tile pixel from registers (its r_{k+1}
+    // and u_{k+1} stay there), then the ring.  An energy launch also sums the
+    // energy of state k at the tile pixel (u_k with its right / lower
+    // neighbours in sUk).
     double eacc = 0.0;
+    float uk1, u1xk1, u1yk1;   // u_{k+1} of the tile pixel
     {
         const int ix = tx + 1, iy = ty + 1;
         const float u = sUk[0][iy][ix], u1x = sUk[1][iy][ix], u1y = sUk[2][iy][ix];
@@ -604,7 +607,10 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile,
         u_update(u, u1x, u1y, t11, t12, t22, sP[0][by][bx], sP[1][by][bx], sP[0][by][lx], sP[1][by][lx],
                  sP[0][uy][bx], sP[1][uy][bx], sQ[0][by][bx], sQ[0][by][lx], sQ[1][by][bx], sQ[1][uy][bx],
                  sQ[2][by][bx], sQ[2][by][lx], sQ[2][uy][bx], ps, a, un, u1xn, u1yn);
-        sU1[0][iy][ix] = un; sU1[1][iy][ix] = u1xn; sU1[2][iy][ix] = u1yn;
+        sA[0][iy + 1][ix + 1] = 2.f * un - u;
+        sA[1][iy + 1][ix + 1] = 2.f * u1xn - u1x;
+        sA[2][iy + 1][ix + 1] = 2.f * u1yn - u1y;
+        uk1 = un; u1xk1 = u1xn; u1yk1 = u1yn;
     }
     for (int k = tid; k < CX * CY - TX * TY; k += NT) {  // ring: top row, bottom row, left column, right column
         int ix, iy;
@@ -628,14 +634,9 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile,
         u_update(u, u1x, u1y, sT[0][by][bx], sT[1][by][bx], sT[2][by][bx], sP[0][by][bx], sP[1][by][bx],
                  sP[0][by][lx], sP[1][by][lx], sP[0][uy][bx], sP[1][uy][bx], sQ[0][by][bx], sQ[0][by][lx],
                  sQ[1][by][bx], sQ[1][uy][bx], sQ[2][by][bx], sQ[2][by][lx], sQ[2][uy][bx], ps, a, un, u1xn, u1yn);
-        sU1[0][iy][ix] = un; sU1[1][iy][ix] = u1xn; sU1[2][iy][ix] = u1yn;
-    }
-    __syncthreads();
-    // ubar_{k+1} on C, into sA (ubar_k is no longer read)
-    for (int i = tid; i < CX * CY; i += NT) {
-        const int iy = i / CX, ix = i - iy * CX;
-#pragma unroll
-        for (int f = 0; f < 3; ++f) sA[f][iy + 1][ix + 1] = 2.f * sU1[f][iy][ix] - sUk[f][iy][ix];
+        sA[0][iy + 1][ix + 1] = 2.f * un - u;
+        sA[1][iy + 1][ix + 1] = 2.f * u1xn - u1x;
+        sA[2][iy + 1][ix + 1] = 2.f * u1yn - u1y;
     }
     __syncthreads();
 
@@ -664,8 +665,7 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile,
     // tile: r and u of iteration k+2
     if (x < W && y < H) {
         const size_t g = (size_t)y * W + x;
-        const int cx = tx + 1, cy = ty + 1;               // C index
-        const float u = sU1[0][cy][cx], u1x = sU1[1][cy][cx], u1y = sU1[2][cy][cx];
+        const float u = uk1, u1x = u1xk1, u1y = u1yk1;
         const float du = u - u0t;
         float ps = 0.f;
 #pragma unroll
