@@ -492,6 +492,16 @@ constexpr int T2_TY = 8, T2_THREADS = 512, T2_MAXN = 16;
 
 __device__ __forceinline__ int clampi(int v, int hi) { return min(max(v, 0), hi); }
 
+// k-th position of the one-pixel ring around a TX x TY tile inside the
+// CX x CY region C: top row, bottom row, left column, right column.
+template <int CX, int CY, int TY>
+__device__ __forceinline__ void ring_pos(int k, int& ix, int& iy) {
+    if (k < CX) { iy = 0; ix = k; }
+    else if (k < 2 * CX) { iy = CY - 1; ix = k - CX; }
+    else if (k < 2 * CX + TY) { iy = k - 2 * CX + 1; ix = 0; }
+    else { iy = k - 2 * CX - TY + 1; ix = CX - 1; }
+}
+
 // One tile pixel per thread (NT == TX * TY): thread t owns pixel (t / TX,
 // t % TX) in both iterations and keeps its r_{k+1}, It, Iu, u0 and T in
 // registers (NS >= a.N sources), so no source plane is read twice.
@@ -612,15 +622,12 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile,
         sA[2][iy + 1][ix + 1] = 2.f * u1yn - u1y;
         uk1 = un; u1xk1 = u1xn; u1yk1 = u1yn;
     }
-    for (int k = tid; k < CX * CY - TX * TY; k += NT) {  // ring: top row, bottom row, left column, right column
+    for (int k = tid; k < CX * CY - TX * TY; k += NT) {
         int ix, iy;
-        if (k < CX) { iy = 0; ix = k; }
-        else if (k < 2 * CX) { iy = CY - 1; ix = k - CX; }
-        else if (k < 2 * CX + TY) { iy = k - 2 * CX + 1; ix = 0; }
-        else { iy = k - 2 * CX - TY + 1; ix = CX - 1; }
+        ring_pos<CX, CY, TY>(k, ix, iy);
         const int gx = x0 - 1 + ix, gy = y0 - 1 + iy;
-        const size_t g = (size_t)clampi(gy, H - 1) * W + clampi(gx, W - 1);
         const float u = sUk[0][iy][ix], u1x = sUk[1][iy][ix], u1y = sUk[2][iy][ix];
+        const size_t g = (size_t)clampi(gy, H - 1) * W + clampi(gx, W - 1);
         const float du = u - a.u0[g];
         float ps = 0.f;
         for (int j = 0; j < a.N; ++j) {
@@ -857,11 +864,12 @@ static bool tgv_double() {
     return !(e && atoi(e) == 0);
 }
 
-// Tile height of the two-iteration kernel: 8 (512 threads) or 4 (256
-// threads), PSW_TGV2_TY; identical results.
+// Tile height of the two-iteration kernel (identical results): 10 (640
+// threads, the default), 12 (768), 8 (512) or 4 (256), PSW_TGV2_TY.
 static int tgv2_ty() {
     const char* e = getenv("PSW_TGV2_TY");
-    return e && atoi(e) == 4 ? 4 : 8;
+    const int v = e ? atoi(e) : 10;
+    return v == 4 || v == 8 || v == 12 ? v : 10;
 }
 
 static int tgv_tiles2(const psw_tgv* t) {
@@ -880,19 +888,24 @@ static void tgv_launch_double_ns(const TgvPtrs& p, const TgvCheck* c, bool energ
     else tgv_iter2_kernel<NS, TY><<<g, NT, 0, st>>>(p);
 }
 
+template <int TY>
+static void tgv_launch_double_ty(const TgvPtrs& p, int N, const TgvCheck* c, bool energy, bool fin, int ntiles,
+                                 hipStream_t st) {
+    if (N <= 8) tgv_launch_double_ns<8, TY>(p, c, energy, fin, ntiles, st);
+    else tgv_launch_double_ns<T2_MAXN, TY>(p, c, energy, fin, ntiles, st);
+}
+
 // Two iterations in one launch; with a check (c non-NULL) the launch may
 // also sum the energy of the current state and / or finalise a previous sum.
 static int tgv_launch_double(psw_tgv* t, const psw_tgv_params* prm, hipStream_t st, const TgvCheck* c = nullptr,
                              bool energy = false, bool fin = false) {
     const TgvPtrs p = tgv_ptrs(t, prm);
     const int nt2 = tgv_tiles2(t);
-    if (tgv2_ty() == 4) {
-        if (t->N <= 8) tgv_launch_double_ns<8, 4>(p, c, energy, fin, nt2, st);
-        else tgv_launch_double_ns<T2_MAXN, 4>(p, c, energy, fin, nt2, st);
-    } else {
-        if (t->N <= 8) tgv_launch_double_ns<8, 8>(p, c, energy, fin, nt2, st);
-        else tgv_launch_double_ns<T2_MAXN, 8>(p, c, energy, fin, nt2, st);
-    }
+    const int ty = tgv2_ty();
+    if (ty == 4) tgv_launch_double_ty<4>(p, t->N, c, energy, fin, nt2, st);
+    else if (ty == 8) tgv_launch_double_ty<8>(p, t->N, c, energy, fin, nt2, st);
+    else if (ty == 12) tgv_launch_double_ty<12>(p, t->N, c, energy, fin, nt2, st);
+    else tgv_launch_double_ty<10>(p, t->N, c, energy, fin, nt2, st);
     const int s = launch_status(energy || fin ? "tgv_iter2_check_kernel" : "tgv_iter2_kernel");
     if (s == PSW_OK) t->k += 2;
     return s;

@@ -187,8 +187,8 @@ def test_tgv_launch_shapes_identical(gpu, views_small, monkeypatch):
     d_ref, d_src = DeviceArray.from_host(v["ref"]), DeviceArray.from_host(v["src"])
     d_init = DeviceArray.from_host(np.clip(v["depth"], 0.5, 6.0).astype(np.float32))
     states = []
-    for double, var, ty2 in (("0", "0", "8"), ("0", "1", "8"), ("0", "2", "8"), ("1", "1", "8"), ("1", "0", "8"),
-                             ("1", "1", "4")):
+    for double, var, ty2 in (("0", "0", "10"), ("0", "1", "10"), ("0", "2", "10"), ("1", "1", "10"),
+                             ("1", "0", "10"), ("1", "1", "4"), ("1", "1", "8"), ("1", "1", "12")):
         monkeypatch.setenv("PSW_TGV_DOUBLE", double)
         monkeypatch.setenv("PSW_TGV_VARIANT", var)
         monkeypatch.setenv("PSW_TGV2_TY", ty2)
@@ -366,8 +366,8 @@ def test_tgv_solve_stops_at_gap(gpu, views_small):
         np.testing.assert_array_equal(sa[name], sb[name], err_msg=name)
 
 
-@pytest.mark.parametrize("variant,double,ty2", [("0", "1", "8"), ("1", "1", "8"), ("2", "1", "8"), ("1", "0", "8"),
-                                               ("1", "1", "4")])
+@pytest.mark.parametrize("variant,double,ty2", [("0", "1", "10"), ("1", "1", "10"), ("2", "1", "10"), ("1", "0", "10"),
+                                               ("1", "1", "4"), ("1", "1", "8"), ("1", "1", "12")])
 def test_tgv_solve_max_iters_and_variants(gpu, views_small, monkeypatch, variant, double, ty2):
     """Every launch shape, with and without two-iteration launches, stops at
     the same iteration with the same state; a solve that reaches max_iters
