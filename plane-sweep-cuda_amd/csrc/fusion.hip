@@ -148,8 +148,13 @@ __global__ void fusion_hist_batch_kernel(FusionView f, CamBatch cb, const float*
 
 // max(-1, median of {centres} U {u, p_1..p_8}), p_i = u + (tau*lambda)*W_i,
 // W_i = -sum_{j<i} h_j + sum_{j>=i} h_j (fusion.h:813-838).
+__device__ inline float prox_hist_h(float u, uint4 hv, float tl);
+
 __device__ inline float prox_hist(float u, const uint16_t* __restrict__ hp, float tl) {
-    const uint4 hv = *reinterpret_cast<const uint4*>(hp);
+    return prox_hist_h(u, *reinterpret_cast<const uint4*>(hp), tl);
+}
+
+__device__ inline float prox_hist_h(float u, uint4 hv, float tl) {
     int h[FB];
     h[0] = hv.x & 0xffff; h[1] = hv.x >> 16; h[2] = hv.y & 0xffff; h[3] = hv.y >> 16;
     h[4] = hv.z & 0xffff; h[5] = hv.z >> 16; h[6] = hv.w & 0xffff; h[7] = hv.w >> 16;
@@ -261,16 +266,12 @@ __global__ void fusion_p_kernel(FusionView f, float sigma) {
 // arithmetic, same result as the owner thread).  p and v are ping-ponged
 // because neighbouring voxels read their old values; u is read and written
 // only at the own voxel.
-__device__ inline void p_new_at(const PdArgs& a, size_t V, int x, int y, int z, size_t i, float& ox, float& oy,
-                                float& oz) {
-    const FusionView& f = a.f;
-    const float vv = a.vin[i];
-    float gx = 0.f, gy = 0.f, gz = 0.f;
-    if (x < f.w - 1) gx = a.vin[i + 1] - vv;
-    if (y < f.h - 1) gy = a.vin[i + f.w] - vv;
-    if (z < f.d - 1) gz = a.vin[i + (size_t)f.w * f.h] - vv;
-    const float ax = a.pin[i] + a.sigma * gx, ay = a.pin[V + i] + a.sigma * gy, az = a.pin[2 * V + i] + a.sigma * gz;
-    // x / fmaxf(1, sqrt(s)) is x itself for s <= 1: no root or division there
+// p + sigma * g projected onto the unit ball (projectUnitBall, fusion.h:
+// 866-870); x / fmaxf(1, sqrt(s)) is x itself for s <= 1, so there is no
+// root or division inside the ball.
+__device__ __forceinline__ void p_step(float p0, float p1, float p2, float gx, float gy, float gz, float sigma,
+                                       float& ox, float& oy, float& oz) {
+    const float ax = p0 + sigma * gx, ay = p1 + sigma * gy, az = p2 + sigma * gz;
     const float s2 = ax * ax + ay * ay + az * az;
     ox = ax;
     oy = ay;
@@ -281,6 +282,17 @@ __device__ inline void p_new_at(const PdArgs& a, size_t V, int x, int y, int z, 
         oy = ay / dn;
         oz = az / dn;
     }
+}
+
+__device__ inline void p_new_at(const PdArgs& a, size_t V, int x, int y, int z, size_t i, float& ox, float& oy,
+                                float& oz) {
+    const FusionView& f = a.f;
+    const float vv = a.vin[i];
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+    if (x < f.w - 1) gx = a.vin[i + 1] - vv;
+    if (y < f.h - 1) gy = a.vin[i + f.w] - vv;
+    if (z < f.d - 1) gz = a.vin[i + (size_t)f.w * f.h] - vv;
+    p_step(a.pin[i], a.pin[V + i], a.pin[2 * V + i], gx, gy, gz, a.sigma, ox, oy, oz);
 }
 
 __global__ void fusion_pd_fused_kernel(PdArgs a) {
@@ -356,6 +368,95 @@ __global__ __launch_bounds__(FZ_BX * FZ_BY) void fusion_pd_zmarch_kernel(PdArgs 
         const float unew = prox_hist(uu, f.hist + i * FB, a.tl);
         a.uout[i] = unew;
         a.vout[i] = 2.f * unew - un;
+    }
+}
+
+// The z-march with its loads software-pipelined: everything plane z+1 needs
+// (v at the voxel and at x+-1, y+-1 neighbours, p of the voxel and of its x-1
+// and y-1 neighbours, the histogram, u) is loaded before plane z's results
+// are stored, and the z+1 values of v that plane z's gradients need come
+// from that bundle.  (The compiler cannot hoist those loads itself: u is read
+// and written in place.)  Same arithmetic and order as fusion_pd_fused_kernel.
+struct ZBundle {
+    float v0, vxp, vyp, vxm, vxmyp, vym, vymxp;
+    float p0, p1, p2, pxm0, pxm1, pxm2, pym0, pym1, pym2;
+    uint4 h;
+    float u;
+};
+
+__device__ __forceinline__ void zbundle_load(const PdArgs& a, size_t V, int x, int y, size_t i, ZBundle& b) {
+    const FusionView& f = a.f;
+    const bool xp = x < f.w - 1, yp = y < f.h - 1, xm = x > 0, ym = y > 0;
+    b.v0 = a.vin[i];
+    b.vxp = xp ? a.vin[i + 1] : 0.f;
+    b.vyp = yp ? a.vin[i + f.w] : 0.f;
+    b.vxm = xm ? a.vin[i - 1] : 0.f;
+    b.vxmyp = xm && yp ? a.vin[i - 1 + f.w] : 0.f;
+    b.vym = ym ? a.vin[i - f.w] : 0.f;
+    b.vymxp = ym && xp ? a.vin[i - f.w + 1] : 0.f;
+    b.p0 = a.pin[i];
+    b.p1 = a.pin[V + i];
+    b.p2 = a.pin[2 * V + i];
+    b.pxm0 = xm ? a.pin[i - 1] : 0.f;
+    b.pxm1 = xm ? a.pin[V + i - 1] : 0.f;
+    b.pxm2 = xm ? a.pin[2 * V + i - 1] : 0.f;
+    b.pym0 = ym ? a.pin[i - f.w] : 0.f;
+    b.pym1 = ym ? a.pin[V + i - f.w] : 0.f;
+    b.pym2 = ym ? a.pin[2 * V + i - f.w] : 0.f;
+    b.h = *reinterpret_cast<const uint4*>(f.hist + i * FB);
+    b.u = a.uin[i];
+}
+
+__global__ __launch_bounds__(FZ_BX * FZ_BY) void fusion_pd_zpipe_kernel(PdArgs a) {
+    const FusionView& f = a.f;
+    const int tx_n = (f.w + FZ_BX - 1) / FZ_BX, ty_n = (f.h + FZ_BY - 1) / FZ_BY;
+    const int ntiles = tx_n * ty_n * ((f.d + FZ_CHUNK - 1) / FZ_CHUNK);
+    const int t = xcd_tile(blockIdx.x, ntiles);
+    const int bx = t % tx_n, by = (t / tx_n) % ty_n, bz = t / (tx_n * ty_n);
+    const int x = bx * FZ_BX + (int)(threadIdx.x % FZ_BX);
+    const int y = by * FZ_BY + (int)(threadIdx.x / FZ_BX);
+    const int z0 = bz * FZ_CHUNK;
+    if (x >= f.w || y >= f.h || z0 >= f.d) return;
+    const int z1 = min(z0 + FZ_CHUNK, f.d);
+    const size_t V = f.V, sxy = (size_t)f.w * f.h;
+    const bool xp = x < f.w - 1, yp = y < f.h - 1, xm = x > 0, ym = y > 0;
+    size_t i = ((size_t)z0 * f.h + y) * f.w + x;
+    float pz_below = 0.f;
+    if (z0 > 0) {
+        float tx, ty;
+        p_new_at(a, V, x, y, z0 - 1, i - sxy, tx, ty, pz_below);
+    }
+    ZBundle cur, nxt;
+    zbundle_load(a, V, x, y, i, cur);
+    for (int z = z0; z < z1; ++z, i += sxy) {
+        const bool zp = z < f.d - 1;
+        if (zp) zbundle_load(a, V, x, y, i + sxy, nxt);   // the next plane (also when it is the next chunk's)
+        float cx, cy, cz;
+        p_step(cur.p0, cur.p1, cur.p2, xp ? cur.vxp - cur.v0 : 0.f, yp ? cur.vyp - cur.v0 : 0.f,
+               zp ? nxt.v0 - cur.v0 : 0.f, a.sigma, cx, cy, cz);
+        a.pout[i] = cx;
+        a.pout[V + i] = cy;
+        a.pout[2 * V + i] = cz;
+        float div = cx + cy + cz;
+        float tx, ty, tz;
+        if (xm) {
+            p_step(cur.pxm0, cur.pxm1, cur.pxm2, cur.v0 - cur.vxm, yp ? cur.vxmyp - cur.vxm : 0.f,
+                   zp ? nxt.vxm - cur.vxm : 0.f, a.sigma, tx, ty, tz);
+            div -= tx;
+        }
+        if (ym) {
+            p_step(cur.pym0, cur.pym1, cur.pym2, xp ? cur.vymxp - cur.vym : 0.f, cur.v0 - cur.vym,
+                   zp ? nxt.vym - cur.vym : 0.f, a.sigma, tx, ty, tz);
+            div -= ty;
+        }
+        if (z > 0) div -= pz_below;
+        pz_below = cz;
+        const float un = cur.u;
+        const float uu = un - a.tau * (-div);
+        const float unew = prox_hist_h(uu, cur.h, a.tl);
+        a.uout[i] = unew;
+        a.vout[i] = 2.f * unew - un;
+        if (zp) cur = nxt;
     }
 }
 
@@ -684,10 +785,11 @@ int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double la
         float* pb[2] = {g->p, p2};
         float* vb[2] = {g->v, v2};
         // PSW_FUSION_PD selects the kernel (identical results): 0 one voxel per
-        // thread, 1 z-marching, 2 z-marching with LDS-shared neighbour duals.
+        // thread, 1 z-marching, 2 z-marching with LDS-shared neighbour duals,
+        // 3 z-marching with software-pipelined loads (the default).
         const char* env = getenv("PSW_FUSION_PD");
-        int mode = env ? atoi(env) : 1;
-        if (mode < 0 || mode > 2) mode = 1;
+        int mode = env ? atoi(env) : 3;
+        if (mode < 0 || mode > 3) mode = 3;
         const bool zmarch = mode >= 1;
         const dim3 blk = mode == 2 ? dim3(FZL_TX, FZL_TY, 1) : dim3(64, 4, 1);
         const dim3 grd = zmarch ? dim3((unsigned)((f.w + blk.x - 1) / blk.x), (unsigned)((f.h + blk.y - 1) / blk.y),
@@ -703,6 +805,9 @@ int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double la
             if (mode == 2) {
                 fusion_pd_zlds_kernel<<<grd, blk, 0, S(s)>>>(a);
                 st = launch_status("fusion_pd_zlds_kernel");
+            } else if (mode == 3) {
+                fusion_pd_zpipe_kernel<<<zm_blocks, FZ_BX * FZ_BY, 0, S(s)>>>(a);
+                st = launch_status("fusion_pd_zpipe_kernel");
             } else if (zmarch) {
                 fusion_pd_zmarch_kernel<<<zm_blocks, FZ_BX * FZ_BY, 0, S(s)>>>(a);
                 st = launch_status("fusion_pd_zmarch_kernel");

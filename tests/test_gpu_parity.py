@@ -492,15 +492,15 @@ def test_fusion_pd_vs_oracle(gpu):
 
 
 def test_fusion_solve_kernels_identical(gpu, monkeypatch):
-    """The z-marching PD kernels (registers only, and with LDS-shared
-    neighbour duals) and the one-voxel-per-thread kernel give bit-identical
+    """The z-marching PD kernels (registers only, with LDS-shared neighbour
+    duals, with pipelined loads) and the one-voxel-per-thread kernel give bit-identical
     grids (depth 28 leaves a partial z chunk; 20 rows a partial 8-row block)."""
     from psw_amd import stages
     from psw_amd.device import DeviceArray
     o, vol, (gw, gh, gd) = _fusion_inputs(n=3)
     H, W = o["depths"].shape[1:]
     out = []
-    for mode in ("1", "0", "2"):
+    for mode in ("1", "0", "2", "3"):
         monkeypatch.setenv("PSW_FUSION_PD", mode)
         f = stages.Fusion(gw, gh, gd, vol)
         for k in range(o["depths"].shape[0]):
