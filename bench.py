@@ -442,13 +442,15 @@ def main():
 
     # Per-launch duration of the fused TGV2 iteration kernel (same stream and
     # buffers): events around back-to-back launches.  By default a launch
-    # does two iterations (tgv_iter2_kernel); PSW_TGV_DOUBLE=0 selects the
-    # one-iteration kernel.
+    # does two iterations (tgv_iter2_kernel); PSW_TGV_TRIPLE=1 selects the
+    # three-iteration kernel, PSW_TGV_DOUBLE=0 the one-iteration kernel.
     two = os.environ.get("PSW_TGV_DOUBLE", "1") != "0"
-    tgv_kernel, iters_per_launch = ("tgv_iter2_kernel", 2) if two else ("tgv_iter_kernel", 1)
+    three = two and os.environ.get("PSW_TGV_TRIPLE", "0") == "1"
+    tgv_kernel, iters_per_launch = (("tgv_iter3_kernel", 3) if three else ("tgv_iter2_kernel", 2) if two
+                                    else ("tgv_iter_kernel", 1))
     pl.tgv.set_u(d_s, stream=s)
     pl.tgv.linearise(pl.K1, pl.Rrel, pl.trel, s)
-    nl = 200
+    nl = 198                     # whole launches of 1, 2 or 3 iterations
     ev[0].record(st)
     pl.tgv.iterate(nl, pl.prm, s)
     ev[1].record(st)

@@ -29,11 +29,16 @@ namespace psw {
 
 constexpr int TG_TX = 64;
 constexpr int TG_THREADS = 256;   // energy kernels
-// u cycles through 6 slots, p, q, r through 3.  Two two-iteration launches
-// from state k (the energy launch and the finalising launch of a check)
-// write u of k+1..k+4 and p, q, r of k+2 and k+4, so a stop at k leaves u_k,
-// u_{k-1}, p_k, q_k, r_k in place.
-constexpr int TG_USLOTS = 6;
+// u cycles through 8 slots, p, q, r through 5.  A launch of m = 1, 2 or 3
+// iterations from state j reads u_j, u_{j-1}, p_j, q_j, r_j and writes u of
+// j+1..j+m (at least the last two) and p, q, r of j+m only, so it never
+// writes a slot it reads (8 does not divide 1..4, 5 does not divide 1..3).
+// The energy launch of a check at state k and the finalising launch after
+// it have the same m; a stop at k is seen while they write u of k+1..k+2m
+// and p, q, r of k+m and k+2m, so u_k, u_{k-1}, p_k, q_k, r_k stay in place
+// (8 divides none of 1..7, 5 none of 1..6).
+constexpr int TG_USLOTS = 8;
+constexpr int TG_PSLOTS = 5;
 
 struct TgvLayout {
     size_t n;
@@ -42,9 +47,9 @@ struct TgvLayout {
         n = (size_t)W * H;
         size_t o = 0;
         U = o;  o += 3 * TG_USLOTS * n;  // [TG_USLOTS slots][u, u1x, u1y]: u_k in slot k % TG_USLOTS
-        P = o;  o += 6 * n;      // [3 slots][px, py]: p_k in slot k % 3
-        Q = o;  o += 9 * n;      // [3 slots][qx, qy, qz]
-        r = o;  o += (size_t)3 * N * n;  // [3 slots][N][n]
+        P = o;  o += 2 * TG_PSLOTS * n;  // [TG_PSLOTS slots][px, py]: p_k in slot k % TG_PSLOTS
+        Q = o;  o += 3 * TG_PSLOTS * n;  // [TG_PSLOTS slots][qx, qy, qz]
+        r = o;  o += (size_t)TG_PSLOTS * N * n;  // [TG_PSLOTS slots][N][n]
         It = o; o += (size_t)N * n;
         Iu = o; o += (size_t)N * n;
         T = o;  o += 3 * n;      // T11, T12 (= T21), T22
@@ -92,22 +97,26 @@ struct TgvPoses {
 
 // Slots by iteration count k (so that launches doing one or two iterations
 // mix freely and a no-op launch after the stop leaves the mapping intact):
-// u_k in U slot k % TG_USLOTS, p_k / q_k / r_k in slot k % 3.  A launch reads the
+// u_k in U slot k % TG_USLOTS, p_k / q_k / r_k in slot k % TG_PSLOTS.  A launch reads the
 // slots of k (and k-1 for u) and writes those of k+1 (and k+2).
 struct TgvPtrs {
     const float* ucur;   // [3][n] u_k
     const float* uprev;  // [3][n] u_{k-1}
     float* unext;        // [3][n] u_{k+1}
     float* unext2;       // [3][n] u_{k+2}
+    float* unext3;       // [3][n] u_{k+3}
     const float* pcur;   // [2][n]
     float* pnext;
     float* pnext2;
+    float* pnext3;
     const float* qcur;   // [3][n]
     float* qnext;
     float* qnext2;
+    float* qnext3;
     const float* rcur;   // [N][n]
     float* rnext;
     float* rnext2;
+    float* rnext3;
     const float* It;
     const float* Iu;
     const float* T;      // [3][n]
@@ -281,9 +290,9 @@ __device__ __forceinline__ void u_update(float u, float u1x, float u1y, float t1
 // halo the forward differences need) and stores one partial per tile; one
 // extra workgroup of the next launch adds those partials in tile order
 // (after the kernel boundary, so with plain loads) and applies the stop
-// test.  A stop at k thus takes effect one launch later; that launch has
-// written state k+2 into other slots (u cycles through 4, p, q, r through
-// 3), so state k stays intact and the launches after it are no-ops.
+// test.  A stop at k thus takes effect one launch later; the two launches
+// wrote later states into other slots (see TG_USLOTS), so state k stays
+// intact, and the launches after them are no-ops.
 struct TgvCheck {
     double* partials;            // this launch's partials, one per tile (energy launches)
     const double* fin_partials;  // the partials to finalise (finalising launches)
@@ -698,6 +707,301 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile,
     }
 }
 
+// Three iterations (k -> k+3) of a 64 x TY tile in one launch: iteration
+// k+1 runs on the tile plus a two-pixel ring (C0), k+2 on the tile plus a
+// one-pixel ring (C1), k+3 on the tile.  Regions, as offsets from the tile
+// origin (positions outside the image read clamped coordinates and never
+// feed positions inside it, as in tgv_iter2_tile):
+//   A  ubar_k, then ubar_{k+1} (on C0), ubar_{k+2} (on C1)   -3 .. +TX+2
+//   B  T and p, q: p, q of k+1 on all of B, of k+2 on -2.., of k+3 on -1..
+//                                                              -3 .. +TX+1
+//   U  u_k (C0)                                                -2 .. +TX+1
+// One tile pixel per thread (registers: r, It, Iu, u0, T through all three
+// iterations); one ring position per thread besides (the C1 ring first, its
+// threads keep r_{k+1} and u_{k+1} for iteration k+2, then the outer ring
+// of C0).  Same per-position arithmetic as tgv_iter_tile, so identical
+// results.  Only u_{k+2}, u_{k+3}, p, q, r of k+3 reach HBM.
+template <int NS, bool ENERGY = false, int TY = 10>
+__device__ __forceinline__ void tgv_iter3_tile(const TgvPtrs& a, const int tile, const TgvCheck* chk = nullptr) {
+    constexpr int TX = TG_TX, NT = TG_TX * TY;
+    constexpr int AX = TX + 6, AY = TY + 6;
+    constexpr int BX = TX + 5, BY = TY + 5;
+    constexpr int UX = TX + 4, UY = TY + 4;
+    constexpr int R1 = 2 * (TX + 2) + 2 * TY;          // C1 ring
+    constexpr int R0 = 2 * (TX + 4) + 2 * (TY + 2);    // C0 outer ring
+    static_assert(R1 + R0 <= NT, "one ring position per thread");
+    __shared__ float sA[3][AY][AX];   // ubar; A index = U index + 1 = B index
+    __shared__ float sU[3][UY][UX];   // u_k
+    __shared__ float sP[2][BY][BX];
+    __shared__ float sQ[3][BY][BX];
+    __shared__ float sT[3][BY][BX];
+
+    const int W = a.W, H = a.H;
+    const size_t n = (size_t)W * H;
+    const int tiles_x = (W + TX - 1) / TX;
+    const int x0 = (tile % tiles_x) * TX, y0 = (tile / tiles_x) * TY;
+    const int tid = threadIdx.x;
+    const float as1 = a.alpha1 * a.sigma, as0 = a.alpha0 * a.sigma;
+    const float sl = a.sigma * a.lambda;
+
+    // ---- loads: T, p_k, q_k on B; ubar_k on A; u_k on U; the tile pixel
+    for (int i = tid; i < BX * BY; i += NT) {
+        const int iy = i / BX, ix = i - iy * BX;
+        const size_t g = (size_t)clampi(y0 - 3 + iy, H - 1) * W + clampi(x0 - 3 + ix, W - 1);
+#pragma unroll
+        for (int f = 0; f < 3; ++f) sT[f][iy][ix] = a.T[f * n + g];
+        sP[0][iy][ix] = a.pcur[g];
+        sP[1][iy][ix] = a.pcur[n + g];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) sQ[f][iy][ix] = a.qcur[f * n + g];
+    }
+    for (int i = tid; i < AX * AY; i += NT) {
+        const int iy = i / AX, ix = i - iy * AX;
+        const size_t g = (size_t)clampi(y0 - 3 + iy, H - 1) * W + clampi(x0 - 3 + ix, W - 1);
+#pragma unroll
+        for (int f = 0; f < 3; ++f) sA[f][iy][ix] = 2.f * a.ucur[f * n + g] - a.uprev[f * n + g];
+    }
+    for (int i = tid; i < UX * UY; i += NT) {
+        const int iy = i / UX, ix = i - iy * UX;
+        const size_t g = (size_t)clampi(y0 - 2 + iy, H - 1) * W + clampi(x0 - 2 + ix, W - 1);
+#pragma unroll
+        for (int f = 0; f < 3; ++f) sU[f][iy][ix] = a.ucur[f * n + g];
+    }
+    const int ty = tid / TX, tx = tid - (tid / TX) * TX;
+    const int x = x0 + tx, y = y0 + ty;
+    const size_t gt = (size_t)clampi(y, H - 1) * W + clampi(x, W - 1);
+    float rv[NS], itv[NS], iuv[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        if (j < a.N) {
+            const size_t gj = (size_t)j * n + gt;
+            rv[j] = a.rcur[gj];
+            itv[j] = a.It[gj];
+            iuv[j] = a.Iu[gj];
+        }
+    }
+    const float u0t = a.u0[gt];
+    const float t11 = a.T[gt], t12 = a.T[n + gt], t22 = a.T[2 * n + gt];
+    __syncthreads();
+
+    // ---- energy of state k at the tile pixel (energy launches)
+    double eacc = 0.0;
+    if (ENERGY && x < W && y < H && chk->mask[gt] != 0.f) {
+        const int ux = tx + 2, uy = ty + 2;
+        const float u = sU[0][uy][ux];
+        const double dud = (double)u - (double)u0t;
+        double ed = 0.0;
+#pragma unroll
+        for (int j = 0; j < NS; ++j)
+            if (j < a.N) ed += fabs((double)itv[j] + dud * (double)iuv[j]);
+        eacc = energy_point(u, sU[0][uy][ux + 1], sU[0][uy + 1][ux], sU[1][uy][ux], sU[1][uy][ux + 1],
+                            sU[1][uy + 1][ux], sU[2][uy][ux], sU[2][uy][ux + 1], sU[2][uy + 1][ux], t11, t12, t22, ed,
+                            chk->lambda, chk->alpha0, chk->alpha1);
+    }
+
+    // ---- p, q of k+1 on B (ubar_k on A, same offset)
+    for (int i = tid; i < BX * BY; i += NT) {
+        const int iy = i / BX, ix = i - iy * BX;
+        const int gx = x0 - 3 + ix, gy = y0 - 3 + iy;
+        const int rx = gx < W - 1 ? ix + 1 : ix, dy = gy < H - 1 ? iy + 1 : iy;
+        float px = sP[0][iy][ix], py = sP[1][iy][ix];
+        float qx = sQ[0][iy][ix], qy = sQ[1][iy][ix], qz = sQ[2][iy][ix];
+        pq_update(px, py, qx, qy, qz, sT[0][iy][ix], sT[1][iy][ix], sT[2][iy][ix], sA[0][iy][ix], sA[0][iy][rx],
+                  sA[0][dy][ix], sA[1][iy][ix], sA[1][iy][rx], sA[1][dy][ix], sA[2][iy][ix], sA[2][iy][rx],
+                  sA[2][dy][ix], as1, as0);
+        sP[0][iy][ix] = px; sP[1][iy][ix] = py;
+        sQ[0][iy][ix] = qx; sQ[1][iy][ix] = qy; sQ[2][iy][ix] = qz;
+    }
+    __syncthreads();
+
+    // ---- r, u of k+1 on C0 and ubar_{k+1} into A (ubar_k is no longer read)
+    float uk1[3];                      // tile pixel's u_{k+1}
+    {
+        const int ux = tx + 2, uy = ty + 2;
+        const float u = sU[0][uy][ux], u1x = sU[1][uy][ux], u1y = sU[2][uy][ux];
+        const float du = u - u0t;
+        float ps = 0.f;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            if (j < a.N) {
+                rv[j] = r_update(rv[j], itv[j], du, iuv[j], sl);
+                ps += rv[j] * iuv[j];
+            }
+        }
+        const int bx = ux + 1, by = uy + 1;
+        const int lx = x > 0 ? bx - 1 : bx, uyb = y > 0 ? by - 1 : by;
+        u_update(u, u1x, u1y, t11, t12, t22, sP[0][by][bx], sP[1][by][bx], sP[0][by][lx], sP[1][by][lx],
+                 sP[0][uyb][bx], sP[1][uyb][bx], sQ[0][by][bx], sQ[0][by][lx], sQ[1][by][bx], sQ[1][uyb][bx],
+                 sQ[2][by][bx], sQ[2][by][lx], sQ[2][uyb][bx], ps, a, uk1[0], uk1[1], uk1[2]);
+        sA[0][by][bx] = 2.f * uk1[0] - u;
+        sA[1][by][bx] = 2.f * uk1[1] - u1x;
+        sA[2][by][bx] = 2.f * uk1[2] - u1y;
+    }
+    // the ring position of this thread (C1 ring for tid < R1, then the outer ring)
+    int rux = 0, ruy = 0;
+    const bool ring = tid < R1 + R0;
+    if (tid < R1) {
+        ring_pos<TX + 2, TY + 2, TY>(tid, rux, ruy);
+        rux += 1;
+        ruy += 1;
+    } else if (ring) {
+        ring_pos<TX + 4, TY + 4, TY + 2>(tid - R1, rux, ruy);
+    }
+    const int rgx = x0 - 2 + rux, rgy = y0 - 2 + ruy;
+    const size_t rg = (size_t)clampi(rgy, H - 1) * W + clampi(rgx, W - 1);
+    float rr1[NS], ru1[3] = {0.f, 0.f, 0.f}, ru0 = 0.f;
+    if (ring) {
+        const float u = sU[0][ruy][rux], u1x = sU[1][ruy][rux], u1y = sU[2][ruy][rux];
+        ru0 = a.u0[rg];
+        const float du = u - ru0;
+        float ps = 0.f;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            if (j < a.N) {
+                const size_t gj = (size_t)j * n + rg;
+                const float iu = a.Iu[gj];
+                rr1[j] = r_update(a.rcur[gj], a.It[gj], du, iu, sl);
+                ps += rr1[j] * iu;
+            }
+        }
+        const int bx = rux + 1, by = ruy + 1;
+        const int lx = rgx > 0 ? bx - 1 : bx, uyb = rgy > 0 ? by - 1 : by;
+        u_update(u, u1x, u1y, sT[0][by][bx], sT[1][by][bx], sT[2][by][bx], sP[0][by][bx], sP[1][by][bx],
+                 sP[0][by][lx], sP[1][by][lx], sP[0][uyb][bx], sP[1][uyb][bx], sQ[0][by][bx], sQ[0][by][lx],
+                 sQ[1][by][bx], sQ[1][uyb][bx], sQ[2][by][bx], sQ[2][by][lx], sQ[2][uyb][bx], ps, a, ru1[0], ru1[1],
+                 ru1[2]);
+        sA[0][by][bx] = 2.f * ru1[0] - u;
+        sA[1][by][bx] = 2.f * ru1[1] - u1x;
+        sA[2][by][bx] = 2.f * ru1[2] - u1y;
+    }
+    __syncthreads();
+
+    // ---- p, q of k+2 on B indices [1, BX-1) x [1, BY-1) (ubar_{k+1})
+    for (int i = tid; i < (BX - 2) * (BY - 2); i += NT) {
+        const int iy = i / (BX - 2) + 1, ix = i - (i / (BX - 2)) * (BX - 2) + 1;
+        const int gx = x0 - 3 + ix, gy = y0 - 3 + iy;
+        const int rx = gx < W - 1 ? ix + 1 : ix, dy = gy < H - 1 ? iy + 1 : iy;
+        float px = sP[0][iy][ix], py = sP[1][iy][ix];
+        float qx = sQ[0][iy][ix], qy = sQ[1][iy][ix], qz = sQ[2][iy][ix];
+        pq_update(px, py, qx, qy, qz, sT[0][iy][ix], sT[1][iy][ix], sT[2][iy][ix], sA[0][iy][ix], sA[0][iy][rx],
+                  sA[0][dy][ix], sA[1][iy][ix], sA[1][iy][rx], sA[1][dy][ix], sA[2][iy][ix], sA[2][iy][rx],
+                  sA[2][dy][ix], as1, as0);
+        sP[0][iy][ix] = px; sP[1][iy][ix] = py;
+        sQ[0][iy][ix] = qx; sQ[1][iy][ix] = qy; sQ[2][iy][ix] = qz;
+    }
+    __syncthreads();
+
+    // ---- r, u of k+2 on C1 and ubar_{k+2} into A
+    float uk2[3];
+    {
+        const int ux = tx + 2, uy = ty + 2;
+        const float du = uk1[0] - u0t;
+        float ps = 0.f;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            if (j < a.N) {
+                rv[j] = r_update(rv[j], itv[j], du, iuv[j], sl);
+                ps += rv[j] * iuv[j];
+            }
+        }
+        const int bx = ux + 1, by = uy + 1;
+        const int lx = x > 0 ? bx - 1 : bx, uyb = y > 0 ? by - 1 : by;
+        u_update(uk1[0], uk1[1], uk1[2], t11, t12, t22, sP[0][by][bx], sP[1][by][bx], sP[0][by][lx], sP[1][by][lx],
+                 sP[0][uyb][bx], sP[1][uyb][bx], sQ[0][by][bx], sQ[0][by][lx], sQ[1][by][bx], sQ[1][uyb][bx],
+                 sQ[2][by][bx], sQ[2][by][lx], sQ[2][uyb][bx], ps, a, uk2[0], uk2[1], uk2[2]);
+        sA[0][by][bx] = 2.f * uk2[0] - uk1[0];
+        sA[1][by][bx] = 2.f * uk2[1] - uk1[1];
+        sA[2][by][bx] = 2.f * uk2[2] - uk1[2];
+    }
+    if (tid < R1) {
+        const float du = ru1[0] - ru0;
+        float ps = 0.f;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            if (j < a.N) {
+                const float iu = a.Iu[(size_t)j * n + rg];
+                ps += r_update(rr1[j], a.It[(size_t)j * n + rg], du, iu, sl) * iu;
+            }
+        }
+        const int bx = rux + 1, by = ruy + 1;
+        const int lx = rgx > 0 ? bx - 1 : bx, uyb = rgy > 0 ? by - 1 : by;
+        float un, u1xn, u1yn;
+        u_update(ru1[0], ru1[1], ru1[2], sT[0][by][bx], sT[1][by][bx], sT[2][by][bx], sP[0][by][bx], sP[1][by][bx],
+                 sP[0][by][lx], sP[1][by][lx], sP[0][uyb][bx], sP[1][uyb][bx], sQ[0][by][bx], sQ[0][by][lx],
+                 sQ[1][by][bx], sQ[1][uyb][bx], sQ[2][by][bx], sQ[2][by][lx], sQ[2][uyb][bx], ps, a, un, u1xn,
+                 u1yn);
+        sA[0][by][bx] = 2.f * un - ru1[0];
+        sA[1][by][bx] = 2.f * u1xn - ru1[1];
+        sA[2][by][bx] = 2.f * u1yn - ru1[2];
+    }
+    __syncthreads();
+
+    // ---- p, q of k+3 on B indices [2, BX-2) x [2, BY-2) (ubar_{k+2}); the tile's go to HBM
+    for (int i = tid; i < (BX - 4) * (BY - 4); i += NT) {
+        const int iy = i / (BX - 4) + 2, ix = i - (i / (BX - 4)) * (BX - 4) + 2;
+        const int gx = x0 - 3 + ix, gy = y0 - 3 + iy;
+        const int rx = gx < W - 1 ? ix + 1 : ix, dy = gy < H - 1 ? iy + 1 : iy;
+        float px = sP[0][iy][ix], py = sP[1][iy][ix];
+        float qx = sQ[0][iy][ix], qy = sQ[1][iy][ix], qz = sQ[2][iy][ix];
+        pq_update(px, py, qx, qy, qz, sT[0][iy][ix], sT[1][iy][ix], sT[2][iy][ix], sA[0][iy][ix], sA[0][iy][rx],
+                  sA[0][dy][ix], sA[1][iy][ix], sA[1][iy][rx], sA[1][dy][ix], sA[2][iy][ix], sA[2][iy][rx],
+                  sA[2][dy][ix], as1, as0);
+        sP[0][iy][ix] = px; sP[1][iy][ix] = py;
+        sQ[0][iy][ix] = qx; sQ[1][iy][ix] = qy; sQ[2][iy][ix] = qz;
+        if (ix >= 3 && iy >= 3 && gx < W && gy < H) {
+            const size_t g = (size_t)gy * W + gx;
+            a.pnext3[g] = px; a.pnext3[n + g] = py;
+            a.qnext3[g] = qx; a.qnext3[n + g] = qy; a.qnext3[2 * n + g] = qz;
+        }
+    }
+    __syncthreads();
+
+    // ---- tile: r and u of k+3
+    if (x < W && y < H) {
+        const size_t g = (size_t)y * W + x;
+        const float du = uk2[0] - u0t;
+        float ps = 0.f;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            if (j < a.N) {
+                const float rr = r_update(rv[j], itv[j], du, iuv[j], sl);
+                a.rnext3[(size_t)j * n + g] = rr;
+                ps += rr * iuv[j];
+            }
+        }
+        const int bx = tx + 3, by = ty + 3;
+        const int lx = x > 0 ? bx - 1 : bx, uyb = y > 0 ? by - 1 : by;
+        float un, u1xn, u1yn;
+        u_update(uk2[0], uk2[1], uk2[2], t11, t12, t22, sP[0][by][bx], sP[1][by][bx], sP[0][by][lx], sP[1][by][lx],
+                 sP[0][uyb][bx], sP[1][uyb][bx], sQ[0][by][bx], sQ[0][by][lx], sQ[1][by][bx], sQ[1][uyb][bx],
+                 sQ[2][by][bx], sQ[2][by][lx], sQ[2][uyb][bx], ps, a, un, u1xn, u1yn);
+        a.unext3[g] = un; a.unext3[n + g] = u1xn; a.unext3[2 * n + g] = u1yn;
+        a.unext2[g] = uk2[0]; a.unext2[n + g] = uk2[1]; a.unext2[2 * n + g] = uk2[2];
+    }
+    if (ENERGY) {
+        const double e = block_sum<NT>(eacc);
+        if (tid == 0) chk->partials[tile] = e;
+    }
+}
+
+template <int NS, int TY>
+__global__ __launch_bounds__(TG_TX * TY) void tgv_iter3_kernel(TgvPtrs a) {
+    if (a.ctrl->done) return;
+    const int ntiles = ((a.W + TG_TX - 1) / TG_TX) * ((a.H + TY - 1) / TY);
+    tgv_iter3_tile<NS, false, TY>(a, xcd_tile(blockIdx.x, ntiles));
+}
+
+template <int NS, int TY, bool ENERGY, bool FINAL>
+__global__ __launch_bounds__(TG_TX * TY) void tgv_iter3_check_kernel(TgvPtrs a, TgvCheck c) {
+    if (a.ctrl->done) return;
+    const int ntiles = ((a.W + TG_TX - 1) / TG_TX) * ((a.H + TY - 1) / TY);
+    if (FINAL && (int)blockIdx.x == ntiles) {
+        check_finalize<TG_TX * TY>(c);
+        return;
+    }
+    tgv_iter3_tile<NS, ENERGY, TY>(a, xcd_tile(blockIdx.x, ntiles), &c);
+}
+
 template <int NS, int TY>
 __global__ __launch_bounds__(TG_TX * TY) void tgv_iter2_kernel(TgvPtrs a) {
     if (a.ctrl->done) return;
@@ -825,15 +1129,19 @@ __host__ __device__ inline TgvPtrs tgv_ptrs_at(float* B, const TgvLayout& L, int
     p.uprev = B + L.U + (size_t)((k + TG_USLOTS - 1) % TG_USLOTS) * 3 * n;
     p.unext = B + L.U + (size_t)((k + 1) % TG_USLOTS) * 3 * n;
     p.unext2 = B + L.U + (size_t)((k + 2) % TG_USLOTS) * 3 * n;
-    p.pcur = B + L.P + (size_t)(k % 3) * 2 * n;
-    p.pnext = B + L.P + (size_t)((k + 1) % 3) * 2 * n;
-    p.pnext2 = B + L.P + (size_t)((k + 2) % 3) * 2 * n;
-    p.qcur = B + L.Q + (size_t)(k % 3) * 3 * n;
-    p.qnext = B + L.Q + (size_t)((k + 1) % 3) * 3 * n;
-    p.qnext2 = B + L.Q + (size_t)((k + 2) % 3) * 3 * n;
-    p.rcur = B + L.r + (size_t)(k % 3) * rn;
-    p.rnext = B + L.r + (size_t)((k + 1) % 3) * rn;
-    p.rnext2 = B + L.r + (size_t)((k + 2) % 3) * rn;
+    p.unext3 = B + L.U + (size_t)((k + 3) % TG_USLOTS) * 3 * n;
+    p.pcur = B + L.P + (size_t)(k % TG_PSLOTS) * 2 * n;
+    p.pnext = B + L.P + (size_t)((k + 1) % TG_PSLOTS) * 2 * n;
+    p.pnext2 = B + L.P + (size_t)((k + 2) % TG_PSLOTS) * 2 * n;
+    p.pnext3 = B + L.P + (size_t)((k + 3) % TG_PSLOTS) * 2 * n;
+    p.qcur = B + L.Q + (size_t)(k % TG_PSLOTS) * 3 * n;
+    p.qnext = B + L.Q + (size_t)((k + 1) % TG_PSLOTS) * 3 * n;
+    p.qnext2 = B + L.Q + (size_t)((k + 2) % TG_PSLOTS) * 3 * n;
+    p.qnext3 = B + L.Q + (size_t)((k + 3) % TG_PSLOTS) * 3 * n;
+    p.rcur = B + L.r + (size_t)(k % TG_PSLOTS) * rn;
+    p.rnext = B + L.r + (size_t)((k + 1) % TG_PSLOTS) * rn;
+    p.rnext2 = B + L.r + (size_t)((k + 2) % TG_PSLOTS) * rn;
+    p.rnext3 = B + L.r + (size_t)((k + 3) % TG_PSLOTS) * rn;
     p.It = B + L.It;
     p.Iu = B + L.Iu;
     p.T = B + L.T;
@@ -911,6 +1219,59 @@ static int tgv_launch_double(psw_tgv* t, const psw_tgv_params* prm, hipStream_t 
     return s;
 }
 
+// Three iterations per launch (tgv_iter3_kernel, 64 x 10 tiles of 640
+// threads) where a launch has three iterations to do, with PSW_TGV_TRIPLE=1;
+// identical results.  Off by default: at 640x480 with 8 sources it moves
+// fewer bytes per iteration than tgv_iter2_kernel but takes 10.0 us per
+// iteration against 9.05 (its extra halo work and a second round of
+// scattered ring loads cost more VALU time than the HBM time it saves).
+constexpr int T3_TY = 10;
+
+static bool tgv_triple() {
+    const char* e = getenv("PSW_TGV_TRIPLE");
+    return e && atoi(e) == 1;
+}
+
+static int tgv_tiles3(const psw_tgv* t) {
+    return ((t->W + TG_TX - 1) / TG_TX) * ((t->H + T3_TY - 1) / T3_TY);
+}
+
+// Most iterations one launch may do for this solver: 3, 2 or 1.
+static int tgv_max_per_launch(const psw_tgv* t) {
+    if (t->N > T2_MAXN || !tgv_double()) return 1;
+    return tgv_triple() ? 3 : 2;
+}
+
+template <int NS>
+static void tgv_launch_triple_ns(const TgvPtrs& p, const TgvCheck* c, bool energy, bool fin, int ntiles,
+                                 hipStream_t st) {
+    const unsigned g = (unsigned)(ntiles + (fin ? 1 : 0));
+    constexpr int NT = TG_TX * T3_TY;
+    if (energy && fin) tgv_iter3_check_kernel<NS, T3_TY, true, true><<<g, NT, 0, st>>>(p, *c);
+    else if (energy) tgv_iter3_check_kernel<NS, T3_TY, true, false><<<g, NT, 0, st>>>(p, *c);
+    else if (fin) tgv_iter3_check_kernel<NS, T3_TY, false, true><<<g, NT, 0, st>>>(p, *c);
+    else tgv_iter3_kernel<NS, T3_TY><<<g, NT, 0, st>>>(p);
+}
+
+// Three iterations in one launch (check arguments as tgv_launch_double).
+static int tgv_launch_triple(psw_tgv* t, const psw_tgv_params* prm, hipStream_t st, const TgvCheck* c = nullptr,
+                             bool energy = false, bool fin = false) {
+    const TgvPtrs p = tgv_ptrs(t, prm);
+    const int nt3 = tgv_tiles3(t);
+    if (t->N <= 8) tgv_launch_triple_ns<8>(p, c, energy, fin, nt3, st);
+    else tgv_launch_triple_ns<T2_MAXN>(p, c, energy, fin, nt3, st);
+    const int s = launch_status(energy || fin ? "tgv_iter3_check_kernel" : "tgv_iter3_kernel");
+    if (s == PSW_OK) t->k += 3;
+    return s;
+}
+
+// Iterations of the next launch when `left` remain and at most `mmax` fit
+// one launch: three while they last, but 4 as 2 + 2 (no single launch).
+static int tgv_launch_size(int left, int mmax) {
+    int m = std::min(left, mmax);
+    if (m == 3 && left == 4) m = 2;
+    return m;
+}
 
 // Partials of the checking launches: one per tile of the smallest tile shape.
 static int tgv_tile_partials(int W, int H) { return ((W + TG_TX - 1) / TG_TX) * ((H + 3) / 4); }
@@ -952,11 +1313,12 @@ static int tgv_launch_iters(psw_tgv* t, int niters, const psw_tgv_params* prm, h
     const int var = tgv_variant();
     const int ty = var == 2 ? 4 : 8;
     const dim3 grid((unsigned)(((t->W + TG_TX - 1) / TG_TX) * ((t->H + ty - 1) / ty)));
-    if (tgv_double() && t->N <= T2_MAXN) {
-        for (; niters >= 2; niters -= 2) {
-            const int s = tgv_launch_double(t, prm, st);
-            if (s != PSW_OK) return s;
-        }
+    const int mmax = tgv_max_per_launch(t);
+    while (niters >= 2 && mmax >= 2) {
+        const int m = tgv_launch_size(niters, mmax);
+        const int s = m == 3 ? tgv_launch_triple(t, prm, st) : tgv_launch_double(t, prm, st);
+        if (s != PSW_OK) return s;
+        niters -= m;
     }
     TgvCheck none{};
     for (int i = 0; i < niters; ++i) {
@@ -1177,8 +1539,8 @@ int psw_tgv_linearise(psw_tgv* t, const float* K, const float* Rrel, const float
     const TgvLayout& L = t->L;
     const size_t n = L.n;
     const int cur = t->k % TG_USLOTS, prev = (t->k + TG_USLOTS - 1) % TG_USLOTS;
-    tgv_warp_reset_kernel<<<1024, 256, 0, S(s)>>>(t->base + L.U, t->base + L.P + (size_t)(t->k % 3) * 2 * n,
-                                                  t->base + L.Q + (size_t)(t->k % 3) * 3 * n, t->base + L.u0, n, cur,
+    tgv_warp_reset_kernel<<<1024, 256, 0, S(s)>>>(t->base + L.U, t->base + L.P + (size_t)(t->k % TG_PSLOTS) * 2 * n,
+                                                  t->base + L.Q + (size_t)(t->k % TG_PSLOTS) * 3 * n, t->base + L.u0, n, cur,
                                                   prev);
     int st = launch_status("tgv_warp_reset_kernel");
     if (st != PSW_OK) return st;
@@ -1203,7 +1565,7 @@ int psw_tgv_linearise(psw_tgv* t, const float* K, const float* Rrel, const float
     pz.fy = (float)(double)K[4];
     const dim3 blk(64, 4), grd((unsigned)((t->W + 63) / 64), (unsigned)((t->H + 3) / 4));
     tgv_linearise_kernel<<<grd, blk, 0, S(s)>>>(pz, t->base + L.src, t->base + L.ref, t->base + L.u0, t->base + L.It,
-                                                t->base + L.Iu, t->base + L.r + (size_t)(t->k % 3) * t->N * n, t->W,
+                                                t->base + L.Iu, t->base + L.r + (size_t)(t->k % TG_PSLOTS) * t->N * n, t->W,
                                                 t->H, t->N);
     st = launch_status("tgv_linearise_kernel");
     if (st != PSW_OK) return st;
@@ -1216,9 +1578,9 @@ int psw_tgv_set_primal(psw_tgv* t, const float* d_u, float u_value, const float*
     const TgvLayout& L = t->L;
     const size_t n = L.n;
     const int cur = t->k % TG_USLOTS, prev = (t->k + TG_USLOTS - 1) % TG_USLOTS;
-    tgv_set_primal_kernel<<<1024, 256, 0, S(s)>>>(t->base + L.U, t->base + L.P + (size_t)(t->k % 3) * 2 * n,
-                                                  t->base + L.Q + (size_t)(t->k % 3) * 3 * n,
-                                                  t->base + L.r + (size_t)(t->k % 3) * t->N * n, d_u,
+    tgv_set_primal_kernel<<<1024, 256, 0, S(s)>>>(t->base + L.U, t->base + L.P + (size_t)(t->k % TG_PSLOTS) * 2 * n,
+                                                  t->base + L.Q + (size_t)(t->k % TG_PSLOTS) * 3 * n,
+                                                  t->base + L.r + (size_t)(t->k % TG_PSLOTS) * t->N * n, d_u,
                                                   u_value, d_u1, n, t->N, cur, prev);
     int st = launch_status("tgv_set_primal_kernel");
     if (st != PSW_OK) return st;
@@ -1229,9 +1591,9 @@ int psw_tgv_set_duals(psw_tgv* t, const float* d_p, const float* d_q, const floa
     PSW_CHECK_ARG(t, "NULL solver");
     const TgvLayout& L = t->L;
     const size_t n = L.n;
-    float* P = t->base + L.P + (size_t)(t->k % 3) * 2 * n;
-    float* Q = t->base + L.Q + (size_t)(t->k % 3) * 3 * n;
-    float* R = t->base + L.r + (size_t)(t->k % 3) * t->N * n;
+    float* P = t->base + L.P + (size_t)(t->k % TG_PSLOTS) * 2 * n;
+    float* Q = t->base + L.Q + (size_t)(t->k % TG_PSLOTS) * 3 * n;
+    float* R = t->base + L.r + (size_t)(t->k % TG_PSLOTS) * t->N * n;
     int st = PSW_OK;
     if (d_p) PSW_TRY_HIP(hipMemcpyAsync(P, d_p, 2 * n * sizeof(float), hipMemcpyDeviceToDevice, S(s)), PSW_ERR_COPY);
     else st = psw_fill_f32(P, 0.f, 2 * n, s);
@@ -1325,15 +1687,15 @@ int psw_tgv_solve(psw_tgv* t, int max_iters, int check_every, double tol, double
     PSW_TRY_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming), PSW_ERR_DEVICE);
     PSW_TRY_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming), PSW_ERR_DEVICE);
     // A check is an energy launch from state k and the finalising launch
-    // after it; both do two iterations (or, near the end of a chunk, both
-    // one), so a stop at k leaves u_k, u_{k-1}, p_k, q_k, r_k in place (slot
-    // counts at TG_USLOTS).  The other launches do two iterations each.
-    const bool dbl = tgv_double() && t->N <= T2_MAXN;
-    const int ntiles2 = tgv_tiles2(t);
+    // after it; both do the same number of iterations, so a stop at k leaves
+    // u_k, u_{k-1}, p_k, q_k, r_k in place (slot counts at TG_USLOTS).  The
+    // other launches do up to three iterations each (tgv_launch_size).
+    const int mmax = tgv_max_per_launch(t);
+    const int ntiles2 = tgv_tiles2(t), ntiles3 = tgv_tiles3(t);
     int done_iters = -1, chunk_idx = 0, nchecks = 0;
     int launched = 0;
     bool stopped = false, pending = false;   // pending: an energy sum not yet finalised
-    bool pend_two = false;
+    int pend_m = 1;                          // iterations of the pending energy launch
     double* pend_buf = nullptr;
     int pend_k = 0, pend_count = 0;
     while (launched < max_iters && rc == PSW_OK) {
@@ -1341,10 +1703,10 @@ int psw_tgv_solve(psw_tgv* t, int max_iters, int check_every, double tol, double
         for (int i = 0; i < c && rc == PSW_OK;) {
             const bool energy = i == 0 && launched > 0;
             const bool fin = pending;
-            bool two;
-            if (fin) two = pend_two;                        // the same kind as its energy launch
-            else if (energy) two = dbl && c - i >= 4;       // room for a two-iteration finalising launch
-            else two = dbl && c - i >= 2;
+            int m;
+            if (fin) m = pend_m;                                      // the same kind as its energy launch
+            else if (energy) m = std::max(1, std::min(mmax, (c - i) / 2));   // room for its finalising launch
+            else m = tgv_launch_size(c - i, mmax);
             if (fin) {
                 chk.fin_partials = pend_buf;
                 chk.fin_k = pend_k;
@@ -1352,16 +1714,18 @@ int psw_tgv_solve(psw_tgv* t, int max_iters, int check_every, double tol, double
             }
             if (energy) chk.partials = t->partials + (size_t)(nchecks & 1) * half;
             const int k_before = t->k;
-            rc = two ? tgv_launch_double(t, prm, st, &chk, energy, fin) : tgv_launch_one(t, prm, chk, energy, fin, st);
+            rc = m == 3   ? tgv_launch_triple(t, prm, st, &chk, energy, fin)
+                 : m == 2 ? tgv_launch_double(t, prm, st, &chk, energy, fin)
+                          : tgv_launch_one(t, prm, chk, energy, fin, st);
             if (energy) {
                 pend_buf = chk.partials;
                 pend_k = k_before;
-                pend_count = two ? ntiles2 : ntiles;
-                pend_two = two;
+                pend_count = m == 3 ? ntiles3 : m == 2 ? ntiles2 : ntiles;
+                pend_m = m;
                 nchecks++;
             }
             pending = energy;
-            i += two ? 2 : 1;
+            i += m;
         }
         if (rc != PSW_OK) break;
         launched += c;
@@ -1414,15 +1778,15 @@ int psw_tgv_get_state(psw_tgv* t, float* d_out, psw_stream_t s) {
         return hipMemcpyAsync(dst, src, cnt * sizeof(float), hipMemcpyDeviceToDevice, S(s));
     };
     const float* U = B + L.U + (size_t)(t->k % TG_USLOTS) * 3 * n;
-    const float* P = B + L.P + (size_t)(t->k % 3) * 2 * n;
-    const float* Q = B + L.Q + (size_t)(t->k % 3) * 3 * n;
+    const float* P = B + L.P + (size_t)(t->k % TG_PSLOTS) * 2 * n;
+    const float* Q = B + L.Q + (size_t)(t->k % TG_PSLOTS) * 3 * n;
     float* o = d_out;
     PSW_TRY_HIP(cp(o, U, 3 * n), PSW_ERR_COPY); o += 3 * n;
     PSW_TRY_HIP(cp(o, P, 2 * n), PSW_ERR_COPY); o += 2 * n;
     PSW_TRY_HIP(cp(o, Q, 3 * n), PSW_ERR_COPY); o += 3 * n;
     PSW_TRY_HIP(cp(o, B + L.It, N * n), PSW_ERR_COPY); o += N * n;
     PSW_TRY_HIP(cp(o, B + L.Iu, N * n), PSW_ERR_COPY); o += N * n;
-    PSW_TRY_HIP(cp(o, B + L.r + (size_t)(t->k % 3) * N * n, N * n), PSW_ERR_COPY); o += N * n;
+    PSW_TRY_HIP(cp(o, B + L.r + (size_t)(t->k % TG_PSLOTS) * N * n, N * n), PSW_ERR_COPY); o += N * n;
     PSW_TRY_HIP(cp(o, B + L.T, 3 * n), PSW_ERR_COPY); o += 3 * n;
     PSW_TRY_HIP(cp(o, B + L.u0, n), PSW_ERR_COPY);
     return PSW_OK;

@@ -172,10 +172,10 @@ def test_tgv_fused_vs_fp64_oracle(gpu, views_small, niters, warps):
 
 def test_tgv_launch_shapes_identical(gpu, views_small, monkeypatch):
     """The three tile/thread shapes of the one-iteration kernel and the
-    two-iterations-per-launch kernel give identical iterates (each pixel's
-    arithmetic does not depend on the tiling); the full state (u, u1, p, q,
-    r) is compared after an odd number of iterations, so the two-iteration
-    path also ends with a one-iteration launch."""
+    two- and three-iterations-per-launch kernels give identical iterates (each
+    pixel's arithmetic does not depend on the tiling); the full state (u, u1,
+    p, q, r) is compared after 37 + 5 + 1 iterations, so the multi-iteration
+    paths mix 3-, 2- and 1-iteration launches."""
     from psw_amd import stages
     from psw_amd.device import DeviceArray
     v = views_small
@@ -187,9 +187,12 @@ def test_tgv_launch_shapes_identical(gpu, views_small, monkeypatch):
     d_ref, d_src = DeviceArray.from_host(v["ref"]), DeviceArray.from_host(v["src"])
     d_init = DeviceArray.from_host(np.clip(v["depth"], 0.5, 6.0).astype(np.float32))
     states = []
-    for double, var, ty2 in (("0", "0", "10"), ("0", "1", "10"), ("0", "2", "10"), ("1", "1", "10"),
-                             ("1", "0", "10"), ("1", "1", "4"), ("1", "1", "8"), ("1", "1", "12")):
+    for double, triple, var, ty2 in (("0", "0", "0", "10"), ("0", "0", "1", "10"), ("0", "0", "2", "10"),
+                                     ("1", "0", "1", "10"), ("1", "0", "0", "10"), ("1", "0", "1", "4"),
+                                     ("1", "0", "1", "8"), ("1", "0", "1", "12"), ("1", "1", "1", "10"),
+                                     ("1", "1", "2", "4")):
         monkeypatch.setenv("PSW_TGV_DOUBLE", double)
+        monkeypatch.setenv("PSW_TGV_TRIPLE", triple)
         monkeypatch.setenv("PSW_TGV_VARIANT", var)
         monkeypatch.setenv("PSW_TGV2_TY", ty2)
         t = stages.TgvSolver(W, Hh, N)
@@ -197,6 +200,8 @@ def test_tgv_launch_shapes_identical(gpu, views_small, monkeypatch):
         t.set_u(d_init)
         t.linearise(K1, Rrel, trel)
         t.iterate(37, prm)
+        t.iterate(5, prm)
+        t.iterate(1, prm)
         states.append(t.get_state())
         # and the reference driver (warps x iterations) through the same kernels
         states[-1]["run"] = stages.tgv_run(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], 25, 2,
@@ -366,17 +371,21 @@ def test_tgv_solve_stops_at_gap(gpu, views_small):
         np.testing.assert_array_equal(sa[name], sb[name], err_msg=name)
 
 
-@pytest.mark.parametrize("variant,double,ty2", [("0", "1", "10"), ("1", "1", "10"), ("2", "1", "10"), ("1", "0", "10"),
-                                               ("1", "1", "4"), ("1", "1", "8"), ("1", "1", "12")])
-def test_tgv_solve_max_iters_and_variants(gpu, views_small, monkeypatch, variant, double, ty2):
-    """Every launch shape, with and without two-iteration launches, stops at
-    the same iteration with the same state; a solve that reaches max_iters
-    reports max_iters and the energy of the final state."""
+@pytest.mark.parametrize("variant,double,ty2,triple", [("0", "1", "10", "0"), ("1", "1", "10", "0"),
+                                                      ("2", "1", "10", "0"), ("1", "0", "10", "0"),
+                                                      ("1", "1", "4", "0"), ("1", "1", "8", "0"),
+                                                      ("1", "1", "12", "0"), ("1", "1", "10", "1"),
+                                                      ("2", "1", "8", "1")])
+def test_tgv_solve_max_iters_and_variants(gpu, views_small, monkeypatch, variant, double, ty2, triple):
+    """Every launch shape, with and without two- and three-iteration
+    launches, stops at the same iteration with the same state; a solve that
+    reaches max_iters reports max_iters and the energy of the final state."""
     from psw_amd import stages
     from psw_amd.device import DeviceArray
     monkeypatch.setenv("PSW_TGV_VARIANT", variant)
     monkeypatch.setenv("PSW_TGV_DOUBLE", double)
     monkeypatch.setenv("PSW_TGV2_TY", ty2)
+    monkeypatch.setenv("PSW_TGV_TRIPLE", triple)
     v = views_small
     K1 = one_based(v["K"])
     prm = stages.tgv_params(**TGV_KW)
