@@ -82,7 +82,10 @@ FUSION_GRID = 256
 FUSION_MAPS = 50
 FUSION_ITERS = 300
 FUSION_VOLUME = np.array([-3.0, -1.6, -4.0, 3.0, 1.6, 6.0], np.float32)
-FUSION_BYTES_PER_VOXEL_ITER = 56   # fused PD: read p 12, v 4, u 4, hist 16; write p 12, u 4, v 4
+# fused PD: read p 12, v 4, u 4, counts 8 (the byte-packed copy the solver
+# reads when every count fits in a byte, as with 50 depthmaps); write p 12,
+# u 4, v 4
+FUSION_BYTES_PER_VOXEL_ITER = 48
 
 
 def fusion_workload(seed):
@@ -113,6 +116,8 @@ def run_fusion(o, thr, stream, ev):
     res["surface_voxels"] = f.surface_count(stream.handle)
     res["voxel_iters_per_s"] = vox * FUSION_ITERS / (res["pd_ms"] * 1e-3)
     res["pd_gbs_algorithmic"] = FUSION_BYTES_PER_VOXEL_ITER * vox * FUSION_ITERS / (res["pd_ms"] * 1e-3) / 1e9
+    res["pd_frac_hbm_peak"] = res["pd_gbs_algorithmic"] / HBM_PEAK_GBS
+    res["pd_bytes_per_voxel_iter"] = FUSION_BYTES_PER_VOXEL_ITER
     res["hist_voxel_maps_per_s"] = vox * FUSION_MAPS / (res["hist_ms"] * 1e-3)
     res["config"] = (f"{g}^3 voxels over the room box, {FUSION_MAPS} depthmaps 640x480 (ray-cast depth + 1 cm "
                      f"noise, orbiting cameras), 8 uint16 bins, band +-{thr:.3f} m (5 voxels), "

@@ -21,7 +21,9 @@
 // smallest of the union with the 8 (sorted, constant) bin centres is
 // selected.
 #include "psw_common.h"
+#include "rn_math.h"
 
+#include <type_traits>
 #include <vector>
 
 namespace psw {
@@ -154,10 +156,27 @@ __device__ inline float prox_hist(float u, const uint16_t* __restrict__ hp, floa
     return prox_hist_h(u, *reinterpret_cast<const uint4*>(hp), tl);
 }
 
+__device__ inline float prox_hist_counts(float u, const int* h, float tl);
+
 __device__ inline float prox_hist_h(float u, uint4 hv, float tl) {
     int h[FB];
     h[0] = hv.x & 0xffff; h[1] = hv.x >> 16; h[2] = hv.y & 0xffff; h[3] = hv.y >> 16;
     h[4] = hv.z & 0xffff; h[5] = hv.z >> 16; h[6] = hv.w & 0xffff; h[7] = hv.w >> 16;
+    return prox_hist_counts(u, h, tl);
+}
+
+// The same from the byte-packed copy of the counts (all counts <= 255).
+__device__ inline float prox_hist_h8(float u, uint2 hv, float tl) {
+    int h[FB];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        h[k] = (hv.x >> (8 * k)) & 0xff;
+        h[4 + k] = (hv.y >> (8 * k)) & 0xff;
+    }
+    return prox_hist_counts(u, h, tl);
+}
+
+__device__ inline float prox_hist_counts(float u, const int* h, float tl) {
     int total = 0;
 #pragma unroll
     for (int k = 0; k < FB; ++k) total += h[k];
@@ -211,6 +230,7 @@ struct PdArgs {
     float* uout;
     const float* vin;
     float* vout;
+    const uint2* hist8;   // byte-packed counts (fusion_pd_zpipe_kernel<true>)
     float tau, lambda, tl, sigma;
 };
 
@@ -271,17 +291,10 @@ __global__ void fusion_p_kernel(FusionView f, float sigma) {
 // root or division inside the ball.
 __device__ __forceinline__ void p_step(float p0, float p1, float p2, float gx, float gy, float gz, float sigma,
                                        float& ox, float& oy, float& oz) {
-    const float ax = p0 + sigma * gx, ay = p1 + sigma * gy, az = p2 + sigma * gz;
-    const float s2 = ax * ax + ay * ay + az * az;
-    ox = ax;
-    oy = ay;
-    oz = az;
-    if (s2 > 1.f) {
-        const float dn = sqrtf(s2);
-        ox = ax / dn;
-        oy = ay / dn;
-        oz = az / dn;
-    }
+    ox = p0 + sigma * gx;
+    oy = p1 + sigma * gy;
+    oz = p2 + sigma * gz;
+    project_ball3(ox * ox + oy * oy + oz * oz, ox, oy, oz);
 }
 
 __device__ inline void p_new_at(const PdArgs& a, size_t V, int x, int y, int z, size_t i, float& ox, float& oy,
@@ -377,14 +390,18 @@ __global__ __launch_bounds__(FZ_BX * FZ_BY) void fusion_pd_zmarch_kernel(PdArgs 
 // are stored, and the z+1 values of v that plane z's gradients need come
 // from that bundle.  (The compiler cannot hoist those loads itself: u is read
 // and written in place.)  Same arithmetic and order as fusion_pd_fused_kernel.
+// H8: the histogram comes from the byte-packed copy (8 B per voxel instead
+// of 16); same counts, same results.
+template <bool H8>
 struct ZBundle {
     float v0, vxp, vyp, vxm, vxmyp, vym, vymxp;
     float p0, p1, p2, pxm0, pxm1, pxm2, pym0, pym1, pym2;
-    uint4 h;
+    typename std::conditional<H8, uint2, uint4>::type h;
     float u;
 };
 
-__device__ __forceinline__ void zbundle_load(const PdArgs& a, size_t V, int x, int y, size_t i, ZBundle& b) {
+template <bool H8>
+__device__ __forceinline__ void zbundle_load(const PdArgs& a, size_t V, int x, int y, size_t i, ZBundle<H8>& b) {
     const FusionView& f = a.f;
     const bool xp = x < f.w - 1, yp = y < f.h - 1, xm = x > 0, ym = y > 0;
     b.v0 = a.vin[i];
@@ -403,10 +420,12 @@ __device__ __forceinline__ void zbundle_load(const PdArgs& a, size_t V, int x, i
     b.pym0 = ym ? a.pin[i - f.w] : 0.f;
     b.pym1 = ym ? a.pin[V + i - f.w] : 0.f;
     b.pym2 = ym ? a.pin[2 * V + i - f.w] : 0.f;
-    b.h = *reinterpret_cast<const uint4*>(f.hist + i * FB);
+    if constexpr (H8) b.h = a.hist8[i];
+    else b.h = *reinterpret_cast<const uint4*>(f.hist + i * FB);
     b.u = a.uin[i];
 }
 
+template <bool H8>
 __global__ __launch_bounds__(FZ_BX * FZ_BY) void fusion_pd_zpipe_kernel(PdArgs a) {
     const FusionView& f = a.f;
     const int tx_n = (f.w + FZ_BX - 1) / FZ_BX, ty_n = (f.h + FZ_BY - 1) / FZ_BY;
@@ -426,7 +445,7 @@ __global__ __launch_bounds__(FZ_BX * FZ_BY) void fusion_pd_zpipe_kernel(PdArgs a
         float tx, ty;
         p_new_at(a, V, x, y, z0 - 1, i - sxy, tx, ty, pz_below);
     }
-    ZBundle cur, nxt;
+    ZBundle<H8> cur, nxt;
     zbundle_load(a, V, x, y, i, cur);
     for (int z = z0; z < z1; ++z, i += sxy) {
         const bool zp = z < f.d - 1;
@@ -453,7 +472,9 @@ __global__ __launch_bounds__(FZ_BX * FZ_BY) void fusion_pd_zpipe_kernel(PdArgs a
         pz_below = cz;
         const float un = cur.u;
         const float uu = un - a.tau * (-div);
-        const float unew = prox_hist_h(uu, cur.h, a.tl);
+        float unew;
+        if constexpr (H8) unew = prox_hist_h8(uu, cur.h, a.tl);
+        else unew = prox_hist_h(uu, cur.h, a.tl);
         a.uout[i] = unew;
         a.vout[i] = 2.f * unew - un;
         if (zp) cur = nxt;
@@ -759,6 +780,39 @@ int psw_fusion_update_iteration(const psw_fusion_grid* g, const float* d_depthma
     return psw_fusion_update_p(g, sigma, s);
 }
 
+// Byte-packed copy of the histogram for the PD iterations, and the largest
+// count (atomicMax per block): the copy is used only if that is <= 255.
+__global__ __launch_bounds__(256) void fusion_hist_pack_kernel(const uint16_t* __restrict__ hist,
+                                                               uint2* __restrict__ out, size_t V,
+                                                               unsigned* __restrict__ maxcount) {
+    __shared__ unsigned sm[256 / 64];
+    unsigned m = 0;
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < V; i += (size_t)gridDim.x * 256) {
+        const uint4 h = *reinterpret_cast<const uint4*>(hist + i * FB);
+        const uint32_t w[4] = {h.x, h.y, h.z, h.w};
+        uint32_t lo = 0, hi = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t a = w[k] & 0xffffu, b = w[k] >> 16;
+            m = max(m, max(a, b));
+            const uint32_t two = (a & 0xffu) | ((b & 0xffu) << 8);
+            if (k < 2) lo |= two << (16 * k);
+            else hi |= two << (16 * (k - 2));
+        }
+        out[i] = make_uint2(lo, hi);
+    }
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned b = sm[0];
+        for (int k = 1; k < 256 / 64; ++k) b = max(b, sm[k]);
+        atomicMax(maxcount, b);
+    }
+}
+
+__global__ void fusion_zero_u32_kernel(unsigned* p) { *p = 0u; }
+
 int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double lambda, double sigma, psw_stream_t s) {
     int st = check_grid(g);
     if (st != PSW_OK) return st;
@@ -781,6 +835,7 @@ int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double la
         }
         PdArgs a;
         a.f = f;
+        a.hist8 = nullptr;
         a.tau = (float)tau; a.lambda = (float)lambda; a.tl = a.tau * a.lambda; a.sigma = (float)sigma;
         float* pb[2] = {g->p, p2};
         float* vb[2] = {g->v, v2};
@@ -797,6 +852,24 @@ int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double la
                                 : vox_grid(f, blk);
         const unsigned zm_blocks = (unsigned)(((f.w + FZ_BX - 1) / FZ_BX) * ((f.h + FZ_BY - 1) / FZ_BY) *
                                               ((f.d + FZ_CHUNK - 1) / FZ_CHUNK));
+        // The pipelined kernel reads the counts from a byte-packed copy when
+        // every count fits in a byte (8 B per voxel-iteration less traffic),
+        // else from the grid's own 16-bit histogram.
+        uint2* h8 = nullptr;
+        unsigned* d_max = nullptr;
+        if (mode == 3 && hipMalloc((void**)&h8, V * sizeof(uint2) + 256) == hipSuccess) {
+            d_max = reinterpret_cast<unsigned*>(reinterpret_cast<char*>(h8) + V * sizeof(uint2));
+            unsigned hmax = 0xffffffffu;
+            fusion_zero_u32_kernel<<<1, 1, 0, S(s)>>>(d_max);
+            fusion_hist_pack_kernel<<<(unsigned)std::min<size_t>((V + 255) / 256, 8192), 256, 0, S(s)>>>(
+                f.hist, h8, V, d_max);
+            st = launch_status("fusion_hist_pack_kernel");
+            if (st == PSW_OK &&
+                (hipMemcpyAsync(&hmax, d_max, sizeof(unsigned), hipMemcpyDeviceToHost, S(s)) != hipSuccess ||
+                 hipStreamSynchronize(S(s)) != hipSuccess))
+                st = PSW_ERR_DEVICE;
+            if (st == PSW_OK && hmax <= 255u) a.hist8 = h8;
+        }
         int cur = 0;
         for (int k = 1; k < niters && st == PSW_OK; ++k) {
             a.pin = pb[cur]; a.pout = pb[cur ^ 1];
@@ -806,7 +879,8 @@ int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double la
                 fusion_pd_zlds_kernel<<<grd, blk, 0, S(s)>>>(a);
                 st = launch_status("fusion_pd_zlds_kernel");
             } else if (mode == 3) {
-                fusion_pd_zpipe_kernel<<<zm_blocks, FZ_BX * FZ_BY, 0, S(s)>>>(a);
+                if (a.hist8) fusion_pd_zpipe_kernel<true><<<zm_blocks, FZ_BX * FZ_BY, 0, S(s)>>>(a);
+                else fusion_pd_zpipe_kernel<false><<<zm_blocks, FZ_BX * FZ_BY, 0, S(s)>>>(a);
                 st = launch_status("fusion_pd_zpipe_kernel");
             } else if (zmarch) {
                 fusion_pd_zmarch_kernel<<<zm_blocks, FZ_BX * FZ_BY, 0, S(s)>>>(a);
@@ -825,6 +899,7 @@ int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double la
         (void)hipStreamSynchronize(S(s));
         (void)hipFree(p2);
         (void)hipFree(v2);
+        if (h8) (void)hipFree(h8);
         if (st != PSW_OK) return st;
     }
     return psw_fusion_update_p(g, sigma, s);

@@ -204,7 +204,7 @@ class TgvSolver:
     def get_u(self, stream=None):
         d = DeviceArray((self.H, self.W))
         call("psw_tgv_get_u", self.h, d.ptr, stream)
-        return d.to_host()
+        return d.to_host(stream)
 
     def state_device(self, stream=None):
         """Full state planes (as get_state) in a new device array."""
@@ -217,7 +217,7 @@ class TgvSolver:
         planes = 8 + 3 * self.N + 3 + 1
         d = DeviceArray((planes, self.H, self.W))
         call("psw_tgv_get_state", self.h, d.ptr, stream)
-        s = d.to_host()
+        s = d.to_host(stream)
         N = self.N
         return {"u": s[0], "u1x": s[1], "u1y": s[2], "px": s[3], "py": s[4], "qx": s[5], "qy": s[6], "qz": s[7],
                 "It": s[8:8 + N], "Iu": s[8 + N:8 + 2 * N], "r": s[8 + 2 * N:8 + 3 * N],
@@ -273,7 +273,7 @@ def tgv_sparse(ref, dense, sparse, zfar, niters, alpha0=2.0, alpha1=1.5, tau=0.0
          float(alpha0), float(alpha1), float(tau), float(sigma), float(theta), float(beta), float(gamma),
          workspace.ptr if workspace is not None else None, workspace.nbytes if workspace is not None else 0,
          d_out.ptr, stream)
-    return d_out if dev else d_out.to_host()
+    return d_out if dev else d_out.to_host(stream)
 
 
 def tgv_sparse_workspace_bytes(W, H):
@@ -334,7 +334,7 @@ class Fusion:
             return np.zeros((0, 3), np.float32)
         d = DeviceArray((n, 3))
         call("psw_fusion_surface_points", ctypes.byref(self.g), d.ptr, n, ctypes.byref(c), stream)
-        return d.to_host()
+        return d.to_host(stream)
 
     def surface_count(self, stream=None):
         c = ctypes.c_int64()

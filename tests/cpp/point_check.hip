@@ -8,7 +8,8 @@
 // covers millions of cases without host buffers:
 //   - roots: s > 1 over every binade up to 2^127 (and +inf);
 //   - quotients x / sqrt(s) with |x| from 2^-149 up to sqrt(s), +-0;
-//   - whole dual updates (pq_update) against the formula with sqrtf and '/'.
+//   - whole dual updates (pq_update) against the formula with sqrtf and '/',
+//     and the fusion dual's 3-vector projection.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -57,7 +58,8 @@ __device__ bool pq_plain(float& px, float& py, float& qx, float& qy, float& qz, 
 __device__ __forceinline__ bool same(float a, float b) { return __float_as_uint(a) == __float_as_uint(b); }
 
 // counts: 0 root mismatches, 1 quotient mismatches, 2 quotients on the fast
-// path, 3 dual-update mismatches, 4 dual updates that projected
+// path, 3 dual-update and fusion-projection mismatches, 4 dual updates that
+// projected
 __global__ void check_kernel(uint64_t n, uint64_t seed, unsigned long long* counts) {
     unsigned long long bad_root = 0, bad_div = 0, fast = 0, bad_pq = 0, proj = 0;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
@@ -97,6 +99,12 @@ __global__ void check_kernel(uint64_t n, uint64_t seed, unsigned long long* coun
         proj += pq_plain(ax, ay, bx, by, bz, t11, t12, t22, ubc, ubr, ubd, u1xc, u1xr, u1xd, u1yc, u1yr, u1yd, as1,
                          as0);
         bad_pq += !(same(px, ax) && same(py, ay) && same(qx, bx) && same(qy, by) && same(qz, bz));
+        // the fusion dual step's projection (projectUnitBall, fusion.h:866-870)
+        float fx = r(h3, 0), fy = r(h3, 1), fz = r(h3, 2);
+        const float s3 = fx * fx + fy * fy + fz * fz, d3 = fmaxf(1.f, sqrtf(s3));
+        const float gx = fx / d3, gy = fy / d3, gz = fz / d3;
+        psw::project_ball3(s3, fx, fy, fz);
+        bad_pq += !(same(fx, gx) && same(fy, gy) && same(fz, gz));
     }
     atomicAdd(&counts[0], bad_root);
     atomicAdd(&counts[1], bad_div);

@@ -521,6 +521,39 @@ def test_fusion_solve_kernels_identical(gpu, monkeypatch):
             np.testing.assert_array_equal(out[0][key], o[key])
 
 
+@pytest.mark.parametrize("top", [255, 1000])
+def test_fusion_solve_histogram_packing(gpu, monkeypatch, top):
+    """The pipelined PD kernel reads the counts from a byte-packed copy when
+    they all fit in a byte (top 255) and from the 16-bit histogram otherwise
+    (top 1000); both give the one-voxel-per-thread kernel's grids bit for
+    bit, and u stays within 1e-4 of the FP64 oracle."""
+    from psw_amd import stages
+    rng = np.random.default_rng(top)
+    vol = np.array([-1.0, -1.0, -1.0, 1.0, 1.0, 1.0], np.float32)
+    gw, gh, gd = 70, 21, 35
+    hist = rng.integers(0, 6, (gd, gh, gw, 8)).astype(np.uint16)
+    hot = rng.random((gd, gh, gw, 8)) < 0.01
+    hist[hot] = rng.integers(200, top + 1, int(hot.sum()))
+    print(f"[case] histogram packing, top count {top}, max {int(hist.max())}", flush=True)
+    tau, lam, sig, n = 0.1, 0.5, 1.0, 9
+    out = []
+    for mode in ("0", "3"):
+        monkeypatch.setenv("PSW_FUSION_PD", mode)
+        f = stages.Fusion(gw, gh, gd, vol)
+        f.upload(hist=hist)
+        f.solve(n, tau, lam, sig)
+        out.append(f.download())
+    for key in ("u", "v", "p", "hist"):
+        np.testing.assert_array_equal(out[0][key], out[1][key], err_msg=key)
+    u64, v64, p64 = np.zeros((gd, gh, gw)), np.zeros((gd, gh, gw)), np.zeros((3, gd, gh, gw))
+    for _ in range(n):
+        oracle.fusion_u(u64, v64, p64, hist, tau, lam, prec="f64")
+        oracle.fusion_p(p64, v64, sig, prec="f64")
+    e = rel_err(out[1]["u"], u64)
+    print(f"u vs FP64 max {e.max():.3e}")
+    assert e.max() <= 1e-4
+
+
 # ---- TVL1 ---------------------------------------------------------------------------------
 
 def test_tvl1_vs_oracle(gpu, views_small):
