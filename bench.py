@@ -269,7 +269,7 @@ def probe(names, seed):
 
 
 def run_batch(args):
-    """The uneven batch (configs[2]): 32 seeded frames, resolution from
+    """The uneven batch: 32 seeded frames, resolution from
     {320x240, 640x480, 1242x375}, 4-8 sources, 64-256 planes per frame; sweep
     and TGV2 (plain start, linearised at the sweep depth, to the energy gap)
     for the whole batch in batched launches, against the same frames one at
@@ -313,7 +313,7 @@ def run_batch(args):
         sweep_all()
         linearise_all()
         return batch.tgv_solve_batch(solvers, prm, e_star, args.max_iters, args.check_every, GAP_TOL,
-                                     best["tgv_variant"], True, s)
+                                     best["tgv_variant"], best["use_graph"], s)
 
     batched()                                   # warm-up
     st.synchronize()
@@ -345,7 +345,9 @@ def run_batch(args):
                    "frames": specs, "megapixels": npx / 1e6, "seed": args.seed},
         "autotune": {"chosen": best,
                      "sweep_ms": {f"groups{g}_wg{w}": v for (g, w), v in timings["sweep"].items()},
-                     "tgv_us_per_iteration": {f"variant{v}": ms * 1e3 for v, ms in timings["tgv"].items()}},
+                     "tgv_us_per_iteration": {f"variant{v}": ms * 1e3 for v, ms in timings["tgv"].items()},
+                     "solve_us_per_iteration": {("graph" if g else "direct"): ms * 1e3
+                                                for g, ms in timings["graph"].items()}},
         "iterations": [int(i) for i in iters], "max_rel_gap_at_stop": float(max(gaps)),
         "sequential_per_frame_ms": seq_ms,
     }
@@ -377,8 +379,8 @@ def main():
                                                 "these small workloads (for rocprofv3) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU design is described in DESIGN.md, section 'Sharding')", file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported (a depthmap's sweep, TGV2 and "
+              "fusion each run on one GPU; frames would be spread over GPUs by the caller)", file=sys.stderr)
         sys.exit(2)
     if args.probe:
         probe(args.probe.split(","), args.seed)

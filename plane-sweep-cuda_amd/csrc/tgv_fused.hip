@@ -333,32 +333,39 @@ __device__ inline double block_sum(double v) {
 // result does not depend on NT), then the stop test.
 constexpr int FIN_THREADS = 256;
 
+// Returns (on thread 0) whether this check stopped the solve.  The host
+// mirror is optional (the batched solve counts stops on the device instead).
 template <int NT>
-__device__ void check_finalize(const TgvCheck& c) {
+__device__ bool check_finalize(const TgvCheck& c) {
     double v = 0.0;
     if (threadIdx.x < FIN_THREADS)
         for (int i = threadIdx.x; i < c.fin_count; i += FIN_THREADS) v += c.fin_partials[i];
     const double E = block_sum<NT>(v);
+    bool stopped = false;
     if (threadIdx.x == 0) {
         TgvCtrl* d = c.ctrl;
         volatile TgvCtrl* m = c.mirror;
         d->energy = E;
-        m->energy = E;
+        if (m) m->energy = E;
         if (!d->done && c.e_star > 0.0 && fabs(E - c.e_star) <= c.tol * c.e_star) {
             d->iters_at_done = c.fin_k;
             d->energy_at_done = E;
             d->done = 1;
-            m->iters_at_done = c.fin_k;
-            m->energy_at_done = E;
-            __threadfence_system();
-            m->done = 1;
+            stopped = true;
+            if (m) {
+                m->iters_at_done = c.fin_k;
+                m->energy_at_done = E;
+                __threadfence_system();
+                m->done = 1;
+            }
         }
     }
+    return stopped;
 }
 
 template <int NT>
 __global__ __launch_bounds__(NT) void tgv_check_finalize_kernel(TgvCheck c) {
-    check_finalize<NT>(c);
+    (void)check_finalize<NT>(c);
 }
 
 // Tile TG_TX x TY pixels, NT threads.  Variants (selected by tgv_variant()):
@@ -483,7 +490,7 @@ __device__ __forceinline__ void tgv_iter_tile(const TgvPtrs& a, const int tile, 
     }
 }
 
-// Two iterations (k -> k+1 -> k+2) of a 64x8 tile in one launch.  Each
+// Two iterations (k -> k+1 -> k+2) of a 64 x TY tile in one launch.  Each
 // iteration widens the dependence by one pixel, so iteration k+1 runs on the
 // tile plus a 1-pixel ring (computing p, q on a 2-pixel ring and reading
 // ubar_k on a 2-pixel ring plus the right / lower neighbours), entirely in
@@ -491,13 +498,15 @@ __device__ __forceinline__ void tgv_iter_tile(const TgvPtrs& a, const int tile, 
 // Regions (global offsets from the tile origin x0, y0; positions outside the
 // image read clamped coordinates and are never used by pixels inside it,
 // because left/up neighbours at the first column/row and right/down
-// neighbours at the last are the position itself, as in the reference):
+// neighbours at the last are the position itself, as in the reference),
+// sizes for TY = 8:
 //   A  ubar_k          x0-2 .. x0+65, y0-2 .. y0+9    (68 x 12)
 //   B  p, q_{k+1}      x0-2 .. x0+64, y0-2 .. y0+8    (67 x 11)
 //   C  u_{k+1}, u_k    x0-1 .. x0+64, y0-1 .. y0+8    (66 x 10)
 //   D  p, q_{k+2}      x0-1 .. x0+63, y0-1 .. y0+7    (65 x  9)
 //   tile u, r_{k+2}    x0   .. x0+63, y0   .. y0+7    (64 x  8)
-constexpr int T2_TY = 8, T2_THREADS = 512, T2_MAXN = 16;
+// T2_MAXN: the most sources of the two- and three-iteration kernels.
+constexpr int T2_TY = 10, T2_MAXN = 16;
 
 __device__ __forceinline__ int clampi(int v, int hi) { return min(max(v, 0), hi); }
 
@@ -996,7 +1005,7 @@ __global__ __launch_bounds__(TG_TX * TY) void tgv_iter3_check_kernel(TgvPtrs a, 
     if (a.ctrl->done) return;
     const int ntiles = ((a.W + TG_TX - 1) / TG_TX) * ((a.H + TY - 1) / TY);
     if (FINAL && (int)blockIdx.x == ntiles) {
-        check_finalize<TG_TX * TY>(c);
+        (void)check_finalize<TG_TX * TY>(c);
         return;
     }
     tgv_iter3_tile<NS, ENERGY, TY>(a, xcd_tile(blockIdx.x, ntiles), &c);
@@ -1017,7 +1026,7 @@ __global__ __launch_bounds__(TG_TX * TY) void tgv_iter2_check_kernel(TgvPtrs a, 
     if (a.ctrl->done) return;
     const int ntiles = ((a.W + TG_TX - 1) / TG_TX) * ((a.H + TY - 1) / TY);
     if (FINAL && (int)blockIdx.x == ntiles) {
-        check_finalize<TG_TX * TY>(c);
+        (void)check_finalize<TG_TX * TY>(c);
         return;
     }
     tgv_iter2_tile<NS, ENERGY, TY>(a, xcd_tile(blockIdx.x, ntiles), &c);
@@ -1038,7 +1047,7 @@ __global__ __launch_bounds__(NT) void tgv_iter_check_kernel(TgvPtrs a, TgvCheck 
     if (a.ctrl->done) return;
     const int ntiles = ((a.W + TG_TX - 1) / TG_TX) * ((a.H + TY - 1) / TY);
     if (FINAL && (int)blockIdx.x == ntiles) {
-        check_finalize<NT>(c);
+        (void)check_finalize<NT>(c);
         return;
     }
     tgv_iter_tile<TY, NT, NS, ENERGY>(a, xcd_tile(blockIdx.x, ntiles), &c);
@@ -1367,20 +1376,30 @@ struct TgvFrameDev {
     TgvLayout L;
     int W, H, N, k0;
     TgvCtrl* ctrl;
+    TgvCtrl* mirror;             // the solver's pinned, mapped host copy (device address)
     double* partials;
     int eblocks;
+    int tiles2;                  // tiles of the two-iteration launches (their energy partials)
     float lambda, alpha0, alpha1, tau, sigma;
     double dlambda, dalpha0, dalpha1, e_star;
 };
 
+// The frame of block b: the number of frames whose block range ends at or
+// before b (off[f] = first block of frame f, off[nf] = total).  Each lane
+// tests one frame end and the wave counts them with a ballot, so a new
+// workgroup finds its frame after one load round instead of a dependent
+// binary search.
 __device__ inline int batch_frame(const int* __restrict__ off, int nf, int b) {
-    int lo = 0, hi = nf - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (off[mid] <= b) lo = mid;
-        else hi = mid - 1;
+    const int lane = (int)(threadIdx.x & 63);
+    int f = 0;
+    for (int base = 0; base < nf; base += 64) {
+        const int j = base + lane;
+        const bool past = j < nf && off[j + 1] <= b;
+        const int c = __popcll(__ballot(past));
+        f += c;
+        if (c < 64) break;
     }
-    return lo;
+    return f;
 }
 
 template <int TY, int NT>
@@ -1389,28 +1408,86 @@ __global__ __launch_bounds__(NT) void tgv_iter_batch_kernel(const TgvFrameDev* _
                                                             int wr) {
     const int kb = kd[rd];
     if (blockIdx.x == 0 && threadIdx.x == 0) kd[wr] = kb + 1;
-    const int f = batch_frame(toff, nf, blockIdx.x);
+    const int vb = xcd_tile((int)blockIdx.x, toff[nf]);   // XCD-local tiles
+    const int f = batch_frame(toff, nf, vb);
     const TgvFrameDev& d = fr[f];
     if (d.ctrl->done) return;
     const TgvPtrs a = tgv_ptrs_at(d.base, d.L, d.W, d.H, d.N, d.k0 + kb, d.ctrl, d.lambda, d.alpha0, d.alpha1, d.tau,
                                   d.sigma);
-    tgv_iter_tile<TY, NT, PSW_MAX_SOURCES>(a, (int)blockIdx.x - toff[f]);
+    tgv_iter_tile<TY, NT, PSW_MAX_SOURCES>(a, vb - toff[f]);
 }
 
-// Two iterations of every active frame (64x8 tiles).
-template <int NS>
-__global__ __launch_bounds__(T2_THREADS) void tgv_iter2_batch_kernel(const TgvFrameDev* __restrict__ fr,
-                                                                    const int* __restrict__ toff, int nf, int* kd,
-                                                                    int rd, int wr) {
+__device__ inline TgvCheck batch_check(const TgvFrameDev& d, double tol) {
+    TgvCheck c;
+    c.partials = d.partials;
+    c.fin_partials = d.partials;
+    c.fin_count = d.tiles2;
+    c.fin_k = 0;
+    c.ctrl = d.ctrl;
+    c.mirror = d.mirror;
+    c.mask = d.base + d.L.mask;
+    c.lambda = d.dlambda;
+    c.alpha0 = d.dalpha0;
+    c.alpha1 = d.dalpha1;
+    c.tol = tol;
+    c.e_star = d.e_star;
+    return c;
+}
+
+// The frames of a two-iteration batch launch travel in the kernel argument
+// (up to TG_BATCH_MAXF; the solver groups frames accordingly): the compiler
+// knows pointers read from kernel arguments to be global and emits global
+// loads through them, while pointers read from a table in memory would be
+// generic (FLAT accesses, which also hold up every LDS wait and cost
+// registers), and a workgroup finds its frame without a memory round trip.
+constexpr int TG_BATCH_MAXF = 8;
+
+struct TgvBatchArgs {
+    int nf;
+    int toff[TG_BATCH_MAXF + 1];   // first tile of each frame, toff[nf] = all tiles
+    TgvFrameDev fr[TG_BATCH_MAXF];
+};
+
+// Two iterations of every frame of A (64 x TY tiles).  ENERGY: the launch
+// also sums each frame's energy of its state at the launch into per-tile
+// partials (the frame's partial buffer, tile order, as the single-frame
+// solve's energy launch); FINAL: one extra workgroup per frame finalises the
+// previous launch's partials (the state two iterations back); a stop is
+// also written to the frame's mapped host copy.
+template <int NS, int TY, bool ENERGY, bool FINAL>
+__global__ __launch_bounds__(TG_TX * TY) void tgv_iter2_batch_kernel(const TgvBatchArgs A, int* kd, int rd, int wr,
+                                                                    double tol) {
     const int kb = kd[rd];
     if (blockIdx.x == 0 && threadIdx.x == 0) kd[wr] = kb + 2;
-    const int f = batch_frame(toff, nf, blockIdx.x);
-    const TgvFrameDev& d = fr[f];
+    const int ntiles = A.toff[A.nf];
+    if (FINAL && (int)blockIdx.x >= ntiles) {
+        const TgvFrameDev& d = A.fr[(int)blockIdx.x - ntiles];
+        if (d.ctrl->done) return;
+        TgvCheck c = batch_check(d, tol);
+        c.fin_k = d.k0 + kb - 2;
+        (void)check_finalize<TG_TX * TY>(c);
+        return;
+    }
+    // XCD-local tiles (as the single-frame kernels): each XCD takes a
+    // contiguous range of the launch's tiles, so halo rows stay in its L2
+    const int vb = xcd_tile((int)blockIdx.x, ntiles);
+    int f = 0;
+#pragma unroll
+    for (int j = 1; j < TG_BATCH_MAXF; ++j) f += (j < A.nf && A.toff[j] <= vb) ? 1 : 0;
+    const TgvFrameDev& d = A.fr[f];
     if (d.ctrl->done) return;
     const TgvPtrs a = tgv_ptrs_at(d.base, d.L, d.W, d.H, d.N, d.k0 + kb, d.ctrl, d.lambda, d.alpha0, d.alpha1, d.tau,
                                   d.sigma);
-    tgv_iter2_tile<NS, false, T2_TY>(a, (int)blockIdx.x - toff[f]);
+    if constexpr (ENERGY) {
+        const TgvCheck c = batch_check(d, tol);
+        tgv_iter2_tile<NS, true, TY>(a, vb - A.toff[f], &c);
+    } else {
+        tgv_iter2_tile<NS, false, TY>(a, vb - A.toff[f]);
+    }
 }
+
+// Moves the batch count back to copy 0 at the end of a chunk.
+__global__ void tgv_batch_count_home_kernel(int* kd) { kd[0] = kd[1]; }
 
 // kdev: the copy of the batch counter that holds the current count.
 __global__ __launch_bounds__(TG_THREADS) void tgv_energy_batch_kernel(const TgvFrameDev* __restrict__ fr,
@@ -1428,13 +1505,73 @@ __global__ __launch_bounds__(TG_THREADS) void tgv_energy_batch_kernel(const TgvF
 }
 
 __global__ __launch_bounds__(TG_THREADS) void tgv_finalize_batch_kernel(const TgvFrameDev* __restrict__ fr,
-                                                                     int* kd, int rd, double tol,
-                                                                     int* __restrict__ ndone) {
+                                                                     int* kd, int rd, double tol) {
     const TgvFrameDev& d = fr[blockIdx.x];
     const int kb = kd[rd];
     const bool stopped = tgv_finalize_block(d.partials, d.eblocks, d.ctrl, d.k0 + kb, tol, d.e_star);
-    if (threadIdx.x == 0 && stopped) atomicAdd(ndone, 1);
+    if (threadIdx.x == 0 && stopped) {
+        volatile TgvCtrl* m = d.mirror;
+        m->iters_at_done = d.ctrl->iters_at_done;
+        m->energy_at_done = d.ctrl->energy_at_done;
+        __threadfence_system();
+        m->done = 1;
+    }
     if (rd != 0 && blockIdx.x == 0 && threadIdx.x == 0) kd[0] = kb;   // the next chunk starts from copy 0
+}
+
+// Frames of a batch in groups, solved one group after another.  Batching
+// pays for frames that cannot fill the GPU alone: a launch's tiles run as
+// one round of resident workgroups (two 64x10 tiles per CU), and a small
+// frame's launch leaves most CUs idle, so frames are packed until their
+// tiles fill one round.  Frames that fill a round by themselves run alone:
+// batching them gains nothing, and their state would no longer fit the
+// Infinity Cache together (the iteration kernels stream the whole state
+// every launch and run about 15 % slower per pixel once it does not fit:
+// 29.4 ps per pixel-iteration at 640x480 with 8 sources, 146 MB of state,
+// against 33.7 ps at 1920x1440, 1.3 GB).  A group also stays within
+// PSW_TGV_BATCH_MB of state (default 192 MB of the MI355X's 256 MB; 0 = no
+// limit) and TG_BATCH_MAXF frames.  Largest frames first, each into the
+// first group with room.
+static std::vector<std::vector<int>> tgv_batch_groups(psw_tgv* const* ts, int n) {
+    const char* e = getenv("PSW_TGV_BATCH_MB");
+    double budget = (e ? atof(e) : 192.0) * 1e6;
+    if (budget <= 0.0) budget = 1e300;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        cus = 256;
+    const int round = 2 * cus;                  // resident 64x10 workgroups
+    const int ty = tgv2_ty() == 8 ? 8 : 10;
+    std::vector<std::vector<int>> groups;
+    std::vector<double> bytes((size_t)n, 0.0);
+    std::vector<int> tiles((size_t)n, 0), order((size_t)n);
+    for (int f = 0; f < n; ++f) {
+        order[(size_t)f] = f;
+        if (ts[f]) {
+            bytes[(size_t)f] = (double)ts[f]->L.total * sizeof(float);
+            tiles[(size_t)f] = ((ts[f]->W + TG_TX - 1) / TG_TX) * ((ts[f]->H + ty - 1) / ty);
+        }
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return bytes[(size_t)a] > bytes[(size_t)b]; });
+    std::vector<double> used;
+    std::vector<int> used_tiles;
+    for (int f : order) {
+        size_t g = 0;
+        while (g < groups.size() &&
+               (used[g] + bytes[(size_t)f] > budget || used_tiles[g] + tiles[(size_t)f] > round ||
+                groups[g].size() >= TG_BATCH_MAXF))
+            ++g;
+        if (g == groups.size()) {
+            groups.emplace_back();
+            used.push_back(0.0);
+            used_tiles.push_back(0);
+        }
+        groups[g].push_back(f);
+        used[g] += bytes[(size_t)f];
+        used_tiles[g] += tiles[(size_t)f];
+    }
+    for (auto& g : groups) std::sort(g.begin(), g.end());
+    return groups;
 }
 
 static int tgv_reset_ctrl(psw_tgv* t, hipStream_t st) {
@@ -1815,9 +1952,10 @@ struct TgvBatch {
     TgvFrameDev* frames = nullptr;
     int* toff = nullptr;
     int* eoff = nullptr;
-    int* kdev = nullptr;                // two copies of the batch counter
-    int* ndone = nullptr;
+    int* kdev = nullptr;                // three copies of the batch counter
     int tiles = 0, eblocks = 0, ty = 8, nt = 512;
+    int tiles2 = 0, ty2 = 10;
+    TgvBatchArgs args{};                // the active frames for the two-iteration launches
     bool dbl = false;                   // two iterations per launch
     int nact = 0;                       // frames in the device tables
     std::vector<TgvFrameDev> all;       // every frame of the batch
@@ -1830,15 +1968,25 @@ struct TgvBatch {
     int upload(hipStream_t st) {
         nact = (int)active.size();
         std::vector<TgvFrameDev> fr;
-        std::vector<int> toff(1, 0), eoff(1, 0);
+        std::vector<int> toff(1, 0), eoff(1, 0), toff2(1, 0);
         for (int f : active) {
             const TgvFrameDev& d = all[(size_t)f];
             fr.push_back(d);
             toff.push_back(toff.back() + ((d.W + TG_TX - 1) / TG_TX) * ((d.H + ty - 1) / ty));
             eoff.push_back(eoff.back() + d.eblocks);
+            toff2.push_back(toff2.back() + d.tiles2);
         }
         tiles = toff.back();
         eblocks = eoff.back();
+        tiles2 = toff2.back();
+        if (dbl && nact > TG_BATCH_MAXF) return PSW_ERR_ARG;   // the solver groups frames
+        args = TgvBatchArgs{};
+        args.nf = dbl ? nact : 0;
+        for (int k = 0; dbl && k < nact; ++k) {
+            args.fr[k] = fr[(size_t)k];
+            args.toff[k] = toff2[(size_t)k];
+        }
+        if (dbl) args.toff[nact] = toff2.back();
         if (nact == 0) return PSW_OK;
         PSW_TRY_HIP(hipMemcpyAsync(frames, fr.data(), sizeof(TgvFrameDev) * fr.size(), hipMemcpyHostToDevice, st),
                     PSW_ERR_COPY);
@@ -1859,8 +2007,12 @@ static int tgv_batch_setup(psw_tgv* const* ts, int nf, const psw_tgv_params* prm
     const int var = variant >= 0 && variant <= 2 ? variant : tgv_variant();
     b.dbl = tgv_double();
     for (int f = 0; f < nf; ++f) b.dbl = b.dbl && ts[f] && ts[f]->N <= T2_MAXN;
-    b.ty = var == 2 && !b.dbl ? 4 : 8;    // the two-iteration kernel tiles 64x8
-    b.nt = var == 1 || b.dbl ? 512 : 256;
+    b.ty = var == 2 ? 4 : 8;              // the one-iteration launches
+    b.nt = var == 1 ? 512 : 256;
+    // The two-iteration launches use the single-frame solve's tiles, so the
+    // checking launches' energies (and the stops) are the single-frame ones
+    // bit for bit.
+    b.ty2 = tgv2_ty() == 8 ? 8 : 10;
     b.all.assign((size_t)nf, TgvFrameDev{});
     b.active.clear();
     for (int f = 0; f < nf; ++f) {
@@ -1868,9 +2020,10 @@ static int tgv_batch_setup(psw_tgv* const* ts, int nf, const psw_tgv_params* prm
         PSW_CHECK_ARG(t, "NULL solver in the batch");
         TgvFrameDev& d = b.all[(size_t)f];
         d.base = t->base; d.L = t->L; d.W = t->W; d.H = t->H; d.N = t->N; d.k0 = t->k;
-        d.ctrl = t->ctrl; d.partials = t->partials;
+        d.ctrl = t->ctrl; d.mirror = t->dmirror; d.partials = t->partials;
         int blocks = (int)((t->L.n + TG_THREADS - 1) / TG_THREADS);
         d.eblocks = blocks > t->npartials ? t->npartials : blocks;   // as tgv_launch_energy
+        d.tiles2 = ((t->W + TG_TX - 1) / TG_TX) * ((t->H + b.ty2 - 1) / b.ty2);
         const psw_tgv_params& q = prm[f];
         d.lambda = (float)q.lambda; d.alpha0 = (float)q.alpha0; d.alpha1 = (float)q.alpha1;
         d.tau = (float)q.tau; d.sigma = (float)q.sigma;
@@ -1884,25 +2037,64 @@ static int tgv_batch_setup(psw_tgv* const* ts, int nf, const psw_tgv_params* prm
     b.toff = reinterpret_cast<int*>(b.dev + fb);
     b.eoff = reinterpret_cast<int*>(b.dev + fb + ob);
     b.kdev = reinterpret_cast<int*>(b.dev + fb + 2 * ob);
-    b.ndone = b.kdev + 2;
     int rc = psw_fill_u32(reinterpret_cast<uint32_t*>(b.kdev), 0u, 3, st);
-    for (int f = 0; f < nf && rc == PSW_OK; ++f) rc = tgv_reset_ctrl(ts[f], st);
+    for (int f = 0; f < nf && rc == PSW_OK; ++f) {
+        rc = tgv_reset_ctrl(ts[f], st);
+        std::memset(ts[f]->hmirror, 0, sizeof(TgvCtrl));   // no launch of this solver writes it now
+    }
     if (rc == PSW_OK) rc = b.upload(st);
     return rc;
 }
 
-// c iterations of every frame, starting from counter copy 0; *cur gets the
-// copy that holds the count afterwards.
-static int tgv_batch_iters(const TgvBatch& b, int c, hipStream_t st, int* cur) {
-    const int nf = b.nact;
-    int rd = 0;
+extern "C++" template <int NS, int TY>
+static void tgv_batch_launch2_ns(const TgvBatch& b, int rd, int wr, bool energy, bool fin, double tol,
+                                 hipStream_t st) {
+    constexpr int NT = TG_TX * TY;
+    const unsigned g = (unsigned)(b.tiles2 + (fin ? b.nact : 0));
+    const TgvBatchArgs& A = b.args;
+    if (!energy && !fin) tgv_iter2_batch_kernel<NS, TY, false, false><<<g, NT, 0, st>>>(A, b.kdev, rd, wr, tol);
+    else if (energy && fin) tgv_iter2_batch_kernel<NS, TY, true, true><<<g, NT, 0, st>>>(A, b.kdev, rd, wr, tol);
+    else if (energy) tgv_iter2_batch_kernel<NS, TY, true, false><<<g, NT, 0, st>>>(A, b.kdev, rd, wr, tol);
+    else tgv_iter2_batch_kernel<NS, TY, false, true><<<g, NT, 0, st>>>(A, b.kdev, rd, wr, tol);
+}
+
+// One two-iteration launch over the active frames, reading the count from
+// copy rd and writing it advanced to copy wr (wr != rd, both in 0..2).
+static void tgv_batch_launch2(const TgvBatch& b, int rd, int wr, bool energy, bool fin, double tol,
+                              hipStream_t st) {
     bool all8 = true;
     for (int f : b.active) all8 = all8 && b.all[(size_t)f].N <= 8;
+    if (b.ty2 == 8) {
+        if (all8) tgv_batch_launch2_ns<8, 8>(b, rd, wr, energy, fin, tol, st);
+        else tgv_batch_launch2_ns<T2_MAXN, 8>(b, rd, wr, energy, fin, tol, st);
+    } else {
+        if (all8) tgv_batch_launch2_ns<8, 10>(b, rd, wr, energy, fin, tol, st);
+        else tgv_batch_launch2_ns<T2_MAXN, 10>(b, rd, wr, energy, fin, tol, st);
+    }
+}
+
+// c iterations of every frame, starting from counter copy 0; *cur gets the
+// copy that holds the count afterwards.  With check (c >= 4, two-iteration
+// launches), the first launch also sums each frame's energy of the state it
+// starts from and the second finalises it (the stop test), as in the
+// single-frame solve.
+static int tgv_batch_iters(const TgvBatch& b, int c, hipStream_t st, int* cur, bool check = false,
+                           double tol = 0.0) {
+    const int nf = b.nact;
+    // Launch j reads the count from copy rd and its block 0 writes the
+    // advanced count to copy wr != rd.  The copies go 0 -> 1 -> 2 -> 1 -> 2
+    // ... and the last launch writes copy 0, so that a chunk of two or more
+    // launches ends where it started (a captured chunk replays unchanged).
+    int nl = 0;
     for (int i = 0; i < c;) {
-        const int wr = 1 - rd;
+        i += b.dbl && c - i >= 2 ? 2 : 1;
+        ++nl;
+    }
+    int rd = 0, j = 0;
+    for (int i = 0; i < c; ++j) {
+        const int wr = j == nl - 1 && nl >= 2 ? 0 : (rd == 1 ? 2 : 1);
         if (b.dbl && c - i >= 2) {
-            if (all8) tgv_iter2_batch_kernel<8><<<b.tiles, T2_THREADS, 0, st>>>(b.frames, b.toff, nf, b.kdev, rd, wr);
-            else tgv_iter2_batch_kernel<T2_MAXN><<<b.tiles, T2_THREADS, 0, st>>>(b.frames, b.toff, nf, b.kdev, rd, wr);
+            tgv_batch_launch2(b, rd, wr, check && i == 0, check && i == 2, tol, st);
             i += 2;
         } else {
             if (b.ty == 4)
@@ -1919,19 +2111,29 @@ static int tgv_batch_iters(const TgvBatch& b, int c, hipStream_t st, int* cur) {
     return launch_status("tgv_iter_batch_kernel");
 }
 
+// A chunk that leaves the count in copy 0 (the next chunk, or a replay of a
+// captured chunk, reads it there).
+static int tgv_batch_chunk(const TgvBatch& b, int c, hipStream_t st, bool check, double tol) {
+    int cur = 0;
+    int rc = tgv_batch_iters(b, c, st, &cur, check, tol);
+    if (rc == PSW_OK && cur != 0) {
+        tgv_batch_count_home_kernel<<<1, 1, 0, st>>>(b.kdev);
+        rc = launch_status("tgv_batch_count_home_kernel");
+    }
+    return rc;
+}
+
 // The energy test of every active frame at the count in copy cur; the
 // finaliser moves the count to copy 0.
 static int tgv_batch_check(const TgvBatch& b, double tol, int cur, hipStream_t st) {
     const int nf = b.nact;
     tgv_energy_batch_kernel<<<b.eblocks, TG_THREADS, 0, st>>>(b.frames, b.eoff, nf, b.kdev + cur);
-    tgv_finalize_batch_kernel<<<nf, TG_THREADS, 0, st>>>(b.frames, b.kdev, cur, tol, b.ndone);
+    tgv_finalize_batch_kernel<<<nf, TG_THREADS, 0, st>>>(b.frames, b.kdev, cur, tol);
     return launch_status("tgv_energy_batch_kernel");
 }
 
-int psw_tgv_iterate_batch(psw_tgv* const* solvers, int nsolvers, int niters, const psw_tgv_params* prm, int variant,
-                          psw_stream_t s) {
-    PSW_CHECK_ARG(niters >= 0, "niters must be >= 0");
-    hipStream_t st = S(s);
+static int tgv_iterate_batch_group(psw_tgv* const* solvers, int nsolvers, int niters, const psw_tgv_params* prm,
+                                   int variant, hipStream_t st) {
     TgvBatch b;
     int rc = tgv_batch_setup(solvers, nsolvers, prm, nullptr, variant, b, st);
     int cur = 0;
@@ -1942,18 +2144,19 @@ int psw_tgv_iterate_batch(psw_tgv* const* solvers, int nsolvers, int niters, con
     return PSW_OK;
 }
 
-int psw_tgv_solve_batch(psw_tgv* const* solvers, int nsolvers, const psw_tgv_params* prm, const double* e_star,
-                        int max_iters, int check_every, double tol, int variant, int use_graph, int* iters_done,
-                        double* energy_out, psw_stream_t s) {
-    PSW_CHECK_ARG(e_star, "NULL e_star");
-    PSW_CHECK_ARG(max_iters >= 0 && check_every >= 1, "bad iteration counts");
-    hipStream_t st = S(s);
+static int tgv_solve_batch_group(psw_tgv* const* solvers, int nsolvers, const psw_tgv_params* prm,
+                                 const double* e_star, int max_iters, int check_every, double tol, int variant,
+                                 int use_graph, int* iters_done, double* energy_out, hipStream_t st) {
     TgvBatch b;
     int rc = tgv_batch_setup(solvers, nsolvers, prm, e_star, variant, b, st);
     if (rc != PSW_OK) return rc;
-    int* h_ndone = nullptr;
-    PSW_TRY_HIP(hipHostMalloc((void**)&h_ndone, sizeof(int), hipHostMallocDefault), PSW_ERR_ALLOC);
-    *h_ndone = 0;
+    // frames stopped so far, from the solvers' mapped host copies (written by
+    // the finalising workgroups; no copy in the stream)
+    auto ndone = [&]() {
+        int n = 0;
+        for (int f = 0; f < nsolvers; ++f) n += ((volatile const TgvCtrl*)solvers[f]->hmirror)->done != 0;
+        return n;
+    };
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -1962,16 +2165,27 @@ int psw_tgv_solve_batch(psw_tgv* const* solvers, int nsolvers, const psw_tgv_par
     if (e != hipSuccess) rc = hip_status(e, "hipEventCreate");
     const int full = max_iters / check_every;
     const bool graphs = use_graph && full > 0 && st != nullptr;   // the null stream cannot be captured
-    // one chunk: check_every iterations, the check, the stopped count to the host
+    // In-launch checks (two-iteration launches, chunks of at least 4): a
+    // chunk's first launch sums the energy of the state the chunk starts
+    // from and its second launch applies the stop test, as in the
+    // single-frame solve (same tiles, so the same energies and stops); the
+    // state after the last chunk is tested by the stand-alone pass at the end.
+    // Otherwise each chunk ends with the stand-alone energy pass and test.
+    const bool inl = b.dbl && check_every >= 4;
+    // one chunk: check_every iterations and the check
     auto capture = [&]() -> int {
         if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
         if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
         hipError_t ce = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
         if (ce != hipSuccess) return hip_status(ce, "hipStreamBeginCapture");
         int cur = 0;
-        int crc = tgv_batch_iters(b, check_every, st, &cur);
-        if (crc == PSW_OK) crc = tgv_batch_check(b, tol, cur, st);
-        if (crc == PSW_OK) ce = hipMemcpyAsync(h_ndone, b.ndone, sizeof(int), hipMemcpyDeviceToHost, st);
+        int crc;
+        if (inl) {
+            crc = tgv_batch_chunk(b, check_every, st, true, tol);
+        } else {
+            crc = tgv_batch_iters(b, check_every, st, &cur);
+            if (crc == PSW_OK) crc = tgv_batch_check(b, tol, cur, st);
+        }
         const hipError_t ee = hipStreamEndCapture(st, &graph);
         if (ce == hipSuccess) ce = ee;
         if (ce == hipSuccess && crc == PSW_OK) ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -1982,54 +2196,64 @@ int psw_tgv_solve_batch(psw_tgv* const* solvers, int nsolvers, const psw_tgv_par
     int launched = 0, chunk = 0, seen_done = 0;
     while (rc == PSW_OK && launched < max_iters) {
         const int c = std::min(check_every, max_iters - launched);
-        if (exec && c == check_every) {
+        if (exec && c == check_every && (!inl || launched > 0)) {
             e = hipGraphLaunch(exec, st);
             if (e != hipSuccess) { rc = hip_status(e, "hipGraphLaunch"); break; }
         } else {
-            int cur = 0;
-            rc = tgv_batch_iters(b, c, st, &cur);
-            if (rc == PSW_OK) rc = tgv_batch_check(b, tol, cur, st);
+            if (inl && launched == 0) {
+                rc = tgv_batch_chunk(b, c, st, false, tol);        // nothing to test yet
+            } else if (inl && c >= 4) {
+                rc = tgv_batch_chunk(b, c, st, true, tol);
+            } else if (inl) {
+                // a short last chunk: the state it starts from is tested on its own
+                rc = tgv_batch_check(b, tol, 0, st);
+                if (rc == PSW_OK) rc = tgv_batch_chunk(b, c, st, false, tol);
+            } else {
+                int cur = 0;
+                rc = tgv_batch_iters(b, c, st, &cur);
+                if (rc == PSW_OK) rc = tgv_batch_check(b, tol, cur, st);
+            }
             if (rc != PSW_OK) break;
-            e = hipMemcpyAsync(h_ndone, b.ndone, sizeof(int), hipMemcpyDeviceToHost, st);
-            if (e != hipSuccess) { rc = hip_status(e, "hipMemcpyAsync"); break; }
         }
         launched += c;
-        e = hipEventRecord(ev[chunk & 1], st);
-        if (e != hipSuccess) { rc = hip_status(e, "hipEventRecord"); break; }
-        // one chunk in flight: wait for the previous chunk's count (a later
-        // chunk may already have raised it; the count only grows)
-        if (chunk >= 1) {
-            if (hipEventSynchronize(ev[(chunk - 1) & 1]) != hipSuccess) { rc = PSW_ERR_DEVICE; break; }
-            if (*h_ndone >= nsolvers) break;
-            if (*h_ndone > seen_done) {
+        // two chunks in flight: the event slot holds the chunk two back; wait
+        // for it and read the stopped frames (a later chunk may already have
+        // stopped more; a stop is never undone), then re-arm the slot
+        hipEvent_t slot = ev[chunk & 1];
+        if (chunk >= 2) {
+            if (hipEventSynchronize(slot) != hipSuccess) { rc = PSW_ERR_DEVICE; break; }
+            const int nd = ndone();
+            if (nd >= nsolvers) break;
+            if (nd > seen_done) {
                 // frames stopped: let the queued chunk finish, drop them from
                 // the tables (their launches would only return) and recapture
                 if (hipStreamSynchronize(st) != hipSuccess) { rc = PSW_ERR_DEVICE; break; }
-                seen_done = *h_ndone;
+                seen_done = ndone();
                 if (seen_done >= nsolvers) break;
                 std::vector<int> still;
-                for (int f : b.active) {
-                    TgvCtrl c2;
-                    if (hipMemcpy(&c2, solvers[f]->ctrl, sizeof(TgvCtrl), hipMemcpyDeviceToHost) != hipSuccess) {
-                        rc = PSW_ERR_COPY;
-                        break;
-                    }
-                    if (!c2.done) still.push_back(f);
-                }
-                if (rc != PSW_OK || still.empty()) break;
+                for (int f : b.active)
+                    if (!((volatile const TgvCtrl*)solvers[f]->hmirror)->done) still.push_back(f);
+                if (still.empty()) break;
                 b.active.swap(still);
                 rc = b.upload(st);
                 if (rc == PSW_OK && graphs) rc = capture();
             }
         }
+        if (rc != PSW_OK) break;
+        e = hipEventRecord(slot, st);
+        if (e != hipSuccess) { rc = hip_status(e, "hipEventRecord"); break; }
         chunk++;
     }
     (void)hipStreamSynchronize(st);
+    if (rc == PSW_OK && inl && ndone() < nsolvers && !b.active.empty()) {
+        // the state after the last chunk (count in copy 0)
+        rc = tgv_batch_check(b, tol, 0, st);
+        (void)hipStreamSynchronize(st);
+    }
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (int k = 0; k < 2; ++k)
         if (ev[k]) (void)hipEventDestroy(ev[k]);
-    (void)hipHostFree(h_ndone);
     if (rc != PSW_OK) return rc;
     for (int f = 0; f < nsolvers; ++f) {
         psw_tgv* t = solvers[f];
@@ -2042,6 +2266,71 @@ int psw_tgv_solve_batch(psw_tgv* const* solvers, int nsolvers, const psw_tgv_par
         if (rc != PSW_OK) return rc;
     }
     PSW_TRY_HIP(hipStreamSynchronize(st), PSW_ERR_DEVICE);
+    return PSW_OK;
+}
+
+int psw_tgv_iterate_batch(psw_tgv* const* solvers, int nsolvers, int niters, const psw_tgv_params* prm, int variant,
+                          psw_stream_t s) {
+    PSW_CHECK_ARG(niters >= 0, "niters must be >= 0");
+    PSW_CHECK_ARG(solvers && prm && nsolvers >= 1, "NULL pointer or no frames");
+    for (const auto& g : tgv_batch_groups(solvers, nsolvers)) {
+        int rc;
+        if (g.size() == 1) {
+            // a group of one frame is the single-frame iteration
+            PSW_CHECK_ARG(solvers[g[0]], "NULL solver in the batch");
+            rc = psw_tgv_iterate(solvers[g[0]], niters, &prm[g[0]], s);
+            if (rc == PSW_OK) PSW_TRY_HIP(hipStreamSynchronize(S(s)), PSW_ERR_DEVICE);
+        } else {
+            std::vector<psw_tgv*> ts;
+            std::vector<psw_tgv_params> ps;
+            for (int f : g) {
+                ts.push_back(solvers[f]);
+                ps.push_back(prm[f]);
+            }
+            rc = tgv_iterate_batch_group(ts.data(), (int)ts.size(), niters, ps.data(), variant, S(s));
+        }
+        if (rc != PSW_OK) return rc;
+    }
+    return PSW_OK;
+}
+
+int psw_tgv_solve_batch(psw_tgv* const* solvers, int nsolvers, const psw_tgv_params* prm, const double* e_star,
+                        int max_iters, int check_every, double tol, int variant, int use_graph, int* iters_done,
+                        double* energy_out, psw_stream_t s) {
+    PSW_CHECK_ARG(e_star, "NULL e_star");
+    PSW_CHECK_ARG(max_iters >= 0 && check_every >= 1, "bad iteration counts");
+    PSW_CHECK_ARG(solvers && prm && nsolvers >= 1, "NULL pointer or no frames");
+    for (const auto& g : tgv_batch_groups(solvers, nsolvers)) {
+        if (g.size() == 1) {
+            // a group of one frame is the single-frame solve (same stops and states)
+            const int f = g[0];
+            PSW_CHECK_ARG(solvers[f], "NULL solver in the batch");
+            int it = 0;
+            double en = 0.0;
+            const int rc = psw_tgv_solve(solvers[f], max_iters, check_every, tol, e_star[f], &prm[f], &it, &en, s);
+            if (rc != PSW_OK) return rc;
+            if (iters_done) iters_done[f] = it;
+            if (energy_out) energy_out[f] = en;
+            continue;
+        }
+        std::vector<psw_tgv*> ts;
+        std::vector<psw_tgv_params> ps;
+        std::vector<double> es;
+        for (int f : g) {
+            ts.push_back(solvers[f]);
+            ps.push_back(prm[f]);
+            es.push_back(e_star[f]);
+        }
+        std::vector<int> it(g.size(), 0);
+        std::vector<double> en(g.size(), 0.0);
+        const int rc = tgv_solve_batch_group(ts.data(), (int)ts.size(), ps.data(), es.data(), max_iters, check_every,
+                                             tol, variant, use_graph, it.data(), en.data(), S(s));
+        if (rc != PSW_OK) return rc;
+        for (size_t k = 0; k < g.size(); ++k) {
+            if (iters_done) iters_done[g[k]] = it[k];
+            if (energy_out) energy_out[g[k]] = en[k];
+        }
+    }
     return PSW_OK;
 }
 

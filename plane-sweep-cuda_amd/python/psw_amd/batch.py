@@ -107,11 +107,13 @@ def _time(fn, stream, reps=3):
     return ev0.elapsed_ms(ev1) / reps
 
 
-def autotune(frames, prm, fast=True, tgv_iters=20, stream=None, log=None):
+def autotune(frames, prm, fast=True, tgv_iters=20, solve_iters=200, stream=None, log=None):
     """Time the launch shapes of the batched sweep (plane groups x resident
-    workgroups) and of the batched TGV2 iteration (tile shape) on these
-    frames; returns the fastest settings and the timings.  Results do not
-    depend on the choice."""
+    workgroups), of the batched TGV2 iteration (tile shape of the
+    one-iteration launches) and the batched solve's chunks replayed from a
+    captured hipGraph or launched directly, on these frames; returns the
+    fastest settings and the timings.  Results do not depend on the choice.
+    Leaves the solvers iterated: linearise again before using them."""
     st = stream or Stream()
     s = st.handle
     sweeps = [f.sweep for f in frames]
@@ -132,4 +134,14 @@ def autotune(frames, prm, fast=True, tgv_iters=20, stream=None, log=None):
             log(f"[autotune] tgv variant {var}: {ms * 1e3:.1f} us per iteration")
     best_sweep = min(timings["sweep"], key=timings["sweep"].get)
     best_var = min(timings["tgv"], key=timings["tgv"].get)
-    return {"sweep_groups": best_sweep[0], "sweep_workgroups": best_sweep[1], "tgv_variant": best_var}, timings
+    # the solve's chunks: hipGraph replay or direct launches (E* = 0: no stop)
+    timings["graph"] = {}
+    zero = [0.0] * len(solvers)
+    for g in (False, True):
+        ms = _time(lambda: tgv_solve_batch(solvers, prm, zero, solve_iters, 10, 1e-4, best_var, g, s), st) / solve_iters
+        timings["graph"][g] = ms
+        if log:
+            log(f"[autotune] tgv solve chunks {'graph' if g else 'direct'}: {ms * 1e3:.1f} us per iteration")
+    best_graph = min(timings["graph"], key=timings["graph"].get)
+    return {"sweep_groups": best_sweep[0], "sweep_workgroups": best_sweep[1], "tgv_variant": best_var,
+            "use_graph": bool(best_graph)}, timings

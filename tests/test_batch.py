@@ -46,8 +46,11 @@ def _linearised(frames, prm):
         f.linearise(prm)
 
 
-def test_tgv_iterate_batch_matches_single(gpu):
+@pytest.mark.parametrize("batch_mb", ["192", "3"])
+def test_tgv_iterate_batch_matches_single(gpu, monkeypatch, batch_mb):
+    """Also with the frames split into several cache-sized groups (3 MB)."""
     from psw_amd import batch, stages
+    monkeypatch.setenv("PSW_TGV_BATCH_MB", batch_mb)
     prm = stages.tgv_params(**TGV_KW)
     a, b = _frames(), _frames()
     _linearised(a, prm)
@@ -61,12 +64,14 @@ def test_tgv_iterate_batch_matches_single(gpu):
             np.testing.assert_array_equal(sa[k], sb[k], err_msg=k)
 
 
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_tgv_solve_batch_matches_single(gpu, use_graph):
+@pytest.mark.parametrize("use_graph,batch_mb", [(False, "192"), (True, "192"), (True, "3")])
+def test_tgv_solve_batch_matches_single(gpu, monkeypatch, use_graph, batch_mb):
     """Per-frame E* from a long run; the batched solve stops every frame at
     the same iteration with the same state as the single solve; replaying
-    the captured graph twice, with eager work in between, repeats it."""
+    the captured graph twice, with eager work in between, repeats it.  With
+    batch_mb 3 the frames are solved in several cache-sized groups."""
     from psw_amd import batch, stages
+    monkeypatch.setenv("PSW_TGV_BATCH_MB", batch_mb)
     from psw_amd.device import Stream
     from psw_amd.device import synchronize as device_synchronize
     st = Stream()
