@@ -425,9 +425,15 @@ __device__ __forceinline__ void zbundle_load(const PdArgs& a, size_t V, int x, i
     b.u = a.uin[i];
 }
 
+// Blocks of the pipelined z-march: 64 x FZP_BY columns (the y-halo rows a
+// block reads, v at y-1 and y+1 and p at y-1, are a smaller share of its
+// loads than with the 64 x 4 blocks of the unpipelined kernel).
+constexpr int FZP_BY = 8, FZP_CHUNK = 16;
+
 template <bool H8>
-__global__ __launch_bounds__(FZ_BX * FZ_BY) void fusion_pd_zpipe_kernel(PdArgs a) {
+__global__ __launch_bounds__(FZ_BX * FZP_BY) void fusion_pd_zpipe_kernel(PdArgs a) {
     const FusionView& f = a.f;
+    constexpr int FZ_BY = FZP_BY, FZ_CHUNK = FZP_CHUNK;
     const int tx_n = (f.w + FZ_BX - 1) / FZ_BX, ty_n = (f.h + FZ_BY - 1) / FZ_BY;
     const int ntiles = tx_n * ty_n * ((f.d + FZ_CHUNK - 1) / FZ_CHUNK);
     const int t = xcd_tile(blockIdx.x, ntiles);
@@ -852,6 +858,8 @@ int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double la
                                 : vox_grid(f, blk);
         const unsigned zm_blocks = (unsigned)(((f.w + FZ_BX - 1) / FZ_BX) * ((f.h + FZ_BY - 1) / FZ_BY) *
                                               ((f.d + FZ_CHUNK - 1) / FZ_CHUNK));
+        const unsigned zp_blocks = (unsigned)(((f.w + FZ_BX - 1) / FZ_BX) * ((f.h + FZP_BY - 1) / FZP_BY) *
+                                              ((f.d + FZP_CHUNK - 1) / FZP_CHUNK));
         // The pipelined kernel reads the counts from a byte-packed copy when
         // every count fits in a byte (8 B per voxel-iteration less traffic),
         // else from the grid's own 16-bit histogram.
@@ -879,8 +887,8 @@ int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double la
                 fusion_pd_zlds_kernel<<<grd, blk, 0, S(s)>>>(a);
                 st = launch_status("fusion_pd_zlds_kernel");
             } else if (mode == 3) {
-                if (a.hist8) fusion_pd_zpipe_kernel<true><<<zm_blocks, FZ_BX * FZ_BY, 0, S(s)>>>(a);
-                else fusion_pd_zpipe_kernel<false><<<zm_blocks, FZ_BX * FZ_BY, 0, S(s)>>>(a);
+                if (a.hist8) fusion_pd_zpipe_kernel<true><<<zp_blocks, FZ_BX * FZP_BY, 0, S(s)>>>(a);
+                else fusion_pd_zpipe_kernel<false><<<zp_blocks, FZ_BX * FZP_BY, 0, S(s)>>>(a);
                 st = launch_status("fusion_pd_zpipe_kernel");
             } else if (zmarch) {
                 fusion_pd_zmarch_kernel<<<zm_blocks, FZ_BX * FZ_BY, 0, S(s)>>>(a);
