@@ -78,6 +78,22 @@ __global__ void tgv_sparse_setup_kernel(float* __restrict__ T, float* __restrict
     Q[2 * n + i] = 0.f;
 }
 
+// TGV2_updateU_sparseDepthTensor at one pixel (both fused kernels, same
+// expression order): c = own duals, l / u = left / upper neighbour's (own at
+// the first column / row); sp = the caller's sparse depth there.
+__device__ __forceinline__ void sparse_u_step(const SparsePtrs& a, float u, float u1x, float u1y, float t11, float t12,
+                                              float t22, float c_px, float c_py, float l_px, float l_py, float u_px,
+                                              float u_py, float q0c, float q0l, float q1c, float q1u, float q2c,
+                                              float q2l, float q2u, float sp, float& un, float& u1xn, float& u1yn) {
+    const float wv = sp > 0.f ? 1.f : 0.f;
+    const float ds = sp * a.sz;
+    un = (u + a.tau * (a.alpha1 * (t11 * (c_px - l_px) + t12 * (c_py - l_py) + t12 * (c_px - u_px) + t22 * (c_py - u_py)) +
+                       wv * ds)) /
+         (1.f + a.tau * wv);
+    u1xn = u1x - a.tau * (-a.alpha1 * (t11 * c_px + t12 * c_py) - a.alpha0 * (q0c - q0l + q2c - q2u));
+    u1yn = u1y - a.tau * (-a.alpha1 * (t12 * c_px + t22 * c_py) - a.alpha0 * (q2c - q2l + q1c - q1u));
+}
+
 // Tile 64 x TY pixels, NT threads (variants chosen by sparse_variant()).
 template <int TS_TY, int TS_THREADS>
 __global__ __launch_bounds__(TS_THREADS) void tgv_sparse_iter_kernel(SparsePtrs a) {
@@ -143,22 +159,136 @@ __global__ __launch_bounds__(TS_THREADS) void tgv_sparse_iter_kernel(SparsePtrs 
         const int cx = tx + 1, cy = ty + 1;       // dual (and u) index of this pixel
         const int lx = x > 0 ? cx - 1 : cx;       // max(x-1, 0)
         const int uy = y > 0 ? cy - 1 : cy;       // max(y-1, 0)
-        const float u = sU[0][cy][cx], u1x = sU[1][cy][cx], u1y = sU[2][cy][cx];
-        const float t11 = a.T[g], t12 = a.T[n + g], t22 = a.T[2 * n + g];
-        const float c_px = sP[0][cy][cx], c_py = sP[1][cy][cx];
-        const float xp_px = sP[0][cy][lx], xp_py = sP[1][cy][lx];
-        const float yp_px = sP[0][uy][cx], yp_py = sP[1][uy][cx];
-        const float sp = a.sparse[g];
-        const float wv = sp > 0.f ? 1.f : 0.f;
-        const float ds = sp * a.sz;
-        const float un = (u + a.tau * (a.alpha1 * (t11 * (c_px - xp_px) + t12 * (c_py - xp_py) +
-                                                   t12 * (c_px - yp_px) + t22 * (c_py - yp_py)) +
-                                       wv * ds)) /
-                         (1.f + a.tau * wv);
-        const float u1xn = u1x - a.tau * (-a.alpha1 * (t11 * c_px + t12 * c_py) -
-                                          a.alpha0 * (sQ[0][cy][cx] - sQ[0][cy][lx] + sQ[2][cy][cx] - sQ[2][uy][cx]));
-        const float u1yn = u1y - a.tau * (-a.alpha1 * (t12 * c_px + t22 * c_py) -
-                                          a.alpha0 * (sQ[2][cy][cx] - sQ[2][cy][lx] + sQ[1][cy][cx] - sQ[1][uy][cx]));
+        float un, u1xn, u1yn;
+        sparse_u_step(a, sU[0][cy][cx], sU[1][cy][cx], sU[2][cy][cx], a.T[g], a.T[n + g], a.T[2 * n + g],
+                      sP[0][cy][cx], sP[1][cy][cx], sP[0][cy][lx], sP[1][cy][lx], sP[0][uy][cx], sP[1][uy][cx],
+                      sQ[0][cy][cx], sQ[0][cy][lx], sQ[1][cy][cx], sQ[1][uy][cx], sQ[2][cy][cx], sQ[2][cy][lx],
+                      sQ[2][uy][cx], a.sparse[g], un, u1xn, u1yn);
+        a.unext[g] = un;
+        a.unext[n + g] = u1xn;
+        a.unext[2 * n + g] = u1yn;
+    }
+}
+
+// Two iterations (k -> k+2) of a 64 x TY tile in one launch.  Iteration k+1
+// runs on the tile plus a one-pixel ring, entirely in LDS (the iteration has
+// no extrapolated primal: P and Q read u_k itself); only u, p, q of k+2
+// reach HBM, 80 B per pixel for both iterations.  Regions (offsets from the
+// tile origin; positions outside the image read clamped coordinates and
+// never feed positions inside it, whose right / lower and left / upper
+// neighbours at the image border are the position itself):
+//   A  u_k, then u_{k+1} on C in place      -2 .. TX+1   (TX+4)
+//   B  T; p, q of k+1, then of k+2 on D     -2 .. TX     (TX+3)
+//   C  u_{k+1}                              -1 .. TX
+//   D  p, q of k+2                          -1 .. TX-1
+// LDS index = offset + 2 in every array.  Each position's arithmetic is
+// the one-iteration kernel's (pq_update, sparse_u_step), so the results
+// are identical.
+template <int TY, int NT>
+__global__ __launch_bounds__(NT) void tgv_sparse_iter2_kernel(SparsePtrs a) {
+    constexpr int TX = TS_TX;
+    constexpr int AX = TX + 4, AY = TY + 4, BX = TX + 3, BY = TY + 3;
+    __shared__ float sU[3][AY][AX];
+    __shared__ float sP[2][BY][BX];
+    __shared__ float sQ[3][BY][BX];
+    __shared__ float sT[3][BY][BX];
+
+    const int W = a.W, H = a.H;
+    const size_t n = (size_t)W * H;
+    const int tiles_x = (W + TX - 1) / TX;
+    const int ntiles = tiles_x * ((H + TY - 1) / TY);
+    const int tile = xcd_tile(blockIdx.x, ntiles);
+    const int x0 = (tile % tiles_x) * TX, y0 = (tile / tiles_x) * TY;
+    const int tid = threadIdx.x;
+    auto gidx = [&](int ix, int iy) {   // clamped global index of LDS position (ix, iy)
+        return (size_t)min(max(y0 - 2 + iy, 0), H - 1) * W + min(max(x0 - 2 + ix, 0), W - 1);
+    };
+
+    for (int i = tid; i < AX * AY; i += NT) {
+        const int iy = i / AX, ix = i - iy * AX;
+        const size_t g = gidx(ix, iy);
+#pragma unroll
+        for (int f = 0; f < 3; ++f) sU[f][iy][ix] = a.ucur[f * n + g];
+    }
+    for (int i = tid; i < BX * BY; i += NT) {
+        const int iy = i / BX, ix = i - iy * BX;
+        const size_t g = gidx(ix, iy);
+        sP[0][iy][ix] = a.pcur[g];
+        sP[1][iy][ix] = a.pcur[n + g];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            sQ[f][iy][ix] = a.qcur[f * n + g];
+            sT[f][iy][ix] = a.T[f * n + g];
+        }
+    }
+    __syncthreads();
+
+    const float as1 = a.alpha1 * a.sigma, as0 = a.alpha0 * a.sigma;
+    // p, q of k+1 on B: [0, BX) x [0, BY)
+    for (int i = tid; i < BX * BY; i += NT) {
+        const int iy = i / BX, ix = i - iy * BX;
+        const int gx = x0 - 2 + ix, gy = y0 - 2 + iy;
+        const int rx = gx < W - 1 ? ix + 1 : ix, dy = gy < H - 1 ? iy + 1 : iy;
+        float px = sP[0][iy][ix], py = sP[1][iy][ix];
+        float qx = sQ[0][iy][ix], qy = sQ[1][iy][ix], qz = sQ[2][iy][ix];
+        pq_update(px, py, qx, qy, qz, sT[0][iy][ix], sT[1][iy][ix], sT[2][iy][ix], sU[0][iy][ix], sU[0][iy][rx],
+                  sU[0][dy][ix], sU[1][iy][ix], sU[1][iy][rx], sU[1][dy][ix], sU[2][iy][ix], sU[2][iy][rx],
+                  sU[2][dy][ix], as1, as0);
+        sP[0][iy][ix] = px; sP[1][iy][ix] = py;
+        sQ[0][iy][ix] = qx; sQ[1][iy][ix] = qy; sQ[2][iy][ix] = qz;
+    }
+    __syncthreads();
+
+    // u of k+1 on C: [1, TX+3) x [1, TY+3), in place (a position reads only
+    // its own u)
+    constexpr int CX = TX + 2, CY = TY + 2;
+    for (int i = tid; i < CX * CY; i += NT) {
+        const int iy = i / CX + 1, ix = i - (i / CX) * CX + 1;
+        const int gx = x0 - 2 + ix, gy = y0 - 2 + iy;
+        const int lx = gx > 0 ? ix - 1 : ix, uy = gy > 0 ? iy - 1 : iy;
+        float un, u1xn, u1yn;
+        sparse_u_step(a, sU[0][iy][ix], sU[1][iy][ix], sU[2][iy][ix], sT[0][iy][ix], sT[1][iy][ix], sT[2][iy][ix],
+                      sP[0][iy][ix], sP[1][iy][ix], sP[0][iy][lx], sP[1][iy][lx], sP[0][uy][ix], sP[1][uy][ix],
+                      sQ[0][iy][ix], sQ[0][iy][lx], sQ[1][iy][ix], sQ[1][uy][ix], sQ[2][iy][ix], sQ[2][iy][lx],
+                      sQ[2][uy][ix], a.sparse[gidx(ix, iy)], un, u1xn, u1yn);
+        sU[0][iy][ix] = un; sU[1][iy][ix] = u1xn; sU[2][iy][ix] = u1yn;
+    }
+    __syncthreads();
+
+    // p, q of k+2 on D: [1, TX+2) x [1, TY+2), in place; the tile's to HBM
+    constexpr int DX = TX + 1, DY = TY + 1;
+    for (int i = tid; i < DX * DY; i += NT) {
+        const int iy = i / DX + 1, ix = i - (i / DX) * DX + 1;
+        const int gx = x0 - 2 + ix, gy = y0 - 2 + iy;
+        const int rx = gx < W - 1 ? ix + 1 : ix, dy = gy < H - 1 ? iy + 1 : iy;
+        float px = sP[0][iy][ix], py = sP[1][iy][ix];
+        float qx = sQ[0][iy][ix], qy = sQ[1][iy][ix], qz = sQ[2][iy][ix];
+        pq_update(px, py, qx, qy, qz, sT[0][iy][ix], sT[1][iy][ix], sT[2][iy][ix], sU[0][iy][ix], sU[0][iy][rx],
+                  sU[0][dy][ix], sU[1][iy][ix], sU[1][iy][rx], sU[1][dy][ix], sU[2][iy][ix], sU[2][iy][rx],
+                  sU[2][dy][ix], as1, as0);
+        sP[0][iy][ix] = px; sP[1][iy][ix] = py;
+        sQ[0][iy][ix] = qx; sQ[1][iy][ix] = qy; sQ[2][iy][ix] = qz;
+        if (ix >= 2 && iy >= 2 && gx < W && gy < H) {
+            const size_t g = (size_t)gy * W + gx;
+            a.pnext[g] = px; a.pnext[n + g] = py;
+            a.qnext[g] = qx; a.qnext[n + g] = qy; a.qnext[2 * n + g] = qz;
+        }
+    }
+    __syncthreads();
+
+    // u of k+2 on the tile
+    for (int i = tid; i < TX * TY; i += NT) {
+        const int ty = i / TX, tx = i - ty * TX;
+        const int x = x0 + tx, y = y0 + ty;
+        if (x >= W || y >= H) continue;
+        const size_t g = (size_t)y * W + x;
+        const int ix = tx + 2, iy = ty + 2;
+        const int lx = x > 0 ? ix - 1 : ix, uy = y > 0 ? iy - 1 : iy;
+        float un, u1xn, u1yn;
+        sparse_u_step(a, sU[0][iy][ix], sU[1][iy][ix], sU[2][iy][ix], sT[0][iy][ix], sT[1][iy][ix], sT[2][iy][ix],
+                      sP[0][iy][ix], sP[1][iy][ix], sP[0][iy][lx], sP[1][iy][lx], sP[0][uy][ix], sP[1][uy][ix],
+                      sQ[0][iy][ix], sQ[0][iy][lx], sQ[1][iy][ix], sQ[1][uy][ix], sQ[2][iy][ix], sQ[2][iy][lx],
+                      sQ[2][uy][ix], a.sparse[g], un, u1xn, u1yn);
         a.unext[g] = un;
         a.unext[n + g] = u1xn;
         a.unext[2 * n + g] = u1yn;
@@ -184,6 +314,19 @@ static hipError_t sparse_launch(const SparsePtrs& a, hipStream_t hs) {
     const int ntiles = ((a.W + TS_TX - 1) / TS_TX) * ((a.H + TY - 1) / TY);
     tgv_sparse_iter_kernel<TY, NT><<<ntiles, NT, 0, hs>>>(a);
     return hipGetLastError();
+}
+
+template <int TY>
+static hipError_t sparse_launch2(const SparsePtrs& a, hipStream_t hs) {
+    const int ntiles = ((a.W + TS_TX - 1) / TS_TX) * ((a.H + TY - 1) / TY);
+    tgv_sparse_iter2_kernel<TY, TS_TX * TY><<<ntiles, TS_TX * TY, 0, hs>>>(a);
+    return hipGetLastError();
+}
+
+// Two iterations per launch unless PSW_TGV_SPARSE_DOUBLE=0 (identical results).
+static bool sparse_double() {
+    const char* e = std::getenv("PSW_TGV_SPARSE_DOUBLE");
+    return !(e && std::atoi(e) == 0);
 }
 
 // Workspace: T 3n, then two slots of (u 3n, p 2n, q 3n).
@@ -237,25 +380,35 @@ int psw_tgv_sparse_denoise(const float* d_ref, const float* d_dense, const float
     a.tau = (float)tau;
     a.sigma = (float)sigma;
     a.sz = sz;
-    for (int k = 0; k < niters && e == hipSuccess; ++k) {
-        float* c = slot[k & 1];
-        float* nx = slot[(k + 1) & 1];
+    // Each launch reads one slot and writes the other (a two-iteration launch
+    // too), so the current state is in slot (launches & 1).
+    const bool dbl = sparse_double();
+    int launches = 0;
+    for (int k = 0; k < niters && e == hipSuccess; ++launches) {
+        float* c = slot[launches & 1];
+        float* nx = slot[(launches + 1) & 1];
         a.ucur = c;
         a.unext = nx;
         a.pcur = c + 3 * n;
         a.pnext = nx + 3 * n;
         a.qcur = c + 5 * n;
         a.qnext = nx + 5 * n;
+        if (dbl && niters - k >= 2) {
+            e = sparse_launch2<8>(a, hs);
+            k += 2;
+            continue;
+        }
         switch (variant) {
             case 0: e = sparse_launch<4, 256>(a, hs); break;
             case 2: e = sparse_launch<8, 256>(a, hs); break;
             case 3: e = sparse_launch<16, 512>(a, hs); break;
             default: e = sparse_launch<8, 512>(a, hs); break;
         }
+        k += 1;
     }
     if (e == hipSuccess) {
         const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
-        tgv_sparse_output_kernel<<<blocks, 256, 0, hs>>>(d_out, slot[niters & 1], n, zfar);
+        tgv_sparse_output_kernel<<<blocks, 256, 0, hs>>>(d_out, slot[launches & 1], n, zfar);
         e = hipGetLastError();
     }
     if (owned) {
