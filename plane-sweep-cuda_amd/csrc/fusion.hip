@@ -487,6 +487,202 @@ __global__ __launch_bounds__(FZ_BX * FZP_BY) void fusion_pd_zpipe_kernel(PdArgs 
     }
 }
 
+// Two fused steps per launch: (P_k, U_{k+1}, P_{k+1}, U_{k+2}).  A block of
+// 64 x 8 columns marches FT_CHUNK planes; at step s it computes, in LDS,
+//   A  p of k+1 at plane s     on the tile + 2 columns left/up, 1 right/down
+//   B  u, v of k+1 at plane s  on the tile + a 1-column ring
+//   C  p of k+2 at plane s-1   on the tile + 1 column left/up
+//   D  u, v of k+2 at plane s-1 on the tile, written to HBM with p of k+2,
+// so a voxel's p, u, v, counts are read once per two steps and its new
+// values written once.  Planes alternate between two LDS buffers by parity;
+// a chunk starts two planes early (stage A) to have the z-1 duals.  The
+// per-position arithmetic and its order are those of fusion_pd_fused_kernel
+// (p_step, the same divergence, prox), so the grids are identical.  u is
+// read on the halo, so unlike the one-step kernels it needs uin != uout.
+// LDS index of a column: (x - x0 + 2, y - y0 + 2); positions outside the
+// grid read clamped coordinates and never feed positions inside it.
+constexpr int FT_X = 64, FT_Y = 8, FT_NT = FT_X * FT_Y, FT_CHUNK = 32;
+constexpr int FT_LX = FT_X + 4, FT_LY = FT_Y + 4;
+
+template <bool H8>
+__global__ __launch_bounds__(FT_NT) void fusion_pd2_kernel(PdArgs a) {
+    using HT = typename std::conditional<H8, uint2, uint4>::type;
+    __shared__ float sV[2][FT_LY][FT_LX];       // v of k, planes by parity
+    __shared__ float sP1[2][3][FT_LY][FT_LX];   // p of k+1
+    __shared__ float sU1[2][FT_LY][FT_LX];      // u of k+1
+    __shared__ float sV1[2][FT_LY][FT_LX];      // v of k+1
+    __shared__ float sP2[2][3][FT_LY][FT_LX];   // p of k+2
+    const FusionView& f = a.f;
+    const int W = f.w, H = f.h, D = f.d;
+    const size_t V = f.V, sxy = (size_t)W * H;
+    const int tx_n = (W + FT_X - 1) / FT_X, ty_n = (H + FT_Y - 1) / FT_Y;
+    const int ntiles = tx_n * ty_n * ((D + FT_CHUNK - 1) / FT_CHUNK);
+    const int t = xcd_tile(blockIdx.x, ntiles);
+    const int bx = t % tx_n, by = (t / tx_n) % ty_n, bz = t / (tx_n * ty_n);
+    const int x0 = bx * FT_X, y0 = by * FT_Y;
+    const int z0 = bz * FT_CHUNK, z1 = min(z0 + FT_CHUNK, D);
+    const int tid = threadIdx.x;
+    auto gxy = [&](int ix, int iy) {   // clamped in-plane index of LDS column (ix, iy)
+        return (size_t)min(max(y0 - 2 + iy, 0), H - 1) * W + min(max(x0 - 2 + ix, 0), W - 1);
+    };
+    auto hist_at = [&](size_t i) -> HT {
+        if constexpr (H8) return a.hist8[i];
+        else return *reinterpret_cast<const uint4*>(f.hist + i * FB);
+    };
+    auto prox = [&](float uu, const HT& h) -> float {
+        if constexpr (H8) return prox_hist_h8(uu, h, a.tl);
+        else return prox_hist_h(uu, h, a.tl);
+    };
+    const int sa = max(z0 - 2, 0), sb = max(z0 - 1, 0);   // first planes of stages A and B (and C)
+    const int slast = min(z1, D - 1);                       // last plane of stages A and B
+    // Each thread's positions are the same on every plane: j-th position of
+    // a region = tid + j * FT_NT.  Their global inputs for the next plane are
+    // loaded into registers right after the current ones are used.
+    constexpr int NV = (FT_LX * FT_LY + FT_NT - 1) / FT_NT;               // v of k (plane s+1)
+    constexpr int NA = ((FT_LX - 1) * (FT_LY - 1) + FT_NT - 1) / FT_NT;   // stage A: p of k
+    constexpr int NB = ((FT_LX - 2) * (FT_LY - 2) + FT_NT - 1) / FT_NT;   // stage B: u of k, counts
+    float rv[NV], rp[NA][3], ru[NB];
+    HT rh[NB], rhd{};
+    const int tcx = tid % FT_X, tcy = tid / FT_X;
+    const bool dlive = x0 + tcx < W && y0 + tcy < H;
+    const size_t dxy = dlive ? (size_t)(y0 + tcy) * W + (x0 + tcx) : 0;
+    auto load_v = [&](int plane) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            const int i = tid + j * FT_NT;
+            if (i < FT_LX * FT_LY) rv[j] = a.vin[(size_t)plane * sxy + gxy(i % FT_LX, i / FT_LX)];
+        }
+    };
+    auto load_a = [&](int plane) {
+#pragma unroll
+        for (int j = 0; j < NA; ++j) {
+            const int i = tid + j * FT_NT;
+            if (i < (FT_LX - 1) * (FT_LY - 1)) {
+                const size_t g = (size_t)plane * sxy + gxy(i % (FT_LX - 1), i / (FT_LX - 1));
+                rp[j][0] = a.pin[g];
+                rp[j][1] = a.pin[V + g];
+                rp[j][2] = a.pin[2 * V + g];
+            }
+        }
+    };
+    auto load_b = [&](int plane) {
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const int i = tid + j * FT_NT;
+            if (i < (FT_LX - 2) * (FT_LY - 2)) {
+                const size_t g = (size_t)plane * sxy + gxy(i % (FT_LX - 2) + 1, i / (FT_LX - 2) + 1);
+                ru[j] = a.uin[g];
+                rh[j] = hist_at(g);
+            }
+        }
+    };
+    for (int i = tid; i < FT_LX * FT_LY; i += FT_NT) {
+        const int iy = i / FT_LX, ix = i - iy * FT_LX;
+        sV[sa & 1][iy][ix] = a.vin[(size_t)sa * sxy + gxy(ix, iy)];
+    }
+    if (sa + 1 < D) load_v(sa + 1);
+    load_a(sa);
+    load_b(sa);
+    for (int s = sa; s <= slast + 1; ++s) {
+        const bool ab = s <= slast;
+        const int cs = s & 1, ps = (s + 1) & 1;   // buffers of planes s and s-1 (and s+1)
+        if (ab && s + 1 < D) {
+#pragma unroll
+            for (int j = 0; j < NV; ++j) {
+                const int i = tid + j * FT_NT;
+                if (i < FT_LX * FT_LY) sV[ps][i / FT_LX][i % FT_LX] = rv[j];
+            }
+        }
+        __syncthreads();
+        // A: p of k+1 at plane s on [0, LX-1) x [0, LY-1)
+        if (ab) {
+#pragma unroll
+            for (int j = 0; j < NA; ++j) {
+                const int i = tid + j * FT_NT;
+                if (i < (FT_LX - 1) * (FT_LY - 1)) {
+                    const int iy = i / (FT_LX - 1), ix = i - iy * (FT_LX - 1);
+                    const int gx = x0 - 2 + ix, gy = y0 - 2 + iy;
+                    const float v0 = sV[cs][iy][ix];
+                    const float gxv = gx < W - 1 ? sV[cs][iy][ix + 1] - v0 : 0.f;
+                    const float gyv = gy < H - 1 ? sV[cs][iy + 1][ix] - v0 : 0.f;
+                    const float gzv = s < D - 1 ? sV[ps][iy][ix] - v0 : 0.f;
+                    float ox, oy, oz;
+                    p_step(rp[j][0], rp[j][1], rp[j][2], gxv, gyv, gzv, a.sigma, ox, oy, oz);
+                    sP1[cs][0][iy][ix] = ox;
+                    sP1[cs][1][iy][ix] = oy;
+                    sP1[cs][2][iy][ix] = oz;
+                }
+            }
+            // the next plane's v of k and p of k
+            if (s + 2 < D && s + 1 <= slast) load_v(s + 2);
+            if (s + 1 <= slast) load_a(s + 1);
+        }
+        __syncthreads();
+        // B: u, v of k+1 at plane s on [1, LX-1) x [1, LY-1)
+        if (ab && s >= sb) {
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                const int i = tid + j * FT_NT;
+                if (i < (FT_LX - 2) * (FT_LY - 2)) {
+                    const int iy = i / (FT_LX - 2) + 1, ix = i - (i / (FT_LX - 2)) * (FT_LX - 2) + 1;
+                    const int gx = x0 - 2 + ix, gy = y0 - 2 + iy;
+                    float div = sP1[cs][0][iy][ix] + sP1[cs][1][iy][ix] + sP1[cs][2][iy][ix];
+                    if (gx > 0) div -= sP1[cs][0][iy][ix - 1];
+                    if (gy > 0) div -= sP1[cs][1][iy - 1][ix];
+                    if (s > 0) div -= sP1[ps][2][iy][ix];
+                    const float un = ru[j];
+                    const float uu = un - a.tau * (-div);
+                    const float unew = prox(uu, rh[j]);
+                    sU1[cs][iy][ix] = unew;
+                    sV1[cs][iy][ix] = 2.f * unew - un;
+                }
+            }
+        }
+        if (ab && s + 1 <= slast) load_b(s + 1);
+        __syncthreads();
+        // C: p of k+2 at plane c = s-1 on [1, LX-2) x [1, LY-2)
+        const int c = s - 1;
+        if (c >= sb && c < z1) {
+            for (int i = tid; i < (FT_LX - 3) * (FT_LY - 3); i += FT_NT) {
+                const int iy = i / (FT_LX - 3) + 1, ix = i - (i / (FT_LX - 3)) * (FT_LX - 3) + 1;
+                const int gx = x0 - 2 + ix, gy = y0 - 2 + iy;
+                const float v0 = sV1[ps][iy][ix];
+                const float gxv = gx < W - 1 ? sV1[ps][iy][ix + 1] - v0 : 0.f;
+                const float gyv = gy < H - 1 ? sV1[ps][iy + 1][ix] - v0 : 0.f;
+                const float gzv = c < D - 1 ? sV1[cs][iy][ix] - v0 : 0.f;
+                float ox, oy, oz;
+                p_step(sP1[ps][0][iy][ix], sP1[ps][1][iy][ix], sP1[ps][2][iy][ix], gxv, gyv, gzv, a.sigma, ox, oy,
+                       oz);
+                sP2[ps][0][iy][ix] = ox;
+                sP2[ps][1][iy][ix] = oy;
+                sP2[ps][2][iy][ix] = oz;
+            }
+        }
+        __syncthreads();
+        // D: u, v of k+2 at plane c on the tile; p, u, v of k+2 to HBM
+        if (c >= z0 && c < z1 && dlive) {
+            const int x = x0 + tcx, y = y0 + tcy;
+            const int ix = tcx + 2, iy = tcy + 2;
+            const size_t g = (size_t)c * sxy + dxy;
+            const float px = sP2[ps][0][iy][ix], py = sP2[ps][1][iy][ix], pz = sP2[ps][2][iy][ix];
+            float div = px + py + pz;
+            if (x > 0) div -= sP2[ps][0][iy][ix - 1];
+            if (y > 0) div -= sP2[ps][1][iy - 1][ix];
+            if (c > 0) div -= sP2[cs][2][iy][ix];
+            const float un = sU1[ps][iy][ix];
+            const float uu = un - a.tau * (-div);
+            const float unew = prox(uu, rhd);
+            a.pout[g] = px;
+            a.pout[V + g] = py;
+            a.pout[2 * V + g] = pz;
+            a.uout[g] = unew;
+            a.vout[g] = 2.f * unew - un;
+        }
+        // the counts of the next step's stage D (plane s)
+        if (dlive && s >= z0 && s < z1) rhd = hist_at((size_t)s * sxy + dxy);
+    }
+}
+
 // z-marching with the x-1 and y-1 neighbours' new duals taken from the
 // threads that own them (through LDS) instead of recomputed: a 64x8 block
 // recomputes only its left column and top row halo.  Same arithmetic and
@@ -831,12 +1027,31 @@ int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double la
     st = psw_fusion_update_u(g, tau, lambda, s);
     if (st != PSW_OK) return st;
     if (niters > 1) {
-        // Second dual/primal buffers for the ping-pong.
-        float *p2 = nullptr, *v2 = nullptr;
-        PSW_TRY_HIP(hipMalloc((void**)&p2, 3 * V * sizeof(float)), PSW_ERR_ALLOC);
-        hipError_t e = hipMalloc((void**)&v2, V * sizeof(float));
+        // PSW_FUSION_PD selects the kernel (identical results): 0 one voxel per
+        // thread, 1 z-marching, 2 z-marching with LDS-shared neighbour duals,
+        // 3 z-marching with software-pipelined loads (the default), 4 two
+        // steps per launch (an odd last step runs on mode 3's kernel; half the
+        // HBM bytes, but at 256^3 it measured 58.9 ms for 300 iterations
+        // against 57.7: four barriers per plane and the halo's partial second
+        // passes cost what the bytes save).
+        const char* env = getenv("PSW_FUSION_PD");
+        int mode = env ? atoi(env) : 3;
+        if (mode < 0 || mode > 4) mode = 3;
+        // Second dual/primal buffers for the ping-pong (u too for the
+        // two-step kernel, which reads u on a halo).
+        float *p2 = nullptr, *v2 = nullptr, *u2 = nullptr;
+        uint2* h8 = nullptr;
+        auto release = [&]() {
+            if (p2) (void)hipFree(p2);
+            if (v2) (void)hipFree(v2);
+            if (u2) (void)hipFree(u2);
+            if (h8) (void)hipFree(h8);
+        };
+        hipError_t e = hipMalloc((void**)&p2, 3 * V * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc((void**)&v2, V * sizeof(float));
+        if (e == hipSuccess && mode == 4) e = hipMalloc((void**)&u2, V * sizeof(float));
         if (e != hipSuccess) {
-            (void)hipFree(p2);
+            release();
             return hip_status(e, "hipMalloc", PSW_ERR_ALLOC);
         }
         PdArgs a;
@@ -845,12 +1060,7 @@ int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double la
         a.tau = (float)tau; a.lambda = (float)lambda; a.tl = a.tau * a.lambda; a.sigma = (float)sigma;
         float* pb[2] = {g->p, p2};
         float* vb[2] = {g->v, v2};
-        // PSW_FUSION_PD selects the kernel (identical results): 0 one voxel per
-        // thread, 1 z-marching, 2 z-marching with LDS-shared neighbour duals,
-        // 3 z-marching with software-pipelined loads (the default).
-        const char* env = getenv("PSW_FUSION_PD");
-        int mode = env ? atoi(env) : 3;
-        if (mode < 0 || mode > 3) mode = 3;
+        float* ub[2] = {g->u, mode == 4 ? u2 : g->u};
         const bool zmarch = mode >= 1;
         const dim3 blk = mode == 2 ? dim3(FZL_TX, FZL_TY, 1) : dim3(64, 4, 1);
         const dim3 grd = zmarch ? dim3((unsigned)((f.w + blk.x - 1) / blk.x), (unsigned)((f.h + blk.y - 1) / blk.y),
@@ -860,13 +1070,13 @@ int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double la
                                               ((f.d + FZ_CHUNK - 1) / FZ_CHUNK));
         const unsigned zp_blocks = (unsigned)(((f.w + FZ_BX - 1) / FZ_BX) * ((f.h + FZP_BY - 1) / FZP_BY) *
                                               ((f.d + FZP_CHUNK - 1) / FZP_CHUNK));
-        // The pipelined kernel reads the counts from a byte-packed copy when
+        const unsigned ft_blocks = (unsigned)(((f.w + FT_X - 1) / FT_X) * ((f.h + FT_Y - 1) / FT_Y) *
+                                              ((f.d + FT_CHUNK - 1) / FT_CHUNK));
+        // The pipelined kernels read the counts from a byte-packed copy when
         // every count fits in a byte (8 B per voxel-iteration less traffic),
         // else from the grid's own 16-bit histogram.
-        uint2* h8 = nullptr;
-        unsigned* d_max = nullptr;
-        if (mode == 3 && hipMalloc((void**)&h8, V * sizeof(uint2) + 256) == hipSuccess) {
-            d_max = reinterpret_cast<unsigned*>(reinterpret_cast<char*>(h8) + V * sizeof(uint2));
+        if (mode >= 3 && hipMalloc((void**)&h8, V * sizeof(uint2) + 256) == hipSuccess) {
+            unsigned* d_max = reinterpret_cast<unsigned*>(reinterpret_cast<char*>(h8) + V * sizeof(uint2));
             unsigned hmax = 0xffffffffu;
             fusion_zero_u32_kernel<<<1, 1, 0, S(s)>>>(d_max);
             fusion_hist_pack_kernel<<<(unsigned)std::min<size_t>((V + 255) / 256, 8192), 256, 0, S(s)>>>(
@@ -879,35 +1089,44 @@ int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double la
             if (st == PSW_OK && hmax <= 255u) a.hist8 = h8;
         }
         int cur = 0;
-        for (int k = 1; k < niters && st == PSW_OK; ++k) {
+        for (int k = 1; k < niters && st == PSW_OK;) {
             a.pin = pb[cur]; a.pout = pb[cur ^ 1];
             a.vin = vb[cur]; a.vout = vb[cur ^ 1];
-            a.uin = g->u; a.uout = g->u;
-            if (mode == 2) {
-                fusion_pd_zlds_kernel<<<grd, blk, 0, S(s)>>>(a);
-                st = launch_status("fusion_pd_zlds_kernel");
-            } else if (mode == 3) {
-                if (a.hist8) fusion_pd_zpipe_kernel<true><<<zp_blocks, FZ_BX * FZP_BY, 0, S(s)>>>(a);
-                else fusion_pd_zpipe_kernel<false><<<zp_blocks, FZ_BX * FZP_BY, 0, S(s)>>>(a);
-                st = launch_status("fusion_pd_zpipe_kernel");
-            } else if (zmarch) {
-                fusion_pd_zmarch_kernel<<<zm_blocks, FZ_BX * FZ_BY, 0, S(s)>>>(a);
-                st = launch_status("fusion_pd_zmarch_kernel");
+            a.uin = ub[cur]; a.uout = ub[cur ^ 1];
+            if (mode == 4 && niters - k >= 2) {
+                if (a.hist8) fusion_pd2_kernel<true><<<ft_blocks, FT_NT, 0, S(s)>>>(a);
+                else fusion_pd2_kernel<false><<<ft_blocks, FT_NT, 0, S(s)>>>(a);
+                st = launch_status("fusion_pd2_kernel");
+                k += 2;
             } else {
-                fusion_pd_fused_kernel<<<grd, blk, 0, S(s)>>>(a);
-                st = launch_status("fusion_pd_fused_kernel");
+                if (mode == 2) {
+                    fusion_pd_zlds_kernel<<<grd, blk, 0, S(s)>>>(a);
+                    st = launch_status("fusion_pd_zlds_kernel");
+                } else if (mode >= 3) {
+                    if (a.hist8) fusion_pd_zpipe_kernel<true><<<zp_blocks, FZ_BX * FZP_BY, 0, S(s)>>>(a);
+                    else fusion_pd_zpipe_kernel<false><<<zp_blocks, FZ_BX * FZP_BY, 0, S(s)>>>(a);
+                    st = launch_status("fusion_pd_zpipe_kernel");
+                } else if (zmarch) {
+                    fusion_pd_zmarch_kernel<<<zm_blocks, FZ_BX * FZ_BY, 0, S(s)>>>(a);
+                    st = launch_status("fusion_pd_zmarch_kernel");
+                } else {
+                    fusion_pd_fused_kernel<<<grd, blk, 0, S(s)>>>(a);
+                    st = launch_status("fusion_pd_fused_kernel");
+                }
+                k += 1;
             }
             cur ^= 1;
         }
         if (st == PSW_OK && cur == 1) {
             // results live in the second buffers: move them home
-            PSW_TRY_HIP(hipMemcpyAsync(g->p, p2, 3 * V * sizeof(float), hipMemcpyDeviceToDevice, S(s)), PSW_ERR_COPY);
-            PSW_TRY_HIP(hipMemcpyAsync(g->v, v2, V * sizeof(float), hipMemcpyDeviceToDevice, S(s)), PSW_ERR_COPY);
+            e = hipMemcpyAsync(g->p, p2, 3 * V * sizeof(float), hipMemcpyDeviceToDevice, S(s));
+            if (e == hipSuccess) e = hipMemcpyAsync(g->v, v2, V * sizeof(float), hipMemcpyDeviceToDevice, S(s));
+            if (e == hipSuccess && ub[1] != g->u)
+                e = hipMemcpyAsync(g->u, ub[1], V * sizeof(float), hipMemcpyDeviceToDevice, S(s));
+            if (e != hipSuccess) st = hip_status(e, "hipMemcpyAsync", PSW_ERR_COPY);
         }
         (void)hipStreamSynchronize(S(s));
-        (void)hipFree(p2);
-        (void)hipFree(v2);
-        if (h8) (void)hipFree(h8);
+        release();
         if (st != PSW_OK) return st;
     }
     return psw_fusion_update_p(g, sigma, s);

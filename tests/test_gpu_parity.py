@@ -500,21 +500,25 @@ def test_fusion_pd_vs_oracle(gpu):
     assert abs(e_gpu - e_orc) <= 1e-12 * abs(e_orc)
 
 
-def test_fusion_solve_kernels_identical(gpu, monkeypatch):
+@pytest.mark.parametrize("niters", [41, 42, 2])
+def test_fusion_solve_kernels_identical(gpu, monkeypatch, niters):
     """The z-marching PD kernels (registers only, with LDS-shared neighbour
-    duals, with pipelined loads) and the one-voxel-per-thread kernel give bit-identical
-    grids (depth 28 leaves a partial z chunk; 20 rows a partial 8-row block)."""
+    duals, with pipelined loads, two steps per launch) and the
+    one-voxel-per-thread kernel give bit-identical grids (depth 28 leaves a
+    partial z chunk; 20 rows a partial 8-row block; 42 iterations end the
+    two-step launches with a one-step launch)."""
     from psw_amd import stages
     from psw_amd.device import DeviceArray
     o, vol, (gw, gh, gd) = _fusion_inputs(n=3)
     H, W = o["depths"].shape[1:]
     out = []
-    for mode in ("1", "0", "2", "3"):
+    print(f"[case] fusion kernels identical, {niters} iterations", flush=True)
+    for mode in ("1", "0", "2", "3", "4"):
         monkeypatch.setenv("PSW_FUSION_PD", mode)
         f = stages.Fusion(gw, gh, gd, vol)
         for k in range(o["depths"].shape[0]):
             f.update_histogram(DeviceArray.from_host(o["depths"][k]), W, H, o["K"], o["R"][k], o["t"][k], 0.25)
-        f.solve(41, 0.1, 0.5, 1.0)
+        f.solve(niters, 0.1, 0.5, 1.0)
         out.append(f.download())
     for o in out[1:]:
         for key in ("u", "v", "p", "hist"):
@@ -537,19 +541,20 @@ def test_fusion_solve_histogram_packing(gpu, monkeypatch, top):
     print(f"[case] histogram packing, top count {top}, max {int(hist.max())}", flush=True)
     tau, lam, sig, n = 0.1, 0.5, 1.0, 9
     out = []
-    for mode in ("0", "3"):
+    for mode in ("0", "3", "4"):
         monkeypatch.setenv("PSW_FUSION_PD", mode)
         f = stages.Fusion(gw, gh, gd, vol)
         f.upload(hist=hist)
         f.solve(n, tau, lam, sig)
         out.append(f.download())
-    for key in ("u", "v", "p", "hist"):
-        np.testing.assert_array_equal(out[0][key], out[1][key], err_msg=key)
+    for o in out[1:]:
+        for key in ("u", "v", "p", "hist"):
+            np.testing.assert_array_equal(out[0][key], o[key], err_msg=key)
     u64, v64, p64 = np.zeros((gd, gh, gw)), np.zeros((gd, gh, gw)), np.zeros((3, gd, gh, gw))
     for _ in range(n):
         oracle.fusion_u(u64, v64, p64, hist, tau, lam, prec="f64")
         oracle.fusion_p(p64, v64, sig, prec="f64")
-    e = rel_err(out[1]["u"], u64)
+    e = rel_err(out[2]["u"], u64)
     print(f"u vs FP64 max {e.max():.3e}")
     assert e.max() <= 1e-4
 
