@@ -501,11 +501,14 @@ __global__ __launch_bounds__(FZ_BX * FZP_BY) void fusion_pd_zpipe_kernel(PdArgs 
 // read on the halo, so unlike the one-step kernels it needs uin != uout.
 // LDS index of a column: (x - x0 + 2, y - y0 + 2); positions outside the
 // grid read clamped coordinates and never feed positions inside it.
-constexpr int FT_X = 64, FT_Y = 8, FT_NT = FT_X * FT_Y, FT_CHUNK = 32;
+// 768 threads: every halo stage (at most 67 x 11 = 737 columns) takes one
+// pass; stage D uses the first 512.  With at most 80 VGPRs (launch bounds)
+// two blocks fit per CU.
+constexpr int FT_X = 64, FT_Y = 8, FT_NT = 768, FT_CHUNK = 32;
 constexpr int FT_LX = FT_X + 4, FT_LY = FT_Y + 4;
 
 template <bool H8>
-__global__ __launch_bounds__(FT_NT) void fusion_pd2_kernel(PdArgs a) {
+__global__ __launch_bounds__(FT_NT, 2) void fusion_pd2_kernel(PdArgs a) {
     using HT = typename std::conditional<H8, uint2, uint4>::type;
     __shared__ float sV[2][FT_LY][FT_LX];       // v of k, planes by parity
     __shared__ float sP1[2][3][FT_LY][FT_LX];   // p of k+1
@@ -543,8 +546,8 @@ __global__ __launch_bounds__(FT_NT) void fusion_pd2_kernel(PdArgs a) {
     constexpr int NB = ((FT_LX - 2) * (FT_LY - 2) + FT_NT - 1) / FT_NT;   // stage B: u of k, counts
     float rv[NV], rp[NA][3], ru[NB];
     HT rh[NB], rhd{};
-    const int tcx = tid % FT_X, tcy = tid / FT_X;
-    const bool dlive = x0 + tcx < W && y0 + tcy < H;
+    const int tcx = tid % FT_X, tcy = tid / FT_X;   // stage D: threads below FT_X * FT_Y
+    const bool dlive = tid < FT_X * FT_Y && x0 + tcx < W && y0 + tcy < H;
     const size_t dxy = dlive ? (size_t)(y0 + tcy) * W + (x0 + tcx) : 0;
     auto load_v = [&](int plane) {
 #pragma unroll
@@ -1029,14 +1032,11 @@ int psw_fusion_solve(const psw_fusion_grid* g, int niters, double tau, double la
     if (niters > 1) {
         // PSW_FUSION_PD selects the kernel (identical results): 0 one voxel per
         // thread, 1 z-marching, 2 z-marching with LDS-shared neighbour duals,
-        // 3 z-marching with software-pipelined loads (the default), 4 two
-        // steps per launch (an odd last step runs on mode 3's kernel; half the
-        // HBM bytes, but at 256^3 it measured 58.9 ms for 300 iterations
-        // against 57.7: four barriers per plane and the halo's partial second
-        // passes cost what the bytes save).
+        // 3 z-marching with software-pipelined loads, 4 two steps per launch
+        // (the default; an odd last step runs on mode 3's kernel).
         const char* env = getenv("PSW_FUSION_PD");
-        int mode = env ? atoi(env) : 3;
-        if (mode < 0 || mode > 4) mode = 3;
+        int mode = env ? atoi(env) : 4;
+        if (mode < 0 || mode > 4) mode = 4;
         // Second dual/primal buffers for the ping-pong (u too for the
         // two-step kernel, which reads u on a halo).
         float *p2 = nullptr, *v2 = nullptr, *u2 = nullptr;
