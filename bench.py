@@ -164,6 +164,53 @@ def run_sparse(pl, stream, ev, seed):
                       "iterations, alpha0 2 alpha1 1.5 tau 0.02 sigma 1"}
 
 
+TVL1_ITERS = 1000
+# per pixel and iteration, with nothing kept on chip between iterations: read
+# depth, Px, Py, R, the scaled input and T11..T22 (36 B), write depth, Px, Py, R (16 B)
+TVL1_BYTES_PER_PIXEL_ITER = 52
+
+
+def run_tvl1(pl, stream, ev):
+    """TVL1 denoising (PlaneSweep::CudaDenoise) of the 640x480 sweep depthmap
+    with the reference's default parameters.  Per-iteration time from a
+    1000- and a 0-iteration call, for the fused launches and for the
+    reference's two launches per iteration (PSW_TVL1_FUSED=0)."""
+    from psw_amd import stages
+    from psw_amd.device import DeviceArray
+    d_depth = pl.sweep_and_linearise()
+    pl.st.synchronize()
+    d_out, d_work = DeviceArray((H, W)), DeviceArray((12, H, W))
+
+    def timed(k):
+        ev[0].record(stream)
+        stages.tvl1_denoise_device(d_depth, pl.v["d_ref"], W, H, ZNEAR, ZFAR, d_out, d_work, k,
+                                   stream=stream.handle)
+        ev[1].record(stream)
+        ev[1].synchronize()
+        return ev[0].elapsed_ms(ev[1])
+
+    res = {}
+    prev = os.environ.get("PSW_TVL1_FUSED")
+    try:
+        for name, flag in (("launch_sequence", "0"), ("fused", "1")):
+            os.environ["PSW_TVL1_FUSED"] = flag
+            timed(20)
+            t_full, t_zero = timed(TVL1_ITERS), timed(0)
+            res[name] = (t_full - t_zero) / TVL1_ITERS
+    finally:
+        if prev is None:
+            os.environ.pop("PSW_TVL1_FUSED", None)
+        else:
+            os.environ["PSW_TVL1_FUSED"] = prev
+    gbs = TVL1_BYTES_PER_PIXEL_ITER * W * H / (res["fused"] * 1e-3) / 1e9
+    return {"kernel": "tvl1_iter_kernel<2>", "iterations_per_launch": 2, "iter_us": res["fused"] * 1e3,
+            "launch_sequence_iter_us": res["launch_sequence"] * 1e3,
+            "speedup_vs_launch_sequence": res["launch_sequence"] / res["fused"], "gbs_algorithmic": gbs,
+            "frac_hbm_peak": gbs / HBM_PEAK_GBS, "bytes_per_pixel_iter": TVL1_BYTES_PER_PIXEL_ITER,
+            "config": f"{W}x{H} sweep depthmap, {TVL1_ITERS} iterations, lambda 0.3 tau 0.02 sigma 6 theta 1 "
+                      "beta 0 gamma 1 (the reference's defaults)"}
+
+
 class Pipeline:
     """Device state of the benchmarked pipeline for one seeded view."""
 
@@ -260,6 +307,10 @@ def probe(names, seed):
             if pl is None:
                 pl = Pipeline(seed, None)
             log(json.dumps(run_sparse(pl, st, [Event() for _ in range(2)], seed)))
+        elif name == "tvl1":
+            if pl is None:
+                pl = Pipeline(seed, None)
+            log(json.dumps(run_tvl1(pl, st, [Event() for _ in range(2)])))
         elif name == "fusion":
             o, thr = fusion_workload(seed)
             ev = [Event() for _ in range(3)]
@@ -377,7 +428,7 @@ def main():
     ap.add_argument("--batch", action="store_true", help="run the uneven 32-frame batch instead (its own JSON line)")
     ap.add_argument("--batch-frames", type=int, default=32)
     ap.add_argument("--batch-estar-iters", type=int, default=20000)
-    ap.add_argument("--probe", default="", help="comma list of tgv,sweep,sweep_exact,unet,copy,fusion,sparse: run only "
+    ap.add_argument("--probe", default="", help="comma list of tgv,sweep,sweep_exact,unet,copy,fusion,sparse,tvl1: run only "
                                                 "these small workloads (for rocprofv3) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -529,6 +580,9 @@ def main():
     rec["sparse_tgv"] = run_sparse(pl, Stream(), [Event() for _ in range(2)], args.seed)
     log(f"sparse TGV2: {rec['sparse_tgv']['iter_us']:.2f} us per iteration "
         f"({rec['sparse_tgv']['gbs_algorithmic']:.0f} GB/s algorithmic)")
+    rec["tvl1"] = run_tvl1(pl, Stream(), [Event() for _ in range(2)])
+    log(f"TVL1: {rec['tvl1']['iter_us']:.2f} us per iteration fused, "
+        f"{rec['tvl1']['launch_sequence_iter_us']:.2f} us as two launches per iteration")
     if not args.no_fusion:
         t0 = time.perf_counter()
         o, thr = fusion_workload(args.seed)

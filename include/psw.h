@@ -432,7 +432,10 @@ int psw_fusion_surface_count(const psw_fusion_grid* g, int64_t* count, psw_strea
 /* ---------------------------------------------------------------------- */
 /* d_depth (device, W*H) is denoised into d_out with the reference's scaling
  * to [0,1] over [znear, zfar]; d_ref holds 0..255 intensities for the tensor.
- * d_work: 8*W*H floats of device workspace. */
+ * d_work: 12*W*H floats of device workspace (the solver state is
+ * double-buffered).  PSW_TVL1_FUSED=0 selects the reference's two launches
+ * per iteration instead of the fused one- and two-iteration launches; the
+ * results are identical. */
 int psw_tvl1_denoise(const float* d_depth, const float* d_ref, int width, int height, float znear, float zfar,
                      int niters, double lambda, double tau, double sigma, double theta, double beta, double gamma,
                      float* d_out, float* d_work, psw_stream_t s);

@@ -6,7 +6,8 @@
 // where it computes in float, double where it declares double temporaries,
 // no FMA contraction.  Driving them in the reference's own host sequence
 // therefore yields the reference's results; the fused kernels of
-// sweep_fused.hip / tgv_fused.hip / fusion.hip are the fast path.
+// sweep_fused.hip / tgv_fused.hip / fusion.hip / tvl1_fused.hip are the fast
+// path.
 //
 // Pointer parameters carry no __restrict__: the reference's host code passes
 // aliased arguments in places (calculate_STD in place, planesweep.cpp:118;
@@ -861,50 +862,3 @@ int psw_calculateWeights_sparseDepth(float* d_w, const float* d_Ds, int width, i
 
 }  // extern "C"
 
-// ---- TVL1 depth denoising driver ---------------------------------------------
-// PlaneSweep::CudaDenoise (src/planesweep.cpp:297-379) over the launchers
-// above: tensor-weighted P step (kernels.cu:283-307) and the plain update
-// (kernels.cu:342-370), with the reference's scaling of depth to [0, 1].
-extern "C" int psw_tvl1_denoise(const float* d_depth, const float* d_ref, int width, int height, float znear,
-                                float zfar, int niters, double lambda, double tau, double sigma, double theta,
-                                double beta, double gamma, float* d_out, float* d_work, psw_stream_t s) {
-    PSW_CHECK_ARG(d_depth && d_ref && d_out && d_work, "NULL pointer");
-    PSW_CHECK_ARG(width >= 2 && height >= 2 && niters >= 0, "bad size or iteration count");
-    const size_t n = (size_t)width * height;
-    float* R = d_work;
-    float* Px = d_work + n;
-    float* Py = d_work + 2 * n;
-    float* raw = d_work + 3 * n;
-    float* T11 = d_work + 4 * n;
-    float* T12 = d_work + 5 * n;
-    float* T21 = d_work + 6 * n;
-    float* T22 = d_work + 7 * n;
-    const psw_dim3 z = {0, 0, 0};
-    int st;
-#define TRY(x) do { if ((st = (x)) != PSW_OK) return st; } while (0)
-    TRY(psw_fill_f32(R, 0.f, 3 * n, s));  // R, Px, Py start at zero
-    TRY(psw_memcpy_dtod(d_out, d_depth, n * sizeof(float), s));
-    TRY(psw_memcpy_dtod(raw, d_depth, n * sizeof(float), s));
-    // normalised reference image in Px for the tensor, then Px := 0 again
-    TRY(psw_memcpy_dtod(Px, d_ref, n * sizeof(float), s));
-    TRY(psw_element_scale(Px, 1 / 255.f, width, height, z, z, s));
-    TRY(psw_Anisotropic_diffusion_tensor(T11, T12, T21, T22, Px, (float)beta, (float)gamma, width, height, z, z, s));
-    TRY(psw_fill_f32(Px, 0.f, n, s));
-    TRY(psw_element_add(d_out, -znear, width, height, z, z, s));
-    TRY(psw_element_add(raw, -znear, width, height, z, z, s));
-    const double xscale = 1.f / (zfar - znear);
-    const double inputscale = -sigma / (double)(zfar - znear);
-    TRY(psw_element_scale(d_out, (float)xscale, width, height, z, z, s));
-    TRY(psw_element_scale(raw, (float)inputscale, width, height, z, z, s));
-    for (int i = 0; i < niters; i++) {
-        const double currsigma = i == 0 ? 1 + sigma : sigma;
-        TRY(psw_denoising_TVL1_calculateP_tensor_weighed(Px, Py, T11, T12, T21, T22, d_out, (float)currsigma, width,
-                                                         height, z, z, s));
-        TRY(psw_denoising_TVL1_update(d_out, R, Px, Py, raw, (float)tau, (float)theta, (float)lambda, (float)sigma,
-                                      width, height, z, z, s));
-    }
-    TRY(psw_element_scale(d_out, zfar - znear, width, height, z, z, s));
-    TRY(psw_element_add(d_out, znear, width, height, z, z, s));
-#undef TRY
-    return PSW_OK;
-}

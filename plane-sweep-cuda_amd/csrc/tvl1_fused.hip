@@ -1,0 +1,303 @@
+// tvl1_fused.hip -- TVL1 depth denoising, PlaneSweep::CudaDenoise
+// (src/planesweep.cpp:297-379): the tensor-weighted P step
+// (src/kernels.cu:283-307) and the plain update (src/kernels.cu:342-370),
+// iterated niters times on a depthmap scaled to [0, 1] over [znear, zfar].
+//
+// The reference launches the two steps separately for every iteration.  Here
+// one launch does one or two whole iterations of a 64x8 tile (64x4 and 64x16
+// selectable): the tile's depth and duals are staged in LDS with a halo of
+// one pixel per iteration, the P step and the update alternate between
+// barriers in LDS, and only the last iteration's state is written back.  Neighbouring tiles read
+// each other's halo, so the state is double-buffered: a launch reads slot rd
+// and writes slot 1 - rd.
+//
+// The per-pixel arithmetic is the launchers' (ref_kernels.hip) operation by
+// operation, double where the reference declares double temporaries, no FMA
+// contraction, so the result is bit-identical to driving the launchers in
+// the reference's host sequence (PSW_TVL1_FUSED=0 does exactly that).
+//
+// Per pixel and launch the kernel reads depth, Px, Py, R, the scaled input
+// and the four tensor entries (36 B) and writes depth, Px, Py, R (16 B).
+#include "psw_common.h"
+
+#include <cstdlib>
+
+namespace psw {
+namespace {
+
+constexpr int TV_X = 64;
+
+struct TvlState {
+    float* out[2];
+    float* px[2];
+    float* py[2];
+    float* r[2];
+    const float* raw;  // scaled input depth, -sigma (d - znear) / (zfar - znear)
+    const float* t11;
+    const float* t12;
+    const float* t21;
+    const float* t22;
+    int w, h;
+    float tau, theta, lambda, sigma;
+};
+
+// P step at one pixel (kernels.cu:283-307).  c is the depth at the pixel, e
+// and s its right and lower neighbours, already clamped to the image.
+__device__ __forceinline__ void tvl1_p(float& px, float& py, float c, float e, float s, float t11, float t12,
+                                       float t21, float t22, float sg) {
+    const double x = (double)(e - c);
+    const double y = (double)(s - c);
+    const double dx = (double)px + (double)sg * ((double)t11 * x + (double)t12 * y);
+    const double dy = (double)py + (double)sg * ((double)t21 * x + (double)t22 * y);
+    const double n2 = dx * dx + dy * dy;
+    if (n2 <= 1.0) {   // sqrt(n2) <= 1, so the divisor is exactly 1 and the quotients are dx, dy
+        px = (float)dx;
+        py = (float)dy;
+        return;
+    }
+    const double d = (double)fmaxf(1.f, (float)sqrt(n2));
+    px = (float)(dx / d);
+    py = (float)(dy / d);
+}
+
+// Update at one pixel (kernels.cu:342-370): the data dual r, then the
+// over-relaxed depth.  pxl / pyu are Px left and Py above (Py above is the
+// pixel's own on the first row); the first column has no x term.
+__device__ __forceinline__ float tvl1_u(float o, float& r, float raw, float pxc, float pxl, float pyc, float pyu,
+                                        bool first_col, float tau, float theta, float lambda, float sigma) {
+    r += raw;
+    r += sigma * o;
+    if (r > lambda) r = lambda;
+    if (r < -lambda) r = -lambda;
+    double x_new;
+    if (first_col) x_new = (double)(o + tau * (pyc - pyu) - tau * r);
+    else x_new = (double)(o + tau * (pxc - pxl + pyc - pyu) - tau * r);
+    return (float)(x_new + (double)theta * (x_new - (double)o));
+}
+
+// M iterations (1 or 2) on one tile.  Region coordinates are local to the
+// tile's staging window, which starts M pixels up and left of the tile: the
+// P step of iteration k covers [k, R-2-k], the update [k+1, R-2-k], so the
+// last update covers exactly the tile.  Pixels outside the image are never
+// computed, and no pixel inside it reads one (the reference clamps its
+// forward differences and drops the backward ones at the first row/column).
+// The tensor is loaded by the first P step and kept in LDS for the second,
+// R by the first update; the scaled input is read where it is used.
+//
+// Measured at 640x480 (two iterations per launch): 8.9 us.  Three
+// restructurings were slower with identical results: every input staged in
+// the first phase (9.5 us; 9.4 us with all loads issued before the first LDS
+// store) and branch-free fixed passes that let the compiler interleave a
+// thread's two positions (9.3 us).  The kernel moves 16.6 MB per launch
+// (11.7 MB read, 4.9 MB written), so it is bound by its five dependent
+// phases, not by HBM.
+template <int M, int TV_Y>
+__global__ __launch_bounds__(TV_X * TV_Y) void tvl1_iter_kernel(const TvlState a, int rd, float sp0, float sp1) {
+    constexpr int TV_NT = TV_X * TV_Y;
+    constexpr int RX = TV_X + 2 * M, RY = TV_Y + 2 * M, RN = RX * RY;
+    __shared__ float s_o[RN], s_px[RN], s_py[RN];
+    __shared__ float s_r[M > 1 ? RN : 1];
+    __shared__ float s_t[M > 1 ? 4 * RN : 1];
+    const int W = a.w, H = a.h;
+    const int tx = (W + TV_X - 1) / TV_X, ty = (H + TV_Y - 1) / TV_Y;
+    const int t = xcd_tile(blockIdx.x, tx * ty);
+    const int gx0 = (t % tx) * TV_X - M, gy0 = (t / tx) * TV_Y - M;  // image position of local (0, 0)
+    const int wr = 1 - rd;
+    {
+        const float* __restrict__ go = a.out[rd];
+        const float* __restrict__ gpx = a.px[rd];
+        const float* __restrict__ gpy = a.py[rd];
+        for (int i = threadIdx.x; i < RN; i += TV_NT) {
+            const int ly = i / RX, lx = i - ly * RX;
+            const int gx = gx0 + lx, gy = gy0 + ly;
+            if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+                const size_t g = (size_t)gy * W + gx;
+                s_o[i] = go[g];
+                s_px[i] = gpx[g];
+                s_py[i] = gpy[g];
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const float sg = k == 0 ? sp0 : sp1;
+        {   // P step over [k, RX-2-k] x [k, RY-2-k]
+            const int PX = RX - 1 - 2 * k, PY = RY - 1 - 2 * k;
+            for (int i = threadIdx.x; i < PX * PY; i += TV_NT) {
+                const int ly = k + i / PX, lx = k + i % PX;
+                const int gx = gx0 + lx, gy = gy0 + ly;
+                if (gx < 0 || gx >= W || gy < 0 || gy >= H) continue;
+                const int li = ly * RX + lx;
+                const float c = s_o[li];
+                const float e = gx + 1 < W ? s_o[li + 1] : c;
+                const float s = gy + 1 < H ? s_o[li + RX] : c;
+                float t11, t12, t21, t22;
+                if (k == 0) {
+                    const size_t g = (size_t)gy * W + gx;
+                    t11 = a.t11[g];
+                    t12 = a.t12[g];
+                    t21 = a.t21[g];
+                    t22 = a.t22[g];
+                    if (M > 1) {
+                        s_t[li] = t11;
+                        s_t[RN + li] = t12;
+                        s_t[2 * RN + li] = t21;
+                        s_t[3 * RN + li] = t22;
+                    }
+                } else {
+                    t11 = s_t[li];
+                    t12 = s_t[RN + li];
+                    t21 = s_t[2 * RN + li];
+                    t22 = s_t[3 * RN + li];
+                }
+                tvl1_p(s_px[li], s_py[li], c, e, s, t11, t12, t21, t22, sg);
+            }
+        }
+        __syncthreads();
+        {   // update over [k+1, RX-2-k] x [k+1, RY-2-k]
+            const int UX = RX - 2 - 2 * k, UY = RY - 2 - 2 * k;
+            for (int i = threadIdx.x; i < UX * UY; i += TV_NT) {
+                const int ly = k + 1 + i / UX, lx = k + 1 + i % UX;
+                const int gx = gx0 + lx, gy = gy0 + ly;
+                if (gx < 0 || gx >= W || gy < 0 || gy >= H) continue;
+                const int li = ly * RX + lx;
+                const size_t g = (size_t)gy * W + gx;
+                float r = k == 0 ? a.r[rd][g] : s_r[li];
+                const float pxc = s_px[li], pyc = s_py[li];
+                const float pxl = gx > 0 ? s_px[li - 1] : pxc;
+                const float pyu = gy > 0 ? s_py[li - RX] : pyc;
+                const float nv = tvl1_u(s_o[li], r, a.raw[g], pxc, pxl, pyc, pyu, gx == 0, a.tau, a.theta, a.lambda,
+                                        a.sigma);
+                if (k == M - 1) {
+                    a.out[wr][g] = nv;
+                    a.r[wr][g] = r;
+                    a.px[wr][g] = pxc;
+                    a.py[wr][g] = pyc;
+                } else {
+                    s_o[li] = nv;
+                    s_r[li] = r;
+                }
+            }
+        }
+        if (k < M - 1) __syncthreads();
+    }
+}
+
+template <int TY>
+int tvl1_launch_ty(const TvlState& tv, int m, int rd, float sp0, float sp1, hipStream_t s) {
+    const int tiles = ((tv.w + TV_X - 1) / TV_X) * ((tv.h + TY - 1) / TY);
+    if (m == 2) {
+        tvl1_iter_kernel<2, TY><<<tiles, TV_X * TY, 0, s>>>(tv, rd, sp0, sp1);
+        return launch_status("tvl1_iter_kernel<2>");
+    }
+    tvl1_iter_kernel<1, TY><<<tiles, TV_X * TY, 0, s>>>(tv, rd, sp0, sp1);
+    return launch_status("tvl1_iter_kernel<1>");
+}
+
+// Tile height (identical results): 8 (512 threads, the default), 4 or 16,
+// PSW_TVL1_TY.
+int tvl1_ty() {
+    const char* e = std::getenv("PSW_TVL1_TY");
+    const int v = e ? std::atoi(e) : 8;
+    return v == 4 || v == 16 ? v : 8;
+}
+
+int tvl1_launch(const TvlState& tv, int m, int rd, float sp0, float sp1, hipStream_t s) {
+    switch (tvl1_ty()) {
+        case 4: return tvl1_launch_ty<4>(tv, m, rd, sp0, sp1, s);
+        case 16: return tvl1_launch_ty<16>(tv, m, rd, sp0, sp1, s);
+        default: return tvl1_launch_ty<8>(tv, m, rd, sp0, sp1, s);
+    }
+}
+
+// Fused iterations unless PSW_TVL1_FUSED=0 (the reference's launch sequence;
+// identical results).
+bool tvl1_fused() {
+    const char* e = std::getenv("PSW_TVL1_FUSED");
+    return !(e && std::atoi(e) == 0);
+}
+
+}  // namespace
+}  // namespace psw
+
+using namespace psw;
+
+// PlaneSweep::CudaDenoise (src/planesweep.cpp:297-379).  Workspace layout
+// (n = width * height floats each): R, Px, Py, scaled input, T11, T12, T21,
+// T22, then the second slot of R, Px, Py and the depth.
+extern "C" int psw_tvl1_denoise(const float* d_depth, const float* d_ref, int width, int height, float znear,
+                                float zfar, int niters, double lambda, double tau, double sigma, double theta,
+                                double beta, double gamma, float* d_out, float* d_work, psw_stream_t s) {
+    PSW_CHECK_ARG(d_depth && d_ref && d_out && d_work, "NULL pointer");
+    PSW_CHECK_ARG(width >= 2 && height >= 2 && niters >= 0, "bad size or iteration count");
+    const size_t n = (size_t)width * height;
+    float* R = d_work;
+    float* Px = d_work + n;
+    float* Py = d_work + 2 * n;
+    float* raw = d_work + 3 * n;
+    float* T11 = d_work + 4 * n;
+    float* T12 = d_work + 5 * n;
+    float* T21 = d_work + 6 * n;
+    float* T22 = d_work + 7 * n;
+    const psw_dim3 z = {0, 0, 0};
+    int st;
+#define TRY(x) do { if ((st = (x)) != PSW_OK) return st; } while (0)
+    TRY(psw_fill_f32(R, 0.f, 3 * n, s));  // R, Px, Py start at zero
+    TRY(psw_memcpy_dtod(d_out, d_depth, n * sizeof(float), s));
+    TRY(psw_memcpy_dtod(raw, d_depth, n * sizeof(float), s));
+    // normalised reference image in Px for the tensor, then Px := 0 again
+    TRY(psw_memcpy_dtod(Px, d_ref, n * sizeof(float), s));
+    TRY(psw_element_scale(Px, 1 / 255.f, width, height, z, z, s));
+    TRY(psw_Anisotropic_diffusion_tensor(T11, T12, T21, T22, Px, (float)beta, (float)gamma, width, height, z, z, s));
+    TRY(psw_fill_f32(Px, 0.f, n, s));
+    TRY(psw_element_add(d_out, -znear, width, height, z, z, s));
+    TRY(psw_element_add(raw, -znear, width, height, z, z, s));
+    const double xscale = 1.f / (zfar - znear);
+    const double inputscale = -sigma / (double)(zfar - znear);
+    TRY(psw_element_scale(d_out, (float)xscale, width, height, z, z, s));
+    TRY(psw_element_scale(raw, (float)inputscale, width, height, z, z, s));
+    if (tvl1_fused()) {
+        TvlState tv;
+        tv.out[0] = d_out;
+        tv.out[1] = d_work + 11 * n;
+        tv.px[0] = Px;
+        tv.px[1] = d_work + 9 * n;
+        tv.py[0] = Py;
+        tv.py[1] = d_work + 10 * n;
+        tv.r[0] = R;
+        tv.r[1] = d_work + 8 * n;
+        tv.raw = raw;
+        tv.t11 = T11;
+        tv.t12 = T12;
+        tv.t21 = T21;
+        tv.t22 = T22;
+        tv.w = width;
+        tv.h = height;
+        tv.tau = (float)tau;
+        tv.theta = (float)theta;
+        tv.lambda = (float)lambda;
+        tv.sigma = (float)sigma;
+        int rd = 0;
+        for (int i = 0; i < niters;) {
+            const int m = niters - i >= 2 ? 2 : 1;
+            const float sp0 = (float)(i == 0 ? 1 + sigma : sigma);
+            TRY(tvl1_launch(tv, m, rd, sp0, (float)sigma, S(s)));
+            rd = 1 - rd;
+            i += m;
+        }
+        if (rd == 1) TRY(psw_memcpy_dtod(d_out, tv.out[1], n * sizeof(float), s));
+    } else {
+        for (int i = 0; i < niters; i++) {
+            const double currsigma = i == 0 ? 1 + sigma : sigma;
+            TRY(psw_denoising_TVL1_calculateP_tensor_weighed(Px, Py, T11, T12, T21, T22, d_out, (float)currsigma,
+                                                             width, height, z, z, s));
+            TRY(psw_denoising_TVL1_update(d_out, R, Px, Py, raw, (float)tau, (float)theta, (float)lambda,
+                                          (float)sigma, width, height, z, z, s));
+        }
+    }
+    TRY(psw_element_scale(d_out, zfar - znear, width, height, z, z, s));
+    TRY(psw_element_add(d_out, znear, width, height, z, z, s));
+#undef TRY
+    return PSW_OK;
+}

@@ -233,7 +233,7 @@ bool PlaneSweep::CudaDenoise(int argc, char** argv, const unsigned int niters, c
     int st = psw_malloc(&p, n * 4);
     if (st != PSW_OK) return fail("CudaDenoise", st);
     d_depthmap = static_cast<float*>(p);
-    DevBuf din(n * 4), dref(n * 4), work(8 * n * 4);
+    DevBuf din(n * 4), dref(n * 4), work(12 * n * 4);
     for (DevBuf* b : {&din, &dref, &work})
         if (b->status != PSW_OK) return fail("CudaDenoise", b->status);
     st = psw_memcpy_htod(din.p, depthmap.data(), n * 4, nullptr);

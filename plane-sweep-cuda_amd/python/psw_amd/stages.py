@@ -383,7 +383,15 @@ def tvl1_denoise(depth, ref, znear, zfar, niters=100, lambda_=0.3, tau=0.02, sig
     depth, ref = _f32(depth), _f32(ref)
     Hh, W = depth.shape
     d_dep, d_ref = DeviceArray.from_host(depth), DeviceArray.from_host(ref)
-    d_out, d_work = DeviceArray((Hh, W)), DeviceArray((8, Hh, W))
-    call("psw_tvl1_denoise", d_dep.ptr, d_ref.ptr, W, Hh, float(znear), float(zfar), int(niters), float(lambda_),
-         float(tau), float(sigma), float(theta), float(beta), float(gamma), d_out.ptr, d_work.ptr, None)
+    d_out, d_work = DeviceArray((Hh, W)), DeviceArray((12, Hh, W))
+    tvl1_denoise_device(d_dep, d_ref, W, Hh, znear, zfar, d_out, d_work, niters, lambda_, tau, sigma, theta, beta,
+                        gamma)
     return d_out.to_host()
+
+
+def tvl1_denoise_device(d_depth, d_ref, W, Hh, znear, zfar, d_out, d_work, niters=100, lambda_=0.3, tau=0.02,
+                        sigma=6.0, theta=1.0, beta=0.0, gamma=1.0, stream=None):
+    """psw_tvl1_denoise on device buffers, asynchronous on `stream`; d_work
+    holds 12*W*H floats."""
+    call("psw_tvl1_denoise", d_depth.ptr, d_ref.ptr, W, Hh, float(znear), float(zfar), int(niters), float(lambda_),
+         float(tau), float(sigma), float(theta), float(beta), float(gamma), d_out.ptr, d_work.ptr, stream)
