@@ -1184,6 +1184,18 @@ static int tgv_variant() {
     return (x < 0 || x > 2) ? 1 : x;
 }
 
+// Chunks of check_every iterations the single-frame solver keeps in flight
+// before it waits for the oldest (PSW_TGV_INFLIGHT, 2..8; identical results:
+// the chunks enqueued after the stop run as no-ops).  At 640x480 x 8 sources
+// depths 2, 3, 4 and 6 measured the same (53.04-53.09 ms to the gap from the
+// learned start): two chunks already keep the device busy.
+constexpr int TG_MAX_INFLIGHT = 8;
+static int tgv_inflight() {
+    const char* e = getenv("PSW_TGV_INFLIGHT");
+    const int v = e ? atoi(e) : 2;
+    return v < 2 ? 2 : v > TG_MAX_INFLIGHT ? TG_MAX_INFLIGHT : v;
+}
+
 // Two iterations per launch (tgv_iter2_kernel) unless PSW_TGV_DOUBLE=0;
 // identical results either way.
 static bool tgv_double() {
@@ -1835,12 +1847,13 @@ int psw_tgv_solve(psw_tgv* t, int max_iters, int check_every, double tol, double
     // after the first sums the energy of the state the previous chunk ended
     // at (into one of two partial buffers, alternately); the next launch
     // finalises it and writes the outcome to pinned host memory.  At most
-    // two chunks are in flight: the host waits for the chunk two back, so the
-    // device never idles and at most two chunks run as no-ops (they test
-    // ctrl->done and return) after the stop.
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    PSW_TRY_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming), PSW_ERR_DEVICE);
-    PSW_TRY_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming), PSW_ERR_DEVICE);
+    // `depth` chunks are in flight: the host waits for the chunk `depth` back,
+    // so the device never idles and at most `depth` chunks run as no-ops (they
+    // test ctrl->done and return) after the stop.
+    const int depth = tgv_inflight();
+    hipEvent_t ev[TG_MAX_INFLIGHT] = {};
+    for (int i = 0; i < depth && rc == PSW_OK; ++i)
+        if (hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess) rc = PSW_ERR_DEVICE;
     // A check is an energy launch from state k and the finalising launch
     // after it; both do the same number of iterations, so a stop at k leaves
     // u_k, u_{k-1}, p_k, q_k, r_k in place (slot counts at TG_USLOTS).  The
@@ -1884,9 +1897,9 @@ int psw_tgv_solve(psw_tgv* t, int max_iters, int check_every, double tol, double
         }
         if (rc != PSW_OK) break;
         launched += c;
-        hipEvent_t e = ev[chunk_idx & 1];
-        if (chunk_idx >= 2) {
-            // the event slot holds the chunk two back: wait for it
+        hipEvent_t e = ev[chunk_idx % depth];
+        if (chunk_idx >= depth) {
+            // the event slot holds the chunk `depth` back: wait for it
             if (hipEventSynchronize(e) != hipSuccess) { rc = PSW_ERR_DEVICE; break; }
             if (mirror->done) { done_iters = mirror->iters_at_done; stopped = true; break; }
         }
@@ -1906,8 +1919,8 @@ int psw_tgv_solve(psw_tgv* t, int max_iters, int check_every, double tol, double
     if (rc == PSW_OK && !stopped) rc = tgv_launch_energy(t, prm, tol, e_star, st);
     (void)hipStreamSynchronize(st);
     (void)hipMemcpy(t->hctrl, t->ctrl, sizeof(TgvCtrl), hipMemcpyDeviceToHost);
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
+    for (int i = 0; i < depth; ++i)
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
     if (rc != PSW_OK) return rc;
     if (t->hctrl->done) done_iters = t->hctrl->iters_at_done;
     if (done_iters >= 0) t->k = done_iters;  // later kernels were no-ops

@@ -371,17 +371,18 @@ def test_tgv_solve_stops_at_gap(gpu, views_small):
         np.testing.assert_array_equal(sa[name], sb[name], err_msg=name)
 
 
-@pytest.mark.parametrize("variant,double,ty2,triple", [("0", "1", "10", "0"), ("1", "1", "10", "0"),
-                                                      ("2", "1", "10", "0"), ("1", "0", "10", "0"),
-                                                      ("1", "1", "4", "0"), ("1", "1", "8", "0"),
-                                                      ("1", "1", "12", "0"), ("1", "1", "10", "1"),
-                                                      ("2", "1", "8", "1")])
-def test_tgv_solve_max_iters_and_variants(gpu, views_small, monkeypatch, variant, double, ty2, triple):
+@pytest.mark.parametrize("variant,double,ty2,triple,inflight", [
+    ("0", "1", "10", "0", "2"), ("1", "1", "10", "0", "2"), ("2", "1", "10", "0", "2"), ("1", "0", "10", "0", "2"),
+    ("1", "1", "4", "0", "2"), ("1", "1", "8", "0", "2"), ("1", "1", "12", "0", "2"), ("1", "1", "10", "1", "2"),
+    ("2", "1", "8", "1", "2"), ("1", "1", "10", "0", "5"), ("1", "0", "10", "0", "8")])
+def test_tgv_solve_max_iters_and_variants(gpu, views_small, monkeypatch, variant, double, ty2, triple, inflight):
     """Every launch shape, with and without two- and three-iteration
-    launches, stops at the same iteration with the same state; a solve that
-    reaches max_iters reports max_iters and the energy of the final state."""
+    launches, and with 2, 5 or 8 chunks in flight, stops at the same
+    iteration with the same state; a solve that reaches max_iters reports
+    max_iters and the energy of the final state."""
     from psw_amd import stages
     from psw_amd.device import DeviceArray
+    monkeypatch.setenv("PSW_TGV_INFLIGHT", inflight)
     monkeypatch.setenv("PSW_TGV_VARIANT", variant)
     monkeypatch.setenv("PSW_TGV_DOUBLE", double)
     monkeypatch.setenv("PSW_TGV2_TY", ty2)
