@@ -203,8 +203,8 @@ int tvl1_ty() {
     return v == 4 || v == 16 ? v : 8;
 }
 
-int tvl1_launch(const TvlState& tv, int m, int rd, float sp0, float sp1, hipStream_t s) {
-    switch (tvl1_ty()) {
+int tvl1_launch(const TvlState& tv, int ty, int m, int rd, float sp0, float sp1, hipStream_t s) {
+    switch (ty) {
         case 4: return tvl1_launch_ty<4>(tv, m, rd, sp0, sp1, s);
         case 16: return tvl1_launch_ty<16>(tv, m, rd, sp0, sp1, s);
         default: return tvl1_launch_ty<8>(tv, m, rd, sp0, sp1, s);
@@ -278,11 +278,12 @@ extern "C" int psw_tvl1_denoise(const float* d_depth, const float* d_ref, int wi
         tv.theta = (float)theta;
         tv.lambda = (float)lambda;
         tv.sigma = (float)sigma;
+        const int ty = tvl1_ty();
         int rd = 0;
         for (int i = 0; i < niters;) {
             const int m = niters - i >= 2 ? 2 : 1;
             const float sp0 = (float)(i == 0 ? 1 + sigma : sigma);
-            TRY(tvl1_launch(tv, m, rd, sp0, (float)sigma, S(s)));
+            TRY(tvl1_launch(tv, ty, m, rd, sp0, (float)sigma, S(s)));
             rd = 1 - rd;
             i += m;
         }
