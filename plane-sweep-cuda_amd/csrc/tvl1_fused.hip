@@ -84,13 +84,14 @@ __device__ __forceinline__ float tvl1_u(float o, float& r, float raw, float pxc,
 // The tensor is loaded by the first P step and kept in LDS for the second,
 // R by the first update; the scaled input is read where it is used.
 //
-// Measured at 640x480 (two iterations per launch): 8.9 us.  Three
+// Measured at 640x480 (two iterations per launch): 8.6 us.  Three
 // restructurings were slower with identical results: every input staged in
 // the first phase (9.5 us; 9.4 us with all loads issued before the first LDS
 // store) and branch-free fixed passes that let the compiler interleave a
 // thread's two positions (9.3 us).  The kernel moves 16.6 MB per launch
-// (11.7 MB read, 4.9 MB written), so it is bound by its five dependent
-// phases, not by HBM.
+// (11.7 MB read, 4.9 MB written), far below what HBM allows in that time.
+// The three-phase kernel below is 3.5 % faster and is the default for
+// two-iteration launches on tiles up to 64x8.
 template <int M, int TV_Y>
 __global__ __launch_bounds__(TV_X * TV_Y) void tvl1_iter_kernel(const TvlState a, int rd, float sp0, float sp1) {
     constexpr int TV_NT = TV_X * TV_Y;
@@ -184,9 +185,110 @@ __global__ __launch_bounds__(TV_X * TV_Y) void tvl1_iter_kernel(const TvlState a
     }
 }
 
+// Two iterations in three phases.  The update is cheap next to the P step
+// (no square root or division), so the first update is not given a phase of
+// its own: the thread of each position of the second P step's region
+// evaluates it at its position and at its right and lower neighbours, the
+// three values the second P step reads, and keeps its own.  The first P step
+// reads its neighbours' depth from global memory (cache hits), so no staging
+// phase precedes it either.  Phases: P1 (global -> LDS), U1 x3 + P2, U2
+// (LDS -> global).  Every evaluation is the per-pixel arithmetic above, so
+// the results are those of tvl1_iter_kernel<2>.  Local (0, 0) is two pixels
+// up and left of the tile.
+template <int TV_Y>
+__global__ __launch_bounds__(TV_X * TV_Y) void tvl1_iter2m_kernel(const TvlState a, int rd, float sp0, float sp1) {
+    constexpr int TV_NT = TV_X * TV_Y;
+    constexpr int RX = TV_X + 4, RY = TV_Y + 4, RN = RX * RY;
+    __shared__ float s_o[RN], s_px[RN], s_py[RN], s_r[RN], s_in[RN];    // state k, P1, inputs: [0, R-2]
+    __shared__ float s_t[4][RN];
+    __shared__ float s_px2[RN], s_py2[RN], s_o1[RN], s_r1[RN];          // P2, depth and R of k+1: [1, R-3]
+    const int W = a.w, H = a.h;
+    const int tx = (W + TV_X - 1) / TV_X, ty = (H + TV_Y - 1) / TV_Y;
+    const int t = xcd_tile(blockIdx.x, tx * ty);
+    const int gx0 = (t % tx) * TV_X - 2, gy0 = (t / tx) * TV_Y - 2;
+    const int wr = 1 - rd;
+    {   // P1 over [0, RX-2] x [0, RY-2]
+        const float* __restrict__ go = a.out[rd];
+        constexpr int PX = RX - 1, PY = RY - 1;
+        for (int i = threadIdx.x; i < PX * PY; i += TV_NT) {
+            const int ly = i / PX, lx = i - ly * PX;
+            const int gx = gx0 + lx, gy = gy0 + ly;
+            if (gx < 0 || gx >= W || gy < 0 || gy >= H) continue;
+            const size_t g = (size_t)gy * W + gx;
+            const float c = go[g];
+            const float e = gx + 1 < W ? go[g + 1] : c;
+            const float s = gy + 1 < H ? go[g + W] : c;
+            const float t11 = a.t11[g], t12 = a.t12[g], t21 = a.t21[g], t22 = a.t22[g];
+            float px = a.px[rd][g], py = a.py[rd][g];
+            tvl1_p(px, py, c, e, s, t11, t12, t21, t22, sp0);
+            const int li = ly * RX + lx;
+            s_o[li] = c;
+            s_px[li] = px;
+            s_py[li] = py;
+            s_t[0][li] = t11;
+            s_t[1][li] = t12;
+            s_t[2][li] = t21;
+            s_t[3][li] = t22;
+            s_r[li] = a.r[rd][g];
+            s_in[li] = a.raw[g];
+        }
+    }
+    __syncthreads();
+    {   // U1 at each position and its right and lower neighbours, P2 at the position: [1, RX-3] x [1, RY-3]
+        constexpr int PX = RX - 3, PY = RY - 3;
+        auto u1 = [&](int lj, int gxj, int gyj, float& r) {
+            r = s_r[lj];
+            const float pxc = s_px[lj], pyc = s_py[lj];
+            const float pxl = gxj > 0 ? s_px[lj - 1] : pxc;
+            const float pyu = gyj > 0 ? s_py[lj - RX] : pyc;
+            return tvl1_u(s_o[lj], r, s_in[lj], pxc, pxl, pyc, pyu, gxj == 0, a.tau, a.theta, a.lambda, a.sigma);
+        };
+        for (int i = threadIdx.x; i < PX * PY; i += TV_NT) {
+            const int ly = 1 + i / PX, lx = 1 + i % PX;
+            const int gx = gx0 + lx, gy = gy0 + ly;
+            if (gx < 0 || gx >= W || gy < 0 || gy >= H) continue;
+            const int li = ly * RX + lx;
+            float r1, rn;
+            const float c1 = u1(li, gx, gy, r1);
+            const float e1 = gx + 1 < W ? u1(li + 1, gx + 1, gy, rn) : c1;
+            const float s1 = gy + 1 < H ? u1(li + RX, gx, gy + 1, rn) : c1;
+            float px = s_px[li], py = s_py[li];
+            tvl1_p(px, py, c1, e1, s1, s_t[0][li], s_t[1][li], s_t[2][li], s_t[3][li], sp1);
+            s_px2[li] = px;
+            s_py2[li] = py;
+            s_o1[li] = c1;
+            s_r1[li] = r1;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < TV_NT; i += TV_NT) {   // U2 over the tile, to global memory
+        const int ly = 2 + i / TV_X, lx = 2 + i % TV_X;
+        const int gx = gx0 + lx, gy = gy0 + ly;
+        if (gx >= W || gy >= H) continue;
+        const int li = ly * RX + lx;
+        float r = s_r1[li];
+        const float pxc = s_px2[li], pyc = s_py2[li];
+        const float pxl = gx > 0 ? s_px2[li - 1] : pxc;
+        const float pyu = gy > 0 ? s_py2[li - RX] : pyc;
+        const float nv = tvl1_u(s_o1[li], r, s_in[li], pxc, pxl, pyc, pyu, gx == 0, a.tau, a.theta, a.lambda,
+                                a.sigma);
+        const size_t g = (size_t)gy * W + gx;
+        a.out[wr][g] = nv;
+        a.r[wr][g] = r;
+        a.px[wr][g] = pxc;
+        a.py[wr][g] = pyc;
+    }
+}
+
 template <int TY>
-int tvl1_launch_ty(const TvlState& tv, int m, int rd, float sp0, float sp1, hipStream_t s) {
+int tvl1_launch_ty(const TvlState& tv, bool merged, int m, int rd, float sp0, float sp1, hipStream_t s) {
     const int tiles = ((tv.w + TV_X - 1) / TV_X) * ((tv.h + TY - 1) / TY);
+    if constexpr (TY <= 8) {   // the three-phase kernel's LDS (13 arrays) fits 64 KB up to 64x8 tiles
+        if (m == 2 && merged) {
+            tvl1_iter2m_kernel<TY><<<tiles, TV_X * TY, 0, s>>>(tv, rd, sp0, sp1);
+            return launch_status("tvl1_iter2m_kernel");
+        }
+    }
     if (m == 2) {
         tvl1_iter_kernel<2, TY><<<tiles, TV_X * TY, 0, s>>>(tv, rd, sp0, sp1);
         return launch_status("tvl1_iter_kernel<2>");
@@ -203,11 +305,18 @@ int tvl1_ty() {
     return v == 4 || v == 16 ? v : 8;
 }
 
-int tvl1_launch(const TvlState& tv, int ty, int m, int rd, float sp0, float sp1, hipStream_t s) {
+// Two-iteration launches in three phases (tvl1_iter2m_kernel) unless
+// PSW_TVL1_MERGED=0; identical results.
+bool tvl1_merged() {
+    const char* e = std::getenv("PSW_TVL1_MERGED");
+    return !(e && std::atoi(e) == 0);
+}
+
+int tvl1_launch(const TvlState& tv, int ty, bool merged, int m, int rd, float sp0, float sp1, hipStream_t s) {
     switch (ty) {
-        case 4: return tvl1_launch_ty<4>(tv, m, rd, sp0, sp1, s);
-        case 16: return tvl1_launch_ty<16>(tv, m, rd, sp0, sp1, s);
-        default: return tvl1_launch_ty<8>(tv, m, rd, sp0, sp1, s);
+        case 4: return tvl1_launch_ty<4>(tv, merged, m, rd, sp0, sp1, s);
+        case 16: return tvl1_launch_ty<16>(tv, merged, m, rd, sp0, sp1, s);
+        default: return tvl1_launch_ty<8>(tv, merged, m, rd, sp0, sp1, s);
     }
 }
 
@@ -279,11 +388,12 @@ extern "C" int psw_tvl1_denoise(const float* d_depth, const float* d_ref, int wi
         tv.lambda = (float)lambda;
         tv.sigma = (float)sigma;
         const int ty = tvl1_ty();
+        const bool merged = tvl1_merged();
         int rd = 0;
         for (int i = 0; i < niters;) {
             const int m = niters - i >= 2 ? 2 : 1;
             const float sp0 = (float)(i == 0 ? 1 + sigma : sigma);
-            TRY(tvl1_launch(tv, ty, m, rd, sp0, (float)sigma, S(s)));
+            TRY(tvl1_launch(tv, ty, merged, m, rd, sp0, (float)sigma, S(s)));
             rd = 1 - rd;
             i += m;
         }
