@@ -203,9 +203,8 @@ def run_tvl1(pl, stream, ev):
         else:
             os.environ["PSW_TVL1_FUSED"] = prev
     gbs = TVL1_BYTES_PER_PIXEL_ITER * W * H / (res["fused"] * 1e-3) / 1e9
-    merged = os.environ.get("PSW_TVL1_MERGED", "1") != "0"     # three-phase two-iteration kernel (default)
-    return {"kernel": "tvl1_iter2m_kernel" if merged else "tvl1_iter_kernel<2>", "iterations_per_launch": 2,
-            "iter_us": res["fused"] * 1e3,
+    m = int(os.environ.get("PSW_TVL1_M", "4"))                # iterations per launch (default 4, 56x8 tiles)
+    return {"kernel": f"tvl1_iter_kernel<{m}>", "iterations_per_launch": m, "iter_us": res["fused"] * 1e3,
             "launch_sequence_iter_us": res["launch_sequence"] * 1e3,
             "speedup_vs_launch_sequence": res["launch_sequence"] / res["fused"], "gbs_algorithmic": gbs,
             "frac_hbm_peak": gbs / HBM_PEAK_GBS, "bytes_per_pixel_iter": TVL1_BYTES_PER_PIXEL_ITER,

@@ -4,9 +4,9 @@
 // iterated niters times on a depthmap scaled to [0, 1] over [znear, zfar].
 //
 // The reference launches the two steps separately for every iteration.  Here
-// one launch does one or two whole iterations of a 64x8 tile (64x4 and 64x16
-// selectable): the tile's depth and duals are staged in LDS with a halo of
-// one pixel per iteration, the P step and the update alternate between
+// one launch does up to four whole iterations of a tile 8 rows high (4 and
+// 16 selectable) and 64 - 2M columns wide for M iterations: the tile's depth
+// and duals are staged in LDS with a halo of one pixel per iteration, the P step and the update alternate between
 // barriers in LDS, and only the last iteration's state is written back.  Neighbouring tiles read
 // each other's halo, so the state is double-buffered: a launch reads slot rd
 // and writes slot 1 - rd.
@@ -20,6 +20,7 @@
 // and the four tensor entries (36 B) and writes depth, Px, Py, R (16 B).
 #include "psw_common.h"
 
+#include <algorithm>
 #include <cstdlib>
 
 namespace psw {
@@ -92,17 +93,22 @@ __device__ __forceinline__ float tvl1_u(float o, float& r, float raw, float pxc,
 // (11.7 MB read, 4.9 MB written), far below what HBM allows in that time.
 // The three-phase kernel below is 3.5 % faster and is the default for
 // two-iteration launches on tiles up to 64x8.
-template <int M, int TV_Y>
+//
+// TX is the tile width; the block has 64 x TV_Y threads either way.  With
+// TX = 64 - 2M the staging window is 64 wide, so a position's window row and
+// column are a shift and a mask of its index, and each pass of the block
+// covers TV_Y whole rows (positions outside a phase's columns idle).
+template <int M, int TV_Y, int TX>
 __global__ __launch_bounds__(TV_X * TV_Y) void tvl1_iter_kernel(const TvlState a, int rd, float sp0, float sp1) {
     constexpr int TV_NT = TV_X * TV_Y;
-    constexpr int RX = TV_X + 2 * M, RY = TV_Y + 2 * M, RN = RX * RY;
+    constexpr int RX = TX + 2 * M, RY = TV_Y + 2 * M, RN = RX * RY;
     __shared__ float s_o[RN], s_px[RN], s_py[RN];
     __shared__ float s_r[M > 1 ? RN : 1];
     __shared__ float s_t[M > 1 ? 4 * RN : 1];
     const int W = a.w, H = a.h;
-    const int tx = (W + TV_X - 1) / TV_X, ty = (H + TV_Y - 1) / TV_Y;
+    const int tx = (W + TX - 1) / TX, ty = (H + TV_Y - 1) / TV_Y;
     const int t = xcd_tile(blockIdx.x, tx * ty);
-    const int gx0 = (t % tx) * TV_X - M, gy0 = (t / tx) * TV_Y - M;  // image position of local (0, 0)
+    const int gx0 = (t % tx) * TX - M, gy0 = (t / tx) * TV_Y - M;  // image position of local (0, 0)
     const int wr = 1 - rd;
     {
         const float* __restrict__ go = a.out[rd];
@@ -123,12 +129,12 @@ __global__ __launch_bounds__(TV_X * TV_Y) void tvl1_iter_kernel(const TvlState a
 #pragma unroll
     for (int k = 0; k < M; ++k) {
         const float sg = k == 0 ? sp0 : sp1;
-        {   // P step over [k, RX-2-k] x [k, RY-2-k]
-            const int PX = RX - 1 - 2 * k, PY = RY - 1 - 2 * k;
-            for (int i = threadIdx.x; i < PX * PY; i += TV_NT) {
-                const int ly = k + i / PX, lx = k + i % PX;
+        {   // P step over [k, RX-2-k] x [k, RY-2-k], by whole window rows
+            const int PY = RY - 1 - 2 * k;
+            for (int i = threadIdx.x; i < PY * RX; i += TV_NT) {
+                const int ly = k + i / RX, lx = i % RX;
                 const int gx = gx0 + lx, gy = gy0 + ly;
-                if (gx < 0 || gx >= W || gy < 0 || gy >= H) continue;
+                if (lx < k || lx > RX - 2 - k || gx < 0 || gx >= W || gy < 0 || gy >= H) continue;
                 const int li = ly * RX + lx;
                 const float c = s_o[li];
                 const float e = gx + 1 < W ? s_o[li + 1] : c;
@@ -156,12 +162,12 @@ __global__ __launch_bounds__(TV_X * TV_Y) void tvl1_iter_kernel(const TvlState a
             }
         }
         __syncthreads();
-        {   // update over [k+1, RX-2-k] x [k+1, RY-2-k]
-            const int UX = RX - 2 - 2 * k, UY = RY - 2 - 2 * k;
-            for (int i = threadIdx.x; i < UX * UY; i += TV_NT) {
-                const int ly = k + 1 + i / UX, lx = k + 1 + i % UX;
+        {   // update over [k+1, RX-2-k] x [k+1, RY-2-k], by whole window rows
+            const int UY = RY - 2 - 2 * k;
+            for (int i = threadIdx.x; i < UY * RX; i += TV_NT) {
+                const int ly = k + 1 + i / RX, lx = i % RX;
                 const int gx = gx0 + lx, gy = gy0 + ly;
-                if (gx < 0 || gx >= W || gy < 0 || gy >= H) continue;
+                if (lx < k + 1 || lx > RX - 2 - k || gx < 0 || gx >= W || gy < 0 || gy >= H) continue;
                 const int li = ly * RX + lx;
                 const size_t g = (size_t)gy * W + gx;
                 float r = k == 0 ? a.r[rd][g] : s_r[li];
@@ -280,21 +286,35 @@ __global__ __launch_bounds__(TV_X * TV_Y) void tvl1_iter2m_kernel(const TvlState
     }
 }
 
+template <int TY, int M>
+int tvl1_launch_m(const TvlState& tv, bool narrow, int rd, float sp0, float sp1, hipStream_t s) {
+    if (narrow) {
+        constexpr int TX = TV_X - 2 * M;   // a 64-wide staging window
+        const int tiles = ((tv.w + TX - 1) / TX) * ((tv.h + TY - 1) / TY);
+        tvl1_iter_kernel<M, TY, TX><<<tiles, TV_X * TY, 0, s>>>(tv, rd, sp0, sp1);
+    } else {
+        const int tiles = ((tv.w + TV_X - 1) / TV_X) * ((tv.h + TY - 1) / TY);
+        tvl1_iter_kernel<M, TY, TV_X><<<tiles, TV_X * TY, 0, s>>>(tv, rd, sp0, sp1);
+    }
+    return launch_status("tvl1_iter_kernel");
+}
+
 template <int TY>
-int tvl1_launch_ty(const TvlState& tv, bool merged, int m, int rd, float sp0, float sp1, hipStream_t s) {
-    const int tiles = ((tv.w + TV_X - 1) / TV_X) * ((tv.h + TY - 1) / TY);
+int tvl1_launch_ty(const TvlState& tv, bool merged, bool narrow, int m, int rd, float sp0, float sp1,
+                   hipStream_t s) {
     if constexpr (TY <= 8) {   // the three-phase kernel's LDS (13 arrays) fits 64 KB up to 64x8 tiles
-        if (m == 2 && merged) {
+        if (m == 2 && merged && !narrow) {
+            const int tiles = ((tv.w + TV_X - 1) / TV_X) * ((tv.h + TY - 1) / TY);
             tvl1_iter2m_kernel<TY><<<tiles, TV_X * TY, 0, s>>>(tv, rd, sp0, sp1);
             return launch_status("tvl1_iter2m_kernel");
         }
     }
-    if (m == 2) {
-        tvl1_iter_kernel<2, TY><<<tiles, TV_X * TY, 0, s>>>(tv, rd, sp0, sp1);
-        return launch_status("tvl1_iter_kernel<2>");
+    switch (m) {
+        case 4: return tvl1_launch_m<TY, 4>(tv, narrow, rd, sp0, sp1, s);
+        case 3: return tvl1_launch_m<TY, 3>(tv, narrow, rd, sp0, sp1, s);
+        case 2: return tvl1_launch_m<TY, 2>(tv, narrow, rd, sp0, sp1, s);
+        default: return tvl1_launch_m<TY, 1>(tv, narrow, rd, sp0, sp1, s);
     }
-    tvl1_iter_kernel<1, TY><<<tiles, TV_X * TY, 0, s>>>(tv, rd, sp0, sp1);
-    return launch_status("tvl1_iter_kernel<1>");
 }
 
 // Tile height (identical results): 8 (512 threads, the default), 4 or 16,
@@ -305,6 +325,15 @@ int tvl1_ty() {
     return v == 4 || v == 16 ? v : 8;
 }
 
+// Iterations per launch at most (PSW_TVL1_M: 2, 3 or 4, the default;
+// identical results).  At 640x480 with 64-wide staging windows: 3.99, 3.62
+// and 3.49 us per iteration for 2, 3 and 4.
+int tvl1_max_per_launch() {
+    const char* e = std::getenv("PSW_TVL1_M");
+    const int v = e ? std::atoi(e) : 4;
+    return v == 2 || v == 3 ? v : 4;
+}
+
 // Two-iteration launches in three phases (tvl1_iter2m_kernel) unless
 // PSW_TVL1_MERGED=0; identical results.
 bool tvl1_merged() {
@@ -312,11 +341,21 @@ bool tvl1_merged() {
     return !(e && std::atoi(e) == 0);
 }
 
-int tvl1_launch(const TvlState& tv, int ty, bool merged, int m, int rd, float sp0, float sp1, hipStream_t s) {
+// Tiles 64 - 2M wide, so that the staging window is 64 wide, unless
+// PSW_TVL1_NARROW=0 (64-wide tiles); identical results.  Two iterations per
+// launch measured 3.99 us per iteration on 60-wide tiles against 4.20 on
+// 64-wide ones (three-phase kernel), three 3.62 against 4.18.
+bool tvl1_narrow() {
+    const char* e = std::getenv("PSW_TVL1_NARROW");
+    return !(e && std::atoi(e) == 0);
+}
+
+int tvl1_launch(const TvlState& tv, int ty, bool merged, bool narrow, int m, int rd, float sp0, float sp1,
+                hipStream_t s) {
     switch (ty) {
-        case 4: return tvl1_launch_ty<4>(tv, merged, m, rd, sp0, sp1, s);
-        case 16: return tvl1_launch_ty<16>(tv, merged, m, rd, sp0, sp1, s);
-        default: return tvl1_launch_ty<8>(tv, merged, m, rd, sp0, sp1, s);
+        case 4: return tvl1_launch_ty<4>(tv, merged, narrow, m, rd, sp0, sp1, s);
+        case 16: return tvl1_launch_ty<16>(tv, merged, narrow, m, rd, sp0, sp1, s);
+        default: return tvl1_launch_ty<8>(tv, merged, narrow, m, rd, sp0, sp1, s);
     }
 }
 
@@ -389,11 +428,13 @@ extern "C" int psw_tvl1_denoise(const float* d_depth, const float* d_ref, int wi
         tv.sigma = (float)sigma;
         const int ty = tvl1_ty();
         const bool merged = tvl1_merged();
+        const int mmax = tvl1_max_per_launch();
+        const bool narrow = tvl1_narrow();
         int rd = 0;
         for (int i = 0; i < niters;) {
-            const int m = niters - i >= 2 ? 2 : 1;
+            const int m = std::min(mmax, niters - i);
             const float sp0 = (float)(i == 0 ? 1 + sigma : sigma);
-            TRY(tvl1_launch(tv, ty, merged, m, rd, sp0, (float)sigma, S(s)));
+            TRY(tvl1_launch(tv, ty, merged, narrow, m, rd, sp0, (float)sigma, S(s)));
             rd = 1 - rd;
             i += m;
         }

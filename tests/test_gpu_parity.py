@@ -575,12 +575,18 @@ def test_tvl1_vs_oracle(gpu, views_small):
     assert np.abs(got - e32).max() <= 1e-5
 
 
-@pytest.mark.parametrize("niters,ty,merged", [
-    (0, "8", "1"), (1, "8", "1"), (2, "8", "1"), (3, "8", "1"), (50, "8", "1"), (3, "4", "1"), (50, "4", "1"),
-    (3, "16", "1"), (50, "16", "1"), (3, "8", "0"), (50, "8", "0"), (50, "4", "0")])
-def test_tvl1_fused_matches_launch_sequence(gpu, monkeypatch, views_small, niters, ty, merged):
-    """The fused one- and two-iteration TVL1 launches (tvl1_fused.hip, tiles
-    64 x ty; two iterations in three phases, or in five with merged=0)
+@pytest.mark.parametrize("niters,ty,merged,mmax,narrow", [
+    (0, "8", "1", "2", "1"), (1, "8", "1", "2", "1"), (2, "8", "1", "2", "1"), (3, "8", "1", "2", "1"),
+    (50, "8", "1", "2", "1"), (3, "4", "1", "2", "1"), (50, "4", "1", "2", "1"), (3, "16", "1", "2", "1"),
+    (50, "16", "1", "2", "1"), (3, "8", "1", "3", "1"), (50, "8", "1", "3", "1"), (4, "8", "1", "4", "1"),
+    (7, "8", "1", "4", "1"), (50, "8", "1", "4", "1"), (50, "16", "1", "4", "1"), (49, "4", "1", "4", "1"),
+    (1, "8", "1", "2", "0"), (3, "8", "1", "2", "0"), (50, "8", "1", "2", "0"), (50, "4", "1", "2", "0"),
+    (50, "16", "1", "2", "0"), (3, "8", "0", "2", "0"), (50, "8", "0", "2", "0"), (50, "8", "1", "3", "0"),
+    (7, "8", "1", "4", "0"), (50, "16", "1", "4", "0")])
+def test_tvl1_fused_matches_launch_sequence(gpu, monkeypatch, views_small, niters, ty, merged, mmax, narrow):
+    """The fused TVL1 launches of up to mmax iterations (tvl1_fused.hip, tiles
+    ty rows high and 64 - 2m (narrow) or 64 columns wide; with 64-wide tiles
+    two iterations run in three phases, or in five with merged=0)
     against the reference's two launches per iteration (PSW_TVL1_FUSED=0):
     bit-identical on the 160x120 view and on sizes that cut the tiles (65x9,
     3x2, 130x17), and within the oracle tolerance of the FP32 oracle."""
@@ -593,12 +599,15 @@ def test_tvl1_fused_matches_launch_sequence(gpu, monkeypatch, views_small, niter
     for depth, ref in cases:
         depth = np.clip(depth + 0.05 * rng.standard_normal(depth.shape), 0.5, 6.0).astype(np.float32)
         ref = np.asarray(ref, np.float32)
-        print(f"[case] tvl1 {depth.shape[1]}x{depth.shape[0]} niters={niters} ty={ty} merged={merged}", flush=True)
+        print(f"[case] tvl1 {depth.shape[1]}x{depth.shape[0]} niters={niters} ty={ty} merged={merged} m={mmax} "
+              f"narrow={narrow}", flush=True)
         monkeypatch.setenv("PSW_TVL1_FUSED", "0")
         seq = stages.tvl1_denoise(depth, ref, 0.5, 6.0, niters=niters)
         monkeypatch.setenv("PSW_TVL1_FUSED", "1")
         monkeypatch.setenv("PSW_TVL1_TY", ty)
         monkeypatch.setenv("PSW_TVL1_MERGED", merged)
+        monkeypatch.setenv("PSW_TVL1_M", mmax)
+        monkeypatch.setenv("PSW_TVL1_NARROW", narrow)
         fused = stages.tvl1_denoise(depth, ref, 0.5, 6.0, niters=niters)
         np.testing.assert_array_equal(fused.view(np.uint32), seq.view(np.uint32))
         if depth.size < 5000:
