@@ -1,10 +1,10 @@
 """ctypes binding of the CPU oracle (oracle/_build/libpsw_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, as the checker.  The oracle restates the
-reference's kernels and host sequencing on the CPU (psw_oracle.cpp); the
-_f32 functions keep the reference's float arithmetic, the _f64 functions run
-the same algorithm in double.
+bench.py's cpu_baseline leg, to compare the GPU results against.  The
+oracle restates the reference's kernels and host sequencing on the CPU
+(psw_oracle.cpp); the _f32 functions keep the reference's float arithmetic,
+the _f64 functions run the same algorithm in double.
 """
 import ctypes
 import os

@@ -3,7 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY: a line-by-line CPU restatement of the reference
  * kernels and of their host sequencing, used by tests/, smoke() and the
- * bench's cpu_baseline leg as the checker.  Nothing in the product links it.
+ * bench's cpu_baseline leg to compare the GPU results against.  Nothing in
+ * the product links it.
  *
  * Build: g++ -O2 -fopenmp -ffp-contract=off -fno-fast-math (see oracle/build.sh).
  * Contraction is off so that every + - * / rounds on its own, which is the

@@ -5,7 +5,8 @@
  * multiview-stereo path (dkavolis/plane-sweep-cuda: src/kernels.cu,
  * src/TGV2_kernels.cu, src/fusion.cu, src/inc/fusion.h, src/inc/structs.h,
  * src/planesweep.cpp, src/pclviewer.cpp).  It is used only by tests/,
- * __graft_entry__.smoke() and bench.py's cpu_baseline leg, as the checker.
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg, to compare the
+ * GPU results against.
  * The product (libpsw.so) never links, loads or calls it.
  *
  * Every function exists in two instantiations:
