@@ -184,9 +184,12 @@ __global__ __launch_bounds__(TS_THREADS) void tgv_sparse_iter_kernel(SparsePtrs 
 // LDS index = offset + 2 in every array.  Each position's arithmetic is
 // the one-iteration kernel's (pq_update, sparse_u_step), so the results
 // are identical.
-template <int TY, int NT>
+//
+// Every phase walks whole rows of the AX-wide window and idles the positions
+// outside its own columns.  With TX = 60 the window is 64 wide, so a
+// position's row and column are a shift and a mask of its index.
+template <int TY, int NT, int TX>
 __global__ __launch_bounds__(NT) void tgv_sparse_iter2_kernel(SparsePtrs a) {
-    constexpr int TX = TS_TX;
     constexpr int AX = TX + 4, AY = TY + 4, BX = TX + 3, BY = TY + 3;
     __shared__ float sU[3][AY][AX];
     __shared__ float sP[2][BY][BX];
@@ -210,8 +213,9 @@ __global__ __launch_bounds__(NT) void tgv_sparse_iter2_kernel(SparsePtrs a) {
 #pragma unroll
         for (int f = 0; f < 3; ++f) sU[f][iy][ix] = a.ucur[f * n + g];
     }
-    for (int i = tid; i < BX * BY; i += NT) {
-        const int iy = i / BX, ix = i - iy * BX;
+    for (int i = tid; i < AX * BY; i += NT) {
+        const int iy = i / AX, ix = i % AX;
+        if (ix >= BX) continue;
         const size_t g = gidx(ix, iy);
         sP[0][iy][ix] = a.pcur[g];
         sP[1][iy][ix] = a.pcur[n + g];
@@ -225,8 +229,9 @@ __global__ __launch_bounds__(NT) void tgv_sparse_iter2_kernel(SparsePtrs a) {
 
     const float as1 = a.alpha1 * a.sigma, as0 = a.alpha0 * a.sigma;
     // p, q of k+1 on B: [0, BX) x [0, BY)
-    for (int i = tid; i < BX * BY; i += NT) {
-        const int iy = i / BX, ix = i - iy * BX;
+    for (int i = tid; i < AX * BY; i += NT) {
+        const int iy = i / AX, ix = i % AX;
+        if (ix >= BX) continue;
         const int gx = x0 - 2 + ix, gy = y0 - 2 + iy;
         const int rx = gx < W - 1 ? ix + 1 : ix, dy = gy < H - 1 ? iy + 1 : iy;
         float px = sP[0][iy][ix], py = sP[1][iy][ix];
@@ -242,8 +247,9 @@ __global__ __launch_bounds__(NT) void tgv_sparse_iter2_kernel(SparsePtrs a) {
     // u of k+1 on C: [1, TX+3) x [1, TY+3), in place (a position reads only
     // its own u)
     constexpr int CX = TX + 2, CY = TY + 2;
-    for (int i = tid; i < CX * CY; i += NT) {
-        const int iy = i / CX + 1, ix = i - (i / CX) * CX + 1;
+    for (int i = tid; i < AX * CY; i += NT) {
+        const int iy = i / AX + 1, ix = i % AX;
+        if (ix < 1 || ix > CX) continue;
         const int gx = x0 - 2 + ix, gy = y0 - 2 + iy;
         const int lx = gx > 0 ? ix - 1 : ix, uy = gy > 0 ? iy - 1 : iy;
         float un, u1xn, u1yn;
@@ -257,8 +263,9 @@ __global__ __launch_bounds__(NT) void tgv_sparse_iter2_kernel(SparsePtrs a) {
 
     // p, q of k+2 on D: [1, TX+2) x [1, TY+2), in place; the tile's to HBM
     constexpr int DX = TX + 1, DY = TY + 1;
-    for (int i = tid; i < DX * DY; i += NT) {
-        const int iy = i / DX + 1, ix = i - (i / DX) * DX + 1;
+    for (int i = tid; i < AX * DY; i += NT) {
+        const int iy = i / AX + 1, ix = i % AX;
+        if (ix < 1 || ix > DX) continue;
         const int gx = x0 - 2 + ix, gy = y0 - 2 + iy;
         const int rx = gx < W - 1 ? ix + 1 : ix, dy = gy < H - 1 ? iy + 1 : iy;
         float px = sP[0][iy][ix], py = sP[1][iy][ix];
@@ -277,10 +284,10 @@ __global__ __launch_bounds__(NT) void tgv_sparse_iter2_kernel(SparsePtrs a) {
     __syncthreads();
 
     // u of k+2 on the tile
-    for (int i = tid; i < TX * TY; i += NT) {
-        const int ty = i / TX, tx = i - ty * TX;
+    for (int i = tid; i < AX * TY; i += NT) {
+        const int ty = i / AX, tx = i % AX;
         const int x = x0 + tx, y = y0 + ty;
-        if (x >= W || y >= H) continue;
+        if (tx >= TX || x >= W || y >= H) continue;
         const size_t g = (size_t)y * W + x;
         const int ix = tx + 2, iy = ty + 2;
         const int lx = x > 0 ? ix - 1 : ix, uy = y > 0 ? iy - 1 : iy;
@@ -316,14 +323,21 @@ static hipError_t sparse_launch(const SparsePtrs& a, hipStream_t hs) {
     return hipGetLastError();
 }
 
-// 64 x TY tiles with NT threads (NT >= 64 * TY: the halo regions, up to
-// 68 x (TY + 4) positions, take fewer passes).
-template <int TY, int NT>
+// TX x TY tiles with NT threads (NT >= TX * TY: the halo regions, up to
+// (TX + 4) x (TY + 4) positions, take fewer passes).
+template <int TY, int NT, int TX>
 static hipError_t sparse_launch2(const SparsePtrs& a, hipStream_t hs) {
-    static_assert(NT >= TS_TX * TY, "at least one thread per tile pixel");
-    const int ntiles = ((a.W + TS_TX - 1) / TS_TX) * ((a.H + TY - 1) / TY);
-    tgv_sparse_iter2_kernel<TY, NT><<<ntiles, NT, 0, hs>>>(a);
+    static_assert(NT >= TX * TY, "at least one thread per tile pixel");
+    const int ntiles = ((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY);
+    tgv_sparse_iter2_kernel<TY, NT, TX><<<ntiles, NT, 0, hs>>>(a);
     return hipGetLastError();
+}
+
+// 60-wide tiles for the two-iteration kernel (a 64-wide window) unless
+// PSW_TGV_SPARSE_NARROW=0 (64-wide tiles); identical results.
+static bool sparse_narrow() {
+    const char* e = std::getenv("PSW_TGV_SPARSE_NARROW");
+    return !(e && std::atoi(e) == 0);
 }
 
 // Two iterations per launch unless PSW_TGV_SPARSE_DOUBLE=0 (identical results).
@@ -386,6 +400,7 @@ int psw_tgv_sparse_denoise(const float* d_ref, const float* d_dense, const float
     // Each launch reads one slot and writes the other (a two-iteration launch
     // too), so the current state is in slot (launches & 1).
     const bool dbl = sparse_double();
+    const bool narrow = sparse_narrow();
     int launches = 0;
     for (int k = 0; k < niters && e == hipSuccess; ++launches) {
         float* c = slot[launches & 1];
@@ -397,7 +412,8 @@ int psw_tgv_sparse_denoise(const float* d_ref, const float* d_dense, const float
         a.qcur = c + 5 * n;
         a.qnext = nx + 5 * n;
         if (dbl && niters - k >= 2) {
-            e = sparse_launch2<8, 512>(a, hs);   // 768 threads measured 5.4 against 5.1 us
+            // 768 threads measured 5.4 against 5.1 us
+            e = narrow ? sparse_launch2<8, 512, 60>(a, hs) : sparse_launch2<8, 512, TS_TX>(a, hs);
             k += 2;
             continue;
         }
