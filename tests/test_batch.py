@@ -39,6 +39,29 @@ def test_sweep_batch_matches_single(gpu, fast, groups):
             np.testing.assert_array_equal(a, b)
 
 
+def test_sweep_batch_many_frames(gpu):
+    """19 uneven frames of 15 shapes in one batched sweep (the persistent
+    kernel's frame lookup over many frames): every frame equals its
+    single-frame sweep."""
+    from psw_amd import batch
+    specs = [{"seed": 40 + i, "W": 48 + 8 * (i % 5), "H": 32 + 8 * (i % 3), "N": 2 + i % 3, "P": 5 + i % 7}
+             for i in range(19)]
+    frames = [batch.Frame(sp, znear=0.5, zfar=6.0, winsize=5) for sp in specs]
+    single = []
+    for f in frames:
+        f.sweep.build_homographies()
+        f.sweep.run(fast=True)
+        single.append(f.sweep.results())
+    for f in frames:
+        for d in (f.sweep.d_arg, f.sweep.d_cmin, f.sweep.d_csec):
+            d.fill(0)
+    print(f"[case] sweep batch of {len(frames)} frames", flush=True)
+    batch.sweep_batch([f.sweep for f in frames], fast=True, groups=0, workgroups=0)
+    for f, ref in zip(frames, single):
+        for a, b in zip(f.sweep.results(), ref):
+            np.testing.assert_array_equal(a, b)
+
+
 def _linearised(frames, prm):
     for f in frames:
         f.sweep.build_homographies()
