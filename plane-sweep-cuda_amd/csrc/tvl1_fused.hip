@@ -76,28 +76,29 @@ __device__ __forceinline__ float tvl1_u(float o, float& r, float raw, float pxc,
     return (float)(x_new + (double)theta * (x_new - (double)o));
 }
 
-// M iterations (1 or 2) on one tile.  Region coordinates are local to the
+// M iterations (1 to 4) on one tile.  Region coordinates are local to the
 // tile's staging window, which starts M pixels up and left of the tile: the
 // P step of iteration k covers [k, R-2-k], the update [k+1, R-2-k], so the
 // last update covers exactly the tile.  Pixels outside the image are never
 // computed, and no pixel inside it reads one (the reference clamps its
 // forward differences and drops the backward ones at the first row/column).
-// The tensor is loaded by the first P step and kept in LDS for the second,
+// The tensor is loaded by the first P step and kept in LDS for the others,
 // R by the first update; the scaled input is read where it is used.
 //
-// Measured at 640x480 (two iterations per launch): 8.6 us.  Three
-// restructurings were slower with identical results: every input staged in
-// the first phase (9.5 us; 9.4 us with all loads issued before the first LDS
-// store) and branch-free fixed passes that let the compiler interleave a
-// thread's two positions (9.3 us).  The kernel moves 16.6 MB per launch
-// (11.7 MB read, 4.9 MB written), far below what HBM allows in that time.
-// The three-phase kernel below is 3.5 % faster and is the default for
-// two-iteration launches on tiles up to 64x8.
-//
 // TX is the tile width; the block has 64 x TV_Y threads either way.  With
-// TX = 64 - 2M the staging window is 64 wide, so a position's window row and
-// column are a shift and a mask of its index, and each pass of the block
-// covers TV_Y whole rows (positions outside a phase's columns idle).
+// TX = 64 - 2M (the default) the staging window is 64 wide, so a position's
+// window row and column are a shift and a mask of its index, and each pass of
+// the block covers TV_Y whole rows (positions outside a phase's columns idle).
+//
+// Measured at 640x480, us per iteration: 3.49 for M = 4 on 56x8 tiles, 3.62
+// for M = 3 on 58x8, 3.99 for M = 2 on 60x8, 4.29 for M = 2 on 64x8.  A
+// four-iteration launch moves 17.3 MB (12.4 MB read, 4.9 MB written), far
+// below what HBM allows in its 14 us: the kernel is bound by instruction
+// issue.  Slower restructurings of the two-iteration case, with identical
+// results: every input staged in the first phase (9.5 us per launch against
+// 8.9; 9.4 us with all loads issued before the first LDS store) and
+// branch-free fixed passes that let the compiler interleave a thread's two
+// positions (9.3 us).
 template <int M, int TV_Y, int TX>
 __global__ __launch_bounds__(TV_X * TV_Y) void tvl1_iter_kernel(const TvlState a, int rd, float sp0, float sp1) {
     constexpr int TV_NT = TV_X * TV_Y;
@@ -200,7 +201,8 @@ __global__ __launch_bounds__(TV_X * TV_Y) void tvl1_iter_kernel(const TvlState a
 // phase precedes it either.  Phases: P1 (global -> LDS), U1 x3 + P2, U2
 // (LDS -> global).  Every evaluation is the per-pixel arithmetic above, so
 // the results are those of tvl1_iter_kernel<2>.  Local (0, 0) is two pixels
-// up and left of the tile.
+// up and left of the tile.  Used for two-iteration launches on 64-wide tiles
+// (PSW_TVL1_NARROW=0), where it measured 4.14 against 4.29 us per iteration.
 template <int TV_Y>
 __global__ __launch_bounds__(TV_X * TV_Y) void tvl1_iter2m_kernel(const TvlState a, int rd, float sp0, float sp1) {
     constexpr int TV_NT = TV_X * TV_Y;
