@@ -155,9 +155,10 @@ def run_sparse(pl, stream, ev, seed):
     t_full, t_zero = timed(SPARSE_ITERS), timed(0)
     it_ms = (t_full - t_zero) / SPARSE_ITERS
     gbs = SPARSE_BYTES_PER_PIXEL_ITER * W * H / (it_ms * 1e-3) / 1e9
-    two = os.environ.get("PSW_TGV_SPARSE_DOUBLE", "1") != "0"     # two iterations per launch (default)
-    return {"kernel": "tgv_sparse_iter2_kernel" if two else "tgv_sparse_iter_kernel",
-            "iterations_per_launch": 2 if two else 1, "iter_us": it_ms * 1e3, "call_ms": t_full,
+    multi = os.environ.get("PSW_TGV_SPARSE_DOUBLE", "1") != "0"   # several iterations per launch (default)
+    m = int(os.environ.get("PSW_TGV_SPARSE_M", "4")) if multi else 1
+    kernel = {1: "tgv_sparse_iter_kernel", 2: "tgv_sparse_iter2_kernel"}.get(m, f"tgv_sparse_iterm_kernel<{m}>")
+    return {"kernel": kernel, "iterations_per_launch": m, "iter_us": it_ms * 1e3, "call_ms": t_full,
             "setup_output_ms": t_zero, "gbs_algorithmic": gbs, "frac_hbm_peak": gbs / HBM_PEAK_GBS,
             "bytes_per_pixel_iter": SPARSE_BYTES_PER_PIXEL_ITER,
             "config": f"{W}x{H}, sweep depth as the dense start, true depth at 5% of pixels, {SPARSE_ITERS} "

@@ -29,6 +29,7 @@
 #include "psw_common.h"
 #include "tgv_point.h"
 
+#include <algorithm>
 #include <cstdlib>
 
 namespace psw {
@@ -302,6 +303,98 @@ __global__ __launch_bounds__(NT) void tgv_sparse_iter2_kernel(SparsePtrs a) {
     }
 }
 
+// M iterations per launch (M = 2..4) on a (64 - 2M) x TY tile, so that the
+// staging window is 64 wide (a position's window row and column are a shift
+// and a mask of its index).  The window starts M pixels up and left of the
+// tile; the dual step of iteration k covers [k, R-2-k], the primal step
+// [k+1, R-2-k], so the last primal step covers exactly the tile.  Positions
+// outside the image are skipped, and none inside it reads one (forward
+// differences clamp to the pixel at the last column / row, backward ones to
+// the pixel at the first).  Each position's arithmetic is the one-iteration
+// kernel's (pq_update, sparse_u_step), so the results are identical.
+template <int M, int TY>
+__global__ __launch_bounds__(64 * TY) void tgv_sparse_iterm_kernel(SparsePtrs a) {
+    constexpr int NT = 64 * TY, TX = 64 - 2 * M;
+    constexpr int RX = TX + 2 * M, RY = TY + 2 * M, RN = RX * RY;
+    __shared__ float sU[3][RN];
+    __shared__ float sP[2][RN];
+    __shared__ float sQ[3][RN];
+    __shared__ float sT[3][RN];
+    const int W = a.W, H = a.H;
+    const size_t n = (size_t)W * H;
+    const int tiles_x = (W + TX - 1) / TX;
+    const int ntiles = tiles_x * ((H + TY - 1) / TY);
+    const int tile = xcd_tile(blockIdx.x, ntiles);
+    const int gx0 = (tile % tiles_x) * TX - M, gy0 = (tile / tiles_x) * TY - M;
+    for (int i = threadIdx.x; i < RN; i += NT) {
+        const int ly = i / RX, lx = i % RX;
+        const int gx = gx0 + lx, gy = gy0 + ly;
+        if (gx < 0 || gx >= W || gy < 0 || gy >= H) continue;
+        const size_t g = (size_t)gy * W + gx;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) sU[f][i] = a.ucur[f * n + g];
+        if (lx <= RX - 2 && ly <= RY - 2) {   // the first dual step's region
+            sP[0][i] = a.pcur[g];
+            sP[1][i] = a.pcur[n + g];
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                sQ[f][i] = a.qcur[f * n + g];
+                sT[f][i] = a.T[f * n + g];
+            }
+        }
+    }
+    __syncthreads();
+    const float as1 = a.alpha1 * a.sigma, as0 = a.alpha0 * a.sigma;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        {   // p, q of k+1 over [k, RX-2-k] x [k, RY-2-k]
+            const int PY = RY - 1 - 2 * k;
+            for (int i = threadIdx.x; i < PY * RX; i += NT) {
+                const int ly = k + i / RX, lx = i % RX;
+                const int gx = gx0 + lx, gy = gy0 + ly;
+                if (lx < k || lx > RX - 2 - k || gx < 0 || gx >= W || gy < 0 || gy >= H) continue;
+                const int li = ly * RX + lx;
+                const int r = gx < W - 1 ? li + 1 : li, d = gy < H - 1 ? li + RX : li;
+                float px = sP[0][li], py = sP[1][li];
+                float qx = sQ[0][li], qy = sQ[1][li], qz = sQ[2][li];
+                pq_update(px, py, qx, qy, qz, sT[0][li], sT[1][li], sT[2][li], sU[0][li], sU[0][r], sU[0][d],
+                          sU[1][li], sU[1][r], sU[1][d], sU[2][li], sU[2][r], sU[2][d], as1, as0);
+                sP[0][li] = px; sP[1][li] = py;
+                sQ[0][li] = qx; sQ[1][li] = qy; sQ[2][li] = qz;
+                if (k == M - 1 && lx >= M && lx < M + TX && ly >= M && ly < M + TY) {
+                    const size_t g = (size_t)gy * W + gx;
+                    a.pnext[g] = px; a.pnext[n + g] = py;
+                    a.qnext[g] = qx; a.qnext[n + g] = qy; a.qnext[2 * n + g] = qz;
+                }
+            }
+        }
+        __syncthreads();
+        {   // u, u1 of k+1 over [k+1, RX-2-k] x [k+1, RY-2-k] (a position reads only its own u)
+            const int UY = RY - 2 - 2 * k;
+            for (int i = threadIdx.x; i < UY * RX; i += NT) {
+                const int ly = k + 1 + i / RX, lx = i % RX;
+                const int gx = gx0 + lx, gy = gy0 + ly;
+                if (lx < k + 1 || lx > RX - 2 - k || gx < 0 || gx >= W || gy < 0 || gy >= H) continue;
+                const int li = ly * RX + lx;
+                const int l = gx > 0 ? li - 1 : li, u = gy > 0 ? li - RX : li;
+                const size_t g = (size_t)gy * W + gx;
+                float un, u1xn, u1yn;
+                sparse_u_step(a, sU[0][li], sU[1][li], sU[2][li], sT[0][li], sT[1][li], sT[2][li], sP[0][li],
+                              sP[1][li], sP[0][l], sP[1][l], sP[0][u], sP[1][u], sQ[0][li], sQ[0][l], sQ[1][li],
+                              sQ[1][u], sQ[2][li], sQ[2][l], sQ[2][u], a.sparse[g], un, u1xn, u1yn);
+                if (k == M - 1) {
+                    a.unext[g] = un;
+                    a.unext[n + g] = u1xn;
+                    a.unext[2 * n + g] = u1yn;
+                } else {
+                    sU[0][li] = un; sU[1][li] = u1xn; sU[2][li] = u1yn;
+                }
+            }
+        }
+        if (k < M - 1) __syncthreads();
+    }
+}
+
 __global__ void tgv_sparse_output_kernel(float* __restrict__ out, const float* __restrict__ u, size_t n, float zfar) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         out[i] = u[i] * zfar;
@@ -338,6 +431,23 @@ static hipError_t sparse_launch2(const SparsePtrs& a, hipStream_t hs) {
 static bool sparse_narrow() {
     const char* e = std::getenv("PSW_TGV_SPARSE_NARROW");
     return !(e && std::atoi(e) == 0);
+}
+
+// Iterations per launch at most (PSW_TGV_SPARSE_M: 2, the two-iteration
+// kernel only, or 3, or 4, the default; identical results).  At 640x480:
+// 4.75, 4.06-4.14 and 3.72-3.77 us per iteration.
+static int sparse_max_per_launch() {
+    const char* e = std::getenv("PSW_TGV_SPARSE_M");
+    const int v = e ? std::atoi(e) : 4;
+    return v == 2 || v == 3 ? v : 4;
+}
+
+template <int M>
+static hipError_t sparse_launchm(const SparsePtrs& a, hipStream_t hs) {
+    constexpr int TX = 64 - 2 * M, TY = 8;
+    const int ntiles = ((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY);
+    tgv_sparse_iterm_kernel<M, TY><<<ntiles, 64 * TY, 0, hs>>>(a);
+    return hipGetLastError();
 }
 
 // Two iterations per launch unless PSW_TGV_SPARSE_DOUBLE=0 (identical results).
@@ -401,6 +511,7 @@ int psw_tgv_sparse_denoise(const float* d_ref, const float* d_dense, const float
     // too), so the current state is in slot (launches & 1).
     const bool dbl = sparse_double();
     const bool narrow = sparse_narrow();
+    const int mmax = sparse_max_per_launch();
     int launches = 0;
     for (int k = 0; k < niters && e == hipSuccess; ++launches) {
         float* c = slot[launches & 1];
@@ -411,6 +522,12 @@ int psw_tgv_sparse_denoise(const float* d_ref, const float* d_dense, const float
         a.pnext = nx + 3 * n;
         a.qcur = c + 5 * n;
         a.qnext = nx + 5 * n;
+        if (dbl && mmax > 2 && niters - k >= 3) {
+            const int m = std::min(mmax, niters - k);
+            e = m == 4 ? sparse_launchm<4>(a, hs) : sparse_launchm<3>(a, hs);
+            k += m;
+            continue;
+        }
         if (dbl && niters - k >= 2) {
             // 768 threads measured 5.4 against 5.1 us
             e = narrow ? sparse_launch2<8, 512, 60>(a, hs) : sparse_launch2<8, 512, TS_TX>(a, hs);
