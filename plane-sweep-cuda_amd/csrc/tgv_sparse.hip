@@ -303,7 +303,7 @@ __global__ __launch_bounds__(NT) void tgv_sparse_iter2_kernel(SparsePtrs a) {
     }
 }
 
-// M iterations per launch (M = 2..4) on a (64 - 2M) x TY tile, so that the
+// M iterations per launch (M = 3..6) on a (64 - 2M) x TY tile, so that the
 // staging window is 64 wide (a position's window row and column are a shift
 // and a mask of its index).  The window starts M pixels up and left of the
 // tile; the dual step of iteration k covers [k, R-2-k], the primal step
@@ -434,12 +434,12 @@ static bool sparse_narrow() {
 }
 
 // Iterations per launch at most (PSW_TGV_SPARSE_M: 2, the two-iteration
-// kernel only, or 3, or 4, the default; identical results).  At 640x480:
-// 4.75, 4.06-4.14 and 3.72-3.77 us per iteration.
+// kernel only, 3 to 6, 4 the default; identical results).  At 640x480:
+// 4.75, 4.06-4.14, 3.72-3.80, 3.72-3.80 and 5.0 us per iteration for 2..6.
 static int sparse_max_per_launch() {
     const char* e = std::getenv("PSW_TGV_SPARSE_M");
     const int v = e ? std::atoi(e) : 4;
-    return v == 2 || v == 3 ? v : 4;
+    return v >= 2 && v <= 6 ? v : 4;
 }
 
 template <int M>
@@ -524,7 +524,10 @@ int psw_tgv_sparse_denoise(const float* d_ref, const float* d_dense, const float
         a.qnext = nx + 5 * n;
         if (dbl && mmax > 2 && niters - k >= 3) {
             const int m = std::min(mmax, niters - k);
-            e = m == 4 ? sparse_launchm<4>(a, hs) : sparse_launchm<3>(a, hs);
+            e = m == 6 ? sparse_launchm<6>(a, hs)
+                : m == 5 ? sparse_launchm<5>(a, hs)
+                : m == 4 ? sparse_launchm<4>(a, hs)
+                         : sparse_launchm<3>(a, hs);
             k += m;
             continue;
         }

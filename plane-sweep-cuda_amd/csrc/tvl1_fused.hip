@@ -76,7 +76,7 @@ __device__ __forceinline__ float tvl1_u(float o, float& r, float raw, float pxc,
     return (float)(x_new + (double)theta * (x_new - (double)o));
 }
 
-// M iterations (1 to 4) on one tile.  Region coordinates are local to the
+// M iterations (1 to 6) on one tile.  Region coordinates are local to the
 // tile's staging window, which starts M pixels up and left of the tile: the
 // P step of iteration k covers [k, R-2-k], the update [k+1, R-2-k], so the
 // last update covers exactly the tile.  Pixels outside the image are never
@@ -312,6 +312,8 @@ int tvl1_launch_ty(const TvlState& tv, bool merged, bool narrow, int m, int rd, 
         }
     }
     switch (m) {
+        case 6: return tvl1_launch_m<TY, 6>(tv, narrow, rd, sp0, sp1, s);
+        case 5: return tvl1_launch_m<TY, 5>(tv, narrow, rd, sp0, sp1, s);
         case 4: return tvl1_launch_m<TY, 4>(tv, narrow, rd, sp0, sp1, s);
         case 3: return tvl1_launch_m<TY, 3>(tv, narrow, rd, sp0, sp1, s);
         case 2: return tvl1_launch_m<TY, 2>(tv, narrow, rd, sp0, sp1, s);
@@ -327,13 +329,14 @@ int tvl1_ty() {
     return v == 4 || v == 16 ? v : 8;
 }
 
-// Iterations per launch at most (PSW_TVL1_M: 2, 3 or 4, the default;
-// identical results).  At 640x480 with 64-wide staging windows: 3.99, 3.62
-// and 3.49 us per iteration for 2, 3 and 4.
+// Iterations per launch at most (PSW_TVL1_M: 2 to 6, 4 the default, at most 4
+// on 64x16 tiles; identical results).  At 640x480 with 64-wide staging
+// windows: 3.99, 3.62, 3.44-3.49, 3.35-3.39 and 4.05-4.11 us per iteration
+// for 2..6.
 int tvl1_max_per_launch() {
     const char* e = std::getenv("PSW_TVL1_M");
     const int v = e ? std::atoi(e) : 4;
-    return v == 2 || v == 3 ? v : 4;
+    return v >= 2 && v <= 6 ? v : 4;
 }
 
 // Two-iteration launches in three phases (tvl1_iter2m_kernel) unless
@@ -430,7 +433,7 @@ extern "C" int psw_tvl1_denoise(const float* d_depth, const float* d_ref, int wi
         tv.sigma = (float)sigma;
         const int ty = tvl1_ty();
         const bool merged = tvl1_merged();
-        const int mmax = tvl1_max_per_launch();
+        const int mmax = std::min(tvl1_max_per_launch(), ty == 16 ? 4 : 6);   // LDS of 64x16 tiles
         const bool narrow = tvl1_narrow();
         int rd = 0;
         for (int i = 0; i < niters;) {

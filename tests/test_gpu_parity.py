@@ -582,7 +582,8 @@ def test_tvl1_vs_oracle(gpu, views_small):
     (7, "8", "1", "4", "1"), (50, "8", "1", "4", "1"), (50, "16", "1", "4", "1"), (49, "4", "1", "4", "1"),
     (1, "8", "1", "2", "0"), (3, "8", "1", "2", "0"), (50, "8", "1", "2", "0"), (50, "4", "1", "2", "0"),
     (50, "16", "1", "2", "0"), (3, "8", "0", "2", "0"), (50, "8", "0", "2", "0"), (50, "8", "1", "3", "0"),
-    (7, "8", "1", "4", "0"), (50, "16", "1", "4", "0")])
+    (7, "8", "1", "4", "0"), (50, "16", "1", "4", "0"), (50, "8", "1", "5", "1"), (11, "8", "1", "6", "1"),
+    (50, "4", "1", "6", "1"), (50, "8", "1", "6", "0"), (50, "16", "1", "6", "1")])
 def test_tvl1_fused_matches_launch_sequence(gpu, monkeypatch, views_small, niters, ty, merged, mmax, narrow):
     """The fused TVL1 launches of up to mmax iterations (tvl1_fused.hip, tiles
     ty rows high and 64 - 2m (narrow) or 64 columns wide; with 64-wide tiles

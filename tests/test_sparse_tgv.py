@@ -201,15 +201,16 @@ def test_compute3d_bit_exact(gpu, alternative):
 @pytest.mark.gpu
 @pytest.mark.parametrize("niters", [25, 26, 1, 2])
 def test_sparse_launch_shapes_identical(gpu, monkeypatch, niters):
-    """The tile shape (PSW_TGV_SPARSE_VARIANT, PSW_TGV_SPARSE_NARROW) and two,
-    three or four iterations per launch (PSW_TGV_SPARSE_DOUBLE,
-    PSW_TGV_SPARSE_M) never change results, for odd and even counts."""
+    """The tile shape (PSW_TGV_SPARSE_VARIANT, PSW_TGV_SPARSE_NARROW) and two
+    to six iterations per launch (PSW_TGV_SPARSE_DOUBLE, PSW_TGV_SPARSE_M)
+    never change results, for odd and even counts."""
     from psw_amd import stages
     v, dense, sparse = _scene(W=161, H=83)
     outs = []
     for dbl, var, narrow, m in (("0", "1", "1", "2"), ("0", "0", "1", "2"), ("0", "2", "1", "2"),
                                 ("0", "3", "1", "2"), ("1", "1", "1", "2"), ("1", "0", "1", "2"),
-                                ("1", "1", "0", "2"), ("1", "1", "1", "3"), ("1", "1", "1", "4")):
+                                ("1", "1", "0", "2"), ("1", "1", "1", "3"), ("1", "1", "1", "4"),
+                                ("1", "1", "1", "5"), ("1", "1", "1", "6")):
         monkeypatch.setenv("PSW_TGV_SPARSE_DOUBLE", dbl)
         monkeypatch.setenv("PSW_TGV_SPARSE_VARIANT", var)
         monkeypatch.setenv("PSW_TGV_SPARSE_NARROW", narrow)
