@@ -42,6 +42,12 @@ _SIGS = {
                          _vp, _vp]),
     "orc_tgv_f64": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _d, _d, _d, _d, _d, _d, _vp,
                          _vp, _vp]),
+    "orc_sweep_agg_volume_f64": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _f, _i, _vp,
+                                      _vp, _vp, _vp]),
+    "orc_reduce_best2_f64": (None, [_vp, _i, _vp, _vp, _vp]),
+    "orc_tgv_linearise_f64": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _d, _d, _vp, _vp, _vp, _vp]),
+    "orc_tgv_iterate_f32": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _d, _d, _d, _d, _d]),
+    "orc_tgv_iterate_f64": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _d, _d, _d, _d, _d]),
     "orc_tensor_f32": (None, [_vp, _i, _i, _f, _f, _vp, _vp, _vp, _vp]),
     "orc_tgv_linearise_f32": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _d, _d, _vp, _vp, _vp, _vp]),
     "orc_tgv_energy": (_d, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _d, _d]),
@@ -172,6 +178,65 @@ def sweep_agg(ref, src, K, Rref, tref, Rsrc, tsrc, planes, winsize, stdthresh, c
     return arg, cmin, csec
 
 
+def sweep_agg_volume(ref, src, K, Rref, tref, Rsrc, tsrc, planes, winsize, stdthresh, coord_offset=0,
+                     alternative=0):
+    """FP64 aggregated sweep that also returns the cost of every plane:
+    (argmin, min, second, volume[P][H][W])."""
+    ref, src, planes = _c(ref), _c(src), _c(planes)
+    Hh, W = ref.shape
+    N = src.shape[0]
+    arg = np.zeros((Hh, W), np.int32)
+    cmin, csec = np.zeros((Hh, W)), np.zeros((Hh, W))
+    vol = np.zeros((planes.size, Hh, W))
+    pose = [_c(x).ravel() for x in (K, Rref, tref, Rsrc, tsrc)]
+    rc = _lib.orc_sweep_agg_volume_f64(_p(ref), _p(src), W, Hh, N, *[_p(x) for x in pose], alternative, _p(planes),
+                                       planes.size, winsize, stdthresh, coord_offset, _p(arg), _p(cmin), _p(csec),
+                                       _p(vol))
+    assert rc == 0, rc
+    return arg, cmin, csec, vol
+
+
+def reduce_best2(costs):
+    """The aggregated sweep's reduction over one pixel's costs in plane order:
+    (argmin, min, second-smallest)."""
+    c = _c(costs, np.float64).ravel()
+    a, m, s = ctypes.c_int32(), ctypes.c_double(), ctypes.c_double()
+    _lib.orc_reduce_best2_f64(_p(c), c.size, ctypes.byref(a), ctypes.byref(m), ctypes.byref(s))
+    return a.value, m.value, s.value
+
+
+TGV_STATE_NAMES = ["u", "u1x", "u1y", "ubar", "u1xbar", "u1ybar", "px", "py", "qx", "qy", "qz", "qw"]
+
+
+def tgv_iterate(state, T4, u0, It, Iu, niters, lambda_=1.0, alpha0=2.0, alpha1=1.5, tau=0.02, sigma=1.0,
+                prec="f64"):
+    """niters TGV2 iterations from a given state inside a given linearisation.
+    state: dict with the planes of TGV_STATE_NAMES and 'r' (N, H, W); missing
+    planes start at 0, except ubar/u1xbar/u1ybar (default u, u1x, u1y) and qw
+    (default qz).  T4 = (T11, T12, T21, T22).  Returns the new state dict."""
+    T = _ft(prec)
+    It, Iu = _c(It, T), _c(Iu, T)
+    N, Hh, W = It.shape
+    dflt = {"ubar": "u", "u1xbar": "u1x", "u1ybar": "u1y", "qw": "qz"}
+    planes = []
+    for k in TGV_STATE_NAMES:
+        v = state.get(k)
+        if v is None and k in dflt:
+            v = state.get(dflt[k])
+        planes.append(np.zeros((Hh, W), T) if v is None else _c(v, T).reshape(Hh, W))
+    r = state.get("r")
+    planes.append(np.zeros((N, Hh, W), T) if r is None else _c(r, T).reshape(N, Hh, W))
+    st = np.ascontiguousarray(np.concatenate([p.reshape(-1) for p in planes]))
+    Tt, u0 = _c(T4, T), _c(u0, T)
+    fn = _lib.orc_tgv_iterate_f32 if prec == "f32" else _lib.orc_tgv_iterate_f64
+    rc = fn(W, Hh, N, _p(Tt), _p(u0), _p(It), _p(Iu), _p(st), int(niters), lambda_, alpha0, alpha1, tau, sigma)
+    assert rc == 0, rc
+    n = W * Hh
+    out = {k: st[i * n:(i + 1) * n].reshape(Hh, W) for i, k in enumerate(TGV_STATE_NAMES)}
+    out["r"] = st[12 * n:].reshape(N, Hh, W)
+    return out
+
+
 def tgv(ref, src, K, Rref, tref, Rsrc, tsrc, niters, warps, lambda_=0.5, alpha0=2.0, alpha1=1.5, tau=0.02,
         sigma=1.0, beta=0.0, gamma=1.0, u_init=None, alternative=0, prec="f32", state=False):
     ref, src = _c(ref), _c(src)
@@ -202,14 +267,17 @@ def tensor(img, beta, gamma):
     return T
 
 
-def tgv_linearise(ref, src, K, Rref, tref, Rsrc, tsrc, u0, beta=0.0, gamma=1.0, alternative=0):
+def tgv_linearise(ref, src, K, Rref, tref, Rsrc, tsrc, u0, beta=0.0, gamma=1.0, alternative=0, prec="f32"):
+    """It (N,H,W), Iu (N,H,W) and T (T11, T12, T21, T22) of one warp at u0."""
     ref, src, u0 = _c(ref), _c(src), _c(u0)
     Hh, W = ref.shape
     N = src.shape[0]
-    It, Iu, Tt = np.zeros((N, Hh, W), np.float32), np.zeros((N, Hh, W), np.float32), np.zeros((4, Hh, W), np.float32)
+    T = _ft(prec)
+    It, Iu, Tt = np.zeros((N, Hh, W), T), np.zeros((N, Hh, W), T), np.zeros((4, Hh, W), T)
     pose = [_c(x).ravel() for x in (K, Rref, tref, Rsrc, tsrc)]
-    rc = _lib.orc_tgv_linearise_f32(_p(ref), _p(src), W, Hh, N, *[_p(x) for x in pose], alternative, beta, gamma,
-                                    _p(u0), _p(It), _p(Iu), _p(Tt))
+    fn = _lib.orc_tgv_linearise_f32 if prec == "f32" else _lib.orc_tgv_linearise_f64
+    rc = fn(_p(ref), _p(src), W, Hh, N, *[_p(x) for x in pose], alternative, beta, gamma,
+            _p(u0), _p(It), _p(Iu), _p(Tt))
     assert rc == 0
     return It, Iu, Tt
 

@@ -281,11 +281,19 @@ int sweep_ref(const float* reff, const float* srcf, int W, int H, int N, const f
     return 0;
 }
 
+/* Running (argmin, minimum, second-smallest) of the aggregated costs in
+ * plane order, strict '<' so that the first minimum wins. */
+template <class T>
+inline void reduce_best2(T c, int p, int32_t& arg, T& cmin, T& csec) {
+    if (c < cmin) { csec = cmin; cmin = c; arg = p; }
+    else if (c < csec) csec = c;
+}
+
 template <class T>
 int sweep_agg(const float* reff, const float* srcf, int W, int H, int N, const float* K,
               const float* Rref, const float* tref, const float* Rsrc, const float* tsrc, int alt,
               const float* planes, int P, int win, float stdthresh, int off,
-              int32_t* argmin, T* cmin, T* csec) {
+              int32_t* argmin, T* cmin, T* csec, T* volume = nullptr) {
     if (!window_ok(W, H, win) || N < 1 || P < 1) return 1;
     const size_t n = (size_t)W * H;
     std::vector<T> ref(reff, reff + n), refmean, refstd, nccv(n), acc(n);
@@ -308,8 +316,8 @@ int sweep_agg(const float* reff, const float* srcf, int W, int H, int N, const f
         }
         for (size_t j = 0; j < n; ++j) {
             T c = -acc[j] / (T)N;
-            if (c < cmin[j]) { csec[j] = cmin[j]; cmin[j] = c; argmin[j] = p; }
-            else if (c < csec[j]) csec[j] = c;
+            if (volume) volume[(size_t)p * n + j] = c;
+            reduce_best2(c, p, argmin[j], cmin[j], csec[j]);
         }
     }
     return 0;
@@ -496,6 +504,34 @@ void tgv_iteration(TgvState<T>& S, T lambda, T alpha0, T alpha1, T tau, T sigma)
         S.u1ybar[i] = (T)2 * u1yn[i] - S.u1y[i];
         S.u[i] = un[i]; S.u1x[i] = u1xn[i]; S.u1y[i] = u1yn[i];
     }
+}
+
+/* Iterations from a caller-given state, inside a given linearisation: the
+ * loop body of src/planesweep.cpp:485-509 without the warp reset of
+ * :442-483.  Planes of W*H in `state` (in and out): u, u1x, u1y, ubar,
+ * u1xbar, u1ybar, px, py, qx, qy, qz, qw, r[N]; Tt = T11, T12, T21, T22. */
+template <class T>
+int tgv_iterate(int W, int H, int N, const T* Tt, const T* u0, const T* It, const T* Iu, T* state, int niters,
+                double lambda, double alpha0, double alpha1, double tau, double sigma) {
+    if (W < 2 || H < 2 || N < 1 || niters < 0) return 1;
+    TgvState<T> S(W, H, N);
+    const size_t n = S.n;
+    std::vector<T>* planes[] = {&S.u, &S.u1x, &S.u1y, &S.ubar, &S.u1xbar, &S.u1ybar, &S.Px, &S.Py, &S.Qx, &S.Qy,
+                                &S.Qz, &S.Qw};
+    const T* in = state;
+    for (auto* v : planes) { std::copy(in, in + n, v->begin()); in += n; }
+    for (int j = 0; j < N; ++j) { std::copy(in, in + n, S.r[j].begin()); in += n; }
+    std::copy(u0, u0 + n, S.u0.begin());
+    for (auto* v : {&S.T1, &S.T2, &S.T3, &S.T4}) { std::copy(Tt, Tt + n, v->begin()); Tt += n; }
+    for (int j = 0; j < N; ++j) {
+        std::copy(It + (size_t)j * n, It + (size_t)(j + 1) * n, S.It[j].begin());
+        std::copy(Iu + (size_t)j * n, Iu + (size_t)(j + 1) * n, S.Iu[j].begin());
+    }
+    for (int it = 0; it < niters; ++it) tgv_iteration(S, (T)lambda, (T)alpha0, (T)alpha1, (T)tau, (T)sigma);
+    T* o = state;
+    for (auto* v : planes) { std::copy(v->begin(), v->end(), o); o += n; }
+    for (int j = 0; j < N; ++j) { std::copy(S.r[j].begin(), S.r[j].end(), o); o += n; }
+    return 0;
 }
 
 template <class T>
@@ -968,6 +1004,21 @@ int orc_sweep_agg_f64(const float* ref, const float* src, int W, int H, int N, c
                              coord_offset, argmin, cost_min, cost_second);
 }
 
+int orc_sweep_agg_volume_f64(const float* ref, const float* src, int W, int H, int N, const float* K,
+                             const float* Rref, const float* tref, const float* Rsrc, const float* tsrc,
+                             int alternative, const float* planes, int P, int winsize, float stdthresh,
+                             int coord_offset, int32_t* argmin, double* cost_min, double* cost_second,
+                             double* volume) {
+    return sweep_agg<double>(ref, src, W, H, N, K, Rref, tref, Rsrc, tsrc, alternative, planes, P, winsize, stdthresh,
+                             coord_offset, argmin, cost_min, cost_second, volume);
+}
+
+void orc_reduce_best2_f64(const double* costs, int P, int32_t* argmin, double* cost_min, double* cost_second) {
+    const double inf = std::numeric_limits<double>::infinity();
+    *argmin = -1; *cost_min = inf; *cost_second = inf;
+    for (int p = 0; p < P; ++p) reduce_best2(costs[p], p, *argmin, *cost_min, *cost_second);
+}
+
 int orc_tgv_f32(const float* ref, const float* src, int W, int H, int N, const float* K, const float* Rref,
                 const float* tref, const float* Rsrc, const float* tsrc, int alternative, int niters, int warps,
                 double lambda, double alpha0, double alpha1, double tau, double sigma, double beta, double gamma,
@@ -988,15 +1039,18 @@ void orc_tensor_f32(const float* img, int W, int H, float beta, float gamma, flo
     tensor<float>(img, W, H, beta, gamma, T11, T12, T21, T22);
 }
 
-int orc_tgv_linearise_f32(const float* ref, const float* src, int W, int H, int N, const float* K,
-                          const float* Rref, const float* tref, const float* Rsrc, const float* tsrc, int alternative,
-                          double beta, double gamma, const float* u0, float* It, float* Iu, float* Tout) {
+}  // extern "C"
+
+template <class T>
+static int tgv_linearise(const float* ref, const float* src, int W, int H, int N, const float* K, const float* Rref,
+                         const float* tref, const float* Rsrc, const float* tsrc, int alternative, double beta,
+                         double gamma, const float* u0, T* It, T* Iu, T* Tout) {
     if (W < 2 || H < 2 || N < 1) return 1;
-    TgvState<float> S(W, H, N);
-    std::vector<M3<float>> Rrel;
-    std::vector<V3<float>> trel;
-    tgv_setup(S, ref, src, K, Rref, tref, Rsrc, tsrc, alternative, (float)beta, (float)gamma, Rrel, trel);
-    for (size_t i = 0; i < S.n; ++i) S.u[i] = u0[i];
+    TgvState<T> S(W, H, N);
+    std::vector<M3<T>> Rrel;
+    std::vector<V3<T>> trel;
+    tgv_setup(S, ref, src, K, Rref, tref, Rsrc, tsrc, alternative, (T)beta, (T)gamma, Rrel, trel);
+    for (size_t i = 0; i < S.n; ++i) S.u[i] = (T)u0[i];
     tgv_warp_reset(S, K, Rrel, trel);
     for (int j = 0; j < N; ++j) {
         std::copy(S.It[j].begin(), S.It[j].end(), It + (size_t)j * S.n);
@@ -1007,6 +1061,32 @@ int orc_tgv_linearise_f32(const float* ref, const float* src, int W, int H, int 
     std::copy(S.T3.begin(), S.T3.end(), Tout + 2 * S.n);
     std::copy(S.T4.begin(), S.T4.end(), Tout + 3 * S.n);
     return 0;
+}
+
+extern "C" {
+
+int orc_tgv_linearise_f32(const float* ref, const float* src, int W, int H, int N, const float* K,
+                          const float* Rref, const float* tref, const float* Rsrc, const float* tsrc, int alternative,
+                          double beta, double gamma, const float* u0, float* It, float* Iu, float* Tout) {
+    return tgv_linearise<float>(ref, src, W, H, N, K, Rref, tref, Rsrc, tsrc, alternative, beta, gamma, u0, It, Iu,
+                                Tout);
+}
+int orc_tgv_linearise_f64(const float* ref, const float* src, int W, int H, int N, const float* K,
+                          const float* Rref, const float* tref, const float* Rsrc, const float* tsrc, int alternative,
+                          double beta, double gamma, const float* u0, double* It, double* Iu, double* Tout) {
+    return tgv_linearise<double>(ref, src, W, H, N, K, Rref, tref, Rsrc, tsrc, alternative, beta, gamma, u0, It, Iu,
+                                 Tout);
+}
+
+int orc_tgv_iterate_f32(int W, int H, int N, const float* Tt, const float* u0, const float* It, const float* Iu,
+                        float* state, int niters, double lambda, double alpha0, double alpha1, double tau,
+                        double sigma) {
+    return tgv_iterate<float>(W, H, N, Tt, u0, It, Iu, state, niters, lambda, alpha0, alpha1, tau, sigma);
+}
+int orc_tgv_iterate_f64(int W, int H, int N, const double* Tt, const double* u0, const double* It, const double* Iu,
+                        double* state, int niters, double lambda, double alpha0, double alpha1, double tau,
+                        double sigma) {
+    return tgv_iterate<double>(W, H, N, Tt, u0, It, Iu, state, niters, lambda, alpha0, alpha1, tau, sigma);
 }
 
 double orc_tgv_energy(const float* u, const float* u1x, const float* u1y, const float* u0, const float* Tt,

@@ -97,6 +97,18 @@ int orc_sweep_agg_f64(const float* ref, const float* src, int W, int H, int N,
                       const float* planes, int P, int winsize, float stdthresh, int coord_offset,
                       int32_t* argmin, double* cost_min, double* cost_second);
 
+/* Aggregated sweep in double that also writes the cost of every plane:
+ * volume[P][H][W] = -(sum over sources of NCC) / N. */
+int orc_sweep_agg_volume_f64(const float* ref, const float* src, int W, int H, int N,
+                             const float* K, const float* Rref, const float* tref,
+                             const float* Rsrc, const float* tsrc, int alternative,
+                             const float* planes, int P, int winsize, float stdthresh, int coord_offset,
+                             int32_t* argmin, double* cost_min, double* cost_second, double* volume);
+
+/* The aggregated sweep's per-pixel reduction over a cost sequence in plane
+ * order: argmin (first minimum), minimum and second-smallest cost. */
+void orc_reduce_best2_f64(const double* costs, int P, int32_t* argmin, double* cost_min, double* cost_second);
+
 /* ---- stage 2: TGV2 ----------------------------------------------------- */
 
 /* PlaneSweep::TGV (src/planesweep.cpp:381-536).  ref/src hold 0..255
@@ -129,6 +141,23 @@ int orc_tgv_linearise_f32(const float* ref, const float* src, int W, int H, int 
                           const float* Rsrc, const float* tsrc, int alternative,
                           double beta, double gamma, const float* u0,
                           float* It, float* Iu, float* T);
+
+int orc_tgv_linearise_f64(const float* ref, const float* src, int W, int H, int N,
+                          const float* K, const float* Rref, const float* tref,
+                          const float* Rsrc, const float* tsrc, int alternative,
+                          double beta, double gamma, const float* u0,
+                          double* It, double* Iu, double* T);
+
+/* niters primal-dual iterations (src/planesweep.cpp:485-509) from a given
+ * state inside a given linearisation (no warp reset).  state (in and out):
+ * planes of H*W u, u1x, u1y, ubar, u1xbar, u1ybar, px, py, qx, qy, qz, qw,
+ * r[N].  T = T11, T12, T21, T22 planes; It, Iu: N planes each. */
+int orc_tgv_iterate_f32(int W, int H, int N, const float* T, const float* u0, const float* It,
+                        const float* Iu, float* state, int niters, double lambda, double alpha0,
+                        double alpha1, double tau, double sigma);
+int orc_tgv_iterate_f64(int W, int H, int N, const double* T, const double* u0, const double* It,
+                        const double* Iu, double* state, int niters, double lambda, double alpha0,
+                        double alpha1, double tau, double sigma);
 
 /* Primal TGV2 energy of a state, in double:
  *   alpha1 * sum |T (grad u - u1)| + alpha0 * sum |eps(u1)|_F
