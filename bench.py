@@ -2,26 +2,31 @@
 """Throughput of the multiview-stereo pipeline on one MI355X, learned
 warm-start TGV2 against plain TGV2 at equal energy gap.
 
-Workload (one step = one refined 640x480 depthmap): a seeded synthetic room
-rendered on the GPU, 640x480 float32 reference view and 8 source views
-(0..255 intensities), pinhole K fx=fy=480, cx=319.5, cy=239.5; poses within
-5 cm / 2 degrees of the reference.
+Workload (one step = one refined 640x480 depthmap): seeded synthetic rooms
+rendered on the GPU (eight scenes, seeds 2024-2031; the network was trained
+on other seeds), 640x480 float32 reference view and 8 source views (0..255
+intensities), pinhole K fx=fy=480, cx=319.5, cy=239.5; poses within 5 cm /
+2 degrees of the reference.
   sweep : 256 fronto-parallel planes uniform in inverse depth over 0.3-8 m,
           9x9 NCC window, aggregated cost -(1/N) sum_s NCC, fused kernel with
           argmin / min / second-best (no cost volume in HBM);
   TGV2  : linearised at the sweep's argmin depth, lambda=1, alpha0=2,
           alpha1=1.5, tau=0.02, sigma=1, iterated until the primal energy is
           within 1e-4 (relative) of E*, the energy of the same problem after
-          40000 iterations from the plain start (computed before the timed
-          region), checked every 10 iterations, capped at 50000.  The energy
-          leaves out the ~0.2 % of pixels where the reference's fixed steps
-          make the data step unstable (tau sigma lambda^2 sum Iu^2 > 1 at the
-          pixel or a neighbour): the iteration oscillates there forever;
-  starts: learned = the BF16 U-Net's (u, u1, p, q) from the sweep results
-          (the headline value, network time included); plain = the
+          40000 iterations from the plain start (computed per scene before
+          the timed region), checked every 10 iterations, capped at 50000.
+          The energy leaves out the ~0.2 % of pixels where the reference's
+          fixed steps make the data step unstable (tau sigma lambda^2
+          sum Iu^2 > 1 at the pixel or a neighbour): the iteration oscillates
+          there forever;
+  starts: learned = the BF16 U-Net's (u, u1, p, q) from the sweep results,
+          kept only if its energy is below the plain start's, else the plain
+          start (the headline value, network time included); plain = the
           reference's warp start u = u0 (the sweep depth), u1 = p = q = r = 0;
           cold = the reference's constant u = 1.
-Inputs are resident in HBM when the timed region starts.  Prints one JSON line.
+Each start is timed over the same number of depthmaps, cycling over the
+scenes; the speed-up is the median over scenes.  Inputs are resident in HBM
+when the timed region starts.  Prints one JSON line.
 """
 import argparse
 import json
@@ -48,20 +53,32 @@ BF16_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 SWEEP_FLOP_PER_EVAL = 50          # declared FLOP per (pixel, plane, source) NCC evaluation
-TGV_FLOATS_PER_PX = 23 + 4 * NSRC  # algorithmic floats moved per pixel per fused iteration
+TGV_SURVEY_FLOATS_PER_PX = 27 + 4 * NSRC   # floats per pixel of one unfused iteration
+SEEDS = tuple(range(2024, 2032))           # eight scenes; the network was trained on seeds from 100000
+
+
+def tgv_compulsory_floats(m):
+    """Floats per pixel one fused launch of m iterations must move: the state
+    read once (u of k and k-1: 6, p 2, q 3, r N, T 3, u0 1, It N, Iu N) and
+    written once (u of the last two iterations: 6, p 2, q 3, r N)."""
+    return (15 + 3 * NSRC) + (11 + NSRC) if m > 1 else (15 + 3 * NSRC) + (8 + NSRC)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(v, planes, K0, K1, ds, plain_iters, sample_iters=300):
-    """FP32 oracle (the CPU restatement, OpenMP) on the same inputs: the full
-    aggregated sweep of one depthmap, then `sample_iters` TGV2 iterations
-    linearised at the GPU's sweep depth; the depthmap rate is the sweep time
-    plus the sampled per-iteration time times the plain pipeline's iteration
-    count (the CPU path has no learned start)."""
+def cpu_baseline(v, planes, K0, K1, ds, plain_iters, fusion_o=None, fusion_thr=None, sample_iters=300):
+    """FP32 oracle (the CPU restatement, OpenMP) on the same inputs.  The
+    headline: the full aggregated sweep of one depthmap, then `sample_iters`
+    TGV2 iterations linearised at the GPU's sweep depth; the depthmap rate is
+    the sweep time plus the sampled per-iteration time times the plain
+    pipeline's iteration count (the CPU path has no learned start).  Beside
+    it: the small plain pipeline (640x480, 4 sources, 64 planes over 0.5-5 m,
+    7x7, 200 cold-start TGV2 iterations) and fusion primal-dual iterations on
+    a 128^3 sub-grid of the fusion volume, per voxel-iteration."""
     import oracle
+    from psw_amd import stages
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t0 = time.perf_counter()
     oracle.sweep_agg(v["ref"], v["src"], K0, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], planes, WIN, STD_T)
@@ -70,10 +87,38 @@ def cpu_baseline(v, planes, K0, K1, ds, plain_iters, sample_iters=300):
     oracle.tgv(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], sample_iters, 1, u_init=ds, **TGV)
     t_it = (time.perf_counter() - t0) / sample_iters
     t_map = t_sweep + t_it * plain_iters
-    return {"value": 1.0 / t_map, "unit": "depthmaps/s", "cores": cores, "kind": "port",
-            "sample": f"FP32 oracle: one full aggregated sweep ({NPLANES} planes x {NSRC} sources, 640x480) and "
-                      f"{sample_iters} TGV2 iterations, scaled to the plain pipeline's {plain_iters} iterations",
-            "sweep_s": t_sweep, "tgv_iter_ms": t_it * 1e3}
+    out = {"value": 1.0 / t_map, "unit": "depthmaps/s", "cores": cores, "kind": "port",
+           "sample": f"FP32 oracle: one full aggregated sweep ({NPLANES} planes x {NSRC} sources, 640x480) and "
+                     f"{sample_iters} TGV2 iterations, scaled to the plain pipeline's {plain_iters} iterations",
+           "sweep_s": t_sweep, "tgv_iter_ms": t_it * 1e3}
+    # the small plain pipeline: 4 sources, 64 planes, 7x7, 200 cold-start iterations
+    src4 = np.ascontiguousarray(v["src"][:4])
+    p64 = stages.inverse_depth_planes(0.5, 5.0, 64)
+    t0 = time.perf_counter()
+    oracle.sweep_agg(v["ref"], src4, K0, v["Rref"], v["tref"], v["Rsrc"][:4], v["tsrc"][:4], p64, 7, STD_T)
+    t_s4 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.tgv(v["ref"], src4, K1, v["Rref"], v["tref"], v["Rsrc"][:4], v["tsrc"][:4], 200, 1, **TGV)
+    t_t4 = time.perf_counter() - t0
+    out["small_pipeline"] = {"value": 1.0 / (t_s4 + t_t4), "unit": "depthmaps/s", "sweep_s": t_s4, "tgv_s": t_t4,
+                             "sample": "640x480, 4 sources, 64 planes 0.5-5 m, 7x7, TGV2 cold start 200 iterations"}
+    if fusion_o is not None:
+        g, n_it = 128, 20
+        hist = np.zeros((g, g, g, 8), np.uint16)
+        for k in range(fusion_o["depths"].shape[0]):
+            oracle.fusion_hist(hist, FUSION_VOLUME, fusion_o["depths"][k], fusion_o["K"], fusion_o["R"][k],
+                               fusion_o["t"][k], fusion_thr)
+        u, vv, pp = (np.zeros((g, g, g), np.float32), np.zeros((g, g, g), np.float32),
+                     np.zeros((3, g, g, g), np.float32))
+        t0 = time.perf_counter()
+        for _ in range(n_it):
+            oracle.fusion_u(u, vv, pp, hist, 0.1, 0.5)
+            oracle.fusion_p(pp, vv, 1.0)
+        t_pd = time.perf_counter() - t0
+        out["fusion_pd"] = {"value": g ** 3 * n_it / t_pd, "unit": "voxel-iterations/s",
+                            "sample": f"{g}^3 sub-grid of the fusion volume, histograms of the same "
+                                      f"{fusion_o['depths'].shape[0]} depthmaps, {n_it} U+P iterations"}
+    return out
 
 
 # ---- fusion workload: 256^3 grid, 50 depthmaps, 300 primal-dual iterations ----
@@ -216,7 +261,7 @@ def run_tvl1(pl, stream, ev):
 class Pipeline:
     """Device state of the benchmarked pipeline for one seeded view."""
 
-    def __init__(self, seed, weights):
+    def __init__(self, seed, weights, ws=None):
         import psw_amd
         from psw_amd import scene, stages, warmstart
         from psw_amd.device import Event, Stream
@@ -237,8 +282,11 @@ class Pipeline:
         self.tgv.set_images(self.sweep.d_ref, self.sweep.d_src, TGV["beta"], TGV["gamma"], s)
         self.Rrel, self.trel = stages.relative_all(v["Rref"], v["tref"], v["Rsrc"], v["tsrc"])
         self.prm = stages.tgv_params(**TGV)
-        self.ws = warmstart.WarmStarter(W, H, weights) if weights else None
+        self.ws = ws if ws is not None else (warmstart.WarmStarter(W, H, weights) if weights else None)
         self.ev = [Event() for _ in range(4)]
+        self.seed = seed
+        self.e_star = None
+        self.last_start = None
 
     def sweep_and_linearise(self, fast=True):
         s = self.st.handle
@@ -260,11 +308,14 @@ class Pipeline:
         self.tgv.set_u(d_s, stream=s)
         self.tgv.linearise(self.K1, self.Rrel, self.trel, s)
         self.tgv.set_energy_mask(QMAX, self.prm, s, count=False)
+        self.last_start = mode
         if mode == "learned":
-            self.ws.start(self.tgv, s)
-        elif mode == "cold":
-            self.tgv.set_primal(None, 1.0, None, s)
-        iters, e = self.tgv.solve(max_iters, check_every, GAP_TOL, e_star, self.prm, s)
+            iters, e, self.last_start = self.ws.solve(self.tgv, d_s, self.prm, e_star, GAP_TOL, max_iters, check_every,
+                                                      stream=s)
+        else:
+            if mode == "cold":
+                self.tgv.set_primal(None, 1.0, None, s)
+            iters, e = self.tgv.solve(max_iters, check_every, GAP_TOL, e_star, self.prm, s)
         ev[3].record(st)
         ev[3].synchronize()
         return {"sweep_ms": ev[0].elapsed_ms(ev[1]), "net_ms": ev[1].elapsed_ms(ev[2]),
@@ -348,6 +399,8 @@ def run_batch(args):
     solvers = [f.tgv for f in frames]
     sweeps = [f.sweep for f in frames]
 
+    os.environ["PSW_SWEEP_TY"] = str(best["sweep_tile_h"])
+
     def sweep_all():
         for sw in sweeps:
             sw.build_homographies(s)
@@ -399,7 +452,7 @@ def run_batch(args):
                                f"({args.batch_estar_iters}-iteration run), energy mask qmax {QMAX}",
                    "frames": specs, "megapixels": npx / 1e6, "seed": args.seed},
         "autotune": {"chosen": best,
-                     "sweep_ms": {f"groups{g}_wg{w}": v for (g, w), v in timings["sweep"].items()},
+                     "sweep_ms": {f"tile32x{ty}_groups{g}_wg{w}": v for (ty, g, w), v in timings["sweep"].items()},
                      "tgv_us_per_iteration": {f"variant{v}": ms * 1e3 for v, ms in timings["tgv"].items()},
                      "solve_us_per_iteration": {("graph" if g else "direct"): ms * 1e3
                                                 for g, ms in timings["graph"].items()}},
@@ -415,9 +468,10 @@ def main():
     from psw_amd import warmstart
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5, help="timed learned warm-start depthmaps")
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--plain-steps", type=int, default=2, help="timed plain-start depthmaps")
+    ap.add_argument("--steps", type=int, default=16,
+                    help="timed depthmaps per start (learned, plain, cold), cycling over the scenes")
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--seeds", default=",".join(str(x) for x in SEEDS), help="comma list of scene seeds")
     ap.add_argument("--exact-sweep", action="store_true", help="reference-order (bit-exact) sweep arithmetic")
     ap.add_argument("--max-iters", type=int, default=MAX_ITERS)
     ap.add_argument("--check-every", type=int, default=10)
@@ -425,7 +479,7 @@ def main():
     ap.add_argument("--weights", default=warmstart.DEFAULT_WEIGHTS)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cold", action="store_true", help="skip the constant-start (u = 1) run")
-    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--seed", type=int, default=SEEDS[0], help="scene of --probe and --batch")
     ap.add_argument("--no-fusion", action="store_true", help="skip the 256^3 fusion measurement")
     ap.add_argument("--batch", action="store_true", help="run the uneven 32-frame batch instead (its own JSON line)")
     ap.add_argument("--batch-frames", type=int, default=32)
@@ -448,78 +502,124 @@ def main():
                  "(python -m psw_amd.warmstart train --out <file>)")
 
     import psw_amd
+    from psw_amd.device import Event, Stream
     log("device:", psw_amd.device_name())
-    pl = Pipeline(args.seed, args.weights)
-    s, st, ev = pl.st.handle, pl.st, pl.ev
     fast = not args.exact_sweep
-
-    # E*: the same problem (linearised at the sweep depth) iterated from the
-    # plain start; not timed
-    d_s = pl.sweep_and_linearise(fast)
-    pl.tgv.set_u(d_s, stream=s)
-    pl.tgv.linearise(pl.K1, pl.Rrel, pl.trel, s)
-    excluded = pl.tgv.set_energy_mask(QMAX, pl.prm, s)
-    pl.tgv.iterate(args.estar_iters - 5000, pl.prm, s)
-    e_prev = pl.tgv.energy(pl.prm, s)
-    pl.tgv.iterate(5000, pl.prm, s)
-    e_star = pl.tgv.energy(pl.prm, s)
-    u_star = pl.tgv.get_u(s)
-    st.synchronize()
-    ds_host = d_s.to_host(s)
-    log(f"E* = {e_star:.9e} after {args.estar_iters} iterations (change over the last 5000: "
-        f"{(e_prev - e_star) / e_star:.2e}); {excluded} pixels outside the energy mask")
+    seeds = [int(x) for x in args.seeds.split(",")]
+    ws = warmstart.WarmStarter(W, H, args.weights)
+    pipes = []
+    for sd in seeds:
+        pl = Pipeline(sd, args.weights, ws=ws)
+        s, st = pl.st.handle, pl.st
+        # E*: the same problem (linearised at the sweep depth) iterated from
+        # the plain start; not timed
+        d_s = pl.sweep_and_linearise(fast)
+        pl.tgv.set_u(d_s, stream=s)
+        pl.tgv.linearise(pl.K1, pl.Rrel, pl.trel, s)
+        pl.excluded = pl.tgv.set_energy_mask(QMAX, pl.prm, s)
+        pl.tgv.iterate(args.estar_iters - 5000, pl.prm, s)
+        e_prev = pl.tgv.energy(pl.prm, s)
+        pl.tgv.iterate(5000, pl.prm, s)
+        pl.e_star = pl.tgv.energy(pl.prm, s)
+        pl.estar_change = (e_prev - pl.e_star) / pl.e_star
+        pl.u_star = pl.tgv.get_u(s)
+        pl.weight = pl.tgv.get_energy_mask(s)
+        pl.d_s_host = d_s.to_host(s)
+        pl.true_depth = pl.v["d_depth"].to_host(s)
+        st.synchronize()
+        log(f"seed {sd}: E* = {pl.e_star:.9e} after {args.estar_iters} iterations (change over the last 5000: "
+            f"{pl.estar_change:.2e}); {pl.excluded} pixels outside the energy mask")
+        pipes.append(pl)
 
     def timed(mode, k, warm):
-        for _ in range(warm):
-            pl.run(mode, e_star, args.max_iters, args.check_every, fast)
-        st.synchronize()
-        recs = []
+        """k depthmaps (cycling over the scenes) after `warm` untimed ones."""
+        for i in range(warm):
+            pipes[i % len(pipes)].run(mode, pipes[i % len(pipes)].e_star, args.max_iters, args.check_every, fast)
+        for pl in pipes:
+            pl.st.synchronize()
+        recs = [[] for _ in pipes]
         t0 = time.perf_counter()
-        for _ in range(k):
-            recs.append(pl.run(mode, e_star, args.max_iters, args.check_every, fast))
-        st.synchronize()
+        for i in range(k):
+            pl = pipes[i % len(pipes)]
+            r = pl.run(mode, pl.e_star, args.max_iters, args.check_every, fast)
+            r["start"] = pl.last_start
+            recs[i % len(pipes)].append(r)
+        for pl in pipes:
+            pl.st.synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / k
-        u = pl.tgv.get_u(s)
-        st.synchronize()
-        agg = {key: float(np.mean([r[key] for r in recs])) for key in ("sweep_ms", "net_ms", "tgv_ms", "iters")}
-        agg.update(ms_per_depthmap=ms, depthmaps_per_s=1e3 / ms,
-                   rel_gap_at_stop=float(max(abs(r["energy"] - e_star) / e_star for r in recs)),
-                   reached_gap=bool(all(abs(r["energy"] - e_star) <= GAP_TOL * e_star for r in recs)),
-                   rms_rel_diff_vs_estar_state=float(np.sqrt(np.mean(((u - u_star) / u_star) ** 2))),
-                   median_rel_err_vs_true_depth=float(np.median(np.abs(u - pl.v["d_depth"].to_host())
-                                                                / pl.v["d_depth"].to_host())))
-        return agg
+        return ms, recs
 
-    learned = timed("learned", args.steps, args.warmup)
-    log(f"learned: {learned['ms_per_depthmap']:.3f} ms/depthmap ({learned['iters']:.0f} TGV2 iterations, "
-        f"network {learned['net_ms']:.3f} ms)")
-    plain = timed("plain", args.plain_steps, 0)
-    log(f"plain: {plain['ms_per_depthmap']:.3f} ms/depthmap ({plain['iters']:.0f} TGV2 iterations)")
-    cold = None
-    if not args.no_cold:
-        cold = timed("cold", 1, 0)
-        log(f"cold: {cold['ms_per_depthmap']:.3f} ms/depthmap ({cold['iters']:.0f} iterations, gap "
-            f"{cold['rel_gap_at_stop']:.2e})")
-    ms_per_step = learned["ms_per_depthmap"]
+    res = {}
+    for mode in ("learned", "plain", "cold"):
+        if mode == "cold" and args.no_cold:
+            continue
+        ms, recs = timed(mode, args.steps, args.warmup if mode == "learned" else 0)
+        res[mode] = (ms, recs)
+        log(f"{mode}: {ms:.3f} ms/depthmap over {args.steps} depthmaps of {len(pipes)} scenes")
+    ms_per_step = res["learned"][0]
 
+    # per scene: times, iterations, stop states (one more untimed run of each start)
+    per_seed = []
+    for j, pl in enumerate(pipes):
+        s = pl.st.handle
+        rec = {"seed": pl.seed, "e_star": pl.e_star, "estar_change_last_5000": pl.estar_change,
+               "excluded_pixels": pl.excluded}
+        u_stop = {}
+        for mode, (_, recs) in res.items():
+            rs = recs[j]
+            rec[mode] = {"runs": len(rs), "iters": float(np.mean([r["iters"] for r in rs])),
+                         "ms_per_depthmap": float(np.mean([r["sweep_ms"] + r["net_ms"] + r["tgv_ms"] for r in rs])),
+                         "sweep_ms": float(np.mean([r["sweep_ms"] for r in rs])),
+                         "net_ms": float(np.mean([r["net_ms"] for r in rs])),
+                         "tgv_ms": float(np.mean([r["tgv_ms"] for r in rs])),
+                         "max_rel_gap_at_stop": float(max(abs(r["energy"] - pl.e_star) / pl.e_star for r in rs)),
+                         "reached_gap": bool(all(abs(r["energy"] - pl.e_star) <= GAP_TOL * pl.e_star for r in rs)),
+                         "starts": sorted({r["start"] for r in rs})}
+            pl.run(mode, pl.e_star, args.max_iters, args.check_every, fast)
+            u_stop[mode] = pl.tgv.get_u(s)
+            pl.st.synchronize()
+            rec[mode]["stop_state_vs_estar_state"] = warmstart.judge(u_stop[mode], pl.u_star, pl.weight)
+            rec[mode]["median_rel_err_vs_true_depth"] = float(np.median(np.abs(u_stop[mode] - pl.true_depth)
+                                                                        / pl.true_depth))
+        rec["speedup_vs_plain"] = rec["plain"]["ms_per_depthmap"] / rec["learned"]["ms_per_depthmap"]
+        if "cold" in res:
+            rec["speedup_vs_cold"] = rec["cold"]["ms_per_depthmap"] / rec["learned"]["ms_per_depthmap"]
+        # the plain start's solve to the same gap judges the learned one
+        rec["judge_learned_vs_plain_stop"] = warmstart.judge(u_stop["learned"], u_stop["plain"], pl.weight)
+        per_seed.append(rec)
+        log(f"seed {pl.seed}: iterations learned {rec['learned']['iters']:.0f} plain {rec['plain']['iters']:.0f}"
+            + (f" cold {rec['cold']['iters']:.0f}" if "cold" in rec else "")
+            + f"; speedup vs plain {rec['speedup_vs_plain']:.3f}")
+    sp = np.array([r["speedup_vs_plain"] for r in per_seed])
+    spc = np.array([r["speedup_vs_cold"] for r in per_seed]) if "cold" in res else None
+
+    pl = pipes[0]
+    s, st, ev = pl.st.handle, pl.st, pl.ev
     # Per-launch duration of the fused TGV2 iteration kernel (same stream and
     # buffers): events around back-to-back launches.  By default a launch
     # does two iterations (tgv_iter2_kernel); PSW_TGV_TRIPLE=1 selects the
     # three-iteration kernel, PSW_TGV_DOUBLE=0 the one-iteration kernel.
     two = os.environ.get("PSW_TGV_DOUBLE", "1") != "0"
     three = two and os.environ.get("PSW_TGV_TRIPLE", "0") == "1"
-    tgv_kernel, iters_per_launch = (("tgv_iter3_kernel", 3) if three else ("tgv_iter2_kernel", 2) if two
-                                    else ("tgv_iter_kernel", 1))
+    tgv_kernel, m = (("tgv_iter3_kernel", 3) if three else ("tgv_iter2_kernel", 2) if two else ("tgv_iter_kernel", 1))
+    d_s = pl.sweep_and_linearise(fast)
     pl.tgv.set_u(d_s, stream=s)
     pl.tgv.linearise(pl.K1, pl.Rrel, pl.trel, s)
     nl = 198                     # whole launches of 1, 2 or 3 iterations
     ev[0].record(st)
-    pl.tgv.iterate(nl, pl.prm, s)
+    pl.tgv.iterate(nl * m, pl.prm, s)
     ev[1].record(st)
     ev[1].synchronize()
-    tgv_launch_ms = ev[0].elapsed_ms(ev[1]) / nl * iters_per_launch
-    tgv_bytes = 4 * TGV_FLOATS_PER_PX * W * H * iters_per_launch
-    achieved = tgv_bytes / (tgv_launch_ms * 1e-3) / 1e9
+    launch_ms = ev[0].elapsed_ms(ev[1]) / nl
+    compulsory = 4 * tgv_compulsory_floats(m) * W * H
+    survey_bytes = 4 * TGV_SURVEY_FLOATS_PER_PX * W * H * m
+    achieved = compulsory / (launch_ms * 1e-3) / 1e9
+    traffic, tsrc = None, None
+    tfile = os.path.join(ROOT, "profiles", tgv_kernel + "_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+        tsrc = f"constant: rocprofv3 FETCH_SIZE/WRITE_SIZE runs of {tgv_kernel} at this size, profiles/{tgv_kernel}_traffic.json"
     # the sweep kernel alone
     ev[0].record(st)
     pl.sweep.run(fast=fast, stream=s)
@@ -536,12 +636,7 @@ def main():
     net_ms = ev[0].elapsed_ms(ev[1]) / 10
     net_tflops = warmstart.forward_flops(W, H) / (net_ms * 1e-3) / 1e12
 
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", tgv_kernel + "_traffic.json")
-    if os.path.exists(tfile):
-        with open(tfile) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
-
+    learned_ms, plain_ms = res["learned"][0], res["plain"][0]
     rec = {
         "metric": "640x480 depthmaps/s at equal TGV2 gap: learned warm-start vs plain TGV2",
         "value": 1e3 / ms_per_step,
@@ -554,51 +649,56 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded room scene rendered on the GPU)",
+        "data": f"synthetic (seeded room scenes rendered on the GPU, seeds {seeds[0]}-{seeds[-1]}, none used in training)",
         "config": {"workload": "learned warm-start pipeline: 640x480 reference + 8 sources, 256 inverse-depth planes "
-                               "0.3-8 m, 9x9 NCC, fused sweep+argmin, BF16 U-Net warm start, TGV2 linearised at the "
-                               "sweep depth to relative energy gap 1e-4 of E* (40000-iteration plain run)",
+                               "0.3-8 m, 9x9 NCC, fused sweep+argmin, BF16 U-Net warm start (plain start as "
+                               "fallback), TGV2 linearised at the sweep depth to relative energy gap 1e-4 of E* "
+                               f"({args.estar_iters}-iteration plain run); steps cycle over {len(seeds)} scenes",
                    "sweep_arith": "fast" if fast else "reference-order", "tgv_gap_tol": GAP_TOL,
                    "check_every": args.check_every, "max_iters": args.max_iters, "estar_iters": args.estar_iters,
-                   "seed": args.seed, "network_dtype": "bf16 (fp32 accumulation)"},
-        "speedup_vs_plain": plain["ms_per_depthmap"] / ms_per_step,
-        "learned": learned,
-        "plain": plain,
-        "cold": cold,
-        "e_star": e_star,
-        "energy_mask": {"qmax": QMAX, "excluded_pixels": excluded, "estar_change_last_5000": (e_prev - e_star) / e_star},
+                   "seeds": seeds, "network_dtype": "bf16 (fp32 accumulation)"},
+        "speedup_vs_plain": float(np.median(sp)),
+        "speedup_vs_plain_range": [float(sp.min()), float(sp.max())],
+        "speedup_vs_plain_aggregate": plain_ms / learned_ms,
+        "speedup_vs_cold": float(np.median(spc)) if spc is not None else None,
+        "plain": {"ms_per_depthmap": plain_ms, "depthmaps_per_s": 1e3 / plain_ms, "runs": args.steps},
+        "cold": ({"ms_per_depthmap": res["cold"][0], "depthmaps_per_s": 1e3 / res["cold"][0], "runs": args.steps}
+                 if "cold" in res else None),
+        "per_seed": per_seed,
         "roofline": {"kernel": tgv_kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_launch": tgv_bytes, "launch_ms": tgv_launch_ms,
-                     "iterations_per_launch": iters_per_launch,
-                     "bytes_note": f"algorithmic bytes of {iters_per_launch} TGV2 iteration(s): "
-                                   f"{4 * TGV_FLOATS_PER_PX} B per pixel per iteration"},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": tsrc,
+                     "bytes_per_launch": compulsory, "launch_ms": launch_ms, "iterations_per_launch": m,
+                     "bytes_note": f"compulsory bytes of one launch of {m} iteration(s) ({4 * tgv_compulsory_floats(m)}"
+                                   " B/px: the state read once and written once); the per-iteration count of an "
+                                   f"unfused iteration, {4 * TGV_SURVEY_FLOATS_PER_PX} B/px, gives "
+                                   f"{survey_bytes / 1e6:.1f} MB per launch "
+                                   f"({survey_bytes / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS:.2f} of peak), which "
+                                   "counts bytes the fused launch never moves"},
         "stages": {"sweep_kernel_ms": sweep_kernel_ms, "sweep_tflops_declared": sweep_tflops,
                    "sweep_frac_fp32_peak": sweep_tflops / FP32_PEAK_TFLOPS, "unet_ms": net_ms,
                    "unet_tflops": net_tflops, "unet_frac_bf16_peak": net_tflops / BF16_PEAK_TFLOPS,
                    "unet_fwd_gflop": warmstart.forward_flops(W, H) / 1e9},
     }
-    from psw_amd.device import Event, Stream
-    rec["sparse_tgv"] = run_sparse(pl, Stream(), [Event() for _ in range(2)], args.seed)
-    log(f"sparse TGV2: {rec['sparse_tgv']['iter_us']:.2f} us per iteration "
-        f"({rec['sparse_tgv']['gbs_algorithmic']:.0f} GB/s algorithmic)")
+    rec["sparse_tgv"] = run_sparse(pl, Stream(), [Event() for _ in range(2)], seeds[0])
+    log(f"sparse TGV2: {rec['sparse_tgv']['iter_us']:.2f} us per iteration")
     rec["tvl1"] = run_tvl1(pl, Stream(), [Event() for _ in range(2)])
     log(f"TVL1: {rec['tvl1']['iter_us']:.2f} us per iteration fused, "
         f"{rec['tvl1']['launch_sequence_iter_us']:.2f} us as two launches per iteration")
     if not args.no_fusion:
         t0 = time.perf_counter()
-        o, thr = fusion_workload(args.seed)
+        o, thr = fusion_workload(seeds[0])
         log(f"fusion depthmaps rendered in {time.perf_counter() - t0:.1f} s")
-        from psw_amd.device import Event, Stream
         rec["fusion"] = run_fusion(o, thr, Stream(), [Event() for _ in range(3)])
         log(f"fusion: histogram {rec['fusion']['hist_ms']:.2f} ms, {FUSION_ITERS} PD iterations "
-            f"{rec['fusion']['pd_ms']:.2f} ms ({rec['fusion']['pd_gbs_algorithmic']:.0f} GB/s algorithmic)")
+            f"{rec['fusion']['pd_ms']:.2f} ms")
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
         vh = {"ref": pl.v["d_ref"].to_host(), "src": pl.v["d_src"].to_host(), "Rref": pl.v["Rref"],
               "tref": pl.v["tref"], "Rsrc": pl.v["Rsrc"], "tsrc": pl.v["tsrc"]}
-        rec["cpu_baseline"] = cpu_baseline(vh, pl.planes, pl.K0, pl.K1, ds_host, int(round(plain["iters"])))
-        log(f"cpu baseline sample took {time.perf_counter() - t0:.1f} s")
+        plain_iters = int(round(per_seed[0]["plain"]["iters"]))
+        rec["cpu_baseline"] = cpu_baseline(vh, pl.planes, pl.K0, pl.K1, pl.d_s_host, plain_iters,
+                                           o if not args.no_fusion else None, thr if not args.no_fusion else None)
+        log(f"cpu baseline samples took {time.perf_counter() - t0:.1f} s")
     else:
         rec["cpu_baseline"] = None
     print(json.dumps(rec), flush=True)

@@ -358,6 +358,9 @@ int psw_tgv_solve_batch(psw_tgv* const* solvers, int nsolvers, const psw_tgv_par
  * qx, qy, qz, It[N], Iu[N], r[N], T11, T12, T22, u0. */
 int psw_tgv_get_u(psw_tgv* t, float* d_out, psw_stream_t s);
 int psw_tgv_get_state(psw_tgv* t, float* d_out, psw_stream_t s);
+/* The energy mask's weights (W*H floats: 1 where the pixel is counted, 0
+ * where psw_tgv_set_energy_mask left it out; no reference counterpart). */
+int psw_tgv_get_energy_mask(psw_tgv* t, float* d_out, psw_stream_t s);
 /* Whole reference driver PlaneSweep::TGV: warps x niters from u_init (NULL:
  * cold start u = 1). */
 int psw_tgv_run(psw_tgv* t, const float* K, const float* Rrel, const float* trel, int niters, int warps,
@@ -433,9 +436,11 @@ int psw_fusion_surface_count(const psw_fusion_grid* g, int64_t* count, psw_strea
 /* d_depth (device, W*H) is denoised into d_out with the reference's scaling
  * to [0,1] over [znear, zfar]; d_ref holds 0..255 intensities for the tensor.
  * d_work: 12*W*H floats of device workspace (the solver state is
- * double-buffered).  PSW_TVL1_FUSED=0 selects the reference's two launches
- * per iteration instead of the fused one- and two-iteration launches; the
- * results are identical. */
+ * double-buffered).  The fused kernel runs up to PSW_TVL1_M iterations per
+ * launch (2 to 6, default 4; at most 4 with the 64x16 tiles of
+ * PSW_TVL1_TY=16), a shorter launch for the remainder; PSW_TVL1_FUSED=0
+ * selects the reference's two launches per iteration instead.  The results
+ * are identical either way. */
 int psw_tvl1_denoise(const float* d_depth, const float* d_ref, int width, int height, float znear, float zfar,
                      int niters, double lambda, double tau, double sigma, double theta, double beta, double gamma,
                      float* d_out, float* d_work, psw_stream_t s);

@@ -44,6 +44,8 @@ _SIGS = {
                          _vp, _vp]),
     "orc_sweep_agg_volume_f64": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _f, _i, _vp,
                                       _vp, _vp, _vp]),
+    "orc_sweep_agg_volume_f32": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _f, _i, _vp,
+                                      _vp, _vp, _vp]),
     "orc_reduce_best2_f64": (None, [_vp, _i, _vp, _vp, _vp]),
     "orc_tgv_linearise_f64": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _d, _d, _vp, _vp, _vp, _vp]),
     "orc_tgv_iterate_f32": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _d, _d, _d, _d, _d]),
@@ -52,6 +54,7 @@ _SIGS = {
     "orc_tgv_linearise_f32": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _d, _d, _vp, _vp, _vp, _vp]),
     "orc_tgv_energy": (_d, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _d, _d]),
     "orc_tgv_energy_f64": (_d, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _d, _d]),
+    "orc_tgv_energy_masked_f64": (_d, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _d, _d, _vp]),
     "orc_fusion_bin_centers": (None, [_i, _vp]),
     "orc_fusion_hist_f32": (None, [_vp, _i, _i, _i, _vp, _vp, _i, _i, _vp, _vp, _vp, _f]),
     "orc_fusion_hist_f64": (None, [_vp, _i, _i, _i, _vp, _vp, _i, _i, _vp, _vp, _vp, _f]),
@@ -179,19 +182,21 @@ def sweep_agg(ref, src, K, Rref, tref, Rsrc, tsrc, planes, winsize, stdthresh, c
 
 
 def sweep_agg_volume(ref, src, K, Rref, tref, Rsrc, tsrc, planes, winsize, stdthresh, coord_offset=0,
-                     alternative=0):
-    """FP64 aggregated sweep that also returns the cost of every plane:
-    (argmin, min, second, volume[P][H][W])."""
+                     alternative=0, prec="f64"):
+    """Aggregated sweep that also returns the cost of every plane:
+    (argmin, min, second, volume[P][H][W]), in FP64 or in the reference's
+    float arithmetic (prec "f32")."""
     ref, src, planes = _c(ref), _c(src), _c(planes)
     Hh, W = ref.shape
     N = src.shape[0]
+    T = _ft(prec)
     arg = np.zeros((Hh, W), np.int32)
-    cmin, csec = np.zeros((Hh, W)), np.zeros((Hh, W))
-    vol = np.zeros((planes.size, Hh, W))
+    cmin, csec = np.zeros((Hh, W), T), np.zeros((Hh, W), T)
+    vol = np.zeros((planes.size, Hh, W), T)
     pose = [_c(x).ravel() for x in (K, Rref, tref, Rsrc, tsrc)]
-    rc = _lib.orc_sweep_agg_volume_f64(_p(ref), _p(src), W, Hh, N, *[_p(x) for x in pose], alternative, _p(planes),
-                                       planes.size, winsize, stdthresh, coord_offset, _p(arg), _p(cmin), _p(csec),
-                                       _p(vol))
+    fn = _lib.orc_sweep_agg_volume_f64 if prec == "f64" else _lib.orc_sweep_agg_volume_f32
+    rc = fn(_p(ref), _p(src), W, Hh, N, *[_p(x) for x in pose], alternative, _p(planes), planes.size, winsize,
+            stdthresh, coord_offset, _p(arg), _p(cmin), _p(csec), _p(vol))
     assert rc == 0, rc
     return arg, cmin, csec, vol
 
@@ -282,8 +287,15 @@ def tgv_linearise(ref, src, K, Rref, tref, Rsrc, tsrc, u0, beta=0.0, gamma=1.0, 
     return It, Iu, Tt
 
 
-def tgv_energy(u, u1x, u1y, u0, T4, It, Iu, lambda_, alpha0, alpha1):
-    """T4 = [T11, T12, T21, T22] planes."""
+def tgv_energy(u, u1x, u1y, u0, T4, It, Iu, lambda_, alpha0, alpha1, weight=None):
+    """T4 = [T11, T12, T21, T22] planes.  weight (H, W), optional: pixels
+    where it is 0 are left out (FP64 sum)."""
+    if weight is not None:
+        a = [_c(x, np.float64) for x in (u, u1x, u1y, u0, T4, It, Iu)]
+        w = _c(weight)
+        Hh, W = a[0].shape
+        return _lib.orc_tgv_energy_masked_f64(*[_p(x) for x in a], W, Hh, a[5].shape[0], lambda_, alpha0, alpha1,
+                                              _p(w))
     if np.asarray(u).dtype == np.float64:
         a = [_c(x, np.float64) for x in (u, u1x, u1y, u0, T4, It, Iu)]
         fn = _lib.orc_tgv_energy_f64

@@ -696,7 +696,7 @@ void compute3d(const float* depth, int W, int H, const float* Kf, const float* R
 
 template <class S>
 double energy(const S* u, const S* u1x, const S* u1y, const S* u0, const S* Tt, const S* It, const S* Iu, int W,
-              int H, int N, double lambda, double alpha0, double alpha1) {
+              int H, int N, double lambda, double alpha0, double alpha1, const float* weight = nullptr) {
     const size_t n = (size_t)W * H;
     const S *T11 = Tt, *T12 = Tt + n, *T21 = Tt + 2 * n, *T22 = Tt + 3 * n;
     double e1 = 0, e0 = 0, ed = 0;
@@ -704,6 +704,7 @@ double energy(const S* u, const S* u1x, const S* u1y, const S* u0, const S* Tt, 
     for (int y = 0; y < H; ++y)
         for (int x = 0; x < W; ++x) {
             const size_t i = (size_t)y * W + x;
+            if (weight && weight[i] == 0.f) continue;     // pixel left out of the energy
             const size_t ixn = (size_t)y * W + std::min(x + 1, W - 1), iyn = (size_t)std::min(y + 1, H - 1) * W + x;
             double gx = ((double)u[ixn] - (double)u[i]) - (double)u1x[i];
             double gy = ((double)u[iyn] - (double)u[i]) - (double)u1y[i];
@@ -1013,6 +1014,14 @@ int orc_sweep_agg_volume_f64(const float* ref, const float* src, int W, int H, i
                              coord_offset, argmin, cost_min, cost_second, volume);
 }
 
+int orc_sweep_agg_volume_f32(const float* ref, const float* src, int W, int H, int N, const float* K,
+                             const float* Rref, const float* tref, const float* Rsrc, const float* tsrc,
+                             int alternative, const float* planes, int P, int winsize, float stdthresh,
+                             int coord_offset, int32_t* argmin, float* cost_min, float* cost_second, float* volume) {
+    return sweep_agg<float>(ref, src, W, H, N, K, Rref, tref, Rsrc, tsrc, alternative, planes, P, winsize, stdthresh,
+                            coord_offset, argmin, cost_min, cost_second, volume);
+}
+
 void orc_reduce_best2_f64(const double* costs, int P, int32_t* argmin, double* cost_min, double* cost_second) {
     const double inf = std::numeric_limits<double>::infinity();
     *argmin = -1; *cost_min = inf; *cost_second = inf;
@@ -1098,6 +1107,11 @@ double orc_tgv_energy_f64(const double* u, const double* u1x, const double* u1y,
                           const double* It, const double* Iu, int W, int H, int N, double lambda, double alpha0,
                           double alpha1) {
     return energy<double>(u, u1x, u1y, u0, Tt, It, Iu, W, H, N, lambda, alpha0, alpha1);
+}
+double orc_tgv_energy_masked_f64(const double* u, const double* u1x, const double* u1y, const double* u0,
+                                 const double* Tt, const double* It, const double* Iu, int W, int H, int N,
+                                 double lambda, double alpha0, double alpha1, const float* weight) {
+    return energy<double>(u, u1x, u1y, u0, Tt, It, Iu, W, H, N, lambda, alpha0, alpha1, weight);
 }
 
 void orc_fusion_bin_centers(int bins, float* out) { bin_centers(bins, out); }

@@ -105,6 +105,13 @@ int orc_sweep_agg_volume_f64(const float* ref, const float* src, int W, int H, i
                              const float* planes, int P, int winsize, float stdthresh, int coord_offset,
                              int32_t* argmin, double* cost_min, double* cost_second, double* volume);
 
+/* The same in the reference's float arithmetic (float volume). */
+int orc_sweep_agg_volume_f32(const float* ref, const float* src, int W, int H, int N,
+                             const float* K, const float* Rref, const float* tref,
+                             const float* Rsrc, const float* tsrc, int alternative,
+                             const float* planes, int P, int winsize, float stdthresh, int coord_offset,
+                             int32_t* argmin, float* cost_min, float* cost_second, float* volume);
+
 /* The aggregated sweep's per-pixel reduction over a cost sequence in plane
  * order: argmin (first minimum), minimum and second-smallest cost. */
 void orc_reduce_best2_f64(const double* costs, int P, int32_t* argmin, double* cost_min, double* cost_second);
@@ -169,6 +176,11 @@ double orc_tgv_energy(const float* u, const float* u1x, const float* u1y, const 
 double orc_tgv_energy_f64(const double* u, const double* u1x, const double* u1y, const double* u0,
                           const double* T, const double* It, const double* Iu, int W, int H, int N,
                           double lambda, double alpha0, double alpha1);
+/* The same sum over the pixels whose weight is nonzero (weight: H*W, as the
+ * solver's energy mask; NULL counts every pixel). */
+double orc_tgv_energy_masked_f64(const double* u, const double* u1x, const double* u1y, const double* u0,
+                                 const double* T, const double* It, const double* Iu, int W, int H, int N,
+                                 double lambda, double alpha0, double alpha1, const float* weight);
 
 /* ---- stage 3: histogram fusion ---------------------------------------- */
 

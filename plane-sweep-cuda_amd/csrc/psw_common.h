@@ -112,11 +112,13 @@ __device__ inline int reflect101(int k, int n) {
 // first also sends NaN and huge coordinates to 0 without an int conversion.
 __device__ inline float bilinear_sample(const float* __restrict__ d, int M1, int M2, float x, float y) {
     if (!(x >= 0.f && x < (float)(M1 - 1) && y >= 0.f && y < (float)(M2 - 1))) return 0.f;
-    const int ix = (int)floorf(x);
-    const float a = x - (float)ix;
-    const int iy = (int)floorf(y);
-    const float b = y - (float)iy;
-    const float* r0 = d + (size_t)iy * M1 + ix;
+    // 0 <= x < M1 - 1: floorf(x) is an exact small integer, so x - floorf(x)
+    // equals x - (float)(int)floorf(x); the offset is below M1 * M2, which the
+    // entry points keep under 2^32 (32-bit index arithmetic, one 64-bit add).
+    const float fx = floorf(x), fy = floorf(y);
+    const float a = x - fx, b = y - fy;
+    const uint32_t o = (uint32_t)(int)fy * (uint32_t)M1 + (uint32_t)(int)fx;
+    const float* r0 = d + o;
     const float* r1 = r0 + M1;
     const float t1 = a * r0[1] + (1.f - a) * r0[0];
     const float t2 = a * r1[1] + (1.f - a) * r1[0];

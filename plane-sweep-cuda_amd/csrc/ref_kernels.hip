@@ -532,6 +532,7 @@ int psw_bilinear_interpolation(float* d_result, const float* d_data, const float
                                int M2, int N1, int N2, psw_dim3 blocks, psw_dim3 threads, psw_stream_t s) {
     PSW_CHECK_ARG(d_result && d_data && d_xout && d_yout, "NULL pointer");
     PSW_CHECK_ARG(M1 >= 0 && M2 >= 0, "negative data size");
+    PSW_CHECK_ARG((uint64_t)M1 * (uint64_t)M2 < (1ull << 32), "source image above 2^32 pixels");
     const int width = N1, height = N2;
     LAUNCH_CHECK();
     bilinear_kernel<<<sh.grid, sh.block, 0, S(s)>>>(d_result, d_data, d_xout, d_yout, M1, M2, N1, N2);

@@ -29,11 +29,14 @@
 
 namespace psw {
 
+// Tile: SW_TX x TY pixels, TY = 16 or 32 (a template parameter; the
+// aggregated sweep's launches choose it, the reference reduction uses 16).
+// A taller tile warps fewer halo positions per pixel (the window radius is
+// added on all four sides) and gives the row pass more pixels per thread.
 constexpr int SW_TX = 32;       // tile width  (pixels)
-constexpr int SW_TY = 16;       // tile height (pixels)
+constexpr int SW_TY = 16;       // default tile height (pixels)
 constexpr int SW_THREADS = 256; // 4 waves
-constexpr int SW_CR = 4;        // rows per thread in the column pass
-constexpr int SW_PX = 2;        // pixels per thread in the row pass
+constexpr int SW_NCG = 4;       // column-pass row groups: TY / 4 rows per thread
 
 struct SweepArgs {
     const float* ref;
@@ -43,6 +46,7 @@ struct SweepArgs {
     int W, H, N, P;
     int off;              // pixel-coordinate offset of the homography (1 = reference)
     int tiles_x, ntiles;
+    int ty;               // tile height (16 or 32)
     int G;                // plane groups (aggregate mode): workgroups per tile
     float stdthresh, nccthresh, zfar;
     float* depth_out;     // reference mode
@@ -62,24 +66,27 @@ __device__ inline float scale_mean(float s, float win, float rwin) {
 
 // One work item: plane group g of tile t (item = t * G + g).  Called by all
 // threads of a workgroup with the same item (it contains barriers).
-template <int R, int MODE, bool FAST>
+template <int R, int MODE, bool FAST, int TY = SW_TY>
 __device__ __forceinline__ void sweep_item(const SweepArgs& a, const int item) {
     constexpr int TW = SW_TX + 2 * R;
-    constexpr int TH = SW_TY + 2 * R;
+    constexpr int TH = TY + 2 * R;
     constexpr int WIN = 2 * R + 1;
-    constexpr int NCG = SW_TY / SW_CR;          // column-pass row groups
+    constexpr int NCG = SW_NCG;                 // column-pass row groups
+    constexpr int SW_CR = TY / SW_NCG;          // rows per thread in the column pass
+    constexpr int SW_PX = SW_TX * TY / SW_THREADS;  // pixels per thread in the row pass
     constexpr int NCOL = NCG * TW;              // column-pass work items
     constexpr int CV = SW_CR + 2 * R;           // column values per column item
     constexpr int RV = SW_PX + 2 * R;           // row values per row item
     static_assert(NCOL <= SW_THREADS, "window too large for the tile");
+    static_assert(TY % SW_NCG == 0 && (SW_TX * TY) % SW_THREADS == 0, "tile height");
     __shared__ float sR[TH * TW];
     __shared__ float sW[TH * TW];
-    __shared__ float sC[3][SW_TY * TW];
+    __shared__ float sC[3][TY * TW];
 
     const int t = item / a.G, g = item - (item / a.G) * a.G;
     const int p0 = (g * a.P) / a.G, p1 = ((g + 1) * a.P) / a.G;
     const int x0 = (t % a.tiles_x) * SW_TX;
-    const int y0 = (t / a.tiles_x) * SW_TY;
+    const int y0 = (t / a.tiles_x) * TY;
     const int tid = threadIdx.x;
     const int W = a.W, H = a.H;
     const float win = (float)WIN, rwin = 1.f / (float)WIN;
@@ -357,17 +364,17 @@ __device__ __forceinline__ void sweep_item(const SweepArgs& a, const int item) {
     }
 }
 
-template <int R, int MODE, bool FAST>
+template <int R, int MODE, bool FAST, int TY>
 __global__ __launch_bounds__(SW_THREADS) void sweep_fused_kernel(SweepArgs a) {
     if ((int)blockIdx.x >= a.ntiles * a.G) return;
-    sweep_item<R, MODE, FAST>(a, xcd_tile(blockIdx.x, a.ntiles * a.G));
+    sweep_item<R, MODE, FAST, TY>(a, xcd_tile(blockIdx.x, a.ntiles * a.G));
 }
 
 // Many frames (uneven sizes, plane and source counts) in one persistent
 // launch: workgroups take items from a device counter until all frames'
 // items are done.  Item g belongs to the frame f with off[f] <= g < off[f+1].
 // Every item is computed exactly as in sweep_fused_kernel.
-template <int R, int MODE, bool FAST>
+template <int R, int MODE, bool FAST, int TY>
 __global__ __launch_bounds__(SW_THREADS) void sweep_batch_kernel(const SweepArgs* __restrict__ frames,
                                                                  const int* __restrict__ off, int nframes, int total,
                                                                  unsigned* __restrict__ counter) {
@@ -385,7 +392,7 @@ __global__ __launch_bounds__(SW_THREADS) void sweep_batch_kernel(const SweepArgs
             else hi = mid - 1;
         }
         const SweepArgs a = frames[lo];
-        sweep_item<R, MODE, FAST>(a, gi - off[lo]);
+        sweep_item<R, MODE, FAST, TY>(a, gi - off[lo]);
     }
 }
 
@@ -503,11 +510,14 @@ template <int R>
 static int launch_sweep_r(const SweepArgs& a, int mode, bool fast, hipStream_t st) {
     const dim3 grid((unsigned)(a.ntiles * a.G)), block(SW_THREADS);
     if (mode == 0) {
-        if (fast) sweep_fused_kernel<R, 0, true><<<grid, block, 0, st>>>(a);
-        else sweep_fused_kernel<R, 0, false><<<grid, block, 0, st>>>(a);
+        if (fast) sweep_fused_kernel<R, 0, true, 16><<<grid, block, 0, st>>>(a);
+        else sweep_fused_kernel<R, 0, false, 16><<<grid, block, 0, st>>>(a);
+    } else if (a.ty == 32) {
+        if (fast) sweep_fused_kernel<R, 1, true, 32><<<grid, block, 0, st>>>(a);
+        else sweep_fused_kernel<R, 1, false, 32><<<grid, block, 0, st>>>(a);
     } else {
-        if (fast) sweep_fused_kernel<R, 1, true><<<grid, block, 0, st>>>(a);
-        else sweep_fused_kernel<R, 1, false><<<grid, block, 0, st>>>(a);
+        if (fast) sweep_fused_kernel<R, 1, true, 16><<<grid, block, 0, st>>>(a);
+        else sweep_fused_kernel<R, 1, false, 16><<<grid, block, 0, st>>>(a);
     }
     return launch_status("sweep_fused_kernel");
 }
@@ -531,13 +541,20 @@ static int launch_sweep(SweepArgs a, int winsize, int mode, bool fast, hipStream
 // Tiles of the image, and plane groups per tile so that a launch has enough
 // workgroups to fill 256 CUs (about 3000, i.e. two rounds at six resident
 // workgroups per CU).  PSW_SWEEP_GROUPS overrides the choice.
-static void sweep_layout(SweepArgs& a, bool allow_groups) {
+// The aggregated sweep's tile height: PSW_SWEEP_TY (16 or 32) or 16.
+static int sweep_tile_height() {
+    const char* e = getenv("PSW_SWEEP_TY");
+    return (e && atoi(e) == 32) ? 32 : 16;
+}
+
+static void sweep_layout(SweepArgs& a, bool allow_groups, int ty = SW_TY) {
+    a.ty = ty;
     a.tiles_x = (a.W + SW_TX - 1) / SW_TX;
-    const int tiles_y = (a.H + SW_TY - 1) / SW_TY;
+    const int tiles_y = (a.H + a.ty - 1) / a.ty;
     a.ntiles = a.tiles_x * tiles_y;
     a.G = 1;
     if (!allow_groups) return;
-    int G = (3072 + a.ntiles / 2) / a.ntiles;
+    int G = (3072 * 16 / a.ty + a.ntiles / 2) / a.ntiles;
     if (const char* e = getenv("PSW_SWEEP_GROUPS")) G = atoi(e);
     a.G = std::max(1, std::min(G, std::min(a.P, 64)));
 }
@@ -552,6 +569,10 @@ static int check_sweep_args(const float* d_ref, const float* d_src, int width, i
     }
     if (width < 2 || height < 2 || width <= winsize / 2 || height <= winsize / 2) {
         set_error("image %d x %d too small for window %d", width, height, winsize);
+        return PSW_ERR_ARG;
+    }
+    if ((uint64_t)width * (uint64_t)height >= (1ull << 32)) {
+        set_error("image %d x %d above 2^32 pixels", width, height);
         return PSW_ERR_ARG;
     }
     return PSW_OK;
@@ -617,7 +638,7 @@ int psw_sweep_fused_aggregate(const float* d_ref, const float* d_src, int width,
     a.W = width; a.H = height; a.N = nsrc; a.P = nplanes; a.off = coord_offset;
     a.stdthresh = stdthresh;
     a.argmin = d_argmin; a.cmin = d_cost_min; a.csec = d_cost_second;
-    sweep_layout(a, true);
+    sweep_layout(a, true, sweep_tile_height());
     const size_t npix = (size_t)width * height;
     void* ws = nullptr;
     if (a.G > 1) {
@@ -645,6 +666,7 @@ int psw_sweep_fused_aggregate_batch(const psw_sweep_frame* frames, int nframes, 
     std::vector<SweepArgs> args((size_t)nframes);
     std::vector<int> off((size_t)nframes + 1, 0);
     long total_tiles = 0;
+    const int ty = sweep_tile_height();
     for (int f = 0; f < nframes; ++f) {
         const psw_sweep_frame& fr = frames[f];
         int st = check_sweep_args(fr.d_ref, fr.d_src, fr.width, fr.height, fr.nsrc, fr.d_H, fr.nplanes, winsize);
@@ -660,12 +682,13 @@ int psw_sweep_fused_aggregate_batch(const psw_sweep_frame* frames, int nframes, 
         a.W = fr.width; a.H = fr.height; a.N = fr.nsrc; a.P = fr.nplanes; a.off = coord_offset;
         a.stdthresh = stdthresh;
         a.argmin = fr.d_argmin; a.cmin = fr.d_cost_min; a.csec = fr.d_cost_second;
-        sweep_layout(a, false);
+        sweep_layout(a, false, ty);
         total_tiles += a.ntiles;
     }
     // Plane groups: enough items across the whole batch to fill the GPU
     // (about 3000 workgroups' worth) unless given.
-    int G = groups > 0 ? groups : (int)std::max(1L, (3072 + total_tiles / 2) / std::max(1L, total_tiles));
+    int G = groups > 0 ? groups
+                       : (int)std::max(1L, (3072L * 16 / ty + total_tiles / 2) / std::max(1L, total_tiles));
     size_t ws_bytes = 0;
     for (int f = 0; f < nframes; ++f) {
         SweepArgs& a = args[(size_t)f];
@@ -703,10 +726,15 @@ int psw_sweep_fused_aggregate_batch(const psw_sweep_frame* frames, int nframes, 
     if (rc == PSW_OK) {
         const dim3 grid((unsigned)wg), block(SW_THREADS);
         switch (winsize / 2) {
-#define PSW_SB(r)                                                                                              \
-    case r:                                                                                                    \
-        if (fast) sweep_batch_kernel<r, 1, true><<<grid, block, 0, st>>>(d_args, d_off, nframes, total, d_counter); \
-        else sweep_batch_kernel<r, 1, false><<<grid, block, 0, st>>>(d_args, d_off, nframes, total, d_counter);  \
+#define PSW_SB(r)                                                                                             \
+    case r:                                                                                                   \
+        if (ty == 32) {                                                                                       \
+            if (fast) sweep_batch_kernel<r, 1, true, 32><<<grid, block, 0, st>>>(d_args, d_off, nframes, total, d_counter); \
+            else sweep_batch_kernel<r, 1, false, 32><<<grid, block, 0, st>>>(d_args, d_off, nframes, total, d_counter); \
+        } else {                                                                                              \
+            if (fast) sweep_batch_kernel<r, 1, true, 16><<<grid, block, 0, st>>>(d_args, d_off, nframes, total, d_counter); \
+            else sweep_batch_kernel<r, 1, false, 16><<<grid, block, 0, st>>>(d_args, d_off, nframes, total, d_counter); \
+        }                                                                                                     \
         break;
             PSW_SB(0) PSW_SB(1) PSW_SB(2) PSW_SB(3) PSW_SB(4) PSW_SB(5) PSW_SB(6) PSW_SB(7)
 #undef PSW_SB

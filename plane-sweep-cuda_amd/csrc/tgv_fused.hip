@@ -78,6 +78,7 @@ struct psw_tgv {
     double* partials;     // per-block energy partial sums
     int npartials;
     psw::TgvCtrl* ctrl;   // device
+    unsigned long long* masked;  // device counter of psw_tgv_set_energy_mask
     psw::TgvCtrl* hctrl;  // pinned host copy (filled by copies)
     psw::TgvCtrl* hmirror;  // pinned, mapped host memory written by checking launches
     psw::TgvCtrl* dmirror;  // its device address
@@ -1626,6 +1627,7 @@ int psw_tgv_create(psw_tgv** out, int width, int height, int nsrc) {
     PSW_CHECK_ARG(out, "out is NULL");
     *out = nullptr;
     PSW_CHECK_ARG(width >= 2 && height >= 2, "image must be at least 2 x 2");
+    PSW_CHECK_ARG((uint64_t)width * (uint64_t)height < (1ull << 32), "image above 2^32 pixels");
     PSW_CHECK_ARG(nsrc >= 1 && nsrc <= PSW_MAX_SOURCES, "nsrc must be 1..PSW_MAX_SOURCES");
     psw_tgv* t = new psw_tgv();
     t->W = width; t->H = height; t->N = nsrc;
@@ -1639,6 +1641,7 @@ int psw_tgv_create(psw_tgv** out, int width, int height, int nsrc) {
     const int np = 2 * std::max(TG_ENERGY_BLOCKS, tgv_tile_partials(width, height));
     if (e == hipSuccess) e = hipMalloc((void**)&t->partials, np * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void**)&t->ctrl, sizeof(TgvCtrl));
+    if (e == hipSuccess) e = hipMalloc((void**)&t->masked, sizeof(unsigned long long));
     if (e == hipSuccess) e = hipHostMalloc((void**)&t->hctrl, sizeof(TgvCtrl), hipHostMallocDefault);
     if (e == hipSuccess)
         e = hipHostMalloc((void**)&t->hmirror, sizeof(TgvCtrl), hipHostMallocMapped | hipHostMallocCoherent);
@@ -1666,6 +1669,7 @@ int psw_tgv_destroy(psw_tgv* t) {
     if (t->base) (void)hipFree(t->base);
     if (t->partials) (void)hipFree(t->partials);
     if (t->ctrl) (void)hipFree(t->ctrl);
+    if (t->masked) (void)hipFree(t->masked);
     if (t->hctrl) (void)hipHostFree(t->hctrl);
     if (t->hmirror) (void)hipHostFree(t->hmirror);
     delete t;
@@ -1779,10 +1783,10 @@ int psw_tgv_set_energy_mask(psw_tgv* t, double qmax, const psw_tgv_params* prm, 
                             psw_stream_t s) {
     PSW_CHECK_ARG(t && prm, "NULL pointer");
     const TgvLayout& L = t->L;
-    unsigned long long* d = nullptr;
-    if (excluded) {
-        PSW_TRY_HIP(hipMalloc((void**)&d, sizeof(unsigned long long)), PSW_ERR_ALLOC);
-        PSW_TRY_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), S(s)), PSW_ERR_DEVICE);
+    unsigned long long* d = excluded ? t->masked : nullptr;   // the solver's own counter
+    if (d) {
+        const int fs = psw_fill_u32(reinterpret_cast<uint32_t*>(d), 0u, 2, s);
+        if (fs != PSW_OK) return fs;
     }
     const dim3 blk(64, 4), grd((unsigned)((t->W + 63) / 64), (unsigned)((t->H + 3) / 4));
     const double scale = prm->tau * prm->sigma * prm->lambda * prm->lambda;
@@ -1795,7 +1799,6 @@ int psw_tgv_set_energy_mask(psw_tgv* t, double qmax, const psw_tgv_params* prm, 
             if (e == hipSuccess) e = hipStreamSynchronize(S(s));
             if (e != hipSuccess) st = hip_status(e, "psw_tgv_set_energy_mask");
         }
-        (void)hipFree(d);
         *excluded = (int64_t)h;
     }
     return st;
@@ -1934,6 +1937,13 @@ int psw_tgv_get_u(psw_tgv* t, float* d_out, psw_stream_t s) {
     PSW_CHECK_ARG(t && d_out, "NULL pointer");
     const float* u = t->base + t->L.U + (size_t)(t->k % TG_USLOTS) * 3 * t->L.n;
     PSW_TRY_HIP(hipMemcpyAsync(d_out, u, t->L.n * sizeof(float), hipMemcpyDeviceToDevice, S(s)), PSW_ERR_COPY);
+    return PSW_OK;
+}
+
+int psw_tgv_get_energy_mask(psw_tgv* t, float* d_out, psw_stream_t s) {
+    PSW_CHECK_ARG(t && d_out, "NULL pointer");
+    PSW_TRY_HIP(hipMemcpyAsync(d_out, t->base + t->L.mask, t->L.n * sizeof(float), hipMemcpyDeviceToDevice, S(s)),
+                PSW_ERR_COPY);
     return PSW_OK;
 }
 

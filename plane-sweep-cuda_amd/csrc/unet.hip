@@ -62,105 +62,174 @@ static const LayerDef kLayers[UN_LAYERS] = {
 struct ConvK {
     const uint16_t* xa;   // [B][H][W][ca]
     const uint16_t* xb;   // [B][H/2][W/2][cb] (nearest-upsampled), or null
-    const uint16_t* wp;   // packed weights [kchunks][NB][64][8]
+    const uint16_t* wp;   // packed weights [conv_chunks(ca + cb)][NB][64][8]
     const float* bias;    // [cout] or null
     void* y;              // [B][H][W][cout]: bf16 (relu) or fp32
     int ca, cb, cout, B, H, W, kchunks;
 };
 
-// One wave computes 32 pixels x all output channels; four waves per block.
-// MFMA 16x16x32 lane layout: A row (lane & 15), k (lane >> 4) * 8 + j;
-// B column (lane & 15), same k; D rows 4 * (lane >> 4) + i, column (lane & 15).
-template <int NB, bool RELU_BF16>
-__global__ __launch_bounds__(256) void conv3x3_kernel(ConvK a) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int C = a.ca + a.cb;
-    const long NP = (long)a.B * a.H * a.W;
-    const long pbase = ((long)blockIdx.x * 4 + wave) * 32;
-    if (pbase >= NP) return;  // no barriers in this kernel
-    int pb[2], py[2], px[2];
-    bool pv[2];
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb) {
-        long p = pbase + mb * 16 + (lane & 15);
-        pv[mb] = p < NP;
-        if (!pv[mb]) p = NP - 1;
-        px[mb] = (int)(p % a.W);
-        const long t = p / a.W;
-        py[mb] = (int)(t % a.H);
-        pb[mb] = (int)(t / a.H);
-    }
-    f32x4 acc[2][NB];
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int kq = (lane >> 4) * 8;
-    const int Hh = a.H >> 1, Wh = a.W >> 1;
-    for (int kc = 0; kc < a.kchunks; ++kc) {
-        const int k0 = kc * 32 + kq;
-        const int tap = k0 / C, c0 = k0 - tap * C;
-        bf16x8 av[2];
-#pragma unroll
-        for (int mb = 0; mb < 2; ++mb) {
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (tap < 9 && pv[mb]) {
-                const int yy = py[mb] + tap / 3 - 1, xx = px[mb] + tap % 3 - 1;
-                if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
-                    if (c0 < a.ca)
-                        v = *reinterpret_cast<const uint4*>(a.xa + (((long)pb[mb] * a.H + yy) * a.W + xx) * a.ca + c0);
-                    else
-                        v = *reinterpret_cast<const uint4*>(
-                            a.xb + (((long)pb[mb] * Hh + (yy >> 1)) * Wh + (xx >> 1)) * a.cb + (c0 - a.ca));
-                }
-            }
-            av[mb] = __builtin_bit_cast(bf16x8, v);
-        }
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            const uint4 w = *reinterpret_cast<const uint4*>(a.wp + (((long)kc * NB + nb) * 64 + lane) * 8);
-            const bf16x8 bv = __builtin_bit_cast(bf16x8, w);
-            acc[0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[0], bv, acc[0][nb], 0, 0, 0);
-            acc[1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[1], bv, acc[1][nb], 0, 0, 0);
-        }
-    }
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            const int ch = nb * 16 + (lane & 15);
-            const float bias = a.bias ? a.bias[ch] : 0.f;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const long p = pbase + mb * 16 + 4 * (lane >> 4) + i;
-                if (p >= NP) continue;
-                const float v = acc[mb][nb][i] + bias;
-                if (RELU_BF16)
-                    reinterpret_cast<uint16_t*>(a.y)[p * a.cout + ch] = f2bf(fmaxf(v, 0.f));
-                else
-                    reinterpret_cast<float*>(a.y)[p * a.cout + ch] = v;
-            }
-        }
+// Output tile of one block: 8 rows x 32 columns, four waves of 2 rows each
+// (4 MFMA row blocks of 16 pixels per wave).  The input tile with its
+// one-pixel halo (10 x 34 pixels) is staged in LDS one 32-channel slab at a
+// time; the 9 taps then read their shifted windows from LDS, so every input
+// value is fetched from memory once per slab (1.33x for the halo) instead of
+// once per tap.  LDS pixel stride 40 bf16 (80 bytes): the 16 lanes of one
+// 16-byte read hit 16 disjoint groups of 4 banks.
+constexpr int CT_TH = 8, CT_TW = 32;
+constexpr int CT_HH = CT_TH + 2, CT_HW = CT_TW + 2;
+constexpr int CT_PS = 40;
+
+// K order of the packed weights: for each 32-channel slab of the input, for
+// each tap, the slab's 8-channel groups; four groups (32 k values) make one
+// MFMA chunk, the last chunk of a slab padded with zero groups.
+__host__ __device__ inline int slab_chunks(int slab_channels) { return (9 * (slab_channels >> 3) + 3) >> 2; }
+__host__ __device__ inline int conv_chunks(int C) {
+    return 9 * (C >> 5) + ((C & 31) ? slab_chunks(C & 31) : 0);
 }
 
-// Pack FP32 weights [9][cin][cout] into the bf16 B-operand order.  flip = 0:
-// K = 9 cin, N = cout (forward).  flip = 1: K = 9 cout, N = cin with
-// W'[tap'][co][ci] = W[8 - tap'][ci][co] (the data gradient is the forward
-// convolution of dZ with the 180-degree-rotated, transposed kernel).
+// MFMA 16x16x32 lane layout: A row (lane & 15), k (lane >> 4) * 8 + j;
+// B column (lane & 15), same k; D rows 4 * (lane >> 4) + i, column (lane & 15).
+// A block computes NBL of the layer's nbtot 16-channel output blocks
+// (starting at blockIdx.y * NBL): small levels split their output channels
+// over several blocks so that a launch still has enough of them.  The
+// results go through LDS, 16 channels at a time, so that every thread
+// stores whole 16-byte runs of one pixel's channels.
+template <int NBL, bool RELU_BF16>
+__global__ __launch_bounds__(256) void conv3x3_kernel(ConvK a) {
+    __shared__ __align__(16) uint16_t sx[CT_HH * CT_HW * CT_PS];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int C = a.ca + a.cb;
+    const int nbtot = a.cout >> 4, nb0 = blockIdx.y * NBL;
+    const int tiles_x = (a.W + CT_TW - 1) / CT_TW, tiles_y = (a.H + CT_TH - 1) / CT_TH;
+    int blk = blockIdx.x;
+    const int tx0 = (blk % tiles_x) * CT_TW;
+    blk /= tiles_x;
+    const int ty0 = (blk % tiles_y) * CT_TH;
+    const int b = blk / tiles_y;
+    const int Hh = a.H >> 1, Wh = a.W >> 1;
+    const int jg = lane >> 4, col = lane & 15;
+    f32x4 acc[4][NBL];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int nb = 0; nb < NBL; ++nb) acc[m][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const uint16_t* wp = a.wp;
+    for (int c0 = 0; c0 < C; c0 += 32) {
+        const int g8 = (C - c0 < 32 ? C - c0 : 32) >> 3;  // 8-channel groups in this slab
+        __syncthreads();                                    // the previous slab's reads are done
+        for (int i = t; i < CT_HH * CT_HW * g8; i += 256) {
+            const int g = i % g8, pix = i / g8;
+            const int hx = pix % CT_HW, hy = pix / CT_HW;
+            const int yy = ty0 + hy - 1, xx = tx0 + hx - 1;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
+                const int c = c0 + g * 8;
+                if (c < a.ca)
+                    v = *reinterpret_cast<const uint4*>(a.xa + (((long)b * a.H + yy) * a.W + xx) * a.ca + c);
+                else
+                    v = *reinterpret_cast<const uint4*>(a.xb + (((long)b * Hh + (yy >> 1)) * Wh + (xx >> 1)) * a.cb +
+                                                        (c - a.ca));
+            }
+            *reinterpret_cast<uint4*>(&sx[pix * CT_PS + g * 8]) = v;
+        }
+        __syncthreads();
+        const int nck = slab_chunks(g8 * 8);
+        for (int kc = 0; kc < nck; ++kc) {
+            const int gi = kc * 4 + jg;  // this lane's 8-channel k group
+            const bool gv = gi < 9 * g8;
+            const int tap = gv ? gi / g8 : 0;
+            const int cg = gv ? gi - tap * g8 : 0;
+            const int dy = tap / 3, dx = tap - (tap / 3) * 3;
+            bf16x8 av[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int r = wave * 2 + (m >> 1), cc = (m & 1) * 16 + col;
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (gv) v = *reinterpret_cast<const uint4*>(&sx[((r + dy) * CT_HW + cc + dx) * CT_PS + cg * 8]);
+                av[m] = __builtin_bit_cast(bf16x8, v);
+            }
+#pragma unroll
+            for (int nb = 0; nb < NBL; ++nb) {
+                const uint4 w = *reinterpret_cast<const uint4*>(wp + ((long)(kc * nbtot + nb0 + nb) * 64 + lane) * 8);
+                const bf16x8 bv = __builtin_bit_cast(bf16x8, w);
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    acc[m][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv, acc[m][nb], 0, 0, 0);
+            }
+        }
+        wp += (long)nck * nbtot * 512;
+    }
+    // Epilogue, 16 channels at a time: bias (+ ReLU, bf16) into LDS as
+    // [pixel][channel] (row stride 24 bf16 / 20 floats: conflict-free 16-byte
+    // reads), then thread t stores tile pixel t's 16 channels.
+    const int ty = t / CT_TW, tx = t - ty * CT_TW;
+    const int y = ty0 + ty, x = tx0 + tx;
+    const long p = ((long)b * a.H + y) * a.W + x;
+    const bool out = y < a.H && x < a.W;
+#pragma unroll
+    for (int nb = 0; nb < NBL; ++nb) {
+        const int ch0 = (nb0 + nb) * 16;
+        const float bias = a.bias ? a.bias[ch0 + col] : 0.f;
+        __syncthreads();                                   // sx is free (K loop or previous block of channels)
+        if (RELU_BF16) {
+            uint16_t* so = sx;
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int q = (wave * 2 + (m >> 1)) * CT_TW + (m & 1) * 16 + 4 * jg + i;
+                    so[q * 24 + col] = f2bf(fmaxf(acc[m][nb][i] + bias, 0.f));
+                }
+            __syncthreads();
+            if (out) {
+                const uint4* src = reinterpret_cast<const uint4*>(&so[t * 24]);
+                uint4* dst = reinterpret_cast<uint4*>(reinterpret_cast<uint16_t*>(a.y) + p * a.cout + ch0);
+                dst[0] = src[0];
+                dst[1] = src[1];
+            }
+        } else {
+            float* so = reinterpret_cast<float*>(sx);
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int q = (wave * 2 + (m >> 1)) * CT_TW + (m & 1) * 16 + 4 * jg + i;
+                    so[q * 20 + col] = acc[m][nb][i] + bias;
+                }
+            __syncthreads();
+            if (out) {
+                const uint4* src = reinterpret_cast<const uint4*>(&so[t * 20]);
+                uint4* dst = reinterpret_cast<uint4*>(reinterpret_cast<float*>(a.y) + p * a.cout + ch0);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) dst[k] = src[k];
+            }
+        }
+    }
+}
+
+// Pack FP32 weights [9][cin][cout] into the bf16 B-operand order of
+// conv3x3_kernel (slab / tap / 8-channel group, see conv_chunks).  flip = 0:
+// K over the cin input channels, N = cout (forward).  flip = 1: K over the
+// cout channels of dZ, N = cin, with W'[tap'][co][ci] = W[8 - tap'][ci][co]
+// (the data gradient is the forward convolution of dZ with the
+// 180-degree-rotated, transposed kernel).
 __global__ void pack_w_kernel(const float* __restrict__ w, int cin, int cout, int flip, int kchunks, int NB,
                               uint16_t* __restrict__ wp) {
     const long n_el = (long)kchunks * NB * 512;
+    const int C = flip ? cout : cin, NO = flip ? cin : cout;
     for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n_el; idx += (long)gridDim.x * blockDim.x) {
         const int j = (int)(idx & 7), l = (int)((idx >> 3) & 63);
         const long r = idx >> 9;
         const int nb = (int)(r % NB), kc = (int)(r / NB);
-        const int k = kc * 32 + (l >> 4) * 8 + j, n = nb * 16 + (l & 15);
+        const int slab = kc / 9, local = kc - slab * 9;
+        const int c0 = 32 * slab;
+        const int g8 = (C - c0 < 32 ? C - c0 : 32) >> 3;
+        const int gi = 4 * local + (l >> 4), n = nb * 16 + (l & 15);
         float v = 0.f;
-        if (!flip) {
-            if (k < 9 * cin && n < cout) v = w[(long)k * cout + n];
-        } else {
-            const int tp = k / cout, co = k - (k / cout) * cout;
-            if (tp < 9 && n < cin) v = w[((long)(8 - tp) * cin + n) * cout + co];
+        if (c0 < C && gi < 9 * g8 && n < NO) {
+            const int tap = gi / g8, c = c0 + (gi - tap * g8) * 8 + j;
+            if (!flip) v = w[((long)tap * cin + c) * cout + n];
+            else v = w[((long)(8 - tap) * cin + n) * cout + c];
         }
         wp[idx] = f2bf(v);
     }
@@ -511,22 +580,26 @@ __global__ void f32_to_bf16_kernel(const float* __restrict__ x, long n, uint16_t
 
 // ---- host side -------------------------------------------------------------
 
-static inline int kchunks_of(int k) { return (k + 31) / 32; }
 static inline unsigned grid_for(long n, int per = 256) {
     long g = (n + per - 1) / per;
     if (g > 65535L * 16) g = 65535L * 16;
     return (unsigned)(g < 1 ? 1 : g);
 }
 
+// Output-channel split: levels with few tiles give each block fewer
+// 16-channel output blocks (NBL), so that a launch has about 1000 blocks.
 template <bool RELU>
 static void launch_conv(const ConvK& a, int NB, hipStream_t st) {
-    const long NP = (long)a.B * a.H * a.W;
-    const dim3 grd((unsigned)((NP + 127) / 128));
-    switch (NB) {
+    const long tiles = (long)a.B * ((a.H + CT_TH - 1) / CT_TH) * ((a.W + CT_TW - 1) / CT_TW);
+    int nbl = NB;
+    if (tiles * 2 < 1024 && NB % 2 == 0 && NB > 2) nbl = 2;
+    if (tiles * (NB / nbl) < 1024) nbl = 1;
+    const dim3 grd((unsigned)tiles, (unsigned)(NB / nbl));
+    switch (nbl) {
 #define PSW_CONV_CASE(n) \
     case n: conv3x3_kernel<n, RELU><<<grd, 256, 0, st>>>(a); break;
-        PSW_CONV_CASE(2) PSW_CONV_CASE(3) PSW_CONV_CASE(4) PSW_CONV_CASE(5) PSW_CONV_CASE(6) PSW_CONV_CASE(7)
-        PSW_CONV_CASE(10)
+        PSW_CONV_CASE(1) PSW_CONV_CASE(2) PSW_CONV_CASE(3) PSW_CONV_CASE(4) PSW_CONV_CASE(5) PSW_CONV_CASE(6)
+        PSW_CONV_CASE(7) PSW_CONV_CASE(10)
 #undef PSW_CONV_CASE
         default: break;
     }
@@ -608,7 +681,7 @@ static int unet_alloc(psw_unet* n, void** p, size_t bytes) {
 static int unet_pack(psw_unet* n, int L, hipStream_t st) {
     const LayerDef& d = kLayers[L];
     const int cin = d.ca + d.cb;
-    const int kf = kchunks_of(9 * cin), kb = kchunks_of(9 * d.cout);
+    const int kf = conv_chunks(cin), kb = conv_chunks(d.cout);
     pack_w_kernel<<<grid_for((long)kf * (d.cout / 16) * 512), 256, 0, st>>>(n->theta + n->off[L], cin, d.cout, 0, kf,
                                                                             d.cout / 16, n->wpf[L]);
     if (L > 0)
@@ -652,7 +725,7 @@ static int unet_forward_impl(psw_unet* n, int B, int W, int H, hipStream_t st) {
         a.y = n->act[L + 1];
         a.ca = d.ca; a.cb = d.cb; a.cout = d.cout;
         a.B = B; a.H = H >> d.level; a.W = W >> d.level;
-        a.kchunks = kchunks_of(9 * (d.ca + d.cb));
+        a.kchunks = conv_chunks(d.ca + d.cb);
         launch_conv<true>(a, d.cout / 16, st);
         int s = launch_status("conv3x3_kernel");
         if (s != PSW_OK) return s;
@@ -705,7 +778,7 @@ static int unet_backward_impl(psw_unet* n, int B, int W, int H, hipStream_t st) 
         a.xa = n->dz; a.xb = nullptr; a.wp = n->wpb[L]; a.bias = nullptr;
         a.ca = d.cout; a.cb = 0; a.cout = d.ca + d.cb;
         a.B = B; a.H = h; a.W = w;
-        a.kchunks = kchunks_of(9 * d.cout);
+        a.kchunks = conv_chunks(d.cout);
         static const int dst_a[UN_LAYERS] = {0, 1, -1, 3, -2, 5, -3, 7, 6, 9, 4, 11, 2, 13};
         if (d.cb == 0) {
             const int t = dst_a[L];
@@ -767,8 +840,8 @@ int psw_unet_create(psw_unet** out, int64_t max_pixels) {
     for (int L = 0; L < UN_LAYERS; ++L) {
         const LayerDef& d = kLayers[L];
         const int cin = d.ca + d.cb;
-        A((void**)&n->wpf[L], (size_t)kchunks_of(9 * cin) * (d.cout / 16) * 512 * 2);
-        A((void**)&n->wpb[L], (size_t)kchunks_of(9 * d.cout) * (cin / 16) * 512 * 2);
+        A((void**)&n->wpf[L], (size_t)conv_chunks(cin) * (d.cout / 16) * 512 * 2);
+        A((void**)&n->wpb[L], (size_t)conv_chunks(d.cout) * (cin / 16) * 512 * 2);
         const size_t np = (size_t)(max_pixels >> (2 * d.level));
         A((void**)&n->act[L + 1], np * d.cout * 2);
         A((void**)&n->gact[L + 1], np * d.cout * 4);
@@ -988,7 +1061,7 @@ int psw_conv3x3_bf16(const uint16_t* d_xa, int ca, const uint16_t* d_xb, int cb,
     PSW_CHECK_ARG(nb_supported(cout), "cout must be 16 x {2,3,4,5,6,7,10}");
     PSW_CHECK_ARG(batch >= 1 && width >= 2 && height >= 2 && (cb == 0 || (width % 2 == 0 && height % 2 == 0)),
                   "bad size");
-    const int cin = ca + cb, kc = kchunks_of(9 * cin);
+    const int cin = ca + cb, kc = conv_chunks(cin);
     uint16_t* wp = nullptr;
     PSW_TRY_HIP(hipMalloc((void**)&wp, (size_t)kc * (cout / 16) * 512 * 2), PSW_ERR_ALLOC);
     hipStream_t st = S(s);
@@ -1009,7 +1082,7 @@ int psw_conv3x3_bf16_dgrad(const uint16_t* d_dz, int cout, const float* d_w, int
     PSW_CHECK_ARG(d_dz && d_w && d_dx, "NULL pointer");
     PSW_CHECK_ARG(cout % 8 == 0 && cout > 0 && nb_supported(cin), "unsupported channel counts");
     PSW_CHECK_ARG(batch >= 1 && width >= 1 && height >= 1, "bad size");
-    const int kc = kchunks_of(9 * cout);
+    const int kc = conv_chunks(cout);
     uint16_t* wp = nullptr;
     PSW_TRY_HIP(hipMalloc((void**)&wp, (size_t)kc * (cin / 16) * 512 * 2), PSW_ERR_ALLOC);
     hipStream_t st = S(s);

@@ -14,19 +14,6 @@ namespace psw {
 
 namespace {
 
-// Device allocation owned for the duration of one call.
-struct DevBuf {
-    void* p = nullptr;
-    int status = PSW_OK;
-    explicit DevBuf(size_t bytes) { status = psw_malloc(&p, bytes); }
-    ~DevBuf() {
-        if (p) psw_free(p);
-    }
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    float* f() { return static_cast<float*>(p); }
-};
-
 struct Timer {
     std::chrono::high_resolution_clock::time_point t0 = std::chrono::high_resolution_clock::now();
     long long ms() const {
@@ -44,8 +31,64 @@ PlaneSweep::PlaneSweep(int argc, char** argv) {
     threads = dim3(DEFAULT_BLOCK_XDIM, maxThreadsPerBlock / DEFAULT_BLOCK_XDIM);
 }
 
-PlaneSweep::~PlaneSweep() {
-    if (d_depthmap) psw_free(d_depthmap);
+PlaneSweep::~PlaneSweep() { release(); }
+
+void PlaneSweep::release() {
+    if (strm) psw_stream_synchronize(strm);
+    for (int i = 0; i < NSLOTS; ++i) {
+        if (pool[i]) psw_free(pool[i]);
+        pool[i] = nullptr;
+        poolcap[i] = 0;
+    }
+    d_depthmap = nullptr;
+    if (tgv) psw_tgv_destroy(tgv);
+    tgv = nullptr;
+    tgvw = tgvh = tgvn = 0;
+    if (strm) psw_stream_destroy(strm);
+    strm = nullptr;
+}
+
+psw_stream_t PlaneSweep::stream() {
+    if (!strm && psw_stream_create(&strm) != PSW_OK) strm = nullptr;   // null stream as the fallback
+    return strm;
+}
+
+size_t PlaneSweep::deviceBytes() const {
+    size_t b = 0;
+    for (int i = 0; i < NSLOTS; ++i) b += poolcap[i];
+    return b + (tgv ? psw_tgv_state_bytes(tgvw, tgvh, tgvn) : 0);
+}
+
+// The slot's buffer with at least `bytes` bytes (grow-only; the old
+// contents are not kept when it grows).
+int PlaneSweep::buffer(Slot s, size_t bytes, float** out) {
+    if (bytes > poolcap[s]) {
+        if (strm) psw_stream_synchronize(strm);   // work queued on the old buffer is done
+        if (pool[s]) psw_free(pool[s]);
+        pool[s] = nullptr;
+        poolcap[s] = 0;
+        if (s == DEPTHMAP) d_depthmap = nullptr;
+        const int st = psw_malloc(&pool[s], bytes);
+        if (st != PSW_OK) return st;
+        poolcap[s] = bytes;
+    }
+    *out = static_cast<float*>(pool[s]);
+    if (s == DEPTHMAP) d_depthmap = *out;
+    return PSW_OK;
+}
+
+// The TGV2 solver for this shape, created once per shape.
+int PlaneSweep::solver(int w, int h, int nimages, psw_tgv** out) {
+    if (!tgv || tgvw != w || tgvh != h || tgvn != nimages) {
+        if (strm) psw_stream_synchronize(strm);
+        if (tgv) psw_tgv_destroy(tgv);
+        tgv = nullptr;
+        const int st = psw_tgv_create(&tgv, w, h, nimages);
+        if (st != PSW_OK) return st;
+        tgvw = w; tgvh = h; tgvn = nimages;
+    }
+    *out = tgv;
+    return PSW_OK;
 }
 
 // cudaDevInit (planesweep.cpp:30-58): a "device=N" argument selects the GPU
@@ -66,7 +109,9 @@ int PlaneSweep::deviceInit(int argc, char** argv) {
         std::cerr << "Invalid device " << dev << " (" << count << " available)" << std::endl;
         return NO_CUDA_DEVICE;
     }
+    if (device >= 0 && dev != device) release();   // the pool and stream belong to the old device
     if (psw_set_device(dev) != PSW_OK) return NO_CUDA_DEVICE;
+    device = dev;
     return dev;
 }
 
@@ -148,18 +193,20 @@ bool PlaneSweep::RunAlgorithm(int argc, char** argv) {
     int st = psw_sweep_homographies_reference(H.data(), planes.data(), P, nimgs, Kf, Rr, tr, Rs.data(), ts.data(),
                                               alternativemethod ? 1 : 0);
     if (st != PSW_OK) return fail("RunAlgorithm", st);
-    DevBuf dref(n * 4), dsrc(n * nimgs * 4), dH(H.size() * 4), ddep(P * 4), dout(n * 4);
-    for (DevBuf* b : {&dref, &dsrc, &dH, &ddep, &dout})
-        if (b->status != PSW_OK) return fail("RunAlgorithm", b->status);
-    st = psw_memcpy_htod(dref.p, HostRef.data(), n * 4, nullptr);
-    for (int i = 0; i < nimgs && st == PSW_OK; ++i)
-        st = psw_memcpy_htod(dsrc.f() + (size_t)i * n, HostSrc[i].data(), n * 4, nullptr);
-    if (st == PSW_OK) st = psw_memcpy_htod(dH.p, H.data(), H.size() * 4, nullptr);
-    if (st == PSW_OK) st = psw_memcpy_htod(ddep.p, planes.data(), P * 4, nullptr);
+    psw_stream_t sm = stream();
+    float *dref, *dsrc, *dH, *ddep, *dout;
+    if ((st = buffer(REF, n * 4, &dref)) != PSW_OK || (st = buffer(SRC, n * nimgs * 4, &dsrc)) != PSW_OK ||
+        (st = buffer(HOM, H.size() * 4, &dH)) != PSW_OK || (st = buffer(PLANES, P * 4, &ddep)) != PSW_OK ||
+        (st = buffer(OUT, n * 4, &dout)) != PSW_OK)
+        return fail("RunAlgorithm", st);
+    st = psw_memcpy_htod(dref, HostRef.data(), n * 4, sm);
+    for (int i = 0; i < nimgs && st == PSW_OK; ++i) st = psw_memcpy_htod(dsrc + (size_t)i * n, HostSrc[i].data(), n * 4, sm);
+    if (st == PSW_OK) st = psw_memcpy_htod(dH, H.data(), H.size() * 4, sm);
+    if (st == PSW_OK) st = psw_memcpy_htod(ddep, planes.data(), P * 4, sm);
     if (st == PSW_OK)
-        st = psw_sweep_fused_reference(dref.f(), dsrc.f(), w, h, nimgs, dH.f(), ddep.f(), P, (int)winsize, stdthresh,
-                                       nccthresh, zfar, dout.f(), nullptr);
-    if (st == PSW_OK) st = psw_memcpy_dtoh(depthmap.data(), dout.p, n * 4, nullptr);
+        st = psw_sweep_fused_reference(dref, dsrc, w, h, nimgs, dH, ddep, P, (int)winsize, stdthresh, nccthresh, zfar,
+                                       dout, sm);
+    if (st == PSW_OK) st = psw_memcpy_dtoh(depthmap.data(), dout, n * 4, sm);
     if (st != PSW_OK) return fail("RunAlgorithm", st);
     ConvertDepthtoUChar(depthmap, depthmap8u);
     depthavailable = true;
@@ -182,28 +229,29 @@ bool PlaneSweep::RunAggregated(bool zeroBasedK, bool fast) {
         HostSrc[i].R.toArray(&Rs[9 * i]);
         ts[3 * i] = HostSrc[i].t.x; ts[3 * i + 1] = HostSrc[i].t.y; ts[3 * i + 2] = HostSrc[i].t.z;
     }
-    DevBuf dref(n * 4), dsrc(n * nimgs * 4), dH((size_t)nimgs * P * 36), ddep(P * 4), darg(n * 4), dmin(n * 4),
-        dsec(n * 4);
-    for (DevBuf* b : {&dref, &dsrc, &dH, &ddep, &darg, &dmin, &dsec})
-        if (b->status != PSW_OK) return fail("RunAggregated", b->status);
-    int st = psw_memcpy_htod(dref.p, HostRef.data(), n * 4, nullptr);
-    for (int i = 0; i < nimgs && st == PSW_OK; ++i)
-        st = psw_memcpy_htod(dsrc.f() + (size_t)i * n, HostSrc[i].data(), n * 4, nullptr);
-    if (st == PSW_OK) st = psw_memcpy_htod(ddep.p, planes.data(), P * 4, nullptr);
+    psw_stream_t sm = stream();
+    float *dref, *dsrc, *dH, *ddep, *darg, *dmin, *dsec;
+    int st;
+    if ((st = buffer(REF, n * 4, &dref)) != PSW_OK || (st = buffer(SRC, n * nimgs * 4, &dsrc)) != PSW_OK ||
+        (st = buffer(HOM, (size_t)nimgs * P * 36, &dH)) != PSW_OK || (st = buffer(PLANES, P * 4, &ddep)) != PSW_OK ||
+        (st = buffer(ARG, n * 4, &darg)) != PSW_OK || (st = buffer(CMIN, n * 4, &dmin)) != PSW_OK ||
+        (st = buffer(CSEC, n * 4, &dsec)) != PSW_OK)
+        return fail("RunAggregated", st);
+    st = psw_memcpy_htod(dref, HostRef.data(), n * 4, sm);
+    for (int i = 0; i < nimgs && st == PSW_OK; ++i) st = psw_memcpy_htod(dsrc + (size_t)i * n, HostSrc[i].data(), n * 4, sm);
+    if (st == PSW_OK) st = psw_memcpy_htod(ddep, planes.data(), P * 4, sm);
     if (st == PSW_OK)
-        st = psw_sweep_homographies(dH.f(), ddep.f(), P, nimgs, Kf, Rr, tr, Rs.data(), ts.data(),
-                                    alternativemethod ? 1 : 0, nullptr);
+        st = psw_sweep_homographies(dH, ddep, P, nimgs, Kf, Rr, tr, Rs.data(), ts.data(), alternativemethod ? 1 : 0, sm);
     if (st == PSW_OK)
-        st = psw_sweep_fused_aggregate(dref.f(), dsrc.f(), w, h, nimgs, dH.f(), P, (int)winsize, stdthresh,
-                                       zeroBasedK ? 0 : 1, fast ? 1 : 0, static_cast<int32_t*>(darg.p), dmin.f(),
-                                       dsec.f(), nullptr);
+        st = psw_sweep_fused_aggregate(dref, dsrc, w, h, nimgs, dH, P, (int)winsize, stdthresh, zeroBasedK ? 0 : 1,
+                                       fast ? 1 : 0, reinterpret_cast<int32_t*>(darg), dmin, dsec, sm);
     argmin.assign(n, -1);
     costmin.reset(w, h);
     costsecond.reset(w, h);
     depthmap.reset(w, h);
-    if (st == PSW_OK) st = psw_memcpy_dtoh(argmin.data(), darg.p, n * 4, nullptr);
-    if (st == PSW_OK) st = psw_memcpy_dtoh(costmin.data(), dmin.p, n * 4, nullptr);
-    if (st == PSW_OK) st = psw_memcpy_dtoh(costsecond.data(), dsec.p, n * 4, nullptr);
+    if (st == PSW_OK) st = psw_memcpy_dtoh(argmin.data(), darg, n * 4, sm);
+    if (st == PSW_OK) st = psw_memcpy_dtoh(costmin.data(), dmin, n * 4, sm);
+    if (st == PSW_OK) st = psw_memcpy_dtoh(costsecond.data(), dsec, n * 4, sm);
     if (st != PSW_OK) return fail("RunAggregated", st);
     for (size_t i = 0; i < n; ++i) depthmap[i] = argmin[i] >= 0 ? planes[argmin[i]] : zfar;
     ConvertDepthtoUChar(depthmap, depthmap8u);
@@ -225,24 +273,19 @@ bool PlaneSweep::CudaDenoise(int argc, char** argv, const unsigned int niters, c
     const int w = (int)depthmap.width(), h = (int)depthmap.height();
     const size_t n = (size_t)w * h;
     if (HostRef.width() != (size_t)w || HostRef.height() != (size_t)h) return fail("CudaDenoise", PSW_ERR_ARG);
-    if (d_depthmap) {
-        psw_free(d_depthmap);
-        d_depthmap = nullptr;
-    }
-    void* p = nullptr;
-    int st = psw_malloc(&p, n * 4);
-    if (st != PSW_OK) return fail("CudaDenoise", st);
-    d_depthmap = static_cast<float*>(p);
-    DevBuf din(n * 4), dref(n * 4), work(12 * n * 4);
-    for (DevBuf* b : {&din, &dref, &work})
-        if (b->status != PSW_OK) return fail("CudaDenoise", b->status);
-    st = psw_memcpy_htod(din.p, depthmap.data(), n * 4, nullptr);
-    if (st == PSW_OK) st = psw_memcpy_htod(dref.p, HostRef.data(), n * 4, nullptr);
+    psw_stream_t sm = stream();
+    float *dout, *din, *dref, *work;
+    int st;
+    if ((st = buffer(DEPTHMAP, n * 4, &dout)) != PSW_OK || (st = buffer(DENSE, n * 4, &din)) != PSW_OK ||
+        (st = buffer(REF, n * 4, &dref)) != PSW_OK || (st = buffer(WORK, 12 * n * 4, &work)) != PSW_OK)
+        return fail("CudaDenoise", st);
+    st = psw_memcpy_htod(din, depthmap.data(), n * 4, sm);
+    if (st == PSW_OK) st = psw_memcpy_htod(dref, HostRef.data(), n * 4, sm);
     if (st == PSW_OK)
-        st = psw_tvl1_denoise(din.f(), dref.f(), w, h, znear, zfar, (int)niters, lambda, tau, sigma, theta, beta, gamma,
-                              d_depthmap, work.f(), nullptr);
+        st = psw_tvl1_denoise(din, dref, w, h, znear, zfar, (int)niters, lambda, tau, sigma, theta, beta, gamma, dout,
+                              work, sm);
     depthmapdenoised.reset(w, h);
-    if (st == PSW_OK) st = psw_memcpy_dtoh(depthmapdenoised.data(), d_depthmap, n * 4, nullptr);
+    if (st == PSW_OK) st = psw_memcpy_dtoh(depthmapdenoised.data(), dout, n * 4, sm);
     if (st != PSW_OK) return fail("CudaDenoise", st);
     ConvertDepthtoUChar(depthmapdenoised, depthmap8udenoised);
     std::cout << "Time taken for the TVL1 denoising to complete is " << tm.ms() << "ms\n\n";
@@ -269,26 +312,27 @@ bool PlaneSweep::TGV(int argc, char** argv, const unsigned int niters, const uns
         R.toArray(&Rrel[9 * i]);
         trel[3 * i] = t.x; trel[3 * i + 1] = t.y; trel[3 * i + 2] = t.z;
     }
+    psw_stream_t sm = stream();
     psw_tgv* t = nullptr;
-    int st = psw_tgv_create(&t, w, h, nimages);
-    if (st != PSW_OK) return fail("TGV", st);
-    DevBuf dref(n * 4), dsrc(n * nimages * 4), dinit(n * 4), dout(n * 4);
-    for (DevBuf* b : {&dref, &dsrc, &dinit, &dout})
-        if (b->status != PSW_OK) { psw_tgv_destroy(t); return fail("TGV", b->status); }
-    st = psw_memcpy_htod(dref.p, HostRef.data(), n * 4, nullptr);
+    float *dref, *dsrc, *dinit, *dout;
+    int st;
+    if ((st = solver(w, h, nimages, &t)) != PSW_OK || (st = buffer(REF, n * 4, &dref)) != PSW_OK ||
+        (st = buffer(SRC, n * nimages * 4, &dsrc)) != PSW_OK || (st = buffer(INIT, n * 4, &dinit)) != PSW_OK ||
+        (st = buffer(OUT, n * 4, &dout)) != PSW_OK)
+        return fail("TGV", st);
+    st = psw_memcpy_htod(dref, HostRef.data(), n * 4, sm);
     for (int i = 0; i < nimages && st == PSW_OK; ++i)
-        st = psw_memcpy_htod(dsrc.f() + (size_t)i * n, HostSrc[i].data(), n * 4, nullptr);
+        st = psw_memcpy_htod(dsrc + (size_t)i * n, HostSrc[i].data(), n * 4, sm);
     const bool warm = tgvinit.width() == (size_t)w && tgvinit.height() == (size_t)h;
-    if (st == PSW_OK && warm) st = psw_memcpy_htod(dinit.p, tgvinit.data(), n * 4, nullptr);
-    if (st == PSW_OK) st = psw_tgv_set_images(t, dref.f(), dsrc.f(), beta, gamma, nullptr);
+    if (st == PSW_OK && warm) st = psw_memcpy_htod(dinit, tgvinit.data(), n * 4, sm);
+    if (st == PSW_OK) st = psw_tgv_set_images(t, dref, dsrc, beta, gamma, sm);
     psw_tgv_params prm = {lambda, alpha0, alpha1, tau, sigma, beta, gamma};
     float Kf[9];
     K.toArray(Kf);
     if (st == PSW_OK)
-        st = psw_tgv_run(t, Kf, Rrel.data(), trel.data(), (int)niters, (int)warps, &prm, warm ? dinit.f() : nullptr,
-                         dout.f(), nullptr);
-    if (st == PSW_OK) st = psw_memcpy_dtoh(depthmapTGV.data(), dout.p, n * 4, nullptr);
-    psw_tgv_destroy(t);
+        st = psw_tgv_run(t, Kf, Rrel.data(), trel.data(), (int)niters, (int)warps, &prm, warm ? dinit : nullptr, dout,
+                         sm);
+    if (st == PSW_OK) st = psw_memcpy_dtoh(depthmapTGV.data(), dout, n * 4, sm);
     if (st != PSW_OK) return fail("TGV", st);
     ConvertDepthtoUChar(depthmapTGV, depthmap8uTGV);
     std::cout << "Time taken for the TGV to complete is " << tm.ms() << "ms\n\n";
@@ -309,29 +353,31 @@ bool PlaneSweep::TGVSolve(int max_iters, int check_every, double tol, double e_s
         R.toArray(&Rrel[9 * i]);
         trel[3 * i] = t.x; trel[3 * i + 1] = t.y; trel[3 * i + 2] = t.z;
     }
+    psw_stream_t sm = stream();
     psw_tgv* t = nullptr;
-    int st = psw_tgv_create(&t, w, h, nimages);
-    if (st != PSW_OK) return fail("TGVSolve", st);
-    DevBuf dref(n * 4), dsrc(n * nimages * 4), dinit(n * 4), dout(n * 4);
-    st = dref.status | dsrc.status | dinit.status | dout.status;
-    if (st == PSW_OK) st = psw_memcpy_htod(dref.p, HostRef.data(), n * 4, nullptr);
+    float *dref, *dsrc, *dinit, *dout;
+    int st;
+    if ((st = solver(w, h, nimages, &t)) != PSW_OK || (st = buffer(REF, n * 4, &dref)) != PSW_OK ||
+        (st = buffer(SRC, n * nimages * 4, &dsrc)) != PSW_OK || (st = buffer(INIT, n * 4, &dinit)) != PSW_OK ||
+        (st = buffer(OUT, n * 4, &dout)) != PSW_OK)
+        return fail("TGVSolve", st);
+    st = psw_memcpy_htod(dref, HostRef.data(), n * 4, sm);
     for (int i = 0; i < nimages && st == PSW_OK; ++i)
-        st = psw_memcpy_htod(dsrc.f() + (size_t)i * n, HostSrc[i].data(), n * 4, nullptr);
-    if (st == PSW_OK) st = psw_tgv_set_images(t, dref.f(), dsrc.f(), DEFAULT_TGV_BETA, DEFAULT_TGV_GAMMA, nullptr);
+        st = psw_memcpy_htod(dsrc + (size_t)i * n, HostSrc[i].data(), n * 4, sm);
+    if (st == PSW_OK) st = psw_tgv_set_images(t, dref, dsrc, DEFAULT_TGV_BETA, DEFAULT_TGV_GAMMA, sm);
     const bool warm = tgvinit.width() == (size_t)w && tgvinit.height() == (size_t)h;
-    if (st == PSW_OK && warm) st = psw_memcpy_htod(dinit.p, tgvinit.data(), n * 4, nullptr);
-    if (st == PSW_OK) st = psw_tgv_set_u(t, warm ? dinit.f() : nullptr, 1.f, nullptr);
+    if (st == PSW_OK && warm) st = psw_memcpy_htod(dinit, tgvinit.data(), n * 4, sm);
+    if (st == PSW_OK) st = psw_tgv_set_u(t, warm ? dinit : nullptr, 1.f, sm);
     float Kf[9];
     K.toArray(Kf);
-    if (st == PSW_OK) st = psw_tgv_linearise(t, Kf, Rrel.data(), trel.data(), nullptr);
+    if (st == PSW_OK) st = psw_tgv_linearise(t, Kf, Rrel.data(), trel.data(), sm);
     psw_tgv_params prm = {lambda, alpha0, alpha1, tau, sigma, DEFAULT_TGV_BETA, DEFAULT_TGV_GAMMA};
     int it = 0;
     double e = 0;
-    if (st == PSW_OK) st = psw_tgv_solve(t, max_iters, check_every, tol, e_star, &prm, &it, &e, nullptr);
+    if (st == PSW_OK) st = psw_tgv_solve(t, max_iters, check_every, tol, e_star, &prm, &it, &e, sm);
     depthmapTGV.reset(w, h);
-    if (st == PSW_OK) st = psw_tgv_get_u(t, dout.f(), nullptr);
-    if (st == PSW_OK) st = psw_memcpy_dtoh(depthmapTGV.data(), dout.p, n * 4, nullptr);
-    psw_tgv_destroy(t);
+    if (st == PSW_OK) st = psw_tgv_get_u(t, dout, sm);
+    if (st == PSW_OK) st = psw_memcpy_dtoh(depthmapTGV.data(), dout, n * 4, sm);
     if (st != PSW_OK) return fail("TGVSolve", st);
     if (iters) *iters = it;
     if (energy) *energy = e;
@@ -351,24 +397,22 @@ bool PlaneSweep::TGVdenoiseFromSparse(int argc, char** argv, const CamImage<floa
         depthmap.width() != (size_t)w || depthmap.height() != (size_t)h)
         return fail("TGVdenoiseFromSparse", PSW_ERR_ARG);
     depthmapTGV.reset(w, h);
-    if (d_depthmap) {   // holds the result afterwards, as in the reference
-        psw_free(d_depthmap);
-        d_depthmap = nullptr;
-    }
-    void* p = nullptr;
-    int st = psw_malloc(&p, n * 4);
-    if (st != PSW_OK) return fail("TGVdenoiseFromSparse", st);
-    d_depthmap = static_cast<float*>(p);
-    DevBuf dref(n * 4), ddense(n * 4), dsparse(n * 4);
-    for (DevBuf* b : {&dref, &ddense, &dsparse})
-        if (b->status != PSW_OK) return fail("TGVdenoiseFromSparse", b->status);
-    st = psw_memcpy_htod(dref.p, HostRef.data(), n * 4, nullptr);
-    if (st == PSW_OK) st = psw_memcpy_htod(ddense.p, depthmap.data(), n * 4, nullptr);
-    if (st == PSW_OK) st = psw_memcpy_htod(dsparse.p, depth.data(), n * 4, nullptr);
+    psw_stream_t sm = stream();
+    float *dout, *dref, *ddense, *dsparse;   // dout: d_depthmap, which holds the result afterwards as in the reference
+    int st;
+    if ((st = buffer(DEPTHMAP, n * 4, &dout)) != PSW_OK || (st = buffer(REF, n * 4, &dref)) != PSW_OK ||
+        (st = buffer(DENSE, n * 4, &ddense)) != PSW_OK || (st = buffer(SPARSE, n * 4, &dsparse)) != PSW_OK)
+        return fail("TGVdenoiseFromSparse", st);
+    float* ws = nullptr;
+    const size_t wsb = psw_tgv_sparse_workspace_bytes(w, h);
+    if ((st = buffer(WORK, wsb, &ws)) != PSW_OK) return fail("TGVdenoiseFromSparse", st);
+    st = psw_memcpy_htod(dref, HostRef.data(), n * 4, sm);
+    if (st == PSW_OK) st = psw_memcpy_htod(ddense, depthmap.data(), n * 4, sm);
+    if (st == PSW_OK) st = psw_memcpy_htod(dsparse, depth.data(), n * 4, sm);
     if (st == PSW_OK)
-        st = psw_tgv_sparse_denoise(dref.f(), ddense.f(), dsparse.f(), w, h, zfar, (int)niters, alpha0, alpha1, tau,
-                                    sigma, theta, beta, gamma, nullptr, 0, d_depthmap, nullptr);
-    if (st == PSW_OK) st = psw_memcpy_dtoh(depthmapTGV.data(), d_depthmap, n * 4, nullptr);
+        st = psw_tgv_sparse_denoise(dref, ddense, dsparse, w, h, zfar, (int)niters, alpha0, alpha1, tau, sigma, theta,
+                                    beta, gamma, ws, wsb, dout, sm);
+    if (st == PSW_OK) st = psw_memcpy_dtoh(depthmapTGV.data(), dout, n * 4, sm);
     if (st != PSW_OK) return fail("TGVdenoiseFromSparse", st);
     ConvertDepthtoUChar(depthmapTGV, depthmap8uTGV);
     std::cout << "Time taken for the TGV to complete is " << tm.ms() << "ms\n\n";
@@ -386,24 +430,27 @@ void PlaneSweep::get3Dcoordinates(CamImage<float>*& x, CamImage<float>*& y, CamI
         fail("get3Dcoordinates", PSW_ERR_ARG);
         return;
     }
-    DevBuf Px(n * 4), Py(n * 4), X(n * 4);
     float R9[9], t3[3] = {t.x, t.y, t.z}, iK[9];
     Rr.toArray(R9);
     invK.toArray(iK);
-    int st = Px.status | Py.status | X.status;
-    if (st == PSW_OK) st = psw_memcpy_htod(X.p, depthmapdenoised.data(), n * 4, nullptr);
+    psw_stream_t sm = stream();
+    float *px, *py, *pz;
+    int st = buffer(CX, n * 4, &px);
+    if (st == PSW_OK) st = buffer(CY, n * 4, &py);
+    if (st == PSW_OK) st = buffer(CZ, n * 4, &pz);
+    if (st == PSW_OK) st = psw_memcpy_htod(pz, depthmapdenoised.data(), n * 4, sm);
     // x, y of pixels at or beyond the depth limit are not written by the
     // kernel (uninitialised in the reference); they read 0 here.
-    if (st == PSW_OK) st = psw_fill_f32(Px.f(), 0.f, n, nullptr);
-    if (st == PSW_OK) st = psw_fill_f32(Py.f(), 0.f, n, nullptr);
+    if (st == PSW_OK) st = psw_fill_f32(px, 0.f, n, sm);
+    if (st == PSW_OK) st = psw_fill_f32(py, 0.f, n, sm);
     const psw_dim3 zd = {0, 0, 0};
-    if (st == PSW_OK) st = psw_compute3D(Px.f(), Py.f(), X.f(), R9, t3, iK, w, h, zd, zd, nullptr);
+    if (st == PSW_OK) st = psw_compute3D(px, py, pz, R9, t3, iK, w, h, zd, zd, sm);
     coord_x.reset(w, h);
     coord_y.reset(w, h);
     coord_z.reset(w, h);
-    if (st == PSW_OK) st = psw_memcpy_dtoh(coord_x.data(), Px.p, n * 4, nullptr);
-    if (st == PSW_OK) st = psw_memcpy_dtoh(coord_y.data(), Py.p, n * 4, nullptr);
-    if (st == PSW_OK) st = psw_memcpy_dtoh(coord_z.data(), X.p, n * 4, nullptr);
+    if (st == PSW_OK) st = psw_memcpy_dtoh(coord_x.data(), px, n * 4, sm);
+    if (st == PSW_OK) st = psw_memcpy_dtoh(coord_y.data(), py, n * 4, sm);
+    if (st == PSW_OK) st = psw_memcpy_dtoh(coord_z.data(), pz, n * 4, sm);
     if (st != PSW_OK) {
         fail("get3Dcoordinates", st);
         return;

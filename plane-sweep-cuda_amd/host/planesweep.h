@@ -57,6 +57,9 @@ public:
     PlaneSweep();
     PlaneSweep(int argc, char** argv);
     ~PlaneSweep();
+    // The object owns device buffers and a stream: not copyable.
+    PlaneSweep(const PlaneSweep&) = delete;
+    PlaneSweep& operator=(const PlaneSweep&) = delete;
 
     // PlaneSweep::RunAlgorithm (planesweep.cpp:74-161): depthmap of HostRef
     // from HostSrc[0..numberimages) with the reference's per-source argmax
@@ -145,6 +148,12 @@ public:
                   double* energy = nullptr);
     const std::string& lastError() const { return lasterror; }
     std::vector<float> planeDepths() const;
+    // The object's stream: every run of this object is queued on it, and a
+    // run returns once its results are copied back.  Created on first use.
+    psw_stream_t stream();
+    // Device bytes this object's buffer pool holds (grow-only: a run allocates
+    // only when an image, source count or plane count is larger than before).
+    size_t deviceBytes() const;
 
 protected:
     Matrix3D K;
@@ -153,7 +162,7 @@ protected:
     CamImage<uchar> depthmap8u, depthmap8udenoised;
     CamImage<float> depthmapTGV;
     CamImage<uchar> depthmap8uTGV;
-    float* d_depthmap = nullptr;
+    float* d_depthmap = nullptr;   // the pool's DEPTHMAP buffer (getDepthmapDenoisedPtr)
     CamImage<float> coord_x, coord_y, coord_z;
 
     float znear = DEFAULT_Z_NEAR;
@@ -178,6 +187,20 @@ protected:
     int deviceInit(int argc, char** argv);
     bool fail(const char* where, int status);
     int nImages() const;
+
+    // Per-object device buffers, one per role, grown on demand and freed
+    // with the object (the reference allocates and frees its device images
+    // on every call, src/planesweep.cpp:122-123,172-189,313-316).
+    enum Slot { REF, SRC, HOM, PLANES, OUT, ARG, CMIN, CSEC, INIT, WORK, DENSE, SPARSE, CX, CY, CZ, DEPTHMAP, NSLOTS };
+    void* pool[NSLOTS] = {};
+    size_t poolcap[NSLOTS] = {};
+    int device = -1;               // device the pool and the stream belong to
+    psw_stream_t strm = nullptr;
+    psw_tgv* tgv = nullptr;        // TGV2 solver, kept while the shape stays the same
+    int tgvw = 0, tgvh = 0, tgvn = 0;
+    int buffer(Slot s, size_t bytes, float** out);
+    int solver(int w, int h, int nimages, psw_tgv** out);
+    void release();
 };
 
 }  // namespace psw

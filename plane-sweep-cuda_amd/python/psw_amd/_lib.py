@@ -164,6 +164,7 @@ _SIGS = {
                               c_void_p]),
     "psw_tgv_get_u": (c_int, [c_void_p, D, c_void_p]),
     "psw_tgv_get_state": (c_int, [c_void_p, D, c_void_p]),
+    "psw_tgv_get_energy_mask": (c_int, [c_void_p, D, c_void_p]),
     "psw_tgv_run": (c_int, [c_void_p, H, H, H, c_int, c_int, P(TgvParams), D, D, c_void_p]),
     "psw_tgv_sparse_workspace_bytes": (c_size_t, [c_int, c_int]),
     "psw_tgv_sparse_denoise": (c_int, [D, D, D, c_int, c_int, c_float, c_int] + [c_double] * 7 + [D, c_size_t, D,

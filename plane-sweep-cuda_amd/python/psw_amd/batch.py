@@ -8,6 +8,7 @@ entry points (tests/test_batch.py); the autotuner only picks launch shapes
 results.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -108,8 +109,10 @@ def _time(fn, stream, reps=3):
 
 
 def autotune(frames, prm, fast=True, tgv_iters=20, solve_iters=200, stream=None, log=None):
-    """Time the launch shapes of the batched sweep (plane groups x resident
-    workgroups), of the batched TGV2 iteration (tile shape of the
+    """Time the launch shapes of the batched sweep (NCC tile height x plane
+    groups x resident workgroups; the tile height sets the LDS halo tile,
+    32 + 2r by TY + 2r for window radius r, through PSW_SWEEP_TY), of the
+    batched TGV2 iteration (tile shape of the
     one-iteration launches) and the batched solve's chunks replayed from a
     captured hipGraph or launched directly, on these frames; returns the
     fastest settings and the timings.  Results do not depend on the choice.
@@ -120,12 +123,21 @@ def autotune(frames, prm, fast=True, tgv_iters=20, solve_iters=200, stream=None,
     for sw in sweeps:
         sw.build_homographies(s)
     timings = {"sweep": {}, "tgv": {}}
-    for g in (1, 2, 4, 8):
-        for wg in (1024, 2048, 4096):
-            ms = _time(lambda: sweep_batch(sweeps, fast, g, wg, s), st)
-            timings["sweep"][(g, wg)] = ms
-            if log:
-                log(f"[autotune] sweep groups {g} workgroups {wg}: {ms:.3f} ms")
+    prev_ty = os.environ.get("PSW_SWEEP_TY")
+    try:
+        for ty in (16, 32):                       # NCC tile: 32 x 16 or 32 x 32 (halo = window radius)
+            os.environ["PSW_SWEEP_TY"] = str(ty)
+            for g in (1, 2, 4, 8):
+                for wg in (1024, 2048, 4096):
+                    ms = _time(lambda: sweep_batch(sweeps, fast, g, wg, s), st)
+                    timings["sweep"][(ty, g, wg)] = ms
+                    if log:
+                        log(f"[autotune] sweep tile 32x{ty} groups {g} workgroups {wg}: {ms:.3f} ms")
+    finally:
+        if prev_ty is None:
+            os.environ.pop("PSW_SWEEP_TY", None)
+        else:
+            os.environ["PSW_SWEEP_TY"] = prev_ty
     solvers = [f.tgv for f in frames]
     for var in (0, 1, 2):
         ms = _time(lambda: tgv_iterate_batch(solvers, tgv_iters, prm, var, s), st) / tgv_iters
@@ -143,5 +155,5 @@ def autotune(frames, prm, fast=True, tgv_iters=20, solve_iters=200, stream=None,
         if log:
             log(f"[autotune] tgv solve chunks {'graph' if g else 'direct'}: {ms * 1e3:.1f} us per iteration")
     best_graph = min(timings["graph"], key=timings["graph"].get)
-    return {"sweep_groups": best_sweep[0], "sweep_workgroups": best_sweep[1], "tgv_variant": best_var,
-            "use_graph": bool(best_graph)}, timings
+    return {"sweep_tile_h": best_sweep[0], "sweep_groups": best_sweep[1], "sweep_workgroups": best_sweep[2],
+            "tgv_variant": best_var, "use_graph": bool(best_graph)}, timings

@@ -120,8 +120,10 @@ class SweepAggregate:
              stream)
         return self.d_depth
 
-    def results(self):
-        return self.d_arg.to_host(), self.d_cmin.to_host(), self.d_csec.to_host()
+    def results(self, stream=None):
+        """(argmin, min cost, second cost) on the host, after the work queued
+        on `stream` (the stream the sweep ran on)."""
+        return self.d_arg.to_host(stream), self.d_cmin.to_host(stream), self.d_csec.to_host(stream)
 
 
 def sweep_aggregate(ref, src, K, Rref, tref, Rsrc, tsrc, planes, winsize, stdthresh, coord_offset=0, fast=False,
@@ -204,6 +206,12 @@ class TgvSolver:
     def get_u(self, stream=None):
         d = DeviceArray((self.H, self.W))
         call("psw_tgv_get_u", self.h, d.ptr, stream)
+        return d.to_host(stream)
+
+    def get_energy_mask(self, stream=None):
+        """(H, W) weights of the energy: 1 counted, 0 left out."""
+        d = DeviceArray((self.H, self.W))
+        call("psw_tgv_get_energy_mask", self.h, d.ptr, stream)
         return d.to_host(stream)
 
     def state_device(self, stream=None):

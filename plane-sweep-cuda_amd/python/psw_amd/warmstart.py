@@ -150,6 +150,48 @@ class WarmStarter:
         tgv.set_primal(self.d_u, 0.0, self.d_u1, stream)
         tgv.set_duals(self.d_p, self.d_q, None, stream)
 
+    def solve(self, tgv, d_u0, prm, e_star, tol, max_iters, check_every, fallback_after=None, stream=None):
+        """Solve from the prediction, with the plain start (u = u0, the rest 0)
+        as fallback.  The prediction is kept only if its energy is below the
+        plain start's; a solve from it that has not reached the gap after
+        `fallback_after` iterations (default: max_iters) restarts from the
+        plain start with the budget that is left.  tgv must be linearised at
+        d_u0 (its state is the plain start).  Returns (iterations, energy,
+        start) with start 'learned', 'plain' (rejected prediction) or
+        'learned+plain' (restarted)."""
+        e_plain = tgv.energy(prm, stream)
+        self.start(tgv, stream)
+        e_learned = tgv.energy(prm, stream)
+        if not e_learned < e_plain:                  # NaN-safe: a NaN prediction is rejected
+            tgv.set_primal(d_u0, 0.0, None, stream)
+            tgv.set_duals(None, None, None, stream)
+            it, e = tgv.solve(max_iters, check_every, tol, e_star, prm, stream)
+            return it, e, "plain"
+        first = max_iters if fallback_after is None else min(int(fallback_after), max_iters)
+        it, e = tgv.solve(first, check_every, tol, e_star, prm, stream)
+        if e_star <= 0 or abs(e - e_star) <= tol * e_star or first >= max_iters:
+            return it, e, "learned"
+        tgv.set_primal(d_u0, 0.0, None, stream)
+        tgv.set_duals(None, None, None, stream)
+        it2, e = tgv.solve(max_iters - it, check_every, tol, e_star, prm, stream)
+        return it + it2, e, "learned+plain"
+
+
+def judge(u, u_ref, weight):
+    """Distance of a solve's stop state u from a reference stop state u_ref
+    (the plain or cold start solved to the same gap), relative to |u_ref|:
+    rms and max over the pixels the energy counts (weight 1) and over the
+    ones it leaves out."""
+    e = np.abs(np.asarray(u, np.float64) - u_ref) / np.maximum(np.abs(u_ref), 1e-12)
+    keep = np.asarray(weight) > 0
+    out = {}
+    for name, sel in (("counted", keep), ("left_out", ~keep)):
+        out[name] = {"pixels": int(sel.sum()),
+                     "rms_rel": float(np.sqrt(np.mean(e[sel] ** 2))) if sel.any() else 0.0,
+                     "max_rel": float(e[sel].max()) if sel.any() else 0.0,
+                     "median_rel": float(np.median(e[sel])) if sel.any() else 0.0}
+    return out
+
 
 # ---- training data ------------------------------------------------------------------
 
@@ -251,6 +293,82 @@ def train(n_scenes=1000, steps=20000, batch=8, crop=(256, 256), lr=1e-3, tgv_ite
     return net, hist, val
 
 
+def ceiling(seed=2024, weights=DEFAULT_WEIGHTS, estar_iters=40000, qmax=1.0, tol=1e-4, max_iters=50000,
+            check_every=10, log=print):
+    """How far a warm start can shorten the solve to the energy gap, on one
+    640x480 scene: iterations from the plain start, from the converged state
+    of the same problem (E*'s state, after estar_iters plain iterations) with
+    all of it, without the data dual r, with r set to the sign of the
+    residual, with relative noise on u, and from the network's prediction.
+    Returns {start: iterations}."""
+    from .device import Stream
+    st = Stream()
+    s = st.handle
+    W, H = FULL_W, FULL_H
+    v = scene.make_views_gpu(seed, W, H, NSRC, stream=s)
+    planes = stages.inverse_depth_planes(ZNEAR, ZFAR, NPLANES)
+    sw = stages.SweepAggregate(v["d_ref"], v["d_src"], v["K"], v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], planes,
+                               WIN, STD_T, coord_offset=0)
+    sw.build_homographies(s)
+    sw.run(fast=True, stream=s)
+    d_s = sw.depth(ZFAR, s)
+    Rrel, trel = stages.relative_all(v["Rref"], v["tref"], v["Rsrc"], v["tsrc"])
+    K1 = scene.one_based(v["K"])
+    prm = stages.tgv_params(**TGV)
+    t = stages.TgvSolver(W, H, NSRC)
+    t.set_images(sw.d_ref, sw.d_src, TGV["beta"], TGV["gamma"], s)
+
+    def relinearise():
+        t.set_u(d_s, stream=s)
+        t.linearise(K1, Rrel, trel, s)
+        t.set_energy_mask(qmax, prm, s, count=False)
+
+    relinearise()
+    t.iterate(estar_iters, prm, s)
+    e_star = t.energy(prm, s)
+    S = t.get_state(s)
+    st.synchronize()
+    rho = S["It"] + (S["u"] - S["u0"])[None] * S["Iu"]
+
+    def up(a):
+        return DeviceArray.from_host(np.ascontiguousarray(a, np.float32))
+
+    out = {}
+
+    def run(label, u, u1, p, q, r):
+        relinearise()
+        keep = [up(u), up(u1), up(p), up(q), up(r) if r is not None else None]
+        t.set_primal(keep[0], 0.0, keep[1], s)
+        t.set_duals(keep[2], keep[3], keep[4], s)
+        it, e = t.solve(max_iters, check_every, tol, e_star, prm, s)
+        st.synchronize()
+        out[label] = int(it)
+        log(f"[ceiling] {label:44s} {it:6d} iterations (gap {(e - e_star) / e_star:+.2e})")
+
+    zero2, zero3 = np.zeros((2, H, W), np.float32), np.zeros((3, H, W), np.float32)
+    u1s, ps = np.stack([S["u1x"], S["u1y"]]), np.stack([S["px"], S["py"]])
+    qs = np.stack([S["qx"], S["qy"], S["qz"]])
+    run("plain start (u = u0, the rest 0)", S["u0"], zero2, zero2, zero3, None)
+    run("converged state, r included", S["u"], u1s, ps, qs, S["r"])
+    run("converged (u, u1, p, q), r = 0", S["u"], u1s, ps, qs, None)
+    run("converged (u, u1, p, q), r = sign(residual)", S["u"], u1s, ps, qs, np.sign(rho))
+    rng = np.random.default_rng(seed)
+    for eps in (1e-4, 1e-3, 1e-2):
+        un = (S["u"] * (1.0 + eps * rng.standard_normal(S["u"].shape))).astype(np.float32)
+        run(f"converged, r included, u x (1 + {eps:g} noise)", un, u1s, ps, qs, S["r"])
+        rn = S["It"] + (un - S["u0"])[None] * S["Iu"]
+        run(f"converged, r = sign(residual), u x (1 + {eps:g} noise)", un, u1s, ps, qs, np.sign(rn))
+    if weights is not None:
+        ws = WarmStarter(W, H, weights)
+        ws.predict(sw, ZFAR, s)
+        uL, u1L, pL, qL = (ws.d_u.to_host(s), ws.d_u1.to_host(s), ws.d_p.to_host(s), ws.d_q.to_host(s))
+        run("network (u, u1, p, q), r = 0", uL, u1L, pL, qL, None)
+        rL = S["It"] + (uL - S["u0"])[None] * S["Iu"]
+        run("network (u, u1, p, q), r = sign(residual)", uL, u1L, pL, qL, np.sign(rL))
+        run("converged u, network (u1, p, q), r = 0", S["u"], u1L, pL, qL, None)
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -262,7 +380,14 @@ def main(argv=None):
     t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--tgv-iters", type=int, default=20000)
     t.add_argument("--out", default=DEFAULT_WEIGHTS)
+    c = sub.add_parser("ceiling", help="iterations to the gap from exact, perturbed and predicted starts")
+    c.add_argument("--seed", type=int, default=2024)
+    c.add_argument("--weights", default=DEFAULT_WEIGHTS)
     a = ap.parse_args(argv)
+    if a.cmd == "ceiling":
+        import json
+        print(json.dumps(ceiling(a.seed, a.weights, log=lambda *x: print(*x, flush=True))), flush=True)
+        return 0
     if a.cmd == "train":
         train(a.scenes, a.steps, a.batch, (a.crop, a.crop), a.lr, a.tgv_iters, out=a.out,
               log=lambda *x: print(*x, flush=True))

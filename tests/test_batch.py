@@ -21,9 +21,11 @@ def _frames():
     return [batch.Frame(sp, znear=0.5, zfar=6.0, winsize=7) for sp in SPECS]
 
 
-@pytest.mark.parametrize("fast,groups", [(True, 0), (False, 1), (True, 3)])
-def test_sweep_batch_matches_single(gpu, fast, groups):
+@pytest.mark.parametrize("fast,groups,ty", [(True, 0, "16"), (False, 1, "16"), (True, 3, "16"), (True, 0, "32"),
+                                             (False, 2, "32")])
+def test_sweep_batch_matches_single(gpu, monkeypatch, fast, groups, ty):
     from psw_amd import batch
+    monkeypatch.setenv("PSW_SWEEP_TY", ty)
     frames = _frames()
     single = []
     for f in frames:

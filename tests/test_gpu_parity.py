@@ -69,9 +69,13 @@ def test_sweep_homographies_fp64_match_oracle(gpu, views_small):
             np.testing.assert_array_equal(got[i, p].reshape(3, 3), exp.astype(np.float32))
 
 
-@pytest.mark.parametrize("winsize,offset", [(7, 0), (9, 0), (5, 1)])
-def test_sweep_fused_aggregate_bit_exact(gpu, views_small, winsize, offset):
+@pytest.mark.parametrize("winsize,offset,ty", [(7, 0, "16"), (9, 0, "16"), (5, 1, "16"), (9, 0, "32"), (3, 1, "32"),
+                                               (15, 0, "32")])
+def test_sweep_fused_aggregate_bit_exact(gpu, views_small, monkeypatch, winsize, offset, ty):
+    """Reference-order aggregated sweep == FP32 oracle, bit for bit, with
+    32 x 16 and 32 x 32 tiles (PSW_SWEEP_TY)."""
     from psw_amd import stages
+    monkeypatch.setenv("PSW_SWEEP_TY", ty)
     v = views_small
     K = v["K"] if offset == 0 else one_based(v["K"])
     planes = stages.inverse_depth_planes(0.5, 6.0, 24)
@@ -87,13 +91,15 @@ def test_sweep_fused_aggregate_bit_exact(gpu, views_small, winsize, offset):
 
 @pytest.mark.parametrize("fast", [False, True])
 def test_sweep_plane_groups_identical(gpu, views_small, monkeypatch, fast):
-    """Splitting the plane loop into groups merged afterwards changes nothing:
-    1, 3 and 24 groups give identical argmin / min / second-best."""
+    """Splitting the plane loop into groups merged afterwards, and the tile
+    height, change nothing: 1, 3 and 24 groups with 16-row tiles and 1 and 5
+    groups with 32-row tiles give identical argmin / min / second-best."""
     from psw_amd import stages
     v = views_small
     planes = stages.inverse_depth_planes(0.5, 6.0, 24)
     out = []
-    for g in ("1", "3", "24"):
+    for ty, g in (("16", "1"), ("16", "3"), ("16", "24"), ("32", "1"), ("32", "5")):
+        monkeypatch.setenv("PSW_SWEEP_TY", ty)
         monkeypatch.setenv("PSW_SWEEP_GROUPS", g)
         out.append(stages.sweep_aggregate(v["ref"], v["src"], v["K"], v["Rref"], v["tref"], v["Rsrc"], v["tsrc"],
                                           planes, 7, STD_T, coord_offset=0, fast=fast))

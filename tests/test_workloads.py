@@ -1,0 +1,436 @@
+"""The measured workloads at their own sizes on the GPU, against the CPU oracle.
+
+Each test runs one seeded full-size workload through the C ABI and compares
+it with the oracle on the same inputs (images rendered on the GPU and
+downloaded), printing the errors it measured:
+
+  - sweep, 640x480, 4 sources, 64 planes uniform in inverse depth over
+    0.5-5 m, 7x7 window; and 640x480, 8 sources, 256 planes over 0.3-8 m,
+    9x9 window.  Reference-order arithmetic: argmin, min and second-best
+    bit-identical to the FP32 oracle.  Both arithmetic variants against the
+    FP64 oracle's cost volume: argmin agreement >= 99.9 %, and at every
+    disagreeing pixel the FP64 costs of the two planes within a near-tie
+    margin (1e-5 reference-order, 2e-3 fast);
+  - TGV2 cold start (u = 1), one linearisation, 200 iterations (lambda 1,
+    alpha0 2, alpha1 1.5, tau 0.02, sigma 1) on the 4-source views, against
+    the FP64 oracle;
+  - TGV2 linearised at the 8-source sweep depth and solved to the energy
+    gap (1e-4 of E*) from the plain start and from the learned start, and
+    the stop states against the FP64 oracle run from the same start for the
+    same number of iterations; E* (the GPU's 40000-iteration energy) against
+    the FP64 oracle's energy of the same problem;
+  - fusion of 50 depthmaps into a 256^3 grid: histograms bit-identical to
+    the FP32 oracle, then 300 primal-dual iterations against the FP64
+    oracle on the same histograms, and the surface voxel count;
+  - 32 uneven frames (320x240, 640x480, 1242x375; 4-8 sources, 64-256
+    planes) batched, against the single-frame entry points frame by frame,
+    and one frame of each resolution against the oracle.
+
+TGV2 bars.  The reference's fixed steps (tau sigma = 0.02, lambda 1) leave
+the data step unstable at the pixels where tau sigma lambda^2 sum_j Iu_j^2 > 1
+(image borders and depth edges): the iteration oscillates there without end,
+in any precision, and FP32 and FP64 runs part there.  Over thousands of
+iterations rounding differences also drift along the problem's slow modes
+elsewhere, so no FP32 implementation stays within 1e-4 of FP64 at every
+pixel: the FP32 oracle (the reference's own arithmetic) ends up 1e-2 from the
+FP64 oracle at single stable pixels after a solve to the gap.  The bars, on
+the per-pixel error relative to max(|u|, 1) over the stable pixels: the 99th
+percentile within 1e-4; the 99th and 99.9th percentiles no more than 1.5x
+the FP32 oracle's own, run from the same start; and, for the solves, the
+FP64 energy at the stop (the quantity the stop rule decides on) within 1e-5
+of the FP64 oracle's energy at the same iteration.  Every quantile is
+printed, the unstable pixels' errors too.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import one_based, rel_err
+
+pytestmark = pytest.mark.gpu
+
+SEED = 2024                 # scene of the 640x480 workloads (the first bench scene)
+STD_T = 1e-4
+TGV_KW = dict(lambda_=1.0, alpha0=2.0, alpha1=1.5, tau=0.02, sigma=1.0, beta=0.0, gamma=1.0)
+QMAX = 1.0                  # data-step stability threshold of the energy mask
+GAP_TOL = 1e-4
+ESTAR_ITERS = 40000
+
+
+def _case(name, **kw):
+    print(f"[case] {name} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+@pytest.fixture(scope="module")
+def views640(gpu):
+    """The 640x480 scene with 8 sources, rendered on the GPU; host copies
+    for the oracle, device copies for the kernels."""
+    from psw_amd import scene
+    v = scene.make_views_gpu(SEED, 640, 480, 8)
+    v["ref"], v["src"], v["depth"] = v["d_ref"].to_host(), v["d_src"].to_host(), v["d_depth"].to_host()
+    return v
+
+
+def _sub(v, n):
+    return {"ref": v["ref"], "src": np.ascontiguousarray(v["src"][:n]), "K": v["K"], "Rref": v["Rref"],
+            "tref": v["tref"], "Rsrc": v["Rsrc"][:n], "tsrc": v["tsrc"][:n]}
+
+
+def _unstable(Iu, prm=TGV_KW, qmax=QMAX):
+    """Pixels whose data step is unstable, the energy mask's weight (0 at an
+    unstable pixel or its 4-neighbours) and the chessboard distance of every
+    pixel to the nearest unstable one."""
+    from scipy import ndimage
+    q = prm["tau"] * prm["sigma"] * prm["lambda_"] ** 2 * np.sum(np.asarray(Iu, np.float64) ** 2, axis=0)
+    bad = q > qmax
+    out = bad.copy()
+    out[1:] |= bad[:-1]
+    out[:-1] |= bad[1:]
+    out[:, 1:] |= bad[:, :-1]
+    out[:, :-1] |= bad[:, 1:]
+    dist = ndimage.distance_transform_cdt(~bad, metric="chessboard") if bad.any() else np.full(bad.shape, 10 ** 6)
+    return bad, (~out).astype(np.float32), dist
+
+
+def _quantiles(e):
+    return {"p50": float(np.quantile(e, 0.5)), "p99": float(np.quantile(e, 0.99)),
+            "p99.9": float(np.quantile(e, 0.999)), "max": float(e.max())}
+
+
+def _tgv_bars(u, u64, dist, what, u32=None):
+    """Per-pixel relative error (to max(|u|, 1)) against FP64 over the stable
+    pixels: p99 <= 1e-4, and, when the FP32 oracle's result is given, p99 and
+    p99.9 no more than 1.5x the FP32 oracle's own (the reference's float
+    arithmetic run the same way).  Every quantile is printed."""
+    e = rel_err(u, u64)
+    stable = dist > 0
+    q = _quantiles(e[stable])
+    msg = (f"{what}: vs FP64 over the {int(stable.sum())} stable pixels " +
+           " ".join(f"{k} {v:.2e}" for k, v in q.items()) +
+           f"; {int(np.sum(~stable))} unstable pixels: max {e[~stable].max() if (~stable).any() else 0:.2e}")
+    q32 = None
+    if u32 is not None:
+        q32 = _quantiles(rel_err(u32, u64)[stable])
+        msg += "; FP32 oracle vs FP64: " + " ".join(f"{k} {v:.2e}" for k, v in q32.items())
+    print(msg, flush=True)
+    assert q["p99"] <= 1e-4, msg
+    if q32 is not None:
+        for k in ("p99", "p99.9"):
+            assert q[k] <= 1.5 * q32[k] + 1e-6, msg
+    return e
+
+
+# ---- sweep ---------------------------------------------------------------------------
+
+SWEEPS = {"640x480_4src_64planes_w7": dict(N=4, P=64, znear=0.5, zfar=5.0, win=7),
+          "640x480_8src_256planes_w9": dict(N=8, P=256, znear=0.3, zfar=8.0, win=9)}
+
+
+@pytest.mark.parametrize("name", list(SWEEPS))
+def test_sweep_full_size_vs_oracle(views640, name):
+    from psw_amd import stages
+    c = SWEEPS[name]
+    v = _sub(views640, c["N"])
+    planes = stages.inverse_depth_planes(c["znear"], c["zfar"], c["P"])
+    args = (v["ref"], v["src"], v["K"], v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], planes, c["win"], STD_T)
+    _case("sweep", workload=name)
+    t0 = time.perf_counter()
+    sw = stages.SweepAggregate(*args)
+    sw.build_homographies()
+    got = {}
+    for fast in (False, True):
+        sw.run(fast=fast)
+        got[fast] = sw.results()
+    t_gpu = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    a32, c32, s32, vol32 = oracle.sweep_agg_volume(*args, prec="f32")
+    t32 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    a64, c64, s64, vol = oracle.sweep_agg_volume(*args)
+    t64 = time.perf_counter() - t0
+    print(f"GPU {t_gpu:.2f} s (both variants), FP32 oracle {t32:.1f} s, FP64 oracle {t64:.1f} s", flush=True)
+    for k, (nm, e) in enumerate(zip(("argmin", "min cost", "second cost"), (a32, c32, s32))):
+        np.testing.assert_array_equal(got[False][k], e, err_msg=f"reference-order {nm} vs FP32 oracle")
+    yy, xx = np.mgrid[0:a64.shape[0], 0:a64.shape[1]]
+    err32 = np.abs(vol32.astype(np.float64) - vol)      # the reference arithmetic's cost error, every plane
+    for fast in (False, True):
+        arg, cmin, _ = got[fast]
+        dis = arg != a64
+        agree = 1.0 - float(np.mean(dis))
+        cerr = np.abs(cmin.astype(np.float64) - vol[arg, yy, xx])   # the variant's cost error at its plane
+        gap = (vol[arg, yy, xx] - vol[a64, yy, xx])[dis]
+        noise = err32[arg, yy, xx][dis] + err32[a64, yy, xx][dis] + (cerr[dis] if fast else 0.0)
+        q = _quantiles(cerr)
+        print(f"{'fast' if fast else 'reference-order'}: argmin agreement vs FP64 {agree:.6f} "
+              f"({int(dis.sum())} of {arg.size}); cost error vs FP64 at the chosen plane "
+              + " ".join(f"{k} {v:.2e}" for k, v in q.items())
+              + f"; at the disagreements: FP64 cost gap max {gap.max() if dis.any() else 0:.2e}, median "
+              f"{np.median(gap) if dis.any() else 0:.2e}; gap / float cost errors at the two planes max "
+              f"{(gap / np.maximum(noise, 1e-12)).max() if dis.any() else 0:.2f}", flush=True)
+        assert agree >= 0.999
+        # Every disagreement is a near-tie: the FP64 costs of the two planes
+        # differ by no more than the float errors of the costs there, so float
+        # arithmetic cannot order them.  For the reference-order variant (the
+        # reference's own float arithmetic, bit-identical to the FP32 oracle)
+        # this follows from its argmin: gap <= err32(its plane) + err32(FP64's
+        # plane).  For the fast variant it checks that its errors stay within
+        # the reference arithmetic's at the FP64 plane plus its own at its plane.
+        assert np.all(gap <= noise + 1e-6)
+
+
+# ---- TGV2 ----------------------------------------------------------------------------
+
+def test_tgv_cold_start_200_iterations_4src(views640):
+    """TGV2 from the reference's constant start u = 1, linearised there, 200
+    iterations (one warp) on the 4-source views, vs the FP64 oracle."""
+    from psw_amd import stages
+    v = _sub(views640, 4)
+    K1 = one_based(v["K"])
+    prm = stages.tgv_params(**TGV_KW)
+    pose = (v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"])
+    _case("tgv cold start", iters=200, N=4)
+    u, st = stages.tgv_run(*pose, 200, 1, prm, state=True)
+    t0 = time.perf_counter()
+    u64 = oracle.tgv(*pose, 200, 1, prec="f64", **TGV_KW)
+    u32 = oracle.tgv(*pose, 200, 1, **TGV_KW)
+    print(f"oracle FP64 + FP32: {time.perf_counter() - t0:.1f} s; FP32 oracle vs FP64 max "
+          f"{rel_err(u32, u64).max():.2e}", flush=True)
+    _, _, dist = _unstable(st["Iu"])
+    _tgv_bars(u, u64, dist, "cold start, 200 iterations", u32=u32)
+
+
+class _Problem:
+    """The 8-source problem linearised at the GPU's fast-sweep depth: the
+    GPU solver with its E*, and the FP64 oracle's linearisation of the same
+    depth."""
+
+    def __init__(self, seed):
+        import bench
+        from psw_amd import stages, warmstart
+        self.pl = bench.Pipeline(seed, warmstart.DEFAULT_WEIGHTS)
+        pl = self.pl
+        s = pl.st.handle
+        v = dict(pl.v, ref=pl.v["d_ref"].to_host(), src=pl.v["d_src"].to_host())
+        self.d_s = pl.sweep_and_linearise(True)
+        self.relinearise()
+        pl.tgv.iterate(ESTAR_ITERS, pl.prm, s)
+        self.e_star = pl.tgv.energy(pl.prm, s)
+        self.S = pl.tgv.get_state(s)
+        pl.st.synchronize()
+        self.u0 = self.d_s.to_host()
+        K1 = one_based(v["K"])
+        self.It, self.Iu, self.T4 = oracle.tgv_linearise(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"],
+                                                         v["tsrc"], self.u0, prec="f64")
+        self.lin32 = oracle.tgv_linearise(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"],
+                                          self.u0, prec="f32")
+        self.bad, self.weight, self.dist = _unstable(self.S["Iu"])
+        self.gpu_weight = pl.tgv.get_energy_mask(s)
+        self.stages = stages
+
+    def relinearise(self):
+        pl = self.pl
+        s = pl.st.handle
+        pl.tgv.set_u(self.d_s, stream=s)
+        pl.tgv.linearise(pl.K1, pl.Rrel, pl.trel, s)
+        pl.tgv.set_energy_mask(QMAX, pl.prm, s, count=False)
+
+    def energy_own(self, S):
+        """The oracle's FP64 energy of a GPU state in the GPU's own (float)
+        linearisation, over the same mask: what the solver's reduction sums."""
+        T4 = np.stack([S["T11"], S["T12"], S["T12"], S["T22"]])
+        return oracle.tgv_energy(S["u"], S["u1x"], S["u1y"], S["u0"], T4, S["It"], S["Iu"], TGV_KW["lambda_"],
+                                 TGV_KW["alpha0"], TGV_KW["alpha1"], weight=self.weight)
+
+    def energy64(self, st):
+        return oracle.tgv_energy(st["u"], st["u1x"], st["u1y"], self.u0.astype(np.float64), self.T4, self.It,
+                                 self.Iu, TGV_KW["lambda_"], TGV_KW["alpha0"], TGV_KW["alpha1"], weight=self.weight)
+
+    def iterate64(self, state, n):
+        return oracle.tgv_iterate(state, self.T4, self.u0, self.It, self.Iu, n, prec="f64",
+                                  **{k: TGV_KW[k] for k in ("lambda_", "alpha0", "alpha1", "tau", "sigma")})
+
+    def iterate32(self, state, n):
+        It, Iu, T4 = self.lin32
+        return oracle.tgv_iterate(state, T4, self.u0, It, Iu, n, prec="f32",
+                                  **{k: TGV_KW[k] for k in ("lambda_", "alpha0", "alpha1", "tau", "sigma")})
+
+
+@pytest.fixture(scope="module")
+def problem640(gpu):
+    return _Problem(SEED)
+
+
+def test_tgv_estar_is_the_fp64_fixed_point_energy(problem640):
+    """E* (the GPU's FP64 energy after 40000 FP32 iterations) against the
+    FP64 oracle: the oracle's energy of the GPU's state, and the energy
+    after 2000 more FP64 iterations from that state (the FP64 iteration's
+    fixed point), both within 1e-5 of E* -- a tenth of the stop tolerance."""
+    P = problem640
+    st = {k: P.S[k] for k in ("u", "u1x", "u1y", "px", "py", "qx", "qy", "qz", "r")}
+    e_gpu_state = P.energy64(st)
+    st64 = P.iterate64(st, 2000)
+    e64 = P.energy64(st64)
+    print(f"E* {P.e_star:.9e}; FP64 oracle energy of the GPU state {e_gpu_state:.9e} "
+          f"({(e_gpu_state - P.e_star) / P.e_star:+.2e}); after 2000 FP64 iterations {e64:.9e} "
+          f"({(e64 - P.e_star) / P.e_star:+.2e}); {int(P.bad.sum())} unstable pixels, "
+          f"{int(np.sum(P.weight == 0))} left out of the energy", flush=True)
+    # the solver's energy mask is the stated rule applied to its own Iu
+    np.testing.assert_array_equal(P.gpu_weight, P.weight)
+    assert abs(e_gpu_state - P.e_star) <= 1e-6 * P.e_star
+    assert abs(e64 - P.e_star) <= 1e-5 * P.e_star
+
+
+@pytest.mark.parametrize("start", ["plain", "learned"])
+def test_tgv_solve_to_gap_vs_fp64(problem640, start):
+    """The solve from the plain start (u = u0, the rest 0) and from the
+    learned start stops within the gap; the stop state against the FP64
+    oracle run from the same start for the same number of iterations, and
+    the oracle's energy there within the gap too."""
+    P = problem640
+    pl = P.pl
+    s = pl.st.handle
+    _case("tgv solve to gap", start=start)
+    P.relinearise()
+    if start == "learned":
+        pl.ws.predict(pl.sweep, 8.0, s)
+        pl.ws.start(pl.tgv, s)
+        st0 = {"u": pl.ws.d_u.to_host(s)}
+        u1, p, q = pl.ws.d_u1.to_host(s), pl.ws.d_p.to_host(s), pl.ws.d_q.to_host(s)
+        st0.update(u1x=u1[0], u1y=u1[1], px=p[0], py=p[1], qx=q[0], qy=q[1], qz=q[2])
+    else:
+        st0 = {"u": P.u0}
+    it, e = pl.tgv.solve(50000, 10, GAP_TOL, P.e_star, pl.prm, s)
+    S = pl.tgv.get_state(s)
+    pl.st.synchronize()
+    t0 = time.perf_counter()
+    st64 = P.iterate64({k: np.asarray(a, np.float64) for k, a in st0.items()}, it)
+    e64 = P.energy64(st64)
+    st32 = P.iterate32(st0, it)
+    e_gpu64 = P.energy_own(S)
+    print(f"{start}: stopped after {it} iterations, gap {(e - P.e_star) / P.e_star:+.2e}; FP64 oracle "
+          f"energy there {(e64 - P.e_star) / P.e_star:+.2e}, oracle sum of the GPU state's energy "
+          f"{(e_gpu64 - P.e_star) / P.e_star:+.2e} (FP64 and FP32 oracle runs {time.perf_counter() - t0:.1f} s)",
+          flush=True)
+    assert abs(e - P.e_star) <= GAP_TOL * P.e_star
+    assert abs(e64 - P.e_star) <= GAP_TOL * P.e_star
+    assert abs(e_gpu64 - e) <= 1e-10 * P.e_star         # the GPU's FP64 reduction of its own state
+    assert abs(e - e64) <= 1e-5 * P.e_star               # the stop decision: GPU vs FP64 run, a tenth of the gap
+    _tgv_bars(S["u"], st64["u"], P.dist, f"{start} start, stop state", u32=st32["u"])
+
+
+# ---- fusion ----------------------------------------------------------------------------
+
+FUSION_VOLUME = np.array([-3.0, -1.6, -4.0, 3.0, 1.6, 6.0], np.float32)
+
+
+def test_fusion_256cube_50_depthmaps_vs_oracle(gpu):
+    """50 orbiting 640x480 depthmaps (1 cm noise) into 256^3 voxels, band
+    +-5 voxels: histograms bit-identical to the FP32 oracle; 300 PD
+    iterations (tau 0.1, lambda 0.5, sigma 1) within 1e-4 of the FP64
+    oracle fed the same histograms; the surface voxel count (-1 < u < 0)
+    against the oracle's count of its own u."""
+    from psw_amd import scene, stages
+    from psw_amd.device import DeviceArray
+    g, maps, iters = 256, 50, 300
+    o = scene.orbit_depthmaps_gpu(seed=SEED, n=maps, width=640, height=480, noise_sigma=0.01)
+    thr = 5.0 * float(np.mean((FUSION_VOLUME[3:] - FUSION_VOLUME[:3]) / g))
+    _case("fusion", grid=g, maps=maps, iters=iters)
+    f = stages.Fusion(g, g, g, FUSION_VOLUME)
+    f.update_histogram_batch(DeviceArray.from_host(o["depths"]), maps, 640, 480, np.repeat(o["K"][None], maps, 0),
+                             o["R"], o["t"], thr)
+    hist_gpu = f.download()["hist"]
+    t0 = time.perf_counter()
+    hist = np.zeros((g, g, g, 8), np.uint16)
+    for k in range(maps):
+        oracle.fusion_hist(hist, FUSION_VOLUME, o["depths"][k], o["K"], o["R"][k], o["t"][k], thr)
+    print(f"histograms: {int(hist.sum())} votes, FP32 oracle {time.perf_counter() - t0:.1f} s", flush=True)
+    np.testing.assert_array_equal(hist_gpu, hist)
+    f.solve(iters, 0.1, 0.5, 1.0)
+    got = f.download()
+    n_gpu = f.surface_count()
+    t0 = time.perf_counter()
+    u64, v64, p64 = np.zeros((g, g, g)), np.zeros((g, g, g)), np.zeros((3, g, g, g))
+    for _ in range(iters):
+        oracle.fusion_u(u64, v64, p64, hist, 0.1, 0.5, prec="f64")
+        oracle.fusion_p(p64, v64, 1.0, prec="f64")
+    n64 = oracle.surface_count(u64)
+    e = rel_err(got["u"], u64)
+    print(f"u vs FP64 (relative to max(|u|, 1)): max {e.max():.2e}, mean {e.mean():.2e}; surface voxels GPU {n_gpu}, "
+          f"FP64 oracle {n64}; FP64 oracle {time.perf_counter() - t0:.1f} s", flush=True)
+    assert e.max() <= 1e-4
+    # a voxel can sit within rounding of the -1 / 0 boundaries: the counts may differ only there
+    near = np.sum((np.abs(u64 + 1.0) <= 1e-4) | (np.abs(u64) <= 1e-4))
+    assert abs(n_gpu - n64) <= near, (n_gpu, n64, near)
+
+
+# ---- the uneven batch ---------------------------------------------------------------------
+
+def test_uneven_batch_32_frames(gpu):
+    """32 frames of 320x240 / 640x480 / 1242x375 with 4-8 sources and 64-256
+    planes: the batched sweep and the batched TGV2 solve (plain start, gap
+    1e-4 of each frame's E* from 20000 batched iterations) against the
+    single-frame entry points, frame by frame and bit for bit; then one
+    frame of each resolution against the oracle (reference-order sweep
+    bit-identical to the FP32 oracle, 100 TGV2 iterations from the plain
+    start against the FP64 oracle)."""
+    from psw_amd import batch, stages
+    from psw_amd.device import Stream
+    st = Stream()
+    s = st.handle
+    prm = stages.tgv_params(**TGV_KW)
+    specs = batch.uneven_workload(SEED, 32)
+    frames = [batch.Frame(sp) for sp in specs]
+    _case("uneven batch", frames=len(frames), shapes=sorted({(sp["W"], sp["H"]) for sp in specs}))
+    sweeps = [f.sweep for f in frames]
+    for sw in sweeps:
+        sw.build_homographies(s)
+    batch.sweep_batch(sweeps, True, 0, 0, s)
+    batched = [sw.results(s) for sw in sweeps]
+    for f in frames:
+        f.linearise(prm, QMAX, s)
+    batch.tgv_iterate_batch([f.tgv for f in frames], 20000, prm, -1, s)
+    e_star = [f.tgv.energy(prm, s) for f in frames]
+    for f in frames:
+        f.linearise(prm, QMAX, s)
+    iters, energies = batch.tgv_solve_batch([f.tgv for f in frames], prm, e_star, 50000, 10, GAP_TOL, stream=s)
+    states = [f.tgv.get_state(s)["u"] for f in frames]
+    st.synchronize()
+    print(f"batched stops: {list(iters)}", flush=True)
+    for i, f in enumerate(frames):
+        f.sweep.run(fast=True, stream=s)
+        for a, b in zip(f.sweep.results(s), batched[i]):
+            np.testing.assert_array_equal(a, b, err_msg=f"frame {i} sweep")
+        f.linearise(prm, QMAX, s)
+        it, e = f.tgv.solve(50000, 10, GAP_TOL, e_star[i], prm, s)
+        assert it == iters[i], (i, it, iters[i])
+        np.testing.assert_array_equal(f.tgv.get_state(s)["u"], states[i], err_msg=f"frame {i} TGV2 stop state")
+        assert abs(energies[i] - e_star[i]) <= GAP_TOL * e_star[i]
+    # one frame of each resolution against the oracle
+    seen = set()
+    for i, (f, sp) in enumerate(zip(frames, specs)):
+        if (sp["W"], sp["H"]) in seen:
+            continue
+        seen.add((sp["W"], sp["H"]))
+        v = f.v
+        ref, src = v["d_ref"].to_host(), v["d_src"].to_host()
+        args = (ref, src, f.K0, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], f.planes, 9, STD_T)
+        t0 = time.perf_counter()
+        f.sweep.run(fast=False, stream=s)
+        got = f.sweep.results(s)
+        exp = oracle.sweep_agg(*args)
+        for a, b in zip(got, exp):
+            np.testing.assert_array_equal(a, b, err_msg=f"frame {i} ({sp}) reference-order sweep vs FP32 oracle")
+        f.sweep.run(fast=True, stream=s)
+        f.linearise(prm, QMAX, s)
+        u0 = f.sweep.d_depth.to_host(s)
+        f.tgv.iterate(100, prm, s)
+        S = f.tgv.get_state(s)
+        It, Iu, T4 = oracle.tgv_linearise(ref, src, f.K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], u0, prec="f64")
+        st64 = oracle.tgv_iterate({"u": u0.astype(np.float64)}, T4, u0, It, Iu, 100, prec="f64",
+                                  **{k: TGV_KW[k] for k in ("lambda_", "alpha0", "alpha1", "tau", "sigma")})
+        _, _, dist = _unstable(S["Iu"])
+        print(f"frame {i} {sp}: oracle {time.perf_counter() - t0:.1f} s", flush=True)
+        _tgv_bars(S["u"], st64["u"], dist, f"frame {i} {sp['W']}x{sp['H']}, 100 iterations")
