@@ -22,7 +22,11 @@
 // With FAST=false every sum, product and quotient is taken in the reference's
 // order (column mean = sum/win, row mean of column means = sum/win), so the
 // NCC is bit-identical to the reference's float arithmetic.  FAST=true
-// multiplies by 1/win and lets the compiler fuse multiply-adds.
+// multiplies by 1/win, keeps running window sums down each column segment
+// (TY/4 rows) and along each row's pixels, and uses explicit fused
+// multiply-adds and the hardware reciprocal: its results are the same in
+// every launch of one tile height (single frame, batched, any plane groups),
+// and differ in the last bits between tile heights.
 #include "psw_common.h"
 
 #include <vector>
@@ -172,13 +176,15 @@ __device__ __forceinline__ void sweep_item(const SweepArgs& a, const int item) {
                 const float px = (float)(wgx[k] + a.off), py = (float)(wgy[k] + a.off);
                 float xs, ys;
                 if constexpr (FAST) {
-#pragma clang fp contract(fast)
-                    const float vx = h0 * px + h1 * py + h2;
-                    const float vy = h3 * px + h4 * py + h5;
-                    const float vz = h6 * px + h7 * py + h8;
+                    // explicit fused multiply-adds (no contraction left to the
+                    // compiler), so every kernel that inlines this computes
+                    // the same bits
+                    const float vx = __builtin_fmaf(h0, px, __builtin_fmaf(h1, py, h2));
+                    const float vy = __builtin_fmaf(h3, px, __builtin_fmaf(h4, py, h5));
+                    const float vz = __builtin_fmaf(h6, px, __builtin_fmaf(h7, py, h8));
                     const float rz = __builtin_amdgcn_rcpf(vz);
-                    xs = vx * rz - offf;
-                    ys = vy * rz - offf;
+                    xs = __builtin_fmaf(vx, rz, -offf);
+                    ys = __builtin_fmaf(vy, rz, -offf);
                 } else {
                     const float vx = dot3(h0, h1, h2, px, py, 1.f);
                     const float vy = dot3(h3, h4, h5, px, py, 1.f);
@@ -195,22 +201,21 @@ __device__ __forceinline__ void sweep_item(const SweepArgs& a, const int item) {
 #pragma unroll
             for (int i = 0; i < CV; ++i) wv[i] = sW[(cg * SW_CR + i) * TW + ccx];
             if constexpr (FAST) {
-#pragma clang fp contract(fast)
                 // running window sums down the column
                 float s1 = 0.f, s2 = 0.f, s3 = 0.f;
 #pragma unroll
                 for (int i = 0; i < WIN; ++i) {
                     s1 += wv[i];
-                    s2 += wv[i] * wv[i];
-                    s3 += rv[i] * wv[i];
+                    s2 = __builtin_fmaf(wv[i], wv[i], s2);
+                    s3 = __builtin_fmaf(rv[i], wv[i], s3);
                 }
 #pragma unroll
                 for (int j = 0; j < SW_CR; ++j) {
                     if (j > 0) {
                         const float wa = wv[j + 2 * R], wd = wv[j - 1];
                         s1 += wa - wd;
-                        s2 += wa * wa - wd * wd;
-                        s3 += rv[j + 2 * R] * wa - rv[j - 1] * wd;
+                        s2 = __builtin_fmaf(wa, wa, __builtin_fmaf(-wd, wd, s2));
+                        s3 = __builtin_fmaf(rv[j + 2 * R], wa, __builtin_fmaf(-rv[j - 1], wd, s3));
                     }
                     sC[0][(cg * SW_CR + j) * TW + ccx] = s1 * rwin;
                     sC[1][(cg * SW_CR + j) * TW + ccx] = s2 * rwin;
@@ -242,7 +247,6 @@ __device__ __forceinline__ void sweep_item(const SweepArgs& a, const int item) {
             c2[i] = sC[2][roy * TW + rox + i];
         }
         if constexpr (FAST) {
-#pragma clang fp contract(fast)
             float m = 0.f, q = 0.f, pm = 0.f;
 #pragma unroll
             for (int k = 0; k < SW_PX; ++k) {
@@ -259,10 +263,10 @@ __device__ __forceinline__ void sweep_item(const SweepArgs& a, const int item) {
                     pm += c2[k + 2 * R] - c2[k - 1];
                 }
                 const float mm = m * rwin, qq = q * rwin, pp = pm * rwin;
-                const float var = qq - mm * mm;
+                const float var = __builtin_fmaf(-mm, mm, qq);
                 const float sd = var > 0.f ? __builtin_amdgcn_sqrtf(var) : 0.f;
                 if ((rstd[k] < a.stdthresh) || (sd < a.stdthresh)) ncc[k] = 0.f;
-                else ncc[k] = (pp - rmean[k] * mm) * __builtin_amdgcn_rcpf(rstd[k] * sd);
+                else ncc[k] = __builtin_fmaf(-rmean[k], mm, pp) * __builtin_amdgcn_rcpf(rstd[k] * sd);
             }
         } else {
 #pragma unroll

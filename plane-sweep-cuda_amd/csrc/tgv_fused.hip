@@ -561,18 +561,17 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile,
 #pragma unroll
         for (int f = 0; f < 3; ++f) sQ[f][iy][ix] = a.qcur[f * n + g];
     }
-    // A: ubar_k; C: u_k
+    // A: ubar_k; its inner part C (one pixel in from each side) also keeps u_k
     for (int i = tid; i < AX * AY; i += NT) {
         const int iy = i / AX, ix = i - iy * AX;
         const size_t g = (size_t)clampi(y0 - 2 + iy, H - 1) * W + clampi(x0 - 2 + ix, W - 1);
+        const bool inC = ix >= 1 && ix <= CX && iy >= 1 && iy <= CY;
 #pragma unroll
-        for (int f = 0; f < 3; ++f) sA[f][iy][ix] = 2.f * a.ucur[f * n + g] - a.uprev[f * n + g];
-    }
-    for (int i = tid; i < CX * CY; i += NT) {
-        const int iy = i / CX, ix = i - iy * CX;
-        const size_t g = (size_t)clampi(y0 - 1 + iy, H - 1) * W + clampi(x0 - 1 + ix, W - 1);
-#pragma unroll
-        for (int f = 0; f < 3; ++f) sUk[f][iy][ix] = a.ucur[f * n + g];
+        for (int f = 0; f < 3; ++f) {
+            const float uk = a.ucur[f * n + g];
+            sA[f][iy][ix] = 2.f * uk - a.uprev[f * n + g];
+            if (inC) sUk[f][iy - 1][ix - 1] = uk;
+        }
     }
     // this thread's tile pixel (clamped coordinates outside the image: computed, never stored)
     const bool tp = NT == TP || tid < TP;
@@ -580,7 +579,7 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile,
     const int x = x0 + tx, y = y0 + ty;
     const size_t gt = (size_t)clampi(y, H - 1) * W + clampi(x, W - 1);
     float rv[NS], itv[NS], iuv[NS];
-    float u0t = 0.f, t11 = 0.f, t12 = 0.f, t22 = 0.f;
+    float u0t = 0.f;
     if (tp) {
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
@@ -592,9 +591,13 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile,
             }
         }
         u0t = a.u0[gt];
-        t11 = a.T[gt]; t12 = a.T[n + gt]; t22 = a.T[2 * n + gt];
     }
     __syncthreads();
+    // the tile pixel's tensor, from the B region staged above (B index = tile index + 2)
+    float t11 = 0.f, t12 = 0.f, t22 = 0.f;
+    if (tp) {
+        t11 = sT[0][ty + 2][tx + 2]; t12 = sT[1][ty + 2][tx + 2]; t22 = sT[2][ty + 2][tx + 2];
+    }
 
     // B: p, q of iteration k+1
     for (int i = tid; i < BX * BY; i += NT) {
