@@ -267,6 +267,24 @@ int psw_sweep_fused_aggregate(const float* d_ref, const float* d_src, int width,
 int psw_sweep_argmin_depth(float* d_depth, const int32_t* d_argmin, const float* d_depths, int npixels,
                            float fallback, psw_stream_t s);
 
+/* psw_sweep_fused_aggregate without any device allocation.  No counterpart
+ * in the reference: for callers that cannot allocate device memory (code
+ * under hipGraph capture, tight serving loops).  The plane-group partials go
+ * to a static device array of PSW_SWEEP_SCRATCH_BYTES (32 MiB, held by every
+ * GPU the library runs on until the process ends) instead of a per-call
+ * allocation; outputs are bit-identical to psw_sweep_fused_aggregate.
+ * groups: plane groups (0 = the library's choice, capped to what the scratch
+ * holds; a count whose partials, groups * width * height * 12 bytes, exceed
+ * the scratch is rejected with PSW_ERR_ARG; 1 needs no scratch).  Every call
+ * on one GPU uses that GPU's one array: calls that overlap in time on the
+ * same GPU (different streams or threads) race.  Each call writes every
+ * partial it reads, so nothing needs clearing between calls. */
+#define PSW_SWEEP_SCRATCH_BYTES (32u << 20)
+int psw_sweep_fused_aggregate_noalloc(const float* d_ref, const float* d_src, int width, int height, int nsrc,
+                                      const float* d_H, int nplanes, int winsize, float stdthresh, int coord_offset,
+                                      int fast, int groups, int32_t* d_argmin, float* d_cost_min,
+                                      float* d_cost_second, psw_stream_t s);
+
 /* Many frames of uneven size, source and plane counts in one persistent
  * launch (no reference counterpart): workgroups take (tile, plane group)
  * items of all frames from a device counter.  Each frame's results are

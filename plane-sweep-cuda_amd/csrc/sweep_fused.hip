@@ -29,6 +29,7 @@
 // and differ in the last bits between tile heights.
 #include "psw_common.h"
 
+#include <mutex>
 #include <vector>
 
 namespace psw {
@@ -582,6 +583,51 @@ static int check_sweep_args(const float* d_ref, const float* d_src, int width, i
     return PSW_OK;
 }
 
+// Device memory of psw_sweep_fused_aggregate_noalloc: the plane-group
+// partials (12 bytes per pixel and group) live in this static array, one
+// copy per GPU for the life of the process, instead of a per-call
+// allocation.  Host code never uses the name itself (on the host it is a
+// stand-in, not a device address): scratch_address() looks up this GPU's
+// copy once with hipGetSymbolAddress and keeps it per device.
+static __device__ __align__(256) unsigned char scratch[PSW_SWEEP_SCRATCH_BYTES];
+
+constexpr int SW_MAX_DEVICES = 64;
+
+static int scratch_address(void** out) {
+    static std::mutex mu;
+    static void* addr[SW_MAX_DEVICES] = {};
+    int dev = 0;
+    PSW_TRY_HIP(hipGetDevice(&dev), PSW_ERR_DEVICE);
+    PSW_CHECK_ARG(dev >= 0 && dev < SW_MAX_DEVICES, "device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_status(e, "hipGetSymbolAddress(scratch)", PSW_ERR_DEVICE);
+        addr[dev] = p;
+    }
+    *out = addr[dev];
+    return PSW_OK;
+}
+
+// The aggregated sweep's launch and, with plane groups, the merge of the
+// partials in `ws` (G * npix * 12 bytes).
+static int sweep_aggregate_run(SweepArgs& a, int winsize, int fast, void* ws, hipStream_t st) {
+    const size_t npix = (size_t)a.W * a.H;
+    if (a.G > 1) {
+        a.ws_cmin = static_cast<float*>(ws);
+        a.ws_csec = a.ws_cmin + (size_t)a.G * npix;
+        a.ws_arg = reinterpret_cast<int32_t*>(a.ws_csec + (size_t)a.G * npix);
+    }
+    int rc = launch_sweep(a, winsize, 1, fast != 0, st);
+    if (rc == PSW_OK && a.G > 1) {
+        sweep_merge_kernel<<<(unsigned)((npix + 255) / 256), 256, 0, st>>>(a.ws_cmin, a.ws_arg, a.ws_csec, a.G, npix,
+                                                                         a.argmin, a.cmin, a.csec);
+        rc = launch_status("sweep_merge_kernel");
+    }
+    return rc;
+}
+
 __global__ void argmin_depth_kernel(float* out, const int32_t* arg, const float* depths, int n, float fallback) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
@@ -645,21 +691,43 @@ int psw_sweep_fused_aggregate(const float* d_ref, const float* d_src, int width,
     sweep_layout(a, true, sweep_tile_height());
     const size_t npix = (size_t)width * height;
     void* ws = nullptr;
-    if (a.G > 1) {
-        // stream-ordered scratch for the per-group partials (pooled by the runtime)
+    if (a.G > 1)   // stream-ordered scratch for the per-group partials (pooled by the runtime)
         PSW_TRY_HIP(hipMallocAsync(&ws, (size_t)a.G * npix * 12, S(s)), PSW_ERR_ALLOC);
-        a.ws_cmin = static_cast<float*>(ws);
-        a.ws_csec = a.ws_cmin + (size_t)a.G * npix;
-        a.ws_arg = reinterpret_cast<int32_t*>(a.ws_csec + (size_t)a.G * npix);
-    }
-    st = launch_sweep(a, winsize, 1, fast != 0, S(s));
-    if (st == PSW_OK && a.G > 1) {
-        sweep_merge_kernel<<<(unsigned)((npix + 255) / 256), 256, 0, S(s)>>>(a.ws_cmin, a.ws_arg, a.ws_csec, a.G, npix,
-                                                                           d_argmin, d_cost_min, d_cost_second);
-        st = launch_status("sweep_merge_kernel");
-    }
+    st = sweep_aggregate_run(a, winsize, fast, ws, S(s));
     if (ws) (void)hipFreeAsync(ws, S(s));
     return st;
+}
+
+int psw_sweep_fused_aggregate_noalloc(const float* d_ref, const float* d_src, int width, int height, int nsrc,
+                                      const float* d_H, int nplanes, int winsize, float stdthresh, int coord_offset,
+                                      int fast, int groups, int32_t* d_argmin, float* d_cost_min,
+                                      float* d_cost_second, psw_stream_t s) {
+    int st = check_sweep_args(d_ref, d_src, width, height, nsrc, d_H, nplanes, winsize);
+    if (st != PSW_OK) return st;
+    PSW_CHECK_ARG(d_argmin && d_cost_min && d_cost_second, "NULL pointer");
+    PSW_CHECK_ARG(coord_offset == 0 || coord_offset == 1, "coord_offset must be 0 or 1");
+    PSW_CHECK_ARG(groups >= 0, "groups must not be negative");
+    SweepArgs a = {};
+    a.ref = d_ref; a.src = d_src; a.Hs = d_H;
+    a.W = width; a.H = height; a.N = nsrc; a.P = nplanes; a.off = coord_offset;
+    a.stdthresh = stdthresh;
+    a.argmin = d_argmin; a.cmin = d_cost_min; a.csec = d_cost_second;
+    sweep_layout(a, true, sweep_tile_height());
+    const size_t per_group = (size_t)width * height * 12;
+    if (groups > 0) {
+        a.G = std::min(groups, std::min(a.P, 64));
+        if (a.G > 1 && (size_t)a.G * per_group > PSW_SWEEP_SCRATCH_BYTES) {
+            set_error("%d plane groups of a %d x %d image need %zu bytes of partials, above the %zu-byte static "
+                      "scratch (fewer groups, or groups = 0 for the most that fit)",
+                      a.G, width, height, (size_t)a.G * per_group, (size_t)PSW_SWEEP_SCRATCH_BYTES);
+            return PSW_ERR_ARG;
+        }
+    } else {
+        a.G = std::max(1, std::min(a.G, (int)(PSW_SWEEP_SCRATCH_BYTES / per_group)));
+    }
+    void* ws = nullptr;
+    if (a.G > 1 && (st = scratch_address(&ws)) != PSW_OK) return st;
+    return sweep_aggregate_run(a, winsize, fast, ws, S(s));
 }
 
 int psw_sweep_fused_aggregate_batch(const psw_sweep_frame* frames, int nframes, int winsize, float stdthresh,

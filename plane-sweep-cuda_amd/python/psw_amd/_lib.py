@@ -142,6 +142,8 @@ _SIGS = {
                                           c_void_p]),
     "psw_sweep_fused_aggregate": (c_int, [D, D, c_int, c_int, c_int, D, c_int, c_int, c_float, c_int, c_int, D, D, D,
                                           c_void_p]),
+    "psw_sweep_fused_aggregate_noalloc": (c_int, [D, D, c_int, c_int, c_int, D, c_int, c_int, c_float, c_int,
+                                                  c_int, c_int, D, D, D, c_void_p]),
     "psw_sweep_argmin_depth": (c_int, [D, D, D, c_int, c_float, c_void_p]),
     # TGV2
     "psw_tgv_create": (c_int, [P(c_void_p), c_int, c_int, c_int]),
