@@ -13,15 +13,19 @@ intensities), pinhole K fx=fy=480, cx=319.5, cy=239.5; poses within 5 cm /
   TGV2  : linearised at the sweep's argmin depth, lambda=1, alpha0=2,
           alpha1=1.5, tau=0.02, sigma=1, iterated until the primal energy is
           within 1e-4 (relative) of E*, the energy of the same problem after
-          40000 iterations from the plain start (computed per scene before
-          the timed region), checked every 10 iterations, capped at 50000.
+          at least 40000 iterations from the plain start, continued in chunks
+          of 10000 until a chunk changes it by less than 1e-6 (computed per
+          scene before the timed region), checked every 10 iterations,
+          capped at 50000.
           The energy leaves out the ~0.2 % of pixels where the reference's
           fixed steps make the data step unstable (tau sigma lambda^2
           sum Iu^2 > 1 at the pixel or a neighbour): the iteration oscillates
           there forever;
   starts: learned = the BF16 U-Net's (u, u1, p, q) from the sweep results,
           kept only if its energy is below the plain start's, else the plain
-          start (the headline value, network time included); plain = the
+          start, and restarted from the plain start if it has not reached the
+          gap after 20000 iterations (the headline value, network time
+          included); plain = the
           reference's warp start u = u0 (the sweep depth), u1 = p = q = r = 0;
           cold = the reference's constant u = 1.
 Each start is timed over the same number of depthmaps, cycling over the
@@ -48,6 +52,8 @@ TGV = dict(lambda_=1.0, alpha0=2.0, alpha1=1.5, tau=0.02, sigma=1.0, beta=0.0, g
 GAP_TOL = 1e-4
 MAX_ITERS = 50000
 ESTAR_ITERS = 40000
+ESTAR_CAP = 200000
+FALLBACK_AFTER = 20000   # a learned solve not at the gap by then restarts from the plain start
 QMAX = 1.0   # energy mask: pixels with tau sigma lambda^2 sum Iu^2 > QMAX (or at a 4-neighbour) left out
 BF16_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
@@ -311,7 +317,7 @@ class Pipeline:
         self.last_start = mode
         if mode == "learned":
             iters, e, self.last_start = self.ws.solve(self.tgv, d_s, self.prm, e_star, GAP_TOL, max_iters, check_every,
-                                                      stream=s)
+                                                      fallback_after=FALLBACK_AFTER, stream=s)
         else:
             if mode == "cold":
                 self.tgv.set_primal(None, 1.0, None, s)
@@ -356,6 +362,35 @@ def probe(names, seed):
             else:
                 for _ in range(3):
                     pl.sweep_and_linearise(fast=(name == "sweep"))
+        elif name == "sweep_shapes":
+            # the aggregated sweep at 640x480 x 8 sources x 256 planes x 9x9 for
+            # each tile height and plane-group count, both arithmetic variants
+            if pl is None:
+                pl = Pipeline(seed, None)
+            pl.sweep.build_homographies(pl.st.handle)
+            ev = [Event(), Event()]
+            times = {}
+            saved = {k: os.environ.get(k) for k in ("PSW_SWEEP_TY", "PSW_SWEEP_GROUPS")}
+            try:
+                for ty in ("16", "32"):
+                    for g in ("1", "2", "3", "5", "8", "12", "16"):
+                        os.environ["PSW_SWEEP_TY"], os.environ["PSW_SWEEP_GROUPS"] = ty, g
+                        for fast in (True, False):
+                            pl.sweep.run(fast=fast, stream=pl.st.handle)
+                            ev[0].record(pl.st)
+                            for _ in range(5):
+                                pl.sweep.run(fast=fast, stream=pl.st.handle)
+                            ev[1].record(pl.st)
+                            ev[1].synchronize()
+                            times[f"tile32x{ty}_groups{g}_{'fast' if fast else 'reference-order'}_ms"] = \
+                                ev[0].elapsed_ms(ev[1]) / 5
+            finally:
+                for k, val in saved.items():
+                    if val is None:
+                        os.environ.pop(k, None)
+                    else:
+                        os.environ[k] = val
+            log(json.dumps(times))
         elif name == "sparse":
             if pl is None:
                 pl = Pipeline(seed, None)
@@ -484,8 +519,8 @@ def main():
     ap.add_argument("--batch", action="store_true", help="run the uneven 32-frame batch instead (its own JSON line)")
     ap.add_argument("--batch-frames", type=int, default=32)
     ap.add_argument("--batch-estar-iters", type=int, default=20000)
-    ap.add_argument("--probe", default="", help="comma list of tgv,sweep,sweep_exact,unet,copy,fusion,sparse,tvl1: run only "
-                                                "these small workloads (for rocprofv3) and exit")
+    ap.add_argument("--probe", default="", help="comma list of tgv,sweep,sweep_exact,sweep_shapes,unet,copy,fusion,"
+                                                "sparse,tvl1: run only these small workloads (for rocprofv3) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported (a depthmap's sweep, TGV2 and "
@@ -517,18 +552,34 @@ def main():
         pl.tgv.set_u(d_s, stream=s)
         pl.tgv.linearise(pl.K1, pl.Rrel, pl.trel, s)
         pl.excluded = pl.tgv.set_energy_mask(QMAX, pl.prm, s)
-        pl.tgv.iterate(args.estar_iters - 5000, pl.prm, s)
-        e_prev = pl.tgv.energy(pl.prm, s)
-        pl.tgv.iterate(5000, pl.prm, s)
-        pl.e_star = pl.tgv.energy(pl.prm, s)
-        pl.estar_change = (e_prev - pl.e_star) / pl.e_star
+        # iterate until the energy settles: at least estar_iters, then chunks
+        # of 10000 until a chunk changes it by less than 1e-6 (cap 200000)
+        pl.tgv.iterate(args.estar_iters, pl.prm, s)
+        e_prev, n_it = pl.tgv.energy(pl.prm, s), args.estar_iters
+        while True:
+            pl.tgv.iterate(10000, pl.prm, s)
+            n_it += 10000
+            e = pl.tgv.energy(pl.prm, s)
+            change, e_prev = (e_prev - e) / e, e
+            if abs(change) < 1e-6 or n_it >= ESTAR_CAP:
+                break
+        pl.e_star, pl.estar_iters, pl.estar_change = e, n_it, change
         pl.u_star = pl.tgv.get_u(s)
         pl.weight = pl.tgv.get_energy_mask(s)
         pl.d_s_host = d_s.to_host(s)
         pl.true_depth = pl.v["d_depth"].to_host(s)
+        # the learned start iterated as long: does it settle at the same energy?
+        pl.tgv.set_u(d_s, stream=s)
+        pl.tgv.linearise(pl.K1, pl.Rrel, pl.trel, s)
+        pl.tgv.set_energy_mask(QMAX, pl.prm, s, count=False)
+        pl.ws.predict(pl.sweep, ZFAR, s)
+        pl.ws.start(pl.tgv, s)
+        pl.tgv.iterate(n_it, pl.prm, s)
+        pl.e_star_learned = pl.tgv.energy(pl.prm, s)
         st.synchronize()
-        log(f"seed {sd}: E* = {pl.e_star:.9e} after {args.estar_iters} iterations (change over the last 5000: "
-            f"{pl.estar_change:.2e}); {pl.excluded} pixels outside the energy mask")
+        log(f"seed {sd}: E* = {pl.e_star:.9e} after {n_it} iterations (change over the last 10000: {change:.2e}); "
+            f"from the learned start after as many: {(pl.e_star_learned - pl.e_star) / pl.e_star:+.2e} relative; "
+            f"{pl.excluded} pixels outside the energy mask")
         pipes.append(pl)
 
     def timed(mode, k, warm):
@@ -562,7 +613,9 @@ def main():
     per_seed = []
     for j, pl in enumerate(pipes):
         s = pl.st.handle
-        rec = {"seed": pl.seed, "e_star": pl.e_star, "estar_change_last_5000": pl.estar_change,
+        rec = {"seed": pl.seed, "e_star": pl.e_star, "estar_iters": pl.estar_iters,
+               "estar_change_last_10000": pl.estar_change,
+               "learned_start_energy_after_as_many_rel": (pl.e_star_learned - pl.e_star) / pl.e_star,
                "excluded_pixels": pl.excluded}
         u_stop = {}
         for mode, (_, recs) in res.items():
@@ -653,7 +706,8 @@ def main():
         "config": {"workload": "learned warm-start pipeline: 640x480 reference + 8 sources, 256 inverse-depth planes "
                                "0.3-8 m, 9x9 NCC, fused sweep+argmin, BF16 U-Net warm start (plain start as "
                                "fallback), TGV2 linearised at the sweep depth to relative energy gap 1e-4 of E* "
-                               f"({args.estar_iters}-iteration plain run); steps cycle over {len(seeds)} scenes",
+                               f"(plain run of >= {args.estar_iters} iterations until a 10000-iteration chunk "
+                               f"changes it by < 1e-6); steps cycle over {len(seeds)} scenes",
                    "sweep_arith": "fast" if fast else "reference-order", "tgv_gap_tol": GAP_TOL,
                    "check_every": args.check_every, "max_iters": args.max_iters, "estar_iters": args.estar_iters,
                    "seeds": seeds, "network_dtype": "bf16 (fp32 accumulation)"},
