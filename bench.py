@@ -12,12 +12,14 @@ intensities), pinhole K fx=fy=480, cx=319.5, cy=239.5; poses within 5 cm /
           argmin / min / second-best (no cost volume in HBM);
   TGV2  : linearised at the sweep's argmin depth, lambda=1, alpha0=2,
           alpha1=1.5, tau=0.02, sigma=1, iterated until the primal energy is
-          within 1e-4 (relative) of E*, the energy of the same problem after
-          at least 40000 iterations from the plain start, continued in chunks
-          of 10000 until a chunk changes it by less than 1e-6 (computed per
-          scene before the timed region), checked every 10 iterations,
-          capped at 50000.
-          The energy leaves out the ~0.2 % of pixels where the reference's
+          at most 1e-4 (relative) above E*, checked every 10 iterations,
+          capped at 50000.  E* is the energy the reference's start settles
+          at (computed per scene before the timed region): the plain start
+          iterated at least 40000 times, continued in chunks of 10000 until
+          a chunk changes the energy by less than 1e-6.  The rule is
+          one-sided because PDHG from another start can settle on a fixed
+          point a few 1e-4 below E* (reported per scene: the learned start
+          iterated as many times).  The energy leaves out the ~0.2 % of pixels where the reference's
           fixed steps make the data step unstable (tau sigma lambda^2
           sum Iu^2 > 1 at the pixel or a neighbour): the iteration oscillates
           there forever;
@@ -373,7 +375,7 @@ def probe(names, seed):
             saved = {k: os.environ.get(k) for k in ("PSW_SWEEP_TY", "PSW_SWEEP_GROUPS")}
             try:
                 for ty in ("16", "32"):
-                    for g in ("1", "2", "3", "5", "8", "12", "16"):
+                    for g in ("1", "2", "3", "5", "8", "12", "16", "24", "32", "48", "64"):
                         os.environ["PSW_SWEEP_TY"], os.environ["PSW_SWEEP_GROUPS"] = ty, g
                         for fast in (True, False):
                             pl.sweep.run(fast=fast, stream=pl.st.handle)
@@ -464,7 +466,7 @@ def run_batch(args):
     iters, energies = batched()
     st.synchronize()
     t_batch = time.perf_counter() - t0
-    gaps = [abs(e - es) / es for e, es in zip(energies, e_star)]
+    gaps = [(e - es) / es for e, es in zip(energies, e_star)]
     # the same frames one at a time through the single-frame entry points
     t0 = time.perf_counter()
     seq_ms = []
@@ -563,7 +565,7 @@ def main():
             change, e_prev = (e_prev - e) / e, e
             if abs(change) < 1e-6 or n_it >= ESTAR_CAP:
                 break
-        pl.e_star, pl.estar_iters, pl.estar_change = e, n_it, change
+        pl.e_star_plain, pl.estar_iters, pl.estar_change = e, n_it, change
         pl.u_star = pl.tgv.get_u(s)
         pl.weight = pl.tgv.get_energy_mask(s)
         pl.d_s_host = d_s.to_host(s)
@@ -576,9 +578,11 @@ def main():
         pl.ws.start(pl.tgv, s)
         pl.tgv.iterate(n_it, pl.prm, s)
         pl.e_star_learned = pl.tgv.energy(pl.prm, s)
+        pl.e_star = pl.e_star_plain
         st.synchronize()
-        log(f"seed {sd}: E* = {pl.e_star:.9e} after {n_it} iterations (change over the last 10000: {change:.2e}); "
-            f"from the learned start after as many: {(pl.e_star_learned - pl.e_star) / pl.e_star:+.2e} relative; "
+        log(f"seed {sd}: E* = {pl.e_star:.9e} after {n_it} iterations of the plain start (change over the "
+            f"last 10000: {change:.2e}); the learned start after as many: "
+            f"{(pl.e_star_learned - pl.e_star) / pl.e_star:+.2e} relative; "
             f"{pl.excluded} pixels outside the energy mask")
         pipes.append(pl)
 
@@ -615,7 +619,7 @@ def main():
         s = pl.st.handle
         rec = {"seed": pl.seed, "e_star": pl.e_star, "estar_iters": pl.estar_iters,
                "estar_change_last_10000": pl.estar_change,
-               "learned_start_energy_after_as_many_rel": (pl.e_star_learned - pl.e_star) / pl.e_star,
+               "learned_start_settled_rel": (pl.e_star_learned - pl.e_star) / pl.e_star,
                "excluded_pixels": pl.excluded}
         u_stop = {}
         for mode, (_, recs) in res.items():
@@ -625,8 +629,8 @@ def main():
                          "sweep_ms": float(np.mean([r["sweep_ms"] for r in rs])),
                          "net_ms": float(np.mean([r["net_ms"] for r in rs])),
                          "tgv_ms": float(np.mean([r["tgv_ms"] for r in rs])),
-                         "max_rel_gap_at_stop": float(max(abs(r["energy"] - pl.e_star) / pl.e_star for r in rs)),
-                         "reached_gap": bool(all(abs(r["energy"] - pl.e_star) <= GAP_TOL * pl.e_star for r in rs)),
+                         "max_rel_gap_at_stop": float(max((r["energy"] - pl.e_star) / pl.e_star for r in rs)),
+                         "reached_gap": bool(all(r["energy"] - pl.e_star <= GAP_TOL * pl.e_star for r in rs)),
                          "starts": sorted({r["start"] for r in rs})}
             pl.run(mode, pl.e_star, args.max_iters, args.check_every, fast)
             u_stop[mode] = pl.tgv.get_u(s)
@@ -705,9 +709,10 @@ def main():
         "data": f"synthetic (seeded room scenes rendered on the GPU, seeds {seeds[0]}-{seeds[-1]}, none used in training)",
         "config": {"workload": "learned warm-start pipeline: 640x480 reference + 8 sources, 256 inverse-depth planes "
                                "0.3-8 m, 9x9 NCC, fused sweep+argmin, BF16 U-Net warm start (plain start as "
-                               "fallback), TGV2 linearised at the sweep depth to relative energy gap 1e-4 of E* "
-                               f"(plain run of >= {args.estar_iters} iterations until a 10000-iteration chunk "
-                               f"changes it by < 1e-6); steps cycle over {len(seeds)} scenes",
+                               "fallback), TGV2 linearised at the sweep depth until the energy is at most 1e-4 "
+                               "(relative) above E* (the plain start's settled energy: >= "
+                               f"{args.estar_iters} iterations until a 10000-iteration chunk changes it by "
+                               f"< 1e-6); steps cycle over {len(seeds)} scenes",
                    "sweep_arith": "fast" if fast else "reference-order", "tgv_gap_tol": GAP_TOL,
                    "check_every": args.check_every, "max_iters": args.max_iters, "estar_iters": args.estar_iters,
                    "seeds": seeds, "network_dtype": "bf16 (fp32 accumulation)"},

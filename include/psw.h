@@ -352,9 +352,12 @@ int psw_tgv_set_energy_mask(psw_tgv* t, double qmax, const psw_tgv_params* prm, 
 int psw_tgv_iterate(psw_tgv* t, int niters, const psw_tgv_params* prm, psw_stream_t s);
 /* FP64 primal energy of the current iterate (synchronous). */
 int psw_tgv_energy(psw_tgv* t, const psw_tgv_params* prm, double* energy, psw_stream_t s);
-/* Iterate until |E - e_star| / e_star <= tol (checked every check_every
- * iterations, FP64 reductions on the GPU) or max_iters; e_star <= 0 disables
- * the test.  iters_done / energy_out (nullable) report the stop. */
+/* Iterate until (E - e_star) / e_star <= tol (checked every check_every
+ * iterations, FP64 reductions on the GPU) or max_iters: e_star is the lowest
+ * energy known for the problem, and an energy below it stops too (PDHG from
+ * different starts can settle on fixed points whose energies differ by more
+ * than tol).  e_star <= 0 disables the test.  iters_done / energy_out
+ * (nullable) report the stop. */
 int psw_tgv_solve(psw_tgv* t, int max_iters, int check_every, double tol, double e_star,
                   const psw_tgv_params* prm, int* iters_done, double* energy_out, psw_stream_t s);
 /* Many solvers (frames of any size and source count) per launch (no

@@ -334,6 +334,12 @@ __device__ inline double block_sum(double v) {
 // result does not depend on NT), then the stop test.
 constexpr int FIN_THREADS = 256;
 
+// The stop rule: the energy is at most tol (relative) above e_star, the
+// lowest energy known for the problem (an energy below it stops too).
+__device__ inline bool tgv_at_gap(double E, double e_star, double tol) {
+    return e_star > 0.0 && E - e_star <= tol * e_star;
+}
+
 // Returns (on thread 0) whether this check stopped the solve.  The host
 // mirror is optional (the batched solve counts stops on the device instead).
 template <int NT>
@@ -348,7 +354,7 @@ __device__ bool check_finalize(const TgvCheck& c) {
         volatile TgvCtrl* m = c.mirror;
         d->energy = E;
         if (m) m->energy = E;
-        if (!d->done && c.e_star > 0.0 && fabs(E - c.e_star) <= c.tol * c.e_star) {
+        if (!d->done && tgv_at_gap(E, c.e_star, c.tol)) {
             d->iters_at_done = c.fin_k;
             d->energy_at_done = E;
             d->done = 1;
@@ -1125,7 +1131,7 @@ __device__ inline bool tgv_finalize_block(const double* __restrict__ partials, i
     bool stopped = false;
     if (threadIdx.x == 0) {
         ctrl->energy = E;
-        if (!ctrl->done && e_star > 0.0 && fabs(E - e_star) <= tol * e_star) {
+        if (!ctrl->done && tgv_at_gap(E, e_star, tol)) {
             ctrl->done = 1;
             ctrl->iters_at_done = k;
             ctrl->energy_at_done = E;

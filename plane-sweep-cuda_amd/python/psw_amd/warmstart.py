@@ -169,7 +169,7 @@ class WarmStarter:
             return it, e, "plain"
         first = max_iters if fallback_after is None else min(int(fallback_after), max_iters)
         it, e = tgv.solve(first, check_every, tol, e_star, prm, stream)
-        if e_star <= 0 or abs(e - e_star) <= tol * e_star or first >= max_iters:
+        if e_star <= 0 or e - e_star <= tol * e_star or first >= max_iters:
             return it, e, "learned"
         tgv.set_primal(d_u0, 0.0, None, stream)
         tgv.set_duals(None, None, None, stream)

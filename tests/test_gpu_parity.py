@@ -337,7 +337,8 @@ def test_tgv_linearisation_vs_oracle(gpu, views_small):
 
 
 def test_tgv_solve_stops_at_gap(gpu, views_small):
-    """psw_tgv_solve stops once |E - E*| <= tol E* (E* from a long run)."""
+    """psw_tgv_solve stops once E - E* <= tol E* (E* from a long run); an
+    energy below E* stops too (at the first check)."""
     from psw_amd import stages
     from psw_amd.device import DeviceArray
     v = views_small
@@ -362,7 +363,7 @@ def test_tgv_solve_stops_at_gap(gpu, views_small):
     iters, e = t.solve(3000, 10, 1e-3, e_star, prm)
     print(f"E* {e_star:.6e}; stopped after {iters} iterations at E {e:.6e}")
     assert iters < 3000 and iters % 10 == 0
-    assert abs(e - e_star) <= 1e-3 * e_star
+    assert e - e_star <= 1e-3 * e_star
     assert abs(t.energy(prm) - e) <= 1e-12 * e   # the state is the one at the stop
     # the same stop as testing the energy after every chunk of 10 with the
     # stand-alone energy pass (the solve tests it inside the next launch)
@@ -371,12 +372,19 @@ def test_tgv_solve_stops_at_gap(gpu, views_small):
     while True:
         ref.iterate(10, prm)
         k += 10
-        if abs(ref.energy(prm) - e_star) <= 1e-3 * e_star:
+        if ref.energy(prm) - e_star <= 1e-3 * e_star:
             break
     assert k == iters
     sa, sb = ref.get_state(), t.get_state()
     for name in ("u", "u1x", "u1y", "px", "py", "qx", "qy", "qz", "r"):
         np.testing.assert_array_equal(sa[name], sb[name], err_msg=name)
+    # one-sided: a target far above the energy after 10 iterations stops there
+    low = fresh()
+    low.iterate(10, prm)
+    e10 = low.energy(prm)
+    t = fresh()
+    iters, e = t.solve(3000, 10, 1e-3, 4.0 * e10, prm)
+    assert iters == 10 and e == pytest.approx(e10, rel=1e-12)
 
 
 @pytest.mark.parametrize("variant,double,ty2,triple,inflight", [
@@ -416,7 +424,7 @@ def test_tgv_solve_max_iters_and_variants(gpu, views_small, monkeypatch, variant
     t.set_u(value=1.0)
     t.linearise(K1, Rrel, trel)
     it2, e2 = t.solve(400, 7, 2e-2, e_star, prm)
-    assert it2 < 400 and it2 % 7 == 0 and abs(e2 - e_star) <= 2e-2 * e_star
+    assert it2 < 400 and it2 % 7 == 0 and e2 - e_star <= 2e-2 * e_star
     assert abs(t.energy(prm) - e2) <= 1e-12 * e2
     s_stop = t.get_state()
     t.iterate(3, prm)                                    # resumes from the stop (needs u_{k-1} intact)

@@ -322,8 +322,10 @@ def test_tgv_solve_to_gap_vs_fp64(problem640, start):
           f"energy there {(e64 - P.e_star) / P.e_star:+.2e}, oracle sum of the GPU state's energy "
           f"{(e_gpu64 - P.e_star) / P.e_star:+.2e} (FP64 and FP32 oracle runs {time.perf_counter() - t0:.1f} s)",
           flush=True)
-    assert abs(e - P.e_star) <= GAP_TOL * P.e_star
-    assert abs(e64 - P.e_star) <= GAP_TOL * P.e_star
+    # the stop rule is one-sided (E - E* <= tol E*); PDHG from another start
+    # may settle a little below E*, never far below the long plain run's
+    assert -1e-3 * P.e_star <= e - P.e_star <= GAP_TOL * P.e_star
+    assert e64 - P.e_star <= GAP_TOL * P.e_star
     assert abs(e_gpu64 - e) <= 1e-10 * P.e_star         # the GPU's FP64 reduction of its own state
     assert abs(e - e64) <= 1e-5 * P.e_star               # the stop decision: GPU vs FP64 run, a tenth of the gap
     _tgv_bars(S["u"], st64["u"], P.dist, f"{start} start, stop state", u32=st32["u"])
@@ -415,7 +417,7 @@ def test_uneven_batch_32_frames(gpu):
         it, e = f.tgv.solve(50000, 10, GAP_TOL, e_star[i], prm, s)
         assert it == iters[i], (i, it, iters[i])
         np.testing.assert_array_equal(f.tgv.get_state(s)["u"], states[i], err_msg=f"frame {i} TGV2 stop state")
-        assert abs(energies[i] - e_star[i]) <= GAP_TOL * e_star[i]
+        assert abs(energies[i] - e) <= 1e-12 * e and energies[i] - e_star[i] <= GAP_TOL * e_star[i]
     # one frame of each resolution against the oracle
     seen = set()
     for i, (f, sp) in enumerate(zip(frames, specs)):
