@@ -22,6 +22,7 @@
 #include "tgv_point.h"
 
 #include <cstring>
+#include <mutex>
 #include <algorithm>
 #include <vector>
 
@@ -82,6 +83,9 @@ struct psw_tgv {
     psw::TgvCtrl* hctrl;  // pinned host copy (filled by copies)
     psw::TgvCtrl* hmirror;  // pinned, mapped host memory written by checking launches
     psw::TgvCtrl* dmirror;  // its device address
+    unsigned long long* rx; // exchange words of the resident solver (NULL: not available), uncached, pooled
+    size_t rx_words;
+    int rx_dev;
     int k;                // iterations since the warp started
     bool have_images;
 };
@@ -1614,6 +1618,712 @@ static std::vector<std::vector<int>> tgv_batch_groups(psw_tgv* const* ts, int n)
     return groups;
 }
 
+// ---- resident solver --------------------------------------------------------
+//
+// One cooperative launch runs the whole solve (or a long psw_tgv_iterate):
+// every 64 x 20 tile is owned by one workgroup of 640 threads (one per CU)
+// for the whole launch, two pixels per thread (rows r and r + 10), and its
+// state (u_k, u_{k-1}, p, q, r and the constants It, Iu, T, u0) stays in
+// registers.  HBM is read once at the start and written once at the end.
+// Per iteration a tile needs from its neighbours only the ubar of the pixels
+// next to its border (and, before an energy check, u of the right / lower
+// neighbours): every tile publishes its edge rows and columns as tagged
+// 64-bit words (iteration in the high half, the float in the low half), and
+// its neighbours poll those words until the tag matches, so no fence or grid
+// barrier is involved.  The duals p, q of the one-pixel ring left of and
+// above the tile (which the u step reads) are computed by the tile itself,
+// from ring copies kept in LDS; they are bit-identical to the neighbour's
+// own values.  Words are double-buffered by iteration parity: a tile cannot
+// publish iteration j+2 before every neighbour has read its iteration j,
+// because it needs their iteration j+1.
+//
+// The launch waits until every workgroup has started (spin-waiting
+// neighbours need all tiles resident at once); if that does not happen in
+// time the launch is called off before anything is written and the caller
+// uses the launch sequence.  Energy checks use the two-iteration launches'
+// partials (one per 64 x 10 half tile, per-wave butterflies and the waves in
+// order, as a 640-thread block) and the same finalising sum (tile 0's
+// workgroup), so a check gives the same bits and the solve stops at the same
+// iteration with the same state.  Every wait is bounded; a wait that runs
+// out sets an error word, after which every workgroup leaves at its next
+// wait, and the host reports the failure.
+constexpr int TR_SUB = 10;                 // the launch sequence's tile height (energy partials)
+constexpr int TR_TY = 2 * TR_SUB, TR_NT = TG_TX * TR_SUB, TR_MAXN = 8, TR_MAXTILES = 256;
+constexpr int TR_MIN_ITERS = 4;            // shorter psw_tgv_iterate calls use the launch sequence
+// words per tile and parity: ubar (3 fields) on the top row, bottom row, left
+// and right columns, then u (3 fields) on the top row and left column
+constexpr int TR_UB_TOP = 0, TR_UB_BOT = 3 * TG_TX, TR_UB_LEFT = 6 * TG_TX, TR_UB_RIGHT = 6 * TG_TX + 3 * TR_TY;
+constexpr int TR_U_TOP = 6 * TG_TX + 6 * TR_TY, TR_U_LEFT = 9 * TG_TX + 6 * TR_TY;
+constexpr int TR_XW = 9 * TG_TX + 9 * TR_TY;
+constexpr int TR_HALO_UB = 3 * (2 * TG_TX + 2 * TR_TY + 2);   // ubar words a tile reads per iteration
+constexpr int TR_HALO_U = 3 * (TG_TX + TR_TY);                 // u words before a check
+static_assert(TR_HALO_UB <= TR_NT && TR_HALO_U <= TR_NT, "one halo word per thread");
+
+struct TgvResArgs {
+    const float* ucur;               // state k0
+    const float* uprev;
+    const float* pcur;
+    const float* qcur;
+    const float* rcur;
+    const float* It;
+    const float* Iu;
+    const float* T;
+    const float* u0;
+    const float* mask;
+    float* U;                        // the slot arrays (the end state is stored by slot)
+    float* P;
+    float* Q;
+    float* Rs;
+    unsigned long long* xch;         // [2][ntiles][TR_XW]
+    unsigned long long* part;        // [2][nsub][2] energy partials (two 32-bit halves)
+    unsigned long long* dec;         // [2] stop decisions
+    unsigned* err;                   // [0] error, [1] started, [2] go (1) / called off (2), [3..7] first failure:
+                                     // tile, word offset from xch, awaited tag, seen tag, iteration
+    TgvCtrl* ctrl;
+    int W, H, N;
+    int k0, niters, check_every;     // check_every 0: no checks
+    int tiles_x, tiles_y, ntiles, sub_y, nsub;
+    unsigned spin, start_spin;
+    float lambda, alpha0, alpha1, tau, sigma;
+    double dlambda, dalpha0, dalpha1, e_star;
+    double gap;                      // tol * e_star (the stop test's bound, the same double product)
+};
+
+__device__ __forceinline__ unsigned long long tr_word(unsigned tag, unsigned bits) {
+    return ((unsigned long long)tag << 32) | bits;
+}
+
+__device__ __forceinline__ void tr_put(unsigned long long* p, unsigned tag, float v) {
+    __hip_atomic_store(p, tr_word(tag, __float_as_uint(v)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ unsigned long long tr_load(unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ unsigned tr_load32(unsigned* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Records the first failure (tile, word offset from the exchange words,
+// awaited tag, seen tag, iteration) and sets the error word.
+__device__ inline void tr_fail(const TgvResArgs& R, unsigned tile, unsigned word, unsigned tag, unsigned seen,
+                               unsigned j) {
+    unsigned expected = 0u;
+    if (__hip_atomic_compare_exchange_strong(R.err, &expected, 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT)) {
+        const unsigned vals[5] = {tile, word, tag, seen, j};
+        for (int k = 0; k < 5; ++k) __hip_atomic_store(R.err + 3 + k, vals[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Poll up to three words (p[i] == nullptr: none) until each carries `tag`;
+// their low halves go to bits[i].  The loads of one round are issued
+// together.  At most R.spin rounds (0 fails every wait at once: the host's
+// way to exercise the failure path); the error word, set by any wait that
+// runs out, ends the others too.
+__device__ inline bool tr_get3(unsigned long long* const p[3], unsigned tag, const TgvResArgs& R, unsigned bits[3],
+                               int tile, int j) {
+    bool ok[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ok[i] = p[i] == nullptr;
+    if (ok[0] && ok[1] && ok[2]) return true;   // nothing to wait for (also with a zero budget)
+    for (unsigned it = 0; it < R.spin; ++it) {
+        unsigned long long w[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) w[i] = ok[i] ? 0ull : tr_load(p[i]);
+        bool all = true;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            if (!ok[i] && (unsigned)(w[i] >> 32) == tag) {
+                ok[i] = true;
+                bits[i] = (unsigned)w[i];
+            }
+            all = all && ok[i];
+        }
+        if (all) return true;
+        if ((it & 63) == 63 && tr_load32(R.err) != 0u) return false;
+        __builtin_amdgcn_s_sleep(2);
+    }
+    // some word is still missing (the early return above), so p[i] is set
+    int i = 0;
+    while (ok[i]) ++i;
+    tr_fail(R, (unsigned)tile, (unsigned)(p[i] - R.xch), tag, (unsigned)(tr_load(p[i]) >> 32), (unsigned)j);
+    return false;
+}
+
+// A halo word of the tile (tcx, tcy): w < TR_HALO_UB are ubar words (cell
+// c = w / 3: the row above, the above-right corner, the column left, the
+// below-left corner, the column right, the row below), the rest u words
+// before a check (cells: the column right, the row below).  Returns false
+// if the cell is outside the image or has no neighbour tile; else the
+// source tile and word, the LDS cell (row, column of the ubar array, or of
+// the u array) and the cell's image position.
+__device__ inline bool tr_halo(const TgvResArgs& R, int tcx, int tcy, int w, int& src, int& word, int& ly, int& lx,
+                               int& gx, int& gy) {
+    constexpr int TX = TG_TX, TY = TR_TY;
+    const int x0 = tcx * TX, y0 = tcy * TY;
+    const int tile = tcy * R.tiles_x + tcx;
+    const bool has_l = tcx > 0, has_r = tcx < R.tiles_x - 1, has_u = tcy > 0, has_d = tcy < R.tiles_y - 1;
+    if (w < TR_HALO_UB) {
+        const int c = w / 3, f = w - (w / 3) * 3;
+        if (c < TX) {
+            gx = x0 + c; gy = y0 - 1; src = tile - R.tiles_x; word = TR_UB_BOT + f * TX + c; ly = 0; lx = c + 1;
+            return has_u && gx < R.W;
+        }
+        if (c == TX) {
+            gx = x0 + TX; gy = y0 - 1; src = tile - R.tiles_x + 1; word = TR_UB_BOT + f * TX; ly = 0; lx = TX + 1;
+            return has_u && has_r;
+        }
+        if (c < TX + 1 + TY) {
+            const int r = c - (TX + 1);
+            gx = x0 - 1; gy = y0 + r; src = tile - 1; word = TR_UB_RIGHT + f * TY + r; ly = r + 1; lx = 0;
+            return has_l && gy < R.H;
+        }
+        if (c == TX + 1 + TY) {
+            gx = x0 - 1; gy = y0 + TY; src = tile + R.tiles_x - 1; word = TR_UB_TOP + f * TX + TX - 1; ly = TY + 1;
+            lx = 0;
+            return has_l && has_d;
+        }
+        if (c < TX + 2 + 2 * TY) {
+            const int r = c - (TX + 2 + TY);
+            gx = x0 + TX; gy = y0 + r; src = tile + 1; word = TR_UB_LEFT + f * TY + r; ly = r + 1; lx = TX + 1;
+            return has_r && gy < R.H;
+        }
+        const int cc = c - (TX + 2 + 2 * TY);
+        gx = x0 + cc; gy = y0 + TY; src = tile + R.tiles_x; word = TR_UB_TOP + f * TX + cc; ly = TY + 1; lx = cc + 1;
+        return has_d && gx < R.W;
+    }
+    const int wu = w - TR_HALO_UB, c = wu / 3, f = wu - (wu / 3) * 3;
+    if (c < TY) {
+        gx = x0 + TX; gy = y0 + c; src = tile + 1; word = TR_U_LEFT + f * TY + c; ly = c; lx = TX;
+        return has_r && gy < R.H;
+    }
+    const int cc = c - TY;
+    gx = x0 + cc; gy = y0 + TY; src = tile + R.tiles_x; word = TR_U_TOP + f * TX + cc; ly = TY; lx = cc;
+    return has_d && gx < R.W;
+}
+
+// The ring position of thread tid: threads 0..63 the row above the tile,
+// 64..83 the column left of it (false where there is no such neighbour).
+// Its image position and its cell of the ring-indexed LDS arrays.
+__device__ __forceinline__ bool tr_ring(const TgvResArgs& R, int tcx, int tcy, int tid, int& gx, int& gy, int& lx,
+                                        int& ly) {
+    const int x0 = tcx * TG_TX, y0 = tcy * TR_TY;
+    if (tid < TG_TX) {
+        gx = x0 + tid; gy = y0 - 1; lx = tid + 1; ly = 0;
+        return tcy > 0 && gx < R.W;
+    }
+    const int r = tid - TG_TX;
+    gx = x0 - 1; gy = y0 + r; lx = 0; ly = r + 1;
+    return r < TR_TY && tcx > 0 && gy < R.H;
+}
+
+template <int NS>
+__global__ __launch_bounds__(TR_NT, 3) void tgv_resident_kernel(TgvResArgs R) {
+    constexpr int TX = TG_TX, TY = TR_TY, SUB = TR_SUB;
+    __shared__ float sUb[3][TY + 2][TX + 2];   // ubar_j: ring row / column 0, own 1..TY / 1..TX, halo TY+1 / TX+1
+    __shared__ float sPQ[5][TY + 1][TX + 1];   // p, q of j+1: ring 0, own 1..
+    __shared__ float sUk[3][TY + 1][TX + 1];   // u_j before a check: own 0.., right / lower halo TY / TX
+    __shared__ float sRT[3][TG_TX + TR_TY];    // T at the ring positions (the ring's p, q live in sPQ)
+    __shared__ double sWave[2][SUB];
+    __shared__ int sFlag[2];                   // [0] a wait failed, [1] stop
+    // tile 0 gathers the energy partials over sUk (read before the gather)
+    static_assert(sizeof(double) * 2 * TR_MAXTILES <= sizeof(float) * 3 * (TR_TY + 1) * (TG_TX + 1), "partials");
+    double* sPart = reinterpret_cast<double*>(&sUk[0][0][0]);
+
+    const int W = R.W, H = R.H;
+    const size_t n = (size_t)W * H;
+    const int tile = xcd_tile(blockIdx.x, R.ntiles);
+    const int tcx = tile % R.tiles_x, tcy = tile / R.tiles_x;
+    const int x0 = tcx * TX, y0 = tcy * TY;
+    const int tid = threadIdx.x;
+    const int tx = tid % TX, ty0 = tid / TX;   // pixels (tx, ty0) and (tx, ty0 + SUB)
+    const int x = x0 + tx;
+    const float as1 = R.alpha1 * R.sigma, as0 = R.alpha0 * R.sigma, sl = R.sigma * R.lambda;
+    if (tid == 0) {
+        sFlag[0] = 0;
+        sFlag[1] = 0;
+        // the last workgroup to start gives the go
+        const unsigned before = __hip_atomic_fetch_add(R.err + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (before + 1 == (unsigned)R.ntiles) {
+            unsigned expected = 0u;
+            __hip_atomic_compare_exchange_strong(R.err + 2, &expected, 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+
+    // own pixels: state k0 and the constants
+    float u[2], u1x[2], u1y[2], v[2], v1x[2], v1y[2], px[2], py[2], qx[2], qy[2], qz[2], t11[2], t12[2], t22[2];
+    float u0t[2], rv[2][NS], itv[2][NS], iuv[2][NS];
+    unsigned counted = 0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int y = y0 + ty0 + h * SUB;
+        const size_t g = (size_t)min(y, H - 1) * W + min(x, W - 1);
+        u[h] = R.ucur[g]; u1x[h] = R.ucur[n + g]; u1y[h] = R.ucur[2 * n + g];
+        v[h] = R.uprev[g]; v1x[h] = R.uprev[n + g]; v1y[h] = R.uprev[2 * n + g];
+        px[h] = R.pcur[g]; py[h] = R.pcur[n + g];
+        qx[h] = R.qcur[g]; qy[h] = R.qcur[n + g]; qz[h] = R.qcur[2 * n + g];
+        t11[h] = R.T[g]; t12[h] = R.T[n + g]; t22[h] = R.T[2 * n + g];
+        u0t[h] = R.u0[g];
+        if (x < W && y < H && R.mask[g] != 0.f) counted |= 1u << h;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            if (s < R.N) {
+                const size_t gs = (size_t)s * n + g;
+                rv[h][s] = R.rcur[gs];
+                itv[h][s] = R.It[gs];
+                iuv[h][s] = R.Iu[gs];
+            }
+        }
+    }
+    // the ring: p, q of state k0 into its sPQ cells (updated there in place
+    // every iteration), T into sRT
+    {
+        int rgx, rgy, rix, riy;
+        if (tr_ring(R, tcx, tcy, tid, rgx, rgy, rix, riy)) {
+            const size_t gr = (size_t)rgy * W + rgx;
+            sPQ[0][riy][rix] = R.pcur[gr]; sPQ[1][riy][rix] = R.pcur[n + gr];
+            sPQ[2][riy][rix] = R.qcur[gr]; sPQ[3][riy][rix] = R.qcur[n + gr]; sPQ[4][riy][rix] = R.qcur[2 * n + gr];
+            sRT[0][tid] = R.T[gr]; sRT[1][tid] = R.T[n + gr]; sRT[2][tid] = R.T[2 * n + gr];
+        }
+    }
+    // the halo of state k0 from the state buffers (later states come from
+    // the neighbours' words)
+    int hoff = -1, hcell = 0, uoff = -1, ucell = 0;
+    {
+        int src, word, ly, lx, gx, gy;
+        if (tid < TR_HALO_UB && tr_halo(R, tcx, tcy, tid, src, word, ly, lx, gx, gy)) {
+            const int f = tid % 3;
+            const size_t gh = (size_t)gy * W + gx;
+            sUb[f][ly][lx] = 2.f * R.ucur[f * n + gh] - R.uprev[f * n + gh];
+            hoff = src * TR_XW + word;
+            hcell = (f * (TY + 2) + ly) * (TX + 2) + lx;
+        }
+        if (tid < TR_HALO_U && tr_halo(R, tcx, tcy, TR_HALO_UB + tid, src, word, ly, lx, gx, gy)) {
+            uoff = src * TR_XW + word;
+            ucell = ((tid % 3) * (TY + 1) + ly) * (TX + 1) + lx;
+        }
+    }
+    // every workgroup resident before any waits on a neighbour
+    if (tid == 0) {
+        unsigned go = tr_load32(R.err + 2);
+        for (unsigned it = 0; go == 0u && it < R.start_spin; ++it) {
+            __builtin_amdgcn_s_sleep(2);
+            go = tr_load32(R.err + 2);
+        }
+        if (go == 0u) {
+            unsigned expected = 0u;
+            __hip_atomic_compare_exchange_strong(R.err + 2, &expected, 2u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            go = tr_load32(R.err + 2);
+        }
+        if (go != 1u) sFlag[0] = 1;
+    }
+    __syncthreads();
+    if (sFlag[0]) return;   // called off: nothing written
+
+    int j = 0, nchk = 0;
+    bool failed = false;
+    for (;; ++j) {
+        const bool check = R.check_every > 0 && j > 0 && j % R.check_every == 0 && j < R.niters;
+        if (j == R.niters && !check) break;
+        // ubar_j (and u_j before a check) of the tile and its halo
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int ty = ty0 + h * SUB;
+            sUb[0][ty + 1][tx + 1] = 2.f * u[h] - v[h];
+            sUb[1][ty + 1][tx + 1] = 2.f * u1x[h] - v1x[h];
+            sUb[2][ty + 1][tx + 1] = 2.f * u1y[h] - v1y[h];
+            if (check) { sUk[0][ty][tx] = u[h]; sUk[1][ty][tx] = u1x[h]; sUk[2][ty][tx] = u1y[h]; }
+        }
+        if (j > 0) {
+            unsigned long long* blk = R.xch + (size_t)(j & 1) * R.ntiles * TR_XW;
+            unsigned long long* const hp[3] = {hoff >= 0 ? blk + hoff : nullptr,
+                                               check && uoff >= 0 ? blk + uoff : nullptr, nullptr};
+            unsigned bits[3] = {0u, 0u, 0u};
+            if (!tr_get3(hp, (unsigned)j, R, bits, tile, j)) sFlag[0] = 1;
+            if (hp[0]) (&sUb[0][0][0])[hcell] = __uint_as_float(bits[0]);
+            if (hp[1]) (&sUk[0][0][0])[ucell] = __uint_as_float(bits[1]);
+        }
+        __syncthreads();
+        if (sFlag[0]) { failed = true; break; }
+        if (check) {
+            // energy of state j: the partials of the two 64 x 10 halves (wave
+            // butterflies, then the ten waves in order, as a 640-thread block)
+            double e2[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int ty = ty0 + h * SUB, y = y0 + ty;
+                e2[h] = 0.0;
+                if (counted & (1u << h)) {
+                    const double dud = (double)u[h] - (double)u0t[h];
+                    double ed = 0.0;
+#pragma unroll
+                    for (int s = 0; s < NS; ++s)
+                        if (s < R.N) ed += fabs((double)itv[h][s] + dud * (double)iuv[h][s]);
+                    const int kr = x < W - 1 ? tx + 1 : tx, kd = y < H - 1 ? ty + 1 : ty;
+                    e2[h] = energy_point(u[h], sUk[0][ty][kr], sUk[0][kd][tx], u1x[h], sUk[1][ty][kr], sUk[1][kd][tx],
+                                         u1y[h], sUk[2][ty][kr], sUk[2][kd][tx], t11[h], t12[h], t22[h], ed,
+                                         R.dlambda, R.dalpha0, R.dalpha1);
+                }
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                e2[0] += __shfl_xor(e2[0], off, 64);
+                e2[1] += __shfl_xor(e2[1], off, 64);
+            }
+            if (tx == 0) { sWave[0][ty0] = e2[0]; sWave[1][ty0] = e2[1]; }
+            __syncthreads();
+            ++nchk;
+            if (tid < 2) {
+                const int sy = 2 * tcy + tid;   // the half tile's row in the launch sequence's tiles
+                if (sy < R.sub_y) {
+                    double e = 0.0;
+                    for (int k = 0; k < SUB; ++k) e += sWave[tid][k];
+                    const unsigned long long eb = (unsigned long long)__double_as_longlong(e);
+                    unsigned long long* pw = R.part + ((size_t)(nchk & 1) * R.nsub + sy * R.tiles_x + tcx) * 2;
+                    __hip_atomic_store(pw, tr_word((unsigned)nchk, (unsigned)(eb >> 32)), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(pw + 1, tr_word((unsigned)nchk, (unsigned)eb), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+            if (tile == 0) {
+                // finalise as the finalising workgroup of the launch sequence:
+                // every half tile's partial, the sum over threads below
+                // FIN_THREADS, the stop test, the decision for every tile
+                for (int i = tid; i < R.nsub; i += TR_NT) {
+                    unsigned long long* q = R.part + ((size_t)(nchk & 1) * R.nsub + i) * 2;
+                    unsigned long long* const pp[3] = {q, q + 1, nullptr};
+                    unsigned hl[3] = {0u, 0u, 0u};
+                    if (!tr_get3(pp, (unsigned)nchk, R, hl, tile, j)) sFlag[0] = 1;
+                    sPart[i] = __longlong_as_double((long long)(((unsigned long long)hl[0] << 32) | hl[1]));
+                }
+                __syncthreads();
+                double vs = 0.0;
+                if (tid < FIN_THREADS)
+                    for (int i = tid; i < R.nsub; i += FIN_THREADS) vs += sPart[i];
+                const double E = block_sum<TR_NT>(vs);
+                if (tid == 0 && !sFlag[0]) {
+                    TgvCtrl* d = R.ctrl;
+                    d->energy = E;
+                    unsigned stop = 0u;
+                    if (!d->done && R.e_star > 0.0 && E - R.e_star <= R.gap) {   // tgv_at_gap
+                        d->iters_at_done = R.k0 + j;
+                        d->energy_at_done = E;
+                        d->done = 1;
+                        stop = 1u;
+                    }
+                    __hip_atomic_store(R.dec + (nchk & 1), tr_word((unsigned)nchk, stop), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+            if (tid == 0) {
+                unsigned long long* const pd[3] = {R.dec + (nchk & 1), nullptr, nullptr};
+                unsigned stop[3] = {0u, 0u, 0u};
+                if (sFlag[0] || !tr_get3(pd, (unsigned)nchk, R, stop, tile, j)) sFlag[0] = 1;
+                sFlag[1] = (int)stop[0];
+            }
+            __syncthreads();
+            if (sFlag[0]) { failed = true; break; }
+            if (sFlag[1]) break;
+        }
+        // p, q of j+1 on the tile and the ring
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int ty = ty0 + h * SUB, y = y0 + ty;
+            const int ib = ty + 1, jb = tx + 1;
+            const int jr = x < W - 1 ? jb + 1 : jb, id = y < H - 1 ? ib + 1 : ib;
+            pq_update(px[h], py[h], qx[h], qy[h], qz[h], t11[h], t12[h], t22[h], sUb[0][ib][jb], sUb[0][ib][jr],
+                      sUb[0][id][jb], sUb[1][ib][jb], sUb[1][ib][jr], sUb[1][id][jb], sUb[2][ib][jb], sUb[2][ib][jr],
+                      sUb[2][id][jb], as1, as0);
+            sPQ[0][ib][jb] = px[h]; sPQ[1][ib][jb] = py[h];
+            sPQ[2][ib][jb] = qx[h]; sPQ[3][ib][jb] = qy[h]; sPQ[4][ib][jb] = qz[h];
+        }
+        {
+            int rgx, rgy, rix, riy;
+            if (tr_ring(R, tcx, tcy, tid, rgx, rgy, rix, riy)) {
+                const int rr = rgx < W - 1 ? rix + 1 : rix, rd = rgy < H - 1 ? riy + 1 : riy;
+                float rpx = sPQ[0][riy][rix], rpy = sPQ[1][riy][rix];
+                float rqx = sPQ[2][riy][rix], rqy = sPQ[3][riy][rix], rqz = sPQ[4][riy][rix];
+                pq_update(rpx, rpy, rqx, rqy, rqz, sRT[0][tid], sRT[1][tid], sRT[2][tid], sUb[0][riy][rix],
+                          sUb[0][riy][rr], sUb[0][rd][rix], sUb[1][riy][rix], sUb[1][riy][rr], sUb[1][rd][rix],
+                          sUb[2][riy][rix], sUb[2][riy][rr], sUb[2][rd][rix], as1, as0);
+                sPQ[0][riy][rix] = rpx; sPQ[1][riy][rix] = rpy;
+                sPQ[2][riy][rix] = rqx; sPQ[3][riy][rix] = rqy; sPQ[4][riy][rix] = rqz;
+            }
+        }
+        __syncthreads();
+        // r, u of j+1 (the reference's expression order, as u_update), then
+        // the tile's edges of state j+1 for the neighbours
+        const bool publish = j + 1 < R.niters;
+        unsigned long long* xb = R.xch + ((size_t)((j + 1) & 1) * R.ntiles + tile) * TR_XW;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int ty = ty0 + h * SUB, y = y0 + ty;
+            const int ib = ty + 1, jb = tx + 1;
+            const int jl = x > 0 ? jb - 1 : jb, iu = y > 0 ? ib - 1 : ib;
+            const float du = u[h] - u0t[h];
+            float ps = 0.f;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                if (s < R.N) {
+                    rv[h][s] = r_update(rv[h][s], itv[h][s], du, iuv[h][s], sl);
+                    ps += rv[h][s] * iuv[h][s];
+                }
+            }
+            const float c_px = sPQ[0][ib][jb], c_py = sPQ[1][ib][jb];
+            const float un = u[h] - R.tau * (-R.alpha1 * (t11[h] * (c_px - sPQ[0][ib][jl]) + t12[h] * (c_py - sPQ[1][ib][jl]) +
+                                                         t12[h] * (c_px - sPQ[0][iu][jb]) + t22[h] * (c_py - sPQ[1][iu][jb])) +
+                                              R.lambda * ps);
+            const float u1xn = u1x[h] - R.tau * (-R.alpha1 * (t11[h] * c_px + t12[h] * c_py) -
+                                                 R.alpha0 * (sPQ[2][ib][jb] - sPQ[2][ib][jl] + sPQ[4][ib][jb] - sPQ[4][iu][jb]));
+            const float u1yn = u1y[h] - R.tau * (-R.alpha1 * (t12[h] * c_px + t22[h] * c_py) -
+                                                 R.alpha0 * (sPQ[4][ib][jb] - sPQ[4][ib][jl] + sPQ[3][ib][jb] - sPQ[3][iu][jb]));
+            v[h] = u[h]; v1x[h] = u1x[h]; v1y[h] = u1y[h];
+            u[h] = un; u1x[h] = u1xn; u1y[h] = u1yn;
+            if (publish && x < W && y < H && (tx == 0 || tx == TX - 1 || ty == 0 || ty == TY - 1)) {
+                const unsigned tag = (unsigned)(j + 1);
+                const float ub[3] = {2.f * u[h] - v[h], 2.f * u1x[h] - v1x[h], 2.f * u1y[h] - v1y[h]};
+                const float uu[3] = {u[h], u1x[h], u1y[h]};
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    if (ty == 0) {
+                        tr_put(xb + TR_UB_TOP + f * TX + tx, tag, ub[f]);
+                        tr_put(xb + TR_U_TOP + f * TX + tx, tag, uu[f]);
+                    }
+                    if (ty == TY - 1) tr_put(xb + TR_UB_BOT + f * TX + tx, tag, ub[f]);
+                    if (tx == 0) {
+                        tr_put(xb + TR_UB_LEFT + f * TY + ty, tag, ub[f]);
+                        tr_put(xb + TR_U_LEFT + f * TY + ty, tag, uu[f]);
+                    }
+                    if (tx == TX - 1) tr_put(xb + TR_UB_RIGHT + f * TY + ty, tag, ub[f]);
+                }
+            }
+        }
+    }
+    if (failed) return;
+    // the end state k0 + j into its slots
+    const int ke = R.k0 + j;
+    float* U1 = R.U + (size_t)(ke % TG_USLOTS) * 3 * n;
+    float* U0 = R.U + (size_t)((ke + TG_USLOTS - 1) % TG_USLOTS) * 3 * n;
+    float* P = R.P + (size_t)(ke % TG_PSLOTS) * 2 * n;
+    float* Q = R.Q + (size_t)(ke % TG_PSLOTS) * 3 * n;
+    float* Rr = R.Rs + (size_t)(ke % TG_PSLOTS) * R.N * n;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int y = y0 + ty0 + h * SUB;
+        if (x >= W || y >= H) continue;
+        const size_t g = (size_t)y * W + x;
+        U1[g] = u[h]; U1[n + g] = u1x[h]; U1[2 * n + g] = u1y[h];
+        U0[g] = v[h]; U0[n + g] = v1x[h]; U0[2 * n + g] = v1y[h];
+        P[g] = px[h]; P[n + g] = py[h];
+        Q[g] = qx[h]; Q[n + g] = qy[h]; Q[2 * n + g] = qz[h];
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            if (s < R.N) Rr[(size_t)s * n + g] = rv[h][s];
+    }
+}
+
+// Exchange words live in uncached device memory: the L2 of each XCD is not
+// kept coherent with the others for ordinary device memory, and a word
+// stored by a tile on one XCD was seen not to reach a poller on another
+// within seconds.  Uncached memory goes past every L2.  Such memory is
+// never freed once a kernel has used it (later allocations at its address
+// have held wrong values): solvers take buffers from a per-device pool and
+// give them back on destroy.
+struct TrPool {
+    std::mutex mu;
+    std::vector<std::pair<unsigned long long*, size_t>> free_bufs[64];
+};
+static TrPool& tr_pool() {
+    static TrPool* p = new TrPool();   // never destroyed (the buffers outlive it)
+    return *p;
+}
+
+static unsigned long long* tr_take(size_t words, int* dev_out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    TrPool& P = tr_pool();
+    std::lock_guard<std::mutex> lock(P.mu);
+    auto& fl = P.free_bufs[dev];
+    for (size_t i = 0; i < fl.size(); ++i) {
+        if (fl[i].second >= words) {
+            unsigned long long* b = fl[i].first;
+            fl.erase(fl.begin() + (long)i);
+            *dev_out = dev;
+            return b;
+        }
+    }
+    void* b = nullptr;
+    if (hipExtMallocWithFlags(&b, words * sizeof(unsigned long long), hipDeviceMallocUncached) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    *dev_out = dev;
+    return static_cast<unsigned long long*>(b);
+}
+
+static void tr_give(unsigned long long* b, size_t words, int dev) {
+    if (!b || dev < 0 || dev >= 64) return;
+    TrPool& P = tr_pool();
+    std::lock_guard<std::mutex> lock(P.mu);
+    P.free_bufs[dev].emplace_back(b, words);
+}
+
+// Workgroups of the resident kernel the device holds at once: one per CU
+// (per device, cached; 0 if the runtime cannot say or the kernel does not
+// fit a CU), and the tile counts whose launches were called off because not
+// every workgroup became resident in time (those use the launch sequence).
+struct TrSlots {
+    std::mutex mu;
+    int cus[64] = {};
+    std::vector<int> declined[64];
+};
+static TrSlots& tr_slots() {
+    static TrSlots* s = new TrSlots();
+    return *s;
+}
+
+static int tgv_resident_slots(int nt) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+    TrSlots& S = tr_slots();
+    std::lock_guard<std::mutex> lock(S.mu);
+    if (S.cus[dev] == 0) {
+        int per = 0, ncu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)tgv_resident_kernel<TR_MAXN>, TR_NT, 0) !=
+                hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+            (void)hipGetLastError();
+            per = 0;
+        }
+        S.cus[dev] = per >= 1 && ncu > 0 ? ncu : -1;
+    }
+    for (int d : S.declined[dev])
+        if (d == nt) return 0;
+    return S.cus[dev] > 0 ? S.cus[dev] : 0;
+}
+
+static void tgv_resident_decline(int nt) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
+    TrSlots& S = tr_slots();
+    std::lock_guard<std::mutex> lock(S.mu);
+    S.declined[dev].push_back(nt);
+}
+
+static int tgv_resident_tiles(int W, int H) { return ((W + TG_TX - 1) / TG_TX) * ((H + TR_TY - 1) / TR_TY); }
+static int tgv_resident_halves(int W, int H) { return ((W + TG_TX - 1) / TG_TX) * ((H + TR_SUB - 1) / TR_SUB); }
+
+// 64-bit words of the resident solver's exchange buffer: edges, partials,
+// decisions, error words.
+static size_t tgv_resident_words(int W, int H) {
+    const size_t nt = (size_t)tgv_resident_tiles(W, H), nh = (size_t)tgv_resident_halves(W, H);
+    return 2 * nt * TR_XW + 4 * nh + 2 + 4;
+}
+
+// The resident kernel runs when it gives what the default launch sequence
+// gives (PSW_TGV_VARIANT, PSW_TGV_DOUBLE, PSW_TGV2_TY and PSW_TGV_TRIPLE at
+// their defaults: the energy checks then use the same 64 x 10 tile partials),
+// the frame has at most TR_MAXN sources and at most one tile per CU.
+// PSW_TGV_RESIDENT=0 turns it off.
+static bool tgv_resident_use(const psw_tgv* t) {
+    const char* e = getenv("PSW_TGV_RESIDENT");
+    if (e && atoi(e) == 0) return false;
+    if (!t->rx || t->N > TR_MAXN) return false;
+    if (tgv_variant() != 1 || !tgv_double() || tgv_triple() || tgv2_ty() != TR_SUB) return false;
+    const int nt = tgv_resident_tiles(t->W, t->H);
+    return nt <= TR_MAXTILES && nt <= tgv_resident_slots(nt);
+}
+
+// Polls per wait (PSW_TGV_RESIDENT_SPIN; default 2^22, seconds at the
+// latency of a device-scope load; 0 fails every wait), and polls of the
+// start wait (PSW_TGV_RESIDENT_START_SPIN, default 2^16; set, it also keeps
+// a called-off launch from being remembered).
+static unsigned tgv_env_u32(const char* name, unsigned dflt) {
+    const char* e = getenv(name);
+    return e ? (unsigned)strtoul(e, nullptr, 10) : dflt;
+}
+
+// niters iterations from state t->k in one cooperative launch (energy checks
+// every check_every iterations when > 0, as in psw_tgv_solve).  *ran is
+// false, with nothing run or written, when the runtime refuses the
+// cooperative launch or the launch is called off.  Synchronous: it reads
+// the error words.
+static int tgv_resident_run(psw_tgv* t, const psw_tgv_params* prm, int niters, int check_every, double tol,
+                            double e_star, hipStream_t st, bool* ran) {
+    *ran = false;
+    const int tiles_x = (t->W + TG_TX - 1) / TG_TX, tiles_y = (t->H + TR_TY - 1) / TR_TY;
+    const int nt = tiles_x * tiles_y, nh = tgv_resident_halves(t->W, t->H);
+    const size_t nx = (size_t)2 * nt * TR_XW, np = (size_t)4 * nh;
+    // tags that match no iteration, and clear error words
+    int rc = psw_fill_u32(reinterpret_cast<uint32_t*>(t->rx), 0xFFFFFFFFu, 2 * (nx + np + 2), st);
+    unsigned* err = reinterpret_cast<unsigned*>(t->rx + nx + np + 2);
+    if (rc == PSW_OK) rc = psw_fill_u32(err, 0u, 8, st);
+    if (rc != PSW_OK) return rc;
+    const TgvPtrs a = tgv_ptrs(t, prm);
+    TgvResArgs R{};
+    R.ucur = a.ucur; R.uprev = a.uprev; R.pcur = a.pcur; R.qcur = a.qcur; R.rcur = a.rcur;
+    R.It = a.It; R.Iu = a.Iu; R.T = a.T; R.u0 = a.u0;
+    R.mask = t->base + t->L.mask;
+    R.U = t->base + t->L.U; R.P = t->base + t->L.P; R.Q = t->base + t->L.Q; R.Rs = t->base + t->L.r;
+    R.xch = t->rx;
+    R.part = t->rx + nx;
+    R.dec = t->rx + nx + np;
+    R.err = err;
+    R.ctrl = t->ctrl;
+    R.W = t->W; R.H = t->H; R.N = t->N;
+    R.k0 = t->k;
+    R.niters = niters;
+    R.check_every = check_every;
+    R.tiles_x = tiles_x;
+    R.tiles_y = tiles_y;
+    R.ntiles = nt;
+    R.sub_y = (t->H + TR_SUB - 1) / TR_SUB;
+    R.nsub = nh;
+    R.spin = tgv_env_u32("PSW_TGV_RESIDENT_SPIN", 1u << 22);
+    R.start_spin = tgv_env_u32("PSW_TGV_RESIDENT_START_SPIN", 1u << 16);
+    R.lambda = a.lambda; R.alpha0 = a.alpha0; R.alpha1 = a.alpha1; R.tau = a.tau; R.sigma = a.sigma;
+    R.dlambda = prm->lambda;
+    R.dalpha0 = prm->alpha0;
+    R.dalpha1 = prm->alpha1;
+    R.e_star = e_star;
+    R.gap = tol * e_star;
+    void* args[] = {&R};
+    const hipError_t e = hipLaunchCooperativeKernel((const void*)tgv_resident_kernel<TR_MAXN>, dim3((unsigned)nt),
+                                                    dim3(TR_NT), args, 0, st);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return PSW_OK;
+    }
+    static_assert(sizeof(TgvCtrl) >= 8 * sizeof(unsigned), "the error words are read through hctrl");
+    PSW_TRY_HIP(hipMemcpyAsync(t->hctrl, err, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, st), PSW_ERR_COPY);
+    PSW_TRY_HIP(hipStreamSynchronize(st), PSW_ERR_DEVICE);
+    unsigned he[8];
+    std::memcpy(he, t->hctrl, sizeof(he));
+    if (he[2] != 1u) {   // called off before any wait on a neighbour: nothing was written
+        if (!getenv("PSW_TGV_RESIDENT_START_SPIN")) tgv_resident_decline(nt);
+        return PSW_OK;
+    }
+    *ran = true;
+    if (he[0]) {
+        const unsigned blk = (unsigned)nt * TR_XW;
+        if (he[4] < 2 * blk)
+            set_error("resident TGV2 solver: tile %u waited %u polls for word %u of tile %u (iteration tag %u, saw %u) "
+                      "at iteration %u; the state is undefined",
+                      he[3], R.spin, he[4] % TR_XW, (he[4] % blk) / TR_XW, he[5], he[6], he[7]);
+        else
+            set_error("resident TGV2 solver: tile %u waited %u polls for an energy word (offset %u, check %u, saw %u) "
+                      "at iteration %u; the state is undefined",
+                      he[3], R.spin, he[4], he[5], he[6], he[7]);
+        return PSW_ERR_DEVICE;
+    }
+    return PSW_OK;
+}
+
 static int tgv_reset_ctrl(psw_tgv* t, hipStream_t st) {
     tgv_ctrl_reset_kernel<<<1, 1, 0, st>>>(t->ctrl);
     return launch_status("tgv_ctrl_reset_kernel");
@@ -1628,8 +2338,11 @@ extern "C" {
 size_t psw_tgv_state_bytes(int width, int height, int nsrc) {
     TgvLayout L;
     L.init(width, height, nsrc);
+    const size_t rx = nsrc <= TR_MAXN && tgv_resident_tiles(width, height) <= TR_MAXTILES
+                          ? tgv_resident_words(width, height) * sizeof(unsigned long long)
+                          : 0;
     return L.total * sizeof(float) +
-           2 * std::max(TG_ENERGY_BLOCKS, tgv_tile_partials(width, height)) * sizeof(double) + sizeof(TgvCtrl);
+           2 * std::max(TG_ENERGY_BLOCKS, tgv_tile_partials(width, height)) * sizeof(double) + sizeof(TgvCtrl) + rx;
 }
 
 int psw_tgv_create(psw_tgv** out, int width, int height, int nsrc) {
@@ -1651,6 +2364,10 @@ int psw_tgv_create(psw_tgv** out, int width, int height, int nsrc) {
     if (e == hipSuccess) e = hipMalloc((void**)&t->partials, np * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void**)&t->ctrl, sizeof(TgvCtrl));
     if (e == hipSuccess) e = hipMalloc((void**)&t->masked, sizeof(unsigned long long));
+    if (e == hipSuccess && nsrc <= TR_MAXN && tgv_resident_tiles(width, height) <= TR_MAXTILES) {
+        t->rx_words = tgv_resident_words(width, height);
+        t->rx = tr_take(t->rx_words, &t->rx_dev);   // NULL: the solver uses the launch sequence
+    }
     if (e == hipSuccess) e = hipHostMalloc((void**)&t->hctrl, sizeof(TgvCtrl), hipHostMallocDefault);
     if (e == hipSuccess)
         e = hipHostMalloc((void**)&t->hmirror, sizeof(TgvCtrl), hipHostMallocMapped | hipHostMallocCoherent);
@@ -1679,6 +2396,7 @@ int psw_tgv_destroy(psw_tgv* t) {
     if (t->partials) (void)hipFree(t->partials);
     if (t->ctrl) (void)hipFree(t->ctrl);
     if (t->masked) (void)hipFree(t->masked);
+    if (t->rx) tr_give(t->rx, t->rx_words, t->rx_dev);
     if (t->hctrl) (void)hipHostFree(t->hctrl);
     if (t->hmirror) (void)hipHostFree(t->hmirror);
     delete t;
@@ -1816,6 +2534,15 @@ int psw_tgv_set_energy_mask(psw_tgv* t, double qmax, const psw_tgv_params* prm, 
 int psw_tgv_iterate(psw_tgv* t, int niters, const psw_tgv_params* prm, psw_stream_t s) {
     PSW_CHECK_ARG(t && prm, "NULL pointer");
     PSW_CHECK_ARG(niters >= 0, "niters must be >= 0");
+    if (niters >= TR_MIN_ITERS && tgv_resident_use(t)) {
+        bool ran = false;
+        const int rc = tgv_resident_run(t, prm, niters, 0, 0.0, 0.0, S(s), &ran);
+        if (rc != PSW_OK) return rc;
+        if (ran) {
+            t->k += niters;
+            return PSW_OK;
+        }
+    }
     return tgv_launch_iters(t, niters, prm, S(s));
 }
 
@@ -1837,6 +2564,28 @@ int psw_tgv_solve(psw_tgv* t, int max_iters, int check_every, double tol, double
     int rc = tgv_reset_ctrl(t, st);
     if (rc != PSW_OK) return rc;
     const int k_start = t->k;
+    if (max_iters >= 1 && tgv_resident_use(t)) {
+        // the whole solve in one resident launch; the state it ends at (the
+        // stop, else k_start + max_iters) is tested afterwards as below
+        bool ran = false;
+        rc = tgv_resident_run(t, prm, max_iters, check_every, tol, e_star, st, &ran);
+        if (rc != PSW_OK) return rc;
+        if (ran) {
+            PSW_TRY_HIP(hipMemcpy(t->hctrl, t->ctrl, sizeof(TgvCtrl), hipMemcpyDeviceToHost), PSW_ERR_COPY);
+            if (t->hctrl->done) {
+                t->k = t->hctrl->iters_at_done;
+            } else {
+                t->k = k_start + max_iters;
+                rc = tgv_launch_energy(t, prm, tol, e_star, st);
+                if (rc != PSW_OK) return rc;
+                PSW_TRY_HIP(hipStreamSynchronize(st), PSW_ERR_DEVICE);
+                PSW_TRY_HIP(hipMemcpy(t->hctrl, t->ctrl, sizeof(TgvCtrl), hipMemcpyDeviceToHost), PSW_ERR_COPY);
+            }
+            if (iters_done) *iters_done = t->k - k_start;
+            if (energy_out) *energy_out = t->hctrl->done ? t->hctrl->energy_at_done : t->hctrl->energy;
+            return tgv_reset_ctrl(t, st) == PSW_OK && hipStreamSynchronize(st) == hipSuccess ? PSW_OK : PSW_ERR_DEVICE;
+        }
+    }
     std::memset(t->hmirror, 0, sizeof(TgvCtrl));   // no launch of this solver writes it now
     volatile const TgvCtrl* mirror = t->hmirror;
     const int half = std::max(TG_ENERGY_BLOCKS, tgv_tile_partials(t->W, t->H));

@@ -57,6 +57,7 @@ class PswRoom(ctypes.Structure):
 
 
 PSW_OK = 0
+PSW_ERR_ARG, PSW_ERR_LAUNCH, PSW_ERR_ALLOC, PSW_ERR_DEVICE, PSW_ERR_COPY, PSW_ERR_UNSUPPORTED = 1, 2, 3, 4, 5, 6
 PSW_MAX_SOURCES = 16
 
 D = c_void_p  # device pointer
