@@ -663,11 +663,28 @@ def main():
     pl.tgv.set_u(d_s, stream=s)
     pl.tgv.linearise(pl.K1, pl.Rrel, pl.trel, s)
     nl = 198                     # whole launches of 1, 2 or 3 iterations
+    resident_env = os.environ.get("PSW_TGV_RESIDENT")
+    os.environ["PSW_TGV_RESIDENT"] = "0"       # this measurement is of the launch sequence
     ev[0].record(st)
     pl.tgv.iterate(nl * m, pl.prm, s)
     ev[1].record(st)
     ev[1].synchronize()
     launch_ms = ev[0].elapsed_ms(ev[1]) / nl
+    if resident_env is None:
+        del os.environ["PSW_TGV_RESIDENT"]
+    else:
+        os.environ["PSW_TGV_RESIDENT"] = resident_env
+    # the resident solver (tgv_resident_kernel: one cooperative launch, the
+    # state on chip), the path the solves above took: per iteration over a
+    # long run, the launch sequence beside it
+    pl.tgv.set_u(d_s, stream=s)
+    pl.tgv.linearise(pl.K1, pl.Rrel, pl.trel, s)
+    n_res = 4000
+    ev[0].record(st)
+    pl.tgv.iterate(n_res, pl.prm, s)
+    ev[1].record(st)
+    ev[1].synchronize()
+    resident_us = ev[0].elapsed_ms(ev[1]) * 1e3 / n_res
     compulsory = 4 * tgv_compulsory_floats(m) * W * H
     survey_bytes = 4 * TGV_SURVEY_FLOATS_PER_PX * W * H * m
     achieved = compulsory / (launch_ms * 1e-3) / 1e9
@@ -733,12 +750,21 @@ def main():
                                    f"{survey_bytes / 1e6:.1f} MB per launch "
                                    f"({survey_bytes / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS:.2f} of peak), which "
                                    "counts bytes the fused launch never moves"},
+        "tgv_resident": {"kernel": "tgv_resident_kernel", "us_per_iteration": resident_us,
+                         "launch_sequence_us_per_iteration": launch_ms * 1e3 / m,
+                         "speedup_vs_launch_sequence": launch_ms * 1e3 / m / resident_us,
+                         "iterations_timed": n_res,
+                         "note": "the solves above run this kernel (bit-identical to the launch sequence); it reads "
+                                 "the state from HBM once per solve, so its bound is the per-iteration exchange of "
+                                 "tile edges between workgroups and VALU issue, not HBM; the HBM roofline above is of "
+                                 "the launch sequence's kernel (batched frames, frames above one tile per CU)"},
         "stages": {"sweep_kernel_ms": sweep_kernel_ms, "sweep_tflops_declared": sweep_tflops,
                    "sweep_frac_fp32_peak": sweep_tflops / FP32_PEAK_TFLOPS, "unet_ms": net_ms,
                    "unet_tflops": net_tflops, "unet_frac_bf16_peak": net_tflops / BF16_PEAK_TFLOPS,
                    "unet_fwd_gflop": warmstart.forward_flops(W, H) / 1e9},
     }
     rec["sparse_tgv"] = run_sparse(pl, Stream(), [Event() for _ in range(2)], seeds[0])
+    log(f"TGV2 resident: {resident_us:.3f} us per iteration; launch sequence {launch_ms * 1e3 / m:.3f} us")
     log(f"sparse TGV2: {rec['sparse_tgv']['iter_us']:.2f} us per iteration")
     rec["tvl1"] = run_tvl1(pl, Stream(), [Event() for _ in range(2)])
     log(f"TVL1: {rec['tvl1']['iter_us']:.2f} us per iteration fused, "

@@ -1819,6 +1819,11 @@ __device__ __forceinline__ bool tr_ring(const TgvResArgs& R, int tcx, int tcy, i
     return r < TR_TY && tcx > 0 && gy < R.H;
 }
 
+// NS: the number of sources (the frame's N exactly).  Thread (wave w, lane
+// tx) owns the pixels of rows 2w and 2w+1 in column tx: its own values, and
+// the pair's vertical neighbours, come from registers; each image row is one
+// wave slot in column order, so the energy's per-row butterflies are those
+// of a 640-thread block over a 64 x 10 half tile.
 template <int NS>
 __global__ __launch_bounds__(TR_NT, 3) void tgv_resident_kernel(TgvResArgs R) {
     constexpr int TX = TG_TX, TY = TR_TY, SUB = TR_SUB;
@@ -1826,7 +1831,7 @@ __global__ __launch_bounds__(TR_NT, 3) void tgv_resident_kernel(TgvResArgs R) {
     __shared__ float sPQ[5][TY + 1][TX + 1];   // p, q of j+1: ring 0, own 1..
     __shared__ float sUk[3][TY + 1][TX + 1];   // u_j before a check: own 0.., right / lower halo TY / TX
     __shared__ float sRT[3][TG_TX + TR_TY];    // T at the ring positions (the ring's p, q live in sPQ)
-    __shared__ double sWave[2][SUB];
+    __shared__ double sWave[TY];               // per-row energy sums
     __shared__ int sFlag[2];                   // [0] a wait failed, [1] stop
     // tile 0 gathers the energy partials over sUk (read before the gather)
     static_assert(sizeof(double) * 2 * TR_MAXTILES <= sizeof(float) * 3 * (TR_TY + 1) * (TG_TX + 1), "partials");
@@ -1838,7 +1843,7 @@ __global__ __launch_bounds__(TR_NT, 3) void tgv_resident_kernel(TgvResArgs R) {
     const int tcx = tile % R.tiles_x, tcy = tile / R.tiles_x;
     const int x0 = tcx * TX, y0 = tcy * TY;
     const int tid = threadIdx.x;
-    const int tx = tid % TX, ty0 = tid / TX;   // pixels (tx, ty0) and (tx, ty0 + SUB)
+    const int tx = tid % TX, wv = tid / TX;    // pixels (tx, 2 wv) and (tx, 2 wv + 1) of the tile
     const int x = x0 + tx;
     const float as1 = R.alpha1 * R.sigma, as0 = R.alpha0 * R.sigma, sl = R.sigma * R.lambda;
     if (tid == 0) {
@@ -1859,7 +1864,7 @@ __global__ __launch_bounds__(TR_NT, 3) void tgv_resident_kernel(TgvResArgs R) {
     unsigned counted = 0;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        const int y = y0 + ty0 + h * SUB;
+        const int y = y0 + 2 * wv + h;
         const size_t g = (size_t)min(y, H - 1) * W + min(x, W - 1);
         u[h] = R.ucur[g]; u1x[h] = R.ucur[n + g]; u1y[h] = R.ucur[2 * n + g];
         v[h] = R.uprev[g]; v1x[h] = R.uprev[n + g]; v1y[h] = R.uprev[2 * n + g];
@@ -1870,12 +1875,10 @@ __global__ __launch_bounds__(TR_NT, 3) void tgv_resident_kernel(TgvResArgs R) {
         if (x < W && y < H && R.mask[g] != 0.f) counted |= 1u << h;
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            if (s < R.N) {
-                const size_t gs = (size_t)s * n + g;
-                rv[h][s] = R.rcur[gs];
-                itv[h][s] = R.It[gs];
-                iuv[h][s] = R.Iu[gs];
-            }
+            const size_t gs = (size_t)s * n + g;
+            rv[h][s] = R.rcur[gs];
+            itv[h][s] = R.It[gs];
+            iuv[h][s] = R.Iu[gs];
         }
     }
     // the ring: p, q of state k0 into its sPQ cells (updated there in place
@@ -1924,19 +1927,40 @@ __global__ __launch_bounds__(TR_NT, 3) void tgv_resident_kernel(TgvResArgs R) {
     __syncthreads();
     if (sFlag[0]) return;   // called off: nothing written
 
+    // clamped neighbours (the reference's min(x+1, W-1) and max(x-1, 0))
+    const bool has_r = x < W - 1, has_l = x > 0;
+    const bool has_d0 = y0 + 2 * wv < H - 1, has_d1 = y0 + 2 * wv + 1 < H - 1, has_u0 = y0 + 2 * wv > 0;
     int j = 0, nchk = 0;
     bool failed = false;
     for (;; ++j) {
         const bool check = R.check_every > 0 && j > 0 && j % R.check_every == 0 && j < R.niters;
         if (j == R.niters && !check) break;
-        // ubar_j (and u_j before a check) of the tile and its halo
+        // r of j+1 and its sum with Iu need only the pixel itself: outside an
+        // energy check (where a stop must keep r_j) they are formed before
+        // the wait for the neighbours' edges, in its shadow
+        float ps[2] = {0.f, 0.f};
+        if (!check) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const float du = u[h] - u0t[h];
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    rv[h][s] = r_update(rv[h][s], itv[h][s], du, iuv[h][s], sl);
+                    ps[h] += rv[h][s] * iuv[h][s];
+                }
+            }
+        }
+        // ubar_j of the pixels (registers and LDS), u_j before a check
+        float ub[2][3];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int ty = ty0 + h * SUB;
-            sUb[0][ty + 1][tx + 1] = 2.f * u[h] - v[h];
-            sUb[1][ty + 1][tx + 1] = 2.f * u1x[h] - v1x[h];
-            sUb[2][ty + 1][tx + 1] = 2.f * u1y[h] - v1y[h];
-            if (check) { sUk[0][ty][tx] = u[h]; sUk[1][ty][tx] = u1x[h]; sUk[2][ty][tx] = u1y[h]; }
+            const int row = 2 * wv + h;
+            ub[h][0] = 2.f * u[h] - v[h];
+            ub[h][1] = 2.f * u1x[h] - v1x[h];
+            ub[h][2] = 2.f * u1y[h] - v1y[h];
+#pragma unroll
+            for (int f = 0; f < 3; ++f) sUb[f][row + 1][tx + 1] = ub[h][f];
+            if (check) { sUk[0][row][tx] = u[h]; sUk[1][row][tx] = u1x[h]; sUk[2][row][tx] = u1y[h]; }
         }
         if (j > 0) {
             unsigned long long* blk = R.xch + (size_t)(j & 1) * R.ntiles * TR_XW;
@@ -1950,23 +1974,22 @@ __global__ __launch_bounds__(TR_NT, 3) void tgv_resident_kernel(TgvResArgs R) {
         __syncthreads();
         if (sFlag[0]) { failed = true; break; }
         if (check) {
-            // energy of state j: the partials of the two 64 x 10 halves (wave
-            // butterflies, then the ten waves in order, as a 640-thread block)
+            // energy of state j: per-row butterflies, then the rows of each
+            // 64 x 10 half tile in order (a 640-thread block's partial)
             double e2[2];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const int ty = ty0 + h * SUB, y = y0 + ty;
+                const int row = 2 * wv + h, y = y0 + row;
                 e2[h] = 0.0;
                 if (counted & (1u << h)) {
                     const double dud = (double)u[h] - (double)u0t[h];
                     double ed = 0.0;
 #pragma unroll
-                    for (int s = 0; s < NS; ++s)
-                        if (s < R.N) ed += fabs((double)itv[h][s] + dud * (double)iuv[h][s]);
-                    const int kr = x < W - 1 ? tx + 1 : tx, kd = y < H - 1 ? ty + 1 : ty;
-                    e2[h] = energy_point(u[h], sUk[0][ty][kr], sUk[0][kd][tx], u1x[h], sUk[1][ty][kr], sUk[1][kd][tx],
-                                         u1y[h], sUk[2][ty][kr], sUk[2][kd][tx], t11[h], t12[h], t22[h], ed,
-                                         R.dlambda, R.dalpha0, R.dalpha1);
+                    for (int s = 0; s < NS; ++s) ed += fabs((double)itv[h][s] + dud * (double)iuv[h][s]);
+                    const int kr = has_r ? tx + 1 : tx, kd = y < H - 1 ? row + 1 : row;
+                    e2[h] = energy_point(u[h], sUk[0][row][kr], sUk[0][kd][tx], u1x[h], sUk[1][row][kr],
+                                         sUk[1][kd][tx], u1y[h], sUk[2][row][kr], sUk[2][kd][tx], t11[h], t12[h],
+                                         t22[h], ed, R.dlambda, R.dalpha0, R.dalpha1);
                 }
             }
 #pragma unroll
@@ -1974,14 +1997,14 @@ __global__ __launch_bounds__(TR_NT, 3) void tgv_resident_kernel(TgvResArgs R) {
                 e2[0] += __shfl_xor(e2[0], off, 64);
                 e2[1] += __shfl_xor(e2[1], off, 64);
             }
-            if (tx == 0) { sWave[0][ty0] = e2[0]; sWave[1][ty0] = e2[1]; }
+            if (tx == 0) { sWave[2 * wv] = e2[0]; sWave[2 * wv + 1] = e2[1]; }
             __syncthreads();
             ++nchk;
             if (tid < 2) {
                 const int sy = 2 * tcy + tid;   // the half tile's row in the launch sequence's tiles
                 if (sy < R.sub_y) {
                     double e = 0.0;
-                    for (int k = 0; k < SUB; ++k) e += sWave[tid][k];
+                    for (int k = 0; k < SUB; ++k) e += sWave[tid * SUB + k];
                     const unsigned long long eb = (unsigned long long)__double_as_longlong(e);
                     unsigned long long* pw = R.part + ((size_t)(nchk & 1) * R.nsub + sy * R.tiles_x + tcx) * 2;
                     __hip_atomic_store(pw, tr_word((unsigned)nchk, (unsigned)(eb >> 32)), __ATOMIC_RELAXED,
@@ -2030,17 +2053,22 @@ __global__ __launch_bounds__(TR_NT, 3) void tgv_resident_kernel(TgvResArgs R) {
             if (sFlag[0]) { failed = true; break; }
             if (sFlag[1]) break;
         }
-        // p, q of j+1 on the tile and the ring
+        // p, q of j+1 on the pixels (the lower pixel of the pair is the upper
+        // one's down neighbour) and the ring
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int ty = ty0 + h * SUB, y = y0 + ty;
-            const int ib = ty + 1, jb = tx + 1;
-            const int jr = x < W - 1 ? jb + 1 : jb, id = y < H - 1 ? ib + 1 : ib;
-            pq_update(px[h], py[h], qx[h], qy[h], qz[h], t11[h], t12[h], t22[h], sUb[0][ib][jb], sUb[0][ib][jr],
-                      sUb[0][id][jb], sUb[1][ib][jb], sUb[1][ib][jr], sUb[1][id][jb], sUb[2][ib][jb], sUb[2][ib][jr],
-                      sUb[2][id][jb], as1, as0);
-            sPQ[0][ib][jb] = px[h]; sPQ[1][ib][jb] = py[h];
-            sPQ[2][ib][jb] = qx[h]; sPQ[3][ib][jb] = qy[h]; sPQ[4][ib][jb] = qz[h];
+            const int row = 2 * wv + h;
+            const bool has_d = h == 0 ? has_d0 : has_d1;
+            float rt[3], dn[3];
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                rt[f] = has_r ? sUb[f][row + 1][tx + 2] : ub[h][f];
+                dn[f] = !has_d ? ub[h][f] : (h == 0 ? ub[1][f] : sUb[f][row + 2][tx + 1]);
+            }
+            pq_update(px[h], py[h], qx[h], qy[h], qz[h], t11[h], t12[h], t22[h], ub[h][0], rt[0], dn[0], ub[h][1],
+                      rt[1], dn[1], ub[h][2], rt[2], dn[2], as1, as0);
+            sPQ[0][row + 1][tx + 1] = px[h]; sPQ[1][row + 1][tx + 1] = py[h];
+            sPQ[2][row + 1][tx + 1] = qx[h]; sPQ[3][row + 1][tx + 1] = qy[h]; sPQ[4][row + 1][tx + 1] = qz[h];
         }
         {
             int rgx, rgy, rix, riy;
@@ -2056,50 +2084,55 @@ __global__ __launch_bounds__(TR_NT, 3) void tgv_resident_kernel(TgvResArgs R) {
             }
         }
         __syncthreads();
-        // r, u of j+1 (the reference's expression order, as u_update), then
-        // the tile's edges of state j+1 for the neighbours
+        // r (at a check) and u of j+1 (the reference's expression order, as
+        // u_update; the upper pixel of the pair is the lower one's up
+        // neighbour), then the tile's edges of state j+1 for the neighbours
         const bool publish = j + 1 < R.niters;
         unsigned long long* xb = R.xch + ((size_t)((j + 1) & 1) * R.ntiles + tile) * TR_XW;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int ty = ty0 + h * SUB, y = y0 + ty;
-            const int ib = ty + 1, jb = tx + 1;
-            const int jl = x > 0 ? jb - 1 : jb, iu = y > 0 ? ib - 1 : ib;
-            const float du = u[h] - u0t[h];
-            float ps = 0.f;
+            const int row = 2 * wv + h, y = y0 + row;
+            if (check) {
+                const float du = u[h] - u0t[h];
 #pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                if (s < R.N) {
+                for (int s = 0; s < NS; ++s) {
                     rv[h][s] = r_update(rv[h][s], itv[h][s], du, iuv[h][s], sl);
-                    ps += rv[h][s] * iuv[h][s];
+                    ps[h] += rv[h][s] * iuv[h][s];
                 }
             }
-            const float c_px = sPQ[0][ib][jb], c_py = sPQ[1][ib][jb];
-            const float un = u[h] - R.tau * (-R.alpha1 * (t11[h] * (c_px - sPQ[0][ib][jl]) + t12[h] * (c_py - sPQ[1][ib][jl]) +
-                                                         t12[h] * (c_px - sPQ[0][iu][jb]) + t22[h] * (c_py - sPQ[1][iu][jb])) +
-                                              R.lambda * ps);
+            const float l_px = has_l ? sPQ[0][row + 1][tx] : px[h], l_py = has_l ? sPQ[1][row + 1][tx] : py[h];
+            const float l_qx = has_l ? sPQ[2][row + 1][tx] : qx[h], l_qz = has_l ? sPQ[4][row + 1][tx] : qz[h];
+            const bool has_u = h == 1 || has_u0;
+            const float u_px = !has_u ? px[h] : (h == 1 ? px[0] : sPQ[0][row][tx + 1]);
+            const float u_py = !has_u ? py[h] : (h == 1 ? py[0] : sPQ[1][row][tx + 1]);
+            const float u_qy = !has_u ? qy[h] : (h == 1 ? qy[0] : sPQ[3][row][tx + 1]);
+            const float u_qz = !has_u ? qz[h] : (h == 1 ? qz[0] : sPQ[4][row][tx + 1]);
+            const float c_px = px[h], c_py = py[h];
+            const float un = u[h] - R.tau * (-R.alpha1 * (t11[h] * (c_px - l_px) + t12[h] * (c_py - l_py) +
+                                                         t12[h] * (c_px - u_px) + t22[h] * (c_py - u_py)) +
+                                              R.lambda * ps[h]);
             const float u1xn = u1x[h] - R.tau * (-R.alpha1 * (t11[h] * c_px + t12[h] * c_py) -
-                                                 R.alpha0 * (sPQ[2][ib][jb] - sPQ[2][ib][jl] + sPQ[4][ib][jb] - sPQ[4][iu][jb]));
+                                                 R.alpha0 * (qx[h] - l_qx + qz[h] - u_qz));
             const float u1yn = u1y[h] - R.tau * (-R.alpha1 * (t12[h] * c_px + t22[h] * c_py) -
-                                                 R.alpha0 * (sPQ[4][ib][jb] - sPQ[4][ib][jl] + sPQ[3][ib][jb] - sPQ[3][iu][jb]));
+                                                 R.alpha0 * (qz[h] - l_qz + qy[h] - u_qy));
             v[h] = u[h]; v1x[h] = u1x[h]; v1y[h] = u1y[h];
             u[h] = un; u1x[h] = u1xn; u1y[h] = u1yn;
-            if (publish && x < W && y < H && (tx == 0 || tx == TX - 1 || ty == 0 || ty == TY - 1)) {
+            if (publish && x < W && y < H && (tx == 0 || tx == TX - 1 || row == 0 || row == TY - 1)) {
                 const unsigned tag = (unsigned)(j + 1);
-                const float ub[3] = {2.f * u[h] - v[h], 2.f * u1x[h] - v1x[h], 2.f * u1y[h] - v1y[h]};
+                const float pub[3] = {2.f * u[h] - v[h], 2.f * u1x[h] - v1x[h], 2.f * u1y[h] - v1y[h]};
                 const float uu[3] = {u[h], u1x[h], u1y[h]};
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
-                    if (ty == 0) {
-                        tr_put(xb + TR_UB_TOP + f * TX + tx, tag, ub[f]);
+                    if (row == 0) {
+                        tr_put(xb + TR_UB_TOP + f * TX + tx, tag, pub[f]);
                         tr_put(xb + TR_U_TOP + f * TX + tx, tag, uu[f]);
                     }
-                    if (ty == TY - 1) tr_put(xb + TR_UB_BOT + f * TX + tx, tag, ub[f]);
+                    if (row == TY - 1) tr_put(xb + TR_UB_BOT + f * TX + tx, tag, pub[f]);
                     if (tx == 0) {
-                        tr_put(xb + TR_UB_LEFT + f * TY + ty, tag, ub[f]);
-                        tr_put(xb + TR_U_LEFT + f * TY + ty, tag, uu[f]);
+                        tr_put(xb + TR_UB_LEFT + f * TY + row, tag, pub[f]);
+                        tr_put(xb + TR_U_LEFT + f * TY + row, tag, uu[f]);
                     }
-                    if (tx == TX - 1) tr_put(xb + TR_UB_RIGHT + f * TY + ty, tag, ub[f]);
+                    if (tx == TX - 1) tr_put(xb + TR_UB_RIGHT + f * TY + row, tag, pub[f]);
                 }
             }
         }
@@ -2111,10 +2144,10 @@ __global__ __launch_bounds__(TR_NT, 3) void tgv_resident_kernel(TgvResArgs R) {
     float* U0 = R.U + (size_t)((ke + TG_USLOTS - 1) % TG_USLOTS) * 3 * n;
     float* P = R.P + (size_t)(ke % TG_PSLOTS) * 2 * n;
     float* Q = R.Q + (size_t)(ke % TG_PSLOTS) * 3 * n;
-    float* Rr = R.Rs + (size_t)(ke % TG_PSLOTS) * R.N * n;
+    float* Rr = R.Rs + (size_t)(ke % TG_PSLOTS) * NS * n;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        const int y = y0 + ty0 + h * SUB;
+        const int y = y0 + 2 * wv + h;
         if (x >= W || y >= H) continue;
         const size_t g = (size_t)y * W + x;
         U1[g] = u[h]; U1[n + g] = u1x[h]; U1[2 * n + g] = u1y[h];
@@ -2122,18 +2155,17 @@ __global__ __launch_bounds__(TR_NT, 3) void tgv_resident_kernel(TgvResArgs R) {
         P[g] = px[h]; P[n + g] = py[h];
         Q[g] = qx[h]; Q[n + g] = qy[h]; Q[2 * n + g] = qz[h];
 #pragma unroll
-        for (int s = 0; s < NS; ++s)
-            if (s < R.N) Rr[(size_t)s * n + g] = rv[h][s];
+        for (int s = 0; s < NS; ++s) Rr[(size_t)s * n + g] = rv[h][s];
     }
 }
 
-// Exchange words live in uncached device memory: the L2 of each XCD is not
-// kept coherent with the others for ordinary device memory, and a word
-// stored by a tile on one XCD was seen not to reach a poller on another
-// within seconds.  Uncached memory goes past every L2.  Such memory is
-// never freed once a kernel has used it (later allocations at its address
-// have held wrong values): solvers take buffers from a per-device pool and
-// give them back on destroy.
+// Exchange words live in uncached device memory, so that a word a tile on
+// one XCD stores never waits in that XCD's L2 for a poller on another
+// (ordinary device memory with device-scope atomics measured the same
+// speed).  Uncached memory is never freed once a kernel has used it (later
+// allocations at its address have held wrong values on these systems):
+// solvers take buffers from a per-device pool and give them back on
+// destroy.
 struct TrPool {
     std::mutex mu;
     std::vector<std::pair<unsigned long long*, size_t>> free_bufs[64];
@@ -2293,8 +2325,18 @@ static int tgv_resident_run(psw_tgv* t, const psw_tgv_params* prm, int niters, i
     R.e_star = e_star;
     R.gap = tol * e_star;
     void* args[] = {&R};
-    const hipError_t e = hipLaunchCooperativeKernel((const void*)tgv_resident_kernel<TR_MAXN>, dim3((unsigned)nt),
-                                                    dim3(TR_NT), args, 0, st);
+    const void* kern = nullptr;
+    switch (t->N) {
+        case 1: kern = (const void*)tgv_resident_kernel<1>; break;
+        case 2: kern = (const void*)tgv_resident_kernel<2>; break;
+        case 3: kern = (const void*)tgv_resident_kernel<3>; break;
+        case 4: kern = (const void*)tgv_resident_kernel<4>; break;
+        case 5: kern = (const void*)tgv_resident_kernel<5>; break;
+        case 6: kern = (const void*)tgv_resident_kernel<6>; break;
+        case 7: kern = (const void*)tgv_resident_kernel<7>; break;
+        default: kern = (const void*)tgv_resident_kernel<8>; break;
+    }
+    const hipError_t e = hipLaunchCooperativeKernel(kern, dim3((unsigned)nt), dim3(TR_NT), args, 0, st);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return PSW_OK;
