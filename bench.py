@@ -347,14 +347,23 @@ def probe(names, seed):
             a.fill(1.0, s)
             for _ in range(5):
                 psw_amd._lib.call("psw_copy_f32", b.ptr, a.ptr, n, s)
-        elif name in ("tgv", "sweep", "sweep_exact", "unet"):
+        elif name in ("tgv", "tgv_resident", "sweep", "sweep_exact", "unet"):
             if pl is None:
                 pl = Pipeline(seed, warmstart.DEFAULT_WEIGHTS if name == "unet" else None)
-            if name == "tgv":
+            if name in ("tgv", "tgv_resident"):
+                # tgv: 100 iterations on the launch sequence (50 two-iteration
+                # launches); tgv_resident: 1000 in one resident launch
                 d_s = pl.sweep_and_linearise()
                 pl.tgv.set_u(d_s, stream=pl.st.handle)
                 pl.tgv.linearise(pl.K1, pl.Rrel, pl.trel, pl.st.handle)
-                pl.tgv.iterate(100, pl.prm, pl.st.handle)
+                saved = os.environ.get("PSW_TGV_RESIDENT")
+                os.environ["PSW_TGV_RESIDENT"] = "1" if name == "tgv_resident" else "0"
+                pl.tgv.iterate(1000 if name == "tgv_resident" else 100, pl.prm, pl.st.handle)
+                if saved is None:
+                    del os.environ["PSW_TGV_RESIDENT"]
+                else:
+                    os.environ["PSW_TGV_RESIDENT"] = saved
+                pl.st.synchronize()
             elif name == "unet":
                 if pl.ws is None:
                     pl.ws = warmstart.WarmStarter(W, H, warmstart.DEFAULT_WEIGHTS)
@@ -521,7 +530,7 @@ def main():
     ap.add_argument("--batch", action="store_true", help="run the uneven 32-frame batch instead (its own JSON line)")
     ap.add_argument("--batch-frames", type=int, default=32)
     ap.add_argument("--batch-estar-iters", type=int, default=20000)
-    ap.add_argument("--probe", default="", help="comma list of tgv,sweep,sweep_exact,sweep_shapes,unet,copy,fusion,"
+    ap.add_argument("--probe", default="", help="comma list of tgv,tgv_resident,sweep,sweep_exact,sweep_shapes,unet,copy,fusion,"
                                                 "sparse,tvl1: run only these small workloads (for rocprofv3) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
