@@ -2779,7 +2779,7 @@ int psw_tgv_run(psw_tgv* t, const float* K, const float* Rrel, const float* trel
     for (int l = 0; l < warps; ++l) {
         st = psw_tgv_linearise(t, K, Rrel, trel, s);
         if (st != PSW_OK) return st;
-        st = tgv_launch_iters(t, niters, prm, S(s));
+        st = psw_tgv_iterate(t, niters, prm, s);   // resident when the frame allows it
         if (st != PSW_OK) return st;
     }
     return psw_tgv_get_u(t, d_u_out, s);
