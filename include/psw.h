@@ -269,16 +269,17 @@ int psw_sweep_argmin_depth(float* d_depth, const int32_t* d_argmin, const float*
 
 /* psw_sweep_fused_aggregate without any device allocation.  No counterpart
  * in the reference: for callers that cannot allocate device memory (code
- * under hipGraph capture, tight serving loops).  The plane-group partials go
- * to a static device array of PSW_SWEEP_SCRATCH_BYTES (32 MiB, held by every
- * GPU the library runs on until the process ends) instead of a per-call
- * allocation; outputs are bit-identical to psw_sweep_fused_aggregate.
- * groups: plane groups (0 = the library's choice, capped to what the scratch
- * holds; a count whose partials, groups * width * height * 12 bytes, exceed
- * the scratch is rejected with PSW_ERR_ARG; 1 needs no scratch).  Every call
- * on one GPU uses that GPU's one array: calls that overlap in time on the
- * same GPU (different streams or threads) race.  Each call writes every
- * partial it reads, so nothing needs clearing between calls. */
+ * under hipGraph capture, tight serving loops).  The plane groups'
+ * per-pixel keys (16 bytes per pixel at any group count) go to a static
+ * device array of PSW_SWEEP_SCRATCH_BYTES (32 MiB, held by every GPU the
+ * library runs on until the process ends: frames up to 2^21 pixels) instead
+ * of a per-call allocation; outputs are bit-identical to
+ * psw_sweep_fused_aggregate.  groups: plane groups (0 = the library's choice,
+ * 1 for frames whose keys exceed the scratch; a count above 1 for such a
+ * frame is rejected with PSW_ERR_ARG; 1 needs no scratch).  Every call on
+ * one GPU uses that GPU's one array: calls that overlap in time on the same
+ * GPU (different streams or threads) race.  Each call clears the keys it
+ * uses (with a fill kernel), so nothing needs clearing between calls. */
 #define PSW_SWEEP_SCRATCH_BYTES (32u << 20)
 int psw_sweep_fused_aggregate_noalloc(const float* d_ref, const float* d_src, int width, int height, int nsrc,
                                       const float* d_H, int nplanes, int winsize, float stdthresh, int coord_offset,

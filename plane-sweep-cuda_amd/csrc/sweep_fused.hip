@@ -55,13 +55,46 @@ struct SweepArgs {
     int G;                // plane groups (aggregate mode): workgroups per tile
     float stdthresh, nccthresh, zfar;
     float* depth_out;     // reference mode
-    int32_t* argmin;      // aggregate mode (G == 1) ...
+    int32_t* argmin;      // aggregate mode: written directly when G == 1 ...
     float* cmin;
     float* csec;
-    int32_t* ws_arg;      // ... or per-group partials [G][H*W] (G > 1)
-    float* ws_cmin;
-    float* ws_csec;
+    unsigned long long* top2;  // ... else every group's best two go to these per-pixel keys [2][H*W]
 };
+
+// The aggregated sweep's plane groups of one tile run in different
+// workgroups.  Each folds its best (cost, plane) and its second-best cost
+// into two 64-bit words per pixel with atomic minima: the key is the cost's
+// order-preserving bits in the high half and the plane in the low half, so
+// the smallest key is the smallest cost at its first plane, which is what the
+// sequential strict-< scan over all planes picks.  -0 is folded into +0 so
+// that equal costs are equal keys (the output then reads +0 where the scan
+// could read -0: the same value).
+__device__ __forceinline__ unsigned long long sweep_key(float c, int plane) {
+    const unsigned u = __float_as_uint(c + 0.0f);
+    const unsigned o = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    return ((unsigned long long)o << 32) | (unsigned)plane;
+}
+
+__device__ __forceinline__ float sweep_key_cost(unsigned long long k) {
+    const unsigned o = (unsigned)(k >> 32);
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+}
+
+// One group's best (c1 at plane p1) and second-best c2 of pixel i into the
+// keys: the first word keeps the smallest key of all groups; every other
+// key offered ends in the second word (a key displaced from the first word
+// is offered there), so the second word keeps the second-smallest key of the
+// whole plane loop.  Costs that are not below +inf count as absent, as in
+// the scan.
+__device__ __forceinline__ void sweep_top2_insert(unsigned long long* top2, size_t npix, size_t i, float c1, int p1,
+                                                  float c2) {
+    const float inf = __builtin_inff();
+    if (!(c1 < inf)) return;
+    const unsigned long long k1 = sweep_key(c1, p1);
+    const unsigned long long old = atomicMin(&top2[i], k1);
+    atomicMin(&top2[npix + i], k1 < old ? old : k1);
+    if (c2 < inf) atomicMin(&top2[npix + i], sweep_key(c2, p1));
+}
 
 template <bool FAST>
 __device__ inline float scale_mean(float s, float win, float rwin) {
@@ -348,22 +381,18 @@ __device__ __forceinline__ void sweep_item(const SweepArgs& a, const int item) {
                 else if (c < csec[k]) csec[k] = c;
             }
         }
-        int32_t* oa = a.argmin;
-        float* oc = a.cmin;
-        float* os = a.csec;
-        if (a.G > 1) {
-            oa = a.ws_arg + (size_t)g * npix;
-            oc = a.ws_cmin + (size_t)g * npix;
-            os = a.ws_csec + (size_t)g * npix;
-        }
 #pragma unroll
         for (int k = 0; k < SW_PX; ++k) {
             const int x = x0 + rox + k, y = y0 + roy;
             if (x < W && y < H) {
                 const size_t i = (size_t)y * W + x;
-                oa[i] = arg[k];
-                oc[i] = cmin[k];
-                os[i] = csec[k];
+                if (a.G > 1) {
+                    sweep_top2_insert(a.top2, npix, i, cmin[k], arg[k], csec[k]);
+                } else {
+                    a.argmin[i] = arg[k];
+                    a.cmin[i] = cmin[k];
+                    a.csec[i] = csec[k];
+                }
             }
         }
     }
@@ -401,32 +430,17 @@ __global__ __launch_bounds__(SW_THREADS) void sweep_batch_kernel(const SweepArgs
     }
 }
 
-// Merge of the plane-group partials in plane order.  For groups A (lower
-// planes) then B: if B.min < A.min the argmin moves to B and the second best
-// is min(A.min, B.second), else it stays and the second best is
-// min(A.second, B.min) -- exactly what the sequential strict-< scan over all
-// planes produces.
-__global__ void sweep_merge_kernel(const float* __restrict__ wc, const int32_t* __restrict__ wa,
-                                   const float* __restrict__ ws, int G, size_t npix, int32_t* __restrict__ arg,
-                                   float* __restrict__ cmin, float* __restrict__ csec) {
+// The outputs from the per-pixel keys of the plane groups (all-ones: no
+// finite cost, as the scan's -1 / +inf / +inf).
+__global__ void sweep_top2_kernel(const unsigned long long* __restrict__ top2, size_t npix, int32_t* __restrict__ arg,
+                                  float* __restrict__ cmin, float* __restrict__ csec) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npix) return;
-    float c = wc[i], s = ws[i];
-    int a = wa[i];
-    for (int g = 1; g < G; ++g) {
-        const float c2 = wc[(size_t)g * npix + i], s2 = ws[(size_t)g * npix + i];
-        const int a2 = wa[(size_t)g * npix + i];
-        if (c2 < c) {
-            s = (s2 < c) ? s2 : c;
-            c = c2;
-            a = a2;
-        } else if (c2 < s) {
-            s = c2;
-        }
-    }
-    arg[i] = a;
-    cmin[i] = c;
-    csec[i] = s;
+    const unsigned long long k0 = top2[i], k1 = top2[npix + i];
+    const float inf = __builtin_inff();
+    arg[i] = k0 == ~0ull ? -1 : (int32_t)(unsigned)k0;
+    cmin[i] = k0 == ~0ull ? inf : sweep_key_cost(k0);
+    csec[i] = k1 == ~0ull ? inf : sweep_key_cost(k1);
 }
 
 // ---- FP64 homographies -----------------------------------------------------
@@ -544,8 +558,10 @@ static int launch_sweep(SweepArgs a, int winsize, int mode, bool fast, hipStream
 }
 
 // Tiles of the image, and plane groups per tile so that a launch has enough
-// workgroups to fill 256 CUs (about 3000, i.e. two rounds at six resident
-// workgroups per CU).  PSW_SWEEP_GROUPS overrides the choice.
+// workgroups to keep 256 CUs busy to the end (about 14400 at 32 x 16 tiles:
+// 24 groups of the 600 tiles of a 640 x 480 frame, measured the fastest of
+// 1..64 groups).  PSW_SWEEP_GROUPS overrides the choice.
+constexpr long SW_TARGET_ITEMS = 14400;
 // The aggregated sweep's tile height: PSW_SWEEP_TY (16 or 32) or 16.
 static int sweep_tile_height() {
     const char* e = getenv("PSW_SWEEP_TY");
@@ -559,7 +575,7 @@ static void sweep_layout(SweepArgs& a, bool allow_groups, int ty = SW_TY) {
     a.ntiles = a.tiles_x * tiles_y;
     a.G = 1;
     if (!allow_groups) return;
-    int G = (3072 * 16 / a.ty + a.ntiles / 2) / a.ntiles;
+    int G = (int)((SW_TARGET_ITEMS * 16 / a.ty + a.ntiles / 2) / a.ntiles);
     if (const char* e = getenv("PSW_SWEEP_GROUPS")) G = atoi(e);
     a.G = std::max(1, std::min(G, std::min(a.P, 64)));
 }
@@ -583,9 +599,9 @@ static int check_sweep_args(const float* d_ref, const float* d_src, int width, i
     return PSW_OK;
 }
 
-// Device memory of psw_sweep_fused_aggregate_noalloc: the plane-group
-// partials (12 bytes per pixel and group) live in this static array, one
-// copy per GPU for the life of the process, instead of a per-call
+// Device memory of psw_sweep_fused_aggregate_noalloc: the plane groups'
+// per-pixel keys (16 bytes per pixel at any group count) live in this static
+// array, one copy per GPU for the life of the process, instead of a per-call
 // allocation.  Host code never uses the name itself (on the host it is a
 // stand-in, not a device address): scratch_address() looks up this GPU's
 // copy once with hipGetSymbolAddress and keeps it per device.
@@ -610,20 +626,21 @@ static int scratch_address(void** out) {
     return PSW_OK;
 }
 
-// The aggregated sweep's launch and, with plane groups, the merge of the
-// partials in `ws` (G * npix * 12 bytes).
+// The aggregated sweep's launch and, with plane groups, the per-pixel keys
+// in `ws` (SW_KEY_BYTES per pixel: cleared first, turned into the outputs
+// after).
+constexpr size_t SW_KEY_BYTES = 2 * sizeof(unsigned long long);
 static int sweep_aggregate_run(SweepArgs& a, int winsize, int fast, void* ws, hipStream_t st) {
     const size_t npix = (size_t)a.W * a.H;
+    int rc = PSW_OK;
     if (a.G > 1) {
-        a.ws_cmin = static_cast<float*>(ws);
-        a.ws_csec = a.ws_cmin + (size_t)a.G * npix;
-        a.ws_arg = reinterpret_cast<int32_t*>(a.ws_csec + (size_t)a.G * npix);
+        a.top2 = static_cast<unsigned long long*>(ws);
+        rc = psw_fill_u32(reinterpret_cast<uint32_t*>(ws), 0xFFFFFFFFu, 4 * npix, st);
     }
-    int rc = launch_sweep(a, winsize, 1, fast != 0, st);
+    if (rc == PSW_OK) rc = launch_sweep(a, winsize, 1, fast != 0, st);
     if (rc == PSW_OK && a.G > 1) {
-        sweep_merge_kernel<<<(unsigned)((npix + 255) / 256), 256, 0, st>>>(a.ws_cmin, a.ws_arg, a.ws_csec, a.G, npix,
-                                                                         a.argmin, a.cmin, a.csec);
-        rc = launch_status("sweep_merge_kernel");
+        sweep_top2_kernel<<<(unsigned)((npix + 255) / 256), 256, 0, st>>>(a.top2, npix, a.argmin, a.cmin, a.csec);
+        rc = launch_status("sweep_top2_kernel");
     }
     return rc;
 }
@@ -691,8 +708,8 @@ int psw_sweep_fused_aggregate(const float* d_ref, const float* d_src, int width,
     sweep_layout(a, true, sweep_tile_height());
     const size_t npix = (size_t)width * height;
     void* ws = nullptr;
-    if (a.G > 1)   // stream-ordered scratch for the per-group partials (pooled by the runtime)
-        PSW_TRY_HIP(hipMallocAsync(&ws, (size_t)a.G * npix * 12, S(s)), PSW_ERR_ALLOC);
+    if (a.G > 1)   // stream-ordered scratch for the plane groups' keys (pooled by the runtime)
+        PSW_TRY_HIP(hipMallocAsync(&ws, npix * SW_KEY_BYTES, S(s)), PSW_ERR_ALLOC);
     st = sweep_aggregate_run(a, winsize, fast, ws, S(s));
     if (ws) (void)hipFreeAsync(ws, S(s));
     return st;
@@ -713,17 +730,17 @@ int psw_sweep_fused_aggregate_noalloc(const float* d_ref, const float* d_src, in
     a.stdthresh = stdthresh;
     a.argmin = d_argmin; a.cmin = d_cost_min; a.csec = d_cost_second;
     sweep_layout(a, true, sweep_tile_height());
-    const size_t per_group = (size_t)width * height * 12;
+    const size_t key_bytes = (size_t)width * height * SW_KEY_BYTES;
     if (groups > 0) {
         a.G = std::min(groups, std::min(a.P, 64));
-        if (a.G > 1 && (size_t)a.G * per_group > PSW_SWEEP_SCRATCH_BYTES) {
-            set_error("%d plane groups of a %d x %d image need %zu bytes of partials, above the %zu-byte static "
-                      "scratch (fewer groups, or groups = 0 for the most that fit)",
-                      a.G, width, height, (size_t)a.G * per_group, (size_t)PSW_SWEEP_SCRATCH_BYTES);
+        if (a.G > 1 && key_bytes > PSW_SWEEP_SCRATCH_BYTES) {
+            set_error("plane groups of a %d x %d image need %zu bytes of per-pixel keys, above the %zu-byte static "
+                      "scratch (groups = 1 needs none, groups = 0 picks 1 here)",
+                      width, height, key_bytes, (size_t)PSW_SWEEP_SCRATCH_BYTES);
             return PSW_ERR_ARG;
         }
-    } else {
-        a.G = std::max(1, std::min(a.G, (int)(PSW_SWEEP_SCRATCH_BYTES / per_group)));
+    } else if (key_bytes > PSW_SWEEP_SCRATCH_BYTES) {
+        a.G = 1;
     }
     void* ws = nullptr;
     if (a.G > 1 && (st = scratch_address(&ws)) != PSW_OK) return st;
@@ -760,29 +777,26 @@ int psw_sweep_fused_aggregate_batch(const psw_sweep_frame* frames, int nframes, 
     // Plane groups: enough items across the whole batch to fill the GPU
     // (about 3000 workgroups' worth) unless given.
     int G = groups > 0 ? groups
-                       : (int)std::max(1L, (3072L * 16 / ty + total_tiles / 2) / std::max(1L, total_tiles));
+                       : (int)std::max(1L, (SW_TARGET_ITEMS * 16 / ty + total_tiles / 2) / std::max(1L, total_tiles));
     size_t ws_bytes = 0;
     for (int f = 0; f < nframes; ++f) {
         SweepArgs& a = args[(size_t)f];
         a.G = std::max(1, std::min(G, std::min(a.P, 64)));
         off[(size_t)f + 1] = off[(size_t)f] + a.ntiles * a.G;
-        if (a.G > 1) ws_bytes += (size_t)a.G * a.W * a.H * 12;
+        if (a.G > 1) ws_bytes += (size_t)a.W * a.H * SW_KEY_BYTES;
     }
     const int total = off[(size_t)nframes];
     const size_t args_bytes = sizeof(SweepArgs) * (size_t)nframes, off_bytes = sizeof(int) * ((size_t)nframes + 1);
     const size_t head = (args_bytes + off_bytes + 16 + 255) / 256 * 256;
     char* dev = nullptr;
     PSW_TRY_HIP(hipMalloc((void**)&dev, head + ws_bytes), PSW_ERR_ALLOC);
-    // per-frame group partials
+    // per-frame keys of the plane groups
     size_t o = head;
     for (int f = 0; f < nframes; ++f) {
         SweepArgs& a = args[(size_t)f];
         if (a.G > 1) {
-            const size_t npix = (size_t)a.W * a.H;
-            a.ws_cmin = reinterpret_cast<float*>(dev + o);
-            a.ws_csec = a.ws_cmin + (size_t)a.G * npix;
-            a.ws_arg = reinterpret_cast<int32_t*>(a.ws_csec + (size_t)a.G * npix);
-            o += (size_t)a.G * npix * 12;
+            a.top2 = reinterpret_cast<unsigned long long*>(dev + o);
+            o += (size_t)a.W * a.H * SW_KEY_BYTES;
         }
     }
     SweepArgs* d_args = reinterpret_cast<SweepArgs*>(dev);
@@ -794,6 +808,8 @@ int psw_sweep_fused_aggregate_batch(const psw_sweep_frame* frames, int nframes, 
     if (e == hipSuccess) e = hipMemcpyAsync(d_off, off.data(), off_bytes, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) rc = hip_status(e, "psw_sweep_fused_aggregate_batch", PSW_ERR_COPY);
     if (rc == PSW_OK) rc = psw_fill_u32(d_counter, 0u, 1, s);
+    if (rc == PSW_OK && ws_bytes > 0)
+        rc = psw_fill_u32(reinterpret_cast<uint32_t*>(dev + head), 0xFFFFFFFFu, ws_bytes / 4, s);
     const int wg = workgroups > 0 ? std::min(workgroups, total) : std::min(total, 2048);
     if (rc == PSW_OK) {
         const dim3 grid((unsigned)wg), block(SW_THREADS);
@@ -820,9 +836,8 @@ int psw_sweep_fused_aggregate_batch(const psw_sweep_frame* frames, int nframes, 
         const SweepArgs& a = args[(size_t)f];
         if (a.G <= 1) continue;
         const size_t npix = (size_t)a.W * a.H;
-        sweep_merge_kernel<<<(unsigned)((npix + 255) / 256), 256, 0, st>>>(a.ws_cmin, a.ws_arg, a.ws_csec, a.G, npix,
-                                                                         a.argmin, a.cmin, a.csec);
-        rc = launch_status("sweep_merge_kernel");
+        sweep_top2_kernel<<<(unsigned)((npix + 255) / 256), 256, 0, st>>>(a.top2, npix, a.argmin, a.cmin, a.csec);
+        rc = launch_status("sweep_top2_kernel");
     }
     // the host arrays and the workspace live until the work is done
     const hipError_t se = hipStreamSynchronize(st);
