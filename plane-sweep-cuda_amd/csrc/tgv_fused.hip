@@ -1824,8 +1824,12 @@ __device__ __forceinline__ bool tr_ring(const TgvResArgs& R, int tcx, int tcy, i
 // the pair's vertical neighbours, come from registers; each image row is one
 // wave slot in column order, so the energy's per-row butterflies are those
 // of a 640-thread block over a 64 x 10 half tile.
+// The per-tile work of a resident launch: `tile` is the tile of R's frame
+// this workgroup owns; `start` counts the launch's workgroups in (start[0])
+// and carries the go (start[1]: 1 go, 2 called off) once all nstart have.
 template <int NS>
-__global__ __launch_bounds__(TR_NT, 3) void tgv_resident_kernel(TgvResArgs R) {
+__device__ __forceinline__ void tgv_resident_body(const TgvResArgs& R, const int tile, unsigned* start,
+                                                  const unsigned nstart) {
     constexpr int TX = TG_TX, TY = TR_TY, SUB = TR_SUB;
     __shared__ float sUb[3][TY + 2][TX + 2];   // ubar_j: ring row / column 0, own 1..TY / 1..TX, halo TY+1 / TX+1
     __shared__ float sPQ[5][TY + 1][TX + 1];   // p, q of j+1: ring 0, own 1..
@@ -1839,7 +1843,6 @@ __global__ __launch_bounds__(TR_NT, 3) void tgv_resident_kernel(TgvResArgs R) {
 
     const int W = R.W, H = R.H;
     const size_t n = (size_t)W * H;
-    const int tile = xcd_tile(blockIdx.x, R.ntiles);
     const int tcx = tile % R.tiles_x, tcy = tile / R.tiles_x;
     const int x0 = tcx * TX, y0 = tcy * TY;
     const int tid = threadIdx.x;
@@ -1850,10 +1853,10 @@ __global__ __launch_bounds__(TR_NT, 3) void tgv_resident_kernel(TgvResArgs R) {
         sFlag[0] = 0;
         sFlag[1] = 0;
         // the last workgroup to start gives the go
-        const unsigned before = __hip_atomic_fetch_add(R.err + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (before + 1 == (unsigned)R.ntiles) {
+        const unsigned before = __hip_atomic_fetch_add(start, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (before + 1 == nstart) {
             unsigned expected = 0u;
-            __hip_atomic_compare_exchange_strong(R.err + 2, &expected, 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+            __hip_atomic_compare_exchange_strong(start + 1, &expected, 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT);
         }
     }
@@ -1911,16 +1914,16 @@ __global__ __launch_bounds__(TR_NT, 3) void tgv_resident_kernel(TgvResArgs R) {
     }
     // every workgroup resident before any waits on a neighbour
     if (tid == 0) {
-        unsigned go = tr_load32(R.err + 2);
+        unsigned go = tr_load32(start + 1);
         for (unsigned it = 0; go == 0u && it < R.start_spin; ++it) {
             __builtin_amdgcn_s_sleep(2);
-            go = tr_load32(R.err + 2);
+            go = tr_load32(start + 1);
         }
         if (go == 0u) {
             unsigned expected = 0u;
-            __hip_atomic_compare_exchange_strong(R.err + 2, &expected, 2u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+            __hip_atomic_compare_exchange_strong(start + 1, &expected, 2u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT);
-            go = tr_load32(R.err + 2);
+            go = tr_load32(start + 1);
         }
         if (go != 1u) sFlag[0] = 1;
     }
@@ -2159,6 +2162,31 @@ __global__ __launch_bounds__(TR_NT, 3) void tgv_resident_kernel(TgvResArgs R) {
     }
 }
 
+// One frame: its tiles in XCD-local order; the start words are the frame's.
+template <int NS>
+__global__ __launch_bounds__(TR_NT, 3) void tgv_resident_kernel(TgvResArgs R) {
+    tgv_resident_body<NS>(R, xcd_tile(blockIdx.x, R.ntiles), R.err + 1, (unsigned)R.ntiles);
+}
+
+// Several frames of the same source count in one launch (at most one tile
+// per CU in all): frame f owns the launch's tiles tile0[f] .. tile0[f+1]-1.
+// Each frame keeps its own exchange words, checks and stop; only the start
+// count is the launch's.
+template <int NS>
+__global__ __launch_bounds__(TR_NT, 3) void tgv_resident_batch_kernel(const TgvResArgs* __restrict__ frames,
+                                                                    const int* __restrict__ tile0, int nframes,
+                                                                    int total, unsigned* start) {
+    const int g = xcd_tile(blockIdx.x, total);
+    int lo = 0, hi = nframes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tile0[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    const TgvResArgs R = frames[lo];
+    tgv_resident_body<NS>(R, g - tile0[lo], start, (unsigned)total);
+}
+
 // Exchange words live in uncached device memory, so that a word a tile on
 // one XCD stores never waits in that XCD's L2 for a poller on another
 // (ordinary device memory with device-scope atomics measured the same
@@ -2280,24 +2308,21 @@ static unsigned tgv_env_u32(const char* name, unsigned dflt) {
     return e ? (unsigned)strtoul(e, nullptr, 10) : dflt;
 }
 
-// niters iterations from state t->k in one cooperative launch (energy checks
-// every check_every iterations when > 0, as in psw_tgv_solve).  *ran is
-// false, with nothing run or written, when the runtime refuses the
-// cooperative launch or the launch is called off.  Synchronous: it reads
-// the error words.
-static int tgv_resident_run(psw_tgv* t, const psw_tgv_params* prm, int niters, int check_every, double tol,
-                            double e_star, hipStream_t st, bool* ran) {
-    *ran = false;
+// The arguments of a resident launch of t from state t->k (energy checks
+// every check_every iterations when > 0, as in psw_tgv_solve), after
+// clearing its exchange words (tags that match no iteration) and its error
+// words.
+static int tgv_resident_args(psw_tgv* t, const psw_tgv_params* prm, int niters, int check_every, double tol,
+                             double e_star, hipStream_t st, TgvResArgs& R) {
     const int tiles_x = (t->W + TG_TX - 1) / TG_TX, tiles_y = (t->H + TR_TY - 1) / TR_TY;
     const int nt = tiles_x * tiles_y, nh = tgv_resident_halves(t->W, t->H);
     const size_t nx = (size_t)2 * nt * TR_XW, np = (size_t)4 * nh;
-    // tags that match no iteration, and clear error words
     int rc = psw_fill_u32(reinterpret_cast<uint32_t*>(t->rx), 0xFFFFFFFFu, 2 * (nx + np + 2), st);
     unsigned* err = reinterpret_cast<unsigned*>(t->rx + nx + np + 2);
     if (rc == PSW_OK) rc = psw_fill_u32(err, 0u, 8, st);
     if (rc != PSW_OK) return rc;
     const TgvPtrs a = tgv_ptrs(t, prm);
-    TgvResArgs R{};
+    R = TgvResArgs{};
     R.ucur = a.ucur; R.uprev = a.uprev; R.pcur = a.pcur; R.qcur = a.qcur; R.rcur = a.rcur;
     R.It = a.It; R.Iu = a.Iu; R.T = a.T; R.u0 = a.u0;
     R.mask = t->base + t->L.mask;
@@ -2324,44 +2349,127 @@ static int tgv_resident_run(psw_tgv* t, const psw_tgv_params* prm, int niters, i
     R.dalpha1 = prm->alpha1;
     R.e_star = e_star;
     R.gap = tol * e_star;
-    void* args[] = {&R};
-    const void* kern = nullptr;
-    switch (t->N) {
-        case 1: kern = (const void*)tgv_resident_kernel<1>; break;
-        case 2: kern = (const void*)tgv_resident_kernel<2>; break;
-        case 3: kern = (const void*)tgv_resident_kernel<3>; break;
-        case 4: kern = (const void*)tgv_resident_kernel<4>; break;
-        case 5: kern = (const void*)tgv_resident_kernel<5>; break;
-        case 6: kern = (const void*)tgv_resident_kernel<6>; break;
-        case 7: kern = (const void*)tgv_resident_kernel<7>; break;
-        default: kern = (const void*)tgv_resident_kernel<8>; break;
+    return PSW_OK;
+}
+
+// The first failed wait of a resident launch of t, as the error message
+// (PSW_ERR_DEVICE), or PSW_OK.  `he` holds the frame's error words.
+static int tgv_resident_report(const TgvResArgs& R, const unsigned* he) {
+    if (!he[0]) return PSW_OK;
+    const unsigned blk = (unsigned)R.ntiles * TR_XW;
+    if (he[4] < 2 * blk)
+        set_error("resident TGV2 solver: tile %u waited %u polls for word %u of tile %u (iteration tag %u, saw %u) "
+                  "at iteration %u; the state is undefined",
+                  he[3], R.spin, he[4] % TR_XW, (he[4] % blk) / TR_XW, he[5], he[6], he[7]);
+    else
+        set_error("resident TGV2 solver: tile %u waited %u polls for an energy word (offset %u, check %u, saw %u) "
+                  "at iteration %u; the state is undefined",
+                  he[3], R.spin, he[4], he[5], he[6], he[7]);
+    return PSW_ERR_DEVICE;
+}
+
+static const void* tgv_resident_kernel_for(int N, bool batch) {
+#define PSW_RK(n) (batch ? (const void*)tgv_resident_batch_kernel<n> : (const void*)tgv_resident_kernel<n>)
+    switch (N) {
+        case 1: return PSW_RK(1);
+        case 2: return PSW_RK(2);
+        case 3: return PSW_RK(3);
+        case 4: return PSW_RK(4);
+        case 5: return PSW_RK(5);
+        case 6: return PSW_RK(6);
+        case 7: return PSW_RK(7);
+        default: return PSW_RK(8);
     }
-    const hipError_t e = hipLaunchCooperativeKernel(kern, dim3((unsigned)nt), dim3(TR_NT), args, 0, st);
+#undef PSW_RK
+}
+
+// niters iterations of t from state t->k in one cooperative launch.  *ran is
+// false, with nothing run or written, when the runtime refuses the
+// cooperative launch or the launch is called off.  Synchronous: it reads
+// the error words.
+static int tgv_resident_run(psw_tgv* t, const psw_tgv_params* prm, int niters, int check_every, double tol,
+                            double e_star, hipStream_t st, bool* ran) {
+    *ran = false;
+    TgvResArgs R;
+    int rc = tgv_resident_args(t, prm, niters, check_every, tol, e_star, st, R);
+    if (rc != PSW_OK) return rc;
+    void* args[] = {&R};
+    const hipError_t e = hipLaunchCooperativeKernel(tgv_resident_kernel_for(t->N, false), dim3((unsigned)R.ntiles),
+                                                    dim3(TR_NT), args, 0, st);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return PSW_OK;
     }
     static_assert(sizeof(TgvCtrl) >= 8 * sizeof(unsigned), "the error words are read through hctrl");
-    PSW_TRY_HIP(hipMemcpyAsync(t->hctrl, err, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, st), PSW_ERR_COPY);
+    PSW_TRY_HIP(hipMemcpyAsync(t->hctrl, R.err, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, st), PSW_ERR_COPY);
     PSW_TRY_HIP(hipStreamSynchronize(st), PSW_ERR_DEVICE);
     unsigned he[8];
     std::memcpy(he, t->hctrl, sizeof(he));
     if (he[2] != 1u) {   // called off before any wait on a neighbour: nothing was written
-        if (!getenv("PSW_TGV_RESIDENT_START_SPIN")) tgv_resident_decline(nt);
+        if (!getenv("PSW_TGV_RESIDENT_START_SPIN")) tgv_resident_decline(R.ntiles);
         return PSW_OK;
     }
     *ran = true;
-    if (he[0]) {
-        const unsigned blk = (unsigned)nt * TR_XW;
-        if (he[4] < 2 * blk)
-            set_error("resident TGV2 solver: tile %u waited %u polls for word %u of tile %u (iteration tag %u, saw %u) "
-                      "at iteration %u; the state is undefined",
-                      he[3], R.spin, he[4] % TR_XW, (he[4] % blk) / TR_XW, he[5], he[6], he[7]);
-        else
-            set_error("resident TGV2 solver: tile %u waited %u polls for an energy word (offset %u, check %u, saw %u) "
-                      "at iteration %u; the state is undefined",
-                      he[3], R.spin, he[4], he[5], he[6], he[7]);
-        return PSW_ERR_DEVICE;
+    return tgv_resident_report(R, he);
+}
+
+// Several frames of one source count (together at most one tile per CU) in
+// one cooperative launch, each from its own state with its own parameters
+// and energy target.  *ran as for tgv_resident_run.  Synchronous.
+static int tgv_resident_batch_run(psw_tgv* const* ts, const psw_tgv_params* prms, int nf, int niters, int check_every,
+                                  double tol, const double* e_stars, hipStream_t st, bool* ran) {
+    *ran = false;
+    std::vector<TgvResArgs> Rs((size_t)nf);
+    std::vector<int> tile0((size_t)nf + 1, 0);
+    for (int f = 0; f < nf; ++f) {
+        const int rc = tgv_resident_args(ts[f], &prms[f], niters, check_every, tol, e_stars ? e_stars[f] : 0.0, st,
+                                         Rs[(size_t)f]);
+        if (rc != PSW_OK) return rc;
+        tile0[(size_t)f + 1] = tile0[(size_t)f] + Rs[(size_t)f].ntiles;
+    }
+    const int total = tile0[(size_t)nf];
+    const size_t fb = sizeof(TgvResArgs) * (size_t)nf, tb = (sizeof(int) * ((size_t)nf + 1) + 15) / 16 * 16;
+    char* dev = nullptr;
+    PSW_TRY_HIP(hipMalloc((void**)&dev, fb + tb + 16), PSW_ERR_ALLOC);
+    TgvResArgs* d_frames = reinterpret_cast<TgvResArgs*>(dev);
+    int* d_tile0 = reinterpret_cast<int*>(dev + fb);
+    unsigned* d_start = reinterpret_cast<unsigned*>(dev + fb + tb);
+    int rc = PSW_OK;
+    hipError_t e = hipMemcpyAsync(d_frames, Rs.data(), fb, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_tile0, tile0.data(), sizeof(int) * ((size_t)nf + 1), hipMemcpyHostToDevice,
+                                            st);
+    if (e != hipSuccess) rc = hip_status(e, "tgv_resident_batch_run", PSW_ERR_COPY);
+    if (rc == PSW_OK) rc = psw_fill_u32(d_start, 0u, 2, st);
+    bool launched = false;
+    if (rc == PSW_OK) {
+        int nfi = nf;
+        void* args[] = {&d_frames, &d_tile0, &nfi, (void*)&total, &d_start};
+        e = hipLaunchCooperativeKernel(tgv_resident_kernel_for(ts[0]->N, true), dim3((unsigned)total), dim3(TR_NT),
+                                       args, 0, st);
+        if (e != hipSuccess) (void)hipGetLastError();
+        else launched = true;
+    }
+    unsigned go = 0u;
+    std::vector<unsigned> he((size_t)nf * 8, 0u);
+    if (rc == PSW_OK && launched) {
+        e = hipMemcpyAsync(&go, d_start + 1, sizeof(unsigned), hipMemcpyDeviceToHost, st);
+        for (int f = 0; f < nf && e == hipSuccess; ++f)
+            e = hipMemcpyAsync(&he[(size_t)f * 8], Rs[(size_t)f].err, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rc = hip_status(e, "tgv_resident_batch_run", PSW_ERR_DEVICE);
+    } else {
+        (void)hipStreamSynchronize(st);
+    }
+    (void)hipFree(dev);
+    if (rc != PSW_OK || !launched) return rc;
+    if (go != 1u) {   // called off: nothing was written
+        if (!getenv("PSW_TGV_RESIDENT_START_SPIN")) tgv_resident_decline(total);
+        return PSW_OK;
+    }
+    *ran = true;
+    for (int f = 0; f < nf; ++f) {
+        const int r = tgv_resident_report(Rs[(size_t)f], &he[(size_t)f * 8]);
+        if (r != PSW_OK) return r;
     }
     return PSW_OK;
 }
@@ -2369,6 +2477,59 @@ static int tgv_resident_run(psw_tgv* t, const psw_tgv_params* prm, int niters, i
 static int tgv_reset_ctrl(psw_tgv* t, hipStream_t st) {
     tgv_ctrl_reset_kernel<<<1, 1, 0, st>>>(t->ctrl);
     return launch_status("tgv_ctrl_reset_kernel");
+}
+
+// After a resident solve of t from k_start: the stop, else k_start +
+// max_iters and the energy of that state tested as the launch sequence tests
+// the state its last chunk ends at; the iterations and the energy out; ctrl
+// cleared.  Synchronous.
+static int tgv_resident_finish(psw_tgv* t, int k_start, int max_iters, double tol, double e_star,
+                               const psw_tgv_params* prm, hipStream_t st, int* iters_done, double* energy_out) {
+    PSW_TRY_HIP(hipMemcpy(t->hctrl, t->ctrl, sizeof(TgvCtrl), hipMemcpyDeviceToHost), PSW_ERR_COPY);
+    if (t->hctrl->done) {
+        t->k = t->hctrl->iters_at_done;
+    } else {
+        t->k = k_start + max_iters;
+        const int rc = tgv_launch_energy(t, prm, tol, e_star, st);
+        if (rc != PSW_OK) return rc;
+        PSW_TRY_HIP(hipStreamSynchronize(st), PSW_ERR_DEVICE);
+        PSW_TRY_HIP(hipMemcpy(t->hctrl, t->ctrl, sizeof(TgvCtrl), hipMemcpyDeviceToHost), PSW_ERR_COPY);
+    }
+    if (iters_done) *iters_done = t->k - k_start;
+    if (energy_out) *energy_out = t->hctrl->done ? t->hctrl->energy_at_done : t->hctrl->energy;
+    return tgv_reset_ctrl(t, st) == PSW_OK && hipStreamSynchronize(st) == hipSuccess ? PSW_OK : PSW_ERR_DEVICE;
+}
+
+// The frames of a batched call that the resident solver takes: launches of
+// one source count and at most one tile per CU in all, filled largest frame
+// first (first fit).  The other frames go to `rest`.  The launch shape
+// `variant` must be the default (the resident checks use its partials).
+static std::vector<std::vector<int>> tgv_resident_packs(psw_tgv* const* ts, int n, int variant,
+                                                        std::vector<int>& rest) {
+    std::vector<std::vector<int>> packs;
+    std::vector<int> ok;
+    for (int f = 0; f < n; ++f) {
+        if ((variant < 0 || variant == 1) && ts[f] && tgv_resident_use(ts[f])) ok.push_back(f);
+        else rest.push_back(f);
+    }
+    const int cus = tgv_resident_slots(0);
+    std::stable_sort(ok.begin(), ok.end(), [&](int a, int b) {
+        return tgv_resident_tiles(ts[a]->W, ts[a]->H) > tgv_resident_tiles(ts[b]->W, ts[b]->H);
+    });
+    std::vector<int> fill, srcs;
+    for (int f : ok) {
+        const int nt = tgv_resident_tiles(ts[f]->W, ts[f]->H);
+        size_t b = 0;
+        while (b < packs.size() && (srcs[b] != ts[f]->N || fill[b] + nt > cus)) ++b;
+        if (b == packs.size()) {
+            packs.emplace_back();
+            fill.push_back(0);
+            srcs.push_back(ts[f]->N);
+        }
+        packs[b].push_back(f);
+        fill[b] += nt;
+    }
+    return packs;
 }
 
 }  // namespace psw
@@ -2612,21 +2773,7 @@ int psw_tgv_solve(psw_tgv* t, int max_iters, int check_every, double tol, double
         bool ran = false;
         rc = tgv_resident_run(t, prm, max_iters, check_every, tol, e_star, st, &ran);
         if (rc != PSW_OK) return rc;
-        if (ran) {
-            PSW_TRY_HIP(hipMemcpy(t->hctrl, t->ctrl, sizeof(TgvCtrl), hipMemcpyDeviceToHost), PSW_ERR_COPY);
-            if (t->hctrl->done) {
-                t->k = t->hctrl->iters_at_done;
-            } else {
-                t->k = k_start + max_iters;
-                rc = tgv_launch_energy(t, prm, tol, e_star, st);
-                if (rc != PSW_OK) return rc;
-                PSW_TRY_HIP(hipStreamSynchronize(st), PSW_ERR_DEVICE);
-                PSW_TRY_HIP(hipMemcpy(t->hctrl, t->ctrl, sizeof(TgvCtrl), hipMemcpyDeviceToHost), PSW_ERR_COPY);
-            }
-            if (iters_done) *iters_done = t->k - k_start;
-            if (energy_out) *energy_out = t->hctrl->done ? t->hctrl->energy_at_done : t->hctrl->energy;
-            return tgv_reset_ctrl(t, st) == PSW_OK && hipStreamSynchronize(st) == hipSuccess ? PSW_OK : PSW_ERR_DEVICE;
-        }
+        if (ran) return tgv_resident_finish(t, k_start, max_iters, tol, e_star, prm, st, iters_done, energy_out);
     }
     std::memset(t->hmirror, 0, sizeof(TgvCtrl));   // no launch of this solver writes it now
     volatile const TgvCtrl* mirror = t->hmirror;
@@ -3114,19 +3261,48 @@ int psw_tgv_iterate_batch(psw_tgv* const* solvers, int nsolvers, int niters, con
                           psw_stream_t s) {
     PSW_CHECK_ARG(niters >= 0, "niters must be >= 0");
     PSW_CHECK_ARG(solvers && prm && nsolvers >= 1, "NULL pointer or no frames");
-    for (const auto& g : tgv_batch_groups(solvers, nsolvers)) {
+    // frames the resident solver takes, several per launch
+    std::vector<int> rest;
+    std::vector<std::vector<int>> packs;
+    if (niters >= TR_MIN_ITERS) packs = tgv_resident_packs(solvers, nsolvers, variant, rest);
+    else for (int f = 0; f < nsolvers; ++f) rest.push_back(f);
+    for (const auto& pk : packs) {
+        std::vector<psw_tgv*> ts;
+        std::vector<psw_tgv_params> ps;
+        for (int f : pk) {
+            ts.push_back(solvers[f]);
+            ps.push_back(prm[f]);
+        }
+        bool ran = false;
+        const int rc = tgv_resident_batch_run(ts.data(), ps.data(), (int)ts.size(), niters, 0, 0.0, nullptr, S(s), &ran);
+        if (rc != PSW_OK) return rc;
+        if (!ran) {
+            rest.insert(rest.end(), pk.begin(), pk.end());
+            continue;
+        }
+        for (psw_tgv* t : ts) t->k += niters;
+    }
+    std::sort(rest.begin(), rest.end());
+    std::vector<psw_tgv*> rs;
+    std::vector<psw_tgv_params> rp;
+    for (int f : rest) {
+        rs.push_back(solvers[f]);
+        rp.push_back(prm[f]);
+    }
+    if (rs.empty()) return PSW_OK;
+    for (const auto& g : tgv_batch_groups(rs.data(), (int)rs.size())) {
         int rc;
         if (g.size() == 1) {
             // a group of one frame is the single-frame iteration
-            PSW_CHECK_ARG(solvers[g[0]], "NULL solver in the batch");
-            rc = psw_tgv_iterate(solvers[g[0]], niters, &prm[g[0]], s);
+            PSW_CHECK_ARG(rs[g[0]], "NULL solver in the batch");
+            rc = psw_tgv_iterate(rs[g[0]], niters, &rp[g[0]], s);
             if (rc == PSW_OK) PSW_TRY_HIP(hipStreamSynchronize(S(s)), PSW_ERR_DEVICE);
         } else {
             std::vector<psw_tgv*> ts;
             std::vector<psw_tgv_params> ps;
             for (int f : g) {
-                ts.push_back(solvers[f]);
-                ps.push_back(prm[f]);
+                ts.push_back(rs[f]);
+                ps.push_back(rp[f]);
             }
             rc = tgv_iterate_batch_group(ts.data(), (int)ts.size(), niters, ps.data(), variant, S(s));
         }
@@ -3141,35 +3317,82 @@ int psw_tgv_solve_batch(psw_tgv* const* solvers, int nsolvers, const psw_tgv_par
     PSW_CHECK_ARG(e_star, "NULL e_star");
     PSW_CHECK_ARG(max_iters >= 0 && check_every >= 1, "bad iteration counts");
     PSW_CHECK_ARG(solvers && prm && nsolvers >= 1, "NULL pointer or no frames");
-    for (const auto& g : tgv_batch_groups(solvers, nsolvers)) {
+    hipStream_t st = S(s);
+    // frames the resident solver takes, several per launch (same stops and
+    // states as one at a time)
+    std::vector<int> rest;
+    std::vector<std::vector<int>> packs;
+    if (max_iters >= 1) packs = tgv_resident_packs(solvers, nsolvers, variant, rest);
+    else for (int f = 0; f < nsolvers; ++f) rest.push_back(f);
+    for (const auto& pk : packs) {
+        std::vector<psw_tgv*> ts;
+        std::vector<psw_tgv_params> ps;
+        std::vector<double> es;
+        std::vector<int> k0;
+        for (int f : pk) {
+            int rc = tgv_reset_ctrl(solvers[f], st);
+            if (rc != PSW_OK) return rc;
+            ts.push_back(solvers[f]);
+            ps.push_back(prm[f]);
+            es.push_back(e_star[f]);
+            k0.push_back(solvers[f]->k);
+        }
+        bool ran = false;
+        int rc = tgv_resident_batch_run(ts.data(), ps.data(), (int)ts.size(), max_iters, check_every, tol, es.data(), st,
+                                        &ran);
+        if (rc != PSW_OK) return rc;
+        if (!ran) {
+            rest.insert(rest.end(), pk.begin(), pk.end());
+            continue;
+        }
+        for (size_t k = 0; k < pk.size(); ++k) {
+            int it = 0;
+            double en = 0.0;
+            rc = tgv_resident_finish(ts[k], k0[k], max_iters, tol, es[k], &ps[k], st, &it, &en);
+            if (rc != PSW_OK) return rc;
+            if (iters_done) iters_done[pk[k]] = it;
+            if (energy_out) energy_out[pk[k]] = en;
+        }
+    }
+    std::sort(rest.begin(), rest.end());
+    std::vector<psw_tgv*> rs;
+    std::vector<psw_tgv_params> rp;
+    std::vector<double> re;
+    for (int f : rest) {
+        rs.push_back(solvers[f]);
+        rp.push_back(prm[f]);
+        re.push_back(e_star[f]);
+    }
+    if (rs.empty()) return PSW_OK;
+    for (const auto& g : tgv_batch_groups(rs.data(), (int)rs.size())) {
         if (g.size() == 1) {
             // a group of one frame is the single-frame solve (same stops and states)
             const int f = g[0];
-            PSW_CHECK_ARG(solvers[f], "NULL solver in the batch");
+            PSW_CHECK_ARG(rs[f], "NULL solver in the batch");
             int it = 0;
             double en = 0.0;
-            const int rc = psw_tgv_solve(solvers[f], max_iters, check_every, tol, e_star[f], &prm[f], &it, &en, s);
+            const int rc = psw_tgv_solve(rs[f], max_iters, check_every, tol, re[f], &rp[f], &it, &en, s);
             if (rc != PSW_OK) return rc;
-            if (iters_done) iters_done[f] = it;
-            if (energy_out) energy_out[f] = en;
+            if (iters_done) iters_done[rest[f]] = it;
+            if (energy_out) energy_out[rest[f]] = en;
             continue;
         }
         std::vector<psw_tgv*> ts;
         std::vector<psw_tgv_params> ps;
         std::vector<double> es;
         for (int f : g) {
-            ts.push_back(solvers[f]);
-            ps.push_back(prm[f]);
-            es.push_back(e_star[f]);
+            ts.push_back(rs[f]);
+            ps.push_back(rp[f]);
+            es.push_back(re[f]);
         }
         std::vector<int> it(g.size(), 0);
         std::vector<double> en(g.size(), 0.0);
         const int rc = tgv_solve_batch_group(ts.data(), (int)ts.size(), ps.data(), es.data(), max_iters, check_every,
-                                             tol, variant, use_graph, it.data(), en.data(), S(s));
+                                             tol, variant, use_graph, it.data(), en.data(), st);
         if (rc != PSW_OK) return rc;
         for (size_t k = 0; k < g.size(); ++k) {
-            if (iters_done) iters_done[g[k]] = it[k];
-            if (energy_out) energy_out[g[k]] = en[k];
+            if (iters_done) iters_done[rest[g[k]]] = it[k];
+            if (energy_out) energy_out[rest[g[k]]] = en[k];
         }
     }
     return PSW_OK;

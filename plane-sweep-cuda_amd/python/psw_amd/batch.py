@@ -108,12 +108,13 @@ def _time(fn, stream, reps=3):
     return ev0.elapsed_ms(ev1) / reps
 
 
-def autotune(frames, prm, fast=True, tgv_iters=20, solve_iters=200, stream=None, log=None):
+def autotune(frames, prm, fast=True, solve_iters=200, stream=None, log=None):
     """Time the launch shapes of the batched sweep (NCC tile height x plane
     groups x resident workgroups; the tile height sets the LDS halo tile,
     32 + 2r by TY + 2r for window radius r, through PSW_SWEEP_TY), of the
-    batched TGV2 iteration (tile shape of the
-    one-iteration launches) and the batched solve's chunks replayed from a
+    batched TGV2 iteration (tile shape of the one-iteration launches; the
+    default shape lets the resident solver take the frames that fit it) and
+    the batched solve's chunks replayed from a
     captured hipGraph or launched directly, on these frames; returns the
     fastest settings and the timings.  Results do not depend on the choice.
     Leaves the solvers iterated: linearise again before using them."""
@@ -139,8 +140,11 @@ def autotune(frames, prm, fast=True, tgv_iters=20, solve_iters=200, stream=None,
         else:
             os.environ["PSW_SWEEP_TY"] = prev_ty
     solvers = [f.tgv for f in frames]
+    # per-iteration cost over solve_iters iterations per call: the default
+    # shape (1) packs the frames the resident solver takes into resident
+    # launches, whose set-up only pays off over many iterations
     for var in (0, 1, 2):
-        ms = _time(lambda: tgv_iterate_batch(solvers, tgv_iters, prm, var, s), st) / tgv_iters
+        ms = _time(lambda: tgv_iterate_batch(solvers, solve_iters, prm, var, s), st) / solve_iters
         timings["tgv"][var] = ms
         if log:
             log(f"[autotune] tgv variant {var}: {ms * 1e3:.1f} us per iteration")

@@ -1,7 +1,8 @@
 """Batched frames of uneven size: every batched result equals, frame by
 frame and bit for bit, the single-frame entry point on that frame (sweep
 outputs, TGV2 states after n iterations, solve stops and states with and
-without hipGraph replay)."""
+without hipGraph replay, on the launch sequence and on the resident solver,
+whose batched form packs frames of one source count into one launch)."""
 import numpy as np
 import pytest
 
@@ -71,11 +72,14 @@ def _linearised(frames, prm):
         f.linearise(prm)
 
 
-@pytest.mark.parametrize("batch_mb", ["192", "3"])
-def test_tgv_iterate_batch_matches_single(gpu, monkeypatch, batch_mb):
-    """Also with the frames split into several cache-sized groups (3 MB)."""
+@pytest.mark.parametrize("batch_mb,resident", [("192", "0"), ("3", "0"), ("192", "1")])
+def test_tgv_iterate_batch_matches_single(gpu, monkeypatch, batch_mb, resident):
+    """Also with the frames split into several cache-sized groups (3 MB), and
+    with the resident solver (frames of one source count packed into one
+    launch)."""
     from psw_amd import batch, stages
     monkeypatch.setenv("PSW_TGV_BATCH_MB", batch_mb)
+    monkeypatch.setenv("PSW_TGV_RESIDENT", resident)
     prm = stages.tgv_params(**TGV_KW)
     a, b = _frames(), _frames()
     _linearised(a, prm)
@@ -89,14 +93,17 @@ def test_tgv_iterate_batch_matches_single(gpu, monkeypatch, batch_mb):
             np.testing.assert_array_equal(sa[k], sb[k], err_msg=k)
 
 
-@pytest.mark.parametrize("use_graph,batch_mb", [(False, "192"), (True, "192"), (True, "3")])
-def test_tgv_solve_batch_matches_single(gpu, monkeypatch, use_graph, batch_mb):
+@pytest.mark.parametrize("use_graph,batch_mb,resident", [(False, "192", "0"), (True, "192", "0"), (True, "3", "0"),
+                                                         (False, "192", "1")])
+def test_tgv_solve_batch_matches_single(gpu, monkeypatch, use_graph, batch_mb, resident):
     """Per-frame E* from a long run; the batched solve stops every frame at
     the same iteration with the same state as the single solve; replaying
     the captured graph twice, with eager work in between, repeats it.  With
-    batch_mb 3 the frames are solved in several cache-sized groups."""
+    batch_mb 3 the frames are solved in several cache-sized groups; with the
+    resident solver the frames of one source count share a launch."""
     from psw_amd import batch, stages
     monkeypatch.setenv("PSW_TGV_BATCH_MB", batch_mb)
+    monkeypatch.setenv("PSW_TGV_RESIDENT", resident)
     from psw_amd.device import Stream
     from psw_amd.device import synchronize as device_synchronize
     st = Stream()
