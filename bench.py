@@ -746,6 +746,12 @@ def main():
         "speedup_vs_plain_range": [float(sp.min()), float(sp.max())],
         "speedup_vs_plain_aggregate": plain_ms / learned_ms,
         "speedup_vs_cold": float(np.median(spc)) if spc is not None else None,
+        "speedup_vs_cold_range": [float(spc.min()), float(spc.max())] if spc is not None else None,
+        "speedup_vs_cold_aggregate": res["cold"][0] / learned_ms if "cold" in res else None,
+        "target_3x_vs_cold_start": ({"comparator": "the reference's cold start (u = 1) on the same problem (linearised "
+                                                   "at the sweep depth), the same gap",
+                                     "median_per_scene": float(np.median(spc)), "met": bool(np.median(spc) >= 3.0)}
+                                    if spc is not None else None),
         "plain": {"ms_per_depthmap": plain_ms, "depthmaps_per_s": 1e3 / plain_ms, "runs": args.steps},
         "cold": ({"ms_per_depthmap": res["cold"][0], "depthmaps_per_s": 1e3 / res["cold"][0], "runs": args.steps}
                  if "cold" in res else None),
