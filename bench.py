@@ -100,13 +100,14 @@ def cpu_baseline(v, planes, K0, K1, ds, plain_iters, fusion_o=None, fusion_thr=N
                      f"{sample_iters} TGV2 iterations, scaled to the plain pipeline's {plain_iters} iterations",
            "sweep_s": t_sweep, "tgv_iter_ms": t_it * 1e3}
     # the small plain pipeline: 4 sources, 64 planes, 7x7, 200 cold-start iterations
-    src4 = np.ascontiguousarray(v["src"][:4])
-    p64 = stages.inverse_depth_planes(0.5, 5.0, 64)
+    n4 = SMALL_NSRC
+    src4 = np.ascontiguousarray(v["src"][:n4])
+    p64 = stages.inverse_depth_planes(SMALL_ZNEAR, SMALL_ZFAR, SMALL_PLANES)
     t0 = time.perf_counter()
-    oracle.sweep_agg(v["ref"], src4, K0, v["Rref"], v["tref"], v["Rsrc"][:4], v["tsrc"][:4], p64, 7, STD_T)
+    oracle.sweep_agg(v["ref"], src4, K0, v["Rref"], v["tref"], v["Rsrc"][:n4], v["tsrc"][:n4], p64, SMALL_WIN, STD_T)
     t_s4 = time.perf_counter() - t0
     t0 = time.perf_counter()
-    oracle.tgv(v["ref"], src4, K1, v["Rref"], v["tref"], v["Rsrc"][:4], v["tsrc"][:4], 200, 1, **TGV)
+    oracle.tgv(v["ref"], src4, K1, v["Rref"], v["tref"], v["Rsrc"][:n4], v["tsrc"][:n4], SMALL_ITERS, 1, **TGV)
     t_t4 = time.perf_counter() - t0
     out["small_pipeline"] = {"value": 1.0 / (t_s4 + t_t4), "unit": "depthmaps/s", "sweep_s": t_s4, "tgv_s": t_t4,
                              "sample": "640x480, 4 sources, 64 planes 0.5-5 m, 7x7, TGV2 cold start 200 iterations"}
@@ -222,6 +223,48 @@ TVL1_ITERS = 1000
 # per pixel and iteration, with nothing kept on chip between iterations: read
 # depth, Px, Py, R, the scaled input and T11..T22 (36 B), write depth, Px, Py, R (16 B)
 TVL1_BYTES_PER_PIXEL_ITER = 52
+
+
+SMALL_NSRC, SMALL_PLANES, SMALL_WIN, SMALL_ZNEAR, SMALL_ZFAR, SMALL_ITERS = 4, 64, 7, 0.5, 5.0, 200
+
+
+def run_small(pl, stream, ev, fast, reps=20):
+    """The small plain pipeline on the first scene's views: the aggregated
+    sweep over the first 4 sources (64 planes uniform in inverse depth over
+    0.5-5 m, 7x7 window), then PlaneSweep::TGV from the constant start u = 1
+    with one linearisation and 200 iterations (the reference's TGV call).
+    Per depthmap over `reps` runs after two untimed ones, inputs resident."""
+    from psw_amd import stages
+    from psw_amd.device import DeviceArray
+    v, s = pl.v, stream.handle
+    d_src4 = DeviceArray.from_host(v["d_src"].to_host()[:SMALL_NSRC])
+    sw = stages.SweepAggregate(v["d_ref"], d_src4, pl.K0, v["Rref"], v["tref"], v["Rsrc"][:SMALL_NSRC],
+                               v["tsrc"][:SMALL_NSRC],
+                               stages.inverse_depth_planes(SMALL_ZNEAR, SMALL_ZFAR, SMALL_PLANES), SMALL_WIN, STD_T,
+                               coord_offset=0)
+    tgv = stages.TgvSolver(W, H, SMALL_NSRC)
+    tgv.set_images(v["d_ref"], d_src4, TGV["beta"], TGV["gamma"], s)
+    Rrel, trel = stages.relative_all(v["Rref"], v["tref"], v["Rsrc"][:SMALL_NSRC], v["tsrc"][:SMALL_NSRC])
+    d_u = DeviceArray((H, W))
+    t_sw, t_tgv = [], []
+    for i in range(reps + 2):
+        ev[0].record(stream)
+        sw.build_homographies(s)
+        sw.run(fast=fast, stream=s)
+        sw.depth(SMALL_ZFAR, s)
+        ev[1].record(stream)
+        tgv.run(pl.K1, Rrel, trel, SMALL_ITERS, 1, pl.prm, d_u, stream=s)
+        ev[2].record(stream)
+        ev[2].synchronize()
+        if i >= 2:
+            t_sw.append(ev[0].elapsed_ms(ev[1]))
+            t_tgv.append(ev[1].elapsed_ms(ev[2]))
+    ms = float(np.mean(t_sw) + np.mean(t_tgv))
+    return {"value": 1e3 / ms, "unit": "depthmaps/s", "ms_per_depthmap": ms, "sweep_ms": float(np.mean(t_sw)),
+            "tgv_ms": float(np.mean(t_tgv)), "tgv_us_per_iteration": float(np.mean(t_tgv)) * 1e3 / SMALL_ITERS,
+            "runs": reps, "sweep_arith": "fast" if fast else "reference-order",
+            "config": f"{W}x{H}, {SMALL_NSRC} sources, {SMALL_PLANES} planes {SMALL_ZNEAR}-{SMALL_ZFAR} m, "
+                      f"{SMALL_WIN}x{SMALL_WIN}, TGV2 cold start {SMALL_ITERS} iterations (one linearisation)"}
 
 
 def run_tvl1(pl, stream, ev):
@@ -782,6 +825,9 @@ def main():
     log(f"TGV2 resident: {resident_us:.3f} us per iteration; launch sequence {launch_ms * 1e3 / m:.3f} us")
     log(f"sparse TGV2: {rec['sparse_tgv']['iter_us']:.2f} us per iteration")
     rec["tvl1"] = run_tvl1(pl, Stream(), [Event() for _ in range(2)])
+    rec["small_pipeline"] = run_small(pl, Stream(), [Event() for _ in range(3)], fast)
+    log(f"small pipeline: {rec['small_pipeline']['ms_per_depthmap']:.3f} ms per depthmap (sweep "
+        f"{rec['small_pipeline']['sweep_ms']:.3f} ms, TGV2 {rec['small_pipeline']['tgv_ms']:.3f} ms)")
     log(f"TVL1: {rec['tvl1']['iter_us']:.2f} us per iteration fused, "
         f"{rec['tvl1']['launch_sequence_iter_us']:.2f} us as two launches per iteration")
     if not args.no_fusion:
@@ -799,6 +845,8 @@ def main():
         rec["cpu_baseline"] = cpu_baseline(vh, pl.planes, pl.K0, pl.K1, pl.d_s_host, plain_iters,
                                            o if not args.no_fusion else None, thr if not args.no_fusion else None)
         log(f"cpu baseline samples took {time.perf_counter() - t0:.1f} s")
+        cs = rec["cpu_baseline"]["small_pipeline"]
+        rec["small_pipeline"]["vs_cpu_baseline"] = rec["small_pipeline"]["value"] / cs["value"]
     else:
         rec["cpu_baseline"] = None
     print(json.dumps(rec), flush=True)

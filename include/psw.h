@@ -257,7 +257,12 @@ int psw_sweep_fused_reference(const float* d_ref, const float* d_src, int width,
  * plane (first minimum), minimum cost and second-smallest cost per pixel.
  * coord_offset 0 = 0-based pixel convention, 1 = the reference's 1-based one.
  * fast = 0: reference-order arithmetic (bit-identical to the float oracle);
- * fast = 1: running window sums, reciprocal scaling and FMA. */
+ * fast = 1: running window sums, reciprocal scaling and FMA.  With more
+ * than one plane group (the default below about 9600 tiles of 32 x 16
+ * pixels; PSW_SWEEP_GROUPS sets it) the groups' keys, 16 bytes per pixel,
+ * come from the library's stream-ordered
+ * workspace pool, which keeps freed blocks for later calls until the
+ * process ends. */
 int psw_sweep_fused_aggregate(const float* d_ref, const float* d_src, int width, int height, int nsrc,
                               const float* d_H, int nplanes, int winsize, float stdthresh, int coord_offset,
                               int fast, int32_t* d_argmin, float* d_cost_min, float* d_cost_second,
@@ -395,9 +400,11 @@ int psw_tgv_run(psw_tgv* t, const float* K, const float* Rrel, const float* trel
  * d_dense: the initial depthmap (the reference's `depthmap`); d_sparse:
  * depth where known (> 0), else <= 0; output depth in d_out.  Semantics of
  * the reference driver: u1 and u1bar are the same arrays in its U step, and
- * P reads u, so theta has no effect.  d_workspace: NULL (allocated and freed
- * here; the call then synchronises the stream) or at least
- * psw_tgv_sparse_workspace_bytes(width, height) bytes of device memory. */
+ * P reads u, so theta has no effect.  d_workspace: NULL (taken from the
+ * library's stream-ordered workspace pool and given back in stream order;
+ * the pool keeps freed blocks for later calls until the process ends) or at
+ * least psw_tgv_sparse_workspace_bytes(width, height) bytes of device
+ * memory. */
 size_t psw_tgv_sparse_workspace_bytes(int width, int height);
 int psw_tgv_sparse_denoise(const float* d_ref, const float* d_dense, const float* d_sparse, int width, int height,
                            float zfar, int niters, double alpha0, double alpha1, double tau, double sigma,

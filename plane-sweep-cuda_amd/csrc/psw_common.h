@@ -32,6 +32,14 @@ int hip_status(hipError_t e, const char* what, int code = PSW_ERR_DEVICE);
 // cudaPeekAtLastError check (src/planesweep.cpp:134).
 int launch_status(const char* kernel);
 
+// Stream-ordered workspace of one call (freed on the same stream when the
+// call's kernels are queued).  It comes from a memory pool of the library's
+// own per device that keeps freed blocks for later calls: the default pool
+// hands them back to the device at every synchronisation, and the next call
+// then maps fresh memory, which costs more than the kernels of a small call.
+int workspace_alloc_async(void** p, size_t bytes, hipStream_t st);
+int workspace_free_async(void* p, hipStream_t st);
+
 #define PSW_CHECK_ARG(cond, msg)                                         \
     do {                                                                 \
         if (!(cond)) {                                                   \

@@ -5,6 +5,7 @@
 #include "psw_common.h"
 
 #include <cstdarg>
+#include <mutex>
 
 namespace psw {
 
@@ -29,6 +30,51 @@ int hip_status(hipError_t e, const char* what, int code) {
 int launch_status(const char* kernel) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_status(e, kernel, PSW_ERR_LAUNCH);
+    return PSW_OK;
+}
+
+constexpr int WS_MAX_DEVICES = 64;
+
+static int workspace_pool(hipMemPool_t* out) {
+    static std::mutex mu;
+    static hipMemPool_t pools[WS_MAX_DEVICES] = {};
+    int dev = 0;
+    PSW_TRY_HIP(hipGetDevice(&dev), PSW_ERR_DEVICE);
+    if (dev < 0 || dev >= WS_MAX_DEVICES) {
+        set_error("device index %d out of range", dev);
+        return PSW_ERR_DEVICE;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    if (!pools[dev]) {
+        hipMemPoolProps props = {};
+        props.allocType = hipMemAllocationTypePinned;
+        props.handleTypes = hipMemHandleTypeNone;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        hipMemPool_t pool = nullptr;
+        PSW_TRY_HIP(hipMemPoolCreate(&pool, &props), PSW_ERR_ALLOC);
+        uint64_t keep = UINT64_MAX;   // never trim at synchronisations
+        const hipError_t e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        if (e != hipSuccess) {
+            (void)hipMemPoolDestroy(pool);
+            return hip_status(e, "hipMemPoolSetAttribute(release threshold)", PSW_ERR_ALLOC);
+        }
+        pools[dev] = pool;
+    }
+    *out = pools[dev];
+    return PSW_OK;
+}
+
+int workspace_alloc_async(void** p, size_t bytes, hipStream_t st) {
+    hipMemPool_t pool = nullptr;
+    const int rc = workspace_pool(&pool);
+    if (rc != PSW_OK) return rc;
+    PSW_TRY_HIP(hipMallocFromPoolAsync(p, bytes, pool, st), PSW_ERR_ALLOC);
+    return PSW_OK;
+}
+
+int workspace_free_async(void* p, hipStream_t st) {
+    if (p) PSW_TRY_HIP(hipFreeAsync(p, st), PSW_ERR_DEVICE);
     return PSW_OK;
 }
 

@@ -708,10 +708,12 @@ int psw_sweep_fused_aggregate(const float* d_ref, const float* d_src, int width,
     sweep_layout(a, true, sweep_tile_height());
     const size_t npix = (size_t)width * height;
     void* ws = nullptr;
-    if (a.G > 1)   // stream-ordered scratch for the plane groups' keys (pooled by the runtime)
-        PSW_TRY_HIP(hipMallocAsync(&ws, npix * SW_KEY_BYTES, S(s)), PSW_ERR_ALLOC);
+    if (a.G > 1) {   // the plane groups' keys, from the library's workspace pool
+        st = workspace_alloc_async(&ws, npix * SW_KEY_BYTES, S(s));
+        if (st != PSW_OK) return st;
+    }
     st = sweep_aggregate_run(a, winsize, fast, ws, S(s));
-    if (ws) (void)hipFreeAsync(ws, S(s));
+    if (ws) (void)workspace_free_async(ws, S(s));
     return st;
 }
 
@@ -789,7 +791,12 @@ int psw_sweep_fused_aggregate_batch(const psw_sweep_frame* frames, int nframes, 
     const size_t args_bytes = sizeof(SweepArgs) * (size_t)nframes, off_bytes = sizeof(int) * ((size_t)nframes + 1);
     const size_t head = (args_bytes + off_bytes + 16 + 255) / 256 * 256;
     char* dev = nullptr;
-    PSW_TRY_HIP(hipMalloc((void**)&dev, head + ws_bytes), PSW_ERR_ALLOC);
+    {
+        void* w = nullptr;
+        const int rc0 = workspace_alloc_async(&w, head + ws_bytes, S(s));
+        if (rc0 != PSW_OK) return rc0;
+        dev = static_cast<char*>(w);
+    }
     // per-frame keys of the plane groups
     size_t o = head;
     for (int f = 0; f < nframes; ++f) {
@@ -839,9 +846,10 @@ int psw_sweep_fused_aggregate_batch(const psw_sweep_frame* frames, int nframes, 
         sweep_top2_kernel<<<(unsigned)((npix + 255) / 256), 256, 0, st>>>(a.top2, npix, a.argmin, a.cmin, a.csec);
         rc = launch_status("sweep_top2_kernel");
     }
-    // the host arrays and the workspace live until the work is done
+    // the host arrays live until the copies are done; the workspace goes
+    // back to the pool in stream order
+    (void)workspace_free_async(dev, st);
     const hipError_t se = hipStreamSynchronize(st);
-    (void)hipFree(dev);
     if (rc == PSW_OK && se != hipSuccess) rc = hip_status(se, "psw_sweep_fused_aggregate_batch");
     return rc;
 }

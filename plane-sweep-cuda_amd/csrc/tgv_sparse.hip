@@ -484,8 +484,11 @@ int psw_tgv_sparse_denoise(const float* d_ref, const float* d_dense, const float
     PSW_CHECK_ARG(!d_workspace || workspace_bytes >= need, "workspace too small");
     float* ws = static_cast<float*>(d_workspace);
     bool owned = false;
-    if (!ws) {
-        PSW_TRY_HIP(hipMalloc((void**)&ws, need), PSW_ERR_ALLOC);
+    if (!ws) {   // stream-ordered, from the library's workspace pool
+        void* w = nullptr;
+        const int rc = workspace_alloc_async(&w, need, S(s));
+        if (rc != PSW_OK) return rc;
+        ws = static_cast<float*>(w);
         owned = true;
     }
     float* T = ws;
@@ -550,12 +553,7 @@ int psw_tgv_sparse_denoise(const float* d_ref, const float* d_dense, const float
         tgv_sparse_output_kernel<<<blocks, 256, 0, hs>>>(d_out, slot[launches & 1], n, zfar);
         e = hipGetLastError();
     }
-    if (owned) {
-        // the frees wait for the launches above; report their errors first
-        const hipError_t e2 = hipStreamSynchronize(hs);
-        if (e == hipSuccess) e = e2;
-        (void)hipFree(ws);
-    }
+    if (owned) (void)workspace_free_async(ws, hs);   // after the launches above, in stream order
     if (e != hipSuccess) return hip_status(e, "psw_tgv_sparse_denoise");
     return PSW_OK;
 }

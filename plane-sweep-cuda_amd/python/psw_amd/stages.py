@@ -192,6 +192,14 @@ class TgvSolver:
     def iterate(self, niters, prm, stream=None):
         call("psw_tgv_iterate", self.h, int(niters), ctypes.byref(prm), stream)
 
+    def run(self, K, Rrel, trel, niters, warps, prm, d_out, d_init=None, stream=None):
+        """PlaneSweep::TGV on the images already set: u from d_init (None:
+        the constant 1), `warps` linearisations of `niters` iterations each;
+        the depth into d_out (H, W)."""
+        K, Rrel, trel = _f32(K).ravel(), _f32(Rrel).ravel(), _f32(trel).ravel()
+        call("psw_tgv_run", self.h, _hp(K), _hp(Rrel), _hp(trel), int(niters), int(warps), ctypes.byref(prm),
+             d_init.ptr if d_init is not None else None, d_out.ptr, stream)
+
     def energy(self, prm, stream=None):
         e = ctypes.c_double()
         call("psw_tgv_energy", self.h, ctypes.byref(prm), ctypes.byref(e), stream)
