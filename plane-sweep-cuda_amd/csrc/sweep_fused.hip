@@ -84,16 +84,22 @@ __device__ __forceinline__ float sweep_key_cost(unsigned long long k) {
 // keys: the first word keeps the smallest key of all groups; every other
 // key offered ends in the second word (a key displaced from the first word
 // is offered there), so the second word keeps the second-smallest key of the
-// whole plane loop.  Costs that are not below +inf count as absent, as in
-// the scan.
+// whole plane loop.  The two keys a group offers the second word (the
+// larger of its best and the displaced one, and its second-best) go as
+// their minimum in one atomic.  Costs that are not below +inf count as
+// absent, as in the scan.  The atomics go past the L2 (the write counters
+// read 16 bytes per atomic), so each one is a memory transaction: two per
+// pixel and group.
 __device__ __forceinline__ void sweep_top2_insert(unsigned long long* top2, size_t npix, size_t i, float c1, int p1,
                                                   float c2) {
     const float inf = __builtin_inff();
     if (!(c1 < inf)) return;
     const unsigned long long k1 = sweep_key(c1, p1);
     const unsigned long long old = atomicMin(&top2[i], k1);
-    atomicMin(&top2[npix + i], k1 < old ? old : k1);
-    if (c2 < inf) atomicMin(&top2[npix + i], sweep_key(c2, p1));
+    const unsigned long long hi = k1 < old ? old : k1;
+    const unsigned long long k2 = c2 < inf ? sweep_key(c2, p1) : ~0ull;
+    const unsigned long long offer = k2 < hi ? k2 : hi;
+    if (offer != ~0ull) atomicMin(&top2[npix + i], offer);
 }
 
 template <bool FAST>

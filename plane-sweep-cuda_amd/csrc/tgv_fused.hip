@@ -274,8 +274,9 @@ __global__ void tgv_mask_kernel(const float* __restrict__ Iu, int W, int H, int 
 // TGV2_updateR at one pixel and source (uses u_k, not ubar).
 __device__ __forceinline__ float r_update(float r, float it, float du, float iu, float sl) {
     const float rr = r + sl * (it + du * iu);
-    // rr / fmaxf(1, |rr|): rr itself for |rr| <= 1, exactly +-1 above
-    return fabsf(rr) > 1.f ? copysignf(1.f, rr) : rr;
+    // rr / fmaxf(1, |rr|): rr itself for |rr| <= 1, exactly +-1 above, which
+    // is the median of (rr, -1, 1) for every finite rr (one instruction)
+    return __builtin_amdgcn_fmed3f(rr, -1.f, 1.f);
 }
 
 // TGV2_updateU_tensor_weighed at one pixel: c = own duals, l = left
@@ -1717,24 +1718,29 @@ __device__ inline void tr_fail(const TgvResArgs& R, unsigned tile, unsigned word
     }
 }
 
-// Poll up to three words (p[i] == nullptr: none) until each carries `tag`;
+// Poll up to K words (p[i] == nullptr: none) until each carries `tag`;
 // their low halves go to bits[i].  The loads of one round are issued
 // together.  At most R.spin rounds (0 fails every wait at once: the host's
 // way to exercise the failure path); the error word, set by any wait that
 // runs out, ends the others too.
-__device__ inline bool tr_get3(unsigned long long* const p[3], unsigned tag, const TgvResArgs& R, unsigned bits[3],
+template <int K>
+__device__ inline bool tr_getn(unsigned long long* const p[K], unsigned tag, const TgvResArgs& R, unsigned bits[K],
                                int tile, int j) {
-    bool ok[3];
+    bool ok[K];
+    bool none = true;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) ok[i] = p[i] == nullptr;
-    if (ok[0] && ok[1] && ok[2]) return true;   // nothing to wait for (also with a zero budget)
+    for (int i = 0; i < K; ++i) {
+        ok[i] = p[i] == nullptr;
+        none = none && ok[i];
+    }
+    if (none) return true;   // nothing to wait for (also with a zero budget)
     for (unsigned it = 0; it < R.spin; ++it) {
-        unsigned long long w[3];
+        unsigned long long w[K];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) w[i] = ok[i] ? 0ull : tr_load(p[i]);
+        for (int i = 0; i < K; ++i) w[i] = ok[i] ? 0ull : tr_load(p[i]);
         bool all = true;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
+        for (int i = 0; i < K; ++i) {
             if (!ok[i] && (unsigned)(w[i] >> 32) == tag) {
                 ok[i] = true;
                 bits[i] = (unsigned)w[i];
@@ -1804,12 +1810,14 @@ __device__ inline bool tr_halo(const TgvResArgs& R, int tcx, int tcy, int w, int
     return has_d && gx < R.W;
 }
 
-// The ring position of thread tid: threads 0..63 the row above the tile,
-// 64..83 the column left of it (false where there is no such neighbour).
-// Its image position and its cell of the ring-indexed LDS arrays.
+// The ring position of ring thread rid: 0..63 the row above the tile,
+// 64..83 the column left of it (false where there is no such neighbour, and
+// for rid outside 0..83).  Its image position and its cell of the
+// ring-indexed LDS arrays.
 __device__ __forceinline__ bool tr_ring(const TgvResArgs& R, int tcx, int tcy, int tid, int& gx, int& gy, int& lx,
                                         int& ly) {
     const int x0 = tcx * TG_TX, y0 = tcy * TR_TY;
+    if (tid < 0) return false;
     if (tid < TG_TX) {
         gx = x0 + tid; gy = y0 - 1; lx = tid + 1; ly = 0;
         return tcy > 0 && gx < R.W;
@@ -1817,6 +1825,38 @@ __device__ __forceinline__ bool tr_ring(const TgvResArgs& R, int tcx, int tcy, i
     const int r = tid - TG_TX;
     gx = x0 - 1; gy = y0 + r; lx = 0; ly = r + 1;
     return r < TR_TY && tcx > 0 && gy < R.H;
+}
+
+// Waves of the resident workgroup with extra duties.  Ten waves share four
+// SIMDs three, three, two and two, so the per-iteration extras go to the
+// waves of the two lighter SIMDs: the ring's duals (waves 2 and 3), and the
+// energy partials' stores; the stop decision's polls and sums are spread
+// over waves 4..7.
+constexpr int TR_RING0 = 2 * TG_TX;        // first thread of the ring (waves 2 and 3)
+constexpr int TR_FIN0 = 4;                 // waves 4..7 sum the energy partials, a quarter each
+constexpr int TR_PART_WAVE = 7;            // lanes 0 and 1 store the tile's energy partials
+static_assert(2 * TR_MAXTILES <= 2 * FIN_THREADS && FIN_THREADS == 4 * 64, "partials: two per finalising thread");
+
+// The energy of a check from every half tile's partial, summed as the
+// launch sequence's finalising block sums them (threads below FIN_THREADS
+// add partials i and i + FIN_THREADS, per-wave butterflies, then the wave
+// sums in order): this wave does the butterfly of finalising wave q, whose
+// sum goes to fin[q].  Polls the partials of check nchk.
+__device__ inline bool tr_fin_quarter(const TgvResArgs& R, int q, int nchk, double* fin, int tile, int j) {
+    const int l = threadIdx.x & 63;
+    const int i0 = 64 * q + l, i1 = i0 + FIN_THREADS;
+    unsigned long long* base = R.part + (size_t)(nchk & 1) * R.nsub * 2;
+    unsigned long long* const pp[4] = {i0 < R.nsub ? base + 2 * i0 : nullptr, i0 < R.nsub ? base + 2 * i0 + 1 : nullptr,
+                                       i1 < R.nsub ? base + 2 * i1 : nullptr, i1 < R.nsub ? base + 2 * i1 + 1 : nullptr};
+    unsigned hl[4] = {0u, 0u, 0u, 0u};
+    const bool ok = tr_getn<4>(pp, (unsigned)nchk, R, hl, tile, j);
+    double v = 0.0;
+    if (i0 < R.nsub) v += __longlong_as_double((long long)(((unsigned long long)hl[0] << 32) | hl[1]));
+    if (i1 < R.nsub) v += __longlong_as_double((long long)(((unsigned long long)hl[2] << 32) | hl[3]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (l == 0) fin[q] = v;
+    return ok;
 }
 
 // NS: the number of sources (the frame's N exactly).  Thread (wave w, lane
@@ -1827,31 +1867,45 @@ __device__ __forceinline__ bool tr_ring(const TgvResArgs& R, int tcx, int tcy, i
 // The per-tile work of a resident launch: `tile` is the tile of R's frame
 // this workgroup owns; `start` counts the launch's workgroups in (start[0])
 // and carries the go (start[1]: 1 go, 2 called off) once all nstart have.
+// An energy check at iteration j saves the state j (u_{j-1}, p, q, r; u_j
+// is still in registers one iteration later) in LDS, stores the tile's
+// partials and goes on; every tile sums all partials during iteration j+1
+// (the same bits as the launch sequence's finalising block) and, if the
+// check stops the solve, writes out the saved state j.  So no iteration
+// waits for the whole grid.  Needs check_every >= 2 (the host's rule): the
+// next check's save must not come before this decision.
 template <int NS>
 __device__ __forceinline__ void tgv_resident_body(const TgvResArgs& R, const int tile, unsigned* start,
                                                   const unsigned nstart) {
     constexpr int TX = TG_TX, TY = TR_TY, SUB = TR_SUB;
+    constexpr int NSAVE = 2 * (5 + NS);        // per thread: (px, py, qx, qy, qz, r) x 2 pixels
     __shared__ float sUb[3][TY + 2][TX + 2];   // ubar_j: ring row / column 0, own 1..TY / 1..TX, halo TY+1 / TX+1
     __shared__ float sPQ[5][TY + 1][TX + 1];   // p, q of j+1: ring 0, own 1..
     __shared__ float sUk[3][TY + 1][TX + 1];   // u_j before a check: own 0.., right / lower halo TY / TX
-    __shared__ float sRT[3][TG_TX + TR_TY];    // T at the ring positions (the ring's p, q live in sPQ)
+    __shared__ float sT[3][TY + 1][TX + 1];    // T: ring 0, own 1.. (as sPQ)
+    __shared__ float sSave[NSAVE][TR_NT];      // the state of the check whose decision is pending (but u_{j-1})
+    __shared__ float sU0[TY][TX];              // u0 of the own pixels
+    // u_{j-1} of the pending check (v, v1x, v1y x 2 pixels), saved once the
+    // check's energy has read sUk, in sUk's memory (the next check writes
+    // sUk only after this one's decision)
+    float* const sSaveV = &sUk[0][0][0];
+    static_assert(6 * TR_NT <= 3 * (TY + 1) * (TX + 1), "sSaveV fits sUk");
     __shared__ double sWave[TY];               // per-row energy sums
-    __shared__ int sFlag[2];                   // [0] a wait failed, [1] stop
-    // tile 0 gathers the energy partials over sUk (read before the gather)
-    static_assert(sizeof(double) * 2 * TR_MAXTILES <= sizeof(float) * 3 * (TR_TY + 1) * (TG_TX + 1), "partials");
-    double* sPart = reinterpret_cast<double*>(&sUk[0][0][0]);
+    __shared__ double sFin[4];                 // the finalising waves' sums
+    __shared__ int sFlag[1];                   // a wait failed
 
     const int W = R.W, H = R.H;
     const size_t n = (size_t)W * H;
     const int tcx = tile % R.tiles_x, tcy = tile / R.tiles_x;
     const int x0 = tcx * TX, y0 = tcy * TY;
     const int tid = threadIdx.x;
-    const int tx = tid % TX, wv = tid / TX;    // pixels (tx, 2 wv) and (tx, 2 wv + 1) of the tile
+    const int tx = tid % TX;
+    const int wv = __builtin_amdgcn_readfirstlane(tid / TX);   // pixels (tx, 2 wv) and (tx, 2 wv + 1) of the tile
     const int x = x0 + tx;
+    const int rid = tid - TR_RING0;           // ring thread (waves 2 and 3)
     const float as1 = R.alpha1 * R.sigma, as0 = R.alpha0 * R.sigma, sl = R.sigma * R.lambda;
     if (tid == 0) {
         sFlag[0] = 0;
-        sFlag[1] = 0;
         // the last workgroup to start gives the go
         const unsigned before = __hip_atomic_fetch_add(start, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (before + 1 == nstart) {
@@ -1862,8 +1916,8 @@ __device__ __forceinline__ void tgv_resident_body(const TgvResArgs& R, const int
     }
 
     // own pixels: state k0 and the constants
-    float u[2], u1x[2], u1y[2], v[2], v1x[2], v1y[2], px[2], py[2], qx[2], qy[2], qz[2], t11[2], t12[2], t22[2];
-    float u0t[2], rv[2][NS], itv[2][NS], iuv[2][NS];
+    float u[2], u1x[2], u1y[2], v[2], v1x[2], v1y[2], px[2], py[2], qx[2], qy[2], qz[2];
+    float rv[2][NS], itv[2][NS], iuv[2][NS];
     unsigned counted = 0;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -1873,8 +1927,9 @@ __device__ __forceinline__ void tgv_resident_body(const TgvResArgs& R, const int
         v[h] = R.uprev[g]; v1x[h] = R.uprev[n + g]; v1y[h] = R.uprev[2 * n + g];
         px[h] = R.pcur[g]; py[h] = R.pcur[n + g];
         qx[h] = R.qcur[g]; qy[h] = R.qcur[n + g]; qz[h] = R.qcur[2 * n + g];
-        t11[h] = R.T[g]; t12[h] = R.T[n + g]; t22[h] = R.T[2 * n + g];
-        u0t[h] = R.u0[g];
+        sT[0][2 * wv + h + 1][tx + 1] = R.T[g]; sT[1][2 * wv + h + 1][tx + 1] = R.T[n + g];
+        sT[2][2 * wv + h + 1][tx + 1] = R.T[2 * n + g];
+        sU0[2 * wv + h][tx] = R.u0[g];
         if (x < W && y < H && R.mask[g] != 0.f) counted |= 1u << h;
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
@@ -1888,11 +1943,11 @@ __device__ __forceinline__ void tgv_resident_body(const TgvResArgs& R, const int
     // every iteration), T into sRT
     {
         int rgx, rgy, rix, riy;
-        if (tr_ring(R, tcx, tcy, tid, rgx, rgy, rix, riy)) {
+        if (tr_ring(R, tcx, tcy, rid, rgx, rgy, rix, riy)) {
             const size_t gr = (size_t)rgy * W + rgx;
             sPQ[0][riy][rix] = R.pcur[gr]; sPQ[1][riy][rix] = R.pcur[n + gr];
             sPQ[2][riy][rix] = R.qcur[gr]; sPQ[3][riy][rix] = R.qcur[n + gr]; sPQ[4][riy][rix] = R.qcur[2 * n + gr];
-            sRT[0][tid] = R.T[gr]; sRT[1][tid] = R.T[n + gr]; sRT[2][tid] = R.T[2 * n + gr];
+            sT[0][riy][rix] = R.T[gr]; sT[1][riy][rix] = R.T[n + gr]; sT[2][riy][rix] = R.T[2 * n + gr];
         }
     }
     // the halo of state k0 from the state buffers (later states come from
@@ -1933,26 +1988,71 @@ __device__ __forceinline__ void tgv_resident_body(const TgvResArgs& R, const int
     // clamped neighbours (the reference's min(x+1, W-1) and max(x-1, 0))
     const bool has_r = x < W - 1, has_l = x > 0;
     const bool has_d0 = y0 + 2 * wv < H - 1, has_d1 = y0 + 2 * wv + 1 < H - 1, has_u0 = y0 + 2 * wv > 0;
-    int j = 0, nchk = 0;
-    bool failed = false;
+    int j = 0, nchk = 0, jpend = -1;          // jpend: the check whose decision is pending
+    int jnext = R.check_every > 0 ? R.check_every : R.niters;   // the next check
+    bool failed = false, stopped = false;
+    // The pending check's energy from the finalising waves' sums (after the
+    // barrier that follows them), as the finalising block adds its waves'
+    // (waves 4.. add nothing); tile 0 records it.  True if it stops the solve.
+    auto stop_at_pending = [&]() {
+        double E = 0.0;
+#pragma unroll
+        for (int w = 0; w < TR_NT / 64; ++w) E += w < 4 ? sFin[w] : 0.0;
+        const bool stop = R.e_star > 0.0 && E - R.e_star <= R.gap;   // tgv_at_gap (ctrl was reset)
+        if (tile == 0 && tid == 0) {
+            TgvCtrl* d = R.ctrl;
+            d->energy = E;
+            if (stop) {
+                d->iters_at_done = R.k0 + jpend;
+                d->energy_at_done = E;
+                d->done = 1;
+            }
+        }
+        return stop;
+    };
     for (;; ++j) {
-        const bool check = R.check_every > 0 && j > 0 && j % R.check_every == 0 && j < R.niters;
-        if (j == R.niters && !check) break;
-        // r of j+1 and its sum with Iu need only the pixel itself: outside an
-        // energy check (where a stop must keep r_j) they are formed before
-        // the wait for the neighbours' edges, in its shadow
-        float ps[2] = {0.f, 0.f};
-        if (!check) {
+        const bool check = j == jnext && j < R.niters;
+        if (check) jnext += R.check_every;
+        const bool decide = jpend >= 0 && j == jpend + 1;
+        if (j == R.niters) {
+            // a check at the last iteration: its decision before the end
+            if (decide) {
+                if (wv >= TR_FIN0 && wv < TR_FIN0 + 4 && !tr_fin_quarter(R, wv - TR_FIN0, nchk, sFin, tile, j))
+                    sFlag[0] = 1;
+                __syncthreads();
+                if (sFlag[0]) failed = true;
+                else stopped = stop_at_pending();
+            }
+            break;
+        }
+        if (check) {
+            // the state j, written out instead of the last one if this check
+            // stops the solve
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const float du = u[h] - u0t[h];
+                const float st[5] = {px[h], py[h], qx[h], qy[h], qz[h]};
+#pragma unroll
+                for (int f = 0; f < 5; ++f) sSave[2 * f + h][tid] = st[f];
+#pragma unroll
+                for (int s = 0; s < NS; ++s) sSave[10 + 2 * s + h][tid] = rv[h][s];
+            }
+        }
+        // r of j+1 and its sum with Iu need only the pixel itself: formed
+        // before the wait for the neighbours' edges, in its shadow (after the
+        // energy at a check, which leaves fewer values live there)
+        float ps[2] = {0.f, 0.f};
+        auto r_step = [&]() {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const float du = u[h] - sU0[2 * wv + h][tx];
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
                     rv[h][s] = r_update(rv[h][s], itv[h][s], du, iuv[h][s], sl);
                     ps[h] += rv[h][s] * iuv[h][s];
                 }
             }
-        }
+        };
+        if (!check) r_step();
         // ubar_j of the pixels (registers and LDS), u_j before a check
         float ub[2][3];
 #pragma unroll
@@ -1970,29 +2070,29 @@ __device__ __forceinline__ void tgv_resident_body(const TgvResArgs& R, const int
             unsigned long long* const hp[3] = {hoff >= 0 ? blk + hoff : nullptr,
                                                check && uoff >= 0 ? blk + uoff : nullptr, nullptr};
             unsigned bits[3] = {0u, 0u, 0u};
-            if (!tr_get3(hp, (unsigned)j, R, bits, tile, j)) sFlag[0] = 1;
+            if (!tr_getn<3>(hp, (unsigned)j, R, bits, tile, j)) sFlag[0] = 1;
             if (hp[0]) (&sUb[0][0][0])[hcell] = __uint_as_float(bits[0]);
             if (hp[1]) (&sUk[0][0][0])[ucell] = __uint_as_float(bits[1]);
         }
         __syncthreads();
         if (sFlag[0]) { failed = true; break; }
         if (check) {
-            // energy of state j: per-row butterflies, then the rows of each
-            // 64 x 10 half tile in order (a 640-thread block's partial)
+            // energy of state j: per-row butterflies (the rows of each 64 x 10
+            // half tile are summed in order once every row is in)
             double e2[2];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int row = 2 * wv + h, y = y0 + row;
                 e2[h] = 0.0;
                 if (counted & (1u << h)) {
-                    const double dud = (double)u[h] - (double)u0t[h];
+                    const double dud = (double)u[h] - (double)sU0[row][tx];
                     double ed = 0.0;
 #pragma unroll
                     for (int s = 0; s < NS; ++s) ed += fabs((double)itv[h][s] + dud * (double)iuv[h][s]);
                     const int kr = has_r ? tx + 1 : tx, kd = y < H - 1 ? row + 1 : row;
                     e2[h] = energy_point(u[h], sUk[0][row][kr], sUk[0][kd][tx], u1x[h], sUk[1][row][kr],
-                                         sUk[1][kd][tx], u1y[h], sUk[2][row][kr], sUk[2][kd][tx], t11[h], t12[h],
-                                         t22[h], ed, R.dlambda, R.dalpha0, R.dalpha1);
+                                         sUk[1][kd][tx], u1y[h], sUk[2][row][kr], sUk[2][kd][tx], sT[0][row + 1][tx + 1], sT[1][row + 1][tx + 1],
+                                         sT[2][row + 1][tx + 1], ed, R.dlambda, R.dalpha0, R.dalpha1);
                 }
             }
 #pragma unroll
@@ -2001,60 +2101,7 @@ __device__ __forceinline__ void tgv_resident_body(const TgvResArgs& R, const int
                 e2[1] += __shfl_xor(e2[1], off, 64);
             }
             if (tx == 0) { sWave[2 * wv] = e2[0]; sWave[2 * wv + 1] = e2[1]; }
-            __syncthreads();
-            ++nchk;
-            if (tid < 2) {
-                const int sy = 2 * tcy + tid;   // the half tile's row in the launch sequence's tiles
-                if (sy < R.sub_y) {
-                    double e = 0.0;
-                    for (int k = 0; k < SUB; ++k) e += sWave[tid * SUB + k];
-                    const unsigned long long eb = (unsigned long long)__double_as_longlong(e);
-                    unsigned long long* pw = R.part + ((size_t)(nchk & 1) * R.nsub + sy * R.tiles_x + tcx) * 2;
-                    __hip_atomic_store(pw, tr_word((unsigned)nchk, (unsigned)(eb >> 32)), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                    __hip_atomic_store(pw + 1, tr_word((unsigned)nchk, (unsigned)eb), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                }
-            }
-            if (tile == 0) {
-                // finalise as the finalising workgroup of the launch sequence:
-                // every half tile's partial, the sum over threads below
-                // FIN_THREADS, the stop test, the decision for every tile
-                for (int i = tid; i < R.nsub; i += TR_NT) {
-                    unsigned long long* q = R.part + ((size_t)(nchk & 1) * R.nsub + i) * 2;
-                    unsigned long long* const pp[3] = {q, q + 1, nullptr};
-                    unsigned hl[3] = {0u, 0u, 0u};
-                    if (!tr_get3(pp, (unsigned)nchk, R, hl, tile, j)) sFlag[0] = 1;
-                    sPart[i] = __longlong_as_double((long long)(((unsigned long long)hl[0] << 32) | hl[1]));
-                }
-                __syncthreads();
-                double vs = 0.0;
-                if (tid < FIN_THREADS)
-                    for (int i = tid; i < R.nsub; i += FIN_THREADS) vs += sPart[i];
-                const double E = block_sum<TR_NT>(vs);
-                if (tid == 0 && !sFlag[0]) {
-                    TgvCtrl* d = R.ctrl;
-                    d->energy = E;
-                    unsigned stop = 0u;
-                    if (!d->done && R.e_star > 0.0 && E - R.e_star <= R.gap) {   // tgv_at_gap
-                        d->iters_at_done = R.k0 + j;
-                        d->energy_at_done = E;
-                        d->done = 1;
-                        stop = 1u;
-                    }
-                    __hip_atomic_store(R.dec + (nchk & 1), tr_word((unsigned)nchk, stop), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                }
-            }
-            if (tid == 0) {
-                unsigned long long* const pd[3] = {R.dec + (nchk & 1), nullptr, nullptr};
-                unsigned stop[3] = {0u, 0u, 0u};
-                if (sFlag[0] || !tr_get3(pd, (unsigned)nchk, R, stop, tile, j)) sFlag[0] = 1;
-                sFlag[1] = (int)stop[0];
-            }
-            __syncthreads();
-            if (sFlag[0]) { failed = true; break; }
-            if (sFlag[1]) break;
+            r_step();
         }
         // p, q of j+1 on the pixels (the lower pixel of the pair is the upper
         // one's down neighbour) and the ring
@@ -2068,41 +2115,67 @@ __device__ __forceinline__ void tgv_resident_body(const TgvResArgs& R, const int
                 rt[f] = has_r ? sUb[f][row + 1][tx + 2] : ub[h][f];
                 dn[f] = !has_d ? ub[h][f] : (h == 0 ? ub[1][f] : sUb[f][row + 2][tx + 1]);
             }
-            pq_update(px[h], py[h], qx[h], qy[h], qz[h], t11[h], t12[h], t22[h], ub[h][0], rt[0], dn[0], ub[h][1],
+            pq_update(px[h], py[h], qx[h], qy[h], qz[h], sT[0][row + 1][tx + 1], sT[1][row + 1][tx + 1],
+                      sT[2][row + 1][tx + 1], ub[h][0], rt[0], dn[0], ub[h][1],
                       rt[1], dn[1], ub[h][2], rt[2], dn[2], as1, as0);
             sPQ[0][row + 1][tx + 1] = px[h]; sPQ[1][row + 1][tx + 1] = py[h];
             sPQ[2][row + 1][tx + 1] = qx[h]; sPQ[3][row + 1][tx + 1] = qy[h]; sPQ[4][row + 1][tx + 1] = qz[h];
         }
         {
             int rgx, rgy, rix, riy;
-            if (tr_ring(R, tcx, tcy, tid, rgx, rgy, rix, riy)) {
+            if (tr_ring(R, tcx, tcy, rid, rgx, rgy, rix, riy)) {
                 const int rr = rgx < W - 1 ? rix + 1 : rix, rd = rgy < H - 1 ? riy + 1 : riy;
                 float rpx = sPQ[0][riy][rix], rpy = sPQ[1][riy][rix];
                 float rqx = sPQ[2][riy][rix], rqy = sPQ[3][riy][rix], rqz = sPQ[4][riy][rix];
-                pq_update(rpx, rpy, rqx, rqy, rqz, sRT[0][tid], sRT[1][tid], sRT[2][tid], sUb[0][riy][rix],
+                pq_update(rpx, rpy, rqx, rqy, rqz, sT[0][riy][rix], sT[1][riy][rix], sT[2][riy][rix], sUb[0][riy][rix],
                           sUb[0][riy][rr], sUb[0][rd][rix], sUb[1][riy][rix], sUb[1][riy][rr], sUb[1][rd][rix],
                           sUb[2][riy][rix], sUb[2][riy][rr], sUb[2][rd][rix], as1, as0);
                 sPQ[0][riy][rix] = rpx; sPQ[1][riy][rix] = rpy;
                 sPQ[2][riy][rix] = rqx; sPQ[3][riy][rix] = rqy; sPQ[4][riy][rix] = rqz;
             }
         }
+        // the pending check's energy: every tile sums all partials
+        if (decide && wv >= TR_FIN0 && wv < TR_FIN0 + 4 && !tr_fin_quarter(R, wv - TR_FIN0, nchk, sFin, tile, j))
+            sFlag[0] = 1;
         __syncthreads();
-        // r (at a check) and u of j+1 (the reference's expression order, as
-        // u_update; the upper pixel of the pair is the lower one's up
-        // neighbour), then the tile's edges of state j+1 for the neighbours
+        if (sFlag[0]) { failed = true; break; }
+        if (decide) {
+            if (stop_at_pending()) { stopped = true; break; }
+            jpend = -1;
+        }
+        if (check) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                sSaveV[(0 + h) * TR_NT + tid] = v[h];
+                sSaveV[(2 + h) * TR_NT + tid] = v1x[h];
+                sSaveV[(4 + h) * TR_NT + tid] = v1y[h];
+            }
+            // the tile's two half-tile partials (a 640-thread block's rows in order)
+            ++nchk;
+            jpend = j;
+            const int pl = tid - TR_PART_WAVE * 64;
+            if (pl >= 0 && pl < 2) {
+                const int sy = 2 * tcy + pl;   // the half tile's row in the launch sequence's tiles
+                if (sy < R.sub_y) {
+                    double e = 0.0;
+                    for (int k = 0; k < SUB; ++k) e += sWave[pl * SUB + k];
+                    const unsigned long long eb = (unsigned long long)__double_as_longlong(e);
+                    unsigned long long* pw = R.part + ((size_t)(nchk & 1) * R.nsub + sy * R.tiles_x + tcx) * 2;
+                    __hip_atomic_store(pw, tr_word((unsigned)nchk, (unsigned)(eb >> 32)), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(pw + 1, tr_word((unsigned)nchk, (unsigned)eb), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        }
+        // u of j+1 (the reference's expression order, as u_update; the upper
+        // pixel of the pair is the lower one's up neighbour), then the
+        // tile's edges of state j+1 for the neighbours
         const bool publish = j + 1 < R.niters;
         unsigned long long* xb = R.xch + ((size_t)((j + 1) & 1) * R.ntiles + tile) * TR_XW;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int row = 2 * wv + h, y = y0 + row;
-            if (check) {
-                const float du = u[h] - u0t[h];
-#pragma unroll
-                for (int s = 0; s < NS; ++s) {
-                    rv[h][s] = r_update(rv[h][s], itv[h][s], du, iuv[h][s], sl);
-                    ps[h] += rv[h][s] * iuv[h][s];
-                }
-            }
             const float l_px = has_l ? sPQ[0][row + 1][tx] : px[h], l_py = has_l ? sPQ[1][row + 1][tx] : py[h];
             const float l_qx = has_l ? sPQ[2][row + 1][tx] : qx[h], l_qz = has_l ? sPQ[4][row + 1][tx] : qz[h];
             const bool has_u = h == 1 || has_u0;
@@ -2111,38 +2184,63 @@ __device__ __forceinline__ void tgv_resident_body(const TgvResArgs& R, const int
             const float u_qy = !has_u ? qy[h] : (h == 1 ? qy[0] : sPQ[3][row][tx + 1]);
             const float u_qz = !has_u ? qz[h] : (h == 1 ? qz[0] : sPQ[4][row][tx + 1]);
             const float c_px = px[h], c_py = py[h];
-            const float un = u[h] - R.tau * (-R.alpha1 * (t11[h] * (c_px - l_px) + t12[h] * (c_py - l_py) +
-                                                         t12[h] * (c_px - u_px) + t22[h] * (c_py - u_py)) +
+            const float t11h = sT[0][row + 1][tx + 1], t12h = sT[1][row + 1][tx + 1], t22h = sT[2][row + 1][tx + 1];
+            const float un = u[h] - R.tau * (-R.alpha1 * (t11h * (c_px - l_px) + t12h * (c_py - l_py) +
+                                                         t12h * (c_px - u_px) + t22h * (c_py - u_py)) +
                                               R.lambda * ps[h]);
-            const float u1xn = u1x[h] - R.tau * (-R.alpha1 * (t11[h] * c_px + t12[h] * c_py) -
+            const float u1xn = u1x[h] - R.tau * (-R.alpha1 * (t11h * c_px + t12h * c_py) -
                                                  R.alpha0 * (qx[h] - l_qx + qz[h] - u_qz));
-            const float u1yn = u1y[h] - R.tau * (-R.alpha1 * (t12[h] * c_px + t22[h] * c_py) -
+            const float u1yn = u1y[h] - R.tau * (-R.alpha1 * (t12h * c_px + t22h * c_py) -
                                                  R.alpha0 * (qz[h] - l_qz + qy[h] - u_qy));
             v[h] = u[h]; v1x[h] = u1x[h]; v1y[h] = u1y[h];
             u[h] = un; u1x[h] = u1xn; u1y[h] = u1yn;
-            if (publish && x < W && y < H && (tx == 0 || tx == TX - 1 || row == 0 || row == TY - 1)) {
+            if (publish && x < W && y < H) {
                 const unsigned tag = (unsigned)(j + 1);
                 const float pub[3] = {2.f * u[h] - v[h], 2.f * u1x[h] - v1x[h], 2.f * u1y[h] - v1y[h]};
                 const float uu[3] = {u[h], u1x[h], u1y[h]};
+                // the top and bottom rows belong to whole waves (wv is uniform)
+                if (row == 0) {
 #pragma unroll
-                for (int f = 0; f < 3; ++f) {
-                    if (row == 0) {
+                    for (int f = 0; f < 3; ++f) {
                         tr_put(xb + TR_UB_TOP + f * TX + tx, tag, pub[f]);
                         tr_put(xb + TR_U_TOP + f * TX + tx, tag, uu[f]);
                     }
-                    if (row == TY - 1) tr_put(xb + TR_UB_BOT + f * TX + tx, tag, pub[f]);
+                }
+                if (row == TY - 1) {
+#pragma unroll
+                    for (int f = 0; f < 3; ++f) tr_put(xb + TR_UB_BOT + f * TX + tx, tag, pub[f]);
+                }
+                // the left and right columns: lanes 0 and 63 of every wave
+                if (tx == 0 || tx == TX - 1) {
+                    unsigned long long* cb = xb + (tx == 0 ? TR_UB_LEFT : TR_UB_RIGHT) + row;
+#pragma unroll
+                    for (int f = 0; f < 3; ++f) tr_put(cb + f * TY, tag, pub[f]);
                     if (tx == 0) {
-                        tr_put(xb + TR_UB_LEFT + f * TY + row, tag, pub[f]);
-                        tr_put(xb + TR_U_LEFT + f * TY + row, tag, uu[f]);
+#pragma unroll
+                        for (int f = 0; f < 3; ++f) tr_put(xb + TR_U_LEFT + f * TY + row, tag, uu[f]);
                     }
-                    if (tx == TX - 1) tr_put(xb + TR_UB_RIGHT + f * TY + row, tag, pub[f]);
                 }
             }
         }
     }
     if (failed) return;
-    // the end state k0 + j into its slots
-    const int ke = R.k0 + j;
+    int ke = R.k0 + j;
+    if (stopped) {
+        // the state of the check that stopped the solve: u_j is in v, the
+        // rest was saved
+        ke = R.k0 + jpend;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            u[h] = v[h]; u1x[h] = v1x[h]; u1y[h] = v1y[h];
+            v[h] = sSaveV[(0 + h) * TR_NT + tid]; v1x[h] = sSaveV[(2 + h) * TR_NT + tid];
+            v1y[h] = sSaveV[(4 + h) * TR_NT + tid];
+            px[h] = sSave[0 + h][tid]; py[h] = sSave[2 + h][tid];
+            qx[h] = sSave[4 + h][tid]; qy[h] = sSave[6 + h][tid]; qz[h] = sSave[8 + h][tid];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) rv[h][s] = sSave[10 + 2 * s + h][tid];
+        }
+    }
+    // the end state into its slots
     float* U1 = R.U + (size_t)(ke % TG_USLOTS) * 3 * n;
     float* U0 = R.U + (size_t)((ke + TG_USLOTS - 1) % TG_USLOTS) * 3 * n;
     float* P = R.P + (size_t)(ke % TG_PSLOTS) * 2 * n;
@@ -2767,9 +2865,10 @@ int psw_tgv_solve(psw_tgv* t, int max_iters, int check_every, double tol, double
     int rc = tgv_reset_ctrl(t, st);
     if (rc != PSW_OK) return rc;
     const int k_start = t->k;
-    if (max_iters >= 1 && tgv_resident_use(t)) {
-        // the whole solve in one resident launch; the state it ends at (the
-        // stop, else k_start + max_iters) is tested afterwards as below
+    if (max_iters >= 1 && check_every >= 2 && tgv_resident_use(t)) {
+        // the whole solve in one resident launch (whose checks need
+        // check_every >= 2); the state it ends at (the stop, else k_start +
+        // max_iters) is tested afterwards as below
         bool ran = false;
         rc = tgv_resident_run(t, prm, max_iters, check_every, tol, e_star, st, &ran);
         if (rc != PSW_OK) return rc;
@@ -3322,7 +3421,7 @@ int psw_tgv_solve_batch(psw_tgv* const* solvers, int nsolvers, const psw_tgv_par
     // states as one at a time)
     std::vector<int> rest;
     std::vector<std::vector<int>> packs;
-    if (max_iters >= 1) packs = tgv_resident_packs(solvers, nsolvers, variant, rest);
+    if (max_iters >= 1 && check_every >= 2) packs = tgv_resident_packs(solvers, nsolvers, variant, rest);
     else for (int f = 0; f < nsolvers; ++f) rest.push_back(f);
     for (const auto& pk : packs) {
         std::vector<psw_tgv*> ts;
