@@ -14,12 +14,14 @@ intensities), pinhole K fx=fy=480, cx=319.5, cy=239.5; poses within 5 cm /
           alpha1=1.5, tau=0.02, sigma=1, iterated until the primal energy is
           at most 1e-4 (relative) above E*, checked every 10 iterations,
           capped at 50000.  E* is the energy the reference's start settles
-          at (computed per scene before the timed region): the plain start
-          iterated at least 40000 times, continued in chunks of 10000 until
-          a chunk changes the energy by less than 1e-6.  The rule is
-          one-sided because PDHG from another start can settle on a fixed
-          point a few 1e-4 below E* (reported per scene: the learned start
-          iterated as many times).  The energy leaves out the ~0.2 % of pixels where the reference's
+          at (computed per scene before the timed region): the energy never
+          settles exactly but moves in a band a few 1e-5 wide (the pixels
+          whose data step is unstable oscillate, and that reaches their
+          neighbours), so E* is the mean of the plain start's energy every
+          10000 iterations from 100000 to 200000 (the band is reported per
+          scene).  The rule is one-sided because PDHG from another start can
+          settle on a fixed point a few 1e-4 below E* (reported per scene:
+          the learned start's mean over the same iterations).  The energy leaves out the ~0.2 % of pixels where the reference's
           fixed steps make the data step unstable (tau sigma lambda^2
           sum Iu^2 > 1 at the pixel or a neighbour): the iteration oscillates
           there forever;
@@ -53,8 +55,8 @@ STD_T = 1e-4
 TGV = dict(lambda_=1.0, alpha0=2.0, alpha1=1.5, tau=0.02, sigma=1.0, beta=0.0, gamma=1.0)
 GAP_TOL = 1e-4
 MAX_ITERS = 50000
-ESTAR_ITERS = 40000
-ESTAR_CAP = 200000
+ESTAR_ITERS = 100000     # the plain start's iterations before E* is sampled
+ESTAR_END = 200000       # E*: the mean of its energy every 10000 iterations from ESTAR_ITERS to here
 FALLBACK_AFTER = 20000   # a learned solve not at the gap by then restarts from the plain start
 QMAX = 1.0   # energy mask: pixels with tau sigma lambda^2 sum Iu^2 > QMAX (or at a 4-neighbour) left out
 BF16_PEAK_TFLOPS = 2500.0
@@ -606,18 +608,23 @@ def main():
         pl.tgv.set_u(d_s, stream=s)
         pl.tgv.linearise(pl.K1, pl.Rrel, pl.trel, s)
         pl.excluded = pl.tgv.set_energy_mask(QMAX, pl.prm, s)
-        # iterate until the energy settles: at least estar_iters, then chunks
-        # of 10000 until a chunk changes it by less than 1e-6 (cap 200000)
-        pl.tgv.iterate(args.estar_iters, pl.prm, s)
-        e_prev, n_it = pl.tgv.energy(pl.prm, s), args.estar_iters
-        while True:
-            pl.tgv.iterate(10000, pl.prm, s)
-            n_it += 10000
-            e = pl.tgv.energy(pl.prm, s)
-            change, e_prev = (e_prev - e) / e, e
-            if abs(change) < 1e-6 or n_it >= ESTAR_CAP:
-                break
-        pl.e_star_plain, pl.estar_iters, pl.estar_change = e, n_it, change
+        # The energy does not settle: after some 10^4 iterations it moves in
+        # a band a few 1e-5 wide for good (the pixels whose data step is
+        # unstable oscillate, and that reaches their neighbours), so one
+        # sample would be an arbitrary point of the band.  E* is the mean of
+        # the samples every 10000 iterations from estar_iters to ESTAR_END.
+        def band(n0):
+            pl.tgv.iterate(n0, pl.prm, s)
+            es, n = [pl.tgv.energy(pl.prm, s)], n0
+            while n < max(ESTAR_END, n0):
+                pl.tgv.iterate(10000, pl.prm, s)
+                n += 10000
+                es.append(pl.tgv.energy(pl.prm, s))
+            return np.array(es), n
+        es, n_it = band(args.estar_iters)
+        pl.e_star_plain, pl.estar_iters = float(np.mean(es)), n_it
+        pl.estar_band = [float((es.min() - es.mean()) / es.mean()), float((es.max() - es.mean()) / es.mean())]
+        pl.estar_samples = len(es)
         pl.u_star = pl.tgv.get_u(s)
         pl.weight = pl.tgv.get_energy_mask(s)
         pl.d_s_host = d_s.to_host(s)
@@ -628,13 +635,13 @@ def main():
         pl.tgv.set_energy_mask(QMAX, pl.prm, s, count=False)
         pl.ws.predict(pl.sweep, ZFAR, s)
         pl.ws.start(pl.tgv, s)
-        pl.tgv.iterate(n_it, pl.prm, s)
-        pl.e_star_learned = pl.tgv.energy(pl.prm, s)
+        es_l, _ = band(args.estar_iters)
+        pl.e_star_learned = float(np.mean(es_l))
         pl.e_star = pl.e_star_plain
         st.synchronize()
-        log(f"seed {sd}: E* = {pl.e_star:.9e} after {n_it} iterations of the plain start (change over the "
-            f"last 10000: {change:.2e}); the learned start after as many: "
-            f"{(pl.e_star_learned - pl.e_star) / pl.e_star:+.2e} relative; "
+        log(f"seed {sd}: E* = {pl.e_star:.9e}, the plain start's mean energy over {pl.estar_samples} samples "
+            f"up to {n_it} iterations (band {pl.estar_band[0]:+.1e} .. {pl.estar_band[1]:+.1e}); the learned "
+            f"start's mean over the same: {(pl.e_star_learned - pl.e_star) / pl.e_star:+.2e} relative; "
             f"{pl.excluded} pixels outside the energy mask")
         pipes.append(pl)
 
@@ -670,8 +677,8 @@ def main():
     for j, pl in enumerate(pipes):
         s = pl.st.handle
         rec = {"seed": pl.seed, "e_star": pl.e_star, "estar_iters": pl.estar_iters,
-               "estar_change_last_10000": pl.estar_change,
-               "learned_start_settled_rel": (pl.e_star_learned - pl.e_star) / pl.e_star,
+               "estar_samples": pl.estar_samples, "estar_band_rel": pl.estar_band,
+               "learned_start_band_mean_rel": (pl.e_star_learned - pl.e_star) / pl.e_star,
                "excluded_pixels": pl.excluded}
         u_stop = {}
         for mode, (_, recs) in res.items():
@@ -779,9 +786,9 @@ def main():
         "config": {"workload": "learned warm-start pipeline: 640x480 reference + 8 sources, 256 inverse-depth planes "
                                "0.3-8 m, 9x9 NCC, fused sweep+argmin, BF16 U-Net warm start (plain start as "
                                "fallback), TGV2 linearised at the sweep depth until the energy is at most 1e-4 "
-                               "(relative) above E* (the plain start's settled energy: >= "
-                               f"{args.estar_iters} iterations until a 10000-iteration chunk changes it by "
-                               f"< 1e-6); steps cycle over {len(seeds)} scenes",
+                               "(relative) above E* (the mean of the plain start's energy every 10000 iterations "
+                               f"from {args.estar_iters} to {max(ESTAR_END, args.estar_iters)}: it moves in a band "
+                               f"of a few 1e-5 for good); steps cycle over {len(seeds)} scenes",
                    "sweep_arith": "fast" if fast else "reference-order", "tgv_gap_tol": GAP_TOL,
                    "check_every": args.check_every, "max_iters": args.max_iters, "estar_iters": args.estar_iters,
                    "seeds": seeds, "network_dtype": "bf16 (fp32 accumulation)"},

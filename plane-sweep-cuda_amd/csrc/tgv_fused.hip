@@ -267,17 +267,41 @@ __global__ void tgv_mask_kernel(const float* __restrict__ Iu, int W, int H, int 
 
 // ---- the fused primal-dual iteration --------------------------------------
 
-// Point updates shared by the one- and two-iteration kernels (same
-// expression order, so both give identical results); the dual update
-// pq_update is in tgv_point.h.
+// Point updates shared by every dense TGV2 kernel (the same operations in
+// the same order, so all give identical results): the reference's
+// expressions with each product that feeds an addition fused into it (one
+// rounding, as a contracting compiler such as nvcc by default emits them);
+// the dual update pq_update is in tgv_point.h.
 
 // TGV2_updateR at one pixel and source (uses u_k, not ubar).
 __device__ __forceinline__ float r_update(float r, float it, float du, float iu, float sl) {
-    const float rr = r + sl * (it + du * iu);
+    const float rr = __builtin_fmaf(sl, __builtin_fmaf(du, iu, it), r);
     // rr / fmaxf(1, |rr|): rr itself for |rr| <= 1, exactly +-1 above, which
     // is the median of (rr, -1, 1) for every finite rr (one instruction)
     return __builtin_amdgcn_fmed3f(rr, -1.f, 1.f);
 }
+
+// The primal steps of TGV2_updateU_tensor_weighed with the products fused
+// into the following additions (as a compiler that contracts them would):
+// u - tau (-alpha1 (T (c - l) + T (c - up)) + lambda ps), and
+// u1 - tau (-alpha1 (T c) - alpha0 div q) for one component of u1.
+__device__ __forceinline__ float u_step(float u, float t11, float t12, float t22, float c_px, float c_py, float l_px,
+                                        float l_py, float u_px, float u_py, float ps, float tau, float alpha1,
+                                        float lambda) {
+    float acc = t11 * (c_px - l_px);
+    acc = __builtin_fmaf(t12, c_py - l_py, acc);
+    acc = __builtin_fmaf(t12, c_px - u_px, acc);
+    acc = __builtin_fmaf(t22, c_py - u_py, acc);
+    return __builtin_fmaf(-tau, __builtin_fmaf(-alpha1, acc, lambda * ps), u);
+}
+
+__device__ __forceinline__ float u1_step(float u1, float ta, float tb, float c_px, float c_py, float divq, float tau,
+                                         float alpha1, float alpha0) {
+    return __builtin_fmaf(-tau, __builtin_fmaf(-alpha1, __builtin_fmaf(ta, c_px, tb * c_py), -alpha0 * divq), u1);
+}
+
+// The data dual's share of the u step: ps + r Iu.
+__device__ __forceinline__ float ps_add(float ps, float r, float iu) { return __builtin_fmaf(r, iu, ps); }
 
 // TGV2_updateU_tensor_weighed at one pixel: c = own duals, l = left
 // neighbour's, u = upper neighbour's (own at the first column / row).
@@ -285,10 +309,9 @@ __device__ __forceinline__ void u_update(float u, float u1x, float u1y, float t1
                                          float c_py, float l_px, float l_py, float u_px, float u_py, float q0c,
                                          float q0l, float q1c, float q1u, float q2c, float q2l, float q2u, float ps,
                                          const TgvPtrs& a, float& un, float& u1xn, float& u1yn) {
-    un = u - a.tau * (-a.alpha1 * (t11 * (c_px - l_px) + t12 * (c_py - l_py) + t12 * (c_px - u_px) + t22 * (c_py - u_py)) +
-                      a.lambda * ps);
-    u1xn = u1x - a.tau * (-a.alpha1 * (t11 * c_px + t12 * c_py) - a.alpha0 * (q0c - q0l + q2c - q2u));
-    u1yn = u1y - a.tau * (-a.alpha1 * (t12 * c_px + t22 * c_py) - a.alpha0 * (q2c - q2l + q1c - q1u));
+    un = u_step(u, t11, t12, t22, c_px, c_py, l_px, l_py, u_px, u_py, ps, a.tau, a.alpha1, a.lambda);
+    u1xn = u1_step(u1x, t11, t12, c_px, c_py, q0c - q0l + q2c - q2u, a.tau, a.alpha1, a.alpha0);
+    u1yn = u1_step(u1y, t12, t22, c_px, c_py, q2c - q2l + q1c - q1u, a.tau, a.alpha1, a.alpha0);
 }
 
 // Energy stop test inside the iteration launches.  The launch that starts
@@ -471,7 +494,7 @@ __device__ __forceinline__ void tgv_iter_tile(const TgvPtrs& a, const int tile, 
             if (j < a.N) {
                 const float rr = r_update(rv[j], itv[j], du, iuv[j], sl);
                 a.rnext[(size_t)j * n + g] = rr;
-                ps += rr * iuv[j];
+                ps = ps_add(ps, rr, iuv[j]);
                 if (CHECK) ed += fabs((double)itv[j] + dud * (double)iuv[j]);
             }
         }
@@ -644,7 +667,7 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile,
             if (j < a.N) {
                 if (ENERGY) ed += fabs((double)itv[j] + dud * (double)iuv[j]);
                 rv[j] = r_update(rv[j], itv[j], du, iuv[j], sl);
-                ps += rv[j] * iuv[j];
+                ps = ps_add(ps, rv[j], iuv[j]);
             }
         }
         if (ENERGY && x < W && y < H && chk->mask[gt] != 0.f)
@@ -676,7 +699,7 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile,
         for (int j = 0; j < a.N; ++j) {
             const size_t gj = (size_t)j * n + g;
             const float iu = a.Iu[gj];
-            ps += r_update(a.rcur[gj], a.It[gj], du, iu, sl) * iu;
+            ps = ps_add(ps, r_update(a.rcur[gj], a.It[gj], du, iu, sl), iu);
         }
         const int bx = ix + 1, by = iy + 1;
         const int lx = gx > 0 ? bx - 1 : bx, uy = gy > 0 ? by - 1 : by;
@@ -723,7 +746,7 @@ __device__ __forceinline__ void tgv_iter2_tile(const TgvPtrs& a, const int tile,
             if (j < a.N) {
                 const float rr = r_update(rv[j], itv[j], du, iuv[j], sl);
                 a.rnext2[(size_t)j * n + g] = rr;
-                ps += rr * iuv[j];
+                ps = ps_add(ps, rr, iuv[j]);
             }
         }
         const int bx = tx + 2, by = ty + 2;               // B index of this pixel (D index + 1)
@@ -859,7 +882,7 @@ __device__ __forceinline__ void tgv_iter3_tile(const TgvPtrs& a, const int tile,
         for (int j = 0; j < NS; ++j) {
             if (j < a.N) {
                 rv[j] = r_update(rv[j], itv[j], du, iuv[j], sl);
-                ps += rv[j] * iuv[j];
+                ps = ps_add(ps, rv[j], iuv[j]);
             }
         }
         const int bx = ux + 1, by = uy + 1;
@@ -895,7 +918,7 @@ __device__ __forceinline__ void tgv_iter3_tile(const TgvPtrs& a, const int tile,
                 const size_t gj = (size_t)j * n + rg;
                 const float iu = a.Iu[gj];
                 rr1[j] = r_update(a.rcur[gj], a.It[gj], du, iu, sl);
-                ps += rr1[j] * iu;
+                ps = ps_add(ps, rr1[j], iu);
             }
         }
         const int bx = rux + 1, by = ruy + 1;
@@ -935,7 +958,7 @@ __device__ __forceinline__ void tgv_iter3_tile(const TgvPtrs& a, const int tile,
         for (int j = 0; j < NS; ++j) {
             if (j < a.N) {
                 rv[j] = r_update(rv[j], itv[j], du, iuv[j], sl);
-                ps += rv[j] * iuv[j];
+                ps = ps_add(ps, rv[j], iuv[j]);
             }
         }
         const int bx = ux + 1, by = uy + 1;
@@ -954,7 +977,7 @@ __device__ __forceinline__ void tgv_iter3_tile(const TgvPtrs& a, const int tile,
         for (int j = 0; j < NS; ++j) {
             if (j < a.N) {
                 const float iu = a.Iu[(size_t)j * n + rg];
-                ps += r_update(rr1[j], a.It[(size_t)j * n + rg], du, iu, sl) * iu;
+                ps = ps_add(ps, r_update(rr1[j], a.It[(size_t)j * n + rg], du, iu, sl), iu);
             }
         }
         const int bx = rux + 1, by = ruy + 1;
@@ -1000,7 +1023,7 @@ __device__ __forceinline__ void tgv_iter3_tile(const TgvPtrs& a, const int tile,
             if (j < a.N) {
                 const float rr = r_update(rv[j], itv[j], du, iuv[j], sl);
                 a.rnext3[(size_t)j * n + g] = rr;
-                ps += rr * iuv[j];
+                ps = ps_add(ps, rr, iuv[j]);
             }
         }
         const int bx = tx + 3, by = ty + 3;
@@ -2048,7 +2071,7 @@ __device__ __forceinline__ void tgv_resident_body(const TgvResArgs& R, const int
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
                     rv[h][s] = r_update(rv[h][s], itv[h][s], du, iuv[h][s], sl);
-                    ps[h] += rv[h][s] * iuv[h][s];
+                    ps[h] = ps_add(ps[h], rv[h][s], iuv[h][s]);
                 }
             }
         };
@@ -2168,7 +2191,7 @@ __device__ __forceinline__ void tgv_resident_body(const TgvResArgs& R, const int
                 }
             }
         }
-        // u of j+1 (the reference's expression order, as u_update; the upper
+        // u of j+1 (u_step and u1_step, as u_update; the upper
         // pixel of the pair is the lower one's up neighbour), then the
         // tile's edges of state j+1 for the neighbours
         const bool publish = j + 1 < R.niters;
@@ -2185,13 +2208,12 @@ __device__ __forceinline__ void tgv_resident_body(const TgvResArgs& R, const int
             const float u_qz = !has_u ? qz[h] : (h == 1 ? qz[0] : sPQ[4][row][tx + 1]);
             const float c_px = px[h], c_py = py[h];
             const float t11h = sT[0][row + 1][tx + 1], t12h = sT[1][row + 1][tx + 1], t22h = sT[2][row + 1][tx + 1];
-            const float un = u[h] - R.tau * (-R.alpha1 * (t11h * (c_px - l_px) + t12h * (c_py - l_py) +
-                                                         t12h * (c_px - u_px) + t22h * (c_py - u_py)) +
-                                              R.lambda * ps[h]);
-            const float u1xn = u1x[h] - R.tau * (-R.alpha1 * (t11h * c_px + t12h * c_py) -
-                                                 R.alpha0 * (qx[h] - l_qx + qz[h] - u_qz));
-            const float u1yn = u1y[h] - R.tau * (-R.alpha1 * (t12h * c_px + t22h * c_py) -
-                                                 R.alpha0 * (qz[h] - l_qz + qy[h] - u_qy));
+            const float un = u_step(u[h], t11h, t12h, t22h, c_px, c_py, l_px, l_py, u_px, u_py, ps[h], R.tau, R.alpha1,
+                                    R.lambda);
+            const float u1xn = u1_step(u1x[h], t11h, t12h, c_px, c_py, qx[h] - l_qx + qz[h] - u_qz, R.tau, R.alpha1,
+                                       R.alpha0);
+            const float u1yn = u1_step(u1y[h], t12h, t22h, c_px, c_py, qz[h] - l_qz + qy[h] - u_qy, R.tau, R.alpha1,
+                                       R.alpha0);
             v[h] = u[h]; v1x[h] = u1x[h]; v1y[h] = u1y[h];
             u[h] = un; u1x[h] = u1xn; u1y[h] = u1yn;
             if (publish && x < W && y < H) {

@@ -30,25 +30,26 @@ __device__ __forceinline__ float with_exp(uint64_t h, int e) {
 }
 
 // The projection as the reference writes it (TGV2_updateP_tensor_weighed,
-// TGV2_updateQ): x / fmaxf(1, sqrt(s)), with sqrtf and IEEE division.
+// TGV2_updateQ): x / fmaxf(1, sqrt(s)), with sqrtf and IEEE division; the
+// steps before it are the kernels' (products fused into the additions).
 __device__ bool pq_plain(float& px, float& py, float& qx, float& qy, float& qz, float t11, float t12, float t22,
                          float ubc, float ubr, float ubd, float u1xc, float u1xr, float u1xd, float u1yc, float u1yr,
                          float u1yd, float as1, float as0) {
     const float gxv = ubr - ubc - u1xc;
     const float gyv = ubd - ubc - u1yc;
-    px = px + as1 * (t11 * gxv + t12 * gyv);
-    py = py + as1 * (t12 * gxv + t22 * gyv);
-    const float dp = fmaxf(1.f, sqrtf(px * px + py * py));
+    px = __builtin_fmaf(as1, __builtin_fmaf(t11, gxv, t12 * gyv), px);
+    py = __builtin_fmaf(as1, __builtin_fmaf(t12, gxv, t22 * gyv), py);
+    const float dp = fmaxf(1.f, sqrtf(__builtin_fmaf(px, px, py * py)));
     px = px / dp;
     py = py / dp;
     const float dx_u1x = u1xr - u1xc;
     const float dy_u1x = u1xd - u1xc;
     const float dx_u1y = u1yr - u1yc;
     const float dy_u1y = u1yd - u1yc;
-    qx = qx + as0 * dx_u1x;
-    qy = qy + as0 * dy_u1y;
-    qz = qz + as0 * (dy_u1x + dx_u1y) / 2.0f;
-    const float dq = fmaxf(1.f, sqrtf(qx * qx + qy * qy + qz * qz + qz * qz));
+    qx = __builtin_fmaf(as0, dx_u1x, qx);
+    qy = __builtin_fmaf(as0, dy_u1y, qy);
+    qz = __builtin_fmaf(as0, 0.5f * (dy_u1x + dx_u1y), qz);
+    const float dq = fmaxf(1.f, sqrtf(__builtin_fmaf(qz, qz, __builtin_fmaf(qz, qz, __builtin_fmaf(qy, qy, qx * qx)))));
     qx = qx / dq;
     qy = qy / dq;
     qz = qz / dq;

@@ -295,8 +295,11 @@ def test_tgv_estar_is_the_fp64_fixed_point_energy(problem640):
 def test_tgv_solve_to_gap_vs_fp64(problem640, start):
     """The solve from the plain start (u = u0, the rest 0) and from the
     learned start stops within the gap; the stop state against the FP64
-    oracle run from the same start for the same number of iterations, and
-    the oracle's energy there within the gap too."""
+    oracle run from the same start for the same number of iterations; that
+    run's energy within a tenth of the gap of the solver's, and the FP64 run
+    inside the gap no more than 5 % later (the FP32 and FP64 trajectories
+    differ by about 1e-6 of E*, which a stop at the first check inside the
+    gap can straddle)."""
     P = problem640
     pl = P.pl
     s = pl.st.handle
@@ -325,10 +328,16 @@ def test_tgv_solve_to_gap_vs_fp64(problem640, start):
     # the stop rule is one-sided (E - E* <= tol E*); PDHG from another start
     # may settle a little below E*, never far below the long plain run's
     assert -1e-3 * P.e_star <= e - P.e_star <= GAP_TOL * P.e_star
-    assert e64 - P.e_star <= GAP_TOL * P.e_star
     assert abs(e_gpu64 - e) <= 1e-10 * P.e_star         # the GPU's FP64 reduction of its own state
     assert abs(e - e64) <= 1e-5 * P.e_star               # the stop decision: GPU vs FP64 run, a tenth of the gap
     _tgv_bars(S["u"], st64["u"], P.dist, f"{start} start, stop state", u32=st32["u"])
+    more, e64m, stm = 0, e64, st64
+    while e64m - P.e_star > GAP_TOL * P.e_star and more < it // 20:
+        stm = P.iterate64(stm, 10)
+        more += 10
+        e64m = P.energy64(stm)
+    print(f"{start}: the FP64 run is inside the gap after {it + more} iterations ({more} more)", flush=True)
+    assert e64m - P.e_star <= GAP_TOL * P.e_star
 
 
 # ---- fusion ----------------------------------------------------------------------------
