@@ -465,12 +465,13 @@ def probe(names, seed):
     log("probe done")
 
 
-def run_batch(args):
+def run_batch(args, emit=True):
     """The uneven batch: 32 seeded frames, resolution from
     {320x240, 640x480, 1242x375}, 4-8 sources, 64-256 planes per frame; sweep
     and TGV2 (plain start, linearised at the sweep depth, to the energy gap)
     for the whole batch in batched launches, against the same frames one at
-    a time through the single-frame entry points."""
+    a time through the single-frame entry points.  Prints its own JSON line
+    when `emit`; returns the record."""
     import psw_amd
     from psw_amd import batch, stages
     from psw_amd.device import Stream
@@ -490,6 +491,7 @@ def run_batch(args):
     solvers = [f.tgv for f in frames]
     sweeps = [f.sweep for f in frames]
 
+    ty_env = os.environ.get("PSW_SWEEP_TY")
     os.environ["PSW_SWEEP_TY"] = str(best["sweep_tile_h"])
 
     def sweep_all():
@@ -552,7 +554,13 @@ def run_batch(args):
     }
     log(f"batch: {t_batch * 1e3:.1f} ms for {len(frames)} frames ({len(frames) / t_batch:.2f} depthmaps/s); "
         f"one at a time: {t_seq * 1e3:.1f} ms")
-    print(json.dumps(rec), flush=True)
+    if ty_env is None:
+        os.environ.pop("PSW_SWEEP_TY", None)
+    else:
+        os.environ["PSW_SWEEP_TY"] = ty_env
+    if emit:
+        print(json.dumps(rec), flush=True)
+    return rec
 
 
 def main():
@@ -573,6 +581,7 @@ def main():
     ap.add_argument("--seed", type=int, default=SEEDS[0], help="scene of --probe and --batch")
     ap.add_argument("--no-fusion", action="store_true", help="skip the 256^3 fusion measurement")
     ap.add_argument("--batch", action="store_true", help="run the uneven 32-frame batch instead (its own JSON line)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the uneven 32-frame batch in the default run")
     ap.add_argument("--batch-frames", type=int, default=32)
     ap.add_argument("--batch-estar-iters", type=int, default=20000)
     ap.add_argument("--probe", default="", help="comma list of tgv,tgv_resident,sweep,sweep_exact,sweep_shapes,unet,copy,fusion,"
@@ -844,6 +853,13 @@ def main():
         rec["fusion"] = run_fusion(o, thr, Stream(), [Event() for _ in range(3)])
         log(f"fusion: histogram {rec['fusion']['hist_ms']:.2f} ms, {FUSION_ITERS} PD iterations "
             f"{rec['fusion']['pd_ms']:.2f} ms")
+    if not args.no_batch:
+        b = run_batch(args, emit=False)
+        rec["uneven_batch"] = {k: b[k] for k in ("value", "unit", "batch_ms", "sequential_ms", "speedup_vs_sequential",
+                                                 "max_rel_gap_at_stop")}
+        rec["uneven_batch"].update(workload=b["config"]["workload"], megapixels=b["config"]["megapixels"],
+                                   iterations_range=[min(b["iterations"]), max(b["iterations"])],
+                                   autotuned=b["autotune"]["chosen"])
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
         vh = {"ref": pl.v["d_ref"].to_host(), "src": pl.v["d_src"].to_host(), "Rref": pl.v["Rref"],
