@@ -2135,7 +2135,10 @@ __device__ __forceinline__ void tgv_resident_body(const TgvResArgs& R, const int
             float rt[3], dn[3];
 #pragma unroll
             for (int f = 0; f < 3; ++f) {
-                rt[f] = has_r ? sUb[f][row + 1][tx + 2] : ub[h][f];
+                // loaded by every lane (the cell exists; unused at the last
+                // column), then selected: no branch on the lane's column
+                const float right = sUb[f][row + 1][tx + 2];
+                rt[f] = has_r ? right : ub[h][f];
                 dn[f] = !has_d ? ub[h][f] : (h == 0 ? ub[1][f] : sUb[f][row + 2][tx + 1]);
             }
             pq_update(px[h], py[h], qx[h], qy[h], qz[h], sT[0][row + 1][tx + 1], sT[1][row + 1][tx + 1],
@@ -2199,8 +2202,10 @@ __device__ __forceinline__ void tgv_resident_body(const TgvResArgs& R, const int
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int row = 2 * wv + h, y = y0 + row;
-            const float l_px = has_l ? sPQ[0][row + 1][tx] : px[h], l_py = has_l ? sPQ[1][row + 1][tx] : py[h];
-            const float l_qx = has_l ? sPQ[2][row + 1][tx] : qx[h], l_qz = has_l ? sPQ[4][row + 1][tx] : qz[h];
+            const float lp0 = sPQ[0][row + 1][tx], lp1 = sPQ[1][row + 1][tx];   // every lane, then select
+            const float lq0 = sPQ[2][row + 1][tx], lq2 = sPQ[4][row + 1][tx];
+            const float l_px = has_l ? lp0 : px[h], l_py = has_l ? lp1 : py[h];
+            const float l_qx = has_l ? lq0 : qx[h], l_qz = has_l ? lq2 : qz[h];
             const bool has_u = h == 1 || has_u0;
             const float u_px = !has_u ? px[h] : (h == 1 ? px[0] : sPQ[0][row][tx + 1]);
             const float u_py = !has_u ? py[h] : (h == 1 ? py[0] : sPQ[1][row][tx + 1]);
