@@ -124,7 +124,7 @@ __device__ __forceinline__ void sweep_item(const SweepArgs& a, const int item) {
     static_assert(NCOL <= SW_THREADS, "window too large for the tile");
     static_assert(TY % SW_NCG == 0 && (SW_TX * TY) % SW_THREADS == 0, "tile height");
     __shared__ float sR[TH * TW];
-    __shared__ float sW[TH * TW];
+    __shared__ __align__(16) float sW[TH * TW];
     __shared__ float sC[3][TY * TW];
 
     const int t = item / a.G, g = item - (item / a.G) * a.G;
@@ -137,18 +137,24 @@ __device__ __forceinline__ void sweep_item(const SweepArgs& a, const int item) {
 
     // Actual (reflected) image coordinate of each halo-tile position this
     // thread warps; positions past W-1+R / H-1+R only feed discarded pixels.
+    // The reference-order sweep takes positions i = tid + k * threads; the
+    // fast one takes pairs (2q, 2q+1) of one row (TW is even), which share
+    // the homography's row terms.
     constexpr int NWARP = (TH * TW + SW_THREADS - 1) / SW_THREADS;
-    int wgx[NWARP], wgy[NWARP];
+    constexpr int NPAIR = (TH * TW / 2 + SW_THREADS - 1) / SW_THREADS;
+    static_assert(TW % 2 == 0, "pairs of one row");
+    constexpr int NPOS = FAST ? 2 * NPAIR : NWARP;
+    int wgx[NPOS], wgy[NPOS];
 #pragma unroll
-    for (int k = 0; k < NWARP; ++k) {
-        const int i = tid + k * SW_THREADS;
+    for (int k = 0; k < NPOS; ++k) {
+        const int i = FAST ? 2 * (tid + (k / 2) * SW_THREADS) + (k & 1) : tid + k * SW_THREADS;
         const int ty = i / TW, tx = i - (i / TW) * TW;
         wgx[k] = reflect101(min(x0 - R + tx, W - 1 + R), W);
         wgy[k] = reflect101(min(y0 - R + ty, H - 1 + R), H);
     }
 #pragma unroll
-    for (int k = 0; k < NWARP; ++k) {
-        const int i = tid + k * SW_THREADS;
+    for (int k = 0; k < NPOS; ++k) {
+        const int i = FAST ? 2 * (tid + (k / 2) * SW_THREADS) + (k & 1) : tid + k * SW_THREADS;
         if (i < TH * TW) sR[i] = a.ref[(size_t)wgy[k] * W + wgx[k]];
     }
     __syncthreads();
@@ -209,30 +215,42 @@ __device__ __forceinline__ void sweep_item(const SweepArgs& a, const int item) {
         const float h0 = Hm[0], h1 = Hm[1], h2 = Hm[2], h3 = Hm[3], h4 = Hm[4], h5 = Hm[5], h6 = Hm[6],
                     h7 = Hm[7], h8 = Hm[8];
         const float offf = (float)a.off;
+        if constexpr (FAST) {
 #pragma unroll
-        for (int k = 0; k < NWARP; ++k) {
-            const int i = tid + k * SW_THREADS;
-            if (i < TH * TW) {
-                const float px = (float)(wgx[k] + a.off), py = (float)(wgy[k] + a.off);
-                float xs, ys;
-                if constexpr (FAST) {
+            for (int k = 0; k < NPAIR; ++k) {
+                const int q = tid + k * SW_THREADS;
+                if (q < TH * TW / 2) {
                     // explicit fused multiply-adds (no contraction left to the
                     // compiler), so every kernel that inlines this computes
-                    // the same bits
-                    const float vx = __builtin_fmaf(h0, px, __builtin_fmaf(h1, py, h2));
-                    const float vy = __builtin_fmaf(h3, px, __builtin_fmaf(h4, py, h5));
-                    const float vz = __builtin_fmaf(h6, px, __builtin_fmaf(h7, py, h8));
-                    const float rz = __builtin_amdgcn_rcpf(vz);
-                    xs = __builtin_fmaf(vx, rz, -offf);
-                    ys = __builtin_fmaf(vy, rz, -offf);
-                } else {
+                    // the same bits; the row terms are the pair's
+                    const float py = (float)(wgy[2 * k] + a.off);
+                    const float cx = __builtin_fmaf(h1, py, h2), cy = __builtin_fmaf(h4, py, h5),
+                                cz = __builtin_fmaf(h7, py, h8);
+                    float out[2];
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const float px = (float)(wgx[2 * k + e] + a.off);
+                        const float vx = __builtin_fmaf(h0, px, cx);
+                        const float vy = __builtin_fmaf(h3, px, cy);
+                        const float vz = __builtin_fmaf(h6, px, cz);
+                        const float rz = __builtin_amdgcn_rcpf(vz);
+                        out[e] = bilinear_sample_fma(src, W, H, __builtin_fmaf(vx, rz, -offf),
+                                                     __builtin_fmaf(vy, rz, -offf));
+                    }
+                    *reinterpret_cast<float2*>(&sW[2 * q]) = make_float2(out[0], out[1]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < NWARP; ++k) {
+                const int i = tid + k * SW_THREADS;
+                if (i < TH * TW) {
+                    const float px = (float)(wgx[k] + a.off), py = (float)(wgy[k] + a.off);
                     const float vx = dot3(h0, h1, h2, px, py, 1.f);
                     const float vy = dot3(h3, h4, h5, px, py, 1.f);
                     const float vz = dot3(h6, h7, h8, px, py, 1.f);
-                    xs = vx / vz - offf;
-                    ys = vy / vz - offf;
+                    sW[i] = bilinear_sample(src, W, H, vx / vz - offf, vy / vz - offf);
                 }
-                sW[i] = bilinear_sample(src, W, H, xs, ys);
             }
         }
         __syncthreads();
