@@ -63,6 +63,7 @@ BF16_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 SWEEP_FLOP_PER_EVAL = 50          # declared FLOP per (pixel, plane, source) NCC evaluation
+SWEEP_HBM_BYTES = 4 * W * H * (1 + NSRC) + 16 * W * H   # images read once, outputs and keys once
 TGV_SURVEY_FLOATS_PER_PX = 27 + 4 * NSRC   # floats per pixel of one unfused iteration
 SEEDS = tuple(range(2024, 2032))           # eight scenes; the network was trained on seeds from 100000
 
@@ -78,6 +79,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _median_run(fn, reps=3):
+    """Seconds: one untimed warm-up run, then the median of `reps` timed runs."""
+    fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
 def cpu_baseline(v, planes, K0, K1, ds, plain_iters, fusion_o=None, fusion_thr=None, sample_iters=300):
     """FP32 oracle (the CPU restatement, OpenMP) on the same inputs.  The
     headline: the full aggregated sweep of one depthmap, then `sample_iters`
@@ -86,31 +98,29 @@ def cpu_baseline(v, planes, K0, K1, ds, plain_iters, fusion_o=None, fusion_thr=N
     pipeline's iteration count (the CPU path has no learned start).  Beside
     it: the small plain pipeline (640x480, 4 sources, 64 planes over 0.5-5 m,
     7x7, 200 cold-start TGV2 iterations) and fusion primal-dual iterations on
-    a 128^3 sub-grid of the fusion volume, per voxel-iteration."""
+    a 128^3 sub-grid of the fusion volume, per voxel-iteration.  Every time is
+    the median of three runs after a warm-up run."""
     import oracle
     from psw_amd import stages
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    t0 = time.perf_counter()
-    oracle.sweep_agg(v["ref"], v["src"], K0, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], planes, WIN, STD_T)
-    t_sweep = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    oracle.tgv(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"], sample_iters, 1, u_init=ds, **TGV)
-    t_it = (time.perf_counter() - t0) / sample_iters
+    t_sweep = _median_run(lambda: oracle.sweep_agg(v["ref"], v["src"], K0, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"],
+                                                   planes, WIN, STD_T))
+    t_it = _median_run(lambda: oracle.tgv(v["ref"], v["src"], K1, v["Rref"], v["tref"], v["Rsrc"], v["tsrc"],
+                                          sample_iters, 1, u_init=ds, **TGV)) / sample_iters
     t_map = t_sweep + t_it * plain_iters
     out = {"value": 1.0 / t_map, "unit": "depthmaps/s", "cores": cores, "kind": "port",
            "sample": f"FP32 oracle: one full aggregated sweep ({NPLANES} planes x {NSRC} sources, 640x480) and "
                      f"{sample_iters} TGV2 iterations, scaled to the plain pipeline's {plain_iters} iterations",
+           "timing": "each time the median of 3 runs after 1 warm-up run",
            "sweep_s": t_sweep, "tgv_iter_ms": t_it * 1e3}
     # the small plain pipeline: 4 sources, 64 planes, 7x7, 200 cold-start iterations
     n4 = SMALL_NSRC
     src4 = np.ascontiguousarray(v["src"][:n4])
     p64 = stages.inverse_depth_planes(SMALL_ZNEAR, SMALL_ZFAR, SMALL_PLANES)
-    t0 = time.perf_counter()
-    oracle.sweep_agg(v["ref"], src4, K0, v["Rref"], v["tref"], v["Rsrc"][:n4], v["tsrc"][:n4], p64, SMALL_WIN, STD_T)
-    t_s4 = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    oracle.tgv(v["ref"], src4, K1, v["Rref"], v["tref"], v["Rsrc"][:n4], v["tsrc"][:n4], SMALL_ITERS, 1, **TGV)
-    t_t4 = time.perf_counter() - t0
+    t_s4 = _median_run(lambda: oracle.sweep_agg(v["ref"], src4, K0, v["Rref"], v["tref"], v["Rsrc"][:n4],
+                                                v["tsrc"][:n4], p64, SMALL_WIN, STD_T))
+    t_t4 = _median_run(lambda: oracle.tgv(v["ref"], src4, K1, v["Rref"], v["tref"], v["Rsrc"][:n4], v["tsrc"][:n4],
+                                          SMALL_ITERS, 1, **TGV))
     out["small_pipeline"] = {"value": 1.0 / (t_s4 + t_t4), "unit": "depthmaps/s", "sweep_s": t_s4, "tgv_s": t_t4,
                              "sample": "640x480, 4 sources, 64 planes 0.5-5 m, 7x7, TGV2 cold start 200 iterations"}
     if fusion_o is not None:
@@ -121,11 +131,11 @@ def cpu_baseline(v, planes, K0, K1, ds, plain_iters, fusion_o=None, fusion_thr=N
                                fusion_o["t"][k], fusion_thr)
         u, vv, pp = (np.zeros((g, g, g), np.float32), np.zeros((g, g, g), np.float32),
                      np.zeros((3, g, g, g), np.float32))
-        t0 = time.perf_counter()
-        for _ in range(n_it):
-            oracle.fusion_u(u, vv, pp, hist, 0.1, 0.5)
-            oracle.fusion_p(pp, vv, 1.0)
-        t_pd = time.perf_counter() - t0
+        def pd():
+            for _ in range(n_it):
+                oracle.fusion_u(u, vv, pp, hist, 0.1, 0.5)
+                oracle.fusion_p(pp, vv, 1.0)
+        t_pd = _median_run(pd)
         out["fusion_pd"] = {"value": g ** 3 * n_it / t_pd, "unit": "voxel-iterations/s",
                             "sample": f"{g}^3 sub-grid of the fusion volume, histograms of the same "
                                       f"{fusion_o['depths'].shape[0]} depthmaps, {n_it} U+P iterations"}
@@ -814,6 +824,9 @@ def main():
         "plain": {"ms_per_depthmap": plain_ms, "depthmaps_per_s": 1e3 / plain_ms, "runs": args.steps},
         "cold": ({"ms_per_depthmap": res["cold"][0], "depthmaps_per_s": 1e3 / res["cold"][0], "runs": args.steps}
                  if "cold" in res else None),
+        "tgv_only": {mode: {"ms_per_depthmap": float(np.mean([r["tgv_ms"] for rs in recs for r in rs])),
+                            "depthmaps_per_s": 1e3 / float(np.mean([r["tgv_ms"] for rs in recs for r in rs]))}
+                     for mode, (_, recs) in res.items()},
         "per_seed": per_seed,
         "roofline": {"kernel": tgv_kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": tsrc,
@@ -833,7 +846,10 @@ def main():
                                  "tile edges between workgroups and VALU issue, not HBM; the HBM roofline above is of "
                                  "the launch sequence's kernel (batched frames, frames above one tile per CU)"},
         "stages": {"sweep_kernel_ms": sweep_kernel_ms, "sweep_tflops_declared": sweep_tflops,
-                   "sweep_frac_fp32_peak": sweep_tflops / FP32_PEAK_TFLOPS, "unet_ms": net_ms,
+                   "sweep_frac_fp32_peak": sweep_tflops / FP32_PEAK_TFLOPS,
+                   # algorithmic bytes: the images read once, outputs written once
+                   "sweep_hbm_bytes": SWEEP_HBM_BYTES, "sweep_frac_hbm_peak":
+                       SWEEP_HBM_BYTES / (sweep_kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "unet_ms": net_ms,
                    "unet_tflops": net_tflops, "unet_frac_bf16_peak": net_tflops / BF16_PEAK_TFLOPS,
                    "unet_fwd_gflop": warmstart.forward_flops(W, H) / 1e9},
     }
@@ -857,9 +873,15 @@ def main():
         b = run_batch(args, emit=False)
         rec["uneven_batch"] = {k: b[k] for k in ("value", "unit", "batch_ms", "sequential_ms", "speedup_vs_sequential",
                                                  "max_rel_gap_at_stop")}
+        lat = {}
+        for fr, ms in zip(b["config"]["frames"], b["sequential_per_frame_ms"]):
+            lat.setdefault(f"{fr['W']}x{fr['H']}", []).append(ms)
         rec["uneven_batch"].update(workload=b["config"]["workload"], megapixels=b["config"]["megapixels"],
                                    iterations_range=[min(b["iterations"]), max(b["iterations"])],
-                                   autotuned=b["autotune"]["chosen"])
+                                   autotuned=b["autotune"]["chosen"],
+                                   sequential_latency_ms={k: {"frames": len(v), "min": float(np.min(v)),
+                                                              "median": float(np.median(v)), "max": float(np.max(v))}
+                                                          for k, v in lat.items()})
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
         vh = {"ref": pl.v["d_ref"].to_host(), "src": pl.v["d_src"].to_host(), "Rref": pl.v["Rref"],
