@@ -87,16 +87,79 @@ __host__ __device__ inline int conv_chunks(int C) {
     return 9 * (C >> 5) + ((C & 31) ? slab_chunks(C & 31) : 0);
 }
 
+// i / d for the slab's 8-channel group count d in 1..4 and 0 <= i < 32768,
+// with m = ceil(65536 / d): one 24-bit multiply and a shift instead of an
+// integer division (the error i (m d - 65536) / 65536 d stays below 1/3).
+__device__ __forceinline__ int div_small(int i, int m) { return (int)(((unsigned)i * (unsigned)m) >> 16); }
+
 // MFMA 16x16x32 lane layout: A row (lane & 15), k (lane >> 4) * 8 + j;
 // B column (lane & 15), same k; D rows 4 * (lane >> 4) + i, column (lane & 15).
+// One slab's products for the wave's 4 row blocks and the block's NBL output
+// blocks, accumulated chunk by chunk in K order; lane group jg of chunk kc
+// holds k group 4 kc + jg = (tap, 8-channel group) of the slab's g8 groups.
+// B fragments come from LDS (sw, the slab's chunks) or, for WLDS false,
+// from memory.  FULL: the slab has 32 channels (the unrolled loop).
+template <int NBL, bool WLDS, bool FULL>
+__device__ __forceinline__ void conv_slab(f32x4 (&acc)[4][NBL], const uint16_t* sx, const uint16_t* sw,
+                                          const uint16_t* wp, int g8, int mg, int nck, int nbtot, int nb0,
+                                          int lane, int wave) {
+    const int jg = lane >> 4, col = lane & 15;
+    const uint16_t* xb = sx + ((wave * 2) * CT_HW + col) * CT_PS;
+    auto step = [&](int kc, bool gv, int dy, int dx, int cg) {
+        bf16x8 av[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (gv)
+                v = *reinterpret_cast<const uint4*>(xb + (((m >> 1) + dy) * CT_HW + (m & 1) * 16 + dx) * CT_PS +
+                                                    cg * 8);
+            av[m] = __builtin_bit_cast(bf16x8, v);
+        }
+#pragma unroll
+        for (int nb = 0; nb < NBL; ++nb) {
+            uint4 w;
+            if constexpr (WLDS) w = *reinterpret_cast<const uint4*>(&sw[((kc * NBL + nb) * 64 + lane) * 8]);
+            else w = *reinterpret_cast<const uint4*>(wp + ((long)(kc * nbtot + nb0 + nb) * 64 + lane) * 8);
+            const bf16x8 bv = __builtin_bit_cast(bf16x8, w);
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                acc[m][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv, acc[m][nb], 0, 0, 0);
+        }
+    };
+    if (FULL) {
+        // a full slab (g8 = 4): chunk kc is tap kc, lane group jg is channel
+        // group jg, and every LDS offset is a constant
+#pragma unroll
+        for (int kc = 0; kc < 9; ++kc) step(kc, true, kc / 3, kc % 3, jg);
+    } else {
+        for (int kc = 0; kc < nck; ++kc) {
+            const int gi = kc * 4 + jg;  // this lane's 8-channel k group
+            const bool gv = gi < 9 * g8;
+            const int tap = gv ? div_small(gi, mg) : 0;
+            const int cg = gv ? gi - tap * g8 : 0;
+            const int dy = div_small(tap, 21846);  // tap / 3
+            step(kc, gv, dy, tap - dy * 3, cg);
+        }
+    }
+}
+
 // A block computes NBL of the layer's nbtot 16-channel output blocks
 // (starting at blockIdx.y * NBL): small levels split their output channels
 // over several blocks so that a launch still has enough of them.  The
 // results go through LDS, 16 channels at a time, so that every thread
 // stores whole 16-byte runs of one pixel's channels.
 template <int NBL, bool RELU_BF16>
-__global__ __launch_bounds__(256) void conv3x3_kernel(ConvK a) {
-    __shared__ __align__(16) uint16_t sx[CT_HH * CT_HW * CT_PS];
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NBL == 1 ? 4 : 1)))
+void conv3x3_kernel(ConvK a) {
+    constexpr int NPIX = CT_HH * CT_HW;               // staged pixels (tile + halo)
+    constexpr int SX_N = (NPIX * 4 + 255) / 256;      // 16-byte input pieces per thread (4 groups)
+    // Blocks of one 16-channel output block (the small levels, where the K
+    // loop is long against the tile) also stage the slab's packed weights in
+    // LDS (9.2 KB); wider blocks read their B fragments from memory.
+    constexpr bool WLDS = NBL == 1;
+    constexpr int SW_N = WLDS ? (9 * NBL * 64 + 255) / 256 : 1;  // 16-byte weight pieces per thread
+    __shared__ __align__(16) uint16_t sx[NPIX * CT_PS];
+    __shared__ __align__(16) uint16_t sw[WLDS ? 9 * NBL * 64 * 8 : 8];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int C = a.ca + a.cb;
     const int nbtot = a.cout >> 4, nb0 = blockIdx.y * NBL;
@@ -107,56 +170,83 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(ConvK a) {
     const int ty0 = (blk % tiles_y) * CT_TH;
     const int b = blk / tiles_y;
     const int Hh = a.H >> 1, Wh = a.W >> 1;
-    const int jg = lane >> 4, col = lane & 15;
+    const int col = lane & 15, jg = lane >> 4;
     f32x4 acc[4][NBL];
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
         for (int nb = 0; nb < NBL; ++nb) acc[m][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const uint16_t* wp = a.wp;
+
+    // One slab's global reads go to registers first, all of them in flight
+    // together, and the next slab's are issued before this slab's matrix
+    // products, so that their latency overlaps the work of the current one.
+    uint4 rx[SX_N], rw[SW_N];
+    auto load_slab = [&](int c0, const uint16_t* wsl) {
+        const int g8 = (C - c0 < 32 ? C - c0 : 32) >> 3;  // 8-channel groups in the slab
+        const int mg = (65536 + g8 - 1) / g8;
+#pragma unroll
+        for (int k = 0; k < SX_N; ++k) {
+            const int i = t + k * 256;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (i < NPIX * g8) {
+                const int pix = div_small(i, mg), g = i - pix * g8;
+                const int hy = pix / CT_HW, hx = pix - hy * CT_HW;
+                const int yy = ty0 + hy - 1, xx = tx0 + hx - 1;
+                if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
+                    const int c = c0 + g * 8;
+                    if (c < a.ca)
+                        v = *reinterpret_cast<const uint4*>(a.xa + (((long)b * a.H + yy) * a.W + xx) * a.ca + c);
+                    else
+                        v = *reinterpret_cast<const uint4*>(a.xb + (((long)b * Hh + (yy >> 1)) * Wh + (xx >> 1)) *
+                                                                       a.cb + (c - a.ca));
+                }
+            }
+            rx[k] = v;
+        }
+        if constexpr (WLDS) {
+            // the slab's chunks kc < nck of output blocks nb0 .. nb0 + NBL - 1
+            const int nck = slab_chunks(g8 * 8);
+#pragma unroll
+            for (int k = 0; k < SW_N; ++k) {
+                const int j = t + k * 256;
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (j < nck * NBL * 64) {
+                    const int kc = j / (NBL * 64), r = j - kc * (NBL * 64);
+                    v = *reinterpret_cast<const uint4*>(wsl + ((long)(kc * nbtot + nb0 + (r >> 6)) * 64 + (r & 63)) * 8);
+                }
+                rw[k] = v;
+            }
+        }
+    };
+    const uint16_t* wp = a.wp;   // packed weights of the current slab
+    load_slab(0, wp);
     for (int c0 = 0; c0 < C; c0 += 32) {
         const int g8 = (C - c0 < 32 ? C - c0 : 32) >> 3;  // 8-channel groups in this slab
+        const int mg = (65536 + g8 - 1) / g8;
+        const int nck = slab_chunks(g8 * 8);
         __syncthreads();                                    // the previous slab's reads are done
-        for (int i = t; i < CT_HH * CT_HW * g8; i += 256) {
-            const int g = i % g8, pix = i / g8;
-            const int hx = pix % CT_HW, hy = pix / CT_HW;
-            const int yy = ty0 + hy - 1, xx = tx0 + hx - 1;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
-                const int c = c0 + g * 8;
-                if (c < a.ca)
-                    v = *reinterpret_cast<const uint4*>(a.xa + (((long)b * a.H + yy) * a.W + xx) * a.ca + c);
-                else
-                    v = *reinterpret_cast<const uint4*>(a.xb + (((long)b * Hh + (yy >> 1)) * Wh + (xx >> 1)) * a.cb +
-                                                        (c - a.ca));
+#pragma unroll
+        for (int k = 0; k < SX_N; ++k) {
+            const int i = t + k * 256;
+            if (i < NPIX * g8) {
+                const int pix = div_small(i, mg), g = i - pix * g8;
+                *reinterpret_cast<uint4*>(&sx[pix * CT_PS + g * 8]) = rx[k];
             }
-            *reinterpret_cast<uint4*>(&sx[pix * CT_PS + g * 8]) = v;
+        }
+        if constexpr (WLDS) {
+#pragma unroll
+            for (int k = 0; k < SW_N; ++k) {
+                const int j = t + k * 256;
+                if (j < nck * NBL * 64) *reinterpret_cast<uint4*>(&sw[j * 8]) = rw[k];
+            }
         }
         __syncthreads();
-        const int nck = slab_chunks(g8 * 8);
-        for (int kc = 0; kc < nck; ++kc) {
-            const int gi = kc * 4 + jg;  // this lane's 8-channel k group
-            const bool gv = gi < 9 * g8;
-            const int tap = gv ? gi / g8 : 0;
-            const int cg = gv ? gi - tap * g8 : 0;
-            const int dy = tap / 3, dx = tap - (tap / 3) * 3;
-            bf16x8 av[4];
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                const int r = wave * 2 + (m >> 1), cc = (m & 1) * 16 + col;
-                uint4 v = make_uint4(0u, 0u, 0u, 0u);
-                if (gv) v = *reinterpret_cast<const uint4*>(&sx[((r + dy) * CT_HW + cc + dx) * CT_PS + cg * 8]);
-                av[m] = __builtin_bit_cast(bf16x8, v);
-            }
-#pragma unroll
-            for (int nb = 0; nb < NBL; ++nb) {
-                const uint4 w = *reinterpret_cast<const uint4*>(wp + ((long)(kc * nbtot + nb0 + nb) * 64 + lane) * 8);
-                const bf16x8 bv = __builtin_bit_cast(bf16x8, w);
-#pragma unroll
-                for (int m = 0; m < 4; ++m)
-                    acc[m][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv, acc[m][nb], 0, 0, 0);
-            }
-        }
+        if (c0 + 32 < C) load_slab(c0 + 32, wp + (long)nck * nbtot * 512);
+        // Narrow blocks (inference) take the unrolled loop for full slabs;
+        // a second MFMA loop gets accumulators of its own, which wide blocks
+        // (training) cannot afford, so they keep the general loop only.
+        if (NBL <= 2 && g8 == 4) conv_slab<NBL, WLDS, NBL <= 2>(acc, sx, sw, wp, g8, mg, nck, nbtot, nb0, lane, wave);
+        else conv_slab<NBL, WLDS, false>(acc, sx, sw, wp, g8, mg, nck, nbtot, nb0, lane, wave);
         wp += (long)nck * nbtot * 512;
     }
     // Epilogue, 16 channels at a time: bias (+ ReLU, bf16) into LDS as
