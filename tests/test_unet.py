@@ -62,6 +62,53 @@ def test_conv3x3_forward(gpu, ca, cb, cout, B, H, W):
     assert np.all(np.abs(gb - r) <= 2.0 ** -8 * r + 1e-5 * mag + 1e-6)
 
 
+# One 8 x 32 image is one tile, so the batch size picks the block width of the
+# launch: 16 output channels per block at small batches (unrolled full-slab
+# loop, weights staged in LDS), 32 in the middle, all of them at 1024 tiles
+# (the general K loop).  The same image must come out bit-identical.
+@pytest.mark.parametrize("ca,cb,cout,batches", [
+    (64, 96, 96, (1, 400, 1024)),   # five full slabs; one, two and six output blocks per block
+    (48, 64, 96, (1, 400, 1024)),   # three full slabs and a 16-channel one
+    (8, 0, 32, (1, 1024)),          # one 8-channel slab; one and two output blocks per block
+])
+def test_conv3x3_block_widths_agree(gpu, ca, cb, cout, batches):
+    from psw_amd._lib import call
+    from psw_amd.device import DeviceArray
+    H, W = 8, 32
+    rng = np.random.default_rng(ca + cb + cout)
+    xa0 = bf16_bits(rng.standard_normal((1, H, W, ca)))
+    xb0 = bf16_bits(rng.standard_normal((1, H // 2, W // 2, cb))) if cb else None
+    w = rng.standard_normal((9, ca + cb, cout)).astype(np.float32) * 0.2
+    b = rng.standard_normal(cout).astype(np.float32)
+    d_w, d_b = _dev(w), _dev(b)
+    outs = []
+    for B in batches:
+        xa = np.concatenate([xa0, bf16_bits(rng.standard_normal((B - 1, H, W, ca)))]) if B > 1 else xa0
+        xb = None
+        if cb:
+            xb = np.concatenate([xb0, bf16_bits(rng.standard_normal((B - 1, H // 2, W // 2, cb)))]) if B > 1 else xb0
+        d_xa = _dev(xa)
+        d_xb = _dev(xb) if cb else None
+        yf = DeviceArray((B, H, W, cout))
+        call("psw_conv3x3_bf16", d_xa.ptr, ca, d_xb.ptr if cb else None, cb, d_w.ptr, d_b.ptr, cout, B, W, H, 0,
+             yf.ptr, None)
+        yb = DeviceArray((B, H, W, cout), np.uint16)
+        call("psw_conv3x3_bf16", d_xa.ptr, ca, d_xb.ptr if cb else None, cb, d_w.ptr, d_b.ptr, cout, B, W, H, 1,
+             yb.ptr, None)
+        outs.append((yf.to_host()[0], yb.to_host()[0]))
+    x = bf16_val(xa0).astype(np.float64)
+    if cb:
+        x = np.concatenate([x, up2(bf16_val(xb0).astype(np.float64))], axis=3)
+    wr = bf16_val(bf16_bits(w)).astype(np.float64)
+    ref = conv_ref(x, wr, b.astype(np.float64))[0]
+    mag = conv_ref(np.abs(x), np.abs(wr))[0] + np.abs(b)
+    assert np.all(np.abs(outs[0][0] - ref) <= 1e-5 * mag + 1e-6)
+    for B, (f, h) in zip(batches[1:], outs[1:]):
+        print(f"batch {B}: fp32 max |diff| to batch 1 {np.abs(f - outs[0][0]).max():.3e}")
+        assert np.array_equal(f.view(np.uint32), outs[0][0].view(np.uint32))
+        assert np.array_equal(h, outs[0][1])
+
+
 @pytest.mark.parametrize("cout,cin,B,H,W", [(32, 32, 1, 12, 20), (96, 160, 1, 4, 8), (48, 112, 2, 6, 8),
                                             (32, 80, 1, 8, 8), (64, 64, 2, 5, 9)])
 def test_conv3x3_dgrad(gpu, cout, cin, B, H, W):
