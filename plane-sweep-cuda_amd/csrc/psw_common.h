@@ -135,12 +135,14 @@ __device__ inline float bilinear_sample(const float* __restrict__ d, int M1, int
 
 // The same sample with each interpolation as one fused multiply-add,
 // v0 + a (v1 - v0): two instructions per interpolation instead of four, and
-// one rounding of the product-sum instead of three (the fast sweep).
+// one rounding of the product-sum instead of three (the fast sweep).  Past
+// the bounds test x, y >= 0, so the hardware fraction (x - floor(x), exact
+// here) and truncation give the same a, b and cell as floorf above, in two
+// instructions per coordinate instead of three.
 __device__ inline float bilinear_sample_fma(const float* __restrict__ d, int M1, int M2, float x, float y) {
     if (!(x >= 0.f && x < (float)(M1 - 1) && y >= 0.f && y < (float)(M2 - 1))) return 0.f;
-    const float fx = floorf(x), fy = floorf(y);
-    const float a = x - fx, b = y - fy;
-    const uint32_t o = (uint32_t)(int)fy * (uint32_t)M1 + (uint32_t)(int)fx;
+    const float a = __builtin_amdgcn_fractf(x), b = __builtin_amdgcn_fractf(y);
+    const uint32_t o = (uint32_t)(int)y * (uint32_t)M1 + (uint32_t)(int)x;
     const float* r0 = d + o;
     const float* r1 = r0 + M1;
     const float t1 = __builtin_fmaf(a, r0[1] - r0[0], r0[0]);
