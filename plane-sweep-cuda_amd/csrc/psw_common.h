@@ -139,12 +139,15 @@ __device__ inline float bilinear_sample(const float* __restrict__ d, int M1, int
 // the bounds test x, y >= 0, so the hardware fraction (x - floor(x), exact
 // here) and truncation give the same a, b and cell as floorf above, in two
 // instructions per coordinate instead of three.
+// The cell's byte offset is formed in 32 bits (a 24-bit multiply: rows and
+// widths stay below 2^24, and the entry points keep images below 2^30
+// pixels), so both rows load from a uniform base plus that offset.
 __device__ inline float bilinear_sample_fma(const float* __restrict__ d, int M1, int M2, float x, float y) {
     if (!(x >= 0.f && x < (float)(M1 - 1) && y >= 0.f && y < (float)(M2 - 1))) return 0.f;
     const float a = __builtin_amdgcn_fractf(x), b = __builtin_amdgcn_fractf(y);
-    const uint32_t o = (uint32_t)(int)y * (uint32_t)M1 + (uint32_t)(int)x;
-    const float* r0 = d + o;
-    const float* r1 = r0 + M1;
+    const uint32_t ob = __umul24((uint32_t)(int)y, 4u * (uint32_t)M1) + 4u * (uint32_t)(int)x;
+    const float* r0 = reinterpret_cast<const float*>(reinterpret_cast<const char*>(d) + ob);
+    const float* r1 = reinterpret_cast<const float*>(reinterpret_cast<const char*>(d + M1) + ob);
     const float t1 = __builtin_fmaf(a, r0[1] - r0[0], r0[0]);
     const float t2 = __builtin_fmaf(a, r1[1] - r1[0], r1[0]);
     return __builtin_fmaf(b, t2 - t1, t1);
