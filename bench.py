@@ -282,8 +282,10 @@ def run_small(pl, stream, ev, fast, reps=20):
 def run_tvl1(pl, stream, ev):
     """TVL1 denoising (PlaneSweep::CudaDenoise) of the 640x480 sweep depthmap
     with the reference's default parameters.  Per-iteration time from a
-    1000- and a 0-iteration call, for the fused launches and for the
-    reference's two launches per iteration (PSW_TVL1_FUSED=0)."""
+    1000- and a 0-iteration call, for the resident solver (the default: one
+    cooperative launch for all iterations; PSW_TVL1_RESIDENT=2 makes a run
+    that falls back an error), the fused launches (PSW_TVL1_RESIDENT=0) and
+    the reference's two launches per iteration (PSW_TVL1_FUSED=0)."""
     from psw_amd import stages
     from psw_amd.device import DeviceArray
     d_depth = pl.sweep_and_linearise()
@@ -299,24 +301,32 @@ def run_tvl1(pl, stream, ev):
         return ev[0].elapsed_ms(ev[1])
 
     res = {}
-    prev = os.environ.get("PSW_TVL1_FUSED")
+    keys = ("PSW_TVL1_FUSED", "PSW_TVL1_RESIDENT")
+    prev = {k: os.environ.get(k) for k in keys}
     try:
-        for name, flag in (("launch_sequence", "0"), ("fused", "1")):
-            os.environ["PSW_TVL1_FUSED"] = flag
+        for name, fused, resident in (("launch_sequence", "0", "0"), ("fused", "1", "0"), ("resident", "1", "2")):
+            os.environ["PSW_TVL1_FUSED"], os.environ["PSW_TVL1_RESIDENT"] = fused, resident
             timed(20)
             t_full, t_zero = timed(TVL1_ITERS), timed(0)
             res[name] = (t_full - t_zero) / TVL1_ITERS
     finally:
-        if prev is None:
-            os.environ.pop("PSW_TVL1_FUSED", None)
-        else:
-            os.environ["PSW_TVL1_FUSED"] = prev
+        for k, val in prev.items():
+            if val is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = val
     gbs = TVL1_BYTES_PER_PIXEL_ITER * W * H / (res["fused"] * 1e-3) / 1e9
     m = int(os.environ.get("PSW_TVL1_M", "4"))                # iterations per launch (default 4, 56x8 tiles)
-    return {"kernel": f"tvl1_iter_kernel<{m}>", "iterations_per_launch": m, "iter_us": res["fused"] * 1e3,
-            "launch_sequence_iter_us": res["launch_sequence"] * 1e3,
-            "speedup_vs_launch_sequence": res["launch_sequence"] / res["fused"], "gbs_algorithmic": gbs,
-            "frac_hbm_peak": gbs / HBM_PEAK_GBS, "bytes_per_pixel_iter": TVL1_BYTES_PER_PIXEL_ITER,
+    return {"kernel": "tvl1_resident_kernel", "iter_us": res["resident"] * 1e3,
+            "speedup_vs_fused": res["fused"] / res["resident"],
+            "speedup_vs_launch_sequence": res["launch_sequence"] / res["resident"],
+            "note": "the default path: all iterations in one cooperative launch of 64x20 tiles that trade only their "
+                    "edges' depth each iteration (bit-identical to the launch sequence); it reads the state once "
+                    "per call, so its bound is the per-iteration edge exchange and VALU issue, not HBM",
+            "fused_kernel": f"tvl1_iter_kernel<{m}>", "fused_iterations_per_launch": m,
+            "fused_iter_us": res["fused"] * 1e3, "launch_sequence_iter_us": res["launch_sequence"] * 1e3,
+            "fused_gbs_per_step_bytes": gbs, "fused_frac_hbm_peak_per_step_bytes": gbs / HBM_PEAK_GBS,
+            "bytes_per_pixel_iter": TVL1_BYTES_PER_PIXEL_ITER,
             "config": f"{W}x{H} sweep depthmap, {TVL1_ITERS} iterations, lambda 0.3 tau 0.02 sigma 6 theta 1 "
                       "beta 0 gamma 1 (the reference's defaults)"}
 
@@ -860,7 +870,7 @@ def main():
     rec["small_pipeline"] = run_small(pl, Stream(), [Event() for _ in range(3)], fast)
     log(f"small pipeline: {rec['small_pipeline']['ms_per_depthmap']:.3f} ms per depthmap (sweep "
         f"{rec['small_pipeline']['sweep_ms']:.3f} ms, TGV2 {rec['small_pipeline']['tgv_ms']:.3f} ms)")
-    log(f"TVL1: {rec['tvl1']['iter_us']:.2f} us per iteration fused, "
+    log(f"TVL1: {rec['tvl1']['iter_us']:.2f} us per iteration resident, {rec['tvl1']['fused_iter_us']:.2f} fused, "
         f"{rec['tvl1']['launch_sequence_iter_us']:.2f} us as two launches per iteration")
     if not args.no_fusion:
         t0 = time.perf_counter()

@@ -22,6 +22,8 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
+#include <vector>
 
 namespace psw {
 namespace {
@@ -371,6 +373,406 @@ bool tvl1_fused() {
     return !(e && std::atoi(e) == 0);
 }
 
+// ---- resident solver ---------------------------------------------------------
+// All iterations in one cooperative launch: workgroup t owns a 64 x 20 tile
+// (ten waves, thread (wave w, lane x) the pixels of rows 2w and 2w+1 in
+// column x), keeps its pixels' depth o, duals Px, Py, R, input and tensor in
+// registers for the whole run, and trades only the depth of its edges with
+// its eight neighbours each iteration.  The P step of the row above the tile
+// and of the column left of it (the ring: the update reads Px on the left
+// and Py above) is recomputed here from the depth halo, so one exchange per
+// iteration suffices.  The per-pixel arithmetic is tvl1_p and tvl1_u, so the
+// results are those of the fused launches and of the reference's sequence.
+//
+// Exchange words are 64-bit (iteration tag, float bits) in uncached device
+// memory, double-buffered by the parity of the iteration: a tile overwrites
+// its words of iteration j only after its neighbours have published j + 1,
+// which they do after reading j.  Every wait is bounded (spin polls); one
+// that runs out sets the error words, which end the other waits too.  A
+// launch whose workgroups are not all resident within start_spin polls is
+// called off before any wait on a neighbour and before anything is written.
+constexpr int TVR_TX = 64, TVR_TY = 20, TVR_NT = TVR_TX * (TVR_TY / 2);
+constexpr int TVR_TOP = 0, TVR_BOT = TVR_TX, TVR_LEFT = 2 * TVR_TX, TVR_RIGHT = 2 * TVR_TX + TVR_TY;
+constexpr int TVR_XW = 2 * TVR_TX + 2 * TVR_TY;       // words a tile publishes per iteration
+constexpr int TVR_HALO = 2 * TVR_TX + 2 * TVR_TY + 2;  // halo cells a tile reads per iteration
+constexpr int TVR_RING0 = 2 * TVR_TX;                  // ring threads: waves 2 and 3
+constexpr int TVR_RING = TVR_TX + TVR_TY;
+constexpr int TVR_MAXTILES = 256;
+constexpr int TVR_PAD = 20480;                         // 80 KB, see tvl1_resident_kernel
+static_assert(TVR_HALO <= TVR_NT && TVR_RING0 + TVR_RING <= TVR_NT, "one halo cell and one ring cell per thread");
+
+struct TvrArgs {
+    TvlState tv;
+    int rd;                     // slot of the start state; the end state goes to slot 1 - rd
+    int niters;
+    float sp0, sp1;             // sigma of the first P step and of the others
+    int tiles_x, tiles_y, ntiles;
+    unsigned long long* xch;    // [2][ntiles][TVR_XW]
+    unsigned* err;              // [0] error, [1] started, [2] go (1) / called off (2), [3..7] first failure:
+                                // tile, word offset from xch, awaited tag, seen tag, iteration
+    unsigned spin, start_spin;
+};
+
+__device__ __forceinline__ unsigned long long tvr_load(unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned tvr_load32(unsigned* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void tvr_put(unsigned long long* p, unsigned tag, float v) {
+    __hip_atomic_store(p, ((unsigned long long)tag << 32) | __float_as_uint(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Poll word p until it carries `tag`; its float bits go to v.  False when
+// the spin budget runs out (the first failure is recorded) or another wait
+// has failed.
+__device__ bool tvr_get(const TvrArgs& A, unsigned long long* p, unsigned tag, float& v, int tile, int j) {
+    for (unsigned it = 0; it < A.spin; ++it) {
+        const unsigned long long w = tvr_load(p);
+        if ((unsigned)(w >> 32) == tag) {
+            v = __uint_as_float((unsigned)w);
+            return true;
+        }
+        if ((it & 63) == 63 && tvr_load32(A.err) != 0u) return false;
+        __builtin_amdgcn_s_sleep(2);
+    }
+    unsigned expected = 0u;
+    if (__hip_atomic_compare_exchange_strong(A.err, &expected, 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT)) {
+        const unsigned vals[5] = {(unsigned)tile, (unsigned)(p - A.xch), tag, (unsigned)(tvr_load(p) >> 32),
+                                  (unsigned)j};
+        for (int k = 0; k < 5; ++k) __hip_atomic_store(A.err + 3 + k, vals[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return false;
+}
+
+// Halo cell c of tile (tcx, tcy): the row above (c < 64), the above-right
+// corner, the column left, the below-left corner, the column right, the row
+// below.  False where the cell is outside the image or has no neighbour
+// tile; else the source tile's word, the LDS cell (row ly, column lx of the
+// 22 x 66 depth array) and the image position.
+__device__ inline bool tvr_halo(const TvrArgs& A, int tcx, int tcy, int c, int& off, int& ly, int& lx, int& gx,
+                                int& gy) {
+    constexpr int TX = TVR_TX, TY = TVR_TY;
+    const int W = A.tv.w, H = A.tv.h;
+    const int x0 = tcx * TX, y0 = tcy * TY, tile = tcy * A.tiles_x + tcx;
+    const bool has_l = tcx > 0, has_r = tcx < A.tiles_x - 1, has_u = tcy > 0, has_d = tcy < A.tiles_y - 1;
+    int src, word;
+    bool ok;
+    if (c < TX) {
+        gx = x0 + c; gy = y0 - 1; src = tile - A.tiles_x; word = TVR_BOT + c; ly = 0; lx = c + 1;
+        ok = has_u && gx < W;
+    } else if (c == TX) {
+        gx = x0 + TX; gy = y0 - 1; src = tile - A.tiles_x + 1; word = TVR_BOT; ly = 0; lx = TX + 1;
+        ok = has_u && has_r;
+    } else if (c < TX + 1 + TY) {
+        const int r = c - (TX + 1);
+        gx = x0 - 1; gy = y0 + r; src = tile - 1; word = TVR_RIGHT + r; ly = r + 1; lx = 0;
+        ok = has_l && gy < H;
+    } else if (c == TX + 1 + TY) {
+        gx = x0 - 1; gy = y0 + TY; src = tile + A.tiles_x - 1; word = TVR_TOP + TX - 1; ly = TY + 1; lx = 0;
+        ok = has_l && has_d;
+    } else if (c < TX + 2 + 2 * TY) {
+        const int r = c - (TX + 2 + TY);
+        gx = x0 + TX; gy = y0 + r; src = tile + 1; word = TVR_LEFT + r; ly = r + 1; lx = TX + 1;
+        ok = has_r && gy < H;
+    } else {
+        const int cc = c - (TX + 2 + 2 * TY);
+        gx = x0 + cc; gy = y0 + TY; src = tile + A.tiles_x; word = TVR_TOP + cc; ly = TY + 1; lx = cc + 1;
+        ok = has_d && gx < W;
+    }
+    off = src * TVR_XW + word;
+    return ok;
+}
+
+__global__ __launch_bounds__(TVR_NT) void tvl1_resident_kernel(const TvrArgs A) {
+    constexpr int TX = TVR_TX, TY = TVR_TY;
+    __shared__ float sO[TY + 2][TX + 2];         // depth o: the tile and its halo
+    __shared__ float sP[2][TY + 1][TX + 1];      // Px, Py: the tile, the row above and the column left
+    __shared__ float sT[4][TY + 1][TX + 1];      // the ring's tensor
+    __shared__ int sFlag[1];
+    // LDS held only so that a CU takes one workgroup (the tiles run in
+    // lockstep, so two on one CU would halve the whole solve's speed)
+    __shared__ float sPad[TVR_PAD];
+    const TvlState& a = A.tv;
+    const int W = a.w, H = a.h;
+    const int tile = xcd_tile(blockIdx.x, A.ntiles);
+    const int tcx = tile % A.tiles_x, tcy = tile / A.tiles_x;
+    const int x0 = tcx * TX, y0 = tcy * TY;
+    const int tid = threadIdx.x, tx = tid % TX;
+    const int wv = __builtin_amdgcn_readfirstlane(tid / TX);
+    const int x = x0 + tx;
+    const int rd = A.rd, wr = 1 - rd;
+    if (tid == 0) {
+        sFlag[0] = 0;
+        // the last workgroup to start gives the go
+        const unsigned before = __hip_atomic_fetch_add(A.err + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (before + 1 == (unsigned)A.ntiles) {
+            unsigned expected = 0u;
+            __hip_atomic_compare_exchange_strong(A.err + 2, &expected, 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    // own pixels: the start state and the constants
+    float o[2], px[2], py[2], r[2], raw[2], t11[2], t12[2], t21[2], t22[2];
+    bool in[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int y = y0 + 2 * wv + h;
+        in[h] = x < W && y < H;
+        const size_t g = (size_t)min(y, H - 1) * W + min(x, W - 1);
+        o[h] = a.out[rd][g]; px[h] = a.px[rd][g]; py[h] = a.py[rd][g]; r[h] = a.r[rd][g];
+        raw[h] = a.raw[g]; t11[h] = a.t11[g]; t12[h] = a.t12[g]; t21[h] = a.t21[g]; t22[h] = a.t22[g];
+        sO[2 * wv + h + 1][tx + 1] = o[h];
+    }
+    // the ring: its duals and tensor of the start state
+    const int rid = tid - TVR_RING0;
+    bool ring = false;
+    int rgx = 0, rgy = 0, rlx = 0, rly = 0;
+    if (rid >= 0 && rid < TVR_RING) {
+        if (rid < TX) { rgx = x0 + rid; rgy = y0 - 1; rlx = rid + 1; rly = 0; ring = tcy > 0 && rgx < W; }
+        else { rgx = x0 - 1; rgy = y0 + rid - TX; rlx = 0; rly = rid - TX + 1; ring = tcx > 0 && rgy < H; }
+        if (ring) {
+            const size_t g = (size_t)rgy * W + rgx;
+            sP[0][rly][rlx] = a.px[rd][g]; sP[1][rly][rlx] = a.py[rd][g];
+            sT[0][rly][rlx] = a.t11[g]; sT[1][rly][rlx] = a.t12[g];
+            sT[2][rly][rlx] = a.t21[g]; sT[3][rly][rlx] = a.t22[g];
+        }
+    }
+    // the depth halo of the start state; later ones come from the neighbours
+    int hoff = -1, hly = 0, hlx = 0;
+    if (tid < TVR_HALO) {
+        int gx, gy;
+        if (tvr_halo(A, tcx, tcy, tid, hoff, hly, hlx, gx, gy)) sO[hly][hlx] = a.out[rd][(size_t)gy * W + gx];
+        else hoff = -1;
+    }
+    // every workgroup resident before any waits on a neighbour
+    if (tid == 0) {
+        unsigned go = tvr_load32(A.err + 2);
+        for (unsigned it = 0; go == 0u && it < A.start_spin; ++it) {
+            __builtin_amdgcn_s_sleep(2);
+            go = tvr_load32(A.err + 2);
+        }
+        if (go == 0u) {
+            unsigned expected = 0u;
+            __hip_atomic_compare_exchange_strong(A.err + 2, &expected, 2u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            go = tvr_load32(A.err + 2);
+        }
+        if (go != 1u) sFlag[0] = 1;
+    }
+    __syncthreads();
+    if (sFlag[0]) return;   // called off: nothing written
+
+    unsigned long long* const xb0 = A.xch + (size_t)tile * TVR_XW;   // this tile's words (parity 0)
+    bool failed = false;
+    for (int j = 0; j < A.niters; ++j) {
+        const float sg = j == 0 ? A.sp0 : A.sp1;
+        // P step of the own pixels and of the ring from the depth of state j
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int row = 2 * wv + h, y = y0 + row;
+            const float c = sO[row + 1][tx + 1];
+            const float e_ = sO[row + 1][tx + 2], s_ = sO[row + 2][tx + 1];   // read by every lane, then chosen
+            if (in[h]) {
+                tvl1_p(px[h], py[h], c, x + 1 < W ? e_ : c, y + 1 < H ? s_ : c, t11[h], t12[h], t21[h], t22[h], sg);
+                sP[0][row + 1][tx + 1] = px[h];
+                sP[1][row + 1][tx + 1] = py[h];
+            }
+        }
+        if (ring) {
+            const float c = sO[rly][rlx];
+            const float e_ = rgx + 1 < W ? sO[rly][rlx + 1] : c;
+            const float s_ = rgy + 1 < H ? sO[rly + 1][rlx] : c;
+            float rpx = sP[0][rly][rlx], rpy = sP[1][rly][rlx];
+            tvl1_p(rpx, rpy, c, e_, s_, sT[0][rly][rlx], sT[1][rly][rlx], sT[2][rly][rlx], sT[3][rly][rlx], sg);
+            sP[0][rly][rlx] = rpx;
+            sP[1][rly][rlx] = rpy;
+        }
+        __syncthreads();
+        // update: R and the over-relaxed depth of state j + 1; the tile's
+        // edges for the neighbours
+        const bool publish = j + 1 < A.niters;
+        unsigned long long* const xb = xb0 + (size_t)((j + 1) & 1) * A.ntiles * TVR_XW;
+        const unsigned tag = (unsigned)(j + 1);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int row = 2 * wv + h, y = y0 + row;
+            const float pxl_ = sP[0][row + 1][tx], pyu_ = sP[1][row][tx + 1];
+            if (in[h]) {
+                const float pxl = x > 0 ? pxl_ : px[h], pyu = y > 0 ? pyu_ : py[h];
+                o[h] = tvl1_u(o[h], r[h], raw[h], px[h], pxl, py[h], pyu, x == 0, a.tau, a.theta, a.lambda, a.sigma);
+                sO[row + 1][tx + 1] = o[h];
+                if (publish) {
+                    if (row == 0) tvr_put(xb + TVR_TOP + tx, tag, o[h]);
+                    if (row == TY - 1) tvr_put(xb + TVR_BOT + tx, tag, o[h]);
+                    if (tx == 0) tvr_put(xb + TVR_LEFT + row, tag, o[h]);
+                    if (tx == TX - 1) tvr_put(xb + TVR_RIGHT + row, tag, o[h]);
+                }
+            }
+        }
+        if (!publish) break;
+        // the neighbours' edges of state j + 1
+        if (hoff >= 0) {
+            float v;
+            if (tvr_get(A, A.xch + (size_t)((j + 1) & 1) * A.ntiles * TVR_XW + hoff, tag, v, tile, j)) sO[hly][hlx] = v;
+            else sFlag[0] = 1;
+        }
+        __syncthreads();
+        if (sFlag[0]) { failed = true; break; }
+    }
+    if (failed) return;
+    // the pad is read on a branch never taken (the host passes niters >= 2),
+    // a select with no arithmetic, so that it stays allocated
+    if (A.niters < 0) {
+        sPad[tid] = o[0];
+        __syncthreads();
+        o[0] = sPad[(tid + 1) % TVR_NT];
+    }
+    // the end state into slot wr
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (!in[h]) continue;
+        const size_t g = (size_t)(y0 + 2 * wv + h) * W + x;
+        a.out[wr][g] = o[h]; a.px[wr][g] = px[h]; a.py[wr][g] = py[h]; a.r[wr][g] = r[h];
+    }
+}
+
+// The exchange buffer of the resident solver on each device: uncached
+// memory for the largest launch (2 x 256 tiles x 168 edge words, 8 error
+// words), allocated once and kept for the process (uncached memory that a
+// kernel has read is never freed).  Also the CU count, and the tile counts
+// whose launches were called off (those keep using the fused launches).
+struct TvrDev {
+    std::mutex mu;
+    unsigned long long* buf[64] = {};
+    int cus[64] = {};
+    std::vector<int> declined[64];
+};
+TvrDev& tvr_dev() {
+    static TvrDev* d = new TvrDev();
+    return *d;
+}
+constexpr size_t TVR_WORDS = 2 * (size_t)TVR_MAXTILES * TVR_XW + 4;
+
+// The resident solver runs unless PSW_TVL1_RESIDENT=0, for frames of at
+// most one 64 x 20 tile per CU (and at most 256 tiles) and runs of at least
+// two iterations; *buf receives the device's exchange buffer.
+// PSW_TVL1_RESIDENT=2 makes a run of two or more iterations that cannot use
+// it an error (tests).
+bool tvl1_resident_use(int W, int H, int niters, unsigned long long** buf) {
+    const char* e = std::getenv("PSW_TVL1_RESIDENT");
+    if ((e && std::atoi(e) == 0) || niters < 2) return false;
+    const int nt = ((W + TVR_TX - 1) / TVR_TX) * ((H + TVR_TY - 1) / TVR_TY);
+    if (nt > TVR_MAXTILES) return false;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        (void)hipGetLastError();
+        return false;
+    }
+    TvrDev& D = tvr_dev();
+    std::lock_guard<std::mutex> lock(D.mu);
+    if (D.cus[dev] == 0) {
+        int per = 0, ncu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)tvl1_resident_kernel, TVR_NT, 0) !=
+                hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+            (void)hipGetLastError();
+            per = 0;
+        }
+        D.cus[dev] = per >= 1 && ncu > 0 ? ncu : -1;
+    }
+    if (D.cus[dev] < nt) return false;
+    for (int d : D.declined[dev])
+        if (d == nt) return false;
+    if (!D.buf[dev]) {
+        void* b = nullptr;
+        if (hipExtMallocWithFlags(&b, TVR_WORDS * sizeof(unsigned long long), hipDeviceMallocUncached) != hipSuccess) {
+            (void)hipGetLastError();
+            D.cus[dev] = -1;
+            return false;
+        }
+        D.buf[dev] = static_cast<unsigned long long*>(b);
+    }
+    *buf = D.buf[dev];
+    return true;
+}
+
+void tvl1_resident_decline(int nt) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
+    TvrDev& D = tvr_dev();
+    std::lock_guard<std::mutex> lock(D.mu);
+    D.declined[dev].push_back(nt);
+}
+
+bool tvl1_resident_required() {
+    const char* e = std::getenv("PSW_TVL1_RESIDENT");
+    return e && std::atoi(e) == 2;
+}
+
+unsigned tvr_env_u32(const char* name, unsigned dflt) {
+    const char* e = std::getenv(name);
+    return e ? (unsigned)std::strtoul(e, nullptr, 10) : dflt;
+}
+
+// niters iterations from slot rd into slot 1 - rd in one cooperative
+// launch.  *ran is false, with nothing written, when the runtime refuses
+// the launch or the launch is called off.  Synchronous (it reads the error
+// words).  Polls per wait: PSW_TVL1_RESIDENT_SPIN (2^22), of the start:
+// PSW_TVL1_RESIDENT_START_SPIN (2^16; set, a called-off launch is not
+// remembered).
+int tvl1_resident_run(const TvlState& tv, unsigned long long* buf, int rd, int niters, float sp0, float sp1,
+                      hipStream_t st, bool* ran) {
+    *ran = false;
+    TvrArgs A;
+    A.tv = tv;
+    A.rd = rd;
+    A.niters = niters;
+    A.sp0 = sp0;
+    A.sp1 = sp1;
+    A.tiles_x = (tv.w + TVR_TX - 1) / TVR_TX;
+    A.tiles_y = (tv.h + TVR_TY - 1) / TVR_TY;
+    A.ntiles = A.tiles_x * A.tiles_y;
+    A.xch = buf;
+    A.err = reinterpret_cast<unsigned*>(buf + 2 * (size_t)A.ntiles * TVR_XW);
+    A.spin = tvr_env_u32("PSW_TVL1_RESIDENT_SPIN", 1u << 22);
+    A.start_spin = tvr_env_u32("PSW_TVL1_RESIDENT_START_SPIN", 1u << 16);
+    // tags that match no iteration, and clear error words
+    int rc = psw_fill_u32(reinterpret_cast<uint32_t*>(buf), 0xFFFFFFFFu, 4 * (size_t)A.ntiles * TVR_XW, st);
+    if (rc == PSW_OK) rc = psw_fill_u32(A.err, 0u, 8, st);
+    if (rc != PSW_OK) return rc;
+    void* args[] = {&A};
+    if (hipLaunchCooperativeKernel((const void*)tvl1_resident_kernel, dim3((unsigned)A.ntiles), dim3(TVR_NT), args, 0,
+                                   st) != hipSuccess) {
+        (void)hipGetLastError();
+        return PSW_OK;
+    }
+    unsigned* he = nullptr;
+    PSW_TRY_HIP(hipHostMalloc((void**)&he, 8 * sizeof(unsigned), hipHostMallocDefault), PSW_ERR_ALLOC);
+    int out = PSW_OK;
+    if (hipMemcpyAsync(he, A.err, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, st) != hipSuccess) out = PSW_ERR_COPY;
+    else if (hipStreamSynchronize(st) != hipSuccess) out = PSW_ERR_DEVICE;
+    if (out == PSW_OK) {
+        if (he[2] != 1u) {   // called off before any wait on a neighbour: nothing was written
+            if (!std::getenv("PSW_TVL1_RESIDENT_START_SPIN")) tvl1_resident_decline(A.ntiles);
+        } else {
+            *ran = true;
+            if (he[0]) {
+                set_error("resident TVL1 solver: tile %u waited %u polls for word %u of tile %u (iteration tag %u, "
+                          "saw %u) at iteration %u; the result is undefined",
+                          he[3], A.spin, he[4] % TVR_XW, (he[4] % ((unsigned)A.ntiles * TVR_XW)) / TVR_XW, he[5],
+                          he[6], he[7]);
+                out = PSW_ERR_DEVICE;
+            }
+        }
+    } else {
+        (void)hipGetLastError();
+    }
+    (void)hipHostFree(he);
+    return out;
+}
 }  // namespace
 }  // namespace psw
 
@@ -435,8 +837,22 @@ extern "C" int psw_tvl1_denoise(const float* d_depth, const float* d_ref, int wi
         const bool merged = tvl1_merged();
         const int mmax = std::min(tvl1_max_per_launch(), ty == 16 ? 4 : 6);   // LDS of 64x16 tiles
         const bool narrow = tvl1_narrow();
-        int rd = 0;
-        for (int i = 0; i < niters;) {
+        int rd = 0, i = 0;
+        unsigned long long* xbuf = nullptr;
+        bool ran = false;
+        if (tvl1_resident_use(width, height, niters, &xbuf)) {
+            TRY(tvl1_resident_run(tv, xbuf, 0, niters, (float)(1 + sigma), (float)sigma, S(s), &ran));
+            if (ran) {
+                rd = 1;
+                i = niters;
+            }
+        }
+        if (!ran && niters >= 2 && tvl1_resident_required()) {
+            set_error("psw_tvl1_denoise: PSW_TVL1_RESIDENT=2 and the resident solver did not run (%dx%d, %d iterations)",
+                      width, height, niters);
+            return PSW_ERR_DEVICE;
+        }
+        for (; i < niters;) {
             const int m = std::min(mmax, niters - i);
             const float sp0 = (float)(i == 0 ? 1 + sigma : sigma);
             TRY(tvl1_launch(tv, ty, merged, narrow, m, rd, sp0, (float)sigma, S(s)));

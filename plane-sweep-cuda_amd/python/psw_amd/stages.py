@@ -407,7 +407,8 @@ def tvl1_denoise(depth, ref, znear, zfar, niters=100, lambda_=0.3, tau=0.02, sig
 
 def tvl1_denoise_device(d_depth, d_ref, W, Hh, znear, zfar, d_out, d_work, niters=100, lambda_=0.3, tau=0.02,
                         sigma=6.0, theta=1.0, beta=0.0, gamma=1.0, stream=None):
-    """psw_tvl1_denoise on device buffers, asynchronous on `stream`; d_work
+    """psw_tvl1_denoise on device buffers, queued on `stream` (a run of the
+    resident solver waits for its launch, whose error words it reads); d_work
     holds 12*W*H floats."""
     call("psw_tvl1_denoise", d_depth.ptr, d_ref.ptr, W, Hh, float(znear), float(zfar), int(niters), float(lambda_),
          float(tau), float(sigma), float(theta), float(beta), float(gamma), d_out.ptr, d_work.ptr, stream)

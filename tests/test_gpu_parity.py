@@ -621,6 +621,7 @@ def test_tvl1_fused_matches_launch_sequence(gpu, monkeypatch, views_small, niter
         monkeypatch.setenv("PSW_TVL1_FUSED", "0")
         seq = stages.tvl1_denoise(depth, ref, 0.5, 6.0, niters=niters)
         monkeypatch.setenv("PSW_TVL1_FUSED", "1")
+        monkeypatch.setenv("PSW_TVL1_RESIDENT", "0")
         monkeypatch.setenv("PSW_TVL1_TY", ty)
         monkeypatch.setenv("PSW_TVL1_MERGED", merged)
         monkeypatch.setenv("PSW_TVL1_M", mmax)
@@ -630,6 +631,65 @@ def test_tvl1_fused_matches_launch_sequence(gpu, monkeypatch, views_small, niter
         if depth.size < 5000:
             e32 = oracle.tvl1(depth, ref, 0.5, 6.0, niters=niters)
             assert np.abs(fused - e32).max() <= 1e-5
+
+
+@pytest.mark.parametrize("niters", [2, 3, 50])
+def test_tvl1_resident_matches_launch_sequence(gpu, monkeypatch, views_small, niters):
+    """The resident TVL1 solver (all iterations in one cooperative launch of
+    64x20 tiles, tvl1_fused.hip; PSW_TVL1_RESIDENT=2 makes it an error if it
+    does not run) against the reference's two launches per iteration:
+    bit-identical on the 160x120 view, on sizes that cut the tiles (65x9,
+    3x2, 130x17, 200x41) and at 640x480 (240 tiles)."""
+    from psw_amd import stages
+    v = views_small
+    rng = np.random.default_rng(200 + niters)
+    cases = [(v["depth"], v["ref"])]
+    for h, w in [(9, 65), (2, 3), (17, 130), (41, 200), (480, 640)]:
+        cases.append((rng.uniform(0.5, 6.0, (h, w)), rng.uniform(0, 255, (h, w))))
+    for depth, ref in cases:
+        depth = np.clip(depth + 0.05 * rng.standard_normal(depth.shape), 0.5, 6.0).astype(np.float32)
+        ref = np.asarray(ref, np.float32)
+        print(f"[case] tvl1 resident {depth.shape[1]}x{depth.shape[0]} niters={niters}", flush=True)
+        monkeypatch.setenv("PSW_TVL1_FUSED", "0")
+        seq = stages.tvl1_denoise(depth, ref, 0.5, 6.0, niters=niters)
+        monkeypatch.setenv("PSW_TVL1_FUSED", "1")
+        monkeypatch.setenv("PSW_TVL1_RESIDENT", "2")
+        res = stages.tvl1_denoise(depth, ref, 0.5, 6.0, niters=niters)
+        np.testing.assert_array_equal(res.view(np.uint32), seq.view(np.uint32))
+
+
+def test_tvl1_resident_fallback_and_failure(gpu, monkeypatch, views_small):
+    """Frames above 256 tiles (1242x375: 380) use the fused launches, with the
+    same bits; a wait that runs out (a zero poll budget, forced from the host)
+    is reported as an error; and the next run is whole again."""
+    from psw_amd import stages
+    from psw_amd._lib import PswError
+    rng = np.random.default_rng(7)
+    depth = rng.uniform(0.5, 6.0, (375, 1242)).astype(np.float32)
+    ref = rng.uniform(0, 255, (375, 1242)).astype(np.float32)
+    print("[case] tvl1 1242x375 resident on (falls back) vs launch sequence", flush=True)
+    monkeypatch.setenv("PSW_TVL1_FUSED", "0")
+    seq = stages.tvl1_denoise(depth, ref, 0.5, 6.0, niters=6)
+    monkeypatch.setenv("PSW_TVL1_FUSED", "1")
+    monkeypatch.setenv("PSW_TVL1_RESIDENT", "1")
+    got = stages.tvl1_denoise(depth, ref, 0.5, 6.0, niters=6)
+    np.testing.assert_array_equal(got.view(np.uint32), seq.view(np.uint32))
+    monkeypatch.setenv("PSW_TVL1_RESIDENT", "2")
+    with pytest.raises(PswError):
+        stages.tvl1_denoise(depth, ref, 0.5, 6.0, niters=6)
+    v = views_small
+    d = np.clip(v["depth"], 0.5, 6.0).astype(np.float32)
+    print("[case] tvl1 160x120 resident with a zero poll budget", flush=True)
+    monkeypatch.setenv("PSW_TVL1_RESIDENT_SPIN", "0")
+    with pytest.raises(PswError) as e:
+        stages.tvl1_denoise(d, v["ref"], 0.5, 6.0, niters=5)
+    assert "resident TVL1 solver" in str(e.value)
+    monkeypatch.delenv("PSW_TVL1_RESIDENT_SPIN")
+    monkeypatch.setenv("PSW_TVL1_FUSED", "0")
+    seq = stages.tvl1_denoise(d, v["ref"], 0.5, 6.0, niters=5)
+    monkeypatch.setenv("PSW_TVL1_FUSED", "1")
+    got = stages.tvl1_denoise(d, v["ref"], 0.5, 6.0, niters=5)
+    np.testing.assert_array_equal(got.view(np.uint32), seq.view(np.uint32))
 
 
 # ---- error paths (forced from the host) -----------------------------------------------------
