@@ -465,11 +465,16 @@ int psw_fusion_surface_count(const psw_fusion_grid* g, int64_t* count, psw_strea
 /* d_depth (device, W*H) is denoised into d_out with the reference's scaling
  * to [0,1] over [znear, zfar]; d_ref holds 0..255 intensities for the tensor.
  * d_work: 12*W*H floats of device workspace (the solver state is
- * double-buffered).  The fused kernel runs up to PSW_TVL1_M iterations per
- * launch (2 to 6, default 4; at most 4 with the 64x16 tiles of
- * PSW_TVL1_TY=16), a shorter launch for the remainder; PSW_TVL1_FUSED=0
+ * double-buffered).  Runs of two or more iterations on frames of at most
+ * 256 tiles of 64 x 20 (and one per CU) go to the resident solver: all
+ * iterations in one cooperative launch whose tiles trade only their edges'
+ * depth each iteration; such a run waits for its launch (it reads the
+ * launch's error words), and PSW_TVL1_RESIDENT=0 turns it off.  Otherwise,
+ * or when the launch is called off, the fused kernel runs up to PSW_TVL1_M
+ * iterations per launch (2 to 6, default 4; at most 4 with the 64x16 tiles
+ * of PSW_TVL1_TY=16), a shorter launch for the remainder; PSW_TVL1_FUSED=0
  * selects the reference's two launches per iteration instead.  The results
- * are identical either way. */
+ * are identical every way. */
 int psw_tvl1_denoise(const float* d_depth, const float* d_ref, int width, int height, float znear, float zfar,
                      int niters, double lambda, double tau, double sigma, double theta, double beta, double gamma,
                      float* d_out, float* d_work, psw_stream_t s);
