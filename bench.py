@@ -198,7 +198,8 @@ SPARSE_BYTES_PER_PIXEL_ITER = 80   # read u 12, p 8, q 12, T 12, sparse 4; write
 def run_sparse(pl, stream, ev, seed):
     """Sparse-depth TGV2 densification at 640x480: the sweep depthmap as the
     dense start, the scene's true depth at 5 % of the pixels as the sparse
-    input.  Per-iteration time from a 500- and a 0-iteration call."""
+    input.  Per-iteration time from a 500- and a 0-iteration call, for the
+    resident solver and for the fused launches."""
     from psw_amd import stages
     from psw_amd.device import DeviceArray
     s = stream.handle
@@ -217,15 +218,34 @@ def run_sparse(pl, stream, ev, seed):
         ev[1].synchronize()
         return ev[0].elapsed_ms(ev[1])
 
-    timed(20)
-    t_full, t_zero = timed(SPARSE_ITERS), timed(0)
-    it_ms = (t_full - t_zero) / SPARSE_ITERS
-    gbs = SPARSE_BYTES_PER_PIXEL_ITER * W * H / (it_ms * 1e-3) / 1e9
+    # the resident solver (the default; =2 makes a run that falls back an
+    # error) and the fused launches (=0)
+    res = {}
+    prev = os.environ.get("PSW_TGV_SPARSE_RESIDENT")
+    try:
+        for name, flag in (("fused", "0"), ("resident", "2")):
+            os.environ["PSW_TGV_SPARSE_RESIDENT"] = flag
+            timed(20)
+            t_full, t_zero = timed(SPARSE_ITERS), timed(0)
+            res[name] = ((t_full - t_zero) / SPARSE_ITERS, t_full, t_zero)
+    finally:
+        if prev is None:
+            os.environ.pop("PSW_TGV_SPARSE_RESIDENT", None)
+        else:
+            os.environ["PSW_TGV_SPARSE_RESIDENT"] = prev
+    it_ms, t_full, t_zero = res["resident"]
+    f_ms = res["fused"][0]
+    gbs = SPARSE_BYTES_PER_PIXEL_ITER * W * H / (f_ms * 1e-3) / 1e9
     multi = os.environ.get("PSW_TGV_SPARSE_DOUBLE", "1") != "0"   # several iterations per launch (default)
     m = int(os.environ.get("PSW_TGV_SPARSE_M", "4")) if multi else 1
     kernel = {1: "tgv_sparse_iter_kernel", 2: "tgv_sparse_iter2_kernel"}.get(m, f"tgv_sparse_iterm_kernel<{m}>")
-    return {"kernel": kernel, "iterations_per_launch": m, "iter_us": it_ms * 1e3, "call_ms": t_full,
-            "setup_output_ms": t_zero, "gbs_algorithmic": gbs, "frac_hbm_peak": gbs / HBM_PEAK_GBS,
+    return {"kernel": "tgv_sparse_resident_kernel", "iter_us": it_ms * 1e3, "call_ms": t_full,
+            "setup_output_ms": t_zero, "speedup_vs_fused": f_ms / it_ms,
+            "note": "the default path: all iterations in one cooperative launch of 64x20 tiles that trade only their "
+                    "edges' u, u1 each iteration (bit-identical to the fused launches); it reads the state once per "
+                    "call, so its bound is the edge exchange and VALU issue, not HBM",
+            "fused_kernel": kernel, "fused_iterations_per_launch": m, "fused_iter_us": f_ms * 1e3,
+            "fused_gbs_per_step_bytes": gbs, "fused_frac_hbm_peak_per_step_bytes": gbs / HBM_PEAK_GBS,
             "bytes_per_pixel_iter": SPARSE_BYTES_PER_PIXEL_ITER,
             "config": f"{W}x{H}, sweep depth as the dense start, true depth at 5% of pixels, {SPARSE_ITERS} "
                       "iterations, alpha0 2 alpha1 1.5 tau 0.02 sigma 1"}
@@ -865,7 +885,8 @@ def main():
     }
     rec["sparse_tgv"] = run_sparse(pl, Stream(), [Event() for _ in range(2)], seeds[0])
     log(f"TGV2 resident: {resident_us:.3f} us per iteration; launch sequence {launch_ms * 1e3 / m:.3f} us")
-    log(f"sparse TGV2: {rec['sparse_tgv']['iter_us']:.2f} us per iteration")
+    log(f"sparse TGV2: {rec['sparse_tgv']['iter_us']:.2f} us per iteration resident, "
+        f"{rec['sparse_tgv']['fused_iter_us']:.2f} fused")
     rec["tvl1"] = run_tvl1(pl, Stream(), [Event() for _ in range(2)])
     rec["small_pipeline"] = run_small(pl, Stream(), [Event() for _ in range(3)], fast)
     log(f"small pipeline: {rec['small_pipeline']['ms_per_depthmap']:.3f} ms per depthmap (sweep "

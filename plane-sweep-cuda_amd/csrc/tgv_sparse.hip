@@ -31,6 +31,8 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
+#include <vector>
 
 namespace psw {
 
@@ -459,6 +461,431 @@ static bool sparse_double() {
 // Workspace: T 3n, then two slots of (u 3n, p 2n, q 3n).
 static size_t sparse_floats(size_t n) { return 3 * n + 2 * 8 * n; }
 
+// ---- resident solver ---------------------------------------------------------
+// All iterations in one cooperative launch, on the scheme of the dense and
+// TVL1 resident solvers: workgroup t owns a 64 x 20 tile (ten waves, thread
+// (wave w, lane x) the pixels of rows 2w and 2w+1 in column x) and keeps its
+// pixels' u, u1x, u1y, p, q, tensor and sparse depth in registers for the
+// whole run; the tiles trade only their edges' u, u1x, u1y each iteration.
+// The dual step of the row above the tile and of the column left of it (the
+// ring, whose p and q the primal step reads) is recomputed here from the
+// halo, so one exchange per iteration suffices.  The per-pixel arithmetic is
+// pq_update and sparse_u_step, so the results are those of the fused
+// launches.  Exchange words are 64-bit (iteration tag, float bits) in
+// uncached device memory, double-buffered by the iteration's parity; every
+// wait is bounded, and a launch whose workgroups are not all resident in
+// time is called off before any wait and any write.
+constexpr int TSR_TX = 64, TSR_TY = 20, TSR_NT = TSR_TX * (TSR_TY / 2);
+constexpr int TSR_F = 3;                                          // u, u1x, u1y
+constexpr int TSR_TOP = 0, TSR_BOT = TSR_F * TSR_TX, TSR_LEFT = 2 * TSR_F * TSR_TX,
+              TSR_RIGHT = 2 * TSR_F * TSR_TX + TSR_F * TSR_TY;     // word f * (64 | 20) + c of each edge
+constexpr int TSR_XW = 2 * TSR_F * (TSR_TX + TSR_TY);            // words a tile publishes per iteration
+constexpr int TSR_HALO = TSR_F * (2 * TSR_TX + 2 * TSR_TY + 2);  // halo words a tile reads per iteration
+constexpr int TSR_RING0 = 2 * TSR_TX, TSR_RING = TSR_TX + TSR_TY;  // ring threads: waves 2 and 3
+constexpr int TSR_MAXTILES = 256;
+constexpr int TSR_PAD = 6144;                                     // 24 KB, see tgv_sparse_resident_kernel
+static_assert(TSR_HALO <= TSR_NT && TSR_RING0 + TSR_RING <= TSR_NT, "one halo word and one ring cell per thread");
+
+struct SparseResArgs {
+    SparsePtrs a;               // the start state in ucur / pcur / qcur, the end state to unext / pnext / qnext
+    int niters;
+    int tiles_x, tiles_y, ntiles;
+    unsigned long long* xch;    // [2][ntiles][TSR_XW]
+    unsigned* err;              // [0] error, [1] started, [2] go (1) / called off (2), [3..7] first failure:
+                                // tile, word offset from xch, awaited tag, seen tag, iteration
+    unsigned spin, start_spin;
+};
+
+__device__ __forceinline__ unsigned long long tsr_load(unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned tsr_load32(unsigned* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void tsr_put(unsigned long long* p, unsigned tag, float v) {
+    __hip_atomic_store(p, ((unsigned long long)tag << 32) | __float_as_uint(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Poll word p until it carries `tag`; its float bits go to v.  False when
+// the spin budget runs out (the first failure is recorded) or another wait
+// has failed.
+__device__ bool tsr_get(const SparseResArgs& A, unsigned long long* p, unsigned tag, float& v, int tile, int j) {
+    for (unsigned it = 0; it < A.spin; ++it) {
+        const unsigned long long w = tsr_load(p);
+        if ((unsigned)(w >> 32) == tag) {
+            v = __uint_as_float((unsigned)w);
+            return true;
+        }
+        if ((it & 63) == 63 && tsr_load32(A.err) != 0u) return false;
+        __builtin_amdgcn_s_sleep(2);
+    }
+    unsigned expected = 0u;
+    if (__hip_atomic_compare_exchange_strong(A.err, &expected, 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT)) {
+        const unsigned vals[5] = {(unsigned)tile, (unsigned)(p - A.xch), tag, (unsigned)(tsr_load(p) >> 32),
+                                  (unsigned)j};
+        for (int k = 0; k < 5; ++k) __hip_atomic_store(A.err + 3 + k, vals[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return false;
+}
+
+// Halo word w of tile (tcx, tcy): field f = w % 3 of cell c = w / 3, the
+// cells being the row above (c < 64), the above-right corner, the column
+// left, the below-left corner, the column right and the row below.  False
+// where the cell is outside the image or has no neighbour tile; else the
+// source tile's word, the LDS cell (row ly, column lx of the 22 x 66 arrays)
+// and the image position.
+__device__ inline bool tsr_halo(const SparseResArgs& A, int tcx, int tcy, int w, int& off, int& ly, int& lx, int& gx,
+                                int& gy) {
+    constexpr int TX = TSR_TX, TY = TSR_TY;
+    const int W = A.a.W, H = A.a.H;
+    const int x0 = tcx * TX, y0 = tcy * TY, tile = tcy * A.tiles_x + tcx;
+    const bool has_l = tcx > 0, has_r = tcx < A.tiles_x - 1, has_u = tcy > 0, has_d = tcy < A.tiles_y - 1;
+    const int c = w / TSR_F, f = w - (w / TSR_F) * TSR_F;
+    int src, word;
+    bool ok;
+    if (c < TX) {
+        gx = x0 + c; gy = y0 - 1; src = tile - A.tiles_x; word = TSR_BOT + f * TX + c; ly = 0; lx = c + 1;
+        ok = has_u && gx < W;
+    } else if (c == TX) {
+        gx = x0 + TX; gy = y0 - 1; src = tile - A.tiles_x + 1; word = TSR_BOT + f * TX; ly = 0; lx = TX + 1;
+        ok = has_u && has_r;
+    } else if (c < TX + 1 + TY) {
+        const int r = c - (TX + 1);
+        gx = x0 - 1; gy = y0 + r; src = tile - 1; word = TSR_RIGHT + f * TY + r; ly = r + 1; lx = 0;
+        ok = has_l && gy < H;
+    } else if (c == TX + 1 + TY) {
+        gx = x0 - 1; gy = y0 + TY; src = tile + A.tiles_x - 1; word = TSR_TOP + f * TX + TX - 1; ly = TY + 1; lx = 0;
+        ok = has_l && has_d;
+    } else if (c < TX + 2 + 2 * TY) {
+        const int r = c - (TX + 2 + TY);
+        gx = x0 + TX; gy = y0 + r; src = tile + 1; word = TSR_LEFT + f * TY + r; ly = r + 1; lx = TX + 1;
+        ok = has_r && gy < H;
+    } else {
+        const int cc = c - (TX + 2 + 2 * TY);
+        gx = x0 + cc; gy = y0 + TY; src = tile + A.tiles_x; word = TSR_TOP + f * TX + cc; ly = TY + 1; lx = cc + 1;
+        ok = has_d && gx < W;
+    }
+    off = src * TSR_XW + word;
+    return ok;
+}
+
+__global__ __launch_bounds__(TSR_NT) void tgv_sparse_resident_kernel(const SparseResArgs A) {
+    constexpr int TX = TSR_TX, TY = TSR_TY;
+    __shared__ float sU[TSR_F][TY + 2][TX + 2];   // u, u1x, u1y: the tile and its halo
+    __shared__ float sPQ[5][TY + 1][TX + 1];      // px, py, qx, qy, qz: the tile, the row above, the column left
+    __shared__ float sT[3][TY + 1][TX + 1];       // the ring's tensor
+    __shared__ int sFlag[1];
+    // LDS held only so that a CU takes one workgroup (the tiles run in
+    // lockstep, so two on one CU would halve the whole solve's speed)
+    __shared__ float sPad[TSR_PAD];
+    const SparsePtrs& a = A.a;
+    const int W = a.W, H = a.H;
+    const size_t n = (size_t)W * H;
+    const int tile = xcd_tile(blockIdx.x, A.ntiles);
+    const int tcx = tile % A.tiles_x, tcy = tile / A.tiles_x;
+    const int x0 = tcx * TX, y0 = tcy * TY;
+    const int tid = threadIdx.x, tx = tid % TX;
+    const int wv = __builtin_amdgcn_readfirstlane(tid / TX);
+    const int x = x0 + tx;
+    const float as1 = a.alpha1 * a.sigma, as0 = a.alpha0 * a.sigma;
+    if (tid == 0) {
+        sFlag[0] = 0;
+        // the last workgroup to start gives the go
+        const unsigned before = __hip_atomic_fetch_add(A.err + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (before + 1 == (unsigned)A.ntiles) {
+            unsigned expected = 0u;
+            __hip_atomic_compare_exchange_strong(A.err + 2, &expected, 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    // own pixels: the start state and the constants
+    float uf[2][TSR_F], px[2], py[2], qx[2], qy[2], qz[2], t11[2], t12[2], t22[2], sp[2];
+    bool in[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int y = y0 + 2 * wv + h;
+        in[h] = x < W && y < H;
+        const size_t g = (size_t)min(y, H - 1) * W + min(x, W - 1);
+#pragma unroll
+        for (int f = 0; f < TSR_F; ++f) {
+            uf[h][f] = a.ucur[f * n + g];
+            sU[f][2 * wv + h + 1][tx + 1] = uf[h][f];
+        }
+        px[h] = a.pcur[g]; py[h] = a.pcur[n + g];
+        qx[h] = a.qcur[g]; qy[h] = a.qcur[n + g]; qz[h] = a.qcur[2 * n + g];
+        t11[h] = a.T[g]; t12[h] = a.T[n + g]; t22[h] = a.T[2 * n + g];
+        sp[h] = a.sparse[g];
+    }
+    // the ring: its duals and tensor of the start state
+    const int rid = tid - TSR_RING0;
+    bool ring = false;
+    int rgx = 0, rgy = 0, rlx = 0, rly = 0;
+    if (rid >= 0 && rid < TSR_RING) {
+        if (rid < TX) { rgx = x0 + rid; rgy = y0 - 1; rlx = rid + 1; rly = 0; ring = tcy > 0 && rgx < W; }
+        else { rgx = x0 - 1; rgy = y0 + rid - TX; rlx = 0; rly = rid - TX + 1; ring = tcx > 0 && rgy < H; }
+        if (ring) {
+            const size_t g = (size_t)rgy * W + rgx;
+            sPQ[0][rly][rlx] = a.pcur[g]; sPQ[1][rly][rlx] = a.pcur[n + g];
+            sPQ[2][rly][rlx] = a.qcur[g]; sPQ[3][rly][rlx] = a.qcur[n + g]; sPQ[4][rly][rlx] = a.qcur[2 * n + g];
+            sT[0][rly][rlx] = a.T[g]; sT[1][rly][rlx] = a.T[n + g]; sT[2][rly][rlx] = a.T[2 * n + g];
+        }
+    }
+    // the halo of the start state; later ones come from the neighbours
+    int hoff = -1, hly = 0, hlx = 0;
+    const int hf = tid % TSR_F;
+    if (tid < TSR_HALO) {
+        int gx, gy;
+        if (tsr_halo(A, tcx, tcy, tid, hoff, hly, hlx, gx, gy)) sU[hf][hly][hlx] = a.ucur[hf * n + (size_t)gy * W + gx];
+        else hoff = -1;
+    }
+    // every workgroup resident before any waits on a neighbour
+    if (tid == 0) {
+        unsigned go = tsr_load32(A.err + 2);
+        for (unsigned it = 0; go == 0u && it < A.start_spin; ++it) {
+            __builtin_amdgcn_s_sleep(2);
+            go = tsr_load32(A.err + 2);
+        }
+        if (go == 0u) {
+            unsigned expected = 0u;
+            __hip_atomic_compare_exchange_strong(A.err + 2, &expected, 2u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            go = tsr_load32(A.err + 2);
+        }
+        if (go != 1u) sFlag[0] = 1;
+    }
+    __syncthreads();
+    if (sFlag[0]) return;   // called off: nothing written
+
+    unsigned long long* const xb0 = A.xch + (size_t)tile * TSR_XW;   // this tile's words (parity 0)
+    bool failed = false;
+    for (int j = 0; j < A.niters; ++j) {
+        // p, q of j + 1 on the own pixels and on the ring (from u of state j)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int row = 2 * wv + h, y = y0 + row;
+            float rt[TSR_F], dn[TSR_F];
+#pragma unroll
+            for (int f = 0; f < TSR_F; ++f) {   // read by every lane, then chosen
+                const float right = sU[f][row + 1][tx + 2], down = sU[f][row + 2][tx + 1];
+                rt[f] = x < W - 1 ? right : uf[h][f];
+                dn[f] = y < H - 1 ? down : uf[h][f];
+            }
+            if (in[h]) {
+                pq_update(px[h], py[h], qx[h], qy[h], qz[h], t11[h], t12[h], t22[h], uf[h][0], rt[0], dn[0], uf[h][1],
+                          rt[1], dn[1], uf[h][2], rt[2], dn[2], as1, as0);
+                sPQ[0][row + 1][tx + 1] = px[h]; sPQ[1][row + 1][tx + 1] = py[h];
+                sPQ[2][row + 1][tx + 1] = qx[h]; sPQ[3][row + 1][tx + 1] = qy[h]; sPQ[4][row + 1][tx + 1] = qz[h];
+            }
+        }
+        if (ring) {
+            const int rr = rgx < W - 1 ? rlx + 1 : rlx, rdn = rgy < H - 1 ? rly + 1 : rly;
+            float rpx = sPQ[0][rly][rlx], rpy = sPQ[1][rly][rlx];
+            float rqx = sPQ[2][rly][rlx], rqy = sPQ[3][rly][rlx], rqz = sPQ[4][rly][rlx];
+            pq_update(rpx, rpy, rqx, rqy, rqz, sT[0][rly][rlx], sT[1][rly][rlx], sT[2][rly][rlx], sU[0][rly][rlx],
+                      sU[0][rly][rr], sU[0][rdn][rlx], sU[1][rly][rlx], sU[1][rly][rr], sU[1][rdn][rlx],
+                      sU[2][rly][rlx], sU[2][rly][rr], sU[2][rdn][rlx], as1, as0);
+            sPQ[0][rly][rlx] = rpx; sPQ[1][rly][rlx] = rpy;
+            sPQ[2][rly][rlx] = rqx; sPQ[3][rly][rlx] = rqy; sPQ[4][rly][rlx] = rqz;
+        }
+        __syncthreads();
+        // u, u1 of j + 1; the tile's edges for the neighbours
+        const bool publish = j + 1 < A.niters;
+        unsigned long long* const xb = xb0 + (size_t)((j + 1) & 1) * A.ntiles * TSR_XW;
+        const unsigned tag = (unsigned)(j + 1);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int row = 2 * wv + h, y = y0 + row;
+            float lp[5], up[5];
+#pragma unroll
+            for (int f = 0; f < 5; ++f) {   // read by every lane, then chosen
+                lp[f] = sPQ[f][row + 1][tx];
+                up[f] = sPQ[f][row][tx + 1];
+            }
+            if (in[h]) {
+                const bool hl = x > 0, hu = y > 0;
+                float un, u1xn, u1yn;
+                sparse_u_step(a, uf[h][0], uf[h][1], uf[h][2], t11[h], t12[h], t22[h], px[h], py[h],
+                              hl ? lp[0] : px[h], hl ? lp[1] : py[h], hu ? up[0] : px[h], hu ? up[1] : py[h], qx[h],
+                              hl ? lp[2] : qx[h], qy[h], hu ? up[3] : qy[h], qz[h], hl ? lp[4] : qz[h],
+                              hu ? up[4] : qz[h], sp[h], un, u1xn, u1yn);
+                uf[h][0] = un; uf[h][1] = u1xn; uf[h][2] = u1yn;
+#pragma unroll
+                for (int f = 0; f < TSR_F; ++f) sU[f][row + 1][tx + 1] = uf[h][f];
+                if (publish) {
+#pragma unroll
+                    for (int f = 0; f < TSR_F; ++f) {
+                        if (row == 0) tsr_put(xb + TSR_TOP + f * TX + tx, tag, uf[h][f]);
+                        if (row == TY - 1) tsr_put(xb + TSR_BOT + f * TX + tx, tag, uf[h][f]);
+                        if (tx == 0) tsr_put(xb + TSR_LEFT + f * TY + row, tag, uf[h][f]);
+                        if (tx == TX - 1) tsr_put(xb + TSR_RIGHT + f * TY + row, tag, uf[h][f]);
+                    }
+                }
+            }
+        }
+        if (!publish) break;
+        // the neighbours' edges of state j + 1
+        if (hoff >= 0) {
+            float v;
+            if (tsr_get(A, A.xch + (size_t)((j + 1) & 1) * A.ntiles * TSR_XW + hoff, tag, v, tile, j))
+                sU[hf][hly][hlx] = v;
+            else sFlag[0] = 1;
+        }
+        __syncthreads();
+        if (sFlag[0]) { failed = true; break; }
+    }
+    if (failed) return;
+    // the pad is read on a branch never taken (the host passes niters >= 2),
+    // a select with no arithmetic, so that it stays allocated
+    if (A.niters < 0) {
+        sPad[tid] = uf[0][0];
+        __syncthreads();
+        uf[0][0] = sPad[(tid + 1) % TSR_NT];
+    }
+    // the end state
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (!in[h]) continue;
+        const size_t g = (size_t)(y0 + 2 * wv + h) * W + x;
+#pragma unroll
+        for (int f = 0; f < TSR_F; ++f) a.unext[f * n + g] = uf[h][f];
+        a.pnext[g] = px[h]; a.pnext[n + g] = py[h];
+        a.qnext[g] = qx[h]; a.qnext[n + g] = qy[h]; a.qnext[2 * n + g] = qz[h];
+    }
+}
+
+// The exchange buffer of the resident solver on each device: uncached
+// memory for the largest launch (2 x 256 tiles x 504 edge words, 8 error
+// words), allocated once and kept for the process (uncached memory that a
+// kernel has read is never freed).  Also the CU count, and the tile counts
+// whose launches were called off (those keep using the fused launches).
+struct TsrDev {
+    std::mutex mu;
+    unsigned long long* buf[64] = {};
+    int cus[64] = {};
+    std::vector<int> declined[64];
+};
+static TsrDev& tsr_dev() {
+    static TsrDev* d = new TsrDev();
+    return *d;
+}
+constexpr size_t TSR_WORDS = 2 * (size_t)TSR_MAXTILES * TSR_XW + 4;
+
+// The resident solver runs unless PSW_TGV_SPARSE_RESIDENT=0, for frames of
+// at most one 64 x 20 tile per CU (and at most 256 tiles) and runs of at
+// least two iterations; *buf receives the device's exchange buffer.
+// PSW_TGV_SPARSE_RESIDENT=2 makes a run of two or more iterations that
+// cannot use it an error (tests).
+static bool sparse_resident_use(int W, int H, int niters, unsigned long long** buf) {
+    const char* e = std::getenv("PSW_TGV_SPARSE_RESIDENT");
+    if ((e && std::atoi(e) == 0) || niters < 2) return false;
+    const int nt = ((W + TSR_TX - 1) / TSR_TX) * ((H + TSR_TY - 1) / TSR_TY);
+    if (nt > TSR_MAXTILES) return false;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        (void)hipGetLastError();
+        return false;
+    }
+    TsrDev& D = tsr_dev();
+    std::lock_guard<std::mutex> lock(D.mu);
+    if (D.cus[dev] == 0) {
+        int per = 0, ncu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)tgv_sparse_resident_kernel, TSR_NT, 0) !=
+                hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+            (void)hipGetLastError();
+            per = 0;
+        }
+        D.cus[dev] = per >= 1 && ncu > 0 ? ncu : -1;
+    }
+    if (D.cus[dev] < nt) return false;
+    for (int d : D.declined[dev])
+        if (d == nt) return false;
+    if (!D.buf[dev]) {
+        void* b = nullptr;
+        if (hipExtMallocWithFlags(&b, TSR_WORDS * sizeof(unsigned long long), hipDeviceMallocUncached) != hipSuccess) {
+            (void)hipGetLastError();
+            D.cus[dev] = -1;
+            return false;
+        }
+        D.buf[dev] = static_cast<unsigned long long*>(b);
+    }
+    *buf = D.buf[dev];
+    return true;
+}
+
+static bool sparse_resident_required() {
+    const char* e = std::getenv("PSW_TGV_SPARSE_RESIDENT");
+    return e && std::atoi(e) == 2;
+}
+
+static void sparse_resident_decline(int nt) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
+    TsrDev& D = tsr_dev();
+    std::lock_guard<std::mutex> lock(D.mu);
+    D.declined[dev].push_back(nt);
+}
+
+static unsigned tsr_env_u32(const char* name, unsigned dflt) {
+    const char* e = std::getenv(name);
+    return e ? (unsigned)std::strtoul(e, nullptr, 10) : dflt;
+}
+
+// niters iterations from a's current state to its next one in one
+// cooperative launch.  *ran is false, with nothing written, when the runtime
+// refuses the launch or the launch is called off.  Synchronous (it reads the
+// error words).  Polls per wait: PSW_TGV_SPARSE_RESIDENT_SPIN (2^22), of the
+// start: PSW_TGV_SPARSE_RESIDENT_START_SPIN (2^16; set, a called-off launch
+// is not remembered).
+static int sparse_resident_run(const SparsePtrs& a, unsigned long long* buf, int niters, hipStream_t st, bool* ran) {
+    *ran = false;
+    SparseResArgs A;
+    A.a = a;
+    A.niters = niters;
+    A.tiles_x = (a.W + TSR_TX - 1) / TSR_TX;
+    A.tiles_y = (a.H + TSR_TY - 1) / TSR_TY;
+    A.ntiles = A.tiles_x * A.tiles_y;
+    A.xch = buf;
+    A.err = reinterpret_cast<unsigned*>(buf + 2 * (size_t)A.ntiles * TSR_XW);
+    A.spin = tsr_env_u32("PSW_TGV_SPARSE_RESIDENT_SPIN", 1u << 22);
+    A.start_spin = tsr_env_u32("PSW_TGV_SPARSE_RESIDENT_START_SPIN", 1u << 16);
+    // tags that match no iteration, and clear error words
+    int rc = psw_fill_u32(reinterpret_cast<uint32_t*>(buf), 0xFFFFFFFFu, 4 * (size_t)A.ntiles * TSR_XW, st);
+    if (rc == PSW_OK) rc = psw_fill_u32(A.err, 0u, 8, st);
+    if (rc != PSW_OK) return rc;
+    void* args[] = {&A};
+    if (hipLaunchCooperativeKernel((const void*)tgv_sparse_resident_kernel, dim3((unsigned)A.ntiles), dim3(TSR_NT),
+                                   args, 0, st) != hipSuccess) {
+        (void)hipGetLastError();
+        return PSW_OK;
+    }
+    unsigned* he = nullptr;
+    PSW_TRY_HIP(hipHostMalloc((void**)&he, 8 * sizeof(unsigned), hipHostMallocDefault), PSW_ERR_ALLOC);
+    int out = PSW_OK;
+    if (hipMemcpyAsync(he, A.err, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, st) != hipSuccess) out = PSW_ERR_COPY;
+    else if (hipStreamSynchronize(st) != hipSuccess) out = PSW_ERR_DEVICE;
+    if (out == PSW_OK) {
+        if (he[2] != 1u) {   // called off before any wait on a neighbour: nothing was written
+            if (!std::getenv("PSW_TGV_SPARSE_RESIDENT_START_SPIN")) sparse_resident_decline(A.ntiles);
+        } else {
+            *ran = true;
+            if (he[0]) {
+                set_error("resident sparse-depth TGV2 solver: tile %u waited %u polls for word %u of tile %u "
+                          "(iteration tag %u, saw %u) at iteration %u; the result is undefined",
+                          he[3], A.spin, he[4] % TSR_XW, (he[4] % ((unsigned)A.ntiles * TSR_XW)) / TSR_XW, he[5],
+                          he[6], he[7]);
+                out = PSW_ERR_DEVICE;
+            }
+        }
+    } else {
+        (void)hipGetLastError();
+    }
+    (void)hipHostFree(he);
+    return out;
+}
+
 }  // namespace psw
 
 using namespace psw;
@@ -515,8 +942,35 @@ int psw_tgv_sparse_denoise(const float* d_ref, const float* d_dense, const float
     const bool dbl = sparse_double();
     const bool narrow = sparse_narrow();
     const int mmax = sparse_max_per_launch();
-    int launches = 0;
-    for (int k = 0; k < niters && e == hipSuccess; ++launches) {
+    int launches = 0, k0 = 0;
+    if (e == hipSuccess) {
+        unsigned long long* xbuf = nullptr;
+        bool ran = false;
+        if (sparse_resident_use(width, height, niters, &xbuf)) {
+            a.ucur = slot[0];
+            a.unext = slot[1];
+            a.pcur = slot[0] + 3 * n;
+            a.pnext = slot[1] + 3 * n;
+            a.qcur = slot[0] + 5 * n;
+            a.qnext = slot[1] + 5 * n;
+            const int rc = sparse_resident_run(a, xbuf, niters, hs, &ran);
+            if (rc != PSW_OK) {
+                if (owned) (void)workspace_free_async(ws, hs);
+                return rc;
+            }
+            if (ran) {
+                launches = 1;
+                k0 = niters;
+            }
+        }
+        if (!ran && niters >= 2 && sparse_resident_required()) {
+            if (owned) (void)workspace_free_async(ws, hs);
+            set_error("psw_tgv_sparse_denoise: PSW_TGV_SPARSE_RESIDENT=2 and the resident solver did not run "
+                      "(%dx%d, %d iterations)", width, height, niters);
+            return PSW_ERR_DEVICE;
+        }
+    }
+    for (int k = k0; k < niters && e == hipSuccess; ++launches) {
         float* c = slot[launches & 1];
         float* nx = slot[(launches + 1) & 1];
         a.ucur = c;

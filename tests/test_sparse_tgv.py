@@ -211,6 +211,7 @@ def test_sparse_launch_shapes_identical(gpu, monkeypatch, niters):
                                 ("0", "3", "1", "2"), ("1", "1", "1", "2"), ("1", "0", "1", "2"),
                                 ("1", "1", "0", "2"), ("1", "1", "1", "3"), ("1", "1", "1", "4"),
                                 ("1", "1", "1", "5"), ("1", "1", "1", "6")):
+        monkeypatch.setenv("PSW_TGV_SPARSE_RESIDENT", "0")
         monkeypatch.setenv("PSW_TGV_SPARSE_DOUBLE", dbl)
         monkeypatch.setenv("PSW_TGV_SPARSE_VARIANT", var)
         monkeypatch.setenv("PSW_TGV_SPARSE_NARROW", narrow)
@@ -218,3 +219,67 @@ def test_sparse_launch_shapes_identical(gpu, monkeypatch, niters):
         outs.append(stages.tgv_sparse(v["ref"], dense, sparse, 8.0, niters, **KW))
     for o in outs[1:]:
         np.testing.assert_array_equal(o, outs[0])
+
+
+def _random_case(rng, H, W, frac=0.05):
+    ref = rng.uniform(0, 255, (H, W)).astype(np.float32)
+    dense = rng.uniform(0.5, 6.0, (H, W)).astype(np.float32)
+    sparse = np.where(rng.random((H, W)) < frac, dense * (1 + 0.05 * rng.standard_normal((H, W))), 0)
+    sparse = sparse.astype(np.float32)
+    sparse[rng.random((H, W)) < 0.01] = -1.0
+    return ref, dense, sparse
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("niters", [2, 3, 25])
+def test_sparse_resident_matches_fused(gpu, monkeypatch, niters):
+    """The resident solver (all iterations in one cooperative launch of 64x20
+    tiles; PSW_TGV_SPARSE_RESIDENT=2 makes it an error if it does not run)
+    against the fused launches: bit-identical on the 96x72 scene and on sizes
+    that cut the tiles (161x83, 65x9, 3x2, 200x41) and at 640x480."""
+    from psw_amd import stages
+    rng = np.random.default_rng(300 + niters)
+    v, dense, sparse = _scene()
+    cases = [(v["ref"], dense, sparse)]
+    for H, W in [(83, 161), (9, 65), (2, 3), (41, 200), (480, 640)]:
+        cases.append(_random_case(rng, H, W))
+    for ref, dense, sparse in cases:
+        print(f"[case] sparse resident {ref.shape[1]}x{ref.shape[0]} niters={niters}", flush=True)
+        monkeypatch.setenv("PSW_TGV_SPARSE_RESIDENT", "0")
+        fused = stages.tgv_sparse(ref, dense, sparse, 8.0, niters, **KW)
+        monkeypatch.setenv("PSW_TGV_SPARSE_RESIDENT", "2")
+        res = stages.tgv_sparse(ref, dense, sparse, 8.0, niters, **KW)
+        np.testing.assert_array_equal(res.view(np.uint32), fused.view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_sparse_resident_fallback_and_failure(gpu, monkeypatch):
+    """Frames above 256 tiles (1242x375: 380) use the fused launches, with the
+    same bits; a wait that runs out (a zero poll budget, forced from the host)
+    is reported as an error; and the next run is whole again."""
+    from psw_amd import stages
+    from psw_amd._lib import PswError
+    rng = np.random.default_rng(11)
+    ref, dense, sparse = _random_case(rng, 375, 1242)
+    print("[case] sparse 1242x375 resident on (falls back) vs fused", flush=True)
+    monkeypatch.setenv("PSW_TGV_SPARSE_RESIDENT", "0")
+    fused = stages.tgv_sparse(ref, dense, sparse, 8.0, 6, **KW)
+    monkeypatch.setenv("PSW_TGV_SPARSE_RESIDENT", "1")
+    got = stages.tgv_sparse(ref, dense, sparse, 8.0, 6, **KW)
+    np.testing.assert_array_equal(got.view(np.uint32), fused.view(np.uint32))
+    monkeypatch.setenv("PSW_TGV_SPARSE_RESIDENT", "2")
+    with pytest.raises(PswError):
+        stages.tgv_sparse(ref, dense, sparse, 8.0, 6, **KW)
+    v, dense, sparse = _scene()
+    print("[case] sparse 96x72 resident with a zero poll budget", flush=True)
+    monkeypatch.setenv("PSW_TGV_SPARSE_RESIDENT_SPIN", "0")
+    with pytest.raises(PswError) as e:
+        stages.tgv_sparse(v["ref"], dense, sparse, 8.0, 5, **KW)
+    assert "resident sparse-depth TGV2 solver" in str(e.value)
+    monkeypatch.delenv("PSW_TGV_SPARSE_RESIDENT_SPIN")
+    monkeypatch.setenv("PSW_TGV_SPARSE_RESIDENT", "0")
+    fused = stages.tgv_sparse(v["ref"], dense, sparse, 8.0, 5, **KW)
+    monkeypatch.setenv("PSW_TGV_SPARSE_RESIDENT", "2")
+    got = stages.tgv_sparse(v["ref"], dense, sparse, 8.0, 5, **KW)
+    np.testing.assert_array_equal(got.view(np.uint32), fused.view(np.uint32))
+
