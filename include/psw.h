@@ -393,7 +393,7 @@ int psw_tgv_get_energy_mask(psw_tgv* t, float* d_out, psw_stream_t s);
 int psw_tgv_run(psw_tgv* t, const float* K, const float* Rrel, const float* trel, int niters, int warps,
                 const psw_tgv_params* prm, const float* d_u_init, float* d_u_out, psw_stream_t s);
 
-/* Sparse-depth TGV2 densification, fused (one launch per iteration).
+/* Sparse-depth TGV2 densification, resident or fused (several iterations per launch).
  * Replaces PlaneSweep::TGVdenoiseFromSparse (src/planesweep.cpp:596-674,
  * kernels TGV2_kernels.cu:340-365, :54-85, :399-445; work in progress in the
  * reference and not called by its GUI).  d_ref: reference image 0..255;
@@ -404,7 +404,12 @@ int psw_tgv_run(psw_tgv* t, const float* K, const float* Rrel, const float* trel
  * library's stream-ordered workspace pool and given back in stream order;
  * the pool keeps freed blocks for later calls until the process ends) or at
  * least psw_tgv_sparse_workspace_bytes(width, height) bytes of device
- * memory. */
+ * memory.  Runs of two or more iterations on frames of at most 256 tiles of
+ * 64 x 20 (and one per CU) go to the resident solver: all iterations in one
+ * cooperative launch whose tiles trade only their edges' u, u1 each
+ * iteration; such a run waits for its launch (it reads the launch's error
+ * words), and PSW_TGV_SPARSE_RESIDENT=0 turns it off.  The results are the
+ * same either way. */
 size_t psw_tgv_sparse_workspace_bytes(int width, int height);
 int psw_tgv_sparse_denoise(const float* d_ref, const float* d_dense, const float* d_sparse, int width, int height,
                            float zfar, int niters, double alpha0, double alpha1, double tau, double sigma,
