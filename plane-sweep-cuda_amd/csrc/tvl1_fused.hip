@@ -399,7 +399,8 @@ constexpr int TVR_RING0 = 2 * TVR_TX;                  // ring threads: waves 2 
 constexpr int TVR_RING = TVR_TX + TVR_TY;
 constexpr int TVR_MAXTILES = 256;
 constexpr int TVR_PAD = 20480;                         // 80 KB, see tvl1_resident_kernel
-static_assert(TVR_HALO <= TVR_NT && TVR_RING0 + TVR_RING <= TVR_NT, "one halo cell and one ring cell per thread");
+static_assert(TVR_HALO <= TVR_NT && TVR_RING0 + TVR_RING + TVR_TY <= 4 * TVR_TX,
+              "one halo cell per thread; the ring and the right column's workers in waves 2 and 3");
 
 struct TvrArgs {
     TvlState tv;
@@ -490,7 +491,7 @@ __global__ __launch_bounds__(TVR_NT) void tvl1_resident_kernel(const TvrArgs A) 
     constexpr int TX = TVR_TX, TY = TVR_TY;
     __shared__ float sO[TY + 2][TX + 2];         // depth o: the tile and its halo
     __shared__ float sP[2][TY + 1][TX + 1];      // Px, Py: the tile, the row above and the column left
-    __shared__ float sT[4][TY + 1][TX + 1];      // the ring's tensor
+    __shared__ float sT[4][TY + 1][TX + 1];      // the tensor of the ring and of the right column
     __shared__ int sFlag[1];
     // LDS held only so that a CU takes one workgroup (the tiles run in
     // lockstep, so two on one CU would halve the whole solve's speed)
@@ -525,6 +526,11 @@ __global__ __launch_bounds__(TVR_NT) void tvl1_resident_kernel(const TvrArgs A) 
         o[h] = a.out[rd][g]; px[h] = a.px[rd][g]; py[h] = a.py[rd][g]; r[h] = a.r[rd][g];
         raw[h] = a.raw[g]; t11[h] = a.t11[g]; t12[h] = a.t12[g]; t21[h] = a.t21[g]; t22[h] = a.t22[g];
         sO[2 * wv + h + 1][tx + 1] = o[h];
+        if (tx == TX - 1) {   // the right column's P step runs from LDS (see the loop)
+            sP[0][2 * wv + h + 1][TX] = px[h]; sP[1][2 * wv + h + 1][TX] = py[h];
+            sT[0][2 * wv + h + 1][TX] = t11[h]; sT[1][2 * wv + h + 1][TX] = t12[h];
+            sT[2][2 * wv + h + 1][TX] = t21[h]; sT[3][2 * wv + h + 1][TX] = t22[h];
+        }
     }
     // the ring: its duals and tensor of the start state
     const int rid = tid - TVR_RING0;
@@ -567,18 +573,48 @@ __global__ __launch_bounds__(TVR_NT) void tvl1_resident_kernel(const TvrArgs A) 
 
     unsigned long long* const xb0 = A.xch + (size_t)tile * TVR_XW;   // this tile's words (parity 0)
     bool failed = false;
+    // The P step of a pixel reads the depth right of it and below it, so
+    // only the tile's right column and bottom row (and the ring) need the
+    // halo: the others are done before the wait for the neighbours' edges,
+    // in its shadow.  The bottom row is wave 9's second row, done by that
+    // wave after the wait; the right column (one lane of every wave) is done
+    // by lanes 20..39 of wave 3 from LDS, and its owners take their duals
+    // back from there for the update.
+    const int rc = tid - (TVR_RING0 + TVR_RING);   // right-column worker: pixel (x0 + 63, y0 + rc)
+    const bool rcol = rc >= 0 && rc < TY && x0 + TX - 1 < W && y0 + rc < H;
+    const bool last_col = tx == TX - 1;
     for (int j = 0; j < A.niters; ++j) {
         const float sg = j == 0 ? A.sp0 : A.sp1;
-        // P step of the own pixels and of the ring from the depth of state j
+        // P step of the pixels off the right column and the bottom row
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int row = 2 * wv + h, y = y0 + row;
+            if (row == TY - 1) continue;   // wave-uniform
             const float c = sO[row + 1][tx + 1];
             const float e_ = sO[row + 1][tx + 2], s_ = sO[row + 2][tx + 1];   // read by every lane, then chosen
-            if (in[h]) {
+            if (in[h] && !last_col) {
                 tvl1_p(px[h], py[h], c, x + 1 < W ? e_ : c, y + 1 < H ? s_ : c, t11[h], t12[h], t21[h], t22[h], sg);
                 sP[0][row + 1][tx + 1] = px[h];
                 sP[1][row + 1][tx + 1] = py[h];
+            }
+        }
+        // the neighbours' edges of state j (from the start state at j = 0)
+        if (j > 0 && hoff >= 0) {
+            float v;
+            if (tvr_get(A, A.xch + (size_t)(j & 1) * A.ntiles * TVR_XW + hoff, (unsigned)j, v, tile, j)) sO[hly][hlx] = v;
+            else sFlag[0] = 1;
+        }
+        __syncthreads();
+        if (sFlag[0]) { failed = true; break; }
+        // P step of the bottom row, the ring and the right column
+        if (2 * wv + 1 == TY - 1) {   // wave-uniform
+            const int row = TY - 1, y = y0 + row;
+            const float c = sO[row + 1][tx + 1];
+            const float e_ = sO[row + 1][tx + 2], s_ = sO[row + 2][tx + 1];
+            if (in[1] && !last_col) {
+                tvl1_p(px[1], py[1], c, x + 1 < W ? e_ : c, y + 1 < H ? s_ : c, t11[1], t12[1], t21[1], t22[1], sg);
+                sP[0][row + 1][tx + 1] = px[1];
+                sP[1][row + 1][tx + 1] = py[1];
             }
         }
         if (ring) {
@@ -590,6 +626,16 @@ __global__ __launch_bounds__(TVR_NT) void tvl1_resident_kernel(const TvrArgs A) 
             sP[0][rly][rlx] = rpx;
             sP[1][rly][rlx] = rpy;
         }
+        if (rcol) {
+            const int gx = x0 + TX - 1, gy = y0 + rc;
+            const float c = sO[rc + 1][TX];
+            const float e_ = gx + 1 < W ? sO[rc + 1][TX + 1] : c;
+            const float s_ = gy + 1 < H ? sO[rc + 2][TX] : c;
+            float cpx = sP[0][rc + 1][TX], cpy = sP[1][rc + 1][TX];
+            tvl1_p(cpx, cpy, c, e_, s_, sT[0][rc + 1][TX], sT[1][rc + 1][TX], sT[2][rc + 1][TX], sT[3][rc + 1][TX], sg);
+            sP[0][rc + 1][TX] = cpx;
+            sP[1][rc + 1][TX] = cpy;
+        }
         __syncthreads();
         // update: R and the over-relaxed depth of state j + 1; the tile's
         // edges for the neighbours
@@ -600,10 +646,11 @@ __global__ __launch_bounds__(TVR_NT) void tvl1_resident_kernel(const TvrArgs A) 
         for (int h = 0; h < 2; ++h) {
             const int row = 2 * wv + h, y = y0 + row;
             const float pxl_ = sP[0][row + 1][tx], pyu_ = sP[1][row][tx + 1];
+            const float pxo = sP[0][row + 1][tx + 1], pyo = sP[1][row + 1][tx + 1];
             if (in[h]) {
+                if (last_col) { px[h] = pxo; py[h] = pyo; }   // the right column's duals, from its worker
                 const float pxl = x > 0 ? pxl_ : px[h], pyu = y > 0 ? pyu_ : py[h];
                 o[h] = tvl1_u(o[h], r[h], raw[h], px[h], pxl, py[h], pyu, x == 0, a.tau, a.theta, a.lambda, a.sigma);
-                sO[row + 1][tx + 1] = o[h];
                 if (publish) {
                     if (row == 0) tvr_put(xb + TVR_TOP + tx, tag, o[h]);
                     if (row == TY - 1) tvr_put(xb + TVR_BOT + tx, tag, o[h]);
@@ -613,14 +660,12 @@ __global__ __launch_bounds__(TVR_NT) void tvl1_resident_kernel(const TvrArgs A) 
             }
         }
         if (!publish) break;
-        // the neighbours' edges of state j + 1
-        if (hoff >= 0) {
-            float v;
-            if (tvr_get(A, A.xch + (size_t)((j + 1) & 1) * A.ntiles * TVR_XW + hoff, tag, v, tile, j)) sO[hly][hlx] = v;
-            else sFlag[0] = 1;
-        }
+        // the own depth of state j + 1 for the next P step (its readers have
+        // passed the barrier above; the next one orders it before them)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+            if (in[h]) sO[2 * wv + h + 1][tx + 1] = o[h];
         __syncthreads();
-        if (sFlag[0]) { failed = true; break; }
     }
     if (failed) return;
     // the pad is read on a branch never taken (the host passes niters >= 2),
