@@ -484,7 +484,8 @@ constexpr int TSR_HALO = TSR_F * (2 * TSR_TX + 2 * TSR_TY + 2);  // halo words a
 constexpr int TSR_RING0 = 2 * TSR_TX, TSR_RING = TSR_TX + TSR_TY;  // ring threads: waves 2 and 3
 constexpr int TSR_MAXTILES = 256;
 constexpr int TSR_PAD = 6144;                                     // 24 KB, see tgv_sparse_resident_kernel
-static_assert(TSR_HALO <= TSR_NT && TSR_RING0 + TSR_RING <= TSR_NT, "one halo word and one ring cell per thread");
+static_assert(TSR_HALO <= TSR_NT && TSR_RING0 + TSR_RING + TSR_TY <= 4 * TSR_TX,
+              "one halo word per thread; the ring and the right column's workers in waves 2 and 3");
 
 struct SparseResArgs {
     SparsePtrs a;               // the start state in ucur / pcur / qcur, the end state to unext / pnext / qnext
@@ -575,7 +576,7 @@ __global__ __launch_bounds__(TSR_NT) void tgv_sparse_resident_kernel(const Spars
     constexpr int TX = TSR_TX, TY = TSR_TY;
     __shared__ float sU[TSR_F][TY + 2][TX + 2];   // u, u1x, u1y: the tile and its halo
     __shared__ float sPQ[5][TY + 1][TX + 1];      // px, py, qx, qy, qz: the tile, the row above, the column left
-    __shared__ float sT[3][TY + 1][TX + 1];       // the ring's tensor
+    __shared__ float sT[3][TY + 1][TX + 1];       // the tensor of the ring and of the right column
     __shared__ int sFlag[1];
     // LDS held only so that a CU takes one workgroup (the tiles run in
     // lockstep, so two on one CU would halve the whole solve's speed)
@@ -617,6 +618,12 @@ __global__ __launch_bounds__(TSR_NT) void tgv_sparse_resident_kernel(const Spars
         qx[h] = a.qcur[g]; qy[h] = a.qcur[n + g]; qz[h] = a.qcur[2 * n + g];
         t11[h] = a.T[g]; t12[h] = a.T[n + g]; t22[h] = a.T[2 * n + g];
         sp[h] = a.sparse[g];
+        if (tx == TX - 1) {   // the right column's dual step runs from LDS (see the loop)
+            const int cy = 2 * wv + h + 1;
+            sPQ[0][cy][TX] = px[h]; sPQ[1][cy][TX] = py[h];
+            sPQ[2][cy][TX] = qx[h]; sPQ[3][cy][TX] = qy[h]; sPQ[4][cy][TX] = qz[h];
+            sT[0][cy][TX] = t11[h]; sT[1][cy][TX] = t12[h]; sT[2][cy][TX] = t22[h];
+        }
     }
     // the ring: its duals and tensor of the start state
     const int rid = tid - TSR_RING0;
@@ -660,11 +667,22 @@ __global__ __launch_bounds__(TSR_NT) void tgv_sparse_resident_kernel(const Spars
 
     unsigned long long* const xb0 = A.xch + (size_t)tile * TSR_XW;   // this tile's words (parity 0)
     bool failed = false;
+    // The dual step of a pixel reads u, u1 right of it and below it, so only
+    // the tile's right column and bottom row (and the ring) need the halo:
+    // the others are done before the wait for the neighbours' edges, in its
+    // shadow.  The bottom row is wave 9's second row, done by that wave after
+    // the wait; the right column (one lane of every wave) is done by lanes
+    // 20..39 of wave 3 from LDS, and its owners take their duals back from
+    // there for the primal step.
+    const int rc = tid - (TSR_RING0 + TSR_RING);   // right-column worker: pixel (x0 + 63, y0 + rc)
+    const bool rcol = rc >= 0 && rc < TY && x0 + TX - 1 < W && y0 + rc < H;
+    const bool last_col = tx == TX - 1;
     for (int j = 0; j < A.niters; ++j) {
-        // p, q of j + 1 on the own pixels and on the ring (from u of state j)
+        // p, q of j + 1 (from u of state j) off the right column and the bottom row
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int row = 2 * wv + h, y = y0 + row;
+            if (row == TY - 1) continue;   // wave-uniform
             float rt[TSR_F], dn[TSR_F];
 #pragma unroll
             for (int f = 0; f < TSR_F; ++f) {   // read by every lane, then chosen
@@ -672,22 +690,51 @@ __global__ __launch_bounds__(TSR_NT) void tgv_sparse_resident_kernel(const Spars
                 rt[f] = x < W - 1 ? right : uf[h][f];
                 dn[f] = y < H - 1 ? down : uf[h][f];
             }
-            if (in[h]) {
+            if (in[h] && !last_col) {
                 pq_update(px[h], py[h], qx[h], qy[h], qz[h], t11[h], t12[h], t22[h], uf[h][0], rt[0], dn[0], uf[h][1],
                           rt[1], dn[1], uf[h][2], rt[2], dn[2], as1, as0);
                 sPQ[0][row + 1][tx + 1] = px[h]; sPQ[1][row + 1][tx + 1] = py[h];
                 sPQ[2][row + 1][tx + 1] = qx[h]; sPQ[3][row + 1][tx + 1] = qy[h]; sPQ[4][row + 1][tx + 1] = qz[h];
             }
         }
-        if (ring) {
-            const int rr = rgx < W - 1 ? rlx + 1 : rlx, rdn = rgy < H - 1 ? rly + 1 : rly;
-            float rpx = sPQ[0][rly][rlx], rpy = sPQ[1][rly][rlx];
-            float rqx = sPQ[2][rly][rlx], rqy = sPQ[3][rly][rlx], rqz = sPQ[4][rly][rlx];
-            pq_update(rpx, rpy, rqx, rqy, rqz, sT[0][rly][rlx], sT[1][rly][rlx], sT[2][rly][rlx], sU[0][rly][rlx],
-                      sU[0][rly][rr], sU[0][rdn][rlx], sU[1][rly][rlx], sU[1][rly][rr], sU[1][rdn][rlx],
-                      sU[2][rly][rlx], sU[2][rly][rr], sU[2][rdn][rlx], as1, as0);
-            sPQ[0][rly][rlx] = rpx; sPQ[1][rly][rlx] = rpy;
-            sPQ[2][rly][rlx] = rqx; sPQ[3][rly][rlx] = rqy; sPQ[4][rly][rlx] = rqz;
+        // the neighbours' edges of state j (from the start state at j = 0)
+        if (j > 0 && hoff >= 0) {
+            float v;
+            if (tsr_get(A, A.xch + (size_t)(j & 1) * A.ntiles * TSR_XW + hoff, (unsigned)j, v, tile, j))
+                sU[hf][hly][hlx] = v;
+            else sFlag[0] = 1;
+        }
+        __syncthreads();
+        if (sFlag[0]) { failed = true; break; }
+        // p, q of the bottom row, the ring and the right column
+        if (2 * wv + 1 == TY - 1) {   // wave-uniform
+            const int row = TY - 1, y = y0 + row;
+            float rt[TSR_F], dn[TSR_F];
+#pragma unroll
+            for (int f = 0; f < TSR_F; ++f) {
+                const float right = sU[f][row + 1][tx + 2], down = sU[f][row + 2][tx + 1];
+                rt[f] = x < W - 1 ? right : uf[1][f];
+                dn[f] = y < H - 1 ? down : uf[1][f];
+            }
+            if (in[1] && !last_col) {
+                pq_update(px[1], py[1], qx[1], qy[1], qz[1], t11[1], t12[1], t22[1], uf[1][0], rt[0], dn[0], uf[1][1],
+                          rt[1], dn[1], uf[1][2], rt[2], dn[2], as1, as0);
+                sPQ[0][row + 1][tx + 1] = px[1]; sPQ[1][row + 1][tx + 1] = py[1];
+                sPQ[2][row + 1][tx + 1] = qx[1]; sPQ[3][row + 1][tx + 1] = qy[1]; sPQ[4][row + 1][tx + 1] = qz[1];
+            }
+        }
+        if (ring || rcol) {
+            // the ring's cell, or the right column's (the same dual step from LDS)
+            const int cly = ring ? rly : rc + 1, clx = ring ? rlx : TX;
+            const int cgx = ring ? rgx : x0 + TX - 1, cgy = ring ? rgy : y0 + rc;
+            const int rr = cgx < W - 1 ? clx + 1 : clx, rdn = cgy < H - 1 ? cly + 1 : cly;
+            float rpx = sPQ[0][cly][clx], rpy = sPQ[1][cly][clx];
+            float rqx = sPQ[2][cly][clx], rqy = sPQ[3][cly][clx], rqz = sPQ[4][cly][clx];
+            pq_update(rpx, rpy, rqx, rqy, rqz, sT[0][cly][clx], sT[1][cly][clx], sT[2][cly][clx], sU[0][cly][clx],
+                      sU[0][cly][rr], sU[0][rdn][clx], sU[1][cly][clx], sU[1][cly][rr], sU[1][rdn][clx],
+                      sU[2][cly][clx], sU[2][cly][rr], sU[2][rdn][clx], as1, as0);
+            sPQ[0][cly][clx] = rpx; sPQ[1][cly][clx] = rpy;
+            sPQ[2][cly][clx] = rqx; sPQ[3][cly][clx] = rqy; sPQ[4][cly][clx] = rqz;
         }
         __syncthreads();
         // u, u1 of j + 1; the tile's edges for the neighbours
@@ -697,13 +744,17 @@ __global__ __launch_bounds__(TSR_NT) void tgv_sparse_resident_kernel(const Spars
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int row = 2 * wv + h, y = y0 + row;
-            float lp[5], up[5];
+            float lp[5], up[5], op[5];
 #pragma unroll
             for (int f = 0; f < 5; ++f) {   // read by every lane, then chosen
                 lp[f] = sPQ[f][row + 1][tx];
                 up[f] = sPQ[f][row][tx + 1];
+                op[f] = sPQ[f][row + 1][tx + 1];
             }
             if (in[h]) {
+                if (last_col) {   // the right column's duals, from its worker
+                    px[h] = op[0]; py[h] = op[1]; qx[h] = op[2]; qy[h] = op[3]; qz[h] = op[4];
+                }
                 const bool hl = x > 0, hu = y > 0;
                 float un, u1xn, u1yn;
                 sparse_u_step(a, uf[h][0], uf[h][1], uf[h][2], t11[h], t12[h], t22[h], px[h], py[h],
@@ -711,8 +762,6 @@ __global__ __launch_bounds__(TSR_NT) void tgv_sparse_resident_kernel(const Spars
                               hl ? lp[2] : qx[h], qy[h], hu ? up[3] : qy[h], qz[h], hl ? lp[4] : qz[h],
                               hu ? up[4] : qz[h], sp[h], un, u1xn, u1yn);
                 uf[h][0] = un; uf[h][1] = u1xn; uf[h][2] = u1yn;
-#pragma unroll
-                for (int f = 0; f < TSR_F; ++f) sU[f][row + 1][tx + 1] = uf[h][f];
                 if (publish) {
 #pragma unroll
                     for (int f = 0; f < TSR_F; ++f) {
@@ -725,15 +774,15 @@ __global__ __launch_bounds__(TSR_NT) void tgv_sparse_resident_kernel(const Spars
             }
         }
         if (!publish) break;
-        // the neighbours' edges of state j + 1
-        if (hoff >= 0) {
-            float v;
-            if (tsr_get(A, A.xch + (size_t)((j + 1) & 1) * A.ntiles * TSR_XW + hoff, tag, v, tile, j))
-                sU[hf][hly][hlx] = v;
-            else sFlag[0] = 1;
-        }
+        // the own u, u1 of state j + 1 for the next dual step (its readers
+        // have passed the barrier above; the next one orders it before them)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+            if (in[h]) {
+#pragma unroll
+                for (int f = 0; f < TSR_F; ++f) sU[f][2 * wv + h + 1][tx + 1] = uf[h][f];
+            }
         __syncthreads();
-        if (sFlag[0]) { failed = true; break; }
     }
     if (failed) return;
     // the pad is read on a branch never taken (the host passes niters >= 2),
