@@ -408,7 +408,8 @@ int psw_tgv_run(psw_tgv* t, const float* K, const float* Rrel, const float* trel
  * 64 x 20 (and one per CU) go to the resident solver: all iterations in one
  * cooperative launch whose tiles trade only their edges' u, u1 each
  * iteration; such a run waits for its launch (it reads the launch's error
- * words), and PSW_TGV_SPARSE_RESIDENT=0 turns it off.  The results are the
+ * words) and for other resident runs on the device, and
+ * PSW_TGV_SPARSE_RESIDENT=0 turns it off.  The results are the
  * same either way. */
 size_t psw_tgv_sparse_workspace_bytes(int width, int height);
 int psw_tgv_sparse_denoise(const float* d_ref, const float* d_dense, const float* d_sparse, int width, int height,
@@ -474,7 +475,8 @@ int psw_fusion_surface_count(const psw_fusion_grid* g, int64_t* count, psw_strea
  * 256 tiles of 64 x 20 (and one per CU) go to the resident solver: all
  * iterations in one cooperative launch whose tiles trade only their edges'
  * depth each iteration; such a run waits for its launch (it reads the
- * launch's error words), and PSW_TVL1_RESIDENT=0 turns it off.  Otherwise,
+ * launch's error words) and for other resident runs on the device, and
+ * PSW_TVL1_RESIDENT=0 turns it off.  Otherwise,
  * or when the launch is called off, the fused kernel runs up to PSW_TVL1_M
  * iterations per launch (2 to 6, default 4; at most 4 with the 64x16 tiles
  * of PSW_TVL1_TY=16), a shorter launch for the remainder; PSW_TVL1_FUSED=0

@@ -694,6 +694,10 @@ struct TvrDev {
     unsigned long long* buf[64] = {};
     int cus[64] = {};
     std::vector<int> declined[64];
+    // one resident run per device at a time (they share the exchange buffer
+    // and would compete for the CUs), and its error words on the host
+    std::mutex run[64];
+    unsigned* herr[64] = {};
 };
 TvrDev& tvr_dev() {
     static TvrDev* d = new TvrDev();
@@ -765,7 +769,8 @@ unsigned tvr_env_u32(const char* name, unsigned dflt) {
 // niters iterations from slot rd into slot 1 - rd in one cooperative
 // launch.  *ran is false, with nothing written, when the runtime refuses
 // the launch or the launch is called off.  Synchronous (it reads the error
-// words).  Polls per wait: PSW_TVL1_RESIDENT_SPIN (2^22), of the start:
+// words); runs on one device take turns, since they share its exchange
+// buffer.  Polls per wait: PSW_TVL1_RESIDENT_SPIN (2^22), of the start:
 // PSW_TVL1_RESIDENT_START_SPIN (2^16; set, a called-off launch is not
 // remembered).
 int tvl1_resident_run(const TvlState& tv, unsigned long long* buf, int rd, int niters, float sp0, float sp1,
@@ -784,6 +789,18 @@ int tvl1_resident_run(const TvlState& tv, unsigned long long* buf, int rd, int n
     A.err = reinterpret_cast<unsigned*>(buf + 2 * (size_t)A.ntiles * TVR_XW);
     A.spin = tvr_env_u32("PSW_TVL1_RESIDENT_SPIN", 1u << 22);
     A.start_spin = tvr_env_u32("PSW_TVL1_RESIDENT_START_SPIN", 1u << 16);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        (void)hipGetLastError();
+        return PSW_OK;
+    }
+    TvrDev& D = tvr_dev();
+    std::lock_guard<std::mutex> run_lock(D.run[dev]);
+    if (!D.herr[dev]) {
+        std::lock_guard<std::mutex> lock(D.mu);
+        PSW_TRY_HIP(hipHostMalloc((void**)&D.herr[dev], 8 * sizeof(unsigned), hipHostMallocDefault), PSW_ERR_ALLOC);
+    }
+    unsigned* const he = D.herr[dev];
     // tags that match no iteration, and clear error words
     int rc = psw_fill_u32(reinterpret_cast<uint32_t*>(buf), 0xFFFFFFFFu, 4 * (size_t)A.ntiles * TVR_XW, st);
     if (rc == PSW_OK) rc = psw_fill_u32(A.err, 0u, 8, st);
@@ -794,8 +811,6 @@ int tvl1_resident_run(const TvlState& tv, unsigned long long* buf, int rd, int n
         (void)hipGetLastError();
         return PSW_OK;
     }
-    unsigned* he = nullptr;
-    PSW_TRY_HIP(hipHostMalloc((void**)&he, 8 * sizeof(unsigned), hipHostMallocDefault), PSW_ERR_ALLOC);
     int out = PSW_OK;
     if (hipMemcpyAsync(he, A.err, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, st) != hipSuccess) out = PSW_ERR_COPY;
     else if (hipStreamSynchronize(st) != hipSuccess) out = PSW_ERR_DEVICE;
@@ -815,7 +830,6 @@ int tvl1_resident_run(const TvlState& tv, unsigned long long* buf, int rd, int n
     } else {
         (void)hipGetLastError();
     }
-    (void)hipHostFree(he);
     return out;
 }
 }  // namespace

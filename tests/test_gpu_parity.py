@@ -692,6 +692,44 @@ def test_tvl1_resident_fallback_and_failure(gpu, monkeypatch, views_small):
     np.testing.assert_array_equal(got.view(np.uint32), seq.view(np.uint32))
 
 
+def test_tvl1_resident_concurrent_calls(gpu, monkeypatch):
+    """Two threads denoising different frames on their own streams at once
+    (the resident runs of one device take turns on its exchange buffer): the
+    same bits as each frame alone."""
+    import threading
+    from psw_amd import stages
+    from psw_amd.device import DeviceArray, Stream
+    monkeypatch.setenv("PSW_TVL1_RESIDENT", "2")
+    rng = np.random.default_rng(5)
+    H, W = 120, 160
+    frames = [(rng.uniform(0.5, 6.0, (H, W)).astype(np.float32), rng.uniform(0, 255, (H, W)).astype(np.float32))
+              for _ in range(2)]
+    alone = [stages.tvl1_denoise(d, r, 0.5, 6.0, niters=200) for d, r in frames]
+    dev = [(DeviceArray.from_host(d), DeviceArray.from_host(r), DeviceArray((H, W)), DeviceArray((12, H, W)), Stream())
+           for d, r in frames]
+    errors = []
+
+    def run(k):
+        d_d, d_r, d_o, d_w, st = dev[k]
+        try:
+            for _ in range(5):
+                stages.tvl1_denoise_device(d_d, d_r, W, H, 0.5, 6.0, d_o, d_w, 200, stream=st.handle)
+                st.synchronize()
+                if not np.array_equal(d_o.to_host().view(np.uint32), alone[k].view(np.uint32)):
+                    errors.append(k)
+        except Exception as e:   # noqa: BLE001  (reported below)
+            errors.append(repr(e))
+
+    print("[case] tvl1 resident, two threads at once", flush=True)
+    th = [threading.Thread(target=run, args=(k,)) for k in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in th)
+    assert not errors, errors
+
+
 # ---- error paths (forced from the host) -----------------------------------------------------
 
 def test_error_paths(gpu, views_tiny):
