@@ -64,7 +64,7 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 SWEEP_FLOP_PER_EVAL = 50          # declared FLOP per (pixel, plane, source) NCC evaluation
 SWEEP_HBM_BYTES = 4 * W * H * (1 + NSRC) + 16 * W * H   # images read once, outputs and keys once
-TGV_SURVEY_FLOATS_PER_PX = 27 + 4 * NSRC   # floats per pixel of one unfused iteration
+TGV_UNFUSED_FLOATS_PER_PX = 27 + 4 * NSRC   # floats per pixel of one unfused iteration
 SEEDS = tuple(range(2024, 2032))           # eight scenes; the network was trained on seeds from 100000
 
 
@@ -794,7 +794,7 @@ def main():
     ev[1].synchronize()
     resident_us = ev[0].elapsed_ms(ev[1]) * 1e3 / n_res
     compulsory = 4 * tgv_compulsory_floats(m) * W * H
-    survey_bytes = 4 * TGV_SURVEY_FLOATS_PER_PX * W * H * m
+    unfused_bytes = 4 * TGV_UNFUSED_FLOATS_PER_PX * W * H * m
     achieved = compulsory / (launch_ms * 1e-3) / 1e9
     traffic, tsrc = None, None
     tfile = os.path.join(ROOT, "profiles", tgv_kernel + "_traffic.json")
@@ -863,9 +863,9 @@ def main():
                      "bytes_per_launch": compulsory, "launch_ms": launch_ms, "iterations_per_launch": m,
                      "bytes_note": f"compulsory bytes of one launch of {m} iteration(s) ({4 * tgv_compulsory_floats(m)}"
                                    " B/px: the state read once and written once); the per-iteration count of an "
-                                   f"unfused iteration, {4 * TGV_SURVEY_FLOATS_PER_PX} B/px, gives "
-                                   f"{survey_bytes / 1e6:.1f} MB per launch "
-                                   f"({survey_bytes / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS:.2f} of peak), which "
+                                   f"unfused iteration, {4 * TGV_UNFUSED_FLOATS_PER_PX} B/px, gives "
+                                   f"{unfused_bytes / 1e6:.1f} MB per launch "
+                                   f"({unfused_bytes / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS:.2f} of peak), which "
                                    "counts bytes the fused launch never moves"},
         "tgv_resident": {"kernel": "tgv_resident_kernel", "us_per_iteration": resident_us,
                          "launch_sequence_us_per_iteration": launch_ms * 1e3 / m,
