@@ -283,3 +283,40 @@ def test_sparse_resident_fallback_and_failure(gpu, monkeypatch):
     got = stages.tgv_sparse(v["ref"], dense, sparse, 8.0, 5, **KW)
     np.testing.assert_array_equal(got.view(np.uint32), fused.view(np.uint32))
 
+
+@pytest.mark.gpu
+def test_sparse_resident_concurrent_calls(gpu, monkeypatch):
+    """Two threads densifying different frames on their own streams at once
+    (the resident runs of one device take turns on its exchange buffer): the
+    same bits as each frame alone."""
+    import threading
+    from psw_amd import stages
+    from psw_amd.device import DeviceArray, Stream
+    monkeypatch.setenv("PSW_TGV_SPARSE_RESIDENT", "2")
+    rng = np.random.default_rng(9)
+    frames = [_random_case(rng, 83, 161) for _ in range(2)]
+    alone = [stages.tgv_sparse(ref, dense, sparse, 8.0, 100, **KW) for ref, dense, sparse in frames]
+    dev = [([DeviceArray.from_host(x) for x in f], DeviceArray((stages.tgv_sparse_workspace_bytes(161, 83) // 4,)),
+            Stream()) for f in frames]
+    errors = []
+
+    def run(k):
+        (d_ref, d_dense, d_sparse), ws, st = dev[k]
+        try:
+            for _ in range(5):
+                out = stages.tgv_sparse(d_ref, d_dense, d_sparse, 8.0, 100, workspace=ws, stream=st.handle, **KW)
+                st.synchronize()
+                if not np.array_equal(out.to_host().view(np.uint32), alone[k].view(np.uint32)):
+                    errors.append(k)
+        except Exception as e:   # noqa: BLE001  (reported below)
+            errors.append(repr(e))
+
+    print("[case] sparse resident, two threads at once", flush=True)
+    th = [threading.Thread(target=run, args=(k,)) for k in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in th)
+    assert not errors, errors
+
